@@ -1,0 +1,382 @@
+"""Seeded synthetic systems in ORAC's internal data contract.
+
+Every array here is what ORAC's nonbonded routines consume, in ORAC's own
+conventions, so the same arrays go unchanged to the HIP library, to the C
+oracle and to the compiled reference:
+
+* reduced coordinates ``xp = oc . x`` with ``co`` = half the cell matrix, so
+  the box spans [-1, 1) (``change_frame.f``, ``setuns.f:43`` boxl=2);
+* charges in internal units ``q / sqrt(unitc)`` (``covchg.f:38``);
+* packed Lennard-Jones tables ``ecc12 = 4 eps sigma^12``, ``ecc6 = 4 eps sigma^6``
+  in internal energy units, ``type_table(i,j) = j(j-1)/2 + i`` for i <= j
+  (``covnb.f:81-106``);
+* groups ``grppt(2, ngrp)``, ``atomg``, ``atomp``, ``pmass`` (mass / group mass,
+  ``redmss.f:15``), group centres ``xpg = sum pmass * xp`` (``appbou.f``),
+  molecule centres of mass ``xpcm``;
+* the non-bonded exclusion map ``mapnl`` = per atom ``[m, j1..jm]`` with
+  ``j > i`` only (``mappa.f:120-130``).
+
+Index arrays are 1-based, as Fortran passes them.  All float arrays are
+float64.  Nothing here calls the GPU.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+# ---- units, restated from setuns.f:34-50 -------------------------------
+PI = 4.0 * math.atan(1.0)
+AVOGAD = 6.022140857e23
+ELECHG = 1.60217662e-19
+EPSO = 8.854187817e-12
+UNITM = 1.0 / (AVOGAD * 1000.0)
+UNITL = 1.0e-10
+UNITT = 1.0e-15
+UNITE = UNITM * (UNITL / UNITT) ** 2
+UNITC = 4.0 * PI * EPSO * UNITL * UNITE / (ELECHG * ELECHG)
+EFACT = UNITE * AVOGAD  # J/mol per internal energy unit
+# covnb.f:96 multiplies by the default-real literal 4.184 (REAL(4)).
+KCAL_F32 = float(np.float32(4.184))
+MASSINFTY = 1.0e20  # blkdta.f90:280
+
+WATER_Q = (-0.834, 0.417, 0.417)
+WATER_MASS = (15.9994, 1.008, 1.008)
+R_OH = 0.9572
+ANG_HOH = 104.52
+O_SIGMA = 3.1507
+O_EPS = 0.1521
+
+
+def charge_internal(q_e):
+    return np.asarray(q_e, dtype=np.float64) / math.sqrt(UNITC)
+
+
+def lj_internal(eps_kcal, sigma):
+    """Return (ecc12, ecc6) for one pair, restating covnb.f:94-99."""
+    tmpi = 4.0 * 1000.0 * eps_kcal * KCAL_F32 / (UNITE * AVOGAD)
+    s6 = sigma ** 6
+    return tmpi * s6 * s6, tmpi * s6
+
+
+@dataclasses.dataclass
+class System:
+    """One periodic system, ready for the nonbonded force path."""
+    L: float
+    xp0: np.ndarray
+    yp0: np.ndarray
+    zp0: np.ndarray
+    charge: np.ndarray
+    mass: np.ndarray
+    nbtype: np.ndarray          # int32, 1-based
+    ntypes: int                 # ``ma``
+    type_table: np.ndarray      # int32 (ma, ma), Fortran order, 1-based packed index
+    ecc12: np.ndarray
+    ecc6: np.ndarray
+    grppt: np.ndarray           # int32 (2, ngrp) Fortran order, 1-based
+    atomg: np.ndarray           # int32, 1-based group of each atom
+    atomp: np.ndarray           # int32, 1-based molecule of each atom
+    ss_index: np.ndarray        # int32, 1 = solute, 2 = solvent
+    pmass: np.ndarray
+    co: np.ndarray              # (3,3) Fortran order
+    oc: np.ndarray
+    volume: float
+    alpha: float
+    nfft: tuple
+    order: int
+    rcut: float
+    rtol: float = 0.3
+    rnei: float = 0.3
+    constraints: np.ndarray = None   # (2, n) 1-based; energy-only correction
+    bonds: np.ndarray = None         # (2, n) 1-based; per-step correction
+    bends: np.ndarray = None         # (2, n) 1-based 1-3 pairs
+    name: str = ""
+
+    @property
+    def natoms(self):
+        return int(self.xp0.shape[0])
+
+    @property
+    def ngrp(self):
+        return int(self.grppt.shape[1])
+
+    @property
+    def nprot(self):
+        return int(self.atomp.max())
+
+    def group_centres(self):
+        """xpg per appbou.f:19-33 (same summation order)."""
+        ng = self.ngrp
+        out = np.zeros((3, ng))
+        for d, c in enumerate((self.xp0, self.yp0, self.zp0)):
+            acc = np.zeros(ng)
+            first = self.grppt[0] - 1
+            size = self.grppt[1] - self.grppt[0] + 1
+            for k in range(int(size.max())):
+                sel = k < size
+                idx = first[sel] + k
+                acc[sel] = acc[sel] + self.pmass[idx] * c[idx]
+            out[d] = acc
+        return out
+
+    def molecule_coms(self):
+        """Molecule centres of mass in reduced coordinates."""
+        npr = self.nprot
+        out = np.zeros((3, npr))
+        mtot = np.bincount(self.atomp - 1, weights=self.mass, minlength=npr)
+        for d, c in enumerate((self.xp0, self.yp0, self.zp0)):
+            out[d] = np.bincount(self.atomp - 1, weights=self.mass * c, minlength=npr) / mtot
+        return out
+
+    def cartesian(self):
+        x = self.co[0, 0] * self.xp0 + self.co[0, 1] * self.yp0 + self.co[0, 2] * self.zp0
+        y = self.co[1, 0] * self.xp0 + self.co[1, 1] * self.yp0 + self.co[1, 2] * self.zp0
+        z = self.co[2, 0] * self.xp0 + self.co[2, 1] * self.yp0 + self.co[2, 2] * self.zp0
+        return x, y, z
+
+    def exclusion_pairs(self):
+        parts = [p for p in (self.constraints, self.bonds, self.bends) if p is not None and p.size]
+        if not parts:
+            return np.zeros((2, 0), dtype=np.int32)
+        return np.concatenate(parts, axis=1).astype(np.int32)
+
+    def mapnl(self):
+        """Exclusion map restating mappa.f: per atom i, [m, j1..jm], j > i ascending."""
+        n = self.natoms
+        pairs = self.exclusion_pairs()
+        lo = np.minimum(pairs[0], pairs[1])
+        hi = np.maximum(pairs[0], pairs[1])
+        keep = lo != hi
+        lo, hi = lo[keep], hi[keep]
+        key = np.unique(lo.astype(np.int64) * (n + 1) + hi)
+        lo = (key // (n + 1)).astype(np.int64)
+        hi = (key % (n + 1)).astype(np.int32)
+        cnt = np.bincount(lo - 1, minlength=n)
+        out = np.empty(n + len(hi), dtype=np.int32)
+        starts = np.concatenate(([0], np.cumsum(cnt + 1)[:-1]))
+        out[starts] = cnt
+        pos = np.repeat(starts + 1, cnt) + (np.arange(len(hi)) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+        out[pos] = hi
+        return out
+
+    def with_coords(self, xp0, yp0, zp0):
+        return dataclasses.replace(self, xp0=np.ascontiguousarray(xp0), yp0=np.ascontiguousarray(yp0),
+                                   zp0=np.ascontiguousarray(zp0))
+
+
+# ---- geometry helpers ----------------------------------------------------
+
+def _random_rotations(rng, n):
+    """Uniform rotations on SO(3) (Shoemake's quaternion method)."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)], axis=1)
+    x, y, z, w = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z); R[:, 0, 1] = 2 * (x * y - z * w); R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w); R[:, 1, 1] = 1 - 2 * (x * x + z * z); R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w); R[:, 2, 1] = 2 * (y * z + x * w); R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def _water_template():
+    half = math.radians(ANG_HOH) / 2.0
+    h1 = np.array([R_OH * math.sin(half), R_OH * math.cos(half), 0.0])
+    h2 = np.array([-R_OH * math.sin(half), R_OH * math.cos(half), 0.0])
+    return np.stack([np.zeros(3), h1, h2])
+
+
+def _place_waters(rng, o_pos):
+    """Rigid 3-site waters with O at o_pos (n,3) and uniform random orientation."""
+    R = _random_rotations(rng, len(o_pos))
+    t = _water_template()
+    sites = np.einsum("nij,aj->nai", R, t) + o_pos[:, None, :]
+    return sites.reshape(-1, 3)
+
+
+def _rsa_sphere(rng, n, radius, dmin):
+    """Random sequential addition of n points in a sphere, pairwise >= dmin."""
+    cell = dmin
+    ncell = int(math.ceil(2 * radius / cell)) + 2
+    grid = {}
+    pts = np.empty((n, 3))
+    k = 0
+    tries = 0
+    while k < n:
+        tries += 1
+        if tries > 200 * n:
+            raise RuntimeError("sphere packing failed")
+        p = (rng.random(3) * 2 - 1) * radius
+        if p @ p > radius * radius:
+            continue
+        c = tuple(((p + radius) // cell).astype(int))
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for j in grid.get((c[0] + dx, c[1] + dy, c[2] + dz), ()):
+                        d = pts[j] - p
+                        if d @ d < dmin * dmin:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if not ok:
+            continue
+        pts[k] = p
+        grid.setdefault(c, []).append(k)
+        k += 1
+    del ncell
+    return pts
+
+
+def _chain_exclusions(pos):
+    """Greedy nearest-unvisited chain through pos: 1-2 (bonds) and 1-3 (bends)."""
+    n = len(pos)
+    order = [0]
+    left = np.ones(n, dtype=bool)
+    left[0] = False
+    cur = 0
+    for _ in range(n - 1):
+        d = np.sum((pos - pos[cur]) ** 2, axis=1)
+        d[~left] = np.inf
+        cur = int(np.argmin(d))
+        left[cur] = False
+        order.append(cur)
+    order = np.array(order)
+    bonds = np.stack([order[:-1], order[1:]])
+    bends = np.stack([order[:-2], order[2:]])
+    return bonds, bends
+
+
+def _lj_tables(sig, eps):
+    """Packed ecc12/ecc6 with Lorentz-Berthelot mixing over per-type sigma/eps."""
+    nt = len(sig)
+    i, j = np.triu_indices(nt)          # i <= j (0-based)
+    ij = (j + 1) * j // 2 + i           # 0-based packed index of (i+1, j+1)
+    s = 0.5 * (sig[i] + sig[j])
+    e = np.sqrt(eps[i] * eps[j])
+    e12, e6 = lj_internal(e, s)
+    ecc12 = np.zeros(nt * (nt + 1) // 2)
+    ecc6 = np.zeros_like(ecc12)
+    ecc12[ij] = e12
+    ecc6[ij] = e6
+    tt = np.zeros((nt, nt), dtype=np.int32)
+    tt[i, j] = ij + 1
+    tt[j, i] = ij + 1
+    return np.asfortranarray(tt), ecc12, ecc6
+
+
+def _assemble(L, solute_pos, solute_q, solute_sig, solute_eps, water_sites, *, alpha, nfft, order, rcut,
+              solute_bonds=None, solute_bends=None, solute_mass=12.011, name=""):
+    ns = len(solute_pos)
+    nw = len(water_sites) // 3
+    n = ns + 3 * nw
+    cart = np.concatenate([solute_pos.reshape(-1, 3), water_sites.reshape(-1, 3)])
+    half = L / 2.0
+    co = np.asfortranarray(np.diag([half, half, half]))
+    oc = np.asfortranarray(np.diag([1.0 / half] * 3))
+    xp = cart / half
+    q = np.concatenate([np.asarray(solute_q, float), np.tile(WATER_Q, nw)])
+    mass = np.concatenate([np.full(ns, solute_mass), np.tile(WATER_MASS, nw)])
+    # types: solute atoms 1..ns (one type each), then O, H
+    sig = np.concatenate([np.asarray(solute_sig, float), [O_SIGMA, 1.0]])
+    eps = np.concatenate([np.asarray(solute_eps, float), [O_EPS, 0.0]])
+    tt, ecc12, ecc6 = _lj_tables(sig, eps)
+    nbtype = np.concatenate([np.arange(1, ns + 1), np.tile([ns + 1, ns + 2, ns + 2], nw)]).astype(np.int32)
+    # groups: each solute atom its own group, each water one group
+    ng = ns + nw
+    first = np.concatenate([np.arange(ns), ns + 3 * np.arange(nw)])
+    last = np.concatenate([np.arange(ns), ns + 3 * np.arange(nw) + 2])
+    grppt = np.asfortranarray(np.stack([first + 1, last + 1]).astype(np.int32))
+    atomg = np.concatenate([np.arange(1, ns + 1), ns + 1 + np.repeat(np.arange(nw), 3)]).astype(np.int32)
+    nsolmol = 1 if ns else 0
+    atomp = np.concatenate([np.ones(ns, dtype=np.int32),
+                            nsolmol + 1 + np.repeat(np.arange(nw), 3)]).astype(np.int32)
+    ss = np.concatenate([np.ones(ns, dtype=np.int32), np.full(3 * nw, 2, dtype=np.int32)])
+    gmass_tot = np.zeros(ng)
+    np.add.at(gmass_tot, atomg - 1, mass)
+    pmass = mass / gmass_tot[atomg - 1]
+    w0 = ns + 3 * np.arange(nw)
+    cons = np.concatenate([np.stack([w0, w0 + 1]), np.stack([w0, w0 + 2]), np.stack([w0 + 1, w0 + 2])], axis=1) + 1
+    sys = System(L=L, xp0=np.ascontiguousarray(xp[:, 0]), yp0=np.ascontiguousarray(xp[:, 1]),
+                 zp0=np.ascontiguousarray(xp[:, 2]), charge=charge_internal(q), mass=mass, nbtype=nbtype,
+                 ntypes=len(sig), type_table=tt, ecc12=ecc12, ecc6=ecc6, grppt=grppt, atomg=atomg, atomp=atomp,
+                 ss_index=ss, pmass=pmass, co=co, oc=oc, volume=8.0 * half ** 3, alpha=alpha,
+                 nfft=tuple(nfft), order=order, rcut=rcut, constraints=cons.astype(np.int32),
+                 bonds=None if solute_bonds is None else (solute_bonds + 1).astype(np.int32),
+                 bends=None if solute_bends is None else (solute_bends + 1).astype(np.int32), name=name)
+    return sys
+
+
+def water_box(seed=1000, L=18.64, nside=6, jitter=0.3, alpha=0.40, nfft=(20, 20, 20), order=5, rcut=8.5):
+    """216 rigid waters on a jittered 6^3 lattice in an 18.64 A box (the small CPU-runnable case)."""
+    rng = np.random.default_rng(seed)
+    a = L / nside
+    g = (np.arange(nside) + 0.5) * a - L / 2.0
+    o = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    o = o + rng.uniform(-jitter, jitter, size=o.shape)
+    sites = _place_waters(rng, o)
+    return _assemble(L, np.zeros((0, 3)), [], [], [], sites, alpha=alpha, nfft=nfft, order=order, rcut=rcut,
+                     name=f"water{len(o)}")
+
+
+def solvated_solute(seed=1001, L=61.0, n_solute=1660, radius=16.0, n_water=7600, alpha=0.35,
+                    nfft=(64, 64, 64), order=5, rcut=10.0, charge_scale=1.0, eps_scale=1.0, solute_seed=None):
+    """Compact random solute (1-2/1-3 chain exclusions) in a box of rigid waters.
+
+    Default sizes: 1,660 solute atoms + 7,600 waters = 24,460 atoms, L = 61 A,
+    grid 64^3, order 5, alpha 0.35, cutoff 10 A.  ``charge_scale`` and
+    ``eps_scale`` scale the solute charges and LJ well depths (Hamiltonian
+    replica scaling: charges by sqrt(s), epsilon by s).  ``solute_seed``
+    fixes the solute independently of the water seed.
+    """
+    rng_s = np.random.default_rng(seed if solute_seed is None else solute_seed)
+    pos = _rsa_sphere(rng_s, n_solute, radius, 1.2)
+    q = rng_s.normal(0.0, 0.35, n_solute)
+    q = q - q.mean()
+    sig = rng_s.uniform(2.0, 3.8, n_solute)
+    eps = rng_s.uniform(0.02, 0.2, n_solute)
+    bonds, bends = _chain_exclusions(pos)
+    rng = np.random.default_rng(seed)
+    if solute_seed is not None:
+        rng = np.random.default_rng(seed + 7)
+    # water oxygens on the finest cubic lattice that leaves room for n_water sites
+    nside = int(math.ceil(L / 3.1067))
+    while True:
+        a = L / nside
+        g = (np.arange(nside) + 0.5) * a - L / 2.0
+        o = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+        far = np.sqrt(np.sum(o * o, axis=1)) > radius + 1.5
+        if far.sum() >= n_water:
+            break
+        nside += 1
+    cand = o[far]
+    pick = np.sort(rng.choice(len(cand), n_water, replace=False))
+    o = cand[pick] + rng.uniform(-0.3, 0.3, size=(n_water, 3))
+    sites = _place_waters(rng, o)
+    sys = _assemble(L, pos, q * math.sqrt(charge_scale), sig, eps * eps_scale, sites, alpha=alpha, nfft=nfft,
+                    order=order, rcut=rcut, solute_bonds=bonds, solute_bends=bends,
+                    name=f"solute{n_solute}+water{n_water}")
+    return sys
+
+
+def config(name, seed=None):
+    """Named workloads: 'water216' (648 atoms) and 'solvated24k' (24,460 atoms)."""
+    if name == "water216":
+        return water_box(seed=1000 if seed is None else seed)
+    if name == "solvated24k":
+        return solvated_solute(seed=1001 if seed is None else seed)
+    raise KeyError(name)
+
+
+def small_solvated(seed=5, L=24.0, n_solute=40, radius=5.0, n_water=380, nfft=(24, 24, 24), rcut=9.0,
+                   alpha=0.35, order=5):
+    """A small solute-in-water box for parity tests that exercise solute exclusions."""
+    return solvated_solute(seed=seed, L=L, n_solute=n_solute, radius=radius, n_water=n_water, alpha=alpha,
+                           nfft=nfft, order=order, rcut=rcut)

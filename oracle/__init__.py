@@ -1,0 +1,397 @@
+"""CPU oracle for the ORAC nonbonded force path -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by ``tests/``, ``__graft_entry__.smoke()`` and bench.py's
+``cpu_baseline`` leg, as the checker.  Two back ends:
+
+* ``liboracle.so``: the plain-C FP64 restatement (``orac_oracle.c``), which
+  also runs on the GPU host;
+* ``_ref/libref.so``: the reference's own Fortran hot path, compiled in
+  place by ``make ref`` in this container only, used to pin the restatement.
+
+Every wrapper takes arrays in the reference's layouts (see ``orac_oracle.h``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+_ref = None
+
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+        _lib = C.CDLL(path)
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libref.so"))
+
+
+def ref():
+    """The compiled reference, or None when it was not built here."""
+    global _ref
+    if _ref is None and ref_available():
+        _ref = C.CDLL(os.path.join(HERE, "_ref", "libref.so"))
+        _ref.refglue_init()
+    return _ref
+
+
+def _f(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+
+
+def _i(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32).ravel(order="F"))
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def recip_of(oc):
+    """recip(i,j) = 0.5*oc(j,i) (fft_pme.f:47-51), Fortran order."""
+    oc = np.asarray(oc, dtype=np.float64)
+    return np.asfortranarray(0.5 * oc.T)
+
+
+# ---------------------------------------------------------------- oracle
+
+def bsp_moduli(order, nfft):
+    out = np.zeros(nfft)
+    lib().orc_load_bsp_moduli(C.c_int(order), C.c_int(nfft), _p(out))
+    return out
+
+
+def fill_bspline(w, order):
+    a, d = np.zeros(order), np.zeros(order)
+    lib().orc_fill_bspline(C.c_double(w), C.c_int(order), _p(a), _p(d))
+    return a, d
+
+
+def charge_grid(x, y, z, q, recip, nfft, order, nthreads=1):
+    """Fractionals, B-splines and the spread grid Q (K3,K2,K1 C-order view)."""
+    n = len(x)
+    k1, k2, k3 = nfft
+    x, y, z, q, r = _f(x), _f(y), _f(z), _f(q), _f(recip)
+    fr = [np.zeros(n) for _ in range(3)]
+    L = lib()
+    L.orc_scaled_fractionals(C.c_int(n), _p(x), _p(y), _p(z), _p(r), C.c_int(k1), C.c_int(k2), C.c_int(k3),
+                             *[_p(a) for a in fr])
+    th = [np.zeros(n * order) for _ in range(6)]
+    L.orc_bspline_coeffs(C.c_int(n), *[_p(a) for a in fr], C.c_int(order), *[_p(a) for a in th])
+    Q = np.zeros(k1 * k2 * k3)
+    L.orc_fill_charge_grid(C.c_int(n), _p(q), _p(th[0]), _p(th[1]), _p(th[2]), *[_p(a) for a in fr],
+                           C.c_int(order), C.c_int(k1), C.c_int(k2), C.c_int(k3), _p(Q), C.c_int(nthreads))
+    return dict(fr=fr, theta=th[:3], dtheta=th[3:], Q=Q.reshape(k3, k2, k1))
+
+
+def fft_r2c(Q, nthreads=1):
+    k3, k2, k1 = Q.shape
+    FQ = np.zeros((k3, k2, k1 // 2 + 1), dtype=np.complex128)
+    Qc = np.ascontiguousarray(Q, dtype=np.float64)
+    lib().orc_fft_r2c(C.c_int(k1), C.c_int(k2), C.c_int(k3), _p(Qc), _p(FQ), C.c_int(nthreads))
+    return FQ
+
+
+def fft_c2r(FQ, k1, nthreads=1):
+    k3, k2, _ = FQ.shape
+    Q = np.zeros((k3, k2, k1))
+    F = np.ascontiguousarray(FQ, dtype=np.complex128)
+    lib().orc_fft_c2r(C.c_int(k1), C.c_int(k2), C.c_int(k3), _p(F), _p(Q), C.c_int(nthreads))
+    return Q
+
+
+def scalar_sum(FQ, alpha, volume, recip, bsp, nfft, rkcut=1e9, nthreads=1):
+    """FFTW-variant scalar_sum; returns (eer, vir(3,3), FQ scaled)."""
+    F = np.ascontiguousarray(FQ, dtype=np.complex128).copy()
+    vir = np.zeros(9)
+    eer = C.c_double()
+    lib().orc_scalar_sum(_p(F), C.c_double(alpha), C.c_double(volume), _p(_f(recip)), _p(_f(bsp[0])),
+                         _p(_f(bsp[1])), _p(_f(bsp[2])), C.c_int(nfft[0]), C.c_int(nfft[1]), C.c_int(nfft[2]),
+                         C.c_double(rkcut), C.byref(eer), _p(vir), C.c_int(nthreads))
+    return eer.value, vir.reshape(3, 3, order="F"), F
+
+
+def fft_pme(s, nthreads=1, pressure=True, rkcut=1e9, fp=None, stressc=None, variant="fftw"):
+    """fft_pme.f semantics on System s: returns eer, forces (+= fp), stressc (+=).
+
+    variant 'fftw' is the production FFTW branch; 'fftpack' the reference's
+    fftpack branch (full complex grid), which the compiled reference runs.
+    """
+    n = s.natoms
+    cm = s.molecule_coms()
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)] if fp is None else [_f(a).copy() for a in fp]
+    st = np.zeros(9) if stressc is None else _f(stressc).copy()
+    bsp = [bsp_moduli(s.order, k) for k in s.nfft]
+    eer = C.c_double()
+    lib().orc_fft_pme(C.c_int(n), _p(_f(s.xp0)), _p(_f(s.yp0)), _p(_f(s.zp0)), _p(_f(cm[0])), _p(_f(cm[1])),
+                      _p(_f(cm[2])), _p(_f(s.charge)), _p(_f(s.co)), _p(_f(s.oc)), C.c_double(s.volume),
+                      C.c_double(s.alpha), C.c_int(s.order), C.c_int(s.nfft[0]), C.c_int(s.nfft[1]),
+                      C.c_int(s.nfft[2]), C.byref(eer), _p(fpx), _p(fpy), _p(fpz), _p(st), _p(_i(s.atomp)),
+                      C.c_int32(1 if pressure else 0), C.c_double(rkcut), _p(bsp[0]), _p(bsp[1]), _p(bsp[2]),
+                      C.c_int(nthreads), C.c_int(0 if variant == "fftw" else 1))
+    return dict(eer=eer.value, f=np.stack([fpx, fpy, fpz]), stress=st.reshape(3, 3, order="F"))
+
+
+def last_pme_times():
+    t = np.zeros(5)
+    lib().orc_last_pme_times(_p(t))
+    return t
+
+
+def lc_index(ncx, ncy, ncz, rcut, co, indmax=2500):
+    nind = np.zeros(2, dtype=np.int32)
+    ii, jj, kk = [np.zeros(2 * indmax, dtype=np.int32) for _ in range(3)]
+    rc = lib().orc_lc_index(C.c_int(indmax), C.c_int(ncx), C.c_int(ncy), C.c_int(ncz), _p(nind), _p(ii), _p(jj),
+                            _p(kk), C.c_double(rcut), _p(_f(co)))
+    if rc != 0:
+        raise ValueError("lc_index: stencil overflow or a cell would be visited twice")
+    return nind, ii, jj, kk
+
+
+def lc_list(ncx, ncy, ncz, stencil, rcut, co, xpg, ypg, zpg, capacity=None):
+    nind, ii, jj, kk = stencil
+    ng = len(xpg)
+    cap = capacity or (ng * 64 + 16)
+    while True:
+        out = np.zeros(cap, dtype=np.int32)
+        total = lib().orc_lc_list(C.c_int(ncx), C.c_int(ncy), C.c_int(ncz), _p(nind), _p(ii), _p(jj), _p(kk),
+                                  C.c_double(rcut), _p(_f(co)), _p(_f(xpg)), _p(_f(ypg)), _p(_f(zpg)),
+                                  C.c_int(ng), _p(out), C.c_long(cap))
+        if total >= 0:
+            return out[:total]
+        cap *= 2
+
+
+def all_groups_list(ngrp):
+    """[ngrp, 1..ngrp]: the list the O(N^2) 'u' update call walks (iz = 0)."""
+    return np.concatenate([[ngrp], np.arange(1, ngrp + 1)]).astype(np.int32)
+
+
+class ForceArgs:
+    """The arrays force.f consumes for System s, in Fortran layouts."""
+
+    def __init__(self, s):
+        self.s = s
+        xpg = s.group_centres()
+        cm = s.molecule_coms()
+        self.xpg, self.ypg, self.zpg = [_f(a) for a in xpg]
+        self.xpcm, self.ypcm, self.zpcm = [_f(a) for a in cm]
+        self.mapnl = _i(s.mapnl())
+        self.tt = _i(s.type_table)
+        self.grppt = _i(s.grppt)
+
+
+def force(s, nnlpp, *, rinn, rout, rtolinn, rtolout, rneigh, iz=1, lskip_ewald=False, ewald=True,
+          nthreads=1, fa=None, massinfty=1.0e20, fp=None, stress=None, nnlpp1_cap=None):
+    fa = fa or ForceArgs(s)
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)] if fp is None else [_f(a).copy() for a in fp]
+    st = np.zeros(9) if stress is None else _f(stress).copy()
+    e = [C.c_double() for _ in range(6)]
+    cap = nnlpp1_cap or (len(nnlpp) + s.ngrp + 16 if iz == 1 else s.ngrp * (s.ngrp // 2 + 2))
+    nn1 = np.zeros(cap, dtype=np.int32)
+    nc1 = C.c_int32()
+    rti = C.c_double(rtolinn)
+    rc = lib().orc_force(
+        _p(_i(s.ss_index)), _p(_f(s.xp0)), _p(_f(s.yp0)), _p(_f(s.zp0)), _p(fa.xpg), _p(fa.ypg), _p(fa.zpg),
+        _p(_f(s.charge)), _p(_i(s.nbtype)), _p(fa.tt), C.c_int(s.ntypes), C.c_int(n), _p(_i(s.atomg)),
+        _p(fa.xpcm), _p(fa.ypcm), _p(fa.zpcm), _p(_i(s.atomp)), _p(_f(s.co)), _p(_f(s.ecc12)), _p(_f(s.ecc6)),
+        C.c_double(s.alpha), _p(fa.mapnl), C.c_int(s.ngrp), _p(fa.grppt), *[C.byref(x) for x in e],
+        _p(fpx), _p(fpy), _p(fpz), _p(st), _p(_i(nnlpp)), _p(nn1), C.c_long(cap), C.c_double(rneigh),
+        C.c_double(rinn), C.c_double(rout), C.byref(rti), C.c_double(rtolout), _p(_f(s.pmass)), C.c_int(iz),
+        C.byref(nc1), _p(_f(s.mass)), C.c_double(massinfty), C.c_int32(int(lskip_ewald)), C.c_int32(int(ewald)),
+        C.c_int(nthreads))
+    if rc != 0:
+        raise RuntimeError("orc_force: nnlpp1 capacity exceeded")
+    names = ["uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"]
+    out = {k: v.value for k, v in zip(names, e)}
+    out.update(f=np.stack([fpx, fpy, fpz]), stress=st.reshape(3, 3, order="F"), nnlpp1=nn1[:nc1.value].copy(),
+               ncount1=nc1.value, rtolinn=rti.value)
+    return out
+
+
+def decide_force(rinn, alpha):
+    return bool(lib().orc_decide_force(C.c_double(rinn), C.c_double(alpha)))
+
+
+def ferrf(s, pairs, iz, fudge=1.0, fp=None):
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)] if fp is None else [_f(a).copy() for a in fp]
+    a, b = C.c_double(), C.c_double()
+    lst = _i(pairs)
+    rc = lib().orc_ferrf(_p(_i(s.ss_index)), C.c_double(s.alpha), _p(_f(s.charge)), C.c_double(fudge), _p(_f(x)),
+                         _p(_f(y)), _p(_f(z)), C.c_int(iz), _p(lst), C.c_int(lst.size // 2), C.byref(a), C.byref(b),
+                         _p(fpx), _p(fpy), _p(fpz), C.c_int32(0), C.c_int(0))
+    assert rc == 0
+    return dict(slt=a.value, slv=b.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def cself(s, kcut=1e9):
+    a, b = C.c_double(), C.c_double()
+    lib().orc_cself(_p(_i(s.ss_index)), C.c_int(s.natoms), C.c_double(s.alpha), C.c_double(kcut), _p(_f(s.charge)),
+                    C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+# ---------------------------------------------------------------- reference
+
+def _big_stack(fn, *args):
+    """Run fn on a thread with a 1 GiB stack (the reference's fft_pme.f keeps
+    ~75 MB of fixed-size local arrays)."""
+    res = {}
+
+    def run():
+        try:
+            res["v"] = fn(*args)
+        except BaseException as e:  # pragma: no cover
+            res["e"] = e
+
+    old = threading.stack_size(1 << 30)
+    try:
+        t = threading.Thread(target=run)
+        t.start()
+        t.join()
+    finally:
+        threading.stack_size(old)
+    if "e" in res:
+        raise res["e"]
+    return res.get("v")
+
+
+def _ri(v):
+    return C.byref(C.c_int32(int(v)))
+
+
+def _rd(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def ref_bsp_moduli(order, nfft, natoms=1):
+    R = ref()
+    R.refglue_pme_init(C.c_int(natoms), C.c_int(nfft[0]), C.c_int(nfft[1]), C.c_int(nfft[2]), C.c_int(order))
+    b = [np.zeros(k) for k in nfft]
+    R.refglue_get_bsp(_p(b[0]), _p(b[1]), _p(b[2]), C.c_int(nfft[0]), C.c_int(nfft[1]), C.c_int(nfft[2]))
+    return b
+
+
+def ref_fft_pme(s, pressure=True, rkcut=1e9):
+    """The reference fft_pme (fftpack branch of do_pmesh_kspace)."""
+    R = ref()
+    n = s.natoms
+    R.refglue_pme_init(C.c_int(n), C.c_int(s.nfft[0]), C.c_int(s.nfft[1]), C.c_int(s.nfft[2]), C.c_int(s.order))
+    cm = s.molecule_coms()
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    st = np.zeros(9)
+    eer = C.c_double()
+    arrs = [_f(s.xp0), _f(s.yp0), _f(s.zp0), _f(cm[0]), _f(cm[1]), _f(cm[2]), _f(s.charge), _f(s.co), _f(s.oc)]
+    atomp, grppt = _i(s.atomp), _i(s.grppt)
+
+    def call():
+        R.fft_pme_(_ri(n), *[_p(a) for a in arrs[:7]], _p(arrs[7]), _p(arrs[8]), _rd(s.volume), _rd(s.alpha),
+                   _ri(s.order), _ri(s.nfft[0]), _ri(s.nfft[1]), _ri(s.nfft[2]), C.byref(eer), _p(fpx), _p(fpy),
+                   _p(fpz), _p(st), _p(atomp), _p(grppt), _ri(1 if pressure else 0), _rd(rkcut))
+
+    _big_stack(call)
+    return dict(eer=eer.value, f=np.stack([fpx, fpy, fpz]), stress=st.reshape(3, 3, order="F"))
+
+
+def ref_scalar_sum_fftw(FQ, alpha, volume, recip, bsp, nfft, rkcut=1e9):
+    R = ref()
+    F = np.ascontiguousarray(FQ, dtype=np.complex128).copy()
+    vir = np.zeros(9)
+    eer = C.c_double()
+    R.scalar_sum_fftw_(_p(F), _rd(alpha), _rd(volume), _p(_f(recip)), _p(_f(bsp[0])), _p(_f(bsp[1])),
+                       _p(_f(bsp[2])), _ri(nfft[0]), _ri(nfft[1]), _ri(nfft[2]), C.byref(eer), _p(vir), _rd(rkcut))
+    return eer.value, vir.reshape(3, 3, order="F"), F
+
+
+def ref_lc_index(ncx, ncy, ncz, rcut, co, indmax=2500):
+    R = ref()
+    nind = np.zeros(2, dtype=np.int32)
+    ii, jj, kk = [np.zeros(2 * indmax, dtype=np.int32) for _ in range(3)]
+    R.lc_index_(_ri(indmax), _ri(ncx), _ri(ncy), _ri(ncz), _p(nind), _p(ii), _p(jj), _p(kk), _rd(rcut), _p(_f(co)))
+    return nind, ii, jj, kk
+
+
+def ref_lc_list(ncx, ncy, ncz, stencil, rcut, co, xpg, ypg, zpg):
+    R = ref()
+    nind, ii, jj, kk = stencil
+    ng = len(xpg)
+    out = np.zeros(ng * 400 + 16, dtype=np.int32)
+    R.lc_list_(_ri(ncx), _ri(ncy), _ri(ncz), _p(nind), _p(ii), _p(jj), _p(kk), _rd(rcut), _p(_f(co)), _p(_f(xpg)),
+               _p(_f(ypg)), _p(_f(zpg)), _ri(ng), _p(out), _ri(6))
+    # walk the per-group [nv, l...] layout to find its length
+    pos = 0
+    for _ in range(ng):
+        pos += int(out[pos]) + 1
+    return out[:pos].copy()
+
+
+def ref_force(s, nnlpp, *, rinn, rout, rtolinn, rtolout, rneigh, iz=1, lskip_ewald=False, ewald=True,
+              massinfty=1.0e20, fa=None):
+    R = ref()
+    fa = fa or ForceArgs(s)
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    st = np.zeros(9)
+    e = [C.c_double() for _ in range(6)]
+    cap = len(nnlpp) + s.ngrp + 16 if iz == 1 else s.ngrp * (s.ngrp // 2 + 2)
+    nn1 = np.zeros(cap, dtype=np.int32)
+    nc1 = C.c_int32()
+    rti = C.c_double(rtolinn)
+    keep = [_i(s.ss_index), _f(s.xp0), _f(s.yp0), _f(s.zp0), _f(s.charge), _i(s.nbtype), _i(s.atomg),
+            _i(s.atomp), _f(s.co), _f(s.ecc12), _f(s.ecc6), _i(nnlpp), _f(s.pmass), _f(s.mass),
+            np.zeros(s.ngrp, dtype=np.int32)]
+    (ss, x, y, z, q, nbt, atg, atp, co, e12, e6, nl, pm, ms, groupp) = keep
+
+    def call():
+        R.force_(_p(ss), _p(x), _p(y), _p(z), _p(fa.xpg), _p(fa.ypg), _p(fa.zpg), _p(q), _p(nbt), _p(fa.tt),
+                 _ri(s.ntypes), _ri(n), _p(atg), _p(fa.xpcm), _p(fa.ypcm), _p(fa.zpcm), _p(groupp), _p(atp),
+                 _p(co), _p(e12), _p(e6), _rd(s.alpha), _p(fa.mapnl), _ri(s.ngrp), _p(fa.grppt),
+                 *[C.byref(v) for v in e], _p(fpx), _p(fpy), _p(fpz), _p(st), _p(nl), _p(nn1), _rd(rneigh),
+                 _rd(rinn), _rd(rout), C.byref(rti), _rd(rtolout), _p(pm), _ri(iz), C.byref(nc1), _p(ms),
+                 _rd(massinfty), _ri(int(lskip_ewald)), _ri(int(ewald)))
+
+    _big_stack(call)
+    names = ["uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"]
+    out = {k: v.value for k, v in zip(names, e)}
+    out.update(f=np.stack([fpx, fpy, fpz]), stress=st.reshape(3, 3, order="F"), nnlpp1=nn1[:nc1.value].copy(),
+               ncount1=nc1.value, rtolinn=rti.value)
+    return out
+
+
+def ref_ferrf(s, pairs, iz, fudge=1.0):
+    R = ref()
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b = C.c_double(), C.c_double()
+    lst = _i(pairs)
+    dummy = np.zeros(8)
+    keep = [_i(s.ss_index), _f(s.charge), _f(x), _f(y), _f(z)]
+    R.ferrf_(_p(keep[0]), _rd(s.alpha), _p(keep[1]), _rd(fudge), _p(keep[2]), _p(keep[3]), _p(keep[4]), _ri(iz),
+             _p(lst), _ri(lst.size // 2), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _ri(0), _p(dummy),
+             _rd(0.0), _rd(0.0), _ri(0), _ri(n))
+    return dict(slt=a.value, slv=b.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def ref_cself(s, kcut=1e9):
+    R = ref()
+    a, b = C.c_double(), C.c_double()
+    R.cself_(_p(_i(s.ss_index)), _ri(s.natoms), _rd(s.alpha), _rd(kcut), _p(_f(s.charge)), C.byref(a), C.byref(b))
+    return a.value, b.value

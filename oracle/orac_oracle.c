@@ -1,0 +1,1261 @@
+/*
+ * orac_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY) for the ORAC
+ * nonbonded force path: an FP64 restatement of the reference routines.
+ *
+ * Only tests, smoke() and bench.py's cpu_baseline leg use this file, as
+ * the checker.  Expressions keep the reference's association order and
+ * the build disables FMA contraction, so the serial path reproduces the
+ * reference's serial build bit for bit wherever the reference is
+ * deterministic.  Every function cites the reference lines it restates.
+ *
+ * The reference's 3-D FFT is FFTW3 (pme_fftw3.f90:114,116), which is not
+ * part of the reference tree; this file has its own mixed-radix FFT with
+ * the same conventions (unnormalised, forward e^{-i}, r2c/c2r over a
+ * K1-fastest grid, c2r ignoring the imaginary parts of the k1 = 0 and
+ * Nyquist planes).
+ */
+#include "orac_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#pragma STDC FP_CONTRACT OFF
+
+/* A&S 7.1.26 coefficients, force.f:123-125 (also ferrf.f, cself.f, fnbgrp.f) */
+static const double A1 = 0.2548296, A2 = -0.28449674, A3 = 1.4214137, A4 = -1.453152, A5 = 1.0614054,
+                    QP = 0.3275911;
+/* the reference's default-real pi literal (scalar_sum.f:18, pme_fftw3.f90:214) */
+static const double PI_F32 = (double)3.14159265358979323846f;
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static double fnint(double x) { return round(x); } /* Fortran ANINT/DNINT: half away from zero */
+
+/* ------------------------------------------------------------------ */
+/* B-splines: pme_fftw3.f90:537-576                                    */
+/* ------------------------------------------------------------------ */
+static void one_pass(double *c, double x, int k) {
+  double div = 1.0 / (k - 1);
+  c[k - 1] = div * x * c[k - 2];
+  for (int j = 1; j <= k - 2; ++j)
+    c[k - j - 1] = div * ((x + j) * c[k - j - 2] + ((double)(k - j) - x) * c[k - j - 1]);
+  c[0] = div * (1 - x) * c[0];
+}
+
+void orc_fill_bspline(double w, int order, double *array, double *darray) {
+  array[order - 1] = 0.0; /* init */
+  array[1] = w;
+  array[0] = 1.0 - w;
+  for (int k = 3; k <= order - 1; ++k) one_pass(array, w, k);
+  darray[0] = -array[0]; /* diff */
+  for (int j = 2; j <= order; ++j) darray[j - 1] = array[j - 2] - array[j - 1];
+  one_pass(array, w, order);
+}
+
+/* pme_fftw3.f90:165-230 */
+void orc_load_bsp_moduli(int order, int nfft, double *bsp_mod) {
+  double array[32], darray[32];
+  double *bsp_arr = (double *)calloc((size_t)nfft + order + 2, sizeof(double));
+  orc_fill_bspline(0.0, order, array, darray);
+  for (int i = 2; i <= order + 1; ++i) bsp_arr[i - 1] = array[i - 2];
+  const double twopi = 2.0 * PI_F32;
+  for (int k = 1; k <= nfft; ++k) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 1; j <= nfft; ++j) {
+      double arg = twopi * (k - 1) * (j - 1) / nfft;
+      s1 = s1 + bsp_arr[j - 1] * cos(arg);
+      s2 = s2 + bsp_arr[j - 1] * sin(arg);
+    }
+    bsp_mod[k - 1] = s1 * s1 + s2 * s2;
+  }
+  for (int k = 1; k <= nfft; ++k)
+    if (bsp_mod[k - 1] < 1.0e-7) bsp_mod[k - 1] = 0.5 * (bsp_mod[k - 2] + bsp_mod[k]);
+  free(bsp_arr);
+}
+
+/* pme_fftw3.f90:132-163 */
+void orc_scaled_fractionals(int n, const double *x, const double *y, const double *z, const double *recip,
+                            int k1, int k2, int k3, double *fr1, double *fr2, double *fr3) {
+  for (int i = 0; i < n; ++i) {
+    double w = x[i] * recip[0] + y[i] * recip[1] + z[i] * recip[2];
+    fr1[i] = k1 * (w - fnint(w) + 0.5);
+    w = x[i] * recip[3] + y[i] * recip[4] + z[i] * recip[5];
+    fr2[i] = k2 * (w - fnint(w) + 0.5);
+    w = x[i] * recip[6] + y[i] * recip[7] + z[i] * recip[8];
+    fr3[i] = k3 * (w - fnint(w) + 0.5);
+  }
+}
+
+/* pme_fftw3.f90:487-512 */
+void orc_bspline_coeffs(int n, const double *fr1, const double *fr2, const double *fr3, int order,
+                        double *th1, double *th2, double *th3, double *dth1, double *dth2, double *dth3) {
+  for (int i = 0; i < n; ++i) {
+    orc_fill_bspline(fr1[i] - (int)fr1[i], order, th1 + (size_t)i * order, dth1 + (size_t)i * order);
+    orc_fill_bspline(fr2[i] - (int)fr2[i], order, th2 + (size_t)i * order, dth2 + (size_t)i * order);
+    orc_fill_bspline(fr3[i] - (int)fr3[i], order, th3 + (size_t)i * order, dth3 + (size_t)i * order);
+  }
+}
+
+/* 0-based grid index for the 0-based offset k0 (pme_fftw3.f90:274: k0+1+(K-isign(K,k0))/2, 1-based) */
+static inline int wrapidx(int k0, int K) { return k0 >= 0 ? k0 : k0 + K; }
+
+/* pme_fftw3.f90:234-298; the OpenMP build reduces private grids (REDUCTION(+:Q)) */
+void orc_fill_charge_grid(int n, const double *q, const double *th1, const double *th2, const double *th3,
+                          const double *fr1, const double *fr2, const double *fr3, int order,
+                          int k1, int k2, int k3, double *Q, int nthreads) {
+  size_t G = (size_t)k1 * k2 * k3;
+  memset(Q, 0, G * sizeof(double));
+  if (nthreads <= 1) {
+    for (int a = 0; a < n; ++a) {
+      int kk0 = (int)fr3[a] - order;
+      for (int i3 = 1; i3 <= order; ++i3) {
+        kk0 += 1;
+        int kk = wrapidx(kk0, k3);
+        int jj0 = (int)fr2[a] - order;
+        for (int i2 = 1; i2 <= order; ++i2) {
+          jj0 += 1;
+          int jj = wrapidx(jj0, k2);
+          double prod = th2[(size_t)a * order + i2 - 1] * th3[(size_t)a * order + i3 - 1] * q[a];
+          int ii0 = (int)fr1[a] - order;
+          for (int i1 = 1; i1 <= order; ++i1) {
+            ii0 += 1;
+            int ii = wrapidx(ii0, k1);
+            size_t g = (size_t)ii + (size_t)k1 * (jj + (size_t)k2 * kk);
+            Q[g] = Q[g] + th1[(size_t)a * order + i1 - 1] * prod;
+          }
+        }
+      }
+    }
+    return;
+  }
+#ifdef _OPENMP
+  double *priv = (double *)calloc(G * (size_t)nthreads, sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+  {
+    int t = omp_get_thread_num();
+    double *Qt = priv + G * (size_t)t;
+#pragma omp for schedule(static)
+    for (int a = 0; a < n; ++a) {
+      int kk0 = (int)fr3[a] - order;
+      for (int i3 = 1; i3 <= order; ++i3) {
+        kk0 += 1;
+        int kk = wrapidx(kk0, k3);
+        int jj0 = (int)fr2[a] - order;
+        for (int i2 = 1; i2 <= order; ++i2) {
+          jj0 += 1;
+          int jj = wrapidx(jj0, k2);
+          double prod = th2[(size_t)a * order + i2 - 1] * th3[(size_t)a * order + i3 - 1] * q[a];
+          int ii0 = (int)fr1[a] - order;
+          for (int i1 = 1; i1 <= order; ++i1) {
+            ii0 += 1;
+            int ii = wrapidx(ii0, k1);
+            size_t g = (size_t)ii + (size_t)k1 * (jj + (size_t)k2 * kk);
+            Qt[g] += th1[(size_t)a * order + i1 - 1] * prod;
+          }
+        }
+      }
+    }
+#pragma omp for schedule(static)
+    for (long g = 0; g < (long)G; ++g) {
+      double s = 0.0;
+      for (int tt = 0; tt < nthreads; ++tt) s += priv[G * (size_t)tt + g];
+      Q[g] = s;
+    }
+  }
+  free(priv);
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* mixed-radix complex FFT (recursive, radix 2/3/4/5 + generic prime)  */
+/* ------------------------------------------------------------------ */
+typedef struct { double re, im; } cpx;
+
+static void fft_rec(int n, const cpx *in, int stride, cpx *out, int sign, const cpx *tw, int twstride, cpx *scratch) {
+  if (n == 1) { out[0] = in[0]; return; }
+  int r;
+  if (n % 4 == 0) r = 4;
+  else if (n % 2 == 0) r = 2;
+  else if (n % 3 == 0) r = 3;
+  else if (n % 5 == 0) r = 5;
+  else {
+    for (r = 7; r * r <= n; r += 2) if (n % r == 0) break;
+    if (r * r > n) r = n;
+  }
+  int m = n / r;
+  /* sub-transforms of the r decimated sequences */
+  for (int s = 0; s < r; ++s)
+    fft_rec(m, in + (size_t)s * stride, stride * r, out + (size_t)s * m, sign, tw, twstride * r, scratch);
+  /* butterflies: X[k + q m] = sum_s W_n^{s(k+q m)} Y_s[k] */
+  cpx y[64];
+  cpx *yy = r <= 64 ? y : scratch;
+  for (int k = 0; k < m; ++k) {
+    for (int s = 0; s < r; ++s) yy[s] = out[(size_t)s * m + k];
+    for (int qq = 0; qq < r; ++qq) {
+      double sr = 0.0, si = 0.0;
+      for (int s = 0; s < r; ++s) {
+        long e = ((long)s * (k + (long)qq * m)) % n;
+        cpx w = tw[e * twstride];
+        double wi = sign < 0 ? w.im : -w.im;
+        sr += yy[s].re * w.re - yy[s].im * wi;
+        si += yy[s].re * wi + yy[s].im * w.re;
+      }
+      out[(size_t)qq * m + k].re = sr;
+      out[(size_t)qq * m + k].im = si;
+    }
+  }
+}
+
+/* tw[e] = exp(-2 pi i e / n) */
+static cpx *make_twiddles(int n) {
+  cpx *tw = (cpx *)malloc(sizeof(cpx) * (size_t)n);
+  for (int e = 0; e < n; ++e) {
+    double a = -2.0 * M_PI * (double)e / (double)n;
+    tw[e].re = cos(a);
+    tw[e].im = sin(a);
+  }
+  return tw;
+}
+
+static void fft_line(int n, cpx *data, int sign, const cpx *tw, cpx *work, cpx *scratch) {
+  fft_rec(n, data, 1, work, sign, tw, 1, scratch);
+  memcpy(data, work, sizeof(cpx) * (size_t)n);
+}
+
+void orc_fft1d(int n, double *data, int sign) {
+  cpx *tw = make_twiddles(n);
+  cpx *work = (cpx *)malloc(sizeof(cpx) * (size_t)n * 2);
+  fft_line(n, (cpx *)data, sign, tw, work, work + n);
+  free(work);
+  free(tw);
+}
+
+/* transform the complex array a(n1,n2,n3) (n1 fastest) along one axis */
+static void fft_axis(cpx *a, int n1, int n2, int n3, int axis, int sign, int nthreads) {
+  int n = axis == 0 ? n1 : (axis == 1 ? n2 : n3);
+  cpx *tw = make_twiddles(n);
+  long nlines = (long)n1 * n2 * n3 / n;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+  {
+    cpx *buf = (cpx *)malloc(sizeof(cpx) * (size_t)n * 3);
+#pragma omp for schedule(static)
+    for (long l = 0; l < nlines; ++l) {
+      size_t base, step;
+      if (axis == 0) { base = (size_t)l * n1; step = 1; }
+      else if (axis == 1) { long i = l % n1, k = l / n1; base = (size_t)i + (size_t)n1 * n2 * k; step = n1; }
+      else { base = (size_t)l; step = (size_t)n1 * n2; }
+      for (int t = 0; t < n; ++t) buf[t] = a[base + step * t];
+      fft_line(n, buf, sign, tw, buf + n, buf + 2 * n);
+      for (int t = 0; t < n; ++t) a[base + step * t] = buf[t];
+    }
+    free(buf);
+  }
+  free(tw);
+}
+
+void orc_fft_r2c(int k1, int k2, int k3, const double *Q, double *FQ, int nthreads) {
+  int h = k1 / 2 + 1;
+  cpx *line = NULL;
+  cpx *F = (cpx *)FQ;
+  /* x: full complex transform of each real line, keep k1 = 0..K1/2 */
+  cpx *tw = make_twiddles(k1);
+  line = (cpx *)malloc(sizeof(cpx) * (size_t)k1 * 3);
+  for (long l = 0; l < (long)k2 * k3; ++l) {
+    for (int i = 0; i < k1; ++i) { line[i].re = Q[(size_t)l * k1 + i]; line[i].im = 0.0; }
+    fft_line(k1, line, -1, tw, line + k1, line + 2 * k1);
+    for (int i = 0; i < h; ++i) F[(size_t)l * h + i] = line[i];
+  }
+  free(line);
+  free(tw);
+  fft_axis(F, h, k2, k3, 1, -1, nthreads);
+  fft_axis(F, h, k2, k3, 2, -1, nthreads);
+}
+
+void orc_fft_c2r(int k1, int k2, int k3, const double *FQ, double *Q, int nthreads) {
+  int h = k1 / 2 + 1;
+  size_t nc = (size_t)h * k2 * k3;
+  cpx *F = (cpx *)malloc(sizeof(cpx) * nc);
+  memcpy(F, FQ, sizeof(cpx) * nc);
+  fft_axis(F, h, k2, k3, 2, +1, nthreads);
+  fft_axis(F, h, k2, k3, 1, +1, nthreads);
+  cpx *tw = make_twiddles(k1);
+  cpx *line = (cpx *)malloc(sizeof(cpx) * (size_t)k1 * 3);
+  for (long l = 0; l < (long)k2 * k3; ++l) {
+    for (int i = 0; i < h; ++i) line[i] = F[(size_t)l * h + i];
+    line[0].im = 0.0;
+    if (k1 % 2 == 0) line[k1 / 2].im = 0.0;
+    for (int i = h; i < k1; ++i) { line[i].re = line[k1 - i].re; line[i].im = -line[k1 - i].im; }
+    fft_line(k1, line, +1, tw, line + k1, line + 2 * k1);
+    for (int i = 0; i < k1; ++i) Q[(size_t)l * k1 + i] = line[i].re;
+  }
+  free(line);
+  free(tw);
+  free(F);
+}
+
+/* ------------------------------------------------------------------ */
+/* scalar_sum.f:1-135 (FFTW variant)                                   */
+/* ------------------------------------------------------------------ */
+void orc_scalar_sum(double *FQ, double ewaldcof, double volume, const double *recip, const double *bsp1,
+                    const double *bsp2, const double *bsp3, int nfft1, int nfft2, int nfft3, double rkcut,
+                    double *eer, double *vir, int nthreads) {
+  const double pi = PI_F32;
+  const int ndim1 = 1 + nfft1 / 2;
+  const long indtop = (long)ndim1 * nfft2 * nfft3;
+  const double rkcut2 = 0.25 * rkcut * rkcut / (pi * pi);
+  const double fac = pi * pi / (ewaldcof * ewaldcof);
+  const long nff = (long)ndim1 * nfft2;
+  int nf1 = nfft1 / 2; if (2 * nf1 < nfft1) nf1 = nf1 + 1;
+  int nf2 = nfft2 / 2; if (2 * nf2 < nfft2) nf2 = nf2 + 1;
+  int nf3 = nfft3 / 2; if (2 * nf3 < nfft3) nf3 = nf3 + 1;
+  double energy = 0.0, v11 = 0, v22 = 0, v33 = 0, v12 = 0, v13 = 0, v23 = 0;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) \
+    reduction(+ : energy, v11, v22, v33, v12, v13, v23) if (nthreads > 1)
+  for (long ind = 1; ind <= indtop - 1; ++ind) {
+    int k3 = (int)(ind / nff) + 1;
+    long jnd = ind - (long)(k3 - 1) * nff;
+    int k2 = (int)(jnd / ndim1) + 1;
+    int k1 = (int)(jnd - (long)(k2 - 1) * ndim1) + 1;
+    int m1 = k1 - 1; if (k1 > nf1) m1 = k1 - 1 - nfft1;
+    int m2 = k2 - 1; if (k2 > nf2) m2 = k2 - 1 - nfft2;
+    int m3 = k3 - 1; if (k3 > nf3) m3 = k3 - 1 - nfft3;
+    double mhat1 = recip[0] * m1 + recip[3] * m2 + recip[6] * m3;
+    double mhat2 = recip[1] * m1 + recip[4] * m2 + recip[7] * m3;
+    double mhat3 = recip[2] * m1 + recip[5] * m2 + recip[8] * m3;
+    double msq = mhat1 * mhat1 + mhat2 * mhat2 + mhat3 * mhat3;
+    double eterm;
+    double *qc = FQ + 2 * ind;
+    if (msq > rkcut2) {
+      eterm = 0.0;
+    } else {
+      double denom = pi * volume * bsp1[k1 - 1] * bsp2[k2 - 1] * bsp3[k3 - 1] * msq;
+      eterm = exp(-fac * msq) / denom;
+      double vterm = 2.0 * (fac * msq + 1.0) / msq;
+      double qr = qc[0], qi = qc[1];
+      double struc2, vvir, fac12, fac13, fac23;
+      if (k1 == 1 || (k1 == ndim1 && nfft1 % 2 == 0)) {
+        struc2 = (eterm * qr) * qr - (eterm * qi) * (-qi);
+        vvir = struc2 * vterm;
+        fac12 = 1.0; fac13 = 1.0; fac23 = 1.0;
+      } else {
+        double e2 = 2.0 * eterm;
+        struc2 = (e2 * qr) * qr - (e2 * qi) * (-qi);
+        vvir = struc2 * vterm;
+        fac12 = (mhat1 * mhat2 < 0.0 && k2 <= ndim1) ? 0.0 : 1.0;
+        fac13 = (mhat1 * mhat3 < 0.0 && k3 <= ndim1) ? 0.0 : 1.0;
+        int neg = mhat2 * mhat3 < 0.0;
+        if (k2 <= ndim1 && k3 <= ndim1) fac23 = neg ? 0.0 : 1.0;
+        else if (k2 > ndim1 && k3 <= ndim1) fac23 = neg ? 1.0 : 0.0;
+        else if (k3 > ndim1 && k2 <= ndim1) fac23 = neg ? 1.0 : 0.0;
+        else fac23 = neg ? 0.0 : 1.0;
+      }
+      energy = energy + struc2;
+      v11 = v11 + struc2 * (vterm * mhat1 * mhat1 - 1.0);
+      v22 = v22 + struc2 * (vterm * mhat2 * mhat2 - 1.0);
+      v33 = v33 + struc2 * (vterm * mhat3 * mhat3 - 1.0);
+      v12 = v12 + vvir * mhat1 * mhat2 * fac12;
+      v13 = v13 + vvir * mhat1 * mhat3 * fac13;
+      v23 = v23 + vvir * mhat2 * mhat3 * fac23;
+    }
+    qc[0] = eterm * qc[0];
+    qc[1] = eterm * qc[1];
+  }
+  *eer = 0.5 * energy;
+  vir[0] = 0.5 * v11; vir[4] = 0.5 * v22; vir[8] = 0.5 * v33;
+  vir[3] = 0.5 * v12; vir[6] = 0.5 * v13; vir[7] = 0.5 * v23;
+  vir[1] = vir[3]; vir[2] = vir[6]; vir[5] = vir[7];
+}
+
+/* scalar_sum.f:137-217 (fftpack variant): every point of the full complex
+ * grid Q(2,K1,K2,K3), weight 1, no virial masks. */
+void orc_scalar_sum_full(double *Qc, double ewaldcof, double volume, const double *recip, const double *bsp1,
+                         const double *bsp2, const double *bsp3, int nfft1, int nfft2, int nfft3, double rkcut,
+                         double *eer, double *vir) {
+  const double pi = PI_F32;
+  const long indtop = (long)nfft1 * nfft2 * nfft3;
+  const double rkcut2 = 0.25 * rkcut * rkcut / (pi * pi);
+  const double fac = pi * pi / (ewaldcof * ewaldcof);
+  const long nff = (long)nfft1 * nfft2;
+  int nf1 = nfft1 / 2; if (2 * nf1 < nfft1) nf1 = nf1 + 1;
+  int nf2 = nfft2 / 2; if (2 * nf2 < nfft2) nf2 = nf2 + 1;
+  int nf3 = nfft3 / 2; if (2 * nf3 < nfft3) nf3 = nf3 + 1;
+  double energy = 0.0, v[9] = {0};
+  for (long ind = 1; ind <= indtop - 1; ++ind) {
+    int k3 = (int)(ind / nff) + 1;
+    long jnd = ind - (long)(k3 - 1) * nff;
+    int k2 = (int)(jnd / nfft1) + 1;
+    int k1 = (int)(jnd - (long)(k2 - 1) * nfft1) + 1;
+    int m1 = k1 - 1; if (k1 > nf1) m1 = k1 - 1 - nfft1;
+    int m2 = k2 - 1; if (k2 > nf2) m2 = k2 - 1 - nfft2;
+    int m3 = k3 - 1; if (k3 > nf3) m3 = k3 - 1 - nfft3;
+    double mhat1 = recip[0] * m1 + recip[3] * m2 + recip[6] * m3;
+    double mhat2 = recip[1] * m1 + recip[4] * m2 + recip[7] * m3;
+    double mhat3 = recip[2] * m1 + recip[5] * m2 + recip[8] * m3;
+    double msq = mhat1 * mhat1 + mhat2 * mhat2 + mhat3 * mhat3;
+    double eterm;
+    double *qc = Qc + 2 * ind;
+    if (msq > rkcut2) {
+      eterm = 0.0;
+    } else {
+      double denom = pi * volume * bsp1[k1 - 1] * bsp2[k2 - 1] * bsp3[k3 - 1] * msq;
+      eterm = exp(-fac * msq) / denom;
+      double vterm = 2.0 * (fac * msq + 1.0) / msq;
+      double struc2 = qc[0] * qc[0] + qc[1] * qc[1];
+      energy = energy + eterm * struc2;
+      v[0] = v[0] + eterm * struc2 * (vterm * mhat1 * mhat1 - 1.0);
+      v[3] = v[3] + eterm * struc2 * (vterm * mhat1 * mhat2);
+      v[6] = v[6] + eterm * struc2 * (vterm * mhat1 * mhat3);
+      v[4] = v[4] + eterm * struc2 * (vterm * mhat2 * mhat2 - 1.0);
+      v[7] = v[7] + eterm * struc2 * (vterm * mhat2 * mhat3);
+      v[8] = v[8] + eterm * struc2 * (vterm * mhat3 * mhat3 - 1.0);
+    }
+    qc[0] = eterm * qc[0];
+    qc[1] = eterm * qc[1];
+  }
+  *eer = 0.5 * energy;
+  v[1] = v[3]; v[2] = v[6]; v[5] = v[7];
+  for (int k = 0; k < 9; ++k) vir[k] = 0.5 * v[k];
+}
+
+/* pme_fftw3.f90:311-372 */
+void orc_grad_sum(int n, const double *q, const double *recip, const double *th1, const double *th2,
+                  const double *th3, const double *dth1, const double *dth2, const double *dth3,
+                  double *fx, double *fy, double *fz, const double *fr1, const double *fr2, const double *fr3,
+                  int order, int k1, int k2, int k3, const double *Q, int nthreads) {
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
+  for (int a = 0; a < n; ++a) {
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    const double *t1 = th1 + (size_t)a * order, *t2 = th2 + (size_t)a * order, *t3 = th3 + (size_t)a * order;
+    const double *d1 = dth1 + (size_t)a * order, *d2 = dth2 + (size_t)a * order, *d3 = dth3 + (size_t)a * order;
+    int kk0 = (int)fr3[a] - order;
+    for (int i3 = 0; i3 < order; ++i3) {
+      kk0 += 1;
+      int kk = wrapidx(kk0, k3);
+      int jj0 = (int)fr2[a] - order;
+      for (int i2 = 0; i2 < order; ++i2) {
+        jj0 += 1;
+        int jj = wrapidx(jj0, k2);
+        int ii0 = (int)fr1[a] - order;
+        for (int i1 = 0; i1 < order; ++i1) {
+          ii0 += 1;
+          int ii = wrapidx(ii0, k1);
+          double term = q[a] * Q[(size_t)ii + (size_t)k1 * (jj + (size_t)k2 * kk)];
+          f1 = f1 - k1 * term * d1[i1] * t2[i2] * t3[i3];
+          f2 = f2 - k2 * term * t1[i1] * d2[i2] * t3[i3];
+          f3 = f3 - k3 * term * t1[i1] * t2[i2] * d3[i3];
+        }
+      }
+    }
+    fx[a] = fx[a] + recip[0] * f1 + recip[3] * f2 + recip[6] * f3;
+    fy[a] = fy[a] + recip[1] * f1 + recip[4] * f2 + recip[7] * f3;
+    fz[a] = fz[a] + recip[2] * f1 + recip[5] * f2 + recip[8] * f3;
+  }
+}
+
+static double g_pme_times[5];
+void orc_last_pme_times(double *t5) { memcpy(t5, g_pme_times, sizeof g_pme_times); }
+
+/* fft_pme.f:1-115 around do_pmesh_kspace (pme_fftw3.f90:56-130, FFTW branch) */
+int orc_fft_pme(int ntap, const double *xp, const double *yp, const double *zp, const double *xpcm,
+                const double *ypcm, const double *zpcm, const double *pmechg, const double *co,
+                const double *oc, double volume, double alphal, int order, int nfft1, int nfft2, int nfft3,
+                double *eer, double *fpx, double *fpy, double *fpz, double *stressc, const int32_t *atomp,
+                int32_t pressure, double rkcut, const double *bsp1, const double *bsp2, const double *bsp3,
+                int nthreads, int variant) {
+  size_t n = (size_t)ntap;
+  double *x = malloc(n * 8), *y = malloc(n * 8), *z = malloc(n * 8);
+  double *fx = calloc(n, 8), *fy = calloc(n, 8), *fz = calloc(n, 8);
+  double *fr1 = malloc(n * 8), *fr2 = malloc(n * 8), *fr3 = malloc(n * 8);
+  double *th = malloc(n * 8 * order * 6);
+  double *th1 = th, *th2 = th + n * order, *th3 = th + 2 * n * order;
+  double *dth1 = th + 3 * n * order, *dth2 = th + 4 * n * order, *dth3 = th + 5 * n * order;
+  size_t G = (size_t)nfft1 * nfft2 * nfft3, Gc = (size_t)(nfft1 / 2 + 1) * nfft2 * nfft3;
+  double *Q = malloc(G * 8), *FQ = malloc(Gc * 16);
+  double recip[9], vir[9];
+  for (size_t i = 0; i < n; ++i) {
+    x[i] = co[0] * xp[i] + co[3] * yp[i] + co[6] * zp[i];
+    y[i] = co[1] * xp[i] + co[4] * yp[i] + co[7] * zp[i];
+    z[i] = co[2] * xp[i] + co[5] * yp[i] + co[8] * zp[i];
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) recip[i + 3 * j] = 0.5 * oc[j + 3 * i];
+  double t0 = now_s();
+  orc_scaled_fractionals(ntap, x, y, z, recip, nfft1, nfft2, nfft3, fr1, fr2, fr3);
+  orc_bspline_coeffs(ntap, fr1, fr2, fr3, order, th1, th2, th3, dth1, dth2, dth3);
+  orc_fill_charge_grid(ntap, pmechg, th1, th2, th3, fr1, fr2, fr3, order, nfft1, nfft2, nfft3, Q, nthreads);
+  double t1 = now_s(), t2, t3, t4;
+  if (variant == 0) {
+    orc_fft_r2c(nfft1, nfft2, nfft3, Q, FQ, nthreads);
+    t2 = now_s();
+    orc_scalar_sum(FQ, alphal, volume, recip, bsp1, bsp2, bsp3, nfft1, nfft2, nfft3, rkcut, eer, vir, nthreads);
+    t3 = now_s();
+    orc_fft_c2r(nfft1, nfft2, nfft3, FQ, Q, nthreads);
+    t4 = now_s();
+  } else {
+    /* fftpack branch (pme_fftw3.f90:119-127): full complex grid, fft_back
+     * (forward e^{-i}), scalar_sum.f:137-217, fft_forward (e^{+i}) */
+    cpx *Qc = (cpx *)malloc(sizeof(cpx) * G);
+    for (size_t g = 0; g < G; ++g) { Qc[g].re = Q[g]; Qc[g].im = 0.0; }
+    for (int ax = 0; ax < 3; ++ax) fft_axis(Qc, nfft1, nfft2, nfft3, ax, -1, nthreads);
+    t2 = now_s();
+    orc_scalar_sum_full((double *)Qc, alphal, volume, recip, bsp1, bsp2, bsp3, nfft1, nfft2, nfft3, rkcut, eer,
+                        vir);
+    t3 = now_s();
+    for (int ax = 0; ax < 3; ++ax) fft_axis(Qc, nfft1, nfft2, nfft3, ax, +1, nthreads);
+    for (size_t g = 0; g < G; ++g) Q[g] = Qc[g].re;
+    free(Qc);
+    t4 = now_s();
+  }
+  orc_grad_sum(ntap, pmechg, recip, th1, th2, th3, dth1, dth2, dth3, fx, fy, fz, fr1, fr2, fr3, order, nfft1,
+               nfft2, nfft3, Q, nthreads);
+  double t5 = now_s();
+  g_pme_times[0] = t1 - t0; g_pme_times[1] = t2 - t1; g_pme_times[2] = t3 - t2;
+  g_pme_times[3] = t4 - t3; g_pme_times[4] = t5 - t4;
+  if (pressure) {
+    double s[9] = {0};
+    for (size_t i = 0; i < n; ++i) {
+      int j = atomp[i] - 1;
+      double sx = xp[i] - xpcm[j], sy = yp[i] - ypcm[j], sz = zp[i] - zpcm[j];
+      double gx = sx * co[0] + sy * co[3] + sz * co[6];
+      double gy = sx * co[1] + sy * co[4] + sz * co[7];
+      double gz = sx * co[2] + sy * co[5] + sz * co[8];
+      s[0] = s[0] - gx * fx[i]; s[1] = s[1] - gy * fx[i]; s[2] = s[2] - gz * fx[i];
+      s[3] = s[3] - gx * fy[i]; s[4] = s[4] - gy * fy[i]; s[5] = s[5] - gz * fy[i];
+      s[6] = s[6] - gx * fz[i]; s[7] = s[7] - gy * fz[i]; s[8] = s[8] - gz * fz[i];
+    }
+    /* stressc(r,c) with stc numbered row-major: stc1 = (1,1), stc2 = (1,2), ... */
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
+        stressc[r + 3 * c] = stressc[r + 3 * c] - vir[r + 3 * c] + s[3 * r + c];
+  }
+  for (size_t i = 0; i < n; ++i) {
+    fpx[i] = fpx[i] + fx[i];
+    fpy[i] = fpy[i] + fy[i];
+    fpz[i] = fpz[i] + fz[i];
+  }
+  free(x); free(y); free(z); free(fx); free(fy); free(fz); free(fr1); free(fr2); free(fr3);
+  free(th); free(Q); free(FQ);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* linked cells: dist_ijk.f, lc_index.f, lc_list.f                     */
+/* ------------------------------------------------------------------ */
+double orc_dist_ijk(int ni, int nj, int nk, double dx, double dy, double dz, const double *co) {
+  static const int nv[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
+                               {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+  const int ndtmax = 500;
+  double dmin = 1.0e8, d;
+  int imin = 0, jmin = 0;
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j) {
+      double lx = (ni + nv[j][0] - nv[i][0]) * dx;
+      double ly = (nj + nv[j][1] - nv[i][1]) * dy;
+      double lz = (nk + nv[j][2] - nv[i][2]) * dz;
+      double mx = co[0] * lx + co[3] * ly + co[6] * lz;
+      double my = co[1] * lx + co[4] * ly + co[7] * lz;
+      double mz = co[2] * lx + co[5] * ly + co[8] * lz;
+      d = mx * mx + my * my + mz * mz;
+      if (d < dmin) { dmin = d; imin = i; jmin = j; }
+    }
+  double lx = (ni + nv[jmin][0] - nv[imin][0]) * dx;
+  double ly = (nj + nv[jmin][1] - nv[imin][1]) * dy;
+  double lz = (nk + nv[jmin][2] - nv[imin][2]) * dz;
+  double mx = co[0] * lx + co[3] * ly + co[6] * lz;
+  double my = co[1] * lx + co[4] * ly + co[7] * lz;
+  double mz = co[2] * lx + co[5] * ly + co[8] * lz;
+  double dcell[3] = {dx, dy, dz};
+  dmin = 1.0e8;
+  for (int ax = 0; ax < 3; ++ax) {
+    /* sign(1., 0.5-nv) is a default-real +-1 */
+    double sgn = (0.5f - (float)nv[jmin][ax]) >= 0.0f ? 1.0 : -1.0;
+    double dt = sgn * dcell[ax] / ndtmax;
+    for (int i = 1; i <= ndtmax; ++i) {
+      double dmx = i * co[0 + 3 * ax] * dt;
+      double dmy = i * co[1 + 3 * ax] * dt;
+      double dmz = i * co[2 + 3 * ax] * dt;
+      d = (mx + dmx) * (mx + dmx) + (my + dmy) * (my + dmy) + (mz + dmz) * (mz + dmz);
+      if (d < dmin) dmin = d;
+    }
+  }
+  return dmin;
+}
+
+/* lc_index.f:3-125; indx arrays are (2, indmax) column-major */
+int orc_lc_index(int indmax, int ncx, int ncy, int ncz, int32_t *nind, int32_t *indxi, int32_t *indxj,
+                 int32_t *indxk, double ctoff, const double *co) {
+  double sqcut = ctoff * ctoff;
+  double dx = 2.0 / ncx, dy = 2.0 / ncy, dz = 2.0 / ncz;
+  int imax = 0, jmax = 0, kmax = 0;
+  nind[0] = 1; indxi[0] = 0; indxj[0] = 0; indxk[0] = 0;
+  nind[1] = 1; indxi[1] = 0; indxj[1] = 0; indxk[1] = 0;
+  for (int n = 1; n <= 2; ++n) {
+    int istart = (n == 1) ? 0 : 1 - ncx;
+    for (int i = istart; i <= ncx - 1; ++i) {
+      int jstart = (n == 1 && i == 0) ? 0 : 1 - ncy;
+      for (int j = jstart; j <= ncy - 1; ++j) {
+        int kstart = (n == 1 && i == 0 && j == 0) ? 0 : 1 - ncz;
+        for (int k = kstart; k <= ncz - 1; ++k) {
+          double rmin = orc_dist_ijk(i, j, k, dx, dy, dz, co);
+          if (rmin < sqcut) {
+            nind[n - 1] += 1;
+            if (nind[n - 1] > indmax) return 1;
+            int m = nind[n - 1];
+            indxi[(n - 1) + 2 * (m - 1)] = i;
+            indxj[(n - 1) + 2 * (m - 1)] = j;
+            indxk[(n - 1) + 2 * (m - 1)] = k;
+            if (imax < abs(i)) imax = abs(i);
+            if (jmax < abs(j)) jmax = abs(j);
+            if (kmax < abs(k)) kmax = abs(k);
+            if (i == 0 && j == 0 && k == 0) nind[n - 1] -= 1;
+          }
+        }
+      }
+    }
+  }
+  if (imax >= (ncx + 1) / 2 || jmax >= (ncy + 1) / 2 || kmax >= (ncz + 1) / 2) return 1;
+  return 0;
+}
+
+static inline int fmodp(int a, int n) { return ((a % n) + n) % n; }
+
+/* cell index along one axis, lc_list.f:94-103 */
+static inline int cell_of(double xr, double d, int nc) {
+  double x1 = xr / d;
+  int ix = (int)x1;
+  double sg = (x1 - ix) >= 0.0 ? 1.0 : -1.0;
+  int nx = (int)((double)ix + (sg - 1.0) / 2.0);
+  return fmodp(nx, nc);
+}
+
+long orc_lc_list(int ncx, int ncy, int ncz, const int32_t *nind, const int32_t *indxi, const int32_t *indxj,
+                 const int32_t *indxk, double rcut, const double *co, const double *xp, const double *yp,
+                 const double *zp, int natp, int32_t *nnlpp, long capacity) {
+  double sqcut = rcut * rcut;
+  double dx = 2.0 / ncx, dy = 2.0 / ncy, dz = 2.0 / ncz;
+  int ncell = ncx * ncy * ncz;
+  int *headp = calloc((size_t)ncell, sizeof(int));
+  int *chainp = malloc(sizeof(int) * ((size_t)natp + 1));
+  int *ci = malloc(sizeof(int) * (size_t)natp * 3);
+  for (int n = 1; n <= natp; ++n) {
+    int nx = cell_of(xp[n - 1], dx, ncx), ny = cell_of(yp[n - 1], dy, ncy), nz = cell_of(zp[n - 1], dz, ncz);
+    ci[3 * (n - 1)] = nx; ci[3 * (n - 1) + 1] = ny; ci[3 * (n - 1) + 2] = nz;
+    int numcell = nz + ncz * (ny + ncy * nx);
+    chainp[n] = headp[numcell];
+    headp[numcell] = n;
+  }
+  long nvtot = 1; /* 1-based position of the next count slot */
+  for (int n = 1; n <= natp; ++n) {
+    double x1 = xp[n - 1], y1 = yp[n - 1], z1 = zp[n - 1];
+    int nv = 0;
+    int i = ci[3 * (n - 1)], j = ci[3 * (n - 1) + 1], k = ci[3 * (n - 1) + 2];
+    for (int m = 1; m <= nind[1]; ++m) {
+      int iv = indxi[1 + 2 * (m - 1)], jv = indxj[1 + 2 * (m - 1)], kv = indxk[1 + 2 * (m - 1)];
+      int nx = fmodp(i + iv, ncx), ny = fmodp(j + jv, ncy), nz = fmodp(k + kv, ncz);
+      int l = headp[nz + ncz * (ny + ncy * nx)];
+      int nmin = (m == 1) ? n : 0;
+      while (l > nmin) {
+        if (l > n) {
+          double x2 = x1 - xp[l - 1], y2 = y1 - yp[l - 1], z2 = z1 - zp[l - 1];
+          x2 = x2 - 2.0 * fnint(0.5 * x2);
+          y2 = y2 - 2.0 * fnint(0.5 * y2);
+          z2 = z2 - 2.0 * fnint(0.5 * z2);
+          double xx = co[0] * x2 + co[3] * y2 + co[6] * z2;
+          double yy = co[1] * x2 + co[4] * y2 + co[7] * z2;
+          double zz = co[2] * x2 + co[5] * y2 + co[8] * z2;
+          double d = xx * xx + yy * yy + zz * zz;
+          if (d < sqcut) {
+            nv += 1;
+            if (nvtot + nv > capacity) { free(headp); free(chainp); free(ci); return -1; }
+            nnlpp[nvtot + nv - 1] = l;
+          }
+        }
+        l = chainp[l];
+      }
+    }
+    nnlpp[nvtot - 1] = nv;
+    nvtot = nvtot + nv + 1;
+  }
+  free(headp); free(chainp); free(ci);
+  return nvtot - 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* force.f:1-917                                                       */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  double st[9];
+  double uconf[3], ucoul[3];
+} force_acc;
+
+/* One owner group i (1-based), restating force.f:235-835 for that group.
+ * `fi` receives the forces on atoms of group i, `fj` on the neighbours
+ * (the same array in the serial build). Scratch arrays are caller-owned. */
+typedef struct {
+  int *index0, *index1;
+  double *xmap0, *ymap0, *zmap0, *xmap1, *ymap1, *zmap1, *xmap2, *ymap2, *zmap2;
+  double *xmap3, *ymap3, *zmap3, *cmap2, *swrs, *dswrs;
+  char *maplg, *mapag;
+} force_scratch;
+
+static force_scratch *scratch_new(int nato, int ngrp, int cap) {
+  force_scratch *s = calloc(1, sizeof *s);
+  s->index0 = malloc(sizeof(int) * cap);
+  s->index1 = malloc(sizeof(int) * cap);
+  double **d[] = {&s->xmap0, &s->ymap0, &s->zmap0, &s->xmap1, &s->ymap1, &s->zmap1, &s->xmap2, &s->ymap2,
+                  &s->zmap2, &s->xmap3, &s->ymap3, &s->zmap3, &s->cmap2, &s->swrs, &s->dswrs};
+  for (size_t k = 0; k < sizeof d / sizeof d[0]; ++k) *d[k] = malloc(sizeof(double) * cap);
+  s->maplg = malloc((size_t)nato + 1);
+  s->mapag = malloc((size_t)ngrp + 1);
+  memset(s->maplg, 1, (size_t)nato + 1);
+  memset(s->mapag, 1, (size_t)ngrp + 1);
+  return s;
+}
+
+static void scratch_free(force_scratch *s) {
+  free(s->index0); free(s->index1);
+  double *d[] = {s->xmap0, s->ymap0, s->zmap0, s->xmap1, s->ymap1, s->zmap1, s->xmap2, s->ymap2,
+                 s->zmap2, s->xmap3, s->ymap3, s->zmap3, s->cmap2, s->swrs, s->dswrs};
+  for (size_t k = 0; k < sizeof d / sizeof d[0]; ++k) free(d[k]);
+  free(s->maplg); free(s->mapag); free(s);
+}
+
+typedef struct {
+  const int32_t *ss_index; const double *xp0, *yp0, *zp0, *xpg, *ypg, *zpg, *charge; const int32_t *nbtype;
+  const int32_t *type; int ma; const int32_t *atomg; const double *xpcm, *ypcm, *zpcm; const int32_t *atomp;
+  const double *co, *ecc12, *ecc6; double alphal; const int32_t *grppt; const double *gmass, *mass;
+  double massinfty; int lskip_ewald, ewald, iz;
+  double r2inn, r2out, rinn0, r2inn0, rout0, r2out0, r2neigh, twrtpi, arsout1, arsout2, arsinn1, arsinn2;
+  double rinn, rout;
+} force_par;
+
+/* pair body shared by the four pair loops; returns qforce, updates energies */
+#define PAIR_GEOM_AND_LJ_EWALD(SWITCHED)                                                     \
+  lij = P->type[(nbti - 1) + (size_t)P->ma * (P->nbtype[j - 1] - 1)];                        \
+  if (P->lskip_ewald && (P->ecc6[lij - 1] < 1e-12) && P->ewald) continue;                     \
+  p2 = P->atomp[j - 1];
+
+static void force_group(const force_par *P, int i, const int32_t *nbr, int m, int mbeg, const int32_t *mapnl_i,
+                        int32_t *nnl1_out, int *map_out, force_scratch *S, double *fpx, double *fpy,
+                        double *fpz, double *fjx, double *fjy, double *fjz, force_acc *A) {
+  const double *co = P->co;
+  double xpgi = P->xpg[i - 1], ypgi = P->ypg[i - 1], zpgi = P->zpg[i - 1];
+  int map = 0, mp0 = 0, mp1 = 0;
+  int typei = P->ss_index[P->grppt[2 * (i - 1)] - 1];
+  for (int jj = mbeg; jj <= m; ++jj) {
+    int j = nbr[jj - 1];
+    double xgg = xpgi - P->xpg[j - 1], ygg = ypgi - P->ypg[j - 1], zgg = zpgi - P->zpg[j - 1];
+    double xmap0j = 2.0 * fnint(0.5 * xgg), ymap0j = 2.0 * fnint(0.5 * ygg), zmap0j = 2.0 * fnint(0.5 * zgg);
+    xgg = xgg - xmap0j; ygg = ygg - ymap0j; zgg = zgg - zmap0j;
+    double xc = co[0] * xgg + co[3] * ygg + co[6] * zgg;
+    double yc = co[4] * ygg + co[7] * zgg;
+    double zc = co[8] * zgg;
+    double drj = xc * xc + yc * yc + zc * zc;
+    if (P->r2inn > 1.0e-5 && drj < P->r2neigh) {
+      map += 1;
+      if (nnl1_out) nnl1_out[map - 1] = j;
+    }
+    if (P->rinn0 > P->rout || P->iz == 0) continue;
+    if (drj >= P->r2inn0 && drj < P->r2out) {
+      mp0 += 1;
+      S->xmap0[mp0 - 1] = xmap0j; S->ymap0[mp0 - 1] = ymap0j; S->zmap0[mp0 - 1] = zmap0j;
+      S->index0[mp0 - 1] = j;
+    } else {
+      if (drj >= P->r2out && drj < P->r2out0) {
+        mp1 += 1;
+        S->xmap1[mp1 - 1] = xmap0j; S->ymap1[mp1 - 1] = ymap0j; S->zmap1[mp1 - 1] = zmap0j;
+        S->xmap2[mp1 - 1] = xc; S->ymap2[mp1 - 1] = yc; S->zmap2[mp1 - 1] = zc;
+        S->index1[mp1 - 1] = j;
+        double rsp = sqrt(drj);
+        double auxa = (P->arsout1 + 2.0 * rsp) / P->arsout2;
+        double auxb = P->rout0 - rsp;
+        S->swrs[mp1 - 1] = auxa * (auxb * auxb);
+        double dsw = -(2.0 * auxa * auxb) + 2.0 * (auxb * auxb) / P->arsout2;
+        S->dswrs[mp1 - 1] = dsw / rsp;
+      } else if (drj >= P->r2inn && drj < P->r2inn0) {
+        mp1 += 1;
+        S->xmap1[mp1 - 1] = xmap0j; S->ymap1[mp1 - 1] = ymap0j; S->zmap1[mp1 - 1] = zmap0j;
+        S->xmap2[mp1 - 1] = xc; S->ymap2[mp1 - 1] = yc; S->zmap2[mp1 - 1] = zc;
+        S->index1[mp1 - 1] = j;
+        double rsp = sqrt(drj);
+        double auxa = (P->arsinn1 + 2.0 * rsp) / P->arsinn2;
+        double auxb = P->rinn0 - rsp;
+        S->swrs[mp1 - 1] = 1.0 - auxa * (auxb * auxb);
+        double dsw = 2.0 * auxa * auxb - 2.0 * (auxb * auxb) / P->arsinn2;
+        S->dswrs[mp1 - 1] = dsw / rsp;
+      }
+    }
+  }
+  *map_out = map;
+  int mapa = mp0, mapb = mp1;
+  for (int jj = 0; jj < mapb; ++jj) S->cmap2[jj] = 0.0;
+  if (mapa == 0 && mapb == 0) return;
+  const int32_t *mp = mapnl_i;
+  for (int i1 = P->grppt[2 * (i - 1)]; i1 <= P->grppt[2 * (i - 1) + 1]; ++i1) {
+    double xpi = P->xp0[i1 - 1], ypi = P->yp0[i1 - 1], zpi = P->zp0[i1 - 1];
+    int nbti = P->nbtype[i1 - 1];
+    double chrgei = P->charge[i1 - 1];
+    int la = mp[0];
+    for (int t = 1; t <= la; ++t) {
+      int mm = mp[t];
+      S->maplg[mm] = 0;
+      S->mapag[P->atomg[mm - 1]] = 0;
+    }
+    int p1 = P->atomp[i1 - 1];
+    double xpcmp1 = P->xpcm[p1 - 1], ypcmp1 = P->ypcm[p1 - 1], zpcmp1 = P->zpcm[p1 - 1];
+    /* normal (non switched) pairs, force.f:396-565 */
+    for (int jj = 1; jj <= mapa; ++jj) {
+      int j1 = S->index0[jj - 1];
+      int typej = P->ss_index[P->grppt[2 * (j1 - 1)] - 1];
+      int typeij = typei + typej - 1;
+      double xd = -S->xmap0[jj - 1], yd = -S->ymap0[jj - 1], zd = -S->zmap0[jj - 1];
+      double xd1 = xpi + xd, yd1 = ypi + yd, zd1 = zpi + zd;
+      xd = xpcmp1 + xd; yd = ypcmp1 + yd; zd = zpcmp1 + zd;
+      double uconfa = 0.0, ucoula = 0.0;
+      int whole = S->mapag[j1];
+      for (int j = P->grppt[2 * (j1 - 1)]; j <= P->grppt[2 * (j1 - 1) + 1]; ++j) {
+        int lij, p2;
+        if (!whole) {
+          if (!(S->maplg[j] && (P->mass[j - 1] < P->massinfty || P->mass[i1 - 1] < P->massinfty))) continue;
+        }
+        PAIR_GEOM_AND_LJ_EWALD(0)
+        double xxcm = xd - P->xpcm[p2 - 1], yycm = yd - P->ypcm[p2 - 1], zzcm = zd - P->zpcm[p2 - 1];
+        double xg = xd1 - P->xp0[j - 1], yg = yd1 - P->yp0[j - 1], zg = zd1 - P->zp0[j - 1];
+        double xc = co[0] * xg + co[3] * yg + co[6] * zg;
+        double yc = co[4] * yg + co[7] * zg;
+        double zc = co[8] * zg;
+        double rsq = xc * xc + yc * yc + zc * zc;
+        double rsqi = 1.0 / rsq;
+        double qforce = 0.0;
+        if (P->ecc6[lij - 1] > 1e-12) {
+          double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+          double ssvir = 12.0 * P->ecc12[lij - 1] * r12 - 6.0 * P->ecc6[lij - 1] * r6;
+          qforce = ssvir * rsqi;
+          double conf = P->ecc12[lij - 1] * r12 - P->ecc6[lij - 1] * r6;
+          uconfa = uconfa + conf;
+        }
+        if (!P->lskip_ewald) {
+          double rsp = sqrt(rsq), rspqi = rsqi / rsp, alphar = P->alphal * rsp;
+          double qt = 1.0 / (1.0 + QP * alphar);
+          double expcst = exp(-(alphar * alphar));
+          double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+          double furpar = chrgei * P->charge[j - 1];
+          ucoula = ucoula + furpar * erfcst / rsp;
+          double aux1 = furpar * (erfcst + P->twrtpi * alphar * expcst) * rspqi;
+          qforce = qforce + aux1;
+        } else if (!P->ewald) {
+          double rsp = sqrt(rsq);
+          double ssvir = chrgei * P->charge[j - 1] / rsp;
+          A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + ssvir;
+          qforce = qforce + ssvir * rsqi;
+        }
+        double emvir = qforce;
+        fpx[i1 - 1] = fpx[i1 - 1] + qforce * xc;
+        fpy[i1 - 1] = fpy[i1 - 1] + qforce * yc;
+        fpz[i1 - 1] = fpz[i1 - 1] + qforce * zc;
+        fjx[j - 1] = fjx[j - 1] - qforce * xc;
+        fjy[j - 1] = fjy[j - 1] - qforce * yc;
+        fjz[j - 1] = fjz[j - 1] - qforce * zc;
+        double qfx = emvir * xc, qfy = emvir * yc, qfz = emvir * zc;
+        A->st[0] += qfx * xxcm; A->st[1] += qfx * yycm; A->st[2] += qfx * zzcm;
+        A->st[3] += qfy * xxcm; A->st[4] += qfy * yycm; A->st[5] += qfy * zzcm;
+        A->st[6] += qfz * xxcm; A->st[7] += qfz * yycm; A->st[8] += qfz * zzcm;
+      }
+      A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + ucoula;
+      A->uconf[typeij - 1] = A->uconf[typeij - 1] + uconfa;
+    }
+    /* switched pairs, force.f:569-731 */
+    for (int jj = 1; jj <= mapb; ++jj) {
+      int j1 = S->index1[jj - 1];
+      int typej = P->ss_index[P->grppt[2 * (j1 - 1)] - 1];
+      int typeij = typei + typej - 1;
+      int whole = S->mapag[j1];
+      double sw = S->swrs[jj - 1];
+      for (int j = P->grppt[2 * (j1 - 1)]; j <= P->grppt[2 * (j1 - 1) + 1]; ++j) {
+        int lij, p2;
+        if (!whole) {
+          if (!(S->maplg[j] && (P->mass[j - 1] < P->massinfty || P->mass[i1 - 1] < P->massinfty))) continue;
+        }
+        PAIR_GEOM_AND_LJ_EWALD(1)
+        double xxcm = xpcmp1 - P->xpcm[p2 - 1] - S->xmap1[jj - 1];
+        double yycm = ypcmp1 - P->ypcm[p2 - 1] - S->ymap1[jj - 1];
+        double zzcm = zpcmp1 - P->zpcm[p2 - 1] - S->zmap1[jj - 1];
+        double xg = xpi - P->xp0[j - 1] - S->xmap1[jj - 1];
+        double yg = ypi - P->yp0[j - 1] - S->ymap1[jj - 1];
+        double zg = zpi - P->zp0[j - 1] - S->zmap1[jj - 1];
+        double xc = co[0] * xg + co[3] * yg + co[6] * zg;
+        double yc = co[4] * yg + co[7] * zg;
+        double zc = co[8] * zg;
+        double rsq = xc * xc + yc * yc + zc * zc;
+        double rsqi = 1.0 / rsq;
+        double qforce = 0.0;
+        if (P->ecc6[lij - 1] > 1e-12) {
+          double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+          double ssvir = 12.0 * P->ecc12[lij - 1] * r12 - 6.0 * P->ecc6[lij - 1] * r6;
+          qforce = ssvir * rsqi * sw;
+          double conf = P->ecc12[lij - 1] * r12 - P->ecc6[lij - 1] * r6;
+          S->cmap2[jj - 1] = S->cmap2[jj - 1] + conf;
+          A->uconf[typeij - 1] = A->uconf[typeij - 1] + sw * conf;
+        }
+        if (!P->lskip_ewald) {
+          double rsp = sqrt(rsq), rspqi = rsqi / rsp, alphar = P->alphal * rsp;
+          double qt = 1.0 / (1.0 + QP * alphar);
+          double expcst = exp(-(alphar * alphar));
+          double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+          double furpar = chrgei * P->charge[j - 1];
+          A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + sw * furpar * erfcst / rsp;
+          S->cmap2[jj - 1] = S->cmap2[jj - 1] + furpar * erfcst / rsp;
+          double aux1 = furpar * (erfcst + P->twrtpi * alphar * expcst) * rspqi * sw;
+          qforce = qforce + aux1;
+        } else if (!P->ewald) {
+          double rsp = sqrt(rsq);
+          double ssvir = chrgei * P->charge[j - 1] / rsp;
+          S->cmap2[jj - 1] = S->cmap2[jj - 1] + ssvir;
+          A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + ssvir * sw;
+          qforce = qforce + ssvir * rsqi * sw;
+        }
+        double emvir = qforce;
+        fpx[i1 - 1] = fpx[i1 - 1] + qforce * xc;
+        fpy[i1 - 1] = fpy[i1 - 1] + qforce * yc;
+        fpz[i1 - 1] = fpz[i1 - 1] + qforce * zc;
+        fjx[j - 1] = fjx[j - 1] - qforce * xc;
+        fjy[j - 1] = fjy[j - 1] - qforce * yc;
+        fjz[j - 1] = fjz[j - 1] - qforce * zc;
+        double qfx = emvir * xc, qfy = emvir * yc, qfz = emvir * zc;
+        A->st[0] += qfx * xxcm; A->st[1] += qfx * yycm; A->st[2] += qfx * zzcm;
+        A->st[3] += qfy * xxcm; A->st[4] += qfy * yycm; A->st[5] += qfy * zzcm;
+        A->st[6] += qfz * xxcm; A->st[7] += qfz * yycm; A->st[8] += qfz * zzcm;
+      }
+    }
+    for (int t = 1; t <= la; ++t) {
+      int mm = mp[t];
+      S->maplg[mm] = 1;
+      S->mapag[P->atomg[mm - 1]] = 1;
+    }
+    mp += la + 1;
+  }
+  /* S'(r) V redistribution by gmass, force.f:756-817 */
+  for (int jj = 0; jj < mapb; ++jj) {
+    S->xmap3[jj] = -(S->dswrs[jj] * S->cmap2[jj] * S->xmap2[jj]);
+    S->ymap3[jj] = -(S->dswrs[jj] * S->cmap2[jj] * S->ymap2[jj]);
+    S->zmap3[jj] = -(S->dswrs[jj] * S->cmap2[jj] * S->zmap2[jj]);
+  }
+  if (mapb != 0) {
+    for (int i1 = P->grppt[2 * (i - 1)]; i1 <= P->grppt[2 * (i - 1) + 1]; ++i1) {
+      double massi = P->gmass[i1 - 1];
+      int p1 = P->atomp[i1 - 1];
+      double xpcmp1 = P->xpcm[p1 - 1], ypcmp1 = P->ypcm[p1 - 1], zpcmp1 = P->zpcm[p1 - 1];
+      for (int jj = 0; jj < mapb; ++jj) {
+        fpx[i1 - 1] = fpx[i1 - 1] + massi * S->xmap3[jj];
+        fpy[i1 - 1] = fpy[i1 - 1] + massi * S->ymap3[jj];
+        fpz[i1 - 1] = fpz[i1 - 1] + massi * S->zmap3[jj];
+        A->st[0] += massi * S->xmap3[jj] * xpcmp1; A->st[1] += massi * S->xmap3[jj] * ypcmp1;
+        A->st[2] += massi * S->xmap3[jj] * zpcmp1; A->st[3] += massi * S->ymap3[jj] * xpcmp1;
+        A->st[4] += massi * S->ymap3[jj] * ypcmp1; A->st[5] += massi * S->ymap3[jj] * zpcmp1;
+        A->st[6] += massi * S->zmap3[jj] * xpcmp1; A->st[7] += massi * S->zmap3[jj] * ypcmp1;
+        A->st[8] += massi * S->zmap3[jj] * zpcmp1;
+      }
+    }
+    for (int jj = 0; jj < mapb; ++jj) {
+      int j = S->index1[jj];
+      double xxcm = S->xmap1[jj], yycm = S->ymap1[jj], zzcm = S->zmap1[jj];
+      for (int j1 = P->grppt[2 * (j - 1)]; j1 <= P->grppt[2 * (j - 1) + 1]; ++j1) {
+        double massj = P->gmass[j1 - 1];
+        fjx[j1 - 1] = fjx[j1 - 1] - massj * S->xmap3[jj];
+        fjy[j1 - 1] = fjy[j1 - 1] - massj * S->ymap3[jj];
+        fjz[j1 - 1] = fjz[j1 - 1] - massj * S->zmap3[jj];
+        int p2 = P->atomp[j1 - 1];
+        double xq = -((P->xpcm[p2 - 1] + xxcm) * massj);
+        double yq = -((P->ypcm[p2 - 1] + yycm) * massj);
+        double zq = -((P->zpcm[p2 - 1] + zzcm) * massj);
+        A->st[0] += S->xmap3[jj] * xq; A->st[1] += S->xmap3[jj] * yq; A->st[2] += S->xmap3[jj] * zq;
+        A->st[3] += S->ymap3[jj] * xq; A->st[4] += S->ymap3[jj] * yq; A->st[5] += S->ymap3[jj] * zq;
+        A->st[6] += S->zmap3[jj] * xq; A->st[7] += S->zmap3[jj] * yq; A->st[8] += S->zmap3[jj] * zq;
+      }
+    }
+  }
+}
+
+int orc_force(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+              const double *xpg, const double *ypg, const double *zpg, const double *charge,
+              const int32_t *nbtype, const int32_t *type, int ma, int nato, const int32_t *atomg,
+              const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *atomp,
+              const double *co, const double *ecc12, const double *ecc6, double alphal, const int32_t *mapnl,
+              int ngrp, const int32_t *grppt, double *uconf_slt, double *uconf_slv, double *uconf_ss,
+              double *ucoul_slt, double *ucoul_slv, double *ucoul_ss, double *fpx, double *fpy, double *fpz,
+              double *stress, const int32_t *nnlpp, int32_t *nnlpp1, long nnlpp1_cap, double rneigh,
+              double rinn, double rout, double *rtolinn, double rtolout, const double *gmass, int iz,
+              int32_t *ncount1, const double *mass, double massinfty, int32_t lskip_ewald, int32_t ewald,
+              int nthreads) {
+  force_par P;
+  P.ss_index = ss_index; P.xp0 = xp0; P.yp0 = yp0; P.zp0 = zp0; P.xpg = xpg; P.ypg = ypg; P.zpg = zpg;
+  P.charge = charge; P.nbtype = nbtype; P.type = type; P.ma = ma; P.atomg = atomg; P.xpcm = xpcm;
+  P.ypcm = ypcm; P.zpcm = zpcm; P.atomp = atomp; P.co = co; P.ecc12 = ecc12; P.ecc6 = ecc6;
+  P.alphal = alphal; P.grppt = grppt; P.gmass = gmass; P.mass = mass; P.massinfty = massinfty;
+  P.lskip_ewald = lskip_ewald; P.ewald = ewald; P.iz = iz; P.rinn = rinn; P.rout = rout;
+  /* force.f:145-166 */
+  P.r2inn = rinn * rinn;
+  P.r2out = rout * rout;
+  P.rinn0 = rinn + *rtolinn;
+  P.r2inn0 = P.rinn0 * P.rinn0;
+  if (rinn < 1.0e-5) *rtolinn = 0.0;
+  P.rout0 = rout + rtolout;
+  P.r2out0 = P.rout0 * P.rout0;
+  P.r2neigh = (P.rinn0 + rneigh) * (P.rinn0 + rneigh);
+  P.twrtpi = 2.0 / sqrt(4.0 * atan(1.0));
+  P.arsout1 = P.rout0 - 3.0 * rout;
+  P.arsout2 = rtolout * rtolout * rtolout;
+  P.arsinn1 = P.rinn0 - 3.0 * rinn;
+  P.arsinn2 = *rtolinn * *rtolinn * *rtolinn;
+
+  /* segment offsets of nnlpp and mapnl per group (serial scan, as the reference walks them) */
+  long *noff = malloc(sizeof(long) * ((size_t)ngrp + 1));
+  long *moff = malloc(sizeof(long) * ((size_t)ngrp + 1));
+  long n = 0, na = 0;
+  int maxm = 0;
+  for (int i = 1; i <= ngrp; ++i) {
+    noff[i - 1] = n;
+    moff[i - 1] = na;
+    int m = nnlpp[n];
+    if (m > maxm) maxm = m;
+    for (int i1 = grppt[2 * (i - 1)]; i1 <= grppt[2 * (i - 1) + 1]; ++i1) na += mapnl[na] + 1;
+    if (iz == 1) n += m + 1;
+  }
+  if (iz == 0) maxm = nnlpp[0];
+  int cap = maxm + 1;
+  int nt = nthreads > 1 ? nthreads : 1;
+  int status = 0;
+  if (nt == 1) {
+    force_scratch *S = scratch_new(nato, ngrp, cap);
+    force_acc A;
+    memset(&A, 0, sizeof A);
+    long c1 = 0;
+    int mbeg = 1;
+    for (int i = 1; i <= ngrp; ++i) {
+      const int32_t *seg = nnlpp + (iz == 1 ? noff[i - 1] : 0);
+      int m = seg[0];
+      if (iz == 0) mbeg = i + 1;
+      int map = 0;
+      int32_t *out1 = NULL;
+      if (P.r2inn > 1.0e-5) {
+        if (c1 + 1 + m > nnlpp1_cap) { status = 1; break; }
+        out1 = nnlpp1 + c1 + 1;
+      }
+      force_group(&P, i, seg + 1, m, mbeg, mapnl + moff[i - 1], out1, &map, S, fpx, fpy, fpz, fpx, fpy, fpz, &A);
+      if (rinn > 1.0e-5) {
+        nnlpp1[c1] = map;
+        c1 += map + 1;
+      }
+    }
+    *ncount1 = (int32_t)c1;
+    scratch_free(S);
+    if (!(iz == 0 || P.rinn0 >= rout)) {
+      *ucoul_slt = A.ucoul[0]; *ucoul_ss = A.ucoul[1]; *ucoul_slv = A.ucoul[2];
+      *uconf_slt = A.uconf[0]; *uconf_ss = A.uconf[1]; *uconf_slv = A.uconf[2];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+          for (int k = 0; k < 3; ++k) stress[r + 3 * c] = stress[r + 3 * c] + A.st[3 * r + k] * co[c + 3 * k];
+    }
+    free(noff); free(moff);
+    return status;
+  }
+#ifdef _OPENMP
+  /* OpenMP variant: static blocks of groups, private force/stress/energy
+   * accumulators reduced in thread order (force.f:189-212, 842-846). */
+  size_t N = (size_t)nato;
+  double *priv = calloc(N * 3 * (size_t)nt, sizeof(double));
+  force_acc *accs = calloc((size_t)nt, sizeof(force_acc));
+  int32_t *inner = NULL;
+  long *ioff = NULL;
+  int *imap = calloc((size_t)ngrp, sizeof(int));
+  if (P.r2inn > 1.0e-5) {
+    ioff = malloc(sizeof(long) * ((size_t)ngrp + 1));
+    long s = 0;
+    for (int i = 1; i <= ngrp; ++i) {
+      ioff[i - 1] = s;
+      s += (iz == 1 ? nnlpp[noff[i - 1]] : nnlpp[0]) + 1;
+    }
+    inner = malloc(sizeof(int32_t) * (size_t)(s + 1));
+  }
+#pragma omp parallel num_threads(nt)
+  {
+    int t = omp_get_thread_num();
+    force_scratch *S = scratch_new(nato, ngrp, cap);
+    double *fx = priv + N * 3 * t, *fy = fx + N, *fz = fy + N;
+#pragma omp for schedule(static)
+    for (int i = 1; i <= ngrp; ++i) {
+      const int32_t *seg = nnlpp + (iz == 1 ? noff[i - 1] : 0);
+      int m = seg[0];
+      int mbeg = iz == 0 ? i + 1 : 1;
+      int map = 0;
+      force_group(&P, i, seg + 1, m, mbeg, mapnl + moff[i - 1], inner ? inner + ioff[i - 1] : NULL, &map, S,
+                  fx, fy, fz, fx, fy, fz, &accs[t]);
+      imap[i - 1] = map;
+    }
+    scratch_free(S);
+  }
+  long c1 = 0;
+  if (rinn > 1.0e-5) {
+    for (int i = 1; i <= ngrp; ++i) {
+      int map = imap[i - 1];
+      if (c1 + 1 + map > nnlpp1_cap) { status = 1; break; }
+      nnlpp1[c1] = map;
+      memcpy(nnlpp1 + c1 + 1, inner + ioff[i - 1], sizeof(int32_t) * (size_t)map);
+      c1 += map + 1;
+    }
+  }
+  *ncount1 = (int32_t)c1;
+  if (!(iz == 0 || P.rinn0 >= rout)) {
+#pragma omp parallel for schedule(static) num_threads(nt)
+    for (long a = 0; a < (long)N; ++a) {
+      double sx = 0, sy = 0, sz = 0;
+      for (int t = 0; t < nt; ++t) {
+        sx += priv[N * 3 * t + a];
+        sy += priv[N * 3 * t + N + a];
+        sz += priv[N * 3 * t + 2 * N + a];
+      }
+      fpx[a] += sx; fpy[a] += sy; fpz[a] += sz;
+    }
+    force_acc A;
+    memset(&A, 0, sizeof A);
+    for (int t = 0; t < nt; ++t) {
+      for (int k = 0; k < 9; ++k) A.st[k] += accs[t].st[k];
+      for (int k = 0; k < 3; ++k) { A.uconf[k] += accs[t].uconf[k]; A.ucoul[k] += accs[t].ucoul[k]; }
+    }
+    *ucoul_slt = A.ucoul[0]; *ucoul_ss = A.ucoul[1]; *ucoul_slv = A.ucoul[2];
+    *uconf_slt = A.uconf[0]; *uconf_ss = A.uconf[1]; *uconf_slv = A.uconf[2];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 3; ++k) stress[r + 3 * c] = stress[r + 3 * c] + A.st[3 * r + k] * co[c + 3 * k];
+  }
+  free(priv); free(accs); free(inner); free(ioff); free(imap);
+#endif
+  free(noff); free(moff);
+  return status;
+}
+
+/* mts_forces.f:212-241 */
+int32_t orc_decide_force(double rinn, double alphal) {
+  double qt = 1.0 / (1.0 + QP * alphal * rinn);
+  double expcst = exp(-alphal * rinn * alphal * rinn);
+  double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+  return erfcst < 10.0e-4;
+}
+
+/* ferrf.f:1-244 (the ERF_CORR spline branches are not restated) */
+int orc_ferrf(const int32_t *ss_index, double alphal, const double *charge, double fudge, const double *x0,
+              const double *y0, const double *z0, int iz, const int32_t *list, int nlist, double *fs_slt,
+              double *fs_slv, double *fpx, double *fpy, double *fpz, int32_t erf_corr, int nbonds_added) {
+  if (erf_corr) return 1;
+  double twrtpi = 2.0 / sqrt(4.0 * atan(1.0));
+  double fs[2] = {0.0, 0.0};
+  for (int i = 0; i < nlist - nbonds_added; ++i) {
+    int ia = list[2 * i], ib = list[2 * i + 1];
+    int type = ss_index[ia - 1];
+    double furpar = iz == 1 ? fudge * charge[ia - 1] * charge[ib - 1] : charge[ia - 1] * charge[ib - 1];
+    double xab = x0[ia - 1] - x0[ib - 1], yab = y0[ia - 1] - y0[ib - 1], zab = z0[ia - 1] - z0[ib - 1];
+    double rsq = xab * xab + yab * yab + zab * zab;
+    double rsp = sqrt(rsq);
+    double alphar = alphal * rsp;
+    double qt = 1.0 / (1.0 + QP * alphar);
+    double expcst = exp(-(alphar * alphar));
+    double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+    double erfst = 1.0 - erfcst;
+    if (iz == 1) {
+      double qforce = -(furpar * (erfst - twrtpi * alphar * expcst) / (rsp * rsq));
+      fs[type - 1] = fs[type - 1] - furpar * erfst / rsp;
+      fpx[ia - 1] = fpx[ia - 1] + qforce * xab;
+      fpy[ia - 1] = fpy[ia - 1] + qforce * yab;
+      fpz[ia - 1] = fpz[ia - 1] + qforce * zab;
+      fpx[ib - 1] = fpx[ib - 1] - qforce * xab;
+      fpy[ib - 1] = fpy[ib - 1] - qforce * yab;
+      fpz[ib - 1] = fpz[ib - 1] - qforce * zab;
+    } else {
+      fs[type - 1] = fs[type - 1] - furpar * erfst / rsp;
+    }
+  }
+  *fs_slt = fs[0];
+  *fs_slv = fs[1];
+  return 0;
+}
+
+/* cself.f:1-64 */
+void orc_cself(const int32_t *ss_index, int nato, double alphal, double kcut, const double *charge,
+               double *self_slt, double *self_slv) {
+  double fs[2] = {0.0, 0.0};
+  double pi = 4.0 * atan(1.0);
+  for (int n = 0; n < nato; ++n) {
+    int i = ss_index[n];
+    fs[i - 1] = fs[i - 1] + charge[n] * charge[n];
+  }
+  for (int n = 0; n < 2; ++n) {
+    double alphar = kcut / (2.0 * alphal);
+    double qt = 1.0 / (1.0 + QP * alphar);
+    double expcst = exp(-(alphar * alphar));
+    double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+    fs[n] = -fs[n] * alphal / sqrt(pi) * (1.0 - erfcst);
+  }
+  *self_slt = fs[0];
+  *self_slv = fs[1];
+}
+
+/* fnbgrp.f:1-213 */
+void orc_fnbgrp(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                const double *charge, const int32_t *type, const double *ecc12, const double *ecc6,
+                int32_t ewald, double alphal, const int32_t *ingrp, int ingrpp, double *fpx, double *fpy,
+                double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt, double *ucoul_slv) {
+  double uconf[2] = {0, 0}, ucoul[2] = {0, 0};
+  *uconf_slt = *uconf_slv = *ucoul_slt = *ucoul_slv = 0.0;
+  if (ingrpp == 0) return;
+  double twrtpi = 2.0 / sqrt(4.0 * atan(1.0));
+  for (int i = 0; i < ingrpp; ++i) {
+    int i1 = ingrp[2 * i], i2 = ingrp[2 * i + 1];
+    int typei = ss_index[i1 - 1];
+    int lj = type[i1 - 1] > type[i2 - 1] ? type[i1 - 1] : type[i2 - 1];
+    int li = type[i1 - 1] < type[i2 - 1] ? type[i1 - 1] : type[i2 - 1];
+    int lij = lj * (lj - 1) / 2 + li;
+    double xa = xp0[i1 - 1] - xp0[i2 - 1], ya = yp0[i1 - 1] - yp0[i2 - 1], za = zp0[i1 - 1] - zp0[i2 - 1];
+    double rsq = xa * xa + ya * ya + za * za;
+    double rsp = sqrt(rsq);
+    double rsqi = 1.0 / rsq;
+    double chi = charge[i1 - 1], chj = charge[i2 - 1];
+    double qforce;
+    if (!ewald) {
+      double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+      double ssvir = 12.0 * ecc12[lij - 1] * r12 - 6.0 * ecc6[lij - 1] * r6;
+      qforce = ssvir * rsqi;
+      uconf[typei - 1] = uconf[typei - 1] + (ecc12[lij - 1] * r12 - ecc6[lij - 1] * r6);
+      ssvir = chi * chj / rsp;
+      ucoul[typei - 1] = ucoul[typei - 1] + ssvir;
+      qforce = qforce + ssvir * rsqi;
+    } else {
+      double rspqi = rsqi / rsp;
+      double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+      double ssvir = 12.0 * ecc12[lij - 1] * r12 - 6.0 * ecc6[lij - 1] * r6;
+      qforce = ssvir * rsqi;
+      uconf[typei - 1] = uconf[typei - 1] + (ecc12[lij - 1] * r12 - ecc6[lij - 1] * r6);
+      double alphar = alphal * rsp;
+      double qt = 1.0 / (1.0 + QP * alphar);
+      double expcst = exp(-(alphar * alphar));
+      double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+      double furpar = chi * chj;
+      ucoul[typei - 1] = ucoul[typei - 1] + furpar * erfcst / rsp;
+      qforce = qforce + furpar * (erfcst + twrtpi * alphar * expcst) * rspqi;
+    }
+    fpx[i1 - 1] = fpx[i1 - 1] + qforce * xa; fpy[i1 - 1] = fpy[i1 - 1] + qforce * ya;
+    fpz[i1 - 1] = fpz[i1 - 1] + qforce * za;
+    fpx[i2 - 1] = fpx[i2 - 1] - qforce * xa; fpy[i2 - 1] = fpy[i2 - 1] - qforce * ya;
+    fpz[i2 - 1] = fpz[i2 - 1] - qforce * za;
+  }
+  *uconf_slt = uconf[0]; *uconf_slv = uconf[1];
+  *ucoul_slt = ucoul[0]; *ucoul_slv = ucoul[1];
+}
