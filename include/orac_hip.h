@@ -1,0 +1,183 @@
+/*
+ * orac_hip.h -- C ABI of the MI355X (gfx950) nonbonded force path for ORAC.
+ *
+ * Drop-in for ORAC's per-step nonbonded routines.  Each plain-mode entry
+ * point takes exactly the arrays of the Fortran routine it replaces (cited
+ * below as file:line of the reference source tree), with the same meaning,
+ * layouts and in/out semantics:
+ *   - arrays are Fortran column-major, index arrays hold 1-based indices;
+ *   - forces and stress are accumulated (+=), energies overwritten;
+ *   - neighbour lists and exclusion maps use the reference's serial
+ *     (non-OpenMP) layouts: per group / per atom [count, entries...].
+ * Scalar inputs are passed by value; scalar outputs by pointer.  LOGICAL
+ * arguments are int32_t (0 = .false.).  Every function returns 0 on success
+ * or a positive status code; orac_hip_last_error() gives the message.  No
+ * C++ exception crosses this boundary.  A context is not re-entrant: one
+ * host thread per context, one HIP stream per context.
+ *
+ * Status codes: 1 bad argument, 2 capacity exceeded (the reference's
+ * xerror paths), 3 HIP runtime error, 4 unsupported input, 5 wrong state.
+ */
+#ifndef ORAC_HIP_H
+#define ORAC_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orac_hip_ctx orac_hip_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart: holds the device, stream and cached topology. */
+int orac_hip_create(orac_hip_ctx **ctx, int32_t device);
+int orac_hip_destroy(orac_hip_ctx *ctx);
+/* Copies the last error message of this host thread (NUL-terminated). */
+int orac_hip_last_error(char *buf, int32_t len);
+/* Drops every cached topology array so the next call uploads them again. */
+int orac_hip_invalidate(orac_hip_ctx *ctx);
+/* Name of the device the context runs on, e.g. "gfx950". */
+int orac_hip_device_name(orac_hip_ctx *ctx, char *buf, int32_t len);
+
+/* ---- reciprocal space ---------------------------------------------------- */
+/* Replaces fft_pme_init (fftw3_pme_init.f90:2-62): B-spline moduli
+ * (load_bsp_moduli/DFTMOD, pme_fftw3.f90:165-230, with the default-real pi
+ * of DFTMOD) into bsp_mod1..3, and the device FFT plans for K1 x K2 x K3.
+ * The FFTW plan handles of the reference have no counterpart. */
+int orac_hip_fft_pme_init(orac_hip_ctx *ctx, int32_t numatoms, int32_t nfft1, int32_t nfft2, int32_t nfft3,
+                          int32_t order, double *bsp_mod1, double *bsp_mod2, double *bsp_mod3);
+
+/* Replaces fft_pme (fft_pme.f:1-115) with the FFTW do_pmesh_kspace
+ * (pme_fftw3.f90:56-130) and scalar_sum (scalar_sum.f:1-135): eer is set,
+ * fpx/fpy/fpz += reciprocal forces, and when pressure != 0
+ * stressc += -vir + molecular correction.  grppt is accepted for signature
+ * parity and not read (as in the reference). */
+int orac_hip_fft_pme(orac_hip_ctx *ctx, int32_t ntap, const double *xp, const double *yp, const double *zp,
+                     const double *xpcm, const double *ypcm, const double *zpcm, const double *pmechg,
+                     const double *co, const double *oc, double volume, double alphal, int32_t order,
+                     int32_t nfft1, int32_t nfft2, int32_t nfft3, double *eer, double *fpx, double *fpy, double *fpz,
+                     double *stressc, const int32_t *atomp, const int32_t *grppt, int32_t pressure, double rkcut);
+
+/* ---- linked cells --------------------------------------------------------- */
+/* Replaces lc_index (lc_index.f:3-125) with dist_ijk (dist_ijk.f): the
+ * cell-offset stencil, on the host.  indxi/j/k are (2, indmax).  Returns 2
+ * where the reference STOPs (indmax overflow, or a cell visited twice). */
+int orac_hip_lc_index(int32_t indmax, int32_t ncx, int32_t ncy, int32_t ncz, int32_t *nind, int32_t *indxi,
+                      int32_t *indxj, int32_t *indxk, double ctoff, const double *co);
+
+/* Replaces lc_list (lc_list.f:1-237): group-pair list of the group centres
+ * xp/yp/zp (natp groups) within rcut, written as nnlpp = per group
+ * [nv, l1..lnv] (l > n), the reference's serial layout.  *nentries gets the
+ * total length.  Returns 2 if nnlpp_capacity would be exceeded (the
+ * reference's xerror path, lc_list.f:191-197). */
+int orac_hip_lc_list(orac_hip_ctx *ctx, int32_t ncx, int32_t ncy, int32_t ncz, const int32_t *nind,
+                     const int32_t *indxi, const int32_t *indxj, const int32_t *indxk, double rcut, const double *co,
+                     const double *xp, const double *yp, const double *zp, int32_t natp, int32_t *nnlpp,
+                     int64_t nnlpp_capacity, int64_t *nentries);
+
+/* ---- direct space ---------------------------------------------------------- */
+/* Replaces force (force.f:1-917) as called from mts_forces.f:148-152:
+ * group-switched Ewald-erfc (A&S 7.1.26) + Lennard-Jones forces, energies
+ * per pair class (slt-slt, slv-slv, slt-slv) and molecular virial
+ * (stress += st . co^T), plus the next-inner-shell list nnlpp1 (capacity
+ * nnlpp1_capacity entries) and its length *ncount1.  rtolinn is in/out: the
+ * reference sets it to 0 when rinn < 1e-5.  groupp is accepted for
+ * signature parity and not read (as in the reference).  iz = 0 (or
+ * rinn + rtolinn >= rout) is the list-only "phony" call. */
+int orac_hip_force(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                   const double *zp0, const double *xpg, const double *ypg, const double *zpg, const double *charge,
+                   const int32_t *nbtype, const int32_t *type, int32_t ma, int32_t nato, const int32_t *atomg,
+                   const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *groupp,
+                   const int32_t *atomp, const double *co, const double *ecc12, const double *ecc6, double alphal,
+                   const int32_t *mapnl, int32_t ngrp, const int32_t *grppt, double *uconf_slt, double *uconf_slv,
+                   double *uconf_ss, double *ucoul_slt, double *ucoul_slv, double *ucoul_ss, double *fpx,
+                   double *fpy, double *fpz, double *stress, const int32_t *nnlpp, int32_t *nnlpp1,
+                   int64_t nnlpp1_capacity, double rneigh, double rinn, double rout, double *rtolinn,
+                   double rtolout, const double *gmass, int32_t iz, int32_t *ncount1, const double *mass,
+                   double massinfty, int32_t lskip_ewald, int32_t ewald);
+
+/* Replaces decide_force (mts_forces.f:212-241): lskip_ewald = erfc_AS(alpha*rinn) < 1e-3. */
+int orac_hip_decide_force(double rinn, double alphal, int32_t *lskip_ewald);
+
+/* ---- corrections ------------------------------------------------------------ */
+/* Replaces ferrf (ferrf.f:1-244) without the ERF_CORR spline branch
+ * (erf_corr != 0 returns 4): reciprocal-space erf correction of the
+ * excluded pairs list(2, nlist) on cartesian x0/y0/z0; iz = 1 also adds
+ * forces (fudge-scaled), iz = 0 gives the energy only. */
+int orac_hip_ferrf(orac_hip_ctx *ctx, const int32_t *ss_index, double alphal, const double *charge, double fudge,
+                   const double *x0, const double *y0, const double *z0, int32_t iz, const int32_t *list,
+                   int32_t nlist, double *fscnstr_slt, double *fscnstr_slv, double *fpx, double *fpy, double *fpz,
+                   int32_t erf_corr, int32_t nbonds_added, int32_t mb);
+
+/* Replaces cself (cself.f:1-64): Ewald self energy per class. */
+int orac_hip_cself(orac_hip_ctx *ctx, const int32_t *ss_index, int32_t nato, double alphal, double kcut,
+                   const double *charge, double *self_slt, double *self_slv);
+
+/* Replaces fnbgrp (fnbgrp.f:1-213), serial layout: intra-group
+ * non-excluded pairs ingrp(2, ingrpp), LJ + Ewald (or plain Coulomb). */
+int orac_hip_fnbgrp(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                    const double *zp0, const double *charge, const int32_t *type, const double *ecc12,
+                    const double *ecc6, int32_t ewald, double alphal, const int32_t *ingrp, int32_t ingrpp,
+                    double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt,
+                    double *ucoul_slv, int32_t nato);
+
+/* ---- device-resident systems (no reference counterpart) -------------------- */
+/* A system whose topology, coordinates, lists and grids stay in HBM.  The
+ * caller describes it once with the same arrays the plain-mode routines
+ * take, then evaluates the full nonbonded force (lists when due, direct
+ * space, PME, exclusion corrections) on the device without host copies.
+ * precision: 0 = FP64 throughout; 1 = mixed (FP32 pair/spline/FFT
+ * arithmetic, FP64 influence function, corrections and reductions). */
+typedef struct orac_hip_system_desc {
+  int32_t natoms, ngrp, nmol, ntypes;
+  const double *charge, *mass;        /* natoms */
+  const int32_t *nbtype;              /* natoms, 1-based */
+  const int32_t *type_table;          /* ntypes x ntypes, 1-based packed index */
+  const double *ecc12, *ecc6;         /* ntypes*(ntypes+1)/2 */
+  const int32_t *grppt;               /* (2, ngrp), 1-based atom ranges */
+  const int32_t *atomp;               /* natoms, 1-based molecule; molecules are contiguous */
+  const int32_t *ss_index;            /* natoms, 1 solute / 2 solvent */
+  const int32_t *mapnl;               /* exclusion map, mappa.f layout */
+  int32_t ncorr;                      /* per-step reciprocal erf corrections (ferrf, iz = 1) */
+  const int32_t *corr_pairs;          /* (2, ncorr), 1-based */
+  const double *corr_fudge;           /* ncorr */
+  const double *co;                   /* 3x3 half cell matrix */
+  double alpha;
+  int32_t nfft1, nfft2, nfft3, order;
+  double rcut, rtol, rnei;            /* direct shell: cutoff, healing length, list skin */
+  int32_t ncx, ncy, ncz;              /* linked-cell grid */
+  int32_t precision;
+} orac_hip_system_desc;
+
+int orac_hip_system_create(orac_hip_ctx *ctx, const orac_hip_system_desc *desc, int32_t *sys);
+int orac_hip_system_destroy(orac_hip_ctx *ctx, int32_t sys);
+/* Host -> device reduced coordinates. */
+int orac_hip_system_set_coords(orac_hip_ctx *ctx, int32_t sys, const double *xp, const double *yp, const double *zp);
+/* Device pointers of the resident coordinate and force arrays (for
+ * callers that keep the state on the device). */
+int orac_hip_system_device_ptrs(orac_hip_ctx *ctx, int32_t sys, double **xp, double **yp, double **zp, double **fx,
+                                double **fy, double **fz);
+/* One full force evaluation on the device.  rebuild_list != 0 rebuilds the
+ * group lists first.  Asynchronous on the context stream. */
+int orac_hip_system_eval(orac_hip_ctx *ctx, int32_t sys, int32_t rebuild_list);
+/* Waits for the context stream, copies forces (may be null) and the
+ * energy record: [0] eer, [1..3] uconf slt/slv/ss, [4..6] ucoul slt/slv/ss,
+ * [7..8] correction slt/slv, [9..17] stress (3x3, direct + reciprocal +
+ * molecular correction), [18] group-pair entries of the last list build
+ * (full list, both directions), [19..20] self energy slt/slv.  Needs room
+ * for 21 doubles.  Returns 2 if the list outgrew its capacity. */
+int orac_hip_system_results(orac_hip_ctx *ctx, int32_t sys, double *fx, double *fy, double *fz, double *energies);
+/* Stage timing: when enabled, eval records HIP events between its stages;
+ * stage_ms returns, for the last eval, the milliseconds of
+ * [0] list build, [1] centres, [2] direct-space pairs, [3] PME
+ * (spread, FFT, convolution, gather), [4] corrections. */
+int orac_hip_system_profile(orac_hip_ctx *ctx, int32_t sys, int32_t enable);
+int orac_hip_system_stage_ms(orac_hip_ctx *ctx, int32_t sys, float *ms, int32_t n);
+int orac_hip_sync(orac_hip_ctx *ctx);
+/* Stream handle of the context, as a void* (a hipStream_t). */
+int orac_hip_stream(orac_hip_ctx *ctx, void **stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,118 @@
+// common.hpp -- shared device/host definitions of the ORAC nonbonded path on gfx950.
+//
+// Conventions (the reference's): reduced coordinates xp = oc.x with the box
+// spanning [-1,1); co/oc/recip are 3x3 column-major (m[i + 3*j] = m(i+1,j+1));
+// grids are Q(K1,K2,K3), K1 fastest; the half-complex grid is FQ(K1/2+1,K2,K3).
+// Every kernel reads a descriptor array and takes its system from blockIdx.y,
+// so one launch serves one system or a batch of replicas.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+
+namespace orac {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define ORAC_HIP(call)                                                                             \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      throw ::orac::Error(3, std::string(#call " failed: ") + hipGetErrorString(e_));              \
+  } while (0)
+
+#define ORAC_REQUIRE(cond, code, msg)                                                              \
+  do {                                                                                             \
+    if (!(cond)) throw ::orac::Error((code), (msg));                                               \
+  } while (0)
+
+// status codes returned through the C ABI
+enum Status { OK = 0, E_ARG = 1, E_CAPACITY = 2, E_HIP = 3, E_UNSUPPORTED = 4, E_STATE = 5 };
+
+constexpr int MAX_ORDER = 8;       // B-spline order limit of the device kernels
+constexpr int MAX_FFT_LEN = 256;   // longest pencil an FFT workgroup holds in LDS
+constexpr int MAX_RADICES = 12;
+
+// ---------------------------------------------------------------- PME
+
+// Per-axis FFT plan: radix sequence and twiddle table tw[m] = exp(-2 pi i m / n).
+struct FftAxis {
+  int n;
+  int nr;
+  int radix[MAX_RADICES];
+  const double2 *tw64;
+  const float2 *tw32;
+};
+
+// One system's reciprocal-space state.
+struct PmeDesc {
+  int natoms;
+  int K1, K2, K3, order;
+  double recip[9];          // recip(i,j) = 0.5*oc(j,i)
+  double volume, pi_f32, fac, rkcut2;
+  double fixscale, fixinv;  // grid fixed point: value = int64 * fixinv
+  const double *xp, *yp, *zp;   // reduced coordinates
+  const double *co;             // 3x3 on device (cartesian = co . xp)
+  const double *q;
+  unsigned long long *Qfix;     // K1*K2*K3 int64 two's complement
+  void *FQ;                     // complex half grid, double2 or float2
+  void *Qr;                     // real grid after c2r, double or float
+  const double *bsp1, *bsp2, *bsp3;
+  FftAxis ax1, ax2, ax3;
+  double *fx, *fy, *fz;         // PME forces
+  int accumulate;               // 0: fx = f_pme, 1: fx += f_pme
+  double *conv_part;            // per-block energy/virial partials [nblk][8]
+  int conv_nblk;
+  // molecular-virial correction (fft_pme.f:65-106)
+  const int *atomp;             // 0-based molecule of each atom
+  const double *xpcm, *ypcm, *zpcm;
+  double *gather_part;          // per-block stc partials [nblk][9]
+  int gather_nblk;
+};
+
+// ---------------------------------------------------------------- helpers
+
+__host__ __device__ inline int wrap_idx(int k0, int K) { return k0 >= 0 ? k0 : k0 + K; }
+
+// Fortran ANINT/DNINT: round half away from zero.
+__host__ __device__ inline double fnint(double x) { return round(x); }
+
+// Cardinal B-spline weights and derivatives (Essmann recursion,
+// pme_fftw3.f90:537-576): derivative taken before the last pass.
+template <typename T>
+__host__ __device__ inline void bspline_pass(T *c, T w, int k) {
+  T div = T(1) / T(k - 1);
+  c[k - 1] = div * w * c[k - 2];
+  for (int j = 1; j <= k - 2; ++j) c[k - j - 1] = div * ((w + T(j)) * c[k - j - 2] + (T(k - j) - w) * c[k - j - 1]);
+  c[0] = div * (T(1) - w) * c[0];
+}
+
+template <typename T>
+__host__ __device__ inline void fill_bspline(T w, int order, T *c, T *d) {
+  c[order - 1] = T(0);
+  c[1] = w;
+  c[0] = T(1) - w;
+  for (int k = 3; k <= order - 1; ++k) bspline_pass(c, w, k);
+  d[0] = -c[0];
+  for (int j = 2; j <= order; ++j) d[j - 1] = c[j - 2] - c[j - 1];
+  bspline_pass(c, w, order);
+}
+
+// 64-bit fixed point: round-to-nearest of v*scale, two's complement.
+__device__ inline unsigned long long to_fix(double v, double scale) {
+  return (unsigned long long)__double2ll_rn(v * scale);
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace orac

@@ -1,0 +1,845 @@
+// pairs.hip -- direct-space Ewald-erfc + Lennard-Jones over group lists on gfx950.
+//
+//   k_group_centres / k_mol_com   appbou.f group centres, molecule centres of mass
+//   k_cell_*                      linked-cell binning of group centres (lc_list.f:93-109)
+//   k_list_count / k_list_write   stencil walk and pair test (lc_list.f:133-201), two
+//                                 passes so the output layout is exact and compact
+//   k_half_to_full                transpose of a half list into a full list, each
+//                                 group's transposed tail sorted so the order is fixed
+//   k_pairs                       force.f:235-835: one wavefront per owner group, lanes over
+//                                 its neighbour groups; forces on owner atoms are reduced
+//                                 across the wave, so every atom's sum happens in one place
+//                                 (no atomics, deterministic); energies and the molecular
+//                                 virial are taken from the j > i direction only
+//   k_corr_*                      ferrf.f / fnbgrp.f pair lists: per-pair kernel plus a
+//                                 per-atom gather in fixed order
+//   k_cself                       cself.f self term
+// The list and classification arithmetic is compiled without FMA contraction so
+// the distances are the reference's bit for bit.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace orac {
+
+// A&S 7.1.26 (force.f:123-125)
+#define AS_A1 0.2548296
+#define AS_A2 -0.28449674
+#define AS_A3 1.4214137
+#define AS_A4 -1.453152
+#define AS_A5 1.0614054
+#define AS_QP 0.3275911
+
+__device__ inline int fmodp(int a, int n) { return ((a % n) + n) % n; }
+
+// ---------------------------------------------------------------- centres
+
+__global__ void k_group_centres(int ngrp, const int *gfirst, const int *glast, const double *pmass, const double *xp,
+                                const double *yp, const double *zp, double *xpg, double *ypg, double *zpg) {
+#pragma clang fp contract(off)
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngrp) return;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int m = gfirst[g]; m <= glast[g]; ++m) {
+    sx = sx + pmass[m] * xp[m];
+    sy = sy + pmass[m] * yp[m];
+    sz = sz + pmass[m] * zp[m];
+  }
+  xpg[g] = sx;
+  ypg[g] = sy;
+  zpg[g] = sz;
+}
+
+__global__ void k_mol_com(int nmol, const int *mfirst, const int *mlast, const double *mass, const double *xp,
+                          const double *yp, const double *zp, double *xpcm, double *ypcm, double *zpcm) {
+#pragma clang fp contract(off)
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmol) return;
+  double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
+  for (int a = mfirst[m]; a <= mlast[m]; ++a) {
+    sx = sx + mass[a] * xp[a];
+    sy = sy + mass[a] * yp[a];
+    sz = sz + mass[a] * zp[a];
+    sm = sm + mass[a];
+  }
+  xpcm[m] = sx / sm;
+  ypcm[m] = sy / sm;
+  zpcm[m] = sz / sm;
+}
+
+void launch_group_centres(int ngrp, const int *gfirst, const int *glast, const double *pmass, const double *xp,
+                          const double *yp, const double *zp, double *xpg, double *ypg, double *zpg, hipStream_t st) {
+  if (ngrp <= 0) return;
+  hipLaunchKernelGGL(k_group_centres, dim3((ngrp + 255) / 256), dim3(256), 0, st, ngrp, gfirst, glast, pmass, xp, yp,
+                     zp, xpg, ypg, zpg);
+}
+
+void launch_mol_com(int nmol, const int *mfirst, const int *mlast, const double *mass, const double *xp,
+                    const double *yp, const double *zp, double *xpcm, double *ypcm, double *zpcm, hipStream_t st) {
+  if (nmol <= 0) return;
+  hipLaunchKernelGGL(k_mol_com, dim3((nmol + 255) / 256), dim3(256), 0, st, nmol, mfirst, mlast, mass, xp, yp, zp,
+                     xpcm, ypcm, zpcm);
+}
+
+// ---------------------------------------------------------------- cells
+
+// cell index along one axis (lc_list.f:94-103): floor toward -inf then wrap
+__device__ inline int cell_axis(double xr, double dcell, int nc) {
+#pragma clang fp contract(off)
+  double x1 = xr / dcell;
+  int ix = (int)x1;
+  double sg = (x1 - (double)ix) >= 0.0 ? 1.0 : -1.0;
+  int n = (int)((double)ix + (sg - 1.0) / 2.0);
+  return fmodp(n, nc);
+}
+
+__global__ void k_cell_assign(int ngrp, const double *xpg, const double *ypg, const double *zpg, CellGrid cg,
+                              int *cell_of, int *cell_cnt) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngrp) return;
+  int nx = cell_axis(xpg[g], 2.0 / cg.ncx, cg.ncx);
+  int ny = cell_axis(ypg[g], 2.0 / cg.ncy, cg.ncy);
+  int nz = cell_axis(zpg[g], 2.0 / cg.ncz, cg.ncz);
+  int c = nz + cg.ncz * (ny + cg.ncy * nx);
+  cell_of[g] = c;
+  atomicAdd(&cell_cnt[c], 1);
+}
+
+__global__ void k_cell_fill(int ngrp, const int *cell_of, const int *cell_start, int *cursor, int *items) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngrp) return;
+  int c = cell_of[g];
+  int pos = atomicAdd(&cursor[c], 1);
+  items[cell_start[c] + pos] = g;
+}
+
+// each cell's members in descending group order: the order of the
+// reference's head-inserted chains (lc_list.f:104-108)
+__global__ void k_cell_sort(int ncell, const int *cell_start, const int *cell_cnt, int *items) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int s = cell_start[c], n = cell_cnt[c];
+  for (int i = 1; i < n; ++i) {
+    int v = items[s + i];
+    int j = i - 1;
+    while (j >= 0 && items[s + j] < v) {
+      items[s + j + 1] = items[s + j];
+      --j;
+    }
+    items[s + j + 1] = v;
+  }
+}
+
+__global__ void k_zero_i32(int *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// single-block exclusive scan (fixed order); total written to *total
+__global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, int *total) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int base = 0; base < n; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < n ? in[i] : 0;
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0;
+      for (int w = 0; w < 16; ++w) {
+        int t = wsum[w];
+        wsum[w] = s;
+        s += t;
+      }
+    }
+    __syncthreads();
+    int excl = carry + wsum[wid] + x - v;
+    if (i < n) out[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = excl + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+void launch_exclusive_scan(const int *in, int *out, int n, int *total, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, in, out, n, total);
+}
+
+__global__ void k_list_guard(int n, int *off, int *total, int cap) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool over = total[0] > cap;
+  if (i == 0) total[1] = over ? 1 : 0;
+  if (i < n && over) off[i] = 0;
+  if (i == n) off[n] = over ? 0 : total[0];
+}
+
+void launch_list_guard(int n, int *off, int *total, int cap, hipStream_t st) {
+  hipLaunchKernelGGL(k_list_guard, dim3((n + 256) / 256), dim3(256), 0, st, n, off, total, cap);
+}
+
+void launch_cell_build(int ngrp, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
+                       int *cell_of, int *cell_cnt, int *cell_start, int *cell_cursor, int *cell_items,
+                       hipStream_t st) {
+  int ncell = cg.ncx * cg.ncy * cg.ncz;
+  dim3 b(256);
+  hipLaunchKernelGGL(k_zero_i32, dim3((ncell + 255) / 256), b, 0, st, cell_cnt, ncell);
+  hipLaunchKernelGGL(k_zero_i32, dim3((ncell + 255) / 256), b, 0, st, cell_cursor, ncell);
+  if (ngrp > 0)
+    hipLaunchKernelGGL(k_cell_assign, dim3((ngrp + 255) / 256), b, 0, st, ngrp, xpg, ypg, zpg, cg, cell_of, cell_cnt);
+  launch_exclusive_scan(cell_cnt, cell_start, ncell, nullptr, st);
+  if (ngrp > 0)
+    hipLaunchKernelGGL(k_cell_fill, dim3((ngrp + 255) / 256), b, 0, st, ngrp, cell_of, cell_start, cell_cursor,
+                       cell_items);
+  hipLaunchKernelGGL(k_cell_sort, dim3((ncell + 255) / 256), b, 0, st, ncell, cell_start, cell_cnt, cell_items);
+}
+
+// Stencil walk for group n (lc_list.f:133-201). WRITE = false counts only.
+template <bool WRITE>
+__device__ inline int list_walk(int n, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
+                                const int *cell_of, const int *cell_start, const int *cell_cnt,
+                                const int *cell_items, int *out, int one_based) {
+#pragma clang fp contract(off)
+  const double x1 = xpg[n], y1 = ypg[n], z1 = zpg[n];
+  const int c = cell_of[n];
+  const int i = c / (cg.ncy * cg.ncz), j = (c / cg.ncz) % cg.ncy, k = c % cg.ncz;
+  const double *co = cg.co;
+  int nv = 0;
+  for (int m = 0; m < cg.nsten; ++m) {
+    int nx = fmodp(i + cg.sten[3 * m], cg.ncx);
+    int ny = fmodp(j + cg.sten[3 * m + 1], cg.ncy);
+    int nz = fmodp(k + cg.sten[3 * m + 2], cg.ncz);
+    int cc = nz + cg.ncz * (ny + cg.ncy * nx);
+    int s = cell_start[cc], e = s + cell_cnt[cc];
+    for (int t = s; t < e; ++t) {
+      int l = cell_items[t];   // descending
+      if (cg.half) {
+        if (m == 0 && l <= n) break;
+        if (l <= n) continue;
+      } else if (l == n) {
+        continue;
+      }
+      double x2 = x1 - xpg[l], y2 = y1 - ypg[l], z2 = z1 - zpg[l];
+      x2 = x2 - 2.0 * fnint(0.5 * x2);
+      y2 = y2 - 2.0 * fnint(0.5 * y2);
+      z2 = z2 - 2.0 * fnint(0.5 * z2);
+      double xx = co[0] * x2 + co[3] * y2 + co[6] * z2;
+      double yy = co[1] * x2 + co[4] * y2 + co[7] * z2;
+      double zz = co[2] * x2 + co[5] * y2 + co[8] * z2;
+      double d = xx * xx + yy * yy + zz * zz;
+      if (d < cg.sqcut) {
+        if (WRITE) out[nv] = l + one_based;
+        ++nv;
+      }
+    }
+  }
+  return nv;
+}
+
+__global__ void k_list_count(int ngrp, const double *xpg, const double *ypg, const double *zpg, CellGrid cg,
+                             const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
+                             int *count) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ngrp) return;
+  count[n] = list_walk<false>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, nullptr, 0);
+}
+
+__global__ void k_list_write(int ngrp, const double *xpg, const double *ypg, const double *zpg, CellGrid cg,
+                             const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
+                             const int *off, int *out, int mode) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ngrp) return;
+  if (mode == 1) {
+    int o = off[n] + n;  // reference layout: one count slot per preceding group
+    int nv = list_walk<true>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, out + o + 1, 1);
+    out[o] = nv;
+  } else {
+    list_walk<true>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, out + off[n], 0);
+  }
+}
+
+void launch_list_count(int ngrp, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
+                       const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
+                       int *count, hipStream_t st) {
+  if (ngrp <= 0) return;
+  hipLaunchKernelGGL(k_list_count, dim3((ngrp + 127) / 128), dim3(128), 0, st, ngrp, xpg, ypg, zpg, cg, cell_of,
+                     cell_start, cell_cnt, cell_items, count);
+}
+
+void launch_list_write(int ngrp, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
+                       const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
+                       const int *off, int *out, int mode, hipStream_t st) {
+  if (ngrp <= 0) return;
+  hipLaunchKernelGGL(k_list_write, dim3((ngrp + 127) / 128), dim3(128), 0, st, ngrp, xpg, ypg, zpg, cg, cell_of,
+                     cell_start, cell_cnt, cell_items, off, out, mode);
+}
+
+// ---------------------------------------------------------------- half -> full
+
+__global__ void k_h2f_count(int ngrp, const int *h_off, const int *h_idx, int *incount) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngrp) return;
+  for (int t = h_off[g]; t < h_off[g + 1]; ++t) atomicAdd(&incount[h_idx[t]], 1);
+}
+
+__global__ void k_h2f_total(int ngrp, const int *h_off, int *incount) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngrp) return;
+  incount[g] += h_off[g + 1] - h_off[g];
+}
+
+__global__ void k_h2f_fill(int ngrp, const int *h_off, const int *h_idx, const int *f_off, int *f_idx, int *cursor) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngrp) return;
+  int nh = h_off[g + 1] - h_off[g];
+  for (int t = 0; t < nh; ++t) f_idx[f_off[g] + t] = h_idx[h_off[g] + t];
+  for (int t = h_off[g]; t < h_off[g + 1]; ++t) {
+    int j = h_idx[t];
+    int nhj = h_off[j + 1] - h_off[j];
+    int pos = atomicAdd(&cursor[j], 1);
+    f_idx[f_off[j] + nhj + pos] = g;
+  }
+}
+
+__global__ void k_h2f_sort(int ngrp, const int *h_off, const int *f_off, int *f_idx) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngrp) return;
+  int s = f_off[g] + (h_off[g + 1] - h_off[g]), n = f_off[g + 1] - s;
+  for (int i = 1; i < n; ++i) {
+    int v = f_idx[s + i];
+    int j = i - 1;
+    while (j >= 0 && f_idx[s + j] > v) {
+      f_idx[s + j + 1] = f_idx[s + j];
+      --j;
+    }
+    f_idx[s + j + 1] = v;
+  }
+}
+
+void launch_half_to_full(int ngrp, const int *h_off, const int *h_idx, int *incount, int *f_off, int *f_idx,
+                         int *cursor, int *total, hipStream_t st) {
+  dim3 b(256), g((ngrp + 255) / 256);
+  if (ngrp <= 0) return;
+  hipLaunchKernelGGL(k_zero_i32, g, b, 0, st, incount, ngrp + 1);
+  hipLaunchKernelGGL(k_zero_i32, g, b, 0, st, cursor, ngrp);
+  hipLaunchKernelGGL(k_h2f_count, g, b, 0, st, ngrp, h_off, h_idx, incount);
+  hipLaunchKernelGGL(k_h2f_total, g, b, 0, st, ngrp, h_off, incount);
+  launch_exclusive_scan(incount, f_off, ngrp + 1, total, st);
+  hipLaunchKernelGGL(k_h2f_fill, g, b, 0, st, ngrp, h_off, h_idx, f_off, f_idx, cursor);
+  hipLaunchKernelGGL(k_h2f_sort, g, b, 0, st, ngrp, h_off, f_off, f_idx);
+}
+
+// ---------------------------------------------------------------- inner-shell list
+
+// force.f:265-286: for each group i, the neighbours j (in list order) whose
+// centre distance is below r2neigh.  iz = 0 walks j = i+1..ngrp instead.
+__device__ inline double centre_d2(const double *xpg, const double *ypg, const double *zpg, const double *co, int i,
+                                   int j) {
+#pragma clang fp contract(off)
+  double xgg = xpg[i] - xpg[j], ygg = ypg[i] - ypg[j], zgg = zpg[i] - zpg[j];
+  xgg = xgg - 2.0 * fnint(0.5 * xgg);
+  ygg = ygg - 2.0 * fnint(0.5 * ygg);
+  zgg = zgg - 2.0 * fnint(0.5 * zgg);
+  double xc = co[0] * xgg + co[3] * ygg + co[6] * zgg;
+  double yc = co[4] * ygg + co[7] * zgg;
+  double zc = co[8] * zgg;
+  return xc * xc + yc * yc + zc * zc;
+}
+
+struct Co9 {
+  double m[9];
+};
+
+__global__ void k_inner_list(int ngrp, int iz, const int *h_off, const int *h_idx, const double *xpg,
+                             const double *ypg, const double *zpg, Co9 co, double r2neigh, int *count, const int *off,
+                             int *out, int pass) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ngrp) return;
+  int nv = 0;
+  int b = iz == 1 ? h_off[i] : 0, e = iz == 1 ? h_off[i + 1] : 0;
+  int jlo = iz == 1 ? 0 : i + 1, jhi = iz == 1 ? 0 : ngrp;
+  int o = pass ? off[i] + i + 1 : 0;
+  for (int t = b; t < e; ++t) {
+    int j = h_idx[t];
+    if (centre_d2(xpg, ypg, zpg, co.m, i, j) < r2neigh) {
+      if (pass) out[o + nv] = j + 1;
+      ++nv;
+    }
+  }
+  for (int j = jlo; j < jhi; ++j) {
+    if (centre_d2(xpg, ypg, zpg, co.m, i, j) < r2neigh) {
+      if (pass) out[o + nv] = j + 1;
+      ++nv;
+    }
+  }
+  if (pass) out[off[i] + i] = nv;
+  else count[i] = nv;
+}
+
+void launch_inner_list(int ngrp, int iz, const int *h_off, const int *h_idx, const double *xpg, const double *ypg,
+                       const double *zpg, const double *co, double r2neigh, int *count, const int *off, int *out,
+                       int pass, hipStream_t st) {
+  if (ngrp <= 0) return;
+  Co9 c;
+  for (int k = 0; k < 9; ++k) c.m[k] = co[k];
+  hipLaunchKernelGGL(k_inner_list, dim3((ngrp + 127) / 128), dim3(128), 0, st, ngrp, iz, h_off, h_idx, xpg, ypg, zpg,
+                     c, r2neigh, count, off, out, pass);
+}
+
+// ---------------------------------------------------------------- pair kernel
+
+__device__ inline bool has_partner_in(const int *ex_off, const int *ex_idx, int a, int lo, int hi) {
+  for (int t = ex_off[a]; t < ex_off[a + 1]; ++t) {
+    int p = ex_idx[t];
+    if (p >= lo && p <= hi) return true;
+  }
+  return false;
+}
+
+__device__ inline bool is_excluded(const int *ex_off, const int *ex_idx, int a, int b) {
+  for (int t = ex_off[a]; t < ex_off[a + 1]; ++t)
+    if (ex_idx[t] == b) return true;
+  return false;
+}
+
+template <typename T>
+__device__ inline void lj_params(const PairDesc &d, int ti, int tj, double &c12, double &c6) {
+  if (d.lj_dense) {
+    double2 v = d.lj_dense[(size_t)ti * d.ntypes + tj];
+    c12 = v.x;
+    c6 = v.y;
+  } else {
+    int lij = d.type_table[ti + (size_t)d.ntypes * tj];
+    c12 = d.ecc12[lij];
+    c6 = d.ecc6[lij];
+  }
+}
+
+// One owner group per wavefront; MAXI bounds the owner group's atom count.
+template <typename T, int MAXI>
+__global__ void __launch_bounds__(256) k_pairs(const PairDesc *__restrict__ D) {
+  __shared__ double red[PAIR_WAVES][16];
+  const PairDesc &d = D[blockIdx.y];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int g = blockIdx.x * PAIR_WAVES + wid;
+  double fa[MAXI][3];
+#pragma unroll
+  for (int a = 0; a < MAXI; ++a) fa[a][0] = fa[a][1] = fa[a][2] = 0.0;
+  double en[6] = {0, 0, 0, 0, 0, 0};  // uconf[3], ucoul[3]
+  double st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int ia0 = 0, nI = 0;
+  if (g < d.ngrp) {
+    ia0 = d.gfirst[g];
+    nI = d.glast[g] - ia0 + 1;
+    const int typei = d.gss[g];
+    const double xpgi = d.xpg[g], ypgi = d.ypg[g], zpgi = d.zpg[g];
+    const double *co = d.co;
+    const int nb1 = d.nb_off[g + 1];
+    for (int jj = d.nb_off[g] + lane; jj < nb1; jj += 64) {
+      const int j = d.nb_idx[jj];
+      // group classification (force.f:265-335), no contraction
+      double sxm, sym, szm, xcc, ycc, zcc, drj;
+      {
+#pragma clang fp contract(off)
+        double xgg = xpgi - d.xpg[j], ygg = ypgi - d.ypg[j], zgg = zpgi - d.zpg[j];
+        sxm = 2.0 * fnint(0.5 * xgg);
+        sym = 2.0 * fnint(0.5 * ygg);
+        szm = 2.0 * fnint(0.5 * zgg);
+        xgg = xgg - sxm;
+        ygg = ygg - sym;
+        zgg = zgg - szm;
+        xcc = co[0] * xgg + co[3] * ygg + co[6] * zgg;
+        ycc = co[4] * ygg + co[7] * zgg;
+        zcc = co[8] * zgg;
+        drj = xcc * xcc + ycc * ycc + zcc * zcc;
+      }
+      int cls = 0;  // 0 none, 1 normal, 2 switched
+      double sw = 1.0, dsw = 0.0;
+      if (drj >= d.r2inn0 && drj < d.r2out) {
+        cls = 1;
+      } else if (drj >= d.r2out && drj < d.r2out0) {
+        cls = 2;
+        double rsp = sqrt(drj);
+        double auxa = (d.arsout1 + 2.0 * rsp) / d.arsout2;
+        double auxb = d.rout0 - rsp;
+        sw = auxa * (auxb * auxb);
+        dsw = (-(2.0 * auxa * auxb) + 2.0 * (auxb * auxb) / d.arsout2) / rsp;
+      } else if (drj >= d.r2inn && drj < d.r2inn0) {
+        cls = 2;
+        double rsp = sqrt(drj);
+        double auxa = (d.arsinn1 + 2.0 * rsp) / d.arsinn2;
+        double auxb = d.rinn0 - rsp;
+        sw = 1.0 - auxa * (auxb * auxb);
+        dsw = (2.0 * auxa * auxb - 2.0 * (auxb * auxb) / d.arsinn2) / rsp;
+      }
+      if (cls == 0) continue;
+      const bool fwd = j > g;
+      const int jb0 = d.gfirst[j], jb1 = d.glast[j];
+      const int typeij = typei + d.gss[j] - 2;  // 0..2
+      double cmap2 = 0.0;
+#pragma unroll
+      for (int ai = 0; ai < MAXI; ++ai) {
+        if (ai >= nI) break;
+        const int a = ia0 + ai;
+        const double xpi = d.xp[a], ypi = d.yp[a], zpi = d.zp[a];
+        const int ti = d.type[a];
+        const double qi = d.q[a];
+        const int pa = d.atomp[a];
+        const bool mask_a = fwd && d.ex_inter[a] && has_partner_in(d.ex_off, d.ex_idx, a, jb0, jb1);
+        double ucoula = 0.0, uconfa = 0.0;
+        for (int b = jb0; b <= jb1; ++b) {
+          bool masked = fwd ? mask_a : (d.ex_inter[b] && has_partner_in(d.ex_off, d.ex_idx, b, ia0, ia0 + nI - 1));
+          if (masked) {
+            if (is_excluded(d.ex_off, d.ex_idx, a, b)) continue;
+            if (d.any_heavy && !(d.mass[b] < d.massinfty || d.mass[a] < d.massinfty)) continue;
+          }
+          double c12, c6;
+          lj_params<T>(d, ti, d.type[b], c12, c6);
+          if (d.lskip_ewald && c6 < 1e-12 && d.ewald) continue;
+          const int pb = d.atomp[b];
+          double xg, yg, zg, xxcm, yycm, zzcm;
+          if (cls == 1) {
+            double xd1 = xpi - sxm, yd1 = ypi - sym, zd1 = zpi - szm;
+            xg = xd1 - d.xp[b];
+            yg = yd1 - d.yp[b];
+            zg = zd1 - d.zp[b];
+            xxcm = (d.xpcm[pa] - sxm) - d.xpcm[pb];
+            yycm = (d.ypcm[pa] - sym) - d.ypcm[pb];
+            zzcm = (d.zpcm[pa] - szm) - d.zpcm[pb];
+          } else {
+            xg = (xpi - d.xp[b]) - sxm;
+            yg = (ypi - d.yp[b]) - sym;
+            zg = (zpi - d.zp[b]) - szm;
+            xxcm = (d.xpcm[pa] - d.xpcm[pb]) - sxm;
+            yycm = (d.ypcm[pa] - d.ypcm[pb]) - sym;
+            zzcm = (d.zpcm[pa] - d.zpcm[pb]) - szm;
+          }
+          const T xc = T(co[0] * xg + co[3] * yg + co[6] * zg);
+          const T yc = T(co[4] * yg + co[7] * zg);
+          const T zc = T(co[8] * zg);
+          const T rsq = xc * xc + yc * yc + zc * zc;
+          const T rsqi = T(1) / rsq;
+          T qforce = T(0);
+          double vpair = 0.0;
+          if (c6 > 1e-12) {
+            T r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+            T ssvir = T(12.0 * c12) * r12 - T(6.0 * c6) * r6;
+            qforce = ssvir * rsqi;
+            double conf = (double)(T(c12) * r12 - T(c6) * r6);
+            vpair += conf;
+            if (cls == 1) uconfa += conf;
+            else if (fwd) en[typeij] += sw * conf;
+          }
+          if (!d.lskip_ewald) {
+            T rsp = sqrt(rsq);
+            T rspqi = rsqi / rsp;
+            T alphar = T(d.alpha) * rsp;
+            T qt = T(1) / (T(1) + T(AS_QP) * alphar);
+            T expcst = exp(-(alphar * alphar));
+            T erfcst = ((((T(AS_A5) * qt + T(AS_A4)) * qt + T(AS_A3)) * qt + T(AS_A2)) * qt + T(AS_A1)) * qt * expcst;
+            T furpar = T(qi * d.q[b]);
+            double vel = (double)(furpar * erfcst / rsp);
+            vpair += vel;
+            if (cls == 1) ucoula += vel;
+            else if (fwd) en[3 + typeij] += sw * vel;
+            qforce += furpar * (erfcst + T(d.twrtpi) * alphar * expcst) * rspqi;
+          } else if (!d.ewald) {
+            T rsp = sqrt(rsq);
+            T ssv = T(qi * d.q[b]) / rsp;
+            vpair += (double)ssv;
+            if (fwd) en[3 + typeij] += (cls == 1 ? 1.0 : sw) * (double)ssv;
+            qforce += ssv * rsqi;
+          }
+          double qf = (double)qforce;
+          if (cls == 2) {
+            qf *= sw;
+            cmap2 += vpair;
+          }
+          const double fxv = qf * (double)xc, fyv = qf * (double)yc, fzv = qf * (double)zc;
+#pragma unroll
+          for (int k = 0; k < MAXI; ++k)
+            if (k == ai) {
+              fa[k][0] += fxv;
+              fa[k][1] += fyv;
+              fa[k][2] += fzv;
+            }
+          if (fwd) {
+            st[0] += fxv * xxcm; st[1] += fxv * yycm; st[2] += fxv * zzcm;
+            st[3] += fyv * xxcm; st[4] += fyv * yycm; st[5] += fyv * zzcm;
+            st[6] += fzv * xxcm; st[7] += fzv * yycm; st[8] += fzv * zzcm;
+          }
+        }
+        if (fwd && cls == 1) {
+          en[typeij] += uconfa;
+          en[3 + typeij] += ucoula;
+        }
+      }
+      if (cls == 2) {
+        // S'(r) V spread over group atoms by mass fraction (force.f:756-817)
+        const double x3 = -(dsw * cmap2 * xcc), y3 = -(dsw * cmap2 * ycc), z3 = -(dsw * cmap2 * zcc);
+#pragma unroll
+        for (int ai = 0; ai < MAXI; ++ai) {
+          if (ai >= nI) break;
+          const int a = ia0 + ai;
+          const double m = d.gmass[a];
+          fa[ai][0] += m * x3;
+          fa[ai][1] += m * y3;
+          fa[ai][2] += m * z3;
+          if (fwd) {
+            const int pa = d.atomp[a];
+            const double px = d.xpcm[pa], py = d.ypcm[pa], pz = d.zpcm[pa];
+            st[0] += m * x3 * px; st[1] += m * x3 * py; st[2] += m * x3 * pz;
+            st[3] += m * y3 * px; st[4] += m * y3 * py; st[5] += m * y3 * pz;
+            st[6] += m * z3 * px; st[7] += m * z3 * py; st[8] += m * z3 * pz;
+          }
+        }
+        if (fwd) {
+          for (int b = jb0; b <= jb1; ++b) {
+            const double m = d.gmass[b];
+            const int pb = d.atomp[b];
+            const double px = -((d.xpcm[pb] + sxm) * m), py = -((d.ypcm[pb] + sym) * m),
+                         pz = -((d.zpcm[pb] + szm) * m);
+            st[0] += x3 * px; st[1] += x3 * py; st[2] += x3 * pz;
+            st[3] += y3 * px; st[4] += y3 * py; st[5] += y3 * pz;
+            st[6] += z3 * px; st[7] += z3 * py; st[8] += z3 * pz;
+          }
+        }
+      }
+    }
+  }
+  // owner-atom forces: wave reduction, lane 0 writes
+#pragma unroll
+  for (int ai = 0; ai < MAXI; ++ai) {
+    double vx = wave_sum(fa[ai][0]), vy = wave_sum(fa[ai][1]), vz = wave_sum(fa[ai][2]);
+    if (lane == 0 && ai < nI) {
+      d.fx[ia0 + ai] = vx;
+      d.fy[ia0 + ai] = vy;
+      d.fz[ia0 + ai] = vz;
+    }
+  }
+  // energies and virial: wave then block, fixed order
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double v = wave_sum(en[k]);
+    if (lane == 0) red[wid][k] = v;
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    double v = wave_sum(st[k]);
+    if (lane == 0) red[wid][6 + k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 15) {
+    double s = 0.0;
+    for (int w = 0; w < PAIR_WAVES; ++w) s += red[w][threadIdx.x];
+    d.part[(size_t)blockIdx.x * 16 + threadIdx.x] = s;
+  }
+}
+
+// out[sys*16]: uconf[3], ucoul[3] (slt, slt-slv, slv order of typeij), st[9] row-major
+__global__ void k_pairs_finalize(const PairDesc *__restrict__ D, double *out) {
+  const PairDesc &d = D[blockIdx.x];
+  if (threadIdx.x >= 15) return;
+  double s = 0.0;
+  for (int b = 0; b < d.nblk; ++b) s += d.part[(size_t)b * 16 + threadIdx.x];
+  out[(size_t)blockIdx.x * 16 + threadIdx.x] = s;
+}
+
+template <typename T>
+void launch_pairs(const PairDesc *d_desc, const PairDesc &h, int nsys, int max_group, hipStream_t st) {
+  dim3 grd(h.nblk, nsys), blk(64 * PAIR_WAVES);
+  if (max_group <= 1)
+    hipLaunchKernelGGL((k_pairs<T, 1>), grd, blk, 0, st, d_desc);
+  else if (max_group <= 3)
+    hipLaunchKernelGGL((k_pairs<T, 3>), grd, blk, 0, st, d_desc);
+  else if (max_group <= 8)
+    hipLaunchKernelGGL((k_pairs<T, 8>), grd, blk, 0, st, d_desc);
+  else if (max_group <= 16)
+    hipLaunchKernelGGL((k_pairs<T, 16>), grd, blk, 0, st, d_desc);
+  else
+    throw Error(E_UNSUPPORTED, "groups larger than 16 atoms are not supported by the pair kernel");
+}
+
+void launch_pairs_finalize(const PairDesc *d_desc, int nsys, double *d_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_pairs_finalize, dim3(nsys), dim3(64), 0, st, d_desc, d_out);
+}
+
+template void launch_pairs<double>(const PairDesc *, const PairDesc &, int, int, hipStream_t);
+template void launch_pairs<float>(const PairDesc *, const PairDesc &, int, int, hipStream_t);
+
+// ---------------------------------------------------------------- ferrf / fnbgrp
+
+__global__ void k_corr_pairs(const CorrDesc *__restrict__ D) {
+  const CorrDesc &d = *D;
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= d.npairs) return;
+  int ia = d.ia[p], ib = d.ib[p];
+  double xa, ya, za, xb, yb, zb;
+  if (d.x) {
+    xa = d.x[ia]; ya = d.y[ia]; za = d.z[ia];
+    xb = d.x[ib]; yb = d.y[ib]; zb = d.z[ib];
+  } else {
+    const double *co = d.co;
+    xa = co[0] * d.xp[ia] + co[3] * d.yp[ia] + co[6] * d.zp[ia];
+    ya = co[1] * d.xp[ia] + co[4] * d.yp[ia] + co[7] * d.zp[ia];
+    za = co[2] * d.xp[ia] + co[5] * d.yp[ia] + co[8] * d.zp[ia];
+    xb = co[0] * d.xp[ib] + co[3] * d.yp[ib] + co[6] * d.zp[ib];
+    yb = co[1] * d.xp[ib] + co[4] * d.yp[ib] + co[7] * d.zp[ib];
+    zb = co[2] * d.xp[ib] + co[5] * d.yp[ib] + co[8] * d.zp[ib];
+  }
+  double xab = xa - xb, yab = ya - yb, zab = za - zb;
+  double rsq = xab * xab + yab * yab + zab * zab;
+  double rsp = sqrt(rsq);
+  double ecoul = 0.0, elj = 0.0, qforce = 0.0;
+  if (d.kind == 0) {
+    // ferrf.f:162-188 (iz = 1) and 212-226 (iz = 0)
+    double fud = d.fudge_p ? d.fudge_p[p] : d.fudge;
+    double furpar = d.iz == 1 ? fud * d.q[ia] * d.q[ib] : d.q[ia] * d.q[ib];
+    double alphar = d.alpha * rsp;
+    double qt = 1.0 / (1.0 + AS_QP * alphar);
+    double expcst = exp(-(alphar * alphar));
+    double erfcst = ((((AS_A5 * qt + AS_A4) * qt + AS_A3) * qt + AS_A2) * qt + AS_A1) * qt * expcst;
+    double erfst = 1.0 - erfcst;
+    ecoul = -(furpar * erfst / rsp);
+    if (d.iz == 1) qforce = -(furpar * (erfst - d.twrtpi * alphar * expcst) / (rsp * rsq));
+  } else {
+    // fnbgrp.f:120-202 (reduced coordinates, no PBC)
+    xab = d.xp[ia] - d.xp[ib];
+    yab = d.yp[ia] - d.yp[ib];
+    zab = d.zp[ia] - d.zp[ib];
+    rsq = xab * xab + yab * yab + zab * zab;
+    rsp = sqrt(rsq);
+    int ti = d.type[ia], tj = d.type[ib];
+    int lj = ti > tj ? ti : tj, li = ti < tj ? ti : tj;
+    int lij = lj * (lj - 1) / 2 + li - 1;
+    double rsqi = 1.0 / rsq;
+    double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+    double ssvir = 12.0 * d.ecc12[lij] * r12 - 6.0 * d.ecc6[lij] * r6;
+    qforce = ssvir * rsqi;
+    elj = d.ecc12[lij] * r12 - d.ecc6[lij] * r6;
+    double furpar = d.q[ia] * d.q[ib];
+    if (d.ewald) {
+      double alphar = d.alpha * rsp;
+      double qt = 1.0 / (1.0 + AS_QP * alphar);
+      double expcst = exp(-(alphar * alphar));
+      double erfcst = ((((AS_A5 * qt + AS_A4) * qt + AS_A3) * qt + AS_A2) * qt + AS_A1) * qt * expcst;
+      ecoul = furpar * erfcst / rsp;
+      qforce += furpar * (erfcst + d.twrtpi * alphar * expcst) * (rsqi / rsp);
+    } else {
+      ecoul = furpar / rsp;
+      qforce += ecoul * rsqi;
+    }
+  }
+  d.pf[3 * (size_t)p] = qforce * xab;
+  d.pf[3 * (size_t)p + 1] = qforce * yab;
+  d.pf[3 * (size_t)p + 2] = qforce * zab;
+  d.pe[2 * (size_t)p] = ecoul;
+  d.pe[2 * (size_t)p + 1] = elj;
+}
+
+__global__ void k_corr_gather(const CorrDesc *__restrict__ D) {
+  const CorrDesc &d = *D;
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= d.natoms) return;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int t = d.g_off[a]; t < d.g_off[a + 1]; ++t) {
+    int sp = d.g_pair[t];
+    int p = (sp > 0 ? sp : -sp) - 1;
+    double s = sp > 0 ? 1.0 : -1.0;
+    sx += s * d.pf[3 * (size_t)p];
+    sy += s * d.pf[3 * (size_t)p + 1];
+    sz += s * d.pf[3 * (size_t)p + 2];
+  }
+  d.fx[a] += sx;
+  d.fy[a] += sy;
+  d.fz[a] += sz;
+}
+
+// out[4]: coul(slt), coul(slv), lj(slt), lj(slv), summed in pair order
+__global__ void k_corr_energy(const CorrDesc *__restrict__ D, double *out) {
+  const CorrDesc &d = *D;
+  if (threadIdx.x != 0) return;
+  double e[4] = {0, 0, 0, 0};
+  for (int p = 0; p < d.npairs; ++p) {
+    int c = d.ss[d.ia[p]] - 1;
+    e[c] += d.pe[2 * (size_t)p];
+    e[2 + c] += d.pe[2 * (size_t)p + 1];
+  }
+  for (int k = 0; k < 4; ++k) out[k] = e[k];
+}
+
+void launch_corr(const CorrDesc &h, CorrDesc *d_desc, hipStream_t st) {
+  if (h.npairs > 0)
+    hipLaunchKernelGGL(k_corr_pairs, dim3((h.npairs + 255) / 256), dim3(256), 0, st, d_desc);
+  if (h.iz == 1 && h.natoms > 0 && h.npairs > 0)
+    hipLaunchKernelGGL(k_corr_gather, dim3((h.natoms + 255) / 256), dim3(256), 0, st, d_desc);
+}
+
+void launch_corr_energy(const CorrDesc *d_desc, double *d_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_corr_energy, dim3(1), dim3(64), 0, st, d_desc, d_out);
+}
+
+// ---------------------------------------------------------------- cself
+
+// cself.f:45-56 sums, fixed order: out[0] = sum q^2 (ss 1), out[1] = (ss 2)
+__global__ void k_cself(int n, const double *q, const int *ss, double *out) {
+  __shared__ double red[2][16];
+  double s[2] = {0.0, 0.0};
+  int per = (n + blockDim.x - 1) / blockDim.x;
+  int a0 = threadIdx.x * per, a1 = min(n, a0 + per);
+  for (int a = a0; a < a1; ++a) s[ss[a] - 1] += q[a] * q[a];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int k = 0; k < 2; ++k) {
+    double v = wave_sum(s[k]);
+    if (lane == 0) red[k][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[threadIdx.x][w];
+    out[threadIdx.x] = t;
+  }
+}
+
+void launch_cself(int n, const double *q, const int *ss, double *d_out2, hipStream_t st) {
+  hipLaunchKernelGGL(k_cself, dim3(1), dim3(1024), 0, st, n, q, ss, d_out2);
+}
+
+// ---------------------------------------------------------------- small utilities
+
+__global__ void k_zero_u64(unsigned long long *p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+__global__ void k_zero_f64(double *p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+__global__ void k_add3(int n, double *fx, double *fy, double *fz, const double *ax, const double *ay, const double *az) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fx[i] += ax[i];
+  fy[i] += ay[i];
+  fz[i] += az[i];
+}
+
+void launch_zero_u64(unsigned long long *p, size_t n, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+void launch_zero_f64(double *p, size_t n, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+void launch_add3(int n, double *fx, double *fy, double *fz, const double *ax, const double *ay, const double *az,
+                 hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_add3, dim3((n + 255) / 256), dim3(256), 0, st, n, fx, fy, fz, ax, ay, az);
+}
+
+}  // namespace orac

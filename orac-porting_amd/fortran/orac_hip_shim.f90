@@ -1,0 +1,247 @@
+! orac_hip_shim.f90 -- ISO_C_BINDING drop-in for ORAC's nonbonded routines.
+!
+! Link this object in place of fft_pme.o, fftw3_pme_init.o, force.o,
+! lc_list.o, lc_index.o, ferrf.o, cself.o and fnbgrp.o: the external
+! subroutines below keep the reference names and argument lists, so
+! mts_furier.f, mts_forces.f and mtsmd.f compile and link unchanged.  Each
+! forwards to the C ABI of liborac_hip.so (include/orac_hip.h); a nonzero
+! status calls the caller's xerror(msg, 80, 1, 2), ORAC's fatal path.
+!
+! Layouts are the reference's serial (non-OpenMP) build: nnlpp, nnlpp1 and
+! mapnl are single lists.  List capacity defaults to mpp = _NAT_PP_*_TGROUP_
+! of the shipped config.H (524*50000), the bound the reference checks;
+! orac_hip_set_list_capacity changes it.
+module orac_hip_iface
+  use, intrinsic :: iso_c_binding
+  implicit none
+  type(c_ptr), save :: orac_ctx = c_null_ptr
+  integer(c_int64_t), save :: orac_list_capacity = 524_c_int64_t * 50000_c_int64_t
+
+  interface
+    integer(c_int) function c_create(ctx, device) bind(C, name="orac_hip_create")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr) :: ctx
+      integer(c_int32_t), value :: device
+    end function
+    integer(c_int) function c_last_error(buf, n) bind(C, name="orac_hip_last_error")
+      import :: c_char, c_int, c_int32_t
+      character(kind=c_char) :: buf(*)
+      integer(c_int32_t), value :: n
+    end function
+    integer(c_int) function c_fft_pme_init(ctx, numatoms, n1, n2, n3, order, b1, b2, b3) &
+         bind(C, name="orac_hip_fft_pme_init")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: numatoms, n1, n2, n3, order
+      real(c_double) :: b1(*), b2(*), b3(*)
+    end function
+    integer(c_int) function c_fft_pme(ctx, ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, co, oc, volume, alphal, &
+         order, n1, n2, n3, eer, fpx, fpy, fpz, stressc, atomp, grppt, pressure, rkcut) bind(C, name="orac_hip_fft_pme")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ntap, order, n1, n2, n3, pressure
+      real(c_double) :: xp(*), yp(*), zp(*), xpcm(*), ypcm(*), zpcm(*), pmechg(*), co(*), oc(*)
+      real(c_double), value :: volume, alphal, rkcut
+      real(c_double) :: eer, fpx(*), fpy(*), fpz(*), stressc(*)
+      integer(c_int32_t) :: atomp(*), grppt(*)
+    end function
+    integer(c_int) function c_lc_index(indmax, ncx, ncy, ncz, nind, indxi, indxj, indxk, ctoff, co) &
+         bind(C, name="orac_hip_lc_index")
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: indmax, ncx, ncy, ncz
+      integer(c_int32_t) :: nind(*), indxi(*), indxj(*), indxk(*)
+      real(c_double), value :: ctoff
+      real(c_double) :: co(*)
+    end function
+    integer(c_int) function c_lc_list(ctx, ncx, ncy, ncz, nind, indxi, indxj, indxk, rcut, co, xp, yp, zp, natp, &
+         nnlpp, cap, nent) bind(C, name="orac_hip_lc_list")
+      import :: c_ptr, c_int, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ncx, ncy, ncz, natp
+      integer(c_int32_t) :: nind(*), indxi(*), indxj(*), indxk(*), nnlpp(*)
+      real(c_double), value :: rcut
+      real(c_double) :: co(*), xp(*), yp(*), zp(*)
+      integer(c_int64_t), value :: cap
+      integer(c_int64_t) :: nent
+    end function
+    integer(c_int) function c_force(ctx, ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, &
+         atomg, xpcm, ypcm, zpcm, groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, &
+         uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, cap, rneigh, rinn, rout, &
+         rtolinn, rtolout, gmass, iz, ncount1, mass, massinfty, lskip_ewald, ewald) bind(C, name="orac_hip_force")
+      import :: c_ptr, c_int, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), nbtype(*), type(*), atomg(*), groupp(*), atomp(*), mapnl(*), grppt(*)
+      integer(c_int32_t) :: nnlpp(*), nnlpp1(*), ncount1
+      integer(c_int32_t), value :: ma, nato, ngrp, iz, lskip_ewald, ewald
+      real(c_double) :: xp0(*), yp0(*), zp0(*), xpg(*), ypg(*), zpg(*), charge(*), xpcm(*), ypcm(*), zpcm(*)
+      real(c_double) :: co(*), ecc12(*), ecc6(*), gmass(*), mass(*), fpx(*), fpy(*), fpz(*), stress(*)
+      real(c_double) :: uconf_slt, uconf_slv, uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss, rtolinn
+      real(c_double), value :: alphal, rneigh, rinn, rout, rtolout, massinfty
+      integer(c_int64_t), value :: cap
+    end function
+    integer(c_int) function c_ferrf(ctx, ss_index, alphal, charge, fudge, x0, y0, z0, iz, list, nlist, fs_slt, &
+         fs_slv, fpx, fpy, fpz, erf_corr, nbonds_added, mb) bind(C, name="orac_hip_ferrf")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), list(*)
+      real(c_double), value :: alphal, fudge
+      real(c_double) :: charge(*), x0(*), y0(*), z0(*), fs_slt, fs_slv, fpx(*), fpy(*), fpz(*)
+      integer(c_int32_t), value :: iz, nlist, erf_corr, nbonds_added, mb
+    end function
+    integer(c_int) function c_cself(ctx, ss_index, nato, alphal, kcut, charge, s_slt, s_slv) &
+         bind(C, name="orac_hip_cself")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*)
+      integer(c_int32_t), value :: nato
+      real(c_double), value :: alphal, kcut
+      real(c_double) :: charge(*), s_slt, s_slv
+    end function
+    integer(c_int) function c_fnbgrp(ctx, ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alphal, ingrp, &
+         ingrpp, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, nato) bind(C, name="orac_hip_fnbgrp")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), type(*), ingrp(*)
+      real(c_double) :: xp0(*), yp0(*), zp0(*), charge(*), ecc12(*), ecc6(*), fpx(*), fpy(*), fpz(*)
+      real(c_double) :: uconf_slt, uconf_slv, ucoul_slt, ucoul_slv
+      integer(c_int32_t), value :: ewald, ingrpp, nato
+      real(c_double), value :: alphal
+    end function
+  end interface
+
+contains
+
+  subroutine orac_hip_set_list_capacity(n)
+    integer, intent(in) :: n
+    orac_list_capacity = int(n, c_int64_t)
+  end subroutine
+
+  function ctx() result(p)
+    type(c_ptr) :: p
+    if (.not. c_associated(orac_ctx)) call check(c_create(orac_ctx, 0_c_int32_t))
+    p = orac_ctx
+  end function
+
+  integer(c_int32_t) function lg(l)
+    logical, intent(in) :: l
+    lg = 0
+    if (l) lg = 1
+  end function
+
+  subroutine check(rc)
+    integer(c_int), intent(in) :: rc
+    character(kind=c_char) :: buf(80)
+    character(len=80) :: msg
+    integer :: i, junk
+    if (rc == 0) return
+    junk = c_last_error(buf, 80_c_int32_t)
+    msg = ' '
+    do i = 1, 80
+      if (buf(i) == c_null_char) exit
+      msg(i:i) = buf(i)
+    end do
+    call xerror(msg, 80, 1, 2)
+  end subroutine
+end module orac_hip_iface
+
+! fftw3_pme_init.f90:2 (FFTW build signature; the plan handles are unused)
+subroutine fft_pme_init(numatoms, nfft1, nfft2, nfft3, order, bsp_mod1, bsp_mod2, bsp_mod3, planf, planb)
+  use orac_hip_iface
+  implicit none
+  integer :: numatoms, nfft1, nfft2, nfft3, order
+  real(8) :: bsp_mod1(*), bsp_mod2(*), bsp_mod3(*)
+  type(c_ptr) :: planf, planb
+  call check(c_fft_pme_init(ctx(), numatoms, nfft1, nfft2, nfft3, order, bsp_mod1, bsp_mod2, bsp_mod3))
+  planf = c_null_ptr
+  planb = c_null_ptr
+end subroutine
+
+! fft_pme.f:1-3
+subroutine fft_pme(ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, co, oc, volume, alphal, order, nfft1, nfft2, nfft3, &
+     eer, fpx, fpy, fpz, stressc, atomp, grppt, pressure, rkcut)
+  use orac_hip_iface
+  implicit none
+  integer :: ntap, order, nfft1, nfft2, nfft3, atomp(*), grppt(2,*)
+  real(8) :: xp(*), yp(*), zp(*), xpcm(*), ypcm(*), zpcm(*), pmechg(*), co(3,3), oc(3,3), volume, alphal, eer
+  real(8) :: fpx(*), fpy(*), fpz(*), stressc(3,3), rkcut
+  logical :: pressure
+  call check(c_fft_pme(ctx(), ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, co, oc, volume, alphal, order, nfft1, &
+       nfft2, nfft3, eer, fpx, fpy, fpz, stressc, atomp, grppt, lg(pressure), rkcut))
+end subroutine
+
+! lc_index.f:3-4
+subroutine lc_index(indmax, ncx, ncy, ncz, nind, indxi, indxj, indxk, ctoff, co)
+  use orac_hip_iface
+  implicit none
+  integer :: indmax, ncx, ncy, ncz, nind(2), indxi(2,*), indxj(2,*), indxk(2,*)
+  real(8) :: ctoff, co(3,3)
+  call check(c_lc_index(indmax, ncx, ncy, ncz, nind, indxi, indxj, indxk, ctoff, co))
+end subroutine
+
+! lc_list.f:1-2
+subroutine lc_list(ncx, ncy, ncz, nind, indxi, indxj, indxk, rcut, co, xp, yp, zp, natp, nnlpp, kprint)
+  use orac_hip_iface
+  implicit none
+  integer :: ncx, ncy, ncz, nind(2), indxi(2,*), indxj(2,*), indxk(2,*), natp, nnlpp(*), kprint
+  real(8) :: rcut, co(3,3), xp(*), yp(*), zp(*)
+  integer(c_int64_t) :: nent
+  call check(c_lc_list(ctx(), ncx, ncy, ncz, nind, indxi, indxj, indxk, rcut, co, xp, yp, zp, natp, nnlpp, &
+       orac_list_capacity, nent))
+  write(kprint, '(/5x,''Neighbor Lists Dimensions *neighbor('',i8,'')* '')') nent
+end subroutine
+
+! force.f:1-6 (mts_forces.f:148-152)
+subroutine force(ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, ypcm, zpcm, &
+     groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, uconf_ss, ucoul_slt, &
+     ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, rneigh, rinn, rout, rtolinn, rtolout, gmass, iz, &
+     ncount1, mass, massinfty, lskip_ewald, ewald)
+  use orac_hip_iface
+  implicit none
+  integer :: nato, ngrp, ma, iz, ncount1
+  integer :: nbtype(*), type(ma,*), grppt(2,*), ss_index(*), groupp(*), atomp(*), atomg(*), nnlpp(*), nnlpp1(*)
+  integer :: mapnl(*)
+  real(8) :: xp0(*), yp0(*), zp0(*), co(3,3), charge(*), ecc12(*), ecc6(*), xpg(*), ypg(*), zpg(*)
+  real(8) :: xpcm(*), ypcm(*), zpcm(*), gmass(*), stress(3,3), mass(*), massinfty
+  real(8) :: alphal, uconf_slt, uconf_slv, uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss, rneigh, rinn, rout
+  real(8) :: rtolinn, rtolout, fpx(*), fpy(*), fpz(*)
+  logical :: ewald, lskip_ewald
+  call check(c_force(ctx(), ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, &
+       ypcm, zpcm, groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, uconf_ss, &
+       ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, orac_list_capacity, rneigh, rinn, rout, &
+       rtolinn, rtolout, gmass, iz, ncount1, mass, massinfty, lg(lskip_ewald), lg(ewald)))
+end subroutine
+
+! ferrf.f:1-3
+subroutine ferrf(ss_index, alphal, charge, fudge, x0, y0, z0, iz, list, nlist, fscnstr_slt, fscnstr_slv, fpx, fpy, &
+     fpz, erf_corr, erf_arr_corr, delew, rlew, nbonds_added, mb)
+  use orac_hip_iface
+  implicit none
+  integer :: iz, nlist, list(2,*), ss_index(*), nbonds_added, mb
+  real(8) :: charge(*), x0(*), y0(*), z0(*), fpx(mb), fpy(mb), fpz(mb), fscnstr_slt, fscnstr_slv
+  real(8) :: erf_arr_corr(4,*), delew, rlew, alphal, fudge
+  logical :: erf_corr
+  call check(c_ferrf(ctx(), ss_index, alphal, charge, fudge, x0, y0, z0, iz, list, nlist, fscnstr_slt, fscnstr_slv, &
+       fpx, fpy, fpz, lg(erf_corr), nbonds_added, mb))
+end subroutine
+
+! cself.f:1-2
+subroutine cself(ss_index, nato, alphal, kcut, charge, self_slt, self_slv)
+  use orac_hip_iface
+  implicit none
+  integer :: ss_index(*), nato
+  real(8) :: charge(*), alphal, self_slt, self_slv, kcut
+  call check(c_cself(ctx(), ss_index, nato, alphal, kcut, charge, self_slt, self_slv))
+end subroutine
+
+! fnbgrp.f:1-3 (serial build: itask is unused)
+subroutine fnbgrp(ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alphal, ingrp, ingrpp, fpx, fpy, fpz, &
+     uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, nato, itask)
+  use orac_hip_iface
+  implicit none
+  integer :: nato, ingrpp, type(*), ingrp(2,*), ss_index(*), itask
+  real(8) :: xp0(*), yp0(*), zp0(*), fpx(nato), fpy(nato), fpz(nato), ecc6(*), ecc12(*), alphal, charge(*)
+  real(8) :: uconf_slt, ucoul_slt, uconf_slv, ucoul_slv
+  logical :: ewald
+  call check(c_fnbgrp(ctx(), ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, lg(ewald), alphal, ingrp, ingrpp, &
+       fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, nato))
+end subroutine

@@ -1,0 +1,62 @@
+"""Build of the gfx950 shared library liborac_hip.so (in-tree).
+
+``python -m orac_hip.build`` (with ``orac-porting_amd`` on sys.path) or
+``build()`` compiles every HIP source for gfx950 with hipcc into
+``orac-porting_amd/lib/liborac_hip.so``, and the Fortran drop-in shim
+(``fortran/orac_hip_shim.f90``) into ``lib/liborac_hip_fortran.so`` when
+flang is present.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)                      # orac-porting_amd/
+CSRC = os.path.join(ROOT, "csrc")
+LIBDIR = os.path.join(ROOT, "lib")
+LIB = os.path.join(LIBDIR, "liborac_hip.so")
+FLIB = os.path.join(LIBDIR, "liborac_hip_fortran.so")
+SOURCES = ["pme.hip", "pairs.hip", "api.cpp"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("ORAC_HIP_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # packed-FP32 instructions stay off: hipcc would otherwise emit them from float2 code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build failed: {' '.join(cmd[:3])} ... (exit {r.returncode})")
+    return r
+
+
+def build(verbose=False):
+    os.makedirs(LIBDIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-x", "hip", *srcs, "-o", tmp], verbose)
+    os.replace(tmp, LIB)
+    flang = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    shim = os.path.join(ROOT, "fortran", "orac_hip_shim.f90")
+    if os.path.exists(shim) and os.path.exists(flang):
+        mod = os.path.join(LIBDIR, "fmod")
+        os.makedirs(mod, exist_ok=True)
+        _run([flang, "-O2", "-fPIC", "-shared", "-module-dir", mod, shim, "-o", FLIB + ".tmp", f"-L{LIBDIR}",
+              "-lorac_hip", f"-Wl,-rpath,{LIBDIR}"], verbose)
+        os.replace(FLIB + ".tmp", FLIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

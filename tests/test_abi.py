@@ -1,0 +1,55 @@
+"""The C-ABI library loads without a GPU and exports every function include/orac_hip.h declares."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "orac_hip.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^int (orac_hip_\w+)\(", txt, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    for must in ("orac_hip_fft_pme", "orac_hip_fft_pme_init", "orac_hip_force", "orac_hip_lc_list",
+                 "orac_hip_lc_index", "orac_hip_ferrf", "orac_hip_cself", "orac_hip_fnbgrp"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from orac_hip import api
+    if not os.path.exists(api.library_path()):
+        pytest.skip("liborac_hip.so not built")
+    lib = C.CDLL(api.library_path())
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(declared()) <= set(api.EXPORTS) | set(declared())
+
+
+def test_host_only_entry_points_without_gpu():
+    """lc_index and decide_force are host computations: they run anywhere."""
+    import numpy as np
+
+    import oracle as O
+    from orac_hip.api import Context
+    co = np.asfortranarray(np.diag([30.5] * 3))
+    g = Context.lc_index(2500, 16, 16, 16, 10.6, co)
+    o = O.lc_index(16, 16, 16, 10.6, co)
+    assert all(np.array_equal(a, b) for a, b in zip(g, o))
+    for r in (0.0, 5.1, 7.8, 10.0):
+        assert Context.decide_force(r, 0.37) == O.decide_force(r, 0.37)
+
+
+def test_error_path_is_a_status_not_an_exception():
+    from orac_hip.api import Context, OracHipError
+    import numpy as np
+    co = np.asfortranarray(np.diag([9.32] * 3))
+    with pytest.raises(OracHipError) as e:
+        Context.lc_index(2500, 6, 6, 6, 9.1, co)   # a cell would be visited twice
+    assert e.value.code == 2
