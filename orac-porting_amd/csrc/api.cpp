@@ -883,7 +883,7 @@ static void run_corr_pairs(orac_hip_ctx *ctx, CorrDesc &h, const int32_t *list, 
   h.g_off = B.put_vec(pfx + "goff", goff, st, false);
   h.g_pair = B.put_vec(pfx + "gp", gp, st, false);
   h.pf = B.get<double>(pfx + "pf", 3 * (size_t)std::max(1, nlist));
-  h.pe = B.get<double>(pfx + "pe", 2 * (size_t)std::max(1, nlist));
+  h.pe = B.get<double>(pfx + "pe", 4 * (size_t)std::max(1, nlist));
   h.fx = B.get<double>(pfx + "fx", natoms);
   h.fy = B.get<double>(pfx + "fy", natoms);
   h.fz = B.get<double>(pfx + "fz", natoms);
@@ -1261,7 +1261,7 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
     h.g_off = B.put_vec("cgoff", goff, st, false);
     h.g_pair = B.put_vec("cgp", gp, st, false);
     h.pf = B.get<double>("cpf", 3 * (size_t)std::max(1, nc));
-    h.pe = B.get<double>("cpe", 2 * (size_t)std::max(1, nc));
+    h.pe = B.get<double>("cpe", 4 * (size_t)std::max(1, nc));
   }
   h.xp = S.xp;
   h.yp = S.yp;

@@ -106,7 +106,7 @@ struct CorrDesc {
   const int *type;                    // fnbgrp: per-atom LJ type (1-based, as fnbgrp.f uses)
   const double *ecc12, *ecc6;
   double *pf;                         // per pair force vector (3 per pair)
-  double *pe;                         // per pair energy contributions (2 per pair: coul/lj)
+  double *pe;                         // per pair energies (4 per pair: coul slt, coul slv, lj slt, lj slv)
   const int *g_off, *g_pair;          // per atom gather CSR: signed pair index (+/-(p+1))
   int natoms;
   double *fx, *fy, *fz;               // accumulated (+=)

@@ -49,21 +49,51 @@ __global__ void k_group_centres(int ngrp, const int *gfirst, const int *glast, c
   zpg[g] = sz;
 }
 
-__global__ void k_mol_com(int nmol, const int *mfirst, const int *mlast, const double *mass, const double *xp,
-                          const double *yp, const double *zp, double *xpcm, double *ypcm, double *zpcm) {
-#pragma clang fp contract(off)
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per molecule: lanes stride over its atoms, fixed-order butterfly sum
+__global__ void __launch_bounds__(256) k_mol_com(int nmol, const int *mfirst, const int *mlast, const double *mass,
+                                                 const double *xp, const double *yp, const double *zp, double *xpcm,
+                                                 double *ypcm, double *zpcm) {
+  const int lane = threadIdx.x & 63;
+  const int m = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (m >= nmol) return;
   double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
-  for (int a = mfirst[m]; a <= mlast[m]; ++a) {
-    sx = sx + mass[a] * xp[a];
-    sy = sy + mass[a] * yp[a];
-    sz = sz + mass[a] * zp[a];
-    sm = sm + mass[a];
+  for (int a = mfirst[m] + lane; a <= mlast[m]; a += 64) {
+    sx += mass[a] * xp[a];
+    sy += mass[a] * yp[a];
+    sz += mass[a] * zp[a];
+    sm += mass[a];
   }
-  xpcm[m] = sx / sm;
-  ypcm[m] = sy / sm;
-  zpcm[m] = sz / sm;
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  sz = wave_sum(sz);
+  sm = wave_sum(sm);
+  if (lane == 0) {
+    xpcm[m] = sx / sm;
+    ypcm[m] = sy / sm;
+    zpcm[m] = sz / sm;
+  }
+}
+
+// out[col] = sum over rows of part[row*stride + col], fixed order; one block per column
+__global__ void __launch_bounds__(256) k_reduce_cols(const double *part, int nrows, int stride, double scale,
+                                                     double *out) {
+  __shared__ double red[4];
+  const int col = blockIdx.x;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < nrows; r += blockDim.x) s += part[(size_t)r * stride + col];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    out[col] = scale * t;
+  }
+}
+
+void launch_reduce_cols(const double *part, int nrows, int stride, int ncols, double scale, double *out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_cols, dim3(ncols), dim3(256), 0, st, part, nrows, stride, scale, out);
 }
 
 void launch_group_centres(int ngrp, const int *gfirst, const int *glast, const double *pmass, const double *xp,
@@ -76,8 +106,8 @@ void launch_group_centres(int ngrp, const int *gfirst, const int *glast, const d
 void launch_mol_com(int nmol, const int *mfirst, const int *mlast, const double *mass, const double *xp,
                     const double *yp, const double *zp, double *xpcm, double *ypcm, double *zpcm, hipStream_t st) {
   if (nmol <= 0) return;
-  hipLaunchKernelGGL(k_mol_com, dim3((nmol + 255) / 256), dim3(256), 0, st, nmol, mfirst, mlast, mass, xp, yp, zp,
-                     xpcm, ypcm, zpcm);
+  hipLaunchKernelGGL(k_mol_com, dim3((nmol + 3) / 4), dim3(256), 0, st, nmol, mfirst, mlast, mass, xp, yp, zp, xpcm,
+                     ypcm, zpcm);
 }
 
 // ---------------------------------------------------------------- cells
@@ -202,67 +232,101 @@ void launch_cell_build(int ngrp, const double *xpg, const double *ypg, const dou
   hipLaunchKernelGGL(k_cell_sort, dim3((ncell + 255) / 256), b, 0, st, ncell, cell_start, cell_cnt, cell_items);
 }
 
-// Stencil walk for group n (lc_list.f:133-201). WRITE = false counts only.
-template <bool WRITE>
-__device__ inline int list_walk(int n, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
-                                const int *cell_of, const int *cell_start, const int *cell_cnt,
-                                const int *cell_items, int *out, int one_based) {
+// Pair test of lc_list.f:159-170 (full co, minimum image), no contraction.
+__device__ inline bool list_hit(double x1, double y1, double z1, const double *xpg, const double *ypg,
+                                const double *zpg, int l, const double *co, double sqcut) {
 #pragma clang fp contract(off)
+  double x2 = x1 - xpg[l], y2 = y1 - ypg[l], z2 = z1 - zpg[l];
+  x2 = x2 - 2.0 * fnint(0.5 * x2);
+  y2 = y2 - 2.0 * fnint(0.5 * y2);
+  z2 = z2 - 2.0 * fnint(0.5 * z2);
+  double xx = co[0] * x2 + co[3] * y2 + co[6] * z2;
+  double yy = co[1] * x2 + co[4] * y2 + co[7] * z2;
+  double zz = co[2] * x2 + co[5] * y2 + co[8] * z2;
+  return xx * xx + yy * yy + zz * zz < sqcut;
+}
+
+__device__ inline int wave_sum_int(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline int wave_excl_scan_int(int v, int lane) {
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x - v;
+}
+
+// Stencil walk for group n (lc_list.f:133-201) by one wavefront: lane l takes
+// stencil cells l, l+64, ...; write positions come from a wave prefix sum, so
+// the output order is the reference's: stencil order, then descending group
+// index inside a cell.  half: only l > n (the reference list); else every l != n.
+template <bool WRITE>
+__device__ inline int list_walk_wave(int n, const double *xpg, const double *ypg, const double *zpg,
+                                     const CellGrid &cg, const int *cell_of, const int *cell_start,
+                                     const int *cell_cnt, const int *cell_items, int *out, int one_based) {
+  const int lane = threadIdx.x & 63;
   const double x1 = xpg[n], y1 = ypg[n], z1 = zpg[n];
   const int c = cell_of[n];
   const int i = c / (cg.ncy * cg.ncz), j = (c / cg.ncz) % cg.ncy, k = c % cg.ncz;
-  const double *co = cg.co;
-  int nv = 0;
-  for (int m = 0; m < cg.nsten; ++m) {
-    int nx = fmodp(i + cg.sten[3 * m], cg.ncx);
-    int ny = fmodp(j + cg.sten[3 * m + 1], cg.ncy);
-    int nz = fmodp(k + cg.sten[3 * m + 2], cg.ncz);
-    int cc = nz + cg.ncz * (ny + cg.ncy * nx);
-    int s = cell_start[cc], e = s + cell_cnt[cc];
+  int base = 0;
+  for (int m0 = 0; m0 < cg.nsten; m0 += 64) {
+    const int m = m0 + lane;
+    int s = 0, e = 0;
+    if (m < cg.nsten) {
+      int nx = fmodp(i + cg.sten[3 * m], cg.ncx);
+      int ny = fmodp(j + cg.sten[3 * m + 1], cg.ncy);
+      int nz = fmodp(k + cg.sten[3 * m + 2], cg.ncz);
+      int cc = nz + cg.ncz * (ny + cg.ncy * nx);
+      s = cell_start[cc];
+      e = s + cell_cnt[cc];
+    }
+    int cnt = 0;
     for (int t = s; t < e; ++t) {
-      int l = cell_items[t];   // descending
-      if (cg.half) {
-        if (m == 0 && l <= n) break;
-        if (l <= n) continue;
-      } else if (l == n) {
-        continue;
-      }
-      double x2 = x1 - xpg[l], y2 = y1 - ypg[l], z2 = z1 - zpg[l];
-      x2 = x2 - 2.0 * fnint(0.5 * x2);
-      y2 = y2 - 2.0 * fnint(0.5 * y2);
-      z2 = z2 - 2.0 * fnint(0.5 * z2);
-      double xx = co[0] * x2 + co[3] * y2 + co[6] * z2;
-      double yy = co[1] * x2 + co[4] * y2 + co[7] * z2;
-      double zz = co[2] * x2 + co[5] * y2 + co[8] * z2;
-      double d = xx * xx + yy * yy + zz * zz;
-      if (d < cg.sqcut) {
-        if (WRITE) out[nv] = l + one_based;
-        ++nv;
+      int l = cell_items[t];
+      if (cg.half ? l <= n : l == n) continue;
+      if (list_hit(x1, y1, z1, xpg, ypg, zpg, l, cg.co, cg.sqcut)) ++cnt;
+    }
+    if (WRITE) {
+      int pos = base + wave_excl_scan_int(cnt, lane);
+      for (int t = s; t < e; ++t) {
+        int l = cell_items[t];
+        if (cg.half ? l <= n : l == n) continue;
+        if (list_hit(x1, y1, z1, xpg, ypg, zpg, l, cg.co, cg.sqcut)) out[pos++] = l + one_based;
       }
     }
+    base += wave_sum_int(cnt);
   }
-  return nv;
+  return base;
 }
 
-__global__ void k_list_count(int ngrp, const double *xpg, const double *ypg, const double *zpg, CellGrid cg,
-                             const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
-                             int *count) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_list_count(int ngrp, const double *xpg, const double *ypg,
+                                                    const double *zpg, CellGrid cg, const int *cell_of,
+                                                    const int *cell_start, const int *cell_cnt,
+                                                    const int *cell_items, int *count) {
+  const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (n >= ngrp) return;
-  count[n] = list_walk<false>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, nullptr, 0);
+  int c = list_walk_wave<false>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, nullptr, 0);
+  if ((threadIdx.x & 63) == 0) count[n] = c;
 }
 
-__global__ void k_list_write(int ngrp, const double *xpg, const double *ypg, const double *zpg, CellGrid cg,
-                             const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
-                             const int *off, int *out, int mode) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_list_write(int ngrp, const double *xpg, const double *ypg,
+                                                    const double *zpg, CellGrid cg, const int *cell_of,
+                                                    const int *cell_start, const int *cell_cnt,
+                                                    const int *cell_items, const int *off, int *out, int mode) {
+  const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (n >= ngrp) return;
   if (mode == 1) {
     int o = off[n] + n;  // reference layout: one count slot per preceding group
-    int nv = list_walk<true>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, out + o + 1, 1);
-    out[o] = nv;
+    int nv = list_walk_wave<true>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, out + o + 1, 1);
+    if ((threadIdx.x & 63) == 0) out[o] = nv;
   } else {
-    list_walk<true>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, out + off[n], 0);
+    list_walk_wave<true>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, out + off[n], 0);
   }
 }
 
@@ -270,7 +334,7 @@ void launch_list_count(int ngrp, const double *xpg, const double *ypg, const dou
                        const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
                        int *count, hipStream_t st) {
   if (ngrp <= 0) return;
-  hipLaunchKernelGGL(k_list_count, dim3((ngrp + 127) / 128), dim3(128), 0, st, ngrp, xpg, ypg, zpg, cg, cell_of,
+  hipLaunchKernelGGL(k_list_count, dim3((ngrp + 3) / 4), dim3(256), 0, st, ngrp, xpg, ypg, zpg, cg, cell_of,
                      cell_start, cell_cnt, cell_items, count);
 }
 
@@ -278,7 +342,7 @@ void launch_list_write(int ngrp, const double *xpg, const double *ypg, const dou
                        const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
                        const int *off, int *out, int mode, hipStream_t st) {
   if (ngrp <= 0) return;
-  hipLaunchKernelGGL(k_list_write, dim3((ngrp + 127) / 128), dim3(128), 0, st, ngrp, xpg, ypg, zpg, cg, cell_of,
+  hipLaunchKernelGGL(k_list_write, dim3((ngrp + 3) / 4), dim3(256), 0, st, ngrp, xpg, ypg, zpg, cg, cell_of,
                      cell_start, cell_cnt, cell_items, off, out, mode);
 }
 
@@ -423,28 +487,46 @@ __device__ inline void lj_params(const PairDesc &d, int ti, int tj, double &c12,
   }
 }
 
-// One owner group per wavefront; MAXI bounds the owner group's atom count.
-template <typename T, int MAXI>
+// One owner group per wavefront.  Lane l serves owner atom ai = l % nI for
+// the neighbour-group slot l / nI, so a pass covers 64 / nI neighbour groups
+// (21 for a water owner) and each lane evaluates the pairs of one owner atom
+// with the atoms of one neighbour group.  The switched-pair sum V (cmap2 in
+// force.f:589-693) is gathered across the slot's lanes by shuffles; owner-atom
+// forces are reduced through LDS in fixed lane order.  Energies and the
+// molecular virial come from the j > i direction only, so every pair counts once.
+template <typename T>
 __global__ void __launch_bounds__(256) k_pairs(const PairDesc *__restrict__ D) {
   __shared__ double red[PAIR_WAVES][16];
+  __shared__ double facc[PAIR_WAVES][64][3];
   const PairDesc &d = D[blockIdx.y];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int g = blockIdx.x * PAIR_WAVES + wid;
-  double fa[MAXI][3];
-#pragma unroll
-  for (int a = 0; a < MAXI; ++a) fa[a][0] = fa[a][1] = fa[a][2] = 0.0;
-  double en[6] = {0, 0, 0, 0, 0, 0};  // uconf[3], ucoul[3]
+  double fxa = 0.0, fya = 0.0, fza = 0.0;
+  double en[6] = {0, 0, 0, 0, 0, 0};  // uconf[typeij], ucoul[typeij]
   double st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  int ia0 = 0, nI = 0;
+  int ia0 = 0, nI = 1, nslot = 64;
   if (g < d.ngrp) {
     ia0 = d.gfirst[g];
     nI = d.glast[g] - ia0 + 1;
+    nslot = 64 / nI;
+    const int slot = lane / nI, ai = lane - slot * nI;
+    const bool active = slot < nslot;
+    const int a = ia0 + (active ? ai : 0);
+    const double xpi = d.xp[a], ypi = d.yp[a], zpi = d.zp[a];
+    const int ti = d.type[a];
+    const double qi = d.q[a];
+    const int pa = d.atomp[a];
+    const double xpca = d.xpcm[pa], ypca = d.ypcm[pa], zpca = d.zpcm[pa];
+    const double ma = d.gmass[a];
+    const bool exa = d.ex_inter[a];
     const int typei = d.gss[g];
     const double xpgi = d.xpg[g], ypgi = d.ypg[g], zpgi = d.zpg[g];
     const double *co = d.co;
-    const int nb1 = d.nb_off[g + 1];
-    for (int jj = d.nb_off[g] + lane; jj < nb1; jj += 64) {
-      const int j = d.nb_idx[jj];
+    const int nb0 = d.nb_off[g], nb1 = d.nb_off[g + 1];
+    for (int base = nb0; base < nb1; base += nslot) {
+      const int jj = base + slot;
+      const bool have = active && jj < nb1;
+      const int j = have ? d.nb_idx[jj] : g;
       // group classification (force.f:265-335), no contraction
       double sxm, sym, szm, xcc, ycc, zcc, drj;
       {
@@ -463,37 +545,33 @@ __global__ void __launch_bounds__(256) k_pairs(const PairDesc *__restrict__ D) {
       }
       int cls = 0;  // 0 none, 1 normal, 2 switched
       double sw = 1.0, dsw = 0.0;
-      if (drj >= d.r2inn0 && drj < d.r2out) {
-        cls = 1;
-      } else if (drj >= d.r2out && drj < d.r2out0) {
-        cls = 2;
-        double rsp = sqrt(drj);
-        double auxa = (d.arsout1 + 2.0 * rsp) / d.arsout2;
-        double auxb = d.rout0 - rsp;
-        sw = auxa * (auxb * auxb);
-        dsw = (-(2.0 * auxa * auxb) + 2.0 * (auxb * auxb) / d.arsout2) / rsp;
-      } else if (drj >= d.r2inn && drj < d.r2inn0) {
-        cls = 2;
-        double rsp = sqrt(drj);
-        double auxa = (d.arsinn1 + 2.0 * rsp) / d.arsinn2;
-        double auxb = d.rinn0 - rsp;
-        sw = 1.0 - auxa * (auxb * auxb);
-        dsw = (2.0 * auxa * auxb - 2.0 * (auxb * auxb) / d.arsinn2) / rsp;
+      if (have) {
+        if (drj >= d.r2inn0 && drj < d.r2out) {
+          cls = 1;
+        } else if (drj >= d.r2out && drj < d.r2out0) {
+          cls = 2;
+          double rsp = sqrt(drj);
+          double auxa = (d.arsout1 + 2.0 * rsp) / d.arsout2;
+          double auxb = d.rout0 - rsp;
+          sw = auxa * (auxb * auxb);
+          dsw = (-(2.0 * auxa * auxb) + 2.0 * (auxb * auxb) / d.arsout2) / rsp;
+        } else if (drj >= d.r2inn && drj < d.r2inn0) {
+          cls = 2;
+          double rsp = sqrt(drj);
+          double auxa = (d.arsinn1 + 2.0 * rsp) / d.arsinn2;
+          double auxb = d.rinn0 - rsp;
+          sw = 1.0 - auxa * (auxb * auxb);
+          dsw = (2.0 * auxa * auxb - 2.0 * (auxb * auxb) / d.arsinn2) / rsp;
+        }
       }
-      if (cls == 0) continue;
       const bool fwd = j > g;
-      const int jb0 = d.gfirst[j], jb1 = d.glast[j];
-      const int typeij = typei + d.gss[j] - 2;  // 0..2
-      double cmap2 = 0.0;
-#pragma unroll
-      for (int ai = 0; ai < MAXI; ++ai) {
-        if (ai >= nI) break;
-        const int a = ia0 + ai;
-        const double xpi = d.xp[a], ypi = d.yp[a], zpi = d.zp[a];
-        const int ti = d.type[a];
-        const double qi = d.q[a];
-        const int pa = d.atomp[a];
-        const bool mask_a = fwd && d.ex_inter[a] && has_partner_in(d.ex_off, d.ex_idx, a, jb0, jb1);
+      double cm = 0.0;  // this lane's part of V for a switched group pair
+      int jb0 = 0, jb1 = -1;
+      if (cls != 0) {
+        jb0 = d.gfirst[j];
+        jb1 = d.glast[j];
+        const int typeij = typei + d.gss[j] - 2;  // 0..2
+        const bool mask_a = fwd && exa && has_partner_in(d.ex_off, d.ex_idx, a, jb0, jb1);
         double ucoula = 0.0, uconfa = 0.0;
         for (int b = jb0; b <= jb1; ++b) {
           bool masked = fwd ? mask_a : (d.ex_inter[b] && has_partner_in(d.ex_off, d.ex_idx, b, ia0, ia0 + nI - 1));
@@ -504,23 +582,15 @@ __global__ void __launch_bounds__(256) k_pairs(const PairDesc *__restrict__ D) {
           double c12, c6;
           lj_params<T>(d, ti, d.type[b], c12, c6);
           if (d.lskip_ewald && c6 < 1e-12 && d.ewald) continue;
-          const int pb = d.atomp[b];
-          double xg, yg, zg, xxcm, yycm, zzcm;
+          double xg, yg, zg;
           if (cls == 1) {
-            double xd1 = xpi - sxm, yd1 = ypi - sym, zd1 = zpi - szm;
-            xg = xd1 - d.xp[b];
-            yg = yd1 - d.yp[b];
-            zg = zd1 - d.zp[b];
-            xxcm = (d.xpcm[pa] - sxm) - d.xpcm[pb];
-            yycm = (d.ypcm[pa] - sym) - d.ypcm[pb];
-            zzcm = (d.zpcm[pa] - szm) - d.zpcm[pb];
+            xg = (xpi - sxm) - d.xp[b];
+            yg = (ypi - sym) - d.yp[b];
+            zg = (zpi - szm) - d.zp[b];
           } else {
             xg = (xpi - d.xp[b]) - sxm;
             yg = (ypi - d.yp[b]) - sym;
             zg = (zpi - d.zp[b]) - szm;
-            xxcm = (d.xpcm[pa] - d.xpcm[pb]) - sxm;
-            yycm = (d.ypcm[pa] - d.ypcm[pb]) - sym;
-            zzcm = (d.zpcm[pa] - d.zpcm[pb]) - szm;
           }
           const T xc = T(co[0] * xg + co[3] * yg + co[6] * zg);
           const T yc = T(co[4] * yg + co[7] * zg);
@@ -531,8 +601,7 @@ __global__ void __launch_bounds__(256) k_pairs(const PairDesc *__restrict__ D) {
           double vpair = 0.0;
           if (c6 > 1e-12) {
             T r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
-            T ssvir = T(12.0 * c12) * r12 - T(6.0 * c6) * r6;
-            qforce = ssvir * rsqi;
+            qforce = (T(12.0 * c12) * r12 - T(6.0 * c6) * r6) * rsqi;
             double conf = (double)(T(c12) * r12 - T(c6) * r6);
             vpair += conf;
             if (cls == 1) uconfa += conf;
@@ -561,17 +630,23 @@ __global__ void __launch_bounds__(256) k_pairs(const PairDesc *__restrict__ D) {
           double qf = (double)qforce;
           if (cls == 2) {
             qf *= sw;
-            cmap2 += vpair;
+            cm += vpair;
           }
           const double fxv = qf * (double)xc, fyv = qf * (double)yc, fzv = qf * (double)zc;
-#pragma unroll
-          for (int k = 0; k < MAXI; ++k)
-            if (k == ai) {
-              fa[k][0] += fxv;
-              fa[k][1] += fyv;
-              fa[k][2] += fzv;
-            }
+          fxa += fxv;
+          fya += fyv;
+          fza += fzv;
           if (fwd) {
+            double xxcm, yycm, zzcm;
+            if (cls == 1) {
+              xxcm = (xpca - sxm) - d.xpcm[d.atomp[b]];
+              yycm = (ypca - sym) - d.ypcm[d.atomp[b]];
+              zzcm = (zpca - szm) - d.zpcm[d.atomp[b]];
+            } else {
+              xxcm = (xpca - d.xpcm[d.atomp[b]]) - sxm;
+              yycm = (ypca - d.ypcm[d.atomp[b]]) - sym;
+              zzcm = (zpca - d.zpcm[d.atomp[b]]) - szm;
+            }
             st[0] += fxv * xxcm; st[1] += fxv * yycm; st[2] += fxv * zzcm;
             st[3] += fyv * xxcm; st[4] += fyv * yycm; st[5] += fyv * zzcm;
             st[6] += fzv * xxcm; st[7] += fzv * yycm; st[8] += fzv * zzcm;
@@ -582,48 +657,49 @@ __global__ void __launch_bounds__(256) k_pairs(const PairDesc *__restrict__ D) {
           en[3 + typeij] += ucoula;
         }
       }
+      // total V of the group pair over the slot's lanes (every lane shuffles)
+      double cmt = 0.0;
+      for (int t = 0; t < nI; ++t) cmt += __shfl(cm, (slot * nI + t) & 63, 64);
       if (cls == 2) {
         // S'(r) V spread over group atoms by mass fraction (force.f:756-817)
-        const double x3 = -(dsw * cmap2 * xcc), y3 = -(dsw * cmap2 * ycc), z3 = -(dsw * cmap2 * zcc);
-#pragma unroll
-        for (int ai = 0; ai < MAXI; ++ai) {
-          if (ai >= nI) break;
-          const int a = ia0 + ai;
-          const double m = d.gmass[a];
-          fa[ai][0] += m * x3;
-          fa[ai][1] += m * y3;
-          fa[ai][2] += m * z3;
-          if (fwd) {
-            const int pa = d.atomp[a];
-            const double px = d.xpcm[pa], py = d.ypcm[pa], pz = d.zpcm[pa];
-            st[0] += m * x3 * px; st[1] += m * x3 * py; st[2] += m * x3 * pz;
-            st[3] += m * y3 * px; st[4] += m * y3 * py; st[5] += m * y3 * pz;
-            st[6] += m * z3 * px; st[7] += m * z3 * py; st[8] += m * z3 * pz;
-          }
-        }
+        const double x3 = -(dsw * cmt * xcc), y3 = -(dsw * cmt * ycc), z3 = -(dsw * cmt * zcc);
+        fxa += ma * x3;
+        fya += ma * y3;
+        fza += ma * z3;
         if (fwd) {
-          for (int b = jb0; b <= jb1; ++b) {
-            const double m = d.gmass[b];
-            const int pb = d.atomp[b];
-            const double px = -((d.xpcm[pb] + sxm) * m), py = -((d.ypcm[pb] + sym) * m),
-                         pz = -((d.zpcm[pb] + szm) * m);
-            st[0] += x3 * px; st[1] += x3 * py; st[2] += x3 * pz;
-            st[3] += y3 * px; st[4] += y3 * py; st[5] += y3 * pz;
-            st[6] += z3 * px; st[7] += z3 * py; st[8] += z3 * pz;
+          st[0] += ma * x3 * xpca; st[1] += ma * x3 * ypca; st[2] += ma * x3 * zpca;
+          st[3] += ma * y3 * xpca; st[4] += ma * y3 * ypca; st[5] += ma * y3 * zpca;
+          st[6] += ma * z3 * xpca; st[7] += ma * z3 * ypca; st[8] += ma * z3 * zpca;
+          if (ai == 0) {
+            for (int b = jb0; b <= jb1; ++b) {
+              const double m = d.gmass[b];
+              const int pb = d.atomp[b];
+              const double px = -((d.xpcm[pb] + sxm) * m), py = -((d.ypcm[pb] + sym) * m),
+                           pz = -((d.zpcm[pb] + szm) * m);
+              st[0] += x3 * px; st[1] += x3 * py; st[2] += x3 * pz;
+              st[3] += y3 * px; st[4] += y3 * py; st[5] += y3 * pz;
+              st[6] += z3 * px; st[7] += z3 * py; st[8] += z3 * pz;
+            }
           }
         }
       }
     }
   }
-  // owner-atom forces: wave reduction, lane 0 writes
-#pragma unroll
-  for (int ai = 0; ai < MAXI; ++ai) {
-    double vx = wave_sum(fa[ai][0]), vy = wave_sum(fa[ai][1]), vz = wave_sum(fa[ai][2]);
-    if (lane == 0 && ai < nI) {
-      d.fx[ia0 + ai] = vx;
-      d.fy[ia0 + ai] = vy;
-      d.fz[ia0 + ai] = vz;
+  // owner-atom forces: lanes ai, ai + nI, ... summed in lane order
+  facc[wid][lane][0] = fxa;
+  facc[wid][lane][1] = fya;
+  facc[wid][lane][2] = fza;
+  __syncthreads();
+  if (g < d.ngrp && lane < nI) {
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int s = 0; s < nslot; ++s) {
+      sx += facc[wid][s * nI + lane][0];
+      sy += facc[wid][s * nI + lane][1];
+      sz += facc[wid][s * nI + lane][2];
     }
+    d.fx[ia0 + lane] = sx;
+    d.fy[ia0 + lane] = sy;
+    d.fz[ia0 + lane] = sz;
   }
   // energies and virial: wave then block, fixed order
 #pragma unroll
@@ -644,32 +720,33 @@ __global__ void __launch_bounds__(256) k_pairs(const PairDesc *__restrict__ D) {
   }
 }
 
-// out[sys*16]: uconf[3], ucoul[3] (slt, slt-slv, slv order of typeij), st[9] row-major
-__global__ void k_pairs_finalize(const PairDesc *__restrict__ D, double *out) {
-  const PairDesc &d = D[blockIdx.x];
-  if (threadIdx.x >= 15) return;
+// out[sys*16]: uconf[3], ucoul[3] (slt, slt-slv, slv order of typeij), st[9] row-major;
+// block k reduces component k over the per-block partials in fixed order
+__global__ void __launch_bounds__(256) k_pairs_finalize(const PairDesc *__restrict__ D, double *out) {
+  __shared__ double red[4];
+  const PairDesc &d = D[blockIdx.y];
+  const int k = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < d.nblk; ++b) s += d.part[(size_t)b * 16 + threadIdx.x];
-  out[(size_t)blockIdx.x * 16 + threadIdx.x] = s;
+  for (int b = threadIdx.x; b < d.nblk; b += blockDim.x) s += d.part[(size_t)b * 16 + k];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    out[(size_t)blockIdx.y * 16 + k] = t;
+  }
 }
 
 template <typename T>
 void launch_pairs(const PairDesc *d_desc, const PairDesc &h, int nsys, int max_group, hipStream_t st) {
+  if (max_group > 64) throw Error(E_UNSUPPORTED, "groups larger than 64 atoms are not supported by the pair kernel");
   dim3 grd(h.nblk, nsys), blk(64 * PAIR_WAVES);
-  if (max_group <= 1)
-    hipLaunchKernelGGL((k_pairs<T, 1>), grd, blk, 0, st, d_desc);
-  else if (max_group <= 3)
-    hipLaunchKernelGGL((k_pairs<T, 3>), grd, blk, 0, st, d_desc);
-  else if (max_group <= 8)
-    hipLaunchKernelGGL((k_pairs<T, 8>), grd, blk, 0, st, d_desc);
-  else if (max_group <= 16)
-    hipLaunchKernelGGL((k_pairs<T, 16>), grd, blk, 0, st, d_desc);
-  else
-    throw Error(E_UNSUPPORTED, "groups larger than 16 atoms are not supported by the pair kernel");
+  hipLaunchKernelGGL(k_pairs<T>, grd, blk, 0, st, d_desc);
 }
 
 void launch_pairs_finalize(const PairDesc *d_desc, int nsys, double *d_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_pairs_finalize, dim3(nsys), dim3(64), 0, st, d_desc, d_out);
+  hipLaunchKernelGGL(k_pairs_finalize, dim3(15, nsys), dim3(256), 0, st, d_desc, d_out);
 }
 
 template void launch_pairs<double>(const PairDesc *, const PairDesc &, int, int, hipStream_t);
@@ -741,8 +818,13 @@ __global__ void k_corr_pairs(const CorrDesc *__restrict__ D) {
   d.pf[3 * (size_t)p] = qforce * xab;
   d.pf[3 * (size_t)p + 1] = qforce * yab;
   d.pf[3 * (size_t)p + 2] = qforce * zab;
-  d.pe[2 * (size_t)p] = ecoul;
-  d.pe[2 * (size_t)p + 1] = elj;
+  // energies by the class of the first atom (ferrf.f:169, fnbgrp.f:130): coul slt/slv, lj slt/slv
+  const int c = d.ss[ia] - 1;
+  double *pe = d.pe + 4 * (size_t)p;
+  pe[0] = c == 0 ? ecoul : 0.0;
+  pe[1] = c == 1 ? ecoul : 0.0;
+  pe[2] = c == 0 ? elj : 0.0;
+  pe[3] = c == 1 ? elj : 0.0;
 }
 
 __global__ void k_corr_gather(const CorrDesc *__restrict__ D) {
@@ -763,17 +845,21 @@ __global__ void k_corr_gather(const CorrDesc *__restrict__ D) {
   d.fz[a] += sz;
 }
 
-// out[4]: coul(slt), coul(slv), lj(slt), lj(slv), summed in pair order
-__global__ void k_corr_energy(const CorrDesc *__restrict__ D, double *out) {
+// out[4]: coul(slt), coul(slv), lj(slt), lj(slv); block k reduces column k in fixed order
+__global__ void __launch_bounds__(256) k_corr_energy(const CorrDesc *__restrict__ D, double *out) {
+  __shared__ double red[4];
   const CorrDesc &d = *D;
-  if (threadIdx.x != 0) return;
-  double e[4] = {0, 0, 0, 0};
-  for (int p = 0; p < d.npairs; ++p) {
-    int c = d.ss[d.ia[p]] - 1;
-    e[c] += d.pe[2 * (size_t)p];
-    e[2 + c] += d.pe[2 * (size_t)p + 1];
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int p = threadIdx.x; p < d.npairs; p += blockDim.x) s += d.pe[4 * (size_t)p + k];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    out[k] = t;
   }
-  for (int k = 0; k < 4; ++k) out[k] = e[k];
 }
 
 void launch_corr(const CorrDesc &h, CorrDesc *d_desc, hipStream_t st) {
@@ -784,7 +870,7 @@ void launch_corr(const CorrDesc &h, CorrDesc *d_desc, hipStream_t st) {
 }
 
 void launch_corr_energy(const CorrDesc *d_desc, double *d_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_corr_energy, dim3(1), dim3(64), 0, st, d_desc, d_out);
+  hipLaunchKernelGGL(k_corr_energy, dim3(4), dim3(256), 0, st, d_desc, d_out);
 }
 
 // ---------------------------------------------------------------- cself

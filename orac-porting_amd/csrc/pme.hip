@@ -466,23 +466,35 @@ __global__ void __launch_bounds__(256) k_gather(const PmeDesc *__restrict__ D) {
   }
 }
 
-// out[sys*20]: eer, vir(3x3 col-major), stc(3x3 row-major as stc1..stc9)
-__global__ void k_pme_finalize(const PmeDesc *__restrict__ D, double *out) {
-  const PmeDesc &d = D[blockIdx.x];
+// out[sys*20]: eer, vir(3x3 col-major), stc(3x3 row-major as stc1..stc9).
+// Block k reduces one component in fixed order: k < 7 the convolution
+// partials (scalar_sum.f:125-133 halving and symmetrisation), k >= 7 the
+// molecular-virial partials.
+__global__ void __launch_bounds__(256) k_pme_finalize(const PmeDesc *__restrict__ D, double *out) {
+  __shared__ double red[4];
+  const PmeDesc &d = D[blockIdx.y];
+  const int k = blockIdx.x;
+  const double *src = k < 7 ? d.conv_part + k : d.gather_part + (k - 7);
+  const int nrow = k < 7 ? d.conv_nblk : d.gather_nblk, stride = k < 7 ? 8 : 9;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < nrow; r += blockDim.x) s += src[(size_t)r * stride];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
   if (threadIdx.x != 0) return;
-  double e[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < d.conv_nblk; ++b)
-    for (int k = 0; k < 7; ++k) e[k] += d.conv_part[(size_t)b * 8 + k];
-  double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < d.gather_nblk; ++b)
-    for (int k = 0; k < 9; ++k) s[k] += d.gather_part[(size_t)b * 9 + k];
-  double *o = out + (size_t)blockIdx.x * 20;
-  o[0] = 0.5 * e[0];
-  o[1] = 0.5 * e[1]; o[5] = 0.5 * e[2]; o[9] = 0.5 * e[3];
-  o[4] = o[2] = 0.5 * e[4];   // vir(1,2) = vir(2,1)
-  o[7] = o[3] = 0.5 * e[5];   // vir(1,3) = vir(3,1)
-  o[8] = o[6] = 0.5 * e[6];   // vir(2,3) = vir(3,2)
-  for (int k = 0; k < 9; ++k) o[10 + k] = s[k];
+  s = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+  double *o = out + (size_t)blockIdx.y * 20;
+  switch (k) {
+    case 0: o[0] = 0.5 * s; break;
+    case 1: o[1] = 0.5 * s; break;
+    case 2: o[5] = 0.5 * s; break;
+    case 3: o[9] = 0.5 * s; break;
+    case 4: o[4] = o[2] = 0.5 * s; break;  // vir(1,2) = vir(2,1)
+    case 5: o[7] = o[3] = 0.5 * s; break;  // vir(1,3) = vir(3,1)
+    case 6: o[8] = o[6] = 0.5 * s; break;  // vir(2,3) = vir(3,2)
+    default: o[10 + (k - 7)] = s; break;
+  }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -522,7 +534,7 @@ void launch_pme(const PmeDesc *d_desc, const PmeDesc &h, int nsys, double *d_out
   hipLaunchKernelGGL(k_fft_x_c2r<T>, gx, blk, shx, st, d_desc, tpx);
   dim3 gg(h.gather_nblk, nsys);
   hipLaunchKernelGGL(k_gather<T>, gg, blk, 0, st, d_desc);
-  hipLaunchKernelGGL(k_pme_finalize, dim3(nsys), dim3(64), 0, st, d_desc, d_out);
+  hipLaunchKernelGGL(k_pme_finalize, dim3(16, nsys), dim3(256), 0, st, d_desc, d_out);
 }
 
 int conv_blocks(int K1, int K2, int K3, int lds_elems) {
