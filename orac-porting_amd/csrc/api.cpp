@@ -296,8 +296,24 @@ static PmeDesc pme_desc_base(BufSet &B, const std::string &pfx, const PmePlan &P
   d.ax3 = P.ax[2];
   d.conv_nblk = conv_blocks(P.K1, P.K2, P.K3, lds_elems);
   d.conv_part = B.get<double>(pfx + "conv_part", (size_t)d.conv_nblk * 8);
-  d.gather_nblk = (natoms + 255) / 256;
+  d.gather_nblk = (natoms + 31) / 32;  // k_gather: 8 lanes per atom, 32 atoms per block
   d.gather_part = B.get<double>(pfx + "gather_part", (size_t)std::max(1, d.gather_nblk) * 9);
+  // LDS-tiled spread when every axis has a tile edge in [order-1, 8] dividing K
+  d.st1 = spread_tile_edge(P.K1, P.order);
+  d.st2 = spread_tile_edge(P.K2, P.order);
+  d.st3 = spread_tile_edge(P.K3, P.order);
+  if (!d.st1 || !d.st2 || !d.st3 || std::getenv("ORAC_HIP_SPREAD_ATOMIC")) d.st1 = d.st2 = d.st3 = 0;
+  if (d.st1) {
+    d.nb1 = P.K1 / d.st1;
+    d.nb2 = P.K2 / d.st2;
+    d.nb3 = P.K3 / d.st3;
+    const size_t nbin = (size_t)d.nb1 * d.nb2 * d.nb3;
+    d.bin_cnt = B.get<int>(pfx + "bin_cnt", nbin);
+    d.bin_start = B.get<int>(pfx + "bin_start", nbin);
+    d.atom_bin = B.get<int2>(pfx + "atom_bin", std::max(1, natoms));
+    d.afr = B.get<double4>(pfx + "afr", std::max(1, natoms));
+    d.sfr = B.get<double4>(pfx + "sfr", std::max(1, natoms));
+  }
   return d;
 }
 
@@ -527,8 +543,6 @@ extern "C" int orac_hip_fft_pme(orac_hip_ctx *ctx, int32_t ntap, const double *x
     }
     PmeDesc *dd = B.put("p_desc", &d, 1, st, false);
     double *dout = B.get<double>("p_out", 20);
-    const size_t G = (size_t)d.K1 * d.K2 * d.K3;
-    launch_zero_u64(d.Qfix, G, st);
     launch_pme<double>(dd, d, 1, dout, st, ctx->tiles);
     ORAC_HIP(hipGetLastError());
     std::vector<double> fx(ntap), fy(ntap), fz(ntap);
@@ -597,6 +611,7 @@ extern "C" int orac_hip_lc_list(orac_hip_ctx *ctx, int32_t ncx, int32_t ncy, int
     hipStream_t st = ctx->stream;
     BufSet &B = ctx->B;
     CellGrid cg = make_cellgrid(B, "l_", ncx, ncy, ncz, nind, indxi, indxj, indxk, rcut, co, 1, st);
+    cg.cpos = B.get<double4>("l_cpos", natp);
     int ncell = ncx * ncy * ncz;
     double *dx = B.put("l_x", xp, natp, st, false), *dy = B.put("l_y", yp, natp, st, false),
            *dz = B.put("l_z", zp, natp, st, false);
@@ -818,6 +833,8 @@ extern "C" int orac_hip_force(orac_hip_ctx *ctx, const int32_t *ss_index, const 
     p.fz = B.get<double>("f_fz", nato);
     p.nblk = std::max(1, (ngrp + PAIR_WAVES - 1) / PAIR_WAVES);
     p.part = B.get<double>("f_part", (size_t)p.nblk * 16);
+    p.apos = B.get<double4>("f_apos", nato);
+    p.ainfo = B.get<int2>("f_ainfo", nato);
     PairDesc *dd = B.put("f_desc", &p, 1, st, false);
     double *dout = B.get<double>("f_out", 16);
     if (ngrp > 0) {
@@ -1185,6 +1202,8 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   p.fz = S.fz;
   p.nblk = (ng + PAIR_WAVES - 1) / PAIR_WAVES;
   p.part = B.get<double>("ppart", (size_t)p.nblk * 16);
+  p.apos = B.get<double4>("apos", n);
+  p.ainfo = B.get<int2>("ainfo", n);
 
   // ---- linked cells and the full list
   {
@@ -1200,6 +1219,7 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
                    "single-cell stencil failed");
     }
     S.cg = make_cellgrid(B, "", ncx, ncy, ncz, nind.data(), ii.data(), jj.data(), kk.data(), rl, d.co, 0, st);
+    S.cg.cpos = B.get<double4>("cpos", ng);
     S.ncell = ncx * ncy * ncz;
     double dens = ng / (S.volume);
     double nb = 4.0 / 3.0 * M_PI * rl * rl * rl * dens;
@@ -1325,7 +1345,6 @@ static void system_eval(orac_hip_ctx *ctx, System &S, bool rebuild) {
     launch_pairs<float>(S.pair_d, p, 1, S.max_group, st);
   launch_pairs_finalize(S.pair_d, 1, S.out + 20, st);
   mark(ST_PME);
-  launch_zero_u64(S.pme_h.Qfix, (size_t)S.plan.K1 * S.plan.K2 * S.plan.K3, st);
   if (S.precision == 0)
     launch_pme<double>(S.pme_d, S.pme_h, 1, S.out, st, ctx->tiles);
   else

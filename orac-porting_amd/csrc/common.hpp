@@ -73,6 +73,12 @@ struct PmeDesc {
   const double *xpcm, *ypcm, *zpcm;
   double *gather_part;          // per-block stc partials [nblk][9]
   int gather_nblk;
+  // LDS-tiled spread: grid tiles of st1 x st2 x st3 points (st = 0: global-atomic spread)
+  int st1, st2, st3, nb1, nb2, nb3;
+  int *bin_cnt, *bin_start;     // [nb1*nb2*nb3] atoms per tile bin, exclusive prefix
+  int2 *atom_bin;               // [natoms] {bin, position in bin}
+  double4 *afr;                 // [natoms] {fr1, fr2, fr3, q}
+  double4 *sfr;                 // [natoms] the same, sorted by bin
 };
 
 // ---------------------------------------------------------------- helpers

@@ -12,6 +12,8 @@ template <typename T>
 void launch_pme(const PmeDesc *d_desc, const PmeDesc &h, int nsys, double *d_out, hipStream_t st,
                 const PmeTiles &tiles);
 int conv_blocks(int K1, int K2, int K3, int lds_elems);
+// tile edge of the LDS-tiled spread along an axis of K points (0: use the atomic spread)
+int spread_tile_edge(int K, int order);
 
 // ---------------------------------------------------------------- direct space
 
@@ -43,6 +45,8 @@ struct PairDesc {
   double *fx, *fy, *fz;             // direct forces (written)
   double *part;                     // per-block partials [nblk][16]: uconf[3] ucoul[3] st[9]
   int nblk;
+  double4 *apos;                    // packed {xp, yp, zp, q} per atom (k_pack_atoms)
+  int2 *ainfo;                      // packed {type, flags} per atom; flags bit 0 = ex_inter
 };
 
 constexpr int PAIR_WAVES = 4;       // owner groups per 256-thread block
@@ -65,6 +69,7 @@ struct CellGrid {
   double co[9];
   double sqcut;
   int half;                  // 1: l > n only (reference lc_list); 0: every l != n
+  double4 *cpos;             // [ngrp] {xpg, ypg, zpg, group} in cell order (filled by the cell build)
 };
 void launch_cell_build(int ngrp, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
                        int *cell_of, int *cell_cnt, int *cell_start, int *cell_cursor, int *cell_items,

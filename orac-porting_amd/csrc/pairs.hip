@@ -159,6 +159,16 @@ __global__ void k_cell_sort(int ncell, const int *cell_start, const int *cell_cn
   }
 }
 
+// centres copied into cell order, with the group index, so a stencil walk
+// reads each cell's candidates from contiguous memory
+__global__ void k_cell_pack(int ngrp, const int *items, const double *xpg, const double *ypg, const double *zpg,
+                            double4 *cpos) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ngrp) return;
+  int g = items[t];
+  cpos[t] = make_double4(xpg[g], ypg[g], zpg[g], (double)g);
+}
+
 __global__ void k_zero_i32(int *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -230,13 +240,15 @@ void launch_cell_build(int ngrp, const double *xpg, const double *ypg, const dou
     hipLaunchKernelGGL(k_cell_fill, dim3((ngrp + 255) / 256), b, 0, st, ngrp, cell_of, cell_start, cell_cursor,
                        cell_items);
   hipLaunchKernelGGL(k_cell_sort, dim3((ncell + 255) / 256), b, 0, st, ncell, cell_start, cell_cnt, cell_items);
+  if (ngrp > 0)
+    hipLaunchKernelGGL(k_cell_pack, dim3((ngrp + 255) / 256), b, 0, st, ngrp, cell_items, xpg, ypg, zpg, cg.cpos);
 }
 
 // Pair test of lc_list.f:159-170 (full co, minimum image), no contraction.
-__device__ inline bool list_hit(double x1, double y1, double z1, const double *xpg, const double *ypg,
-                                const double *zpg, int l, const double *co, double sqcut) {
+// p = {xpg, ypg, zpg, group} of the candidate, from the cell-ordered copy.
+__device__ inline bool list_hit(double x1, double y1, double z1, double4 p, const double *co, double sqcut) {
 #pragma clang fp contract(off)
-  double x2 = x1 - xpg[l], y2 = y1 - ypg[l], z2 = z1 - zpg[l];
+  double x2 = x1 - p.x, y2 = y1 - p.y, z2 = z1 - p.z;
   x2 = x2 - 2.0 * fnint(0.5 * x2);
   y2 = y2 - 2.0 * fnint(0.5 * y2);
   z2 = z2 - 2.0 * fnint(0.5 * z2);
@@ -286,18 +298,33 @@ __device__ inline int list_walk_wave(int n, const double *xpg, const double *ypg
       s = cell_start[cc];
       e = s + cell_cnt[cc];
     }
+    // hits of this lane's cell as a bit mask over its (at most 64) first members
     int cnt = 0;
+    unsigned long long hits = 0ull;
     for (int t = s; t < e; ++t) {
-      int l = cell_items[t];
+      const double4 p = cg.cpos[t];
+      const int l = (int)p.w;
       if (cg.half ? l <= n : l == n) continue;
-      if (list_hit(x1, y1, z1, xpg, ypg, zpg, l, cg.co, cg.sqcut)) ++cnt;
+      if (list_hit(x1, y1, z1, p, cg.co, cg.sqcut)) {
+        ++cnt;
+        if (t - s < 64) hits |= 1ull << (t - s);
+      }
     }
     if (WRITE) {
       int pos = base + wave_excl_scan_int(cnt, lane);
       for (int t = s; t < e; ++t) {
-        int l = cell_items[t];
-        if (cg.half ? l <= n : l == n) continue;
-        if (list_hit(x1, y1, z1, xpg, ypg, zpg, l, cg.co, cg.sqcut)) out[pos++] = l + one_based;
+        const int r = t - s;
+        bool hit;
+        int l;
+        if (r < 64) {
+          hit = (hits >> r) & 1ull;
+          l = hit ? (int)cg.cpos[t].w : 0;
+        } else {
+          const double4 p = cg.cpos[t];
+          l = (int)p.w;
+          hit = !(cg.half ? l <= n : l == n) && list_hit(x1, y1, z1, p, cg.co, cg.sqcut);
+        }
+        if (hit) out[pos++] = l + one_based;
       }
     }
     base += wave_sum_int(cnt);
@@ -457,300 +484,6 @@ void launch_inner_list(int ngrp, int iz, const int *h_off, const int *h_idx, con
   hipLaunchKernelGGL(k_inner_list, dim3((ngrp + 127) / 128), dim3(128), 0, st, ngrp, iz, h_off, h_idx, xpg, ypg, zpg,
                      c, r2neigh, count, off, out, pass);
 }
-
-// ---------------------------------------------------------------- pair kernel
-
-__device__ inline bool has_partner_in(const int *ex_off, const int *ex_idx, int a, int lo, int hi) {
-  for (int t = ex_off[a]; t < ex_off[a + 1]; ++t) {
-    int p = ex_idx[t];
-    if (p >= lo && p <= hi) return true;
-  }
-  return false;
-}
-
-__device__ inline bool is_excluded(const int *ex_off, const int *ex_idx, int a, int b) {
-  for (int t = ex_off[a]; t < ex_off[a + 1]; ++t)
-    if (ex_idx[t] == b) return true;
-  return false;
-}
-
-template <typename T>
-__device__ inline void lj_params(const PairDesc &d, int ti, int tj, double &c12, double &c6) {
-  if (d.lj_dense) {
-    double2 v = d.lj_dense[(size_t)ti * d.ntypes + tj];
-    c12 = v.x;
-    c6 = v.y;
-  } else {
-    int lij = d.type_table[ti + (size_t)d.ntypes * tj];
-    c12 = d.ecc12[lij];
-    c6 = d.ecc6[lij];
-  }
-}
-
-// One owner group per wavefront.  Lane l serves owner atom ai = l % nI for
-// the neighbour-group slot l / nI, so a pass covers 64 / nI neighbour groups
-// (21 for a water owner) and each lane evaluates the pairs of one owner atom
-// with the atoms of one neighbour group.  The switched-pair sum V (cmap2 in
-// force.f:589-693) is gathered across the slot's lanes by shuffles; owner-atom
-// forces are reduced through LDS in fixed lane order.  Energies and the
-// molecular virial come from the j > i direction only, so every pair counts once.
-template <typename T>
-__global__ void __launch_bounds__(256) k_pairs(const PairDesc *__restrict__ D) {
-  __shared__ double red[PAIR_WAVES][16];
-  __shared__ double facc[PAIR_WAVES][64][3];
-  const PairDesc &d = D[blockIdx.y];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int g = blockIdx.x * PAIR_WAVES + wid;
-  double fxa = 0.0, fya = 0.0, fza = 0.0;
-  double en[6] = {0, 0, 0, 0, 0, 0};  // uconf[typeij], ucoul[typeij]
-  double st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  int ia0 = 0, nI = 1, nslot = 64;
-  if (g < d.ngrp) {
-    ia0 = d.gfirst[g];
-    nI = d.glast[g] - ia0 + 1;
-    nslot = 64 / nI;
-    const int slot = lane / nI, ai = lane - slot * nI;
-    const bool active = slot < nslot;
-    const int a = ia0 + (active ? ai : 0);
-    const double xpi = d.xp[a], ypi = d.yp[a], zpi = d.zp[a];
-    const int ti = d.type[a];
-    const double qi = d.q[a];
-    const int pa = d.atomp[a];
-    const double xpca = d.xpcm[pa], ypca = d.ypcm[pa], zpca = d.zpcm[pa];
-    const double ma = d.gmass[a];
-    const bool exa = d.ex_inter[a];
-    const int typei = d.gss[g];
-    const double xpgi = d.xpg[g], ypgi = d.ypg[g], zpgi = d.zpg[g];
-    const double *co = d.co;
-    const int nb0 = d.nb_off[g], nb1 = d.nb_off[g + 1];
-    for (int base = nb0; base < nb1; base += nslot) {
-      const int jj = base + slot;
-      const bool have = active && jj < nb1;
-      const int j = have ? d.nb_idx[jj] : g;
-      // group classification (force.f:265-335), no contraction
-      double sxm, sym, szm, xcc, ycc, zcc, drj;
-      {
-#pragma clang fp contract(off)
-        double xgg = xpgi - d.xpg[j], ygg = ypgi - d.ypg[j], zgg = zpgi - d.zpg[j];
-        sxm = 2.0 * fnint(0.5 * xgg);
-        sym = 2.0 * fnint(0.5 * ygg);
-        szm = 2.0 * fnint(0.5 * zgg);
-        xgg = xgg - sxm;
-        ygg = ygg - sym;
-        zgg = zgg - szm;
-        xcc = co[0] * xgg + co[3] * ygg + co[6] * zgg;
-        ycc = co[4] * ygg + co[7] * zgg;
-        zcc = co[8] * zgg;
-        drj = xcc * xcc + ycc * ycc + zcc * zcc;
-      }
-      int cls = 0;  // 0 none, 1 normal, 2 switched
-      double sw = 1.0, dsw = 0.0;
-      if (have) {
-        if (drj >= d.r2inn0 && drj < d.r2out) {
-          cls = 1;
-        } else if (drj >= d.r2out && drj < d.r2out0) {
-          cls = 2;
-          double rsp = sqrt(drj);
-          double auxa = (d.arsout1 + 2.0 * rsp) / d.arsout2;
-          double auxb = d.rout0 - rsp;
-          sw = auxa * (auxb * auxb);
-          dsw = (-(2.0 * auxa * auxb) + 2.0 * (auxb * auxb) / d.arsout2) / rsp;
-        } else if (drj >= d.r2inn && drj < d.r2inn0) {
-          cls = 2;
-          double rsp = sqrt(drj);
-          double auxa = (d.arsinn1 + 2.0 * rsp) / d.arsinn2;
-          double auxb = d.rinn0 - rsp;
-          sw = 1.0 - auxa * (auxb * auxb);
-          dsw = (2.0 * auxa * auxb - 2.0 * (auxb * auxb) / d.arsinn2) / rsp;
-        }
-      }
-      const bool fwd = j > g;
-      double cm = 0.0;  // this lane's part of V for a switched group pair
-      int jb0 = 0, jb1 = -1;
-      if (cls != 0) {
-        jb0 = d.gfirst[j];
-        jb1 = d.glast[j];
-        const int typeij = typei + d.gss[j] - 2;  // 0..2
-        const bool mask_a = fwd && exa && has_partner_in(d.ex_off, d.ex_idx, a, jb0, jb1);
-        double ucoula = 0.0, uconfa = 0.0;
-        for (int b = jb0; b <= jb1; ++b) {
-          bool masked = fwd ? mask_a : (d.ex_inter[b] && has_partner_in(d.ex_off, d.ex_idx, b, ia0, ia0 + nI - 1));
-          if (masked) {
-            if (is_excluded(d.ex_off, d.ex_idx, a, b)) continue;
-            if (d.any_heavy && !(d.mass[b] < d.massinfty || d.mass[a] < d.massinfty)) continue;
-          }
-          double c12, c6;
-          lj_params<T>(d, ti, d.type[b], c12, c6);
-          if (d.lskip_ewald && c6 < 1e-12 && d.ewald) continue;
-          double xg, yg, zg;
-          if (cls == 1) {
-            xg = (xpi - sxm) - d.xp[b];
-            yg = (ypi - sym) - d.yp[b];
-            zg = (zpi - szm) - d.zp[b];
-          } else {
-            xg = (xpi - d.xp[b]) - sxm;
-            yg = (ypi - d.yp[b]) - sym;
-            zg = (zpi - d.zp[b]) - szm;
-          }
-          const T xc = T(co[0] * xg + co[3] * yg + co[6] * zg);
-          const T yc = T(co[4] * yg + co[7] * zg);
-          const T zc = T(co[8] * zg);
-          const T rsq = xc * xc + yc * yc + zc * zc;
-          const T rsqi = T(1) / rsq;
-          T qforce = T(0);
-          double vpair = 0.0;
-          if (c6 > 1e-12) {
-            T r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
-            qforce = (T(12.0 * c12) * r12 - T(6.0 * c6) * r6) * rsqi;
-            double conf = (double)(T(c12) * r12 - T(c6) * r6);
-            vpair += conf;
-            if (cls == 1) uconfa += conf;
-            else if (fwd) en[typeij] += sw * conf;
-          }
-          if (!d.lskip_ewald) {
-            T rsp = sqrt(rsq);
-            T rspqi = rsqi / rsp;
-            T alphar = T(d.alpha) * rsp;
-            T qt = T(1) / (T(1) + T(AS_QP) * alphar);
-            T expcst = exp(-(alphar * alphar));
-            T erfcst = ((((T(AS_A5) * qt + T(AS_A4)) * qt + T(AS_A3)) * qt + T(AS_A2)) * qt + T(AS_A1)) * qt * expcst;
-            T furpar = T(qi * d.q[b]);
-            double vel = (double)(furpar * erfcst / rsp);
-            vpair += vel;
-            if (cls == 1) ucoula += vel;
-            else if (fwd) en[3 + typeij] += sw * vel;
-            qforce += furpar * (erfcst + T(d.twrtpi) * alphar * expcst) * rspqi;
-          } else if (!d.ewald) {
-            T rsp = sqrt(rsq);
-            T ssv = T(qi * d.q[b]) / rsp;
-            vpair += (double)ssv;
-            if (fwd) en[3 + typeij] += (cls == 1 ? 1.0 : sw) * (double)ssv;
-            qforce += ssv * rsqi;
-          }
-          double qf = (double)qforce;
-          if (cls == 2) {
-            qf *= sw;
-            cm += vpair;
-          }
-          const double fxv = qf * (double)xc, fyv = qf * (double)yc, fzv = qf * (double)zc;
-          fxa += fxv;
-          fya += fyv;
-          fza += fzv;
-          if (fwd) {
-            double xxcm, yycm, zzcm;
-            if (cls == 1) {
-              xxcm = (xpca - sxm) - d.xpcm[d.atomp[b]];
-              yycm = (ypca - sym) - d.ypcm[d.atomp[b]];
-              zzcm = (zpca - szm) - d.zpcm[d.atomp[b]];
-            } else {
-              xxcm = (xpca - d.xpcm[d.atomp[b]]) - sxm;
-              yycm = (ypca - d.ypcm[d.atomp[b]]) - sym;
-              zzcm = (zpca - d.zpcm[d.atomp[b]]) - szm;
-            }
-            st[0] += fxv * xxcm; st[1] += fxv * yycm; st[2] += fxv * zzcm;
-            st[3] += fyv * xxcm; st[4] += fyv * yycm; st[5] += fyv * zzcm;
-            st[6] += fzv * xxcm; st[7] += fzv * yycm; st[8] += fzv * zzcm;
-          }
-        }
-        if (fwd && cls == 1) {
-          en[typeij] += uconfa;
-          en[3 + typeij] += ucoula;
-        }
-      }
-      // total V of the group pair over the slot's lanes (every lane shuffles)
-      double cmt = 0.0;
-      for (int t = 0; t < nI; ++t) cmt += __shfl(cm, (slot * nI + t) & 63, 64);
-      if (cls == 2) {
-        // S'(r) V spread over group atoms by mass fraction (force.f:756-817)
-        const double x3 = -(dsw * cmt * xcc), y3 = -(dsw * cmt * ycc), z3 = -(dsw * cmt * zcc);
-        fxa += ma * x3;
-        fya += ma * y3;
-        fza += ma * z3;
-        if (fwd) {
-          st[0] += ma * x3 * xpca; st[1] += ma * x3 * ypca; st[2] += ma * x3 * zpca;
-          st[3] += ma * y3 * xpca; st[4] += ma * y3 * ypca; st[5] += ma * y3 * zpca;
-          st[6] += ma * z3 * xpca; st[7] += ma * z3 * ypca; st[8] += ma * z3 * zpca;
-          if (ai == 0) {
-            for (int b = jb0; b <= jb1; ++b) {
-              const double m = d.gmass[b];
-              const int pb = d.atomp[b];
-              const double px = -((d.xpcm[pb] + sxm) * m), py = -((d.ypcm[pb] + sym) * m),
-                           pz = -((d.zpcm[pb] + szm) * m);
-              st[0] += x3 * px; st[1] += x3 * py; st[2] += x3 * pz;
-              st[3] += y3 * px; st[4] += y3 * py; st[5] += y3 * pz;
-              st[6] += z3 * px; st[7] += z3 * py; st[8] += z3 * pz;
-            }
-          }
-        }
-      }
-    }
-  }
-  // owner-atom forces: lanes ai, ai + nI, ... summed in lane order
-  facc[wid][lane][0] = fxa;
-  facc[wid][lane][1] = fya;
-  facc[wid][lane][2] = fza;
-  __syncthreads();
-  if (g < d.ngrp && lane < nI) {
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int s = 0; s < nslot; ++s) {
-      sx += facc[wid][s * nI + lane][0];
-      sy += facc[wid][s * nI + lane][1];
-      sz += facc[wid][s * nI + lane][2];
-    }
-    d.fx[ia0 + lane] = sx;
-    d.fy[ia0 + lane] = sy;
-    d.fz[ia0 + lane] = sz;
-  }
-  // energies and virial: wave then block, fixed order
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    double v = wave_sum(en[k]);
-    if (lane == 0) red[wid][k] = v;
-  }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    double v = wave_sum(st[k]);
-    if (lane == 0) red[wid][6 + k] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < 15) {
-    double s = 0.0;
-    for (int w = 0; w < PAIR_WAVES; ++w) s += red[w][threadIdx.x];
-    d.part[(size_t)blockIdx.x * 16 + threadIdx.x] = s;
-  }
-}
-
-// out[sys*16]: uconf[3], ucoul[3] (slt, slt-slv, slv order of typeij), st[9] row-major;
-// block k reduces component k over the per-block partials in fixed order
-__global__ void __launch_bounds__(256) k_pairs_finalize(const PairDesc *__restrict__ D, double *out) {
-  __shared__ double red[4];
-  const PairDesc &d = D[blockIdx.y];
-  const int k = blockIdx.x;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < d.nblk; b += blockDim.x) s += d.part[(size_t)b * 16 + k];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-    out[(size_t)blockIdx.y * 16 + k] = t;
-  }
-}
-
-template <typename T>
-void launch_pairs(const PairDesc *d_desc, const PairDesc &h, int nsys, int max_group, hipStream_t st) {
-  if (max_group > 64) throw Error(E_UNSUPPORTED, "groups larger than 64 atoms are not supported by the pair kernel");
-  dim3 grd(h.nblk, nsys), blk(64 * PAIR_WAVES);
-  hipLaunchKernelGGL(k_pairs<T>, grd, blk, 0, st, d_desc);
-}
-
-void launch_pairs_finalize(const PairDesc *d_desc, int nsys, double *d_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_pairs_finalize, dim3(15, nsys), dim3(256), 0, st, d_desc, d_out);
-}
-
-template void launch_pairs<double>(const PairDesc *, const PairDesc &, int, int, hipStream_t);
-template void launch_pairs<float>(const PairDesc *, const PairDesc &, int, int, hipStream_t);
 
 // ---------------------------------------------------------------- ferrf / fnbgrp
 

@@ -186,6 +186,150 @@ __global__ void __launch_bounds__(256) k_spread(const PmeDesc *__restrict__ D) {
   }
 }
 
+// ---------------------------------------------------------------- tiled spread
+//
+// Atoms are binned by the tile of their spline base point b = int(fr).  The
+// tile (T1,T2,T3) of st1 x st2 x st3 grid points receives contributions from
+// atoms with b in [t0, t0 + st + order - 2] per axis, i.e. from bins T and
+// T + 1 (mod nb) when st >= order - 1 and st divides K.  One workgroup per
+// tile accumulates those contributions in LDS with int64 atomics and stores
+// its points: every grid point has one owner, no global atomics, no clearing.
+
+__global__ void __launch_bounds__(256) k_bin_zero(const PmeDesc *__restrict__ D) {
+  const PmeDesc &d = D[blockIdx.y];
+  int nb = d.nb1 * d.nb2 * d.nb3;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) d.bin_cnt[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_bin_count(const PmeDesc *__restrict__ D) {
+  const PmeDesc &d = D[blockIdx.y];
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= d.natoms) return;
+  double x, y, z;
+  cartesian(d.co, d.xp[a], d.yp[a], d.zp[a], x, y, z);
+  double fr1 = frac_coord(x, y, z, d.recip, 0, d.K1);
+  double fr2 = frac_coord(x, y, z, d.recip, 1, d.K2);
+  double fr3 = frac_coord(x, y, z, d.recip, 2, d.K3);
+  if (!(fr1 >= 0.0 && fr1 <= d.K1 && fr2 >= 0.0 && fr2 <= d.K2 && fr3 >= 0.0 && fr3 <= d.K3)) {
+    d.atom_bin[a] = make_int2(-1, 0);  // non-finite coordinates are not spread
+    return;
+  }
+  // fr = K (w - anint(w) = 0.5 exactly) is the grid origin: same spline weights
+  if (fr1 >= d.K1) fr1 -= d.K1;
+  if (fr2 >= d.K2) fr2 -= d.K2;
+  if (fr3 >= d.K3) fr3 -= d.K3;
+  d.afr[a] = make_double4(fr1, fr2, fr3, d.q[a]);
+  int b1 = (int)fr1, b2 = (int)fr2, b3 = (int)fr3;
+  int bin = b1 / d.st1 + d.nb1 * (b2 / d.st2 + d.nb2 * (b3 / d.st3));
+  int pos = atomicAdd(&d.bin_cnt[bin], 1);
+  d.atom_bin[a] = make_int2(bin, pos);
+}
+
+// one block per system: exclusive prefix of the bin counts, fixed order
+__global__ void __launch_bounds__(1024) k_bin_scan(const PmeDesc *__restrict__ D) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const PmeDesc &d = D[blockIdx.x];
+  const int n = d.nb1 * d.nb2 * d.nb3;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int base = 0; base < n; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < n ? d.bin_cnt[i] : 0;
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0;
+      for (int w = 0; w < 16; ++w) {
+        int t = wsum[w];
+        wsum[w] = s;
+        s += t;
+      }
+    }
+    __syncthreads();
+    int excl = carry + wsum[wid] + x - v;
+    if (i < n) d.bin_start[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = excl + v;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) k_bin_fill(const PmeDesc *__restrict__ D) {
+  const PmeDesc &d = D[blockIdx.y];
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= d.natoms) return;
+  int2 bp = d.atom_bin[a];
+  if (bp.x < 0) return;
+  d.sfr[d.bin_start[bp.x] + bp.y] = d.afr[a];
+}
+
+constexpr int SPREAD_TILE_MAX = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_spread_tile(const PmeDesc *__restrict__ D) {
+  __shared__ unsigned long long tile[SPREAD_TILE_MAX * SPREAD_TILE_MAX * SPREAD_TILE_MAX];
+  const PmeDesc &d = D[blockIdx.y];
+  const int order = d.order;
+  const int T1 = blockIdx.x % d.nb1, T2 = (blockIdx.x / d.nb1) % d.nb2, T3 = blockIdx.x / (d.nb1 * d.nb2);
+  const int st1 = d.st1, st2 = d.st2, st3 = d.st3;
+  const int t01 = T1 * st1, t02 = T2 * st2, t03 = T3 * st3;
+  const int npts = st1 * st2 * st3;
+  for (int i = threadIdx.x; i < npts; i += blockDim.x) tile[i] = 0ull;
+  __syncthreads();
+  const double scale = d.fixscale;
+  for (int sb = 0; sb < 8; ++sb) {
+    const int B1 = (T1 + (sb & 1)) % d.nb1, B2 = (T2 + ((sb >> 1) & 1)) % d.nb2, B3 = (T3 + (sb >> 2)) % d.nb3;
+    // with a single bin along an axis, the +1 neighbour is the same bin: visit it once
+    if (((sb & 1) && d.nb1 == 1) || (((sb >> 1) & 1) && d.nb2 == 1) || ((sb >> 2) && d.nb3 == 1)) continue;
+    const int bin = B1 + d.nb1 * (B2 + d.nb2 * B3);
+    const int s = d.bin_start[bin], n = d.bin_cnt[bin];
+    for (int w = threadIdx.x; w < n * order; w += blockDim.x) {
+      const int at = w / order, i3 = w - at * order;
+      const double4 f = d.sfr[s + at];
+      const int b1 = (int)f.x, b2 = (int)f.y, b3 = (int)f.z;
+      // tile-local z of this stencil plane
+      const int kk = b3 - order + i3 + 1;
+      const int lz = ((kk - t03) % d.K3 + d.K3) % d.K3;
+      if (lz >= st3) continue;
+      T th1[MAX_ORDER], th2[MAX_ORDER], th3[MAX_ORDER], dd[MAX_ORDER];
+      fill_bspline<T>(T(f.x - b1), order, th1, dd);
+      fill_bspline<T>(T(f.y - b2), order, th2, dd);
+      fill_bspline<T>(T(f.z - b3), order, th3, dd);
+      const T q3 = th3[i3] * T(f.w);
+      for (int i2 = 0; i2 < order; ++i2) {
+        const int ly = ((b2 - order + i2 + 1 - t02) % d.K2 + d.K2) % d.K2;
+        if (ly >= st2) continue;
+        const T prod = th2[i2] * q3;
+        for (int i1 = 0; i1 < order; ++i1) {
+          const int lx = ((b1 - order + i1 + 1 - t01) % d.K1 + d.K1) % d.K1;
+          if (lx >= st1) continue;
+          atomicAdd(&tile[lx + st1 * (ly + st2 * lz)], to_fix((double)(th1[i1] * prod), scale));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < npts; i += blockDim.x) {
+    const int lx = i % st1, ly = (i / st1) % st2, lz = i / (st1 * st2);
+    d.Qfix[(size_t)(t01 + lx) + (size_t)d.K1 * ((t02 + ly) + (size_t)d.K2 * (t03 + lz))] = tile[i];
+  }
+}
+
+// tile edge along one axis: a divisor of K in [order - 1, 8], largest first; 0 if none
+int spread_tile_edge(int K, int order) {
+  for (int s = SPREAD_TILE_MAX; s >= 4 && s >= order - 1; --s)
+    if (K % s == 0) return s;
+  return 0;
+}
+
 // ---------------------------------------------------------------- FFT passes
 
 // Rows along K1: fixed-point grid -> half spectrum.  TP rows per block.
@@ -380,12 +524,16 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(const PmeDesc *__restrict__ D
 
 // ---------------------------------------------------------------- gather
 
+// Eight lanes per atom: lane i3 < order sums the z-plane i3 of the atom's
+// order^3 stencil; the partial f1, f2, f3 are reduced over the 8-lane group
+// with xor-shuffles (fixed order).  32 atoms per 256-thread block.
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather(const PmeDesc *__restrict__ D) {
   __shared__ double red[8][9];
   const PmeDesc &d = D[blockIdx.y];
   const int order = d.order;
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sub = threadIdx.x & 7;
+  int a = blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3);
   double stc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double xp = 0, yp = 0, zp = 0, fr1 = -1, fr2 = -1, fr3 = -1;
   if (a < d.natoms) {
@@ -398,32 +546,44 @@ __global__ void __launch_bounds__(256) k_gather(const PmeDesc *__restrict__ D) {
     fr2 = frac_coord(x, y, z, d.recip, 1, d.K2);
     fr3 = frac_coord(x, y, z, d.recip, 2, d.K3);
   }
-  if (a < d.natoms && fr1 >= 0.0 && fr1 <= d.K1 && fr2 >= 0.0 && fr2 <= d.K2 && fr3 >= 0.0 && fr3 <= d.K3) {
-    int b1 = (int)fr1, b2 = (int)fr2, b3 = (int)fr3;
+  const bool valid =
+      a < d.natoms && fr1 >= 0.0 && fr1 <= d.K1 && fr2 >= 0.0 && fr2 <= d.K2 && fr3 >= 0.0 && fr3 <= d.K3;
+  double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+  int b1 = 0, b2 = 0, b3 = 0;
+  if (valid && sub < order) {
+    b1 = (int)fr1;
+    b2 = (int)fr2;
+    b3 = (int)fr3;
     T th1[MAX_ORDER], th2[MAX_ORDER], th3[MAX_ORDER], dt1[MAX_ORDER], dt2[MAX_ORDER], dt3[MAX_ORDER];
     fill_bspline<T>(T(fr1 - b1), order, th1, dt1);
     fill_bspline<T>(T(fr2 - b2), order, th2, dt2);
     fill_bspline<T>(T(fr3 - b3), order, th3, dt3);
     const T *Qr = reinterpret_cast<const T *>(d.Qr);
-    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
-    const double q = d.q[a];
-    for (int i3 = 0; i3 < order; ++i3) {
-      int kk = wrapK(b3 - order + i3 + 1, d.K3);
-      for (int i2 = 0; i2 < order; ++i2) {
-        int jj = wrapK(b2 - order + i2 + 1, d.K2);
-        const T *row = Qr + (size_t)d.K1 * (jj + (size_t)d.K2 * kk);
-        T s1 = T(0), s23 = T(0);
-        for (int i1 = 0; i1 < order; ++i1) {
-          int ii = wrapK(b1 - order + i1 + 1, d.K1);
-          T g = row[ii];
-          s1 += dt1[i1] * g;
-          s23 += th1[i1] * g;
-        }
-        f1 -= (double)(s1 * th2[i2] * th3[i3]);
-        f2 -= (double)(s23 * dt2[i2] * th3[i3]);
-        f3 -= (double)(s23 * th2[i2] * dt3[i3]);
+    const int i3 = sub;
+    int kk = wrapK(b3 - order + i3 + 1, d.K3);
+    for (int i2 = 0; i2 < order; ++i2) {
+      int jj = wrapK(b2 - order + i2 + 1, d.K2);
+      const T *row = Qr + (size_t)d.K1 * (jj + (size_t)d.K2 * kk);
+      T s1 = T(0), s23 = T(0);
+      for (int i1 = 0; i1 < order; ++i1) {
+        int ii = wrapK(b1 - order + i1 + 1, d.K1);
+        T g = row[ii];
+        s1 += dt1[i1] * g;
+        s23 += th1[i1] * g;
       }
+      f1 -= (double)(s1 * th2[i2] * th3[i3]);
+      f2 -= (double)(s23 * dt2[i2] * th3[i3]);
+      f3 -= (double)(s23 * th2[i2] * dt3[i3]);
     }
+  }
+#pragma unroll
+  for (int off = 4; off > 0; off >>= 1) {
+    f1 += __shfl_xor(f1, off, 64);
+    f2 += __shfl_xor(f2, off, 64);
+    f3 += __shfl_xor(f3, off, 64);
+  }
+  if (valid && sub == 0) {
+    const double q = d.q[a];
     f1 *= d.K1 * q;
     f2 *= d.K2 * q;
     f3 *= d.K3 * q;
@@ -512,7 +672,16 @@ void launch_pme(const PmeDesc *d_desc, const PmeDesc &h, int nsys, double *d_out
   const int order = h.order;
   const int H = h.K1 / 2 + 1;
   dim3 blk(256);
-  {
+  if (h.st1 > 0) {
+    const int nbin = h.nb1 * h.nb2 * h.nb3;
+    dim3 ga((h.natoms + 255) / 256, nsys);
+    hipLaunchKernelGGL(k_bin_zero, dim3((nbin + 255) / 256, nsys), blk, 0, st, d_desc);
+    hipLaunchKernelGGL(k_bin_count, ga, blk, 0, st, d_desc);
+    hipLaunchKernelGGL(k_bin_scan, dim3(nsys), dim3(1024), 0, st, d_desc);
+    hipLaunchKernelGGL(k_bin_fill, ga, blk, 0, st, d_desc);
+    hipLaunchKernelGGL(k_spread_tile<T>, dim3(nbin, nsys), blk, 0, st, d_desc);
+  } else {
+    launch_zero_u64(h.Qfix, (size_t)h.K1 * h.K2 * h.K3, st);  // single system: the fallback is not batched
     long work = (long)h.natoms * order;
     dim3 grd((unsigned)((work + 255) / 256), nsys);
     hipLaunchKernelGGL(k_spread<T>, grd, blk, 0, st, d_desc);

@@ -19,7 +19,7 @@ CSRC = os.path.join(ROOT, "csrc")
 LIBDIR = os.path.join(ROOT, "lib")
 LIB = os.path.join(LIBDIR, "liborac_hip.so")
 FLIB = os.path.join(LIBDIR, "liborac_hip_fortran.so")
-SOURCES = ["pme.hip", "pairs.hip", "api.cpp"]
+SOURCES = ["pme.hip", "pairs.hip", "force_pairs.hip", "api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ORAC_HIP_ARCH", "gfx950")
 

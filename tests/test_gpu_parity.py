@@ -87,9 +87,12 @@ def test_fft_pme(gpu_ctx, sysname, request):
     assert _stress_close(g["stress"], o["stress"])
 
 
-def test_fft_pme_noncubic_grid(gpu_ctx, small_solute):
+@pytest.mark.parametrize("nfft,order", [((30, 24, 20), 4), ((27, 24, 20), 4), ((24, 30, 25), 6)])
+def test_fft_pme_noncubic_grid(gpu_ctx, small_solute, nfft, order):
+    """Tiled spread (every axis has a tile edge dividing K) and the atomic
+    fallback (K = 27 has none in [order-1, 8])."""
     import dataclasses
-    s = dataclasses.replace(small_solute, nfft=(30, 24, 20), order=4)
+    s = dataclasses.replace(small_solute, nfft=nfft, order=order)
     g = _pme_gpu(gpu_ctx, s)
     o = O.fft_pme(s, variant="fftw")
     assert rel(g["eer"], o["eer"]) < E_TOL
