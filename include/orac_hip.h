@@ -169,13 +169,78 @@ int orac_hip_system_eval(orac_hip_ctx *ctx, int32_t sys, int32_t rebuild_list);
 int orac_hip_system_results(orac_hip_ctx *ctx, int32_t sys, double *fx, double *fy, double *fz, double *energies);
 /* Stage timing: when enabled, eval records HIP events between its stages;
  * stage_ms returns, for the last eval, the milliseconds of
- * [0] list build, [1] centres, [2] direct-space pairs, [3] PME
- * (spread, FFT, convolution, gather), [4] corrections. */
+ * [0] group and molecule centres, [1] list build, [2] direct-space pairs,
+ * [3] PME (spread, FFT, convolution, gather), [4] corrections, and
+ * [5] the pair kernel alone (within [2]). */
 int orac_hip_system_profile(orac_hip_ctx *ctx, int32_t sys, int32_t enable);
 int orac_hip_system_stage_ms(orac_hip_ctx *ctx, int32_t sys, float *ms, int32_t n);
+/* Device bytes held by a system (for admission against an HBM budget). */
+int orac_hip_system_bytes(orac_hip_ctx *ctx, int32_t sys, int64_t *bytes);
 int orac_hip_sync(orac_hip_ctx *ctx);
 /* Stream handle of the context, as a void* (a hipStream_t). */
 int orac_hip_stream(orac_hip_ctx *ctx, void **stream);
+
+/* ---- batches (no reference counterpart) ------------------------------------ */
+/* An ordered set of systems (replicas, windows, requests; sizes may differ)
+ * evaluated together: each stage is one kernel launch for the whole batch.
+ * All systems of a batch have the same precision.  A system's forces are the
+ * same alone and batched; energies may differ in their last bits when the
+ * batch's FFT tiling differs from the system's own.  Destroying a system
+ * destroys the batches that hold it. */
+int orac_hip_batch_create(orac_hip_ctx *ctx, int32_t nsys, const int32_t *systems, int32_t *batch);
+int orac_hip_batch_destroy(orac_hip_ctx *ctx, int32_t batch);
+/* One force evaluation of every system.  use_graph != 0 replays a HIP graph
+ * of the step, captured on first use (one graph with and one without the
+ * list build).  Asynchronous; results per system via system_results. */
+int orac_hip_batch_eval(orac_hip_ctx *ctx, int32_t batch, int32_t rebuild_list, int32_t use_graph);
+int orac_hip_batch_profile(orac_hip_ctx *ctx, int32_t batch, int32_t enable);
+int orac_hip_batch_stage_ms(orac_hip_ctx *ctx, int32_t batch, float *ms, int32_t n);
+/* Tuning vector: [0] LDS elements per FFT buffer (power of two, 64..2048),
+ * [1] pair-kernel occupancy target in waves/SIMD (0 default, 2, 3, 4),
+ * [2] LDS-tiled spread (1) or global atomics (0), [3] XCD-grouped pair
+ * blocks (1) or hardware order (0). */
+int orac_hip_batch_set_tuning(orac_hip_ctx *ctx, int32_t batch, const int32_t *tuning, int32_t n);
+int orac_hip_batch_get_tuning(orac_hip_ctx *ctx, int32_t batch, int32_t *tuning, int32_t n);
+/* Times the batch's step for the tuning candidates (reps evaluations each)
+ * and keeps the fastest; best_ms (may be null) receives its time. */
+int orac_hip_batch_autotune(orac_hip_ctx *ctx, int32_t batch, int32_t reps, int32_t rebuild_list, double *best_ms);
+
+/* ---- dynamics (no reference counterpart: the caller side of the path) ------ */
+/* On-device r-RESPA NVE integration in the scheme of ORAC's multiple-time-step
+ * integrator (mtsmd.f): a fast level of harmonic distance terms
+ * E = k (r - r0)^2 (terms (2, nterm), 1-based; k in energy / A^2) integrated
+ * with ninner substeps, and the full nonbonded force of the system once per
+ * outer step.  Rigid units are 3 consecutive atoms (rigid_first, 1-based)
+ * whose distances (0,1), (0,2), (1,2) are kept at rigid_len (3, nrigid) by
+ * SHAKE after every drift and RATTLE after every kick (tolerance 1e-12).
+ * Units: positions A (Cartesian), time fs, velocities A/fs, masses amu,
+ * energies in the system's internal units (amu A^2 / fs^2).  Setup takes the
+ * positions from the system's reduced coordinates and zeroes velocities. */
+int orac_hip_md_setup(orac_hip_ctx *ctx, int32_t sys, int32_t nrigid, const int32_t *rigid_first,
+                      const double *rigid_len, int32_t nterm, const int32_t *terms, const double *term_k,
+                      const double *term_r0);
+/* Cartesian positions and/or velocities (either triple may be null);
+ * velocities are projected onto the rigid-unit constraints. */
+int orac_hip_md_set_state(orac_hip_ctx *ctx, int32_t sys, const double *x, const double *y, const double *z,
+                          const double *vx, const double *vy, const double *vz);
+int orac_hip_md_get_state(orac_hip_ctx *ctx, int32_t sys, double *x, double *y, double *z, double *vx, double *vy,
+                          double *vz);
+/* nsteps outer steps of dt fs for every system of the batch; the group
+ * list is rebuilt every list_every steps; use_graph replays a captured HIP
+ * graph of one outer step.  Asynchronous. */
+int orac_hip_batch_md_run(orac_hip_ctx *ctx, int32_t batch, int32_t nsteps, double dt, int32_t ninner,
+                          int32_t list_every, int32_t use_graph);
+/* niter steepest-descent steps: every unit moves along its total force by
+ * hstep F/m, at most dmax A per atom, rigid units re-constrained; velocities
+ * are zeroed. */
+int orac_hip_batch_md_relax(orac_hip_ctx *ctx, int32_t batch, int32_t niter, double hstep, double dmax,
+                            int32_t list_every);
+/* v *= scale[k] for system k of the batch (velocity rescaling). */
+int orac_hip_batch_md_scale_velocities(orac_hip_ctx *ctx, int32_t batch, const double *scale);
+/* out[5]: kinetic energy, harmonic-term energy, total momentum (x, y, z) at
+ * the current state; *fail (may be null): constraint solves that did not
+ * converge since setup.  The nonbonded energies come from system_results. */
+int orac_hip_system_md_energies(orac_hip_ctx *ctx, int32_t sys, double *out, int32_t *fail);
 
 #ifdef __cplusplus
 }

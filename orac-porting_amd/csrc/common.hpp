@@ -66,8 +66,10 @@ struct PmeDesc {
   FftAxis ax1, ax2, ax3;
   double *fx, *fy, *fz;         // PME forces
   int accumulate;               // 0: fx = f_pme, 1: fx += f_pme
-  double *conv_part;            // per-block energy/virial partials [nblk][8]
-  int conv_nblk;
+  int tpx, tpy, tpz;            // pencils per FFT workgroup along K1, K2, K3
+  int gx, gy;                   // workgroups of the K1 and K2 passes
+  double *conv_part;            // per-block energy/virial partials [conv_nblk][8]
+  int conv_nblk;                // workgroups of the K3 pass
   // molecular-virial correction (fft_pme.f:65-106)
   const int *atomp;             // 0-based molecule of each atom
   const double *xpcm, *ypcm, *zpcm;
@@ -79,6 +81,8 @@ struct PmeDesc {
   int2 *atom_bin;               // [natoms] {bin, position in bin}
   double4 *afr;                 // [natoms] {fr1, fr2, fr3, q}
   double4 *sfr;                 // [natoms] the same, sorted by bin
+  int tiled;                    // 1: LDS-tiled spread, 0: global-atomic spread
+  double *out;                  // [20]: eer, vir (3x3 col-major), stc1..stc9
 };
 
 // ---------------------------------------------------------------- helpers
@@ -119,6 +123,54 @@ __device__ inline T wave_sum(T v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// Exclusive prefix sum of in[0..n) into out by one 1024-thread workgroup, in
+// fixed order; returns the total to every thread.
+__device__ inline int block_exclusive_scan(const int *in, int *out, int n) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int v = i < n ? in[i] : 0;
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0;
+      for (int w = 0; w < 16; ++w) {
+        int t = wsum[w];
+        wsum[w] = s;
+        s += t;
+      }
+    }
+    __syncthreads();
+    const int excl = carry + wsum[wid] + x - v;
+    if (i < n) out[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = excl + v;
+    __syncthreads();
+  }
+  const int total = carry;
+  __syncthreads();  // every thread has read carry before a later call resets it
+  return total;
+}
+
+// Workgroups are dealt to the 8 XCDs round robin (hardware id h runs on XCD
+// h % 8).  Mapping h to logical block (h % 8) * (n / 8) + h / 8 gives each XCD
+// one contiguous range of logical blocks, so a system's blocks share an L2.
+__device__ inline int xcd_remap(int h, int n) {
+  const int q = n >> 3;
+  if (h >= (q << 3)) return h;
+  return (h & 7) * q + (h >> 3);
 }
 
 }  // namespace orac

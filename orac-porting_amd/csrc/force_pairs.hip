@@ -15,6 +15,9 @@
 // with F_a the total direct force on atom a.  The kernel accumulates that
 // form: the shift terms are nonzero only for pairs across the cell boundary,
 // and the per-atom term is added once per atom after the force reduction.
+#include <algorithm>
+#include <cstdlib>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -64,12 +67,19 @@ __device__ inline void lj_params(const PairDesc &d, int ti, int tj, double &c12,
 
 // W = minimum waves per SIMD the register allocation must allow (occupancy target)
 template <typename T, int W>
-__global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D) {
+__global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D, int remap) {
   __shared__ double red[PAIR_WAVES][16];
   __shared__ double facc[PAIR_WAVES][64][3];
-  const PairDesc &d = D[blockIdx.y];
+  int sys = blockIdx.y, blk = blockIdx.x;
+  if (remap) {
+    const int l = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    sys = l / gridDim.x;
+    blk = l - sys * gridDim.x;
+  }
+  const PairDesc &d = D[sys];
+  if (blk >= d.nblk) return;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int g = blockIdx.x * PAIR_WAVES + wid;
+  const int g = blk * PAIR_WAVES + wid;
   double fxa = 0.0, fya = 0.0, fza = 0.0;
   double en[6] = {0, 0, 0, 0, 0, 0};  // uconf[typeij], ucoul[typeij]
   double st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -272,13 +282,13 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
   if (threadIdx.x < 15) {
     double s = 0.0;
     for (int w = 0; w < PAIR_WAVES; ++w) s += red[w][threadIdx.x];
-    d.part[(size_t)blockIdx.x * 16 + threadIdx.x] = s;
+    d.part[(size_t)blk * 16 + threadIdx.x] = s;
   }
 }
 
-// out[sys*16]: uconf[3], ucoul[3] (slt, slt-slv, slv order of typeij), st[9] row-major;
+// d.out[15]: uconf[3], ucoul[3] (slt, slt-slv, slv order of typeij), st[9] row-major;
 // block k reduces component k over the per-block partials in fixed order
-__global__ void __launch_bounds__(256) k_pairs_finalize(const PairDesc *__restrict__ D, double *out) {
+__global__ void __launch_bounds__(256) k_pairs_finalize(const PairDesc *__restrict__ D) {
   __shared__ double red[4];
   const PairDesc &d = D[blockIdx.y];
   const int k = blockIdx.x;
@@ -290,15 +300,16 @@ __global__ void __launch_bounds__(256) k_pairs_finalize(const PairDesc *__restri
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-    out[(size_t)blockIdx.y * 16 + k] = t;
+    d.out[k] = t;
   }
 }
 
 // Occupancy target of k_pairs: measured best on MI355X for the 24.5k-atom
-// system is 3 waves/SIMD for FP64 (4 spills) and 4 for FP32.
-// ORAC_HIP_PAIR_WAVES=2|3|4 overrides it.
+// system is 3 waves/SIMD for FP64 (4 spills) and 4 for FP32.  The tuning
+// value, then ORAC_HIP_PAIR_WAVES=2|3|4, override it.
 template <typename T>
-static int pair_occupancy() {
+static int pair_occupancy(int tuned) {
+  if (tuned == 2 || tuned == 3 || tuned == 4) return tuned;
   static int w = [] {
     const char *e = getenv("ORAC_HIP_PAIR_WAVES");
     int v = e ? atoi(e) : (sizeof(T) == 8 ? 3 : 4);
@@ -308,22 +319,31 @@ static int pair_occupancy() {
 }
 
 template <typename T>
-void launch_pairs(const PairDesc *d_desc, const PairDesc &h, int nsys, int max_group, hipStream_t st) {
-  if (max_group > 64) throw Error(E_UNSUPPORTED, "groups larger than 64 atoms are not supported by the pair kernel");
-  hipLaunchKernelGGL(k_pack_atoms, dim3((h.natoms + 255) / 256, nsys), dim3(256), 0, st, d_desc);
-  dim3 grd(h.nblk, nsys), blk(64 * PAIR_WAVES);
-  switch (pair_occupancy<T>()) {
-    case 3: hipLaunchKernelGGL((k_pairs<T, 3>), grd, blk, 0, st, d_desc); break;
-    case 4: hipLaunchKernelGGL((k_pairs<T, 4>), grd, blk, 0, st, d_desc); break;
-    default: hipLaunchKernelGGL((k_pairs<T, 2>), grd, blk, 0, st, d_desc); break;
+void launch_pairs(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream_t st, const Tuning &tu,
+                  hipEvent_t kbeg, hipEvent_t kend) {
+  int max_atoms = 1, max_blk = 1;
+  for (int s = 0; s < nsys; ++s) {
+    if (h[s].max_group > 64)
+      throw Error(E_UNSUPPORTED, "groups larger than 64 atoms are not supported by the pair kernel");
+    max_atoms = std::max(max_atoms, h[s].natoms);
+    max_blk = std::max(max_blk, h[s].nblk);
   }
+  hipLaunchKernelGGL(k_pack_atoms, dim3((max_atoms + 255) / 256, nsys), dim3(256), 0, st, d_desc);
+  dim3 grd(max_blk, nsys), blk(64 * PAIR_WAVES);
+  const int remap = tu.xcd_remap ? 1 : 0;
+  if (kbeg) ORAC_HIP(hipEventRecord(kbeg, st));
+  switch (pair_occupancy<T>(tu.pair_waves)) {
+    case 3: hipLaunchKernelGGL((k_pairs<T, 3>), grd, blk, 0, st, d_desc, remap); break;
+    case 4: hipLaunchKernelGGL((k_pairs<T, 4>), grd, blk, 0, st, d_desc, remap); break;
+    default: hipLaunchKernelGGL((k_pairs<T, 2>), grd, blk, 0, st, d_desc, remap); break;
+  }
+  if (kend) ORAC_HIP(hipEventRecord(kend, st));
+  hipLaunchKernelGGL(k_pairs_finalize, dim3(15, nsys), dim3(256), 0, st, d_desc);
 }
 
-void launch_pairs_finalize(const PairDesc *d_desc, int nsys, double *d_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_pairs_finalize, dim3(15, nsys), dim3(256), 0, st, d_desc, d_out);
-}
-
-template void launch_pairs<double>(const PairDesc *, const PairDesc &, int, int, hipStream_t);
-template void launch_pairs<float>(const PairDesc *, const PairDesc &, int, int, hipStream_t);
+template void launch_pairs<double>(const PairDesc *, const PairDesc *, int, hipStream_t, const Tuning &, hipEvent_t,
+                                   hipEvent_t);
+template void launch_pairs<float>(const PairDesc *, const PairDesc *, int, hipStream_t, const Tuning &, hipEvent_t,
+                                  hipEvent_t);
 
 }  // namespace orac

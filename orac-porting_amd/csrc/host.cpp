@@ -1,0 +1,292 @@
+// host.cpp -- host-side tables and plans of the nonbonded path: FFT plans and
+// twiddles, B-spline moduli, linked-cell stencils, exclusion maps, PME descriptors.
+#include <cmath>
+#include <cstdlib>
+
+#include "host.hpp"
+
+namespace orac {
+
+thread_local std::string g_last_error;
+
+// Twiddles exp(-2 pi i m / n) in extended precision, rounded once.
+static void make_twiddles(int n, std::vector<double2> &t64, std::vector<float2> &t32) {
+  t64.resize(n);
+  t32.resize(n);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int m = 0; m < n; ++m) {
+    long double a = -two_pi * (long double)m / (long double)n;
+    double c = (double)cosl(a), s = (double)sinl(a);
+    t64[m] = make_double2(c, s);
+    t32[m] = make_float2((float)cosl(a), (float)sinl(a));
+  }
+}
+
+static bool factor_radices(int n, FftAxis &ax) {
+  ax.n = n;
+  ax.nr = 0;
+  int m = n;
+  const int order[4] = {4, 2, 3, 5};
+  for (int r : order)
+    while (m % r == 0) {
+      if (ax.nr >= MAX_RADICES) return false;
+      ax.radix[ax.nr++] = r;
+      m /= r;
+    }
+  return m == 1 && n <= MAX_FFT_LEN && n >= 2;
+}
+
+// load_bsp_moduli + DFTMOD (pme_fftw3.f90:165-230), with DFTMOD's 2*pi
+// built from the default-real (REAL(4)) pi literal.
+void bsp_moduli(int order, int nfft, double *out) {
+  double arr[MAX_ORDER + 2], darr[MAX_ORDER + 2];
+  fill_bspline<double>(0.0, order, arr, darr);
+  std::vector<double> b(nfft + order + 2, 0.0);
+  for (int i = 2; i <= order + 1; ++i) b[i - 1] = arr[i - 2];
+  const double twopi = 2.0 * (double)3.14159265358979323846f;
+  for (int k = 1; k <= nfft; ++k) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 1; j <= nfft; ++j) {
+      double arg = twopi * (k - 1) * (j - 1) / nfft;
+      s1 = s1 + b[j - 1] * std::cos(arg);
+      s2 = s2 + b[j - 1] * std::sin(arg);
+    }
+    out[k - 1] = s1 * s1 + s2 * s2;
+  }
+  for (int k = 1; k <= nfft; ++k)
+    if (out[k - 1] < 1.0e-7) out[k - 1] = 0.5 * (out[k - 2] + out[k]);
+}
+
+// dist_ijk.f: corner search and 500-step edge scan between cell (0,0,0) and (ni,nj,nk)
+static double dist_ijk(int ni, int nj, int nk, double dx, double dy, double dz, const double *co) {
+  static const int nv[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1}, {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+  const int ndtmax = 500;
+  double dmin = 1.0e8, d;
+  int imin = 0, jmin = 0;
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j) {
+      double lx = (ni + nv[j][0] - nv[i][0]) * dx, ly = (nj + nv[j][1] - nv[i][1]) * dy,
+             lz = (nk + nv[j][2] - nv[i][2]) * dz;
+      double mx = co[0] * lx + co[3] * ly + co[6] * lz;
+      double my = co[1] * lx + co[4] * ly + co[7] * lz;
+      double mz = co[2] * lx + co[5] * ly + co[8] * lz;
+      d = mx * mx + my * my + mz * mz;
+      if (d < dmin) {
+        dmin = d;
+        imin = i;
+        jmin = j;
+      }
+    }
+  double lx = (ni + nv[jmin][0] - nv[imin][0]) * dx, ly = (nj + nv[jmin][1] - nv[imin][1]) * dy,
+         lz = (nk + nv[jmin][2] - nv[imin][2]) * dz;
+  double mx = co[0] * lx + co[3] * ly + co[6] * lz;
+  double my = co[1] * lx + co[4] * ly + co[7] * lz;
+  double mz = co[2] * lx + co[5] * ly + co[8] * lz;
+  const double dc[3] = {dx, dy, dz};
+  dmin = 1.0e8;
+  for (int ax = 0; ax < 3; ++ax) {
+    double sgn = (0.5f - (float)nv[jmin][ax]) >= 0.0f ? 1.0 : -1.0;  // sign(1., 0.5-nv), default real
+    double dt = sgn * dc[ax] / ndtmax;
+    for (int i = 1; i <= ndtmax; ++i) {
+      double dmx = i * co[0 + 3 * ax] * dt, dmy = i * co[1 + 3 * ax] * dt, dmz = i * co[2 + 3 * ax] * dt;
+      d = (mx + dmx) * (mx + dmx) + (my + dmy) * (my + dmy) + (mz + dmz) * (mz + dmz);
+      if (d < dmin) dmin = d;
+    }
+  }
+  return dmin;
+}
+
+// lc_index.f:3-125; returns false where the reference STOPs
+bool lc_index(int indmax, int ncx, int ncy, int ncz, int32_t *nind, int32_t *indxi, int32_t *indxj,
+                     int32_t *indxk, double ctoff, const double *co) {
+  double sqcut = ctoff * ctoff;
+  double dx = 2.0 / ncx, dy = 2.0 / ncy, dz = 2.0 / ncz;
+  int imax = 0, jmax = 0, kmax = 0;
+  nind[0] = 1;
+  indxi[0] = indxj[0] = indxk[0] = 0;
+  nind[1] = 1;
+  indxi[1] = indxj[1] = indxk[1] = 0;
+  for (int n = 1; n <= 2; ++n) {
+    int istart = n == 1 ? 0 : 1 - ncx;
+    for (int i = istart; i <= ncx - 1; ++i) {
+      int jstart = (n == 1 && i == 0) ? 0 : 1 - ncy;
+      for (int j = jstart; j <= ncy - 1; ++j) {
+        int kstart = (n == 1 && i == 0 && j == 0) ? 0 : 1 - ncz;
+        for (int k = kstart; k <= ncz - 1; ++k) {
+          if (dist_ijk(i, j, k, dx, dy, dz, co) < sqcut) {
+            nind[n - 1] += 1;
+            if (nind[n - 1] > indmax) return false;
+            int m = nind[n - 1];
+            indxi[(n - 1) + 2 * (m - 1)] = i;
+            indxj[(n - 1) + 2 * (m - 1)] = j;
+            indxk[(n - 1) + 2 * (m - 1)] = k;
+            imax = std::max(imax, std::abs(i));
+            jmax = std::max(jmax, std::abs(j));
+            kmax = std::max(kmax, std::abs(k));
+            if (i == 0 && j == 0 && k == 0) nind[n - 1] -= 1;
+          }
+        }
+      }
+    }
+  }
+  return !(imax >= (ncx + 1) / 2 || jmax >= (ncy + 1) / 2 || kmax >= (ncz + 1) / 2);
+}
+
+double fix_scale_for(const double *q, int n) {
+  // every grid point is bounded by sum |q|; keep one spare bit below 2^63
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += std::fabs(q[i]);
+  if (s <= 0.0) s = 1.0;
+  int e = (int)std::ceil(std::log2(s));
+  return std::ldexp(1.0, 61 - e);
+}
+
+// symmetric exclusion CSR from the mapnl layout (mappa.f: per atom [m, j > i ...])
+void exclusions_from_mapnl(const int32_t *mapnl, int nato, const std::vector<int> &atomg,
+                                  std::vector<int> &off, std::vector<int> &idx, std::vector<unsigned char> &inter) {
+  std::vector<std::vector<int>> part(nato);
+  size_t na = 0;
+  for (int i = 0; i < nato; ++i) {
+    int m = mapnl[na];
+    for (int t = 1; t <= m; ++t) {
+      int j = mapnl[na + t] - 1;
+      if (j < 0 || j >= nato) throw Error(E_ARG, "mapnl holds an atom index out of range");
+      if (j == i) continue;
+      part[i].push_back(j);
+      part[j].push_back(i);
+    }
+    na += (size_t)m + 1;
+  }
+  off.assign(nato + 1, 0);
+  inter.assign(nato, 0);
+  idx.clear();
+  for (int i = 0; i < nato; ++i) {
+    auto &v = part[i];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    for (int j : v) {
+      idx.push_back(j);
+      if (atomg[j] != atomg[i]) inter[i] = 1;
+    }
+    off[i + 1] = (int)idx.size();
+  }
+  if (idx.empty()) idx.push_back(0);
+}
+
+void build_pme_plan(BufSet &B, const std::string &pfx, PmePlan &P, int k1, int k2, int k3, int order,
+                           hipStream_t st) {
+  ORAC_REQUIRE(order >= 3 && order <= MAX_ORDER, E_UNSUPPORTED, "spline order must be 3..8");
+  int K[3] = {k1, k2, k3};
+  for (int d = 0; d < 3; ++d) {
+    ORAC_REQUIRE(factor_radices(K[d], P.ax[d]), E_UNSUPPORTED,
+                 "grid dimensions must be 2^a 3^b 5^c and at most 256");
+    ORAC_REQUIRE(K[d] >= order, E_ARG, "grid dimension smaller than the spline order");
+    std::vector<double2> t64;
+    std::vector<float2> t32;
+    make_twiddles(K[d], t64, t32);
+    P.ax[d].tw64 = B.put_vec(pfx + "tw64_" + std::to_string(d), t64, st, false);
+    P.ax[d].tw32 = B.put_vec(pfx + "tw32_" + std::to_string(d), t32, st, false);
+    P.bsp[d].resize(K[d]);
+    bsp_moduli(order, K[d], P.bsp[d].data());
+  }
+  P.K1 = k1;
+  P.K2 = k2;
+  P.K3 = k3;
+  P.order = order;
+}
+
+// mapnl length by walking its per-atom records
+size_t mapnl_length(const int32_t *mapnl, int nato) {
+  size_t na = 0;
+  for (int i = 0; i < nato; ++i) {
+    ORAC_REQUIRE(mapnl[na] >= 0, E_ARG, "negative count in mapnl");
+    na += (size_t)mapnl[na] + 1;
+  }
+  return na;
+}
+
+CellGrid make_cellgrid(BufSet &B, const std::string &pfx, int ncx, int ncy, int ncz, const int32_t *nind,
+                              const int32_t *indxi, const int32_t *indxj, const int32_t *indxk, double rcut,
+                              const double *co, int half, hipStream_t st) {
+  CellGrid cg;
+  cg.ncx = ncx;
+  cg.ncy = ncy;
+  cg.ncz = ncz;
+  cg.nsten = nind[1];
+  std::vector<int> sten(3 * (size_t)cg.nsten);
+  for (int m = 0; m < cg.nsten; ++m) {
+    sten[3 * m] = indxi[1 + 2 * m];
+    sten[3 * m + 1] = indxj[1 + 2 * m];
+    sten[3 * m + 2] = indxk[1 + 2 * m];
+  }
+  cg.sten = B.put_vec(pfx + "sten", sten, st, true);
+  for (int k = 0; k < 9; ++k) cg.co[k] = co[k];
+  cg.sqcut = rcut * rcut;
+  cg.half = half;
+  return cg;
+}
+
+// Device PME descriptor of one system: grids, B-spline moduli, partial-sum
+// buffers sized for any tuning, spread bins when the grid admits tiles.
+PmeDesc pme_desc_base(BufSet &B, const std::string &pfx, const PmePlan &P, int natoms, const double *oc,
+                      double volume, double alpha, double rkcut, bool fp64, const Tuning &tu, hipStream_t st) {
+  PmeDesc d;
+  std::memset(&d, 0, sizeof d);
+  d.natoms = natoms;
+  d.K1 = P.K1;
+  d.K2 = P.K2;
+  d.K3 = P.K3;
+  d.order = P.order;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) d.recip[i + 3 * j] = 0.5 * oc[j + 3 * i];
+  d.volume = volume;
+  d.pi_f32 = (double)3.14159265358979323846f;
+  d.fac = d.pi_f32 * d.pi_f32 / (alpha * alpha);
+  d.rkcut2 = 0.25 * rkcut * rkcut / (d.pi_f32 * d.pi_f32);
+  const int H = P.K1 / 2 + 1;
+  const size_t G = (size_t)P.K1 * P.K2 * P.K3, Gc = (size_t)H * P.K2 * P.K3;
+  d.Qfix = B.get<unsigned long long>(pfx + "Qfix", G);
+  d.FQ = fp64 ? (void *)B.get<double2>(pfx + "FQ", Gc) : (void *)B.get<float2>(pfx + "FQ32", Gc);
+  d.Qr = fp64 ? (void *)B.get<double>(pfx + "Qr", G) : (void *)B.get<float>(pfx + "Qr32", G);
+  d.bsp1 = B.put_vec(pfx + "bsp1", P.bsp[0], st, true);
+  d.bsp2 = B.put_vec(pfx + "bsp2", P.bsp[1], st, true);
+  d.bsp3 = B.put_vec(pfx + "bsp3", P.bsp[2], st, true);
+  d.ax1 = P.ax[0];
+  d.ax2 = P.ax[1];
+  d.ax3 = P.ax[2];
+  // one K3 pencil per workgroup is the most blocks any tiling uses
+  d.conv_part = B.get<double>(pfx + "conv_part", (size_t)H * P.K2 * 8);
+  d.gather_part = B.get<double>(pfx + "gather_part", (size_t)std::max(1, (natoms + 31) / 32) * 9);
+  // LDS-tiled spread when every axis has a tile edge in [order-1, 8] dividing K
+  d.st1 = spread_tile_edge(P.K1, P.order);
+  d.st2 = spread_tile_edge(P.K2, P.order);
+  d.st3 = spread_tile_edge(P.K3, P.order);
+  if (!d.st1 || !d.st2 || !d.st3) d.st1 = d.st2 = d.st3 = 0;
+  if (d.st1) {
+    d.nb1 = P.K1 / d.st1;
+    d.nb2 = P.K2 / d.st2;
+    d.nb3 = P.K3 / d.st3;
+    const size_t nbin = (size_t)d.nb1 * d.nb2 * d.nb3;
+    d.bin_cnt = B.get<int>(pfx + "bin_cnt", nbin);
+    d.bin_start = B.get<int>(pfx + "bin_start", nbin);
+    d.atom_bin = B.get<int2>(pfx + "atom_bin", std::max(1, natoms));
+    d.afr = B.get<double4>(pfx + "afr", std::max(1, natoms));
+    d.sfr = B.get<double4>(pfx + "sfr", std::max(1, natoms));
+  }
+  pme_apply_tuning(d, tu);
+  return d;
+}
+
+void pme_apply_tuning(PmeDesc &d, const Tuning &tu) {
+  pme_set_tiles(d, tu.lds_elems);
+  d.tiled = (d.st1 > 0 && tu.spread_tiled && !std::getenv("ORAC_HIP_SPREAD_ATOMIC")) ? 1 : 0;
+}
+
+double erfc_as(double x) {
+  const double a1 = 0.2548296, a2 = -0.28449674, a3 = 1.4214137, a4 = -1.453152, a5 = 1.0614054, qp = 0.3275911;
+  double qt = 1.0 / (1.0 + qp * x);
+  double expcst = std::exp(-(x * x));
+  return ((((a5 * qt + a4) * qt + a3) * qt + a2) * qt + a1) * qt * expcst;
+}
+
+}  // namespace orac

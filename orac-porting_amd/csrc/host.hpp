@@ -1,0 +1,147 @@
+// host.hpp -- host runtime shared by the plain entry points (api.cpp) and the
+// resident engine (engine.cpp): device buffers, PME plans, the context.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/orac_hip.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace orac {
+
+extern thread_local std::string g_last_error;
+
+// ---------------------------------------------------------------- device buffers
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void *ensure(size_t n) {
+    if (n == 0) n = 8;
+    if (n > bytes) {
+      if (p) ORAC_HIP(hipFree(p));
+      p = nullptr;
+      size_t cap = std::max(n, bytes + bytes / 4);
+      ORAC_HIP(hipMalloc(&p, cap));
+      bytes = cap;
+    }
+    return p;
+  }
+};
+
+// Named grow-only device buffers.  A buffer keeps its address until a larger
+// size is asked for.
+struct BufSet {
+  std::map<std::string, std::unique_ptr<DevBuf>> bufs;
+  std::map<std::string, std::vector<char>> cache;
+  size_t bytes() const {
+    size_t s = 0;
+    for (auto &kv : bufs) s += kv.second->bytes;
+    return s;
+  }
+  template <class T>
+  T *get(const std::string &name, size_t n) {
+    auto &b = bufs[name];
+    if (!b) b.reset(new DevBuf());
+    return static_cast<T *>(b->ensure(n * sizeof(T)));
+  }
+  // upload n elements; cached uploads are skipped when the bytes are unchanged
+  template <class T>
+  T *put(const std::string &name, const T *host, size_t n, hipStream_t st, bool cached) {
+    T *d = get<T>(name, n);
+    size_t nb = n * sizeof(T);
+    if (cached) {
+      auto &c = cache[name];
+      if (c.size() == nb && nb && std::memcmp(c.data(), host, nb) == 0) return d;
+      c.assign(reinterpret_cast<const char *>(host), reinterpret_cast<const char *>(host) + nb);
+    }
+    if (nb) ORAC_HIP(hipMemcpyAsync(d, host, nb, hipMemcpyHostToDevice, st));
+    return d;
+  }
+  template <class T>
+  T *put_vec(const std::string &name, const std::vector<T> &v, hipStream_t st, bool cached) {
+    return put<T>(name, v.data(), v.size(), st, cached);
+  }
+};
+
+// ---------------------------------------------------------------- tables and plans
+
+struct PmePlan {
+  int K1 = 0, K2 = 0, K3 = 0, order = 0;
+  FftAxis ax[3];
+  std::vector<double> bsp[3];
+};
+
+void bsp_moduli(int order, int nfft, double *out);
+bool lc_index(int indmax, int ncx, int ncy, int ncz, int32_t *nind, int32_t *indxi, int32_t *indxj, int32_t *indxk,
+              double ctoff, const double *co);
+double fix_scale_for(const double *q, int n);
+void exclusions_from_mapnl(const int32_t *mapnl, int nato, const std::vector<int> &atomg, std::vector<int> &off,
+                           std::vector<int> &idx, std::vector<unsigned char> &inter);
+size_t mapnl_length(const int32_t *mapnl, int nato);
+void build_pme_plan(BufSet &B, const std::string &pfx, PmePlan &P, int k1, int k2, int k3, int order,
+                    hipStream_t st);
+// descriptor of one system's reciprocal sum with its grids and partial-sum buffers
+PmeDesc pme_desc_base(BufSet &B, const std::string &pfx, const PmePlan &P, int natoms, const double *oc,
+                      double volume, double alpha, double rkcut, bool fp64, const Tuning &tu, hipStream_t st);
+// apply the tuning's FFT tiling and spread path to a descriptor built by pme_desc_base
+void pme_apply_tuning(PmeDesc &d, const Tuning &tu);
+CellGrid make_cellgrid(BufSet &B, const std::string &pfx, int ncx, int ncy, int ncz, const int32_t *nind,
+                       const int32_t *indxi, const int32_t *indxj, const int32_t *indxk, double rcut,
+                       const double *co, int half, hipStream_t st);
+// the A&S 7.1.26 erfc of force.f:123-125, 443-451
+double erfc_as(double x);
+
+struct Engine;  // resident systems and batches (engine.cpp)
+void engine_destroy(Engine *e);
+
+}  // namespace orac
+
+struct orac_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  orac::BufSet B;
+  orac::PmePlan plan;
+  bool plan_ok = false;
+  orac::Tuning tuning;
+  orac::Engine *engine = nullptr;
+};
+
+namespace orac {
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return OK;
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return E_HIP;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return E_HIP;
+  }
+}
+
+inline void set_device(orac_hip_ctx *c) {
+  ORAC_REQUIRE(c != nullptr, E_ARG, "null context");
+  ORAC_HIP(hipSetDevice(c->device));
+}
+
+}  // namespace orac

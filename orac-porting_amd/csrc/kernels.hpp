@@ -1,17 +1,29 @@
 // kernels.hpp -- device descriptors and host launchers shared by the .hip files.
+//
+// Every launcher takes a device array of descriptors and the same array on
+// the host, one entry per system.  Grids are sized by the largest system of
+// the batch; blockIdx.y selects the system and each kernel bounds its block
+// range by that system's own sizes, so systems of different sizes share a
+// launch and each system's result does not depend on what it is batched with.
 #pragma once
 #include "common.hpp"
 
 namespace orac {
 
-struct PmeTiles {
+// Launch choices that do not change any result (tile counts of the FFT
+// partial sums aside, which change the energy only in its last bits).
+struct Tuning {
   int lds_elems = 512;  // complex elements per LDS buffer in the FFT kernels
+  int pair_waves = 0;   // occupancy target of the pair kernel (0: default for the precision)
+  int spread_tiled = 1; // LDS-tiled spread where the grid allows it, else global atomics
+  int xcd_remap = 1;    // keep each system's pair blocks on one XCD (L2 locality)
 };
 
+// ---------------------------------------------------------------- reciprocal space
+
+void pme_set_tiles(PmeDesc &d, int lds_elems);
 template <typename T>
-void launch_pme(const PmeDesc *d_desc, const PmeDesc &h, int nsys, double *d_out, hipStream_t st,
-                const PmeTiles &tiles);
-int conv_blocks(int K1, int K2, int K3, int lds_elems);
+void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st, const Tuning &tu);
 // tile edge of the LDS-tiled spread along an axis of K points (0: use the atomic spread)
 int spread_tile_edge(int K, int order);
 
@@ -23,12 +35,14 @@ struct PairDesc {
   const double *xp, *yp, *zp;       // reduced coordinates
   const double *q;
   const int *type;                  // LJ type of each atom
-  const double *mass, *gmass;
+  const double *mass, *gmass;       // atom mass, mass fraction within its group
   const int *atomp;                 // molecule of each atom
-  const double *xpcm, *ypcm, *zpcm; // molecule centres (reduced)
+  double *xpcm, *ypcm, *zpcm;       // molecule centres (reduced), written by the centre kernels
+  int nmol;
+  const int *mfirst, *mlast;        // atom range of each molecule
   const int *gfirst, *glast;        // atom range of each group
   const int *gss;                   // ss_index of the group's first atom (1 solute, 2 solvent)
-  const double *xpg, *ypg, *zpg;    // group centres
+  double *xpg, *ypg, *zpg;          // group centres, written by the centre kernels
   int ntypes;
   const int *type_table;            // ma x ma, 0-based packed index
   const double *ecc12, *ecc6;
@@ -47,19 +61,19 @@ struct PairDesc {
   int nblk;
   double4 *apos;                    // packed {xp, yp, zp, q} per atom (k_pack_atoms)
   int2 *ainfo;                      // packed {type, flags} per atom; flags bit 0 = ex_inter
+  double *out;                      // [15]: uconf[3], ucoul[3] by typeij, st[9] row-major
+  int max_group;                    // largest group (atoms)
 };
 
 constexpr int PAIR_WAVES = 4;       // owner groups per 256-thread block
 
+// kbeg/kend (optional) are recorded right before and after the pair kernel itself
 template <typename T>
-void launch_pairs(const PairDesc *d_desc, const PairDesc &h, int nsys, int max_group, hipStream_t st);
-void launch_pairs_finalize(const PairDesc *d_desc, int nsys, double *d_out, hipStream_t st);
+void launch_pairs(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream_t st, const Tuning &tu,
+                  hipEvent_t kbeg = nullptr, hipEvent_t kend = nullptr);
 
-// group centres (appbou.f) and molecule centres of mass
-void launch_group_centres(int ngrp, const int *gfirst, const int *glast, const double *pmass, const double *xp,
-                          const double *yp, const double *zp, double *xpg, double *ypg, double *zpg, hipStream_t st);
-void launch_mol_com(int nmol, const int *mfirst, const int *mlast, const double *mass, const double *xp,
-                    const double *yp, const double *zp, double *xpcm, double *ypcm, double *zpcm, hipStream_t st);
+// group centres (appbou.f) and molecule centres of mass of every system
+void launch_centres(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream_t st);
 
 // linked cells (lc_list.f)
 struct CellGrid {
@@ -71,19 +85,27 @@ struct CellGrid {
   int half;                  // 1: l > n only (reference lc_list); 0: every l != n
   double4 *cpos;             // [ngrp] {xpg, ypg, zpg, group} in cell order (filled by the cell build)
 };
-void launch_cell_build(int ngrp, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
-                       int *cell_of, int *cell_cnt, int *cell_start, int *cell_cursor, int *cell_items,
-                       hipStream_t st);
-void launch_list_count(int ngrp, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
-                       const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
-                       int *count, hipStream_t st);
-// mode 0: CSR 0-based (off[n], idx), mode 1: reference layout [nv, l..] 1-based at off[n]
-void launch_list_write(int ngrp, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
-                       const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
-                       const int *off, int *out, int mode, hipStream_t st);
+
+// One system's list build: cells of the group centres, then the stencil walk.
+struct ListDesc {
+  int ngrp;
+  const double *xpg, *ypg, *zpg;
+  CellGrid cg;
+  int *cell_of, *cell_cnt, *cell_start, *cell_cursor, *cell_items;
+  int *count;                // [ngrp] entries per group
+  int *off;                  // [ngrp+1] CSR offsets (mode 0) / prefix of count (mode 1)
+  int *out;                  // list entries
+  long long cap;             // capacity of out (ints, count slots included in mode 1)
+  int *total;                // [0] entries (count slots excluded), [1] overflow flag
+  int mode;                  // 0: CSR, 0-based; 1: reference layout per group [nv, l1..lnv], 1-based
+  const int *step;           // optional step counter of the system: the build runs only
+  int every;                 //   when *step % every == 0 (device-side update schedule)
+};
+enum ListPhase { LIST_COUNT = 1, LIST_WRITE = 2 };
+// LIST_COUNT: cells, per-group counts, offsets and the capacity check;
+// LIST_WRITE: the entries.  On overflow total[1] = 1 and nothing is written.
+void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int phases, hipStream_t st);
 void launch_exclusive_scan(const int *in, int *out, int n, int *total, hipStream_t st);
-// off[n] = total[0]; if total[0] > cap: total[1] = 1 and all offsets zero, so nothing is written or read
-void launch_list_guard(int n, int *off, int *total, int cap, hipStream_t st);
 
 // half list (CSR) -> full list (CSR) with deterministic order
 void launch_half_to_full(int ngrp, const int *h_off, const int *h_idx, int *incount, int *f_off, int *f_idx,
@@ -93,6 +115,8 @@ void launch_half_to_full(int ngrp, const int *h_off, const int *h_idx, int *inco
 void launch_inner_list(int ngrp, int iz, const int *h_off, const int *h_idx, const double *xpg, const double *ypg,
                        const double *zpg, const double *co, double r2neigh, int *count, const int *off, int *out,
                        int pass, hipStream_t st);
+
+// ---------------------------------------------------------------- corrections
 
 // ferrf / fnbgrp style pair lists: per pair kernel then per atom gather
 struct CorrDesc {
@@ -115,14 +139,46 @@ struct CorrDesc {
   const int *g_off, *g_pair;          // per atom gather CSR: signed pair index (+/-(p+1))
   int natoms;
   double *fx, *fy, *fz;               // accumulated (+=)
+  double *out;                        // [4]: coul slt, coul slv, lj slt, lj slv
 };
-void launch_corr(const CorrDesc &h, CorrDesc *d_desc, hipStream_t st);
-void launch_corr_energy(const CorrDesc *d_desc, double *d_out, hipStream_t st);
+void launch_corr(const CorrDesc *d_desc, const CorrDesc *h, int nsys, hipStream_t st);
+
+// ---------------------------------------------------------------- integration
+
+// One system's molecular-dynamics state.  A unit is one atom or one rigid
+// 3-atom molecule (consecutive atoms, SHAKE/RATTLE on its three distances).
+struct MdDesc {
+  int natoms, nunits, nbond;
+  double *x, *y, *z;                 // Cartesian positions
+  double *vx, *vy, *vz;              // velocities
+  double *xp, *yp, *zp;              // reduced copy read by the force path
+  const double *fx, *fy, *fz;        // slow forces (the nonbonded evaluation)
+  double *bfx, *bfy, *bfz;           // fast forces (harmonic distance terms)
+  const double *invm;
+  const int *ufirst;                 // first atom of each unit
+  const unsigned char *urigid;       // 1: rigid 3-atom unit
+  const double *ud2;                 // [nunits][3] squared lengths (0,1), (0,2), (1,2)
+  const int *bi, *bj;                // harmonic terms E = k (r - r0)^2
+  const double *bk, *br0;
+  double *bpf, *bpe;                 // per-term force vector and energy
+  const int *bg_off, *bg_term;       // per-atom gather CSR, signed term index +/-(t+1)
+  double oc[9];
+  double *out;                       // [5]: kinetic, bond energy, momentum x, y, z
+  int *fail;                         // constraint iterations that did not converge
+  int *step;                         // outer steps taken (device counter, drives the list schedule)
+};
+// flags: 1 kick by the slow forces (cslow), 2 kick by the fast forces (cfast),
+// 4 drift by hdt with SHAKE, 8 RATTLE
+void launch_md_units(const MdDesc *d, const MdDesc *h, int nsys, double cslow, double cfast, double hdt, int flags,
+                     hipStream_t st);
+void launch_md_fast_forces(const MdDesc *d, const MdDesc *h, int nsys, hipStream_t st);
+void launch_md_energy(const MdDesc *d, int nsys, hipStream_t st);
+void launch_md_tick(const MdDesc *d, int nsys, hipStream_t st);
+void launch_md_relax(const MdDesc *d, const MdDesc *h, int nsys, double hstep, double dmax, hipStream_t st);
+void launch_md_scale_v(const MdDesc *d, const MdDesc *h, int nsys, const double *scale, hipStream_t st);
 
 void launch_cself(int n, const double *q, const int *ss, double *d_out2, hipStream_t st);
 void launch_zero_u64(unsigned long long *p, size_t n, hipStream_t st);
 void launch_zero_f64(double *p, size_t n, hipStream_t st);
-void launch_add3(int n, double *fx, double *fy, double *fz, const double *ax, const double *ay, const double *az,
-                 hipStream_t st);
 
 }  // namespace orac

@@ -1,21 +1,22 @@
-// pairs.hip -- direct-space Ewald-erfc + Lennard-Jones over group lists on gfx950.
+// pairs.hip -- centres, linked-cell lists and the correction pair lists on gfx950.
 //
 //   k_group_centres / k_mol_com   appbou.f group centres, molecule centres of mass
 //   k_cell_*                      linked-cell binning of group centres (lc_list.f:93-109)
-//   k_list_count / k_list_write   stencil walk and pair test (lc_list.f:133-201), two
+//   k_list_count / k_list_offsets / k_list_write
+//                                 stencil walk and pair test (lc_list.f:133-201), two
 //                                 passes so the output layout is exact and compact
-//   k_half_to_full                transpose of a half list into a full list, each
+//   k_h2f_*                       transpose of a half list into a full list, each
 //                                 group's transposed tail sorted so the order is fixed
-//   k_pairs                       force.f:235-835: one wavefront per owner group, lanes over
-//                                 its neighbour groups; forces on owner atoms are reduced
-//                                 across the wave, so every atom's sum happens in one place
-//                                 (no atomics, deterministic); energies and the molecular
-//                                 virial are taken from the j > i direction only
+//   k_inner_list                  next-inner-shell list of force.f:259-286
 //   k_corr_*                      ferrf.f / fnbgrp.f pair lists: per-pair kernel plus a
 //                                 per-atom gather in fixed order
 //   k_cself                       cself.f self term
+// The pair force kernel itself is in force_pairs.hip.  Every batched kernel
+// takes its system from blockIdx.y (blockIdx.x for one-block-per-system kernels).
 // The list and classification arithmetic is compiled without FMA contraction so
 // the distances are the reference's bit for bit.
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -33,121 +34,122 @@ __device__ inline int fmodp(int a, int n) { return ((a % n) + n) % n; }
 
 // ---------------------------------------------------------------- centres
 
-__global__ void k_group_centres(int ngrp, const int *gfirst, const int *glast, const double *pmass, const double *xp,
-                                const double *yp, const double *zp, double *xpg, double *ypg, double *zpg) {
+// group centres (appbou.f): mass-fraction weighted sum, no contraction
+__global__ void __launch_bounds__(256) k_group_centres(const PairDesc *__restrict__ D) {
 #pragma clang fp contract(off)
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ngrp) return;
+  const PairDesc &d = D[blockIdx.y];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.ngrp) return;
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int m = gfirst[g]; m <= glast[g]; ++m) {
-    sx = sx + pmass[m] * xp[m];
-    sy = sy + pmass[m] * yp[m];
-    sz = sz + pmass[m] * zp[m];
+  for (int m = d.gfirst[g]; m <= d.glast[g]; ++m) {
+    sx = sx + d.gmass[m] * d.xp[m];
+    sy = sy + d.gmass[m] * d.yp[m];
+    sz = sz + d.gmass[m] * d.zp[m];
   }
-  xpg[g] = sx;
-  ypg[g] = sy;
-  zpg[g] = sz;
+  d.xpg[g] = sx;
+  d.ypg[g] = sy;
+  d.zpg[g] = sz;
 }
 
 // one wavefront per molecule: lanes stride over its atoms, fixed-order butterfly sum
-__global__ void __launch_bounds__(256) k_mol_com(int nmol, const int *mfirst, const int *mlast, const double *mass,
-                                                 const double *xp, const double *yp, const double *zp, double *xpcm,
-                                                 double *ypcm, double *zpcm) {
+__global__ void __launch_bounds__(256) k_mol_com(const PairDesc *__restrict__ D) {
+  const PairDesc &d = D[blockIdx.y];
   const int lane = threadIdx.x & 63;
   const int m = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (m >= nmol) return;
+  if (m >= d.nmol) return;
   double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
-  for (int a = mfirst[m] + lane; a <= mlast[m]; a += 64) {
-    sx += mass[a] * xp[a];
-    sy += mass[a] * yp[a];
-    sz += mass[a] * zp[a];
-    sm += mass[a];
+  for (int a = d.mfirst[m] + lane; a <= d.mlast[m]; a += 64) {
+    sx += d.mass[a] * d.xp[a];
+    sy += d.mass[a] * d.yp[a];
+    sz += d.mass[a] * d.zp[a];
+    sm += d.mass[a];
   }
   sx = wave_sum(sx);
   sy = wave_sum(sy);
   sz = wave_sum(sz);
   sm = wave_sum(sm);
   if (lane == 0) {
-    xpcm[m] = sx / sm;
-    ypcm[m] = sy / sm;
-    zpcm[m] = sz / sm;
+    d.xpcm[m] = sx / sm;
+    d.ypcm[m] = sy / sm;
+    d.zpcm[m] = sz / sm;
   }
 }
 
-// out[col] = sum over rows of part[row*stride + col], fixed order; one block per column
-__global__ void __launch_bounds__(256) k_reduce_cols(const double *part, int nrows, int stride, double scale,
-                                                     double *out) {
-  __shared__ double red[4];
-  const int col = blockIdx.x;
-  double s = 0.0;
-  for (int r = threadIdx.x; r < nrows; r += blockDim.x) s += part[(size_t)r * stride + col];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-    out[col] = scale * t;
+void launch_centres(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream_t st) {
+  int max_mol = 0, max_grp = 0;
+  for (int s = 0; s < nsys; ++s) {
+    max_mol = std::max(max_mol, h[s].mfirst ? h[s].nmol : 0);
+    max_grp = std::max(max_grp, h[s].ngrp);
   }
-}
-
-void launch_reduce_cols(const double *part, int nrows, int stride, int ncols, double scale, double *out,
-                        hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce_cols, dim3(ncols), dim3(256), 0, st, part, nrows, stride, scale, out);
-}
-
-void launch_group_centres(int ngrp, const int *gfirst, const int *glast, const double *pmass, const double *xp,
-                          const double *yp, const double *zp, double *xpg, double *ypg, double *zpg, hipStream_t st) {
-  if (ngrp <= 0) return;
-  hipLaunchKernelGGL(k_group_centres, dim3((ngrp + 255) / 256), dim3(256), 0, st, ngrp, gfirst, glast, pmass, xp, yp,
-                     zp, xpg, ypg, zpg);
-}
-
-void launch_mol_com(int nmol, const int *mfirst, const int *mlast, const double *mass, const double *xp,
-                    const double *yp, const double *zp, double *xpcm, double *ypcm, double *zpcm, hipStream_t st) {
-  if (nmol <= 0) return;
-  hipLaunchKernelGGL(k_mol_com, dim3((nmol + 3) / 4), dim3(256), 0, st, nmol, mfirst, mlast, mass, xp, yp, zp, xpcm,
-                     ypcm, zpcm);
+  if (max_mol > 0) hipLaunchKernelGGL(k_mol_com, dim3((max_mol + 3) / 4, nsys), dim3(256), 0, st, d_desc);
+  if (max_grp > 0) hipLaunchKernelGGL(k_group_centres, dim3((max_grp + 255) / 256, nsys), dim3(256), 0, st, d_desc);
 }
 
 // ---------------------------------------------------------------- cells
 
-// cell index along one axis (lc_list.f:94-103): floor toward -inf then wrap
+// cell index along one axis (lc_list.f:94-103): floor toward -inf then wrap.
+// A non-finite or absurdly large coordinate goes to cell 0 instead of an
+// undefined integer conversion, so no index leaves the grid.
 __device__ inline int cell_axis(double xr, double dcell, int nc) {
 #pragma clang fp contract(off)
   double x1 = xr / dcell;
+  if (!(fabs(x1) < 1.0e9)) return 0;
   int ix = (int)x1;
   double sg = (x1 - (double)ix) >= 0.0 ? 1.0 : -1.0;
   int n = (int)((double)ix + (sg - 1.0) / 2.0);
   return fmodp(n, nc);
 }
 
-__global__ void k_cell_assign(int ngrp, const double *xpg, const double *ypg, const double *zpg, CellGrid cg,
-                              int *cell_of, int *cell_cnt) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ngrp) return;
-  int nx = cell_axis(xpg[g], 2.0 / cg.ncx, cg.ncx);
-  int ny = cell_axis(ypg[g], 2.0 / cg.ncy, cg.ncy);
-  int nz = cell_axis(zpg[g], 2.0 / cg.ncz, cg.ncz);
-  int c = nz + cg.ncz * (ny + cg.ncy * nx);
-  cell_of[g] = c;
-  atomicAdd(&cell_cnt[c], 1);
+// the build of a system with a step counter runs only on its update steps
+__device__ inline bool list_due(const ListDesc &d) { return !d.step || (*d.step % d.every) == 0; }
+
+__global__ void __launch_bounds__(256) k_list_zero(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.y];
+  if (!list_due(d)) return;
+  const int ncell = d.cg.ncx * d.cg.ncy * d.cg.ncz;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += gridDim.x * blockDim.x) {
+    d.cell_cnt[i] = 0;
+    d.cell_cursor[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2) d.total[threadIdx.x] = 0;
 }
 
-__global__ void k_cell_fill(int ngrp, const int *cell_of, const int *cell_start, int *cursor, int *items) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ngrp) return;
-  int c = cell_of[g];
-  int pos = atomicAdd(&cursor[c], 1);
-  items[cell_start[c] + pos] = g;
+__global__ void __launch_bounds__(256) k_cell_assign(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.y];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.ngrp || !list_due(d)) return;
+  const CellGrid &cg = d.cg;
+  int nx = cell_axis(d.xpg[g], 2.0 / cg.ncx, cg.ncx);
+  int ny = cell_axis(d.ypg[g], 2.0 / cg.ncy, cg.ncy);
+  int nz = cell_axis(d.zpg[g], 2.0 / cg.ncz, cg.ncz);
+  int c = nz + cg.ncz * (ny + cg.ncy * nx);
+  d.cell_of[g] = c;
+  atomicAdd(&d.cell_cnt[c], 1);
+}
+
+__global__ void __launch_bounds__(1024) k_cell_scan(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.x];
+  if (!list_due(d)) return;
+  block_exclusive_scan(d.cell_cnt, d.cell_start, d.cg.ncx * d.cg.ncy * d.cg.ncz);
+}
+
+__global__ void __launch_bounds__(256) k_cell_fill(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.y];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.ngrp || !list_due(d)) return;
+  const int c = d.cell_of[g];
+  const int pos = atomicAdd(&d.cell_cursor[c], 1);
+  d.cell_items[d.cell_start[c] + pos] = g;
 }
 
 // each cell's members in descending group order: the order of the
 // reference's head-inserted chains (lc_list.f:104-108)
-__global__ void k_cell_sort(int ncell, const int *cell_start, const int *cell_cnt, int *items) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int s = cell_start[c], n = cell_cnt[c];
+__global__ void __launch_bounds__(256) k_cell_sort(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.y];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.cg.ncx * d.cg.ncy * d.cg.ncz || !list_due(d)) return;
+  int *items = d.cell_items;
+  const int s = d.cell_start[c], n = d.cell_cnt[c];
   for (int i = 1; i < n; ++i) {
     int v = items[s + i];
     int j = i - 1;
@@ -161,12 +163,12 @@ __global__ void k_cell_sort(int ncell, const int *cell_start, const int *cell_cn
 
 // centres copied into cell order, with the group index, so a stencil walk
 // reads each cell's candidates from contiguous memory
-__global__ void k_cell_pack(int ngrp, const int *items, const double *xpg, const double *ypg, const double *zpg,
-                            double4 *cpos) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ngrp) return;
-  int g = items[t];
-  cpos[t] = make_double4(xpg[g], ypg[g], zpg[g], (double)g);
+__global__ void __launch_bounds__(256) k_cell_pack(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.y];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.ngrp || !list_due(d)) return;
+  const int g = d.cell_items[t];
+  d.cg.cpos[t] = make_double4(d.xpg[g], d.ypg[g], d.zpg[g], (double)g);
 }
 
 __global__ void k_zero_i32(int *p, int n) {
@@ -176,72 +178,12 @@ __global__ void k_zero_i32(int *p, int n) {
 
 // single-block exclusive scan (fixed order); total written to *total
 __global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, int *total) {
-  __shared__ int wsum[16];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int base = 0; base < n; base += 1024) {
-    int i = base + threadIdx.x;
-    int v = i < n ? in[i] : 0;
-    int x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      int y = __shfl_up(x, off, 64);
-      if (lane >= off) x += y;
-    }
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int s = 0;
-      for (int w = 0; w < 16; ++w) {
-        int t = wsum[w];
-        wsum[w] = s;
-        s += t;
-      }
-    }
-    __syncthreads();
-    int excl = carry + wsum[wid] + x - v;
-    if (i < n) out[i] = excl;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry = excl + v;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && total) *total = carry;
+  const int t = block_exclusive_scan(in, out, n);
+  if (threadIdx.x == 0 && total) *total = t;
 }
 
 void launch_exclusive_scan(const int *in, int *out, int n, int *total, hipStream_t st) {
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, in, out, n, total);
-}
-
-__global__ void k_list_guard(int n, int *off, int *total, int cap) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool over = total[0] > cap;
-  if (i == 0) total[1] = over ? 1 : 0;
-  if (i < n && over) off[i] = 0;
-  if (i == n) off[n] = over ? 0 : total[0];
-}
-
-void launch_list_guard(int n, int *off, int *total, int cap, hipStream_t st) {
-  hipLaunchKernelGGL(k_list_guard, dim3((n + 256) / 256), dim3(256), 0, st, n, off, total, cap);
-}
-
-void launch_cell_build(int ngrp, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
-                       int *cell_of, int *cell_cnt, int *cell_start, int *cell_cursor, int *cell_items,
-                       hipStream_t st) {
-  int ncell = cg.ncx * cg.ncy * cg.ncz;
-  dim3 b(256);
-  hipLaunchKernelGGL(k_zero_i32, dim3((ncell + 255) / 256), b, 0, st, cell_cnt, ncell);
-  hipLaunchKernelGGL(k_zero_i32, dim3((ncell + 255) / 256), b, 0, st, cell_cursor, ncell);
-  if (ngrp > 0)
-    hipLaunchKernelGGL(k_cell_assign, dim3((ngrp + 255) / 256), b, 0, st, ngrp, xpg, ypg, zpg, cg, cell_of, cell_cnt);
-  launch_exclusive_scan(cell_cnt, cell_start, ncell, nullptr, st);
-  if (ngrp > 0)
-    hipLaunchKernelGGL(k_cell_fill, dim3((ngrp + 255) / 256), b, 0, st, ngrp, cell_of, cell_start, cell_cursor,
-                       cell_items);
-  hipLaunchKernelGGL(k_cell_sort, dim3((ncell + 255) / 256), b, 0, st, ncell, cell_start, cell_cnt, cell_items);
-  if (ngrp > 0)
-    hipLaunchKernelGGL(k_cell_pack, dim3((ngrp + 255) / 256), b, 0, st, ngrp, cell_items, xpg, ypg, zpg, cg.cpos);
 }
 
 // Pair test of lc_list.f:159-170 (full co, minimum image), no contraction.
@@ -279,12 +221,11 @@ __device__ inline int wave_excl_scan_int(int v, int lane) {
 // the output order is the reference's: stencil order, then descending group
 // index inside a cell.  half: only l > n (the reference list); else every l != n.
 template <bool WRITE>
-__device__ inline int list_walk_wave(int n, const double *xpg, const double *ypg, const double *zpg,
-                                     const CellGrid &cg, const int *cell_of, const int *cell_start,
-                                     const int *cell_cnt, const int *cell_items, int *out, int one_based) {
+__device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, int one_based) {
+  const CellGrid &cg = d.cg;
   const int lane = threadIdx.x & 63;
-  const double x1 = xpg[n], y1 = ypg[n], z1 = zpg[n];
-  const int c = cell_of[n];
+  const double x1 = d.xpg[n], y1 = d.ypg[n], z1 = d.zpg[n];
+  const int c = d.cell_of[n];
   const int i = c / (cg.ncy * cg.ncz), j = (c / cg.ncz) % cg.ncy, k = c % cg.ncz;
   int base = 0;
   for (int m0 = 0; m0 < cg.nsten; m0 += 64) {
@@ -295,8 +236,8 @@ __device__ inline int list_walk_wave(int n, const double *xpg, const double *ypg
       int ny = fmodp(j + cg.sten[3 * m + 1], cg.ncy);
       int nz = fmodp(k + cg.sten[3 * m + 2], cg.ncz);
       int cc = nz + cg.ncz * (ny + cg.ncy * nx);
-      s = cell_start[cc];
-      e = s + cell_cnt[cc];
+      s = d.cell_start[cc];
+      e = s + d.cell_cnt[cc];
     }
     // hits of this lane's cell as a bit mask over its (at most 64) first members
     int cnt = 0;
@@ -332,45 +273,65 @@ __device__ inline int list_walk_wave(int n, const double *xpg, const double *ypg
   return base;
 }
 
-__global__ void __launch_bounds__(256) k_list_count(int ngrp, const double *xpg, const double *ypg,
-                                                    const double *zpg, CellGrid cg, const int *cell_of,
-                                                    const int *cell_start, const int *cell_cnt,
-                                                    const int *cell_items, int *count) {
+__global__ void __launch_bounds__(256) k_list_count(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.y];
   const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (n >= ngrp) return;
-  int c = list_walk_wave<false>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, nullptr, 0);
-  if ((threadIdx.x & 63) == 0) count[n] = c;
+  if (n >= d.ngrp || !list_due(d)) return;
+  const int c = list_walk_wave<false>(n, d, nullptr, 0);
+  if ((threadIdx.x & 63) == 0) d.count[n] = c;
 }
 
-__global__ void __launch_bounds__(256) k_list_write(int ngrp, const double *xpg, const double *ypg,
-                                                    const double *zpg, CellGrid cg, const int *cell_of,
-                                                    const int *cell_start, const int *cell_cnt,
-                                                    const int *cell_items, const int *off, int *out, int mode) {
-  const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (n >= ngrp) return;
-  if (mode == 1) {
-    int o = off[n] + n;  // reference layout: one count slot per preceding group
-    int nv = list_walk_wave<true>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, out + o + 1, 1);
-    if ((threadIdx.x & 63) == 0) out[o] = nv;
-  } else {
-    list_walk_wave<true>(n, xpg, ypg, zpg, cg, cell_of, cell_start, cell_cnt, cell_items, out + off[n], 0);
+// one block per system: offsets, total and the capacity check.  On overflow
+// every offset is zero and total[1] = 1, so the write pass writes nothing and
+// a pair kernel reading the list sees empty neighbour lists.
+__global__ void __launch_bounds__(1024) k_list_offsets(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.x];
+  if (!list_due(d)) return;
+  const int n = d.ngrp;
+  const int tot = block_exclusive_scan(d.count, d.off, n);
+  const long long need = (long long)tot + (d.mode == 1 ? n : 0);
+  const bool over = need > d.cap;
+  if (over)
+    for (int i = threadIdx.x; i < n; i += blockDim.x) d.off[i] = 0;
+  if (threadIdx.x == 0) {
+    d.off[n] = over ? 0 : tot;
+    d.total[0] = tot;
+    d.total[1] = over ? 1 : 0;
   }
 }
 
-void launch_list_count(int ngrp, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
-                       const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
-                       int *count, hipStream_t st) {
-  if (ngrp <= 0) return;
-  hipLaunchKernelGGL(k_list_count, dim3((ngrp + 3) / 4), dim3(256), 0, st, ngrp, xpg, ypg, zpg, cg, cell_of,
-                     cell_start, cell_cnt, cell_items, count);
+__global__ void __launch_bounds__(256) k_list_write(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.y];
+  const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (n >= d.ngrp || d.total[1] || !list_due(d)) return;
+  if (d.mode == 1) {
+    const int o = d.off[n] + n;  // reference layout: one count slot per preceding group
+    const int nv = list_walk_wave<true>(n, d, d.out + o + 1, 1);
+    if ((threadIdx.x & 63) == 0) d.out[o] = nv;
+  } else {
+    list_walk_wave<true>(n, d, d.out + d.off[n], 0);
+  }
 }
 
-void launch_list_write(int ngrp, const double *xpg, const double *ypg, const double *zpg, const CellGrid &cg,
-                       const int *cell_of, const int *cell_start, const int *cell_cnt, const int *cell_items,
-                       const int *off, int *out, int mode, hipStream_t st) {
-  if (ngrp <= 0) return;
-  hipLaunchKernelGGL(k_list_write, dim3((ngrp + 3) / 4), dim3(256), 0, st, ngrp, xpg, ypg, zpg, cg, cell_of,
-                     cell_start, cell_cnt, cell_items, off, out, mode);
+void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int phases, hipStream_t st) {
+  int max_cell = 1, max_grp = 1;
+  for (int s = 0; s < nsys; ++s) {
+    max_cell = std::max(max_cell, h[s].cg.ncx * h[s].cg.ncy * h[s].cg.ncz);
+    max_grp = std::max(max_grp, h[s].ngrp);
+  }
+  const dim3 b(256), gg((max_grp + 255) / 256, nsys), gc((max_cell + 255) / 256, nsys),
+      gw((max_grp + 3) / 4, nsys);
+  if (phases & LIST_COUNT) {
+    hipLaunchKernelGGL(k_list_zero, dim3(std::min(64, (max_cell + 255) / 256), nsys), b, 0, st, d_desc);
+    hipLaunchKernelGGL(k_cell_assign, gg, b, 0, st, d_desc);
+    hipLaunchKernelGGL(k_cell_scan, dim3(nsys), dim3(1024), 0, st, d_desc);
+    hipLaunchKernelGGL(k_cell_fill, gg, b, 0, st, d_desc);
+    hipLaunchKernelGGL(k_cell_sort, gc, b, 0, st, d_desc);
+    hipLaunchKernelGGL(k_cell_pack, gg, b, 0, st, d_desc);
+    hipLaunchKernelGGL(k_list_count, gw, b, 0, st, d_desc);
+    hipLaunchKernelGGL(k_list_offsets, dim3(nsys), dim3(1024), 0, st, d_desc);
+  }
+  if (phases & LIST_WRITE) hipLaunchKernelGGL(k_list_write, gw, b, 0, st, d_desc);
 }
 
 // ---------------------------------------------------------------- half -> full
@@ -487,8 +448,8 @@ void launch_inner_list(int ngrp, int iz, const int *h_off, const int *h_idx, con
 
 // ---------------------------------------------------------------- ferrf / fnbgrp
 
-__global__ void k_corr_pairs(const CorrDesc *__restrict__ D) {
-  const CorrDesc &d = *D;
+__global__ void __launch_bounds__(256) k_corr_pairs(const CorrDesc *__restrict__ D) {
+  const CorrDesc &d = D[blockIdx.y];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= d.npairs) return;
   int ia = d.ia[p], ib = d.ib[p];
@@ -560,10 +521,10 @@ __global__ void k_corr_pairs(const CorrDesc *__restrict__ D) {
   pe[3] = c == 1 ? elj : 0.0;
 }
 
-__global__ void k_corr_gather(const CorrDesc *__restrict__ D) {
-  const CorrDesc &d = *D;
+__global__ void __launch_bounds__(256) k_corr_gather(const CorrDesc *__restrict__ D) {
+  const CorrDesc &d = D[blockIdx.y];
   int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= d.natoms) return;
+  if (d.iz != 1 || a >= d.natoms || d.npairs <= 0) return;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   for (int t = d.g_off[a]; t < d.g_off[a + 1]; ++t) {
     int sp = d.g_pair[t];
@@ -578,10 +539,10 @@ __global__ void k_corr_gather(const CorrDesc *__restrict__ D) {
   d.fz[a] += sz;
 }
 
-// out[4]: coul(slt), coul(slv), lj(slt), lj(slv); block k reduces column k in fixed order
-__global__ void __launch_bounds__(256) k_corr_energy(const CorrDesc *__restrict__ D, double *out) {
+// d.out[4]: coul(slt), coul(slv), lj(slt), lj(slv); block k reduces column k in fixed order
+__global__ void __launch_bounds__(256) k_corr_energy(const CorrDesc *__restrict__ D) {
   __shared__ double red[4];
-  const CorrDesc &d = *D;
+  const CorrDesc &d = D[blockIdx.y];
   const int k = blockIdx.x;
   double s = 0.0;
   for (int p = threadIdx.x; p < d.npairs; p += blockDim.x) s += d.pe[4 * (size_t)p + k];
@@ -591,19 +552,21 @@ __global__ void __launch_bounds__(256) k_corr_energy(const CorrDesc *__restrict_
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-    out[k] = t;
+    d.out[k] = t;
   }
 }
 
-void launch_corr(const CorrDesc &h, CorrDesc *d_desc, hipStream_t st) {
-  if (h.npairs > 0)
-    hipLaunchKernelGGL(k_corr_pairs, dim3((h.npairs + 255) / 256), dim3(256), 0, st, d_desc);
-  if (h.iz == 1 && h.natoms > 0 && h.npairs > 0)
-    hipLaunchKernelGGL(k_corr_gather, dim3((h.natoms + 255) / 256), dim3(256), 0, st, d_desc);
-}
-
-void launch_corr_energy(const CorrDesc *d_desc, double *d_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_corr_energy, dim3(4), dim3(256), 0, st, d_desc, d_out);
+void launch_corr(const CorrDesc *d_desc, const CorrDesc *h, int nsys, hipStream_t st) {
+  int max_pairs = 0, max_atoms = 0;
+  for (int s = 0; s < nsys; ++s) {
+    max_pairs = std::max(max_pairs, h[s].npairs);
+    if (h[s].iz == 1 && h[s].npairs > 0) max_atoms = std::max(max_atoms, h[s].natoms);
+  }
+  if (max_pairs > 0)
+    hipLaunchKernelGGL(k_corr_pairs, dim3((max_pairs + 255) / 256, nsys), dim3(256), 0, st, d_desc);
+  if (max_atoms > 0)
+    hipLaunchKernelGGL(k_corr_gather, dim3((max_atoms + 255) / 256, nsys), dim3(256), 0, st, d_desc);
+  hipLaunchKernelGGL(k_corr_energy, dim3(4, nsys), dim3(256), 0, st, d_desc);
 }
 
 // ---------------------------------------------------------------- cself
@@ -642,23 +605,10 @@ __global__ void k_zero_f64(double *p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
 }
-__global__ void k_add3(int n, double *fx, double *fy, double *fz, const double *ax, const double *ay, const double *az) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  fx[i] += ax[i];
-  fy[i] += ay[i];
-  fz[i] += az[i];
-}
-
 void launch_zero_u64(unsigned long long *p, size_t n, hipStream_t st) {
   if (n) hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
 }
 void launch_zero_f64(double *p, size_t n, hipStream_t st) {
   if (n) hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
 }
-void launch_add3(int n, double *fx, double *fy, double *fz, const double *ax, const double *ay, const double *az,
-                 hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_add3, dim3((n + 255) / 256), dim3(256), 0, st, n, fx, fy, fz, ax, ay, az);
-}
-
 }  // namespace orac

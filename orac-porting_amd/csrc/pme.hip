@@ -153,9 +153,18 @@ __device__ inline void cartesian(const double *co, double xp, double yp, double 
 }
 
 // one thread per (atom, z-offset): order^2 fixed-point atomic adds
+__global__ void __launch_bounds__(256) k_grid_zero(const PmeDesc *__restrict__ D) {
+  const PmeDesc &d = D[blockIdx.y];
+  if (d.tiled) return;
+  const size_t G = (size_t)d.K1 * d.K2 * d.K3;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < G; i += (size_t)gridDim.x * blockDim.x)
+    d.Qfix[i] = 0ull;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_spread(const PmeDesc *__restrict__ D) {
   const PmeDesc &d = D[blockIdx.y];
+  if (d.tiled) return;
   const int order = d.order;
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   int a = tid / order, i3 = tid - a * order;
@@ -172,12 +181,14 @@ __global__ void __launch_bounds__(256) k_spread(const PmeDesc *__restrict__ D) {
   fill_bspline<T>(T(fr1 - b1), order, th1, dd);
   fill_bspline<T>(T(fr2 - b2), order, th2, dd);
   fill_bspline<T>(T(fr3 - b3), order, th3, dd);
-  const T q = T(d.q[a]);
+  // same product order as the tiled spread, th1 * (th2 * (th3 * q)), so both
+  // paths put identical fixed-point values on the grid
+  const T q3 = th3[i3] * T(d.q[a]);
   const int kk = wrapK(b3 - order + i3 + 1, d.K3);
   const double scale = d.fixscale;
   for (int i2 = 0; i2 < order; ++i2) {
     int jj = wrapK(b2 - order + i2 + 1, d.K2);
-    T prod = th2[i2] * th3[i3] * q;
+    T prod = th2[i2] * q3;
     size_t row = (size_t)d.K1 * (jj + (size_t)d.K2 * kk);
     for (int i1 = 0; i1 < order; ++i1) {
       int ii = wrapK(b1 - order + i1 + 1, d.K1);
@@ -197,6 +208,7 @@ __global__ void __launch_bounds__(256) k_spread(const PmeDesc *__restrict__ D) {
 
 __global__ void __launch_bounds__(256) k_bin_zero(const PmeDesc *__restrict__ D) {
   const PmeDesc &d = D[blockIdx.y];
+  if (!d.tiled) return;
   int nb = d.nb1 * d.nb2 * d.nb3;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) d.bin_cnt[i] = 0;
 }
@@ -204,7 +216,7 @@ __global__ void __launch_bounds__(256) k_bin_zero(const PmeDesc *__restrict__ D)
 __global__ void __launch_bounds__(256) k_bin_count(const PmeDesc *__restrict__ D) {
   const PmeDesc &d = D[blockIdx.y];
   int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= d.natoms) return;
+  if (!d.tiled || a >= d.natoms) return;
   double x, y, z;
   cartesian(d.co, d.xp[a], d.yp[a], d.zp[a], x, y, z);
   double fr1 = frac_coord(x, y, z, d.recip, 0, d.K1);
@@ -227,45 +239,15 @@ __global__ void __launch_bounds__(256) k_bin_count(const PmeDesc *__restrict__ D
 
 // one block per system: exclusive prefix of the bin counts, fixed order
 __global__ void __launch_bounds__(1024) k_bin_scan(const PmeDesc *__restrict__ D) {
-  __shared__ int wsum[16];
-  __shared__ int carry;
   const PmeDesc &d = D[blockIdx.x];
-  const int n = d.nb1 * d.nb2 * d.nb3;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int base = 0; base < n; base += 1024) {
-    int i = base + threadIdx.x;
-    int v = i < n ? d.bin_cnt[i] : 0;
-    int x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      int y = __shfl_up(x, off, 64);
-      if (lane >= off) x += y;
-    }
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int s = 0;
-      for (int w = 0; w < 16; ++w) {
-        int t = wsum[w];
-        wsum[w] = s;
-        s += t;
-      }
-    }
-    __syncthreads();
-    int excl = carry + wsum[wid] + x - v;
-    if (i < n) d.bin_start[i] = excl;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry = excl + v;
-    __syncthreads();
-  }
+  if (!d.tiled) return;
+  block_exclusive_scan(d.bin_cnt, d.bin_start, d.nb1 * d.nb2 * d.nb3);
 }
 
 __global__ void __launch_bounds__(256) k_bin_fill(const PmeDesc *__restrict__ D) {
   const PmeDesc &d = D[blockIdx.y];
   int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= d.natoms) return;
+  if (!d.tiled || a >= d.natoms) return;
   int2 bp = d.atom_bin[a];
   if (bp.x < 0) return;
   d.sfr[d.bin_start[bp.x] + bp.y] = d.afr[a];
@@ -277,6 +259,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_spread_tile(const PmeDesc *__restrict__ D) {
   __shared__ unsigned long long tile[SPREAD_TILE_MAX * SPREAD_TILE_MAX * SPREAD_TILE_MAX];
   const PmeDesc &d = D[blockIdx.y];
+  if (!d.tiled || (int)blockIdx.x >= d.nb1 * d.nb2 * d.nb3) return;
   const int order = d.order;
   const int T1 = blockIdx.x % d.nb1, T2 = (blockIdx.x / d.nb1) % d.nb2, T3 = blockIdx.x / (d.nb1 * d.nb2);
   const int st1 = d.st1, st2 = d.st2, st3 = d.st3;
@@ -334,11 +317,11 @@ int spread_tile_edge(int K, int order) {
 
 // Rows along K1: fixed-point grid -> half spectrum.  TP rows per block.
 template <typename T>
-__global__ void __launch_bounds__(256) k_fft_x_r2c(const PmeDesc *__restrict__ D, int TP) {
+__global__ void __launch_bounds__(256) k_fft_x_r2c(const PmeDesc *__restrict__ D) {
   extern __shared__ __align__(16) unsigned char smem[];
   const PmeDesc &d = D[blockIdx.y];
-  const int N = d.K1, H = N / 2 + 1, rows = d.K2 * d.K3;
-  if ((int)blockIdx.x * TP >= rows) return;
+  const int N = d.K1, H = N / 2 + 1, rows = d.K2 * d.K3, TP = d.tpx;
+  if ((int)blockIdx.x >= d.gx) return;
   cx<T> *a = reinterpret_cast<cx<T> *>(smem);
   cx<T> *b = a + TP * N;
   const int row0 = blockIdx.x * TP;
@@ -363,10 +346,11 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(const PmeDesc *__restrict__ D
 // Complex pencils along a strided axis of the half grid: pencil p has base
 // (p % H) + (p / H) * S1 and element stride Se.
 template <typename T, int S>
-__global__ void __launch_bounds__(256) k_fft_pencil(const PmeDesc *__restrict__ D, int axis, int TP) {
+__global__ void __launch_bounds__(256) k_fft_pencil(const PmeDesc *__restrict__ D, int axis) {
   extern __shared__ __align__(16) unsigned char smem[];
   const PmeDesc &d = D[blockIdx.y];
-  const int H = d.K1 / 2 + 1;
+  const int H = d.K1 / 2 + 1, TP = d.tpy;
+  if ((int)blockIdx.x >= d.gy) return;
   const FftAxis &ax = axis == 2 ? d.ax2 : d.ax3;
   const int N = ax.n;
   const int npen = axis == 2 ? H * d.K3 : H * d.K2;
@@ -397,11 +381,12 @@ __global__ void __launch_bounds__(256) k_fft_pencil(const PmeDesc *__restrict__ 
 // FFTW variant, with its fac12/fac13/fac23 virial masks and the unscaled
 // m = 0 term), inverse FFT.  Per-block partials: energy, v11 v22 v33 v12 v13 v23.
 template <typename T>
-__global__ void __launch_bounds__(256) k_fft_z_conv(const PmeDesc *__restrict__ D, int TP) {
+__global__ void __launch_bounds__(256) k_fft_z_conv(const PmeDesc *__restrict__ D) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[8][8];
   const PmeDesc &d = D[blockIdx.y];
-  const int H = d.K1 / 2 + 1, N = d.K3;
+  if ((int)blockIdx.x >= d.conv_nblk) return;
+  const int H = d.K1 / 2 + 1, N = d.K3, TP = d.tpz;
   const int npen = H * d.K2;
   const size_t Se = (size_t)H * d.K2;
   const int p0 = blockIdx.x * TP;
@@ -494,11 +479,11 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(const PmeDesc *__restrict__ 
 
 // Half spectrum rows -> real rows (imaginary parts of k1 = 0 and Nyquist ignored).
 template <typename T>
-__global__ void __launch_bounds__(256) k_fft_x_c2r(const PmeDesc *__restrict__ D, int TP) {
+__global__ void __launch_bounds__(256) k_fft_x_c2r(const PmeDesc *__restrict__ D) {
   extern __shared__ __align__(16) unsigned char smem[];
   const PmeDesc &d = D[blockIdx.y];
-  const int N = d.K1, H = N / 2 + 1, rows = d.K2 * d.K3;
-  if ((int)blockIdx.x * TP >= rows) return;
+  const int N = d.K1, H = N / 2 + 1, rows = d.K2 * d.K3, TP = d.tpx;
+  if ((int)blockIdx.x >= d.gx) return;
   cx<T> *a = reinterpret_cast<cx<T> *>(smem);
   cx<T> *b = a + TP * N;
   const int row0 = blockIdx.x * TP;
@@ -531,6 +516,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_gather(const PmeDesc *__restrict__ D) {
   __shared__ double red[8][9];
   const PmeDesc &d = D[blockIdx.y];
+  if ((int)blockIdx.x >= d.gather_nblk) return;
   const int order = d.order;
   const int sub = threadIdx.x & 7;
   int a = blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3);
@@ -626,11 +612,11 @@ __global__ void __launch_bounds__(256) k_gather(const PmeDesc *__restrict__ D) {
   }
 }
 
-// out[sys*20]: eer, vir(3x3 col-major), stc(3x3 row-major as stc1..stc9).
+// d.out[20]: eer, vir(3x3 col-major), stc(3x3 row-major as stc1..stc9).
 // Block k reduces one component in fixed order: k < 7 the convolution
 // partials (scalar_sum.f:125-133 halving and symmetrisation), k >= 7 the
 // molecular-virial partials.
-__global__ void __launch_bounds__(256) k_pme_finalize(const PmeDesc *__restrict__ D, double *out) {
+__global__ void __launch_bounds__(256) k_pme_finalize(const PmeDesc *__restrict__ D) {
   __shared__ double red[4];
   const PmeDesc &d = D[blockIdx.y];
   const int k = blockIdx.x;
@@ -644,7 +630,7 @@ __global__ void __launch_bounds__(256) k_pme_finalize(const PmeDesc *__restrict_
   if (threadIdx.x != 0) return;
   s = 0.0;
   for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
-  double *o = out + (size_t)blockIdx.y * 20;
+  double *o = d.out;
   switch (k) {
     case 0: o[0] = 0.5 * s; break;
     case 1: o[1] = 0.5 * s; break;
@@ -666,53 +652,67 @@ static int tile_pencils(int n, int npen, int elems) {
   return tp;
 }
 
+// FFT tiling of one system for LDS buffers of lds_elems complex elements
+void pme_set_tiles(PmeDesc &d, int lds_elems) {
+  const int H = d.K1 / 2 + 1;
+  d.tpx = tile_pencils(d.K1, d.K2 * d.K3, lds_elems);
+  d.gx = (d.K2 * d.K3 + d.tpx - 1) / d.tpx;
+  d.tpy = tile_pencils(d.K2, H * d.K3, lds_elems);
+  d.gy = (H * d.K3 + d.tpy - 1) / d.tpy;
+  d.tpz = tile_pencils(d.K3, H * d.K2, lds_elems);
+  d.conv_nblk = (H * d.K2 + d.tpz - 1) / d.tpz;
+  d.gather_nblk = (d.natoms + 31) / 32;  // k_gather: 8 lanes per atom, 32 atoms per block
+}
+
 template <typename T>
-void launch_pme(const PmeDesc *d_desc, const PmeDesc &h, int nsys, double *d_out, hipStream_t st,
-                const PmeTiles &tiles) {
-  const int order = h.order;
-  const int H = h.K1 / 2 + 1;
+void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st, const Tuning &tu) {
+  (void)tu;
+  if (nsys <= 0) return;
+  int max_atoms = 1, max_bins = 1, any_tiled = 0, any_atomic = 0, max_gx = 1, max_gy = 1, max_gz = 1, max_gg = 1;
+  size_t max_grid = 1, shx = 0, shy = 0, shz = 0, max_work = 1;
+  for (int s = 0; s < nsys; ++s) {
+    const PmeDesc &d = h[s];
+    max_atoms = std::max(max_atoms, d.natoms);
+    max_gx = std::max(max_gx, d.gx);
+    max_gy = std::max(max_gy, d.gy);
+    max_gz = std::max(max_gz, d.conv_nblk);
+    max_gg = std::max(max_gg, d.gather_nblk);
+    shx = std::max(shx, 2 * (size_t)d.tpx * d.K1 * sizeof(cx<T>));
+    shy = std::max(shy, 2 * (size_t)d.tpy * d.K2 * sizeof(cx<T>));
+    shz = std::max(shz, 2 * (size_t)d.tpz * d.K3 * sizeof(cx<T>));
+    if (d.tiled) {
+      any_tiled = 1;
+      max_bins = std::max(max_bins, d.nb1 * d.nb2 * d.nb3);
+    } else {
+      any_atomic = 1;
+      max_grid = std::max(max_grid, (size_t)d.K1 * d.K2 * d.K3);
+      max_work = std::max(max_work, (size_t)d.natoms * d.order);
+    }
+  }
   dim3 blk(256);
-  if (h.st1 > 0) {
-    const int nbin = h.nb1 * h.nb2 * h.nb3;
-    dim3 ga((h.natoms + 255) / 256, nsys);
-    hipLaunchKernelGGL(k_bin_zero, dim3((nbin + 255) / 256, nsys), blk, 0, st, d_desc);
+  if (any_tiled) {
+    dim3 ga((max_atoms + 255) / 256, nsys);
+    hipLaunchKernelGGL(k_bin_zero, dim3((max_bins + 255) / 256, nsys), blk, 0, st, d_desc);
     hipLaunchKernelGGL(k_bin_count, ga, blk, 0, st, d_desc);
     hipLaunchKernelGGL(k_bin_scan, dim3(nsys), dim3(1024), 0, st, d_desc);
     hipLaunchKernelGGL(k_bin_fill, ga, blk, 0, st, d_desc);
-    hipLaunchKernelGGL(k_spread_tile<T>, dim3(nbin, nsys), blk, 0, st, d_desc);
-  } else {
-    launch_zero_u64(h.Qfix, (size_t)h.K1 * h.K2 * h.K3, st);  // single system: the fallback is not batched
-    long work = (long)h.natoms * order;
-    dim3 grd((unsigned)((work + 255) / 256), nsys);
-    hipLaunchKernelGGL(k_spread<T>, grd, blk, 0, st, d_desc);
+    hipLaunchKernelGGL(k_spread_tile<T>, dim3(max_bins, nsys), blk, 0, st, d_desc);
   }
-  const int E = tiles.lds_elems;
-  int tpx = tile_pencils(h.K1, h.K2 * h.K3, E);
-  size_t shx = 2 * (size_t)tpx * h.K1 * sizeof(cx<T>);
-  dim3 gx((h.K2 * h.K3 + tpx - 1) / tpx, nsys);
-  hipLaunchKernelGGL(k_fft_x_r2c<T>, gx, blk, shx, st, d_desc, tpx);
-  int tpy = tile_pencils(h.K2, H * h.K3, E);
-  size_t shy = 2 * (size_t)tpy * h.K2 * sizeof(cx<T>);
-  dim3 gy((H * h.K3 + tpy - 1) / tpy, nsys);
-  hipLaunchKernelGGL((k_fft_pencil<T, -1>), gy, blk, shy, st, d_desc, 2, tpy);
-  int tpz = tile_pencils(h.K3, H * h.K2, E);
-  size_t shz = 2 * (size_t)tpz * h.K3 * sizeof(cx<T>);
-  dim3 gz(h.conv_nblk, nsys);
-  hipLaunchKernelGGL(k_fft_z_conv<T>, gz, blk, shz, st, d_desc, tpz);
-  hipLaunchKernelGGL((k_fft_pencil<T, +1>), gy, blk, shy, st, d_desc, 2, tpy);
-  hipLaunchKernelGGL(k_fft_x_c2r<T>, gx, blk, shx, st, d_desc, tpx);
-  dim3 gg(h.gather_nblk, nsys);
-  hipLaunchKernelGGL(k_gather<T>, gg, blk, 0, st, d_desc);
-  hipLaunchKernelGGL(k_pme_finalize, dim3(16, nsys), dim3(256), 0, st, d_desc, d_out);
+  if (any_atomic) {
+    unsigned gzero = (unsigned)std::min<size_t>((max_grid + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_grid_zero, dim3(gzero, nsys), blk, 0, st, d_desc);
+    hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)((max_work + 255) / 256), nsys), blk, 0, st, d_desc);
+  }
+  hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(max_gx, nsys), blk, shx, st, d_desc);
+  hipLaunchKernelGGL((k_fft_pencil<T, -1>), dim3(max_gy, nsys), blk, shy, st, d_desc, 2);
+  hipLaunchKernelGGL(k_fft_z_conv<T>, dim3(max_gz, nsys), blk, shz, st, d_desc);
+  hipLaunchKernelGGL((k_fft_pencil<T, +1>), dim3(max_gy, nsys), blk, shy, st, d_desc, 2);
+  hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(max_gx, nsys), blk, shx, st, d_desc);
+  hipLaunchKernelGGL(k_gather<T>, dim3(max_gg, nsys), blk, 0, st, d_desc);
+  hipLaunchKernelGGL(k_pme_finalize, dim3(16, nsys), dim3(256), 0, st, d_desc);
 }
 
-int conv_blocks(int K1, int K2, int K3, int lds_elems) {
-  int H = K1 / 2 + 1;
-  int tpz = tile_pencils(K3, H * K2, lds_elems);
-  return (H * K2 + tpz - 1) / tpz;
-}
-
-template void launch_pme<double>(const PmeDesc *, const PmeDesc &, int, double *, hipStream_t, const PmeTiles &);
-template void launch_pme<float>(const PmeDesc *, const PmeDesc &, int, double *, hipStream_t, const PmeTiles &);
+template void launch_pme<double>(const PmeDesc *, const PmeDesc *, int, hipStream_t, const Tuning &);
+template void launch_pme<float>(const PmeDesc *, const PmeDesc *, int, hipStream_t, const Tuning &);
 
 }  // namespace orac

@@ -9,7 +9,8 @@ arguments, and scalar outputs are returned.  A nonzero status from the
 library raises ``OracHipError`` (the reference's fatal ``xerror`` path).
 
 ``ResidentSystem`` drives the device-resident path (topology, coordinates,
-lists and grids kept in HBM) used for throughput measurements.
+lists and grids kept in HBM); ``Batch`` evaluates several resident systems
+(replicas, windows, requests) with one kernel launch per stage.
 
 The library must be built (``orac_hip.build``); there is no CPU fallback.
 """
@@ -53,9 +54,17 @@ EXPORTS = [
     "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
     "orac_hip_decide_force", "orac_hip_ferrf", "orac_hip_cself", "orac_hip_fnbgrp", "orac_hip_system_create",
     "orac_hip_system_destroy", "orac_hip_system_set_coords", "orac_hip_system_device_ptrs", "orac_hip_system_eval",
-    "orac_hip_system_results", "orac_hip_system_profile", "orac_hip_system_stage_ms", "orac_hip_sync",
-    "orac_hip_stream",
+    "orac_hip_system_results", "orac_hip_system_profile", "orac_hip_system_stage_ms", "orac_hip_system_bytes",
+    "orac_hip_sync", "orac_hip_stream", "orac_hip_batch_create", "orac_hip_batch_destroy", "orac_hip_batch_eval",
+    "orac_hip_batch_profile", "orac_hip_batch_stage_ms", "orac_hip_batch_set_tuning", "orac_hip_batch_get_tuning",
+    "orac_hip_batch_autotune", "orac_hip_md_setup", "orac_hip_md_set_state", "orac_hip_md_get_state",
+    "orac_hip_batch_md_run", "orac_hip_batch_md_relax", "orac_hip_batch_md_scale_velocities",
+    "orac_hip_system_md_energies",
 ]
+
+# stage_ms order of the resident evaluation
+STAGES = ["centres", "list", "pairs", "pme", "corrections", "pair_kernel"]
+TUNING_FIELDS = ["lds_elems", "pair_waves", "spread_tiled", "xcd_remap"]
 
 
 def _check(rc):
@@ -305,11 +314,66 @@ class ResidentSystem:
         _check(lib().orac_hip_system_profile(self.ctx._h, self.id, C.c_int32(1 if enable else 0)))
 
     def stage_ms(self):
-        ms = (C.c_float * 5)()
-        _check(lib().orac_hip_system_stage_ms(self.ctx._h, self.id, ms, C.c_int32(5)))
-        return list(ms)
+        """Milliseconds of the last eval's stages, keyed by ``STAGES``."""
+        ms = (C.c_float * len(STAGES))()
+        _check(lib().orac_hip_system_stage_ms(self.ctx._h, self.id, ms, C.c_int32(len(STAGES))))
+        return dict(zip(STAGES, ms))
+
+    def device_bytes(self):
+        b = C.c_int64()
+        _check(lib().orac_hip_system_bytes(self.ctx._h, self.id, C.byref(b)))
+        return b.value
 
     def close(self):
         if self.id.value:
             _check(lib().orac_hip_system_destroy(self.ctx._h, self.id))
+            self.id = C.c_int32(0)
+
+
+class Batch:
+    """Resident systems evaluated together, one kernel launch per stage."""
+
+    def __init__(self, ctx, systems):
+        self.ctx = ctx
+        self.systems = list(systems)
+        ids = np.array([s.id.value for s in self.systems], np.int32)
+        self.id = C.c_int32()
+        _check(lib().orac_hip_batch_create(ctx._h, C.c_int32(len(ids)), _p(ids), C.byref(self.id)))
+
+    def eval(self, rebuild_list=True, graph=False):
+        _check(lib().orac_hip_batch_eval(self.ctx._h, self.id, C.c_int32(1 if rebuild_list else 0),
+                                         C.c_int32(1 if graph else 0)))
+
+    def results(self):
+        return [s.results() for s in self.systems]
+
+    def profile(self, enable=True):
+        _check(lib().orac_hip_batch_profile(self.ctx._h, self.id, C.c_int32(1 if enable else 0)))
+
+    def stage_ms(self):
+        ms = (C.c_float * len(STAGES))()
+        _check(lib().orac_hip_batch_stage_ms(self.ctx._h, self.id, ms, C.c_int32(len(STAGES))))
+        return dict(zip(STAGES, ms))
+
+    def tuning(self):
+        t = np.zeros(len(TUNING_FIELDS), np.int32)
+        _check(lib().orac_hip_batch_get_tuning(self.ctx._h, self.id, _p(t), C.c_int32(len(t))))
+        return dict(zip(TUNING_FIELDS, (int(v) for v in t)))
+
+    def set_tuning(self, **kw):
+        cur = self.tuning()
+        cur.update(kw)
+        t = np.array([cur[k] for k in TUNING_FIELDS], np.int32)
+        _check(lib().orac_hip_batch_set_tuning(self.ctx._h, self.id, _p(t), C.c_int32(len(t))))
+
+    def autotune(self, reps=5, rebuild_list=True):
+        """Pick the fastest tuning by timing the batch; returns (tuning, ms per eval)."""
+        ms = C.c_double()
+        _check(lib().orac_hip_batch_autotune(self.ctx._h, self.id, C.c_int32(reps),
+                                             C.c_int32(1 if rebuild_list else 0), C.byref(ms)))
+        return self.tuning(), ms.value
+
+    def close(self):
+        if self.id.value:
+            _check(lib().orac_hip_batch_destroy(self.ctx._h, self.id))
             self.id = C.c_int32(0)

@@ -12,6 +12,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)                      # orac-porting_amd/
@@ -19,12 +20,12 @@ CSRC = os.path.join(ROOT, "csrc")
 LIBDIR = os.path.join(ROOT, "lib")
 LIB = os.path.join(LIBDIR, "liborac_hip.so")
 FLIB = os.path.join(LIBDIR, "liborac_hip_fortran.so")
-SOURCES = ["pme.hip", "pairs.hip", "force_pairs.hip", "api.cpp"]
+SOURCES = ["pme.hip", "pairs.hip", "force_pairs.hip", "md.hip", "host.cpp", "api.cpp", "engine.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ORAC_HIP_ARCH", "gfx950")
 
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # packed-FP32 instructions stay off: hipcc would otherwise emit them from float2 code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
@@ -43,9 +44,16 @@ def _run(cmd, verbose):
 
 def build(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    # every translation unit launches only its own kernels, so the objects
+    # compile independently (in parallel) and link without relocatable device code
+    cmds = [[HIPCC, *HIP_FLAGS, "-c", "-x", "hip", os.path.join(CSRC, s), "-o", o] for s, o in zip(SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as pool:
+        list(pool.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-x", "hip", *srcs, "-o", tmp], verbose)
+    _run([HIPCC, *HIP_FLAGS, "-shared", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     flang = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
     shim = os.path.join(ROOT, "fortran", "orac_hip_shim.f90")

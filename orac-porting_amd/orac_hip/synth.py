@@ -22,6 +22,7 @@ float64.  Nothing here calls the GPU.
 from __future__ import annotations
 
 import dataclasses
+import functools
 import math
 
 import numpy as np
@@ -273,7 +274,9 @@ def _lj_tables(sig, eps):
 
 
 def _assemble(L, solute_pos, solute_q, solute_sig, solute_eps, water_sites, *, alpha, nfft, order, rcut,
-              solute_bonds=None, solute_bends=None, solute_mass=12.011, name=""):
+              solute_bonds=None, solute_bends=None, solute_mass=12.011, name="", solute_types=None):
+    """Build a System.  Solute LJ: one type per solute atom (sigma/eps per atom),
+    or, with solute_types (0-based type of each solute atom), sigma/eps per type."""
     ns = len(solute_pos)
     nw = len(water_sites) // 3
     n = ns + 3 * nw
@@ -284,11 +287,13 @@ def _assemble(L, solute_pos, solute_q, solute_sig, solute_eps, water_sites, *, a
     xp = cart / half
     q = np.concatenate([np.asarray(solute_q, float), np.tile(WATER_Q, nw)])
     mass = np.concatenate([np.full(ns, solute_mass), np.tile(WATER_MASS, nw)])
-    # types: solute atoms 1..ns (one type each), then O, H
+    # types: the solute's (one per atom unless solute_types is given), then O, H
+    nst = ns if solute_types is None else len(solute_sig)
     sig = np.concatenate([np.asarray(solute_sig, float), [O_SIGMA, 1.0]])
     eps = np.concatenate([np.asarray(solute_eps, float), [O_EPS, 0.0]])
     tt, ecc12, ecc6 = _lj_tables(sig, eps)
-    nbtype = np.concatenate([np.arange(1, ns + 1), np.tile([ns + 1, ns + 2, ns + 2], nw)]).astype(np.int32)
+    st = np.arange(1, ns + 1) if solute_types is None else np.asarray(solute_types) + 1
+    nbtype = np.concatenate([st, np.tile([nst + 1, nst + 2, nst + 2], nw)]).astype(np.int32)
     # groups: each solute atom its own group, each water one group
     ng = ns + nw
     first = np.concatenate([np.arange(ns), ns + 3 * np.arange(nw)])
@@ -326,6 +331,21 @@ def water_box(seed=1000, L=18.64, nside=6, jitter=0.3, alpha=0.40, nfft=(20, 20,
                      name=f"water{len(o)}")
 
 
+@functools.lru_cache(maxsize=8)
+def _solute(seed, n_solute, radius):
+    """Solute atoms of one seed: positions, charges (net 0), LJ sigma/eps, chain exclusions."""
+    rng_s = np.random.default_rng(seed)
+    pos = _rsa_sphere(rng_s, n_solute, radius, 1.2)
+    q = rng_s.normal(0.0, 0.35, n_solute)
+    q = q - q.mean()
+    sig = rng_s.uniform(2.0, 3.8, n_solute)
+    eps = rng_s.uniform(0.02, 0.2, n_solute)
+    bonds, bends = _chain_exclusions(pos)
+    for a in (pos, q, sig, eps, bonds, bends):
+        a.setflags(write=False)
+    return pos, q, sig, eps, bonds, bends
+
+
 def solvated_solute(seed=1001, L=61.0, n_solute=1660, radius=16.0, n_water=7600, alpha=0.35,
                     nfft=(64, 64, 64), order=5, rcut=10.0, charge_scale=1.0, eps_scale=1.0, solute_seed=None):
     """Compact random solute (1-2/1-3 chain exclusions) in a box of rigid waters.
@@ -336,13 +356,7 @@ def solvated_solute(seed=1001, L=61.0, n_solute=1660, radius=16.0, n_water=7600,
     replica scaling: charges by sqrt(s), epsilon by s).  ``solute_seed``
     fixes the solute independently of the water seed.
     """
-    rng_s = np.random.default_rng(seed if solute_seed is None else solute_seed)
-    pos = _rsa_sphere(rng_s, n_solute, radius, 1.2)
-    q = rng_s.normal(0.0, 0.35, n_solute)
-    q = q - q.mean()
-    sig = rng_s.uniform(2.0, 3.8, n_solute)
-    eps = rng_s.uniform(0.02, 0.2, n_solute)
-    bonds, bends = _chain_exclusions(pos)
+    pos, q, sig, eps, bonds, bends = _solute(seed if solute_seed is None else solute_seed, n_solute, radius)
     rng = np.random.default_rng(seed)
     if solute_seed is not None:
         rng = np.random.default_rng(seed + 7)
@@ -373,6 +387,22 @@ def config(name, seed=None):
     if name == "solvated24k":
         return solvated_solute(seed=1001 if seed is None else seed)
     raise KeyError(name)
+
+
+def hrem_scales(n=32, s_min=0.3):
+    """Hamiltonian scaling factors s_r, geometric from 1 down to s_min."""
+    if n == 1:
+        return np.ones(1)
+    return s_min ** (np.arange(n) / (n - 1))
+
+
+def hrem_replicas(n=32, s_min=0.3, solute_seed=1001, water_seed=1002, **kw):
+    """HREM replicas of one solvated solute: replica r scales the solute charges
+    by sqrt(s_r) and the solute LJ epsilon by s_r (s geometric from 1 to s_min)
+    and has its own water placement (seed water_seed + 7919 r).  Defaults are
+    the 24,460-atom system; keyword arguments go to ``solvated_solute``."""
+    return [solvated_solute(seed=water_seed + 7919 * r, solute_seed=solute_seed, charge_scale=float(sr),
+                            eps_scale=float(sr), **kw) for r, sr in enumerate(hrem_scales(n, s_min))]
 
 
 def small_solvated(seed=5, L=24.0, n_solute=40, radius=5.0, n_water=380, nfft=(24, 24, 24), rcut=9.0,
