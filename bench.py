@@ -9,7 +9,10 @@ Ewald-erfc + LJ pairs, SPME (spread, r2c FFT, influence function, c2r FFT,
 gather) and the per-step exclusion corrections.
 
 Prints one JSON line.  ``value`` is FP64 force evaluations per second of
-the single system (the plain path).  Beside it: the same system in mixed
+the single system (the plain path).  ``dynamics`` gives ns/day of on-device
+NVE r-RESPA runs (plain FP64 single system; 32 replicas batched in mixed
+precision) with their NVE energy drift; ``serving`` the p50/p99 latency and
+aggregate ns/day of a continuous-batching run of MD segment requests.  Beside it: the same system in mixed
 precision, and ``batched``: 32 HREM replicas of the system (solute charges
 scaled by sqrt(s), solute LJ epsilon by s, s geometric 1 -> 0.3, independent
 water placements) in mixed precision, one kernel launch per stage for all
@@ -127,6 +130,64 @@ def time_batched(ctx, reps, precision, steps, warmup, tune=True):
     return dict(dt=dt, res0=res0, stages=stages, tuning=tuning, tune_ms=tune_ms, device_bytes=mem)
 
 
+def prepare_md(ctx, systems, precision, relax=200, temperature=300.0, seed=2024):
+    from orac_hip import md
+    from orac_hip.api import Batch
+    rss = [resident(ctx, s, precision) for s in systems]
+    dyns = [md.Dynamics(rs, s) for rs, s in zip(rss, systems)]
+    b = Batch(ctx, rss)
+    md.relax_batch(b, relax, dmax=0.03)
+    for k, d in enumerate(dyns):
+        d.maxwell_boltzmann(temperature, seed + k)
+    return rss, dyns, b
+
+
+def time_md(ctx, b, nsteps, warmup, dt=2.0, ninner=4, list_every=10):
+    from orac_hip import md
+    md.run_batch(b, warmup, dt=dt, ninner=ninner, list_every=list_every)
+    ctx.sync()
+    t0 = time.perf_counter()
+    md.run_batch(b, nsteps, dt=dt, ninner=ninner, list_every=list_every)
+    ctx.sync()
+    return time.perf_counter() - t0
+
+
+def nve_drift(ctx, b, dyn, nsteps, dt=2.0, ninner=4, list_every=10):
+    """Total-energy change over nsteps (per ns, kJ/mol) of the first system."""
+    from orac_hip import md, synth
+    e0 = dyn.energies()
+    md.run_batch(b, nsteps, dt=dt, ninner=ninner, list_every=list_every)
+    ctx.sync()
+    e1 = dyn.energies()
+    kj = synth.EFACT / 1000.0
+    ns = nsteps * dt * 1e-6
+    return dict(kj_mol_per_ns=(e1["total"] - e0["total"]) * kj / ns,
+                kj_mol_per_ns_per_atom=(e1["total"] - e0["total"]) * kj / ns / dyn.s.natoms,
+                rel_per_ns=(e1["total"] - e0["total"]) / abs(e0["total"]) / ns,
+                temperature=(e0["temperature"], e1["temperature"]), steps=nsteps, fail=e1["fail"])
+
+
+def ns_per_day(steps, wall, dt=2.0):
+    return steps * dt * 1e-6 / wall * 86400.0
+
+
+def serving_leg(ctx, nreq=16, rate=200.0, nsteps=500, budget_gib=64, seed=4242):
+    from orac_hip import serve
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(2000), np.log(150000), nreq)).astype(int)
+    arrivals = serve.poisson_arrivals(rng, nreq, rate)
+    reqs = [serve.Request(rid=k, system=serve.serving_request(seed + k, int(n), rcut=float(rng.uniform(9.0, 10.0))),
+                          nsteps=nsteps, arrival=float(arrivals[k]), seed=seed + k) for k, n in enumerate(sizes)]
+    log(f"serving: {nreq} requests, {sum(r.system.natoms for r in reqs)} atoms in all; relaxing start positions")
+    serve.relax_requests(ctx, reqs, niter=200)
+    srv = serve.Server(ctx, hbm_budget=budget_gib << 30, chunk=25)
+    srv.progress = log
+    st = srv.run(reqs)
+    st.update(sizes_atoms=[r.system.natoms for r in reqs], rate_per_s=rate, steps_per_request=nsteps,
+              budget_gib=budget_gib, fail=sum(r.energies["fail"] for r in reqs))
+    return st
+
+
 def cpu_baseline(s, corr, nthreads):
     import oracle as O
     rc = s.rcut + s.rtol + s.rnei
@@ -160,6 +221,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--replicas", type=int, default=32, help="HREM replicas of the batched leg (0 skips it)")
+    ap.add_argument("--md-steps", type=int, default=200, help="outer steps of each dynamics leg (0 skips them)")
+    ap.add_argument("--serve-requests", type=int, default=8, help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--serve-steps", type=int, default=500, help="outer steps per served request")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -208,6 +272,36 @@ def main():
             log(f"batched fp64 x{args.replicas}: {1e3 * bd['dt'] / bsteps:.3f} ms/step")
             batched = dict(bm=bm, bd=bd, bsteps=bsteps, err=relrms(bm["res0"]["f"], f0),
                            err64=relrms(bd["res0"]["f"], f0))
+        dyn = None
+        if args.md_steps > 0:
+            K = args.md_steps
+            # plain FP64, one system at a time (what 32 replicas cost sequentially, per replica)
+            rss, dyns, b = prepare_md(ctx, [s], 0)
+            w64 = time_md(ctx, b, K, 5)
+            drift64 = nve_drift(ctx, b, dyns[0], K)
+            b.close()
+            rss[0].close()
+            log(f"md fp64 single: {1e3 * w64 / K:.3f} ms/step, NVE {drift64}")
+            dyn = dict(fp64=dict(ns_per_day=ns_per_day(K, w64), ms_per_step=1e3 * w64 / K, nve=drift64))
+            if args.replicas > 0:
+                rss, dyns, b = prepare_md(ctx, reps, 1)
+                b.autotune(reps=2, rebuild_list=False)
+                wm = time_md(ctx, b, K, 5)
+                driftm = nve_drift(ctx, b, dyns[0], K)
+                log(f"md mixed x{args.replicas}: {1e3 * wm / K:.3f} ms/step, NVE {driftm}")
+                dyn["batched_mixed"] = dict(replicas=args.replicas, ns_per_day_aggregate=ns_per_day(K * args.replicas,
+                                                                                                   wm),
+                                            ms_per_step=1e3 * wm / K, nve=driftm, tuning=b.tuning())
+                dyn["speedup_batched_mixed_vs_plain_fp64"] = (dyn["batched_mixed"]["ns_per_day_aggregate"]
+                                                              / dyn["fp64"]["ns_per_day"])
+                b.close()
+                for x in rss:
+                    x.close()
+        served = None
+        if args.serve_requests > 0:
+            served = serving_leg(ctx, nreq=args.serve_requests, nsteps=args.serve_steps)
+            log(f"serving: p50 {served['p50_ms']:.1f} ms p99 {served['p99_ms']:.1f} ms, "
+                f"{served['aggregate_ns_per_day']:.1f} ns/day")
 
     evals64 = args.steps / dt64
     evals32 = args.steps / dt32
@@ -256,6 +350,16 @@ def main():
             "speedup_mixed_batched_vs_plain_fp64": agg / evals64,
             "force_relrms_mixed_vs_f64": batched["err"], "force_relrms_batched_f64_vs_alone": batched["err64"],
             "tuning": bm["tuning"], "stage_ms_mixed": rnd(bm["stages"]), "device_bytes": bm["device_bytes"]}
+    if dyn:
+        dyn["workload"] = ("NVE r-RESPA: 2 fs outer step, 4 inner steps for the solute harmonic terms, rigid waters "
+                           "(SHAKE/RATTLE), list every 10 steps; after 200 steepest-descent steps and "
+                           "Maxwell-Boltzmann velocities at 300 K")
+        out["dynamics"] = dyn
+    if served:
+        served["workload"] = ("Poisson stream of MD segment requests (lattice solutes of 2k-150k atoms, log-uniform, "
+                              "in water boxes; grid spacing <= 1 A; order 5; cutoff 9-10 A), mixed precision, "
+                              "continuous batching with HBM-budget admission")
+        out["serving"] = served
     if not args.no_cpu:
         nthr = args.cpu_threads or min(16, os.cpu_count() or 1)
         v, nsample, cref = cpu_baseline(s, corr, nthr)

@@ -208,9 +208,9 @@ extern "C" int orac_hip_lc_list(orac_hip_ctx *ctx, int32_t ncx, int32_t ncy, int
     L.ypg = B.put("l_y", yp, natp, st, false);
     L.zpg = B.put("l_z", zp, natp, st, false);
     L.cell_of = B.get<int>("l_cellof", natp);
-    L.cell_cnt = B.get<int>("l_cnt", ncell);
-    L.cell_start = B.get<int>("l_start", ncell);
-    L.cell_cursor = B.get<int>("l_cur", ncell);
+    L.cell_cnt = B.get<int>("l_cnt", ncell + 1);
+    L.cell_start = B.get<int>("l_start", ncell + 1);
+    L.cell_cursor = B.get<int>("l_cur", ncell + 1);
     L.cell_items = B.get<int>("l_items", natp);
     L.count = B.get<int>("l_count", natp + 1);
     L.off = B.get<int>("l_off", natp + 1);
@@ -434,8 +434,10 @@ extern "C" int orac_hip_force(orac_hip_ctx *ctx, const int32_t *ss_index, const 
     p.fz = B.get<double>("f_fz", nato);
     p.nblk = std::max(1, (ngrp + PAIR_WAVES - 1) / PAIR_WAVES);
     p.part = B.get<double>("f_part", (size_t)p.nblk * 16);
-    p.apos = B.get<double4>("f_apos", nato);
-    p.ainfo = B.get<int2>("f_ainfo", nato);
+    p.sgp = B.get<double4>("f_sgp", std::max(1, ngrp));
+    p.sgi = B.get<int4>("f_sgi", std::max(1, ngrp));
+    p.sap = B.get<double4>("f_sap", std::max(1, nato));
+    p.sai = B.get<int2>("f_sai", std::max(1, nato));
     p.max_group = max_group;
     double *dout = B.get<double>("f_out", 16);
     p.out = dout;

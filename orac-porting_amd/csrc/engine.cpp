@@ -287,8 +287,10 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   p.fz = S.fz;
   p.nblk = (ng + PAIR_WAVES - 1) / PAIR_WAVES;
   p.part = B.get<double>("ppart", (size_t)p.nblk * 16);
-  p.apos = B.get<double4>("apos", n);
-  p.ainfo = B.get<int2>("ainfo", n);
+  p.sgp = B.get<double4>("sgp", ng);
+  p.sgi = B.get<int4>("sgi", ng);
+  p.sap = B.get<double4>("sap", n);
+  p.sai = B.get<int2>("sai", n);
   p.out = S.out + 20;
   p.max_group = S.max_group;
 
@@ -318,9 +320,9 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
     L.ypg = ypg;
     L.zpg = zpg;
     L.cell_of = B.get<int>("cellof", ng);
-    L.cell_cnt = B.get<int>("ccnt", ncell);
-    L.cell_start = B.get<int>("cstart", ncell);
-    L.cell_cursor = B.get<int>("ccur", ncell);
+    L.cell_cnt = B.get<int>("ccnt", ncell + 1);
+    L.cell_start = B.get<int>("cstart", ncell + 1);
+    L.cell_cursor = B.get<int>("ccur", ncell + 1);
     L.cell_items = B.get<int>("citems", ng);
     L.count = B.get<int>("lcount", ng + 1);
     L.off = B.get<int>("nboff", ng + 1);
@@ -328,10 +330,16 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
     L.cap = (long long)S.list_cap;
     L.total = B.get<int>("ltotal", 2);
     L.mode = 0;
+    std::vector<int> gsize(ng);
+    for (int g = 0; g < ng; ++g) gsize[g] = glast[g] - gfirst[g] + 1;
+    L.gsize = B.put_vec("gsize", gsize, st, false);
+    L.soff = B.get<int>("soff", ng);
     launch_zero_u64(reinterpret_cast<unsigned long long *>(L.total), 1, st);
   }
   p.nb_off = L.off;
   p.nb_idx = L.out;
+  p.perm = L.cell_items;  // cell order of the last list build
+  p.soff = L.soff;
 
   // ---- reciprocal space
   build_pme_plan(B, "", S.plan, d.nfft1, d.nfft2, d.nfft3, d.order, st);

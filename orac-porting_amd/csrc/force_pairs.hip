@@ -31,12 +31,24 @@ namespace orac {
 #define AS_A5 1.0614054
 #define AS_QP 0.3275911
 
-__global__ void __launch_bounds__(256) k_pack_atoms(const PairDesc *__restrict__ D) {
+// Cell-ordered copies of the step's group and atom data: sorted position t
+// holds group perm[t] (identity without a permutation) and its atoms from
+// soff[t] on.  Neighbours listed next to each other are then next to each
+// other in memory, and one group is one 32-byte and one 16-byte record.
+__global__ void __launch_bounds__(256) k_pack_sorted(const PairDesc *__restrict__ D) {
   const PairDesc &d = D[blockIdx.y];
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= d.natoms) return;
-  d.apos[a] = make_double4(d.xp[a], d.yp[a], d.zp[a], d.q[a]);
-  d.ainfo[a] = make_int2(d.type[a], d.ex_inter[a] ? 1 : 0);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.ngrp) return;
+  const int g = d.perm ? d.perm[t] : t;
+  const int f0 = d.gfirst[g], n = d.glast[g] - f0 + 1;
+  const int sf = d.soff ? d.soff[t] : f0;
+  d.sgp[t] = make_double4(d.xpg[g], d.ypg[g], d.zpg[g], 0.0);
+  d.sgi[t] = make_int4(sf, f0, n | (d.gss[g] << 8), g);
+  for (int k = 0; k < n; ++k) {
+    const int a = f0 + k;
+    d.sap[sf + k] = make_double4(d.xp[a], d.yp[a], d.zp[a], d.q[a]);
+    d.sai[sf + k] = make_int2(d.type[a], d.ex_inter[a] ? 1 : 0);
+  }
 }
 
 __device__ inline bool has_partner_in(const int *ex_off, const int *ex_idx, int a, int lo, int hi) {
@@ -79,34 +91,44 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
   const PairDesc &d = D[sys];
   if (blk >= d.nblk) return;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int g = blk * PAIR_WAVES + wid;
+  const int t = blk * PAIR_WAVES + wid;  // owner group, sorted position
   double fxa = 0.0, fya = 0.0, fza = 0.0;
   double en[6] = {0, 0, 0, 0, 0, 0};  // uconf[typeij], ucoul[typeij]
   double st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  int ia0 = 0, nI = 1, nslot = 64;
-  if (g < d.ngrp) {
-    ia0 = d.gfirst[g];
-    nI = d.glast[g] - ia0 + 1;
+  int ia0 = 0, nI = 1, nslot = 64;  // ia0: owner's first atom, original numbering
+  if (t < d.ngrp) {
+    const int4 gi = d.sgi[t];
+    const int g = gi.w;
+    ia0 = gi.y;
+    nI = gi.z & 255;
     nslot = 64 / nI;
     const int slot = lane / nI, ai = lane - slot * nI;
     const bool active = slot < nslot;
-    const int a = ia0 + (active ? ai : 0);
-    const double4 pa4 = d.apos[a];
-    const int2 ia4 = d.ainfo[a];
+    const int a = ia0 + (active ? ai : 0);        // original index
+    const int as = gi.x + (active ? ai : 0);      // sorted index
+    const double4 pa4 = d.sap[as];
+    const int2 ia4 = d.sai[as];
     const double ma = d.gmass[a];
-    const int typei = d.gss[g];
-    const double xpgi = d.xpg[g], ypgi = d.ypg[g], zpgi = d.zpg[g];
+    const int typei = gi.z >> 8;
+    const double4 gpi = d.sgp[t];
+    const double xpgi = gpi.x, ypgi = gpi.y, zpgi = gpi.z;
     const double co0 = d.co[0], co3 = d.co[3], co6 = d.co[6], co4 = d.co[4], co7 = d.co[7], co8 = d.co[8];
-    const int nb0 = d.nb_off[g], nb1 = d.nb_off[g + 1];
+    const int nb0 = d.nb_off[t], nb1 = d.nb_off[t + 1];
+    int tjn = (active && nb0 + slot < nb1) ? d.nb_idx[nb0 + slot] : t;
     for (int base = nb0; base < nb1; base += nslot) {
       const int jj = base + slot;
       const bool have = active && jj < nb1;
-      const int j = have ? d.nb_idx[jj] : g;
+      const int tj = tjn;
+      // the next neighbour's index is fetched before this one is worked on
+      tjn = (active && jj + nslot < nb1) ? d.nb_idx[jj + nslot] : t;
+      const double4 gpj = d.sgp[tj];
+      const int4 gij = d.sgi[tj];
+      const int j = gij.w;
       // group classification (force.f:265-335), no contraction
       double sxm, sym, szm, xcc, ycc, zcc, drj;
       {
 #pragma clang fp contract(off)
-        double xgg = xpgi - d.xpg[j], ygg = ypgi - d.ypg[j], zgg = zpgi - d.zpg[j];
+        double xgg = xpgi - gpj.x, ygg = ypgi - gpj.y, zgg = zpgi - gpj.z;
         sxm = 2.0 * fnint(0.5 * xgg);
         sym = 2.0 * fnint(0.5 * ygg);
         szm = 2.0 * fnint(0.5 * zgg);
@@ -144,12 +166,14 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
       double cm = 0.0;                      // this lane's part of V (switched pairs)
       double fgx = 0.0, fgy = 0.0, fgz = 0.0;  // this lane's pair forces for the group pair
       if (cls != 0) {
-        const int jb0 = d.gfirst[j], jb1 = d.glast[j];
-        const int typeij = typei + d.gss[j] - 2;  // 0..2
+        const int nJ = gij.z & 255;
+        const int jb0 = gij.y, jb1 = jb0 + nJ - 1, jbs = gij.x;  // original range, sorted start
+        const int typeij = typei + (gij.z >> 8) - 2;  // 0..2
         const bool mask_a = fwd && (ia4.y & 1) && has_partner_in(d.ex_off, d.ex_idx, a, jb0, jb1);
         double ucoula = 0.0, uconfa = 0.0;
-        for (int b = jb0; b <= jb1; ++b) {
-          const int2 ib4 = d.ainfo[b];
+        for (int k = 0; k < nJ; ++k) {
+          const int b = jb0 + k;
+          const int2 ib4 = d.sai[jbs + k];
           const bool masked =
               fwd ? mask_a : ((ib4.y & 1) && has_partner_in(d.ex_off, d.ex_idx, b, ia0, ia0 + nI - 1));
           if (masked) {
@@ -159,7 +183,7 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
           double c12, c6;
           lj_params(d, ia4.x, ib4.x, c12, c6);
           if (d.lskip_ewald && c6 < 1e-12 && d.ewald) continue;
-          const double4 pb4 = d.apos[b];
+          const double4 pb4 = d.sap[jbs + k];
           double xg, yg, zg;
           if (cls == 1) {
             xg = (pa4.x - sxm) - pb4.x;
@@ -225,7 +249,7 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
       }
       // total V of the group pair over the slot's lanes (every lane shuffles)
       double cmt = 0.0;
-      for (int t = 0; t < nI; ++t) cmt += __shfl(cm, (slot * nI + t) & 63, 64);
+      for (int u = 0; u < nI; ++u) cmt += __shfl(cm, (slot * nI + u) & 63, 64);
       if (cls == 2) {
         // S'(r) V spread over group atoms by mass fraction (force.f:756-817)
         const double x3 = -(dsw * cmt * xcc), y3 = -(dsw * cmt * ycc), z3 = -(dsw * cmt * zcc);
@@ -250,7 +274,7 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
   facc[wid][lane][1] = fya;
   facc[wid][lane][2] = fza;
   __syncthreads();
-  if (g < d.ngrp && lane < nI) {
+  if (t < d.ngrp && lane < nI) {
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int s = 0; s < nslot; ++s) {
       sx += facc[wid][s * nI + lane][0];
@@ -321,14 +345,14 @@ static int pair_occupancy(int tuned) {
 template <typename T>
 void launch_pairs(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream_t st, const Tuning &tu,
                   hipEvent_t kbeg, hipEvent_t kend) {
-  int max_atoms = 1, max_blk = 1;
+  int max_grp = 1, max_blk = 1;
   for (int s = 0; s < nsys; ++s) {
     if (h[s].max_group > 64)
       throw Error(E_UNSUPPORTED, "groups larger than 64 atoms are not supported by the pair kernel");
-    max_atoms = std::max(max_atoms, h[s].natoms);
+    max_grp = std::max(max_grp, h[s].ngrp);
     max_blk = std::max(max_blk, h[s].nblk);
   }
-  hipLaunchKernelGGL(k_pack_atoms, dim3((max_atoms + 255) / 256, nsys), dim3(256), 0, st, d_desc);
+  hipLaunchKernelGGL(k_pack_sorted, dim3((max_grp + 255) / 256, nsys), dim3(256), 0, st, d_desc);
   dim3 grd(max_blk, nsys), blk(64 * PAIR_WAVES);
   const int remap = tu.xcd_remap ? 1 : 0;
   if (kbeg) ORAC_HIP(hipEventRecord(kbeg, st));

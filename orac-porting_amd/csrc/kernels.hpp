@@ -51,7 +51,7 @@ struct PairDesc {
   const unsigned char *ex_inter;    // atom has an excluded partner outside its own group
   int any_heavy;
   double massinfty;
-  const int *nb_off, *nb_idx;       // full neighbour list of each group (CSR)
+  const int *nb_off, *nb_idx;       // full neighbour list (CSR): owners and entries are sorted positions
   double co[9];
   double alpha, twrtpi;
   double r2inn, r2out, rinn0, r2inn0, rout0, r2out0, arsout1, arsout2, arsinn1, arsinn2;
@@ -59,8 +59,15 @@ struct PairDesc {
   double *fx, *fy, *fz;             // direct forces (written)
   double *part;                     // per-block partials [nblk][16]: uconf[3] ucoul[3] st[9]
   int nblk;
-  double4 *apos;                    // packed {xp, yp, zp, q} per atom (k_pack_atoms)
-  int2 *ainfo;                      // packed {type, flags} per atom; flags bit 0 = ex_inter
+  // Cell-ordered copies written every evaluation by k_pack_sorted.  Sorted
+  // position t holds group perm[t] (t itself without a permutation), whose
+  // atoms start at sorted atom soff[t] (gfirst[t] without).
+  const int *perm;
+  const int *soff;
+  double4 *sgp;                     // [ngrp] {xpg, ypg, zpg, 0}
+  int4 *sgi;                        // [ngrp] {sorted first atom, first atom, natoms | gss << 8, group}
+  double4 *sap;                     // [natoms] {xp, yp, zp, q}
+  int2 *sai;                        // [natoms] {type, flags}; flags bit 0 = ex_inter
   double *out;                      // [15]: uconf[3], ucoul[3] by typeij, st[9] row-major
   int max_group;                    // largest group (atoms)
 };
@@ -100,6 +107,10 @@ struct ListDesc {
   int mode;                  // 0: CSR, 0-based; 1: reference layout per group [nv, l1..lnv], 1-based
   const int *step;           // optional step counter of the system: the build runs only
   int every;                 //   when *step % every == 0 (device-side update schedule)
+  // mode 0 only: owners are taken in cell order (cell_items) and entries are
+  // cell-order positions; soff[t] = first atom of the t-th group in that order
+  const int *gsize;          // [ngrp] atoms per group (original numbering)
+  int *soff;
 };
 enum ListPhase { LIST_COUNT = 1, LIST_WRITE = 2 };
 // LIST_COUNT: cells, per-group counts, offsets and the capacity check;

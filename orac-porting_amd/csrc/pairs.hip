@@ -106,7 +106,7 @@ __device__ inline bool list_due(const ListDesc &d) { return !d.step || (*d.step 
 __global__ void __launch_bounds__(256) k_list_zero(const ListDesc *__restrict__ D) {
   const ListDesc &d = D[blockIdx.y];
   if (!list_due(d)) return;
-  const int ncell = d.cg.ncx * d.cg.ncy * d.cg.ncz;
+  const int ncell = d.cg.ncx * d.cg.ncy * d.cg.ncz + 1;  // + the parking cell
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += gridDim.x * blockDim.x) {
     d.cell_cnt[i] = 0;
     d.cell_cursor[i] = 0;
@@ -119,10 +119,18 @@ __global__ void __launch_bounds__(256) k_cell_assign(const ListDesc *__restrict_
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.ngrp || !list_due(d)) return;
   const CellGrid &cg = d.cg;
-  int nx = cell_axis(d.xpg[g], 2.0 / cg.ncx, cg.ncx);
-  int ny = cell_axis(d.ypg[g], 2.0 / cg.ncy, cg.ncy);
-  int nz = cell_axis(d.zpg[g], 2.0 / cg.ncz, cg.ncz);
-  int c = nz + cg.ncz * (ny + cg.ncy * nx);
+  const double x = d.xpg[g], y = d.ypg[g], z = d.zpg[g];
+  int c;
+  if (fabs(x) < 1.0e8 && fabs(y) < 1.0e8 && fabs(z) < 1.0e8) {
+    int nx = cell_axis(x, 2.0 / cg.ncx, cg.ncx);
+    int ny = cell_axis(y, 2.0 / cg.ncy, cg.ncy);
+    int nz = cell_axis(z, 2.0 / cg.ncz, cg.ncz);
+    c = nz + cg.ncz * (ny + cg.ncy * nx);
+  } else {
+    // non-finite (or absurd) centre: the parking cell after the grid, which no
+    // stencil visits; the group gets no neighbours and is nobody's neighbour
+    c = cg.ncx * cg.ncy * cg.ncz;
+  }
   d.cell_of[g] = c;
   atomicAdd(&d.cell_cnt[c], 1);
 }
@@ -130,7 +138,7 @@ __global__ void __launch_bounds__(256) k_cell_assign(const ListDesc *__restrict_
 __global__ void __launch_bounds__(1024) k_cell_scan(const ListDesc *__restrict__ D) {
   const ListDesc &d = D[blockIdx.x];
   if (!list_due(d)) return;
-  block_exclusive_scan(d.cell_cnt, d.cell_start, d.cg.ncx * d.cg.ncy * d.cg.ncz);
+  block_exclusive_scan(d.cell_cnt, d.cell_start, d.cg.ncx * d.cg.ncy * d.cg.ncz + 1);
 }
 
 __global__ void __launch_bounds__(256) k_cell_fill(const ListDesc *__restrict__ D) {
@@ -147,7 +155,7 @@ __global__ void __launch_bounds__(256) k_cell_fill(const ListDesc *__restrict__ 
 __global__ void __launch_bounds__(256) k_cell_sort(const ListDesc *__restrict__ D) {
   const ListDesc &d = D[blockIdx.y];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= d.cg.ncx * d.cg.ncy * d.cg.ncz || !list_due(d)) return;
+  if (c > d.cg.ncx * d.cg.ncy * d.cg.ncz || !list_due(d)) return;
   int *items = d.cell_items;
   const int s = d.cell_start[c], n = d.cell_cnt[c];
   for (int i = 1; i < n; ++i) {
@@ -169,6 +177,14 @@ __global__ void __launch_bounds__(256) k_cell_pack(const ListDesc *__restrict__ 
   if (t >= d.ngrp || !list_due(d)) return;
   const int g = d.cell_items[t];
   d.cg.cpos[t] = make_double4(d.xpg[g], d.ypg[g], d.zpg[g], (double)g);
+  if (d.soff) d.soff[t] = d.gsize[g];  // group sizes in cell order; scanned in place below
+}
+
+// first atom of each group in cell order (exclusive prefix of the sizes), one block per system
+__global__ void __launch_bounds__(1024) k_soff_scan(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.x];
+  if (!d.soff || !list_due(d)) return;
+  block_exclusive_scan(d.soff, d.soff, d.ngrp);
 }
 
 __global__ void k_zero_i32(int *p, int n) {
@@ -220,12 +236,14 @@ __device__ inline int wave_excl_scan_int(int v, int lane) {
 // stencil cells l, l+64, ...; write positions come from a wave prefix sum, so
 // the output order is the reference's: stencil order, then descending group
 // index inside a cell.  half: only l > n (the reference list); else every l != n.
+// Entries are group numbers + 1 (sorted = false) or cell-order positions.
 template <bool WRITE>
-__device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, int one_based) {
+__device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, bool sorted) {
   const CellGrid &cg = d.cg;
   const int lane = threadIdx.x & 63;
   const double x1 = d.xpg[n], y1 = d.ypg[n], z1 = d.zpg[n];
   const int c = d.cell_of[n];
+  if (c >= cg.ncx * cg.ncy * cg.ncz) return 0;  // parked (non-finite centre)
   const int i = c / (cg.ncy * cg.ncz), j = (c / cg.ncz) % cg.ncy, k = c % cg.ncz;
   int base = 0;
   for (int m0 = 0; m0 < cg.nsten; m0 += 64) {
@@ -259,13 +277,13 @@ __device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, int one
         int l;
         if (r < 64) {
           hit = (hits >> r) & 1ull;
-          l = hit ? (int)cg.cpos[t].w : 0;
+          l = (hit && !sorted) ? (int)cg.cpos[t].w : 0;
         } else {
           const double4 p = cg.cpos[t];
           l = (int)p.w;
           hit = !(cg.half ? l <= n : l == n) && list_hit(x1, y1, z1, p, cg.co, cg.sqcut);
         }
-        if (hit) out[pos++] = l + one_based;
+        if (hit) out[pos++] = sorted ? t : l + 1;
       }
     }
     base += wave_sum_int(cnt);
@@ -273,12 +291,14 @@ __device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, int one
   return base;
 }
 
+// owners: group order in mode 1, cell order in mode 0 (owner o is group cell_items[o])
 __global__ void __launch_bounds__(256) k_list_count(const ListDesc *__restrict__ D) {
   const ListDesc &d = D[blockIdx.y];
-  const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (n >= d.ngrp || !list_due(d)) return;
-  const int c = list_walk_wave<false>(n, d, nullptr, 0);
-  if ((threadIdx.x & 63) == 0) d.count[n] = c;
+  const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (o >= d.ngrp || !list_due(d)) return;
+  const int n = d.mode == 0 ? d.cell_items[o] : o;
+  const int c = list_walk_wave<false>(n, d, nullptr, false);
+  if ((threadIdx.x & 63) == 0) d.count[o] = c;
 }
 
 // one block per system: offsets, total and the capacity check.  On overflow
@@ -302,21 +322,21 @@ __global__ void __launch_bounds__(1024) k_list_offsets(const ListDesc *__restric
 
 __global__ void __launch_bounds__(256) k_list_write(const ListDesc *__restrict__ D) {
   const ListDesc &d = D[blockIdx.y];
-  const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (n >= d.ngrp || d.total[1] || !list_due(d)) return;
+  const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (o >= d.ngrp || d.total[1] || !list_due(d)) return;
   if (d.mode == 1) {
-    const int o = d.off[n] + n;  // reference layout: one count slot per preceding group
-    const int nv = list_walk_wave<true>(n, d, d.out + o + 1, 1);
-    if ((threadIdx.x & 63) == 0) d.out[o] = nv;
+    const int w = d.off[o] + o;  // reference layout: one count slot per preceding group
+    const int nv = list_walk_wave<true>(o, d, d.out + w + 1, false);
+    if ((threadIdx.x & 63) == 0) d.out[w] = nv;
   } else {
-    list_walk_wave<true>(n, d, d.out + d.off[n], 0);
+    list_walk_wave<true>(d.cell_items[o], d, d.out + d.off[o], true);
   }
 }
 
 void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int phases, hipStream_t st) {
   int max_cell = 1, max_grp = 1;
   for (int s = 0; s < nsys; ++s) {
-    max_cell = std::max(max_cell, h[s].cg.ncx * h[s].cg.ncy * h[s].cg.ncz);
+    max_cell = std::max(max_cell, h[s].cg.ncx * h[s].cg.ncy * h[s].cg.ncz + 1);  // + the parking cell
     max_grp = std::max(max_grp, h[s].ngrp);
   }
   const dim3 b(256), gg((max_grp + 255) / 256, nsys), gc((max_cell + 255) / 256, nsys),
@@ -328,6 +348,7 @@ void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int 
     hipLaunchKernelGGL(k_cell_fill, gg, b, 0, st, d_desc);
     hipLaunchKernelGGL(k_cell_sort, gc, b, 0, st, d_desc);
     hipLaunchKernelGGL(k_cell_pack, gg, b, 0, st, d_desc);
+    hipLaunchKernelGGL(k_soff_scan, dim3(nsys), dim3(1024), 0, st, d_desc);
     hipLaunchKernelGGL(k_list_count, gw, b, 0, st, d_desc);
     hipLaunchKernelGGL(k_list_offsets, dim3(nsys), dim3(1024), 0, st, d_desc);
   }

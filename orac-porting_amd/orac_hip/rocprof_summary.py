@@ -1,0 +1,87 @@
+"""Short text summaries of rocprofv3 CSV output (for the profiles/ folder).
+
+    python -m orac_hip.rocprof_summary stats <kernel_stats.csv> <out.txt> "<profiled command>"
+    python -m orac_hip.rocprof_summary pmc <counter_collection.csv> <out.txt> "<profiled command>"
+
+``stats`` lists every kernel's calls and average/min/max/total duration from
+``rocprofv3 --kernel-trace --stats --output-format csv``.  ``pmc`` averages
+each counter per dispatch and kernel from ``rocprofv3 --pmc ... --output-format
+csv`` (FETCH_SIZE and WRITE_SIZE are in KiB per dispatch).  Column names are
+matched loosely, so minor CSV layout changes between rocprofv3 versions do
+not matter.
+"""
+from __future__ import annotations
+
+import csv
+import sys
+from collections import defaultdict
+
+
+def _col(header, *keys, exclude=()):
+    low = [h.lower() for h in header]
+    for k in keys:
+        for i, h in enumerate(low):
+            if k in h and not any(x in h for x in exclude):
+                return i
+    raise KeyError(f"no column like {keys} in {header}")
+
+
+def stats(path, out, cmd):
+    with open(path, newline="") as f:
+        rows = list(csv.reader(f))
+    h, body = rows[0], rows[1:]
+    iname = _col(h, "name")
+    icalls = _col(h, "calls")
+    itot = _col(h, "totaldurationns", "total")
+    iavg = _col(h, "averagens", "average")
+    imin = _col(h, "minns", "min")
+    imax = _col(h, "maxns", "max")
+    body.sort(key=lambda r: -float(r[itot]))
+    tot = sum(float(r[itot]) for r in body) or 1.0
+    with open(out, "w") as f:
+        f.write(f"command: {cmd}\n")
+        f.write("per-kernel durations from rocprofv3 --kernel-trace --stats, microseconds\n\n")
+        f.write(f"{'calls':>6} {'avg_us':>9} {'min_us':>9} {'max_us':>9} {'total_us':>10} {'pct':>6}  kernel\n")
+        for r in body:
+            t = float(r[itot])
+            f.write(f"{int(float(r[icalls])):6d} {float(r[iavg]) / 1e3:9.2f} {float(r[imin]) / 1e3:9.2f} "
+                    f"{float(r[imax]) / 1e3:9.2f} {t / 1e3:10.1f} {100 * t / tot:6.2f}  {r[iname][:110]}\n")
+
+
+def pmc(path, out, cmd):
+    with open(path, newline="") as f:
+        rows = list(csv.reader(f))
+    h, body = rows[0], rows[1:]
+    iname = _col(h, "kernel_name", "kernel-name", "name", exclude=("counter",))
+    icn = _col(h, "counter_name", "counter-name")
+    icv = _col(h, "counter_value", "counter-value", "value")
+    idisp = None
+    try:
+        idisp = _col(h, "dispatch_id", "dispatch-id", "correlation")
+    except KeyError:
+        pass
+    # sum over the per-dimension rows of one dispatch, then average over dispatches
+    per = defaultdict(float)
+    for r in body:
+        key = (r[iname], r[icn], r[idisp] if idisp is not None else len(per))
+        per[key] += float(r[icv])
+    agg = defaultdict(list)
+    for (name, cn, _), v in per.items():
+        agg[(name, cn)].append(v)
+    with open(out, "w") as f:
+        f.write(f"command: {cmd}\n")
+        f.write("per-kernel counter averages per dispatch from rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE in KiB)\n\n")
+        f.write(f"{'dispatches':>10} {'average':>14}  counter  kernel\n")
+        for (name, cn), vals in sorted(agg.items(), key=lambda kv: -sum(kv[1]) / len(kv[1])):
+            f.write(f"{len(vals):10d} {sum(vals) / len(vals):14.1f}  {cn:10s} {name[:100]}\n")
+
+
+def main(argv):
+    if len(argv) != 5 or argv[1] not in ("stats", "pmc"):
+        sys.exit(__doc__)
+    (stats if argv[1] == "stats" else pmc)(argv[2], argv[3], argv[4])
+    print(open(argv[3]).read())
+
+
+if __name__ == "__main__":
+    main(sys.argv)

@@ -7,7 +7,8 @@ together: each tick is ``chunk`` outer steps of one batch that holds every
 active request, one kernel launch per stage for all of them.  Between ticks
 finished requests leave (their final state is copied out) and waiting
 requests are admitted, in arrival order, while the device bytes of the
-active set stay within the HBM budget.  A request's trajectory does not
+active set stay within the HBM budget (each request is charged the larger of
+its estimated and its actual device bytes).  A request's trajectory does not
 depend on what it is batched with: each system keeps its own step counter
 and list schedule, and forces of already running systems are never
 recomputed when the batch changes.
@@ -44,7 +45,7 @@ def fft_size(n):
         m += 1
 
 
-def _lattice_solute(rng, n, a=1.55, jitter=0.12):
+def _lattice_solute(rng, n, a=2.5, jitter=0.12):
     """n points of a cubic lattice (spacing a) nearest the origin, jittered,
     in snake order so that chain neighbours are lattice neighbours."""
     r = (3.0 * n * a ** 3 / (4.0 * math.pi)) ** (1.0 / 3.0) + 2.0 * a
@@ -63,22 +64,25 @@ def _lattice_solute(rng, n, a=1.55, jitter=0.12):
     return p, radius
 
 
-def serving_request(seed, n_solute, rcut=9.5, alpha=0.35, order=5, shell=11.0, max_bond=2.6, ntypes=64):
+def serving_request(seed, n_solute, rcut=9.5, alpha=0.35, order=5, shell=11.0, max_bond=3.2, ntypes=64,
+                    sigma=(2.0, 2.6)):
     """A solute of n_solute lattice atoms in a cubic box of rigid waters.
 
-    Solute charges ~N(0, 0.35) e (net 0); ntypes LJ types with sigma U[2.0, 3.8] A,
-    eps U[0.02, 0.2] kcal/mol, one drawn per atom (a force field's type count,
-    so the type tables stay small for any solute size); chain 1-2 and 1-3
-    exclusions along the snake (links longer than max_bond A are dropped).
-    The box edge leaves `shell` A of water around the solute sphere; waters
-    on a jittered lattice outside it.
+    The solute sits on a jittered cubic lattice of 2.5 A (0.064 atoms/A^3, a
+    protein's heavy-atom density), so that with LJ sigma U[2.0, 2.6] A it
+    relaxes in a few hundred steepest-descent steps.  Charges ~N(0, 0.35) e
+    (net 0); ntypes LJ types (sigma as above, eps U[0.02, 0.2] kcal/mol), one
+    drawn per atom, so the type tables stay small for any solute size; chain
+    1-2 and 1-3 exclusions along the snake (links longer than max_bond A are
+    dropped).  The box edge leaves `shell` A of water around the solute
+    sphere; waters on a jittered lattice outside it.
     """
     rng = np.random.default_rng(seed)
     pos, radius = _lattice_solute(rng, n_solute)
     q = rng.normal(0.0, 0.35, n_solute)
     q -= q.mean()
     nt = min(ntypes, n_solute)
-    sig = rng.uniform(2.0, 3.8, nt)
+    sig = rng.uniform(sigma[0], sigma[1], nt)
     eps = rng.uniform(0.02, 0.2, nt)
     types = rng.integers(0, nt, n_solute)
     link = np.sqrt(((pos[1:] - pos[:-1]) ** 2).sum(axis=1)) < max_bond
@@ -169,6 +173,7 @@ class Server:
         self.dt, self.ninner, self.list_every, self.precision = dt, ninner, list_every, precision
         self.autotune, self.clock = autotune, clock
         self.log = []
+        self.progress = None  # optional callable(str): admission and completion events
 
     def run(self, requests):
         """Serve the requests (arrival times relative to the start); returns stats."""
@@ -189,8 +194,12 @@ class Server:
                 req = queue.pop(0)
                 req.admitted = self.clock() - t0
                 a = _Active(self.ctx, req, self.precision)
-                used += a.req.device_bytes
+                a.charge = max(need, req.device_bytes)  # the larger of the estimate and the bytes held
+                used += a.charge
                 active.append(a)
+                if self.progress:
+                    self.progress(f"t={req.admitted:.3f}s admit request {req.rid} ({req.system.natoms} atoms), "
+                                  f"{len(active)} active, {used / 2**30:.2f} GiB charged")
                 changed = True
             if not active:
                 if pending:
@@ -217,11 +226,14 @@ class Server:
                     r.final_positions, r.final_velocities = a.dyn.state()
                     r.energies = a.dyn.energies()
                     r.finished = self.clock() - t0
+                    if self.progress:
+                        self.progress(f"t={r.finished:.3f}s done request {r.rid}: latency {r.latency * 1e3:.0f} ms, "
+                                      f"T {r.energies['temperature']:.0f} K")
                 active = [a for a in active if a.left > 0]
                 batch.close()
                 batch = None
                 for a in done:
-                    used -= a.req.device_bytes
+                    used -= a.charge
                     a.close()
         wall = self.clock() - t0
         lat = np.array([r.latency for r in requests])
