@@ -70,6 +70,9 @@ def evaluated_pairs(s):
 
 
 def corrections(s):
+    """Per-step reciprocal corrections of the solute's flexible 1-2 and 1-3 pairs."""
+    if s.bonds is None:
+        return np.zeros((2, 0), np.int32), np.zeros(0)
     pairs = np.concatenate([s.bonds, s.bends], axis=1)
     return pairs, np.ones(pairs.shape[1])
 
@@ -130,7 +133,9 @@ def time_batched(ctx, reps, precision, steps, warmup, tune=True):
     return dict(dt=dt, res0=res0, stages=stages, tuning=tuning, tune_ms=tune_ms, device_bytes=mem)
 
 
-def prepare_md(ctx, systems, precision, relax=200, temperature=300.0, seed=2024):
+def prepare_md(ctx, systems, precision, relax=300, equil=200, temperature=300.0, seed=2024):
+    """Steepest descent, Maxwell-Boltzmann velocities, then velocity rescaling to
+    the temperature (every 10 steps) while the strained start relaxes."""
     from orac_hip import md
     from orac_hip.api import Batch
     rss = [resident(ctx, s, precision) for s in systems]
@@ -139,6 +144,7 @@ def prepare_md(ctx, systems, precision, relax=200, temperature=300.0, seed=2024)
     md.relax_batch(b, relax, dmax=0.03)
     for k, d in enumerate(dyns):
         d.maxwell_boltzmann(temperature, seed + k)
+    md.equilibrate(b, dyns, temperature, equil)
     return rss, dyns, b
 
 
@@ -188,6 +194,34 @@ def serving_leg(ctx, nreq=16, rate=200.0, nsteps=500, budget_gib=64, seed=4242):
     return st
 
 
+def c0_leg(ctx, steps):
+    """The 648-atom water box: one FP64 evaluation against the mixed one, and
+    100 outer r-RESPA steps in FP64 (ns/day, NVE drift)."""
+    from orac_hip import synth
+    from orac_hip.api import ResidentSystem
+    w = synth.config("water216")
+    res = {}
+    for prec in (0, 1):
+        rs = ResidentSystem(ctx, w, precision=prec)
+        rs.eval(rebuild_list=True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rs.eval(rebuild_list=True)
+        ctx.sync()
+        res[prec] = (time.perf_counter() - t0, rs.results())
+        rs.close()
+    rss, dyns, b = prepare_md(ctx, [w], 0, equil=100)
+    wall = time_md(ctx, b, 100, 5, list_every=5)
+    drift = nve_drift(ctx, b, dyns[0], 100, list_every=5)
+    b.close()
+    rss[0].close()
+    return {"workload": "216 rigid waters (648 atoms), L=18.64 A, PME 20^3 order 5, alpha 0.40, rc 8.5 A",
+            "fp64_evals_per_s": steps / res[0][0], "mixed_evals_per_s": steps / res[1][0],
+            "force_relrms_mixed_vs_f64": relrms(res[1][1]["f"], res[0][1]["f"]),
+            "md_100_steps": {"ns_per_day": ns_per_day(100, wall), "ms_per_step": 10 * wall, "nve": drift}}
+
+
 def cpu_baseline(s, corr, nthreads):
     import oracle as O
     rc = s.rcut + s.rtol + s.rnei
@@ -209,6 +243,32 @@ def cpu_baseline(s, corr, nthreads):
             break
     dt = time.perf_counter() - t0
     return nsample / dt, nsample, dict(f=f, eer=p["eer"])
+
+
+def cpu_reference(s, corr, budget_s=20.0):
+    """The reference's own Fortran (oracle/_ref, compiled in place from the
+    reference sources, serial, fftpack branch) on the same evaluation: list,
+    force, fft_pme, ferrf; as many evaluations as fit in budget_s (at least one)."""
+    import oracle as O
+    if not O.ref_available():
+        return None
+    rc = s.rcut + s.rtol + s.rnei
+    fa = O.ForceArgs(s)
+    nc = int(s.L // 3.0)
+    nsample, f = 0, None
+    t0 = time.perf_counter()
+    while True:
+        st = O.ref_lc_index(nc, nc, nc, rc, s.co)
+        nnlpp = O.ref_lc_list(nc, nc, nc, st, rc, s.co, fa.xpg, fa.ypg, fa.zpg)
+        d = O.ref_force(s, nnlpp, rinn=0.0, rout=s.rcut, rtolinn=0.0, rtolout=s.rtol, rneigh=0.0, fa=fa)
+        p = O.ref_fft_pme(s)
+        c = O.ref_ferrf(s, corr[0], 1, 1.0)
+        f = d["f"] + p["f"] + c["f"]
+        nsample += 1
+        if time.perf_counter() - t0 > budget_s / 2:
+            break
+    dt = time.perf_counter() - t0
+    return nsample / dt, nsample, f
 
 
 def relrms(a, b):
@@ -272,6 +332,8 @@ def main():
             log(f"batched fp64 x{args.replicas}: {1e3 * bd['dt'] / bsteps:.3f} ms/step")
             batched = dict(bm=bm, bd=bd, bsteps=bsteps, err=relrms(bm["res0"]["f"], f0),
                            err64=relrms(bd["res0"]["f"], f0))
+        c0 = c0_leg(ctx, args.steps)
+        log(f"water216: {c0['fp64_evals_per_s']:.0f} FP64 evals/s, md {c0['md_100_steps']['ms_per_step']:.3f} ms/step")
         dyn = None
         if args.md_steps > 0:
             K = args.md_steps
@@ -350,6 +412,7 @@ def main():
             "speedup_mixed_batched_vs_plain_fp64": agg / evals64,
             "force_relrms_mixed_vs_f64": batched["err"], "force_relrms_batched_f64_vs_alone": batched["err64"],
             "tuning": bm["tuning"], "stage_ms_mixed": rnd(bm["stages"]), "device_bytes": bm["device_bytes"]}
+    out["water216"] = c0
     if dyn:
         dyn["workload"] = ("NVE r-RESPA: 2 fs outer step, 4 inner steps for the solute harmonic terms, rigid waters "
                            "(SHAKE/RATTLE), list every 10 steps; after 200 steepest-descent steps and "
@@ -367,6 +430,13 @@ def main():
                                "sample": f"{nsample} full FP64 evaluations (list + force + fft_pme + ferrf) of the "
                                          f"same 24,460-atom system, OpenMP C oracle",
                                "gpu_vs_cpu_force_relrms": relrms(f64, cref["f"])}
+        ref = cpu_reference(s, corr)
+        if ref:
+            out["cpu_baseline"]["reference_serial"] = {
+                "value": ref[0], "unit": "force evals/s", "cores": 1, "kind": "reference",
+                "sample": f"{ref[1]} evaluations by the reference Fortran compiled in place (serial, fftpack "
+                          "branch; FFTW is not in the image)",
+                "gpu_vs_reference_force_relrms": relrms(f64, ref[2])}
     print(json.dumps(out), flush=True)
 
 

@@ -33,11 +33,13 @@ struct System {
   ListDesc list;
   CorrDesc corr;
   size_t list_cap = 0;
+  int slab = 0;           // neighbour-list entries per owner (slab layout)
   double *xp = nullptr, *yp = nullptr, *zp = nullptr;
   double *fx = nullptr, *fy = nullptr, *fz = nullptr;
   double *out = nullptr;  // device: pme[20], pairs[15] at 20, corrections[4] at 36
   double self_e[2] = {0, 0};
   bool have_list = false;
+  int list_fmt = -1;      // cell_order tuning the current list was built with
   int solo = 0;           // id of this system's own one-system batch (0: none yet)
   int gen = 0;            // bumped when a device buffer behind the descriptors moves
   // molecular dynamics (set up by orac_hip_md_setup)
@@ -314,7 +316,10 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
     const int ncell = ncx * ncy * ncz;
     const double dens = ng / S.volume;
     const double nb = 4.0 / 3.0 * M_PI * rl * rl * rl * dens;
-    S.list_cap = std::min<size_t>((size_t)(ng * (1.6 * nb + 64.0)), (size_t)ng * (size_t)ng);
+    // slab layout: room for twice the mean neighbour count per owner until the
+    // first coordinates give the real largest count (size_list)
+    S.slab = std::min(ng, ((int)(2.0 * nb) + 64 + 31) / 32 * 32);
+    S.list_cap = (size_t)ng * (size_t)S.slab;
     L.ngrp = ng;
     L.xpg = xpg;
     L.ypg = ypg;
@@ -328,16 +333,19 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
     L.off = B.get<int>("nboff", ng + 1);
     L.out = B.get<int>("nbidx", S.list_cap);
     L.cap = (long long)S.list_cap;
-    L.total = B.get<int>("ltotal", 2);
+    L.total = B.get<int>("ltotal", 4);
     L.mode = 0;
+    L.slab = S.slab;
     std::vector<int> gsize(ng);
     for (int g = 0; g < ng; ++g) gsize[g] = glast[g] - gfirst[g] + 1;
     L.gsize = B.put_vec("gsize", gsize, st, false);
     L.soff = B.get<int>("soff", ng);
-    launch_zero_u64(reinterpret_cast<unsigned long long *>(L.total), 1, st);
+    launch_zero_u64(reinterpret_cast<unsigned long long *>(L.total), 2, st);
   }
   p.nb_off = L.off;
   p.nb_idx = L.out;
+  p.nb_cnt = L.count;
+  p.nb_slab = L.slab;
   p.perm = L.cell_items;  // cell order of the last list build
   p.soff = L.soff;
 
@@ -438,6 +446,11 @@ static void batch_upload(orac_hip_ctx *ctx, Batch &b) {
     pme_apply_tuning(b.hp[k], b.tu);
     b.hq[k] = S.pair;
     b.hl[k] = S.list;
+    b.hl[k].cellorder = b.tu.cell_order ? 1 : 0;
+    if (!b.tu.cell_order) {
+      b.hq[k].perm = nullptr;  // owners and neighbour records in group order
+      b.hq[k].soff = nullptr;
+    }
     b.hc[k] = S.corr;
     if (S.corr.npairs > 0) b.any_corr = true;
   }
@@ -518,7 +531,7 @@ static void batch_enqueue(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool events
 static void batch_eval(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool use_graph) {
   ensure_fresh(ctx, b);
   for (System *S : b.sys)
-    if (!S->have_list) rebuild = true;
+    if (!S->have_list || S->list_fmt != b.tu.cell_order) rebuild = true;
   hipStream_t st = ctx->stream;
   if (use_graph && !b.profile) {
     const int k = rebuild ? 1 : 0;
@@ -541,7 +554,10 @@ static void batch_eval(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool use_graph
   }
   ORAC_HIP(hipGetLastError());
   if (rebuild)
-    for (System *S : b.sys) S->have_list = true;
+    for (System *S : b.sys) {
+      S->have_list = true;
+      S->list_fmt = b.tu.cell_order;
+    }
 }
 
 static int solo_batch(orac_hip_ctx *ctx, int32_t sys) {
@@ -551,9 +567,9 @@ static int solo_batch(orac_hip_ctx *ctx, int32_t sys) {
   return S.solo;
 }
 
-// Size the neighbour list from the pair count at the current coordinates:
-// capacity = 1.3 x count + 64 per group (at least the estimate made at
-// creation).  Run whenever the host sets coordinates.
+// Size the neighbour list from the counts at the current coordinates: room
+// per owner for 1.25 x the largest count + 32 (at least the creation estimate).
+// Run whenever the host sets coordinates.
 static void size_list(orac_hip_ctx *ctx, int32_t id) {
   System &S = get_sys(ctx, id);
   Batch &b = get_batch(ctx, solo_batch(ctx, id));
@@ -561,18 +577,22 @@ static void size_list(orac_hip_ctx *ctx, int32_t id) {
   hipStream_t st = ctx->stream;
   launch_centres(b.dq, b.hq.data(), 1, st);
   launch_list_build(b.dl, b.hl.data(), 1, LIST_COUNT, st);
-  int tot[2] = {0, 0};
+  int tot[4] = {0, 0, 0, 0};
   ORAC_HIP(hipMemcpyAsync(tot, S.list.total, sizeof tot, hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipStreamSynchronize(st));
-  const size_t ng = (size_t)S.ngrp;
-  const size_t need = std::min(ng * ng, (size_t)(1.3 * tot[0]) + 64 * ng);
-  S.have_list = false;  // the count pass rewrote the offsets
-  if (need > S.list_cap) {
-    ORAC_REQUIRE(need < (size_t)INT32_MAX, E_CAPACITY, "neighbour list larger than 2^31 entries");
-    S.list_cap = need;
-    S.list.out = S.B.get<int>("nbidx", need);
-    S.list.cap = (long long)need;
+  const int ng = S.ngrp;
+  const int need = std::min(ng, ((int)(1.25 * tot[2]) + 32 + 31) / 32 * 32);
+  S.have_list = false;  // the count pass rewrote the counts
+  if (need > S.slab) {
+    ORAC_REQUIRE((size_t)ng * (size_t)need < (size_t)INT32_MAX, E_CAPACITY,
+                 "neighbour list larger than 2^31 entries");
+    S.slab = need;
+    S.list_cap = (size_t)ng * (size_t)need;
+    S.list.out = S.B.get<int>("nbidx", S.list_cap);
+    S.list.cap = (long long)S.list_cap;
+    S.list.slab = need;
     S.pair.nb_idx = S.list.out;
+    S.pair.nb_slab = need;
     S.gen++;
   }
 }
@@ -799,6 +819,13 @@ static void md_run(orac_hip_ctx *ctx, Batch &b, int nsteps, double dt, int ninne
   ORAC_REQUIRE(nsteps >= 0 && dt > 0.0 && ninner >= 1 && list_every >= 1, E_ARG, "bad integration parameters");
   md_upload(ctx, b, list_every);
   md_prime(ctx, b);
+  // lists built in another layout are rebuilt (and the forces with them)
+  bool fmt = false;
+  for (System *S : b.sys) fmt = fmt || S->list_fmt != b.tu.cell_order;
+  if (fmt) {
+    launch_md_fast_forces(b.dm, b.hm.data(), (int)b.sys.size(), ctx->stream);
+    batch_eval(ctx, b, true, false);
+  }
   if (b.md_dt != dt || b.md_ninner != ninner) {
     b.drop_md_graphs();
     b.md_dt = dt;
@@ -1016,6 +1043,7 @@ static Tuning tuning_from(const int32_t *t, int32_t n, Tuning base) {
   if (n > 1) base.pair_waves = t[1];
   if (n > 2) base.spread_tiled = t[2];
   if (n > 3) base.xcd_remap = t[3];
+  if (n > 4) base.cell_order = t[4];
   ORAC_REQUIRE(base.lds_elems >= 64 && base.lds_elems <= 2048 && (base.lds_elems & (base.lds_elems - 1)) == 0, E_ARG,
                "lds_elems must be a power of two in [64, 2048]");
   ORAC_REQUIRE(base.pair_waves >= 0 && base.pair_waves <= 4 && base.pair_waves != 1, E_ARG,
@@ -1024,8 +1052,8 @@ static Tuning tuning_from(const int32_t *t, int32_t n, Tuning base) {
 }
 
 static void tuning_to(const Tuning &tu, int32_t *t, int32_t n) {
-  const int32_t v[4] = {tu.lds_elems, tu.pair_waves, tu.spread_tiled, tu.xcd_remap};
-  for (int k = 0; k < n && k < 4; ++k) t[k] = v[k];
+  const int32_t v[5] = {tu.lds_elems, tu.pair_waves, tu.spread_tiled, tu.xcd_remap, tu.cell_order};
+  for (int k = 0; k < n && k < 5; ++k) t[k] = v[k];
 }
 
 extern "C" int orac_hip_batch_set_tuning(orac_hip_ctx *ctx, int32_t batch, const int32_t *tuning, int32_t n) {
@@ -1062,13 +1090,14 @@ extern "C" int orac_hip_batch_autotune(orac_hip_ctx *ctx, int32_t batch, int32_t
     Tuning best = b.tu;
     batch_upload(ctx, b);
     double tbest = batch_time(ctx, b, r, rebuild);
-    const std::vector<int> lds = {256, 512, 1024}, waves = {2, 3, 4}, tiled = {1, 0}, remap = {1, 0};
+    const std::vector<int> cands[5] = {{256, 512, 1024}, {2, 3, 4}, {1, 0}, {1, 0}, {0, 1}};
     for (int sweep = 0; sweep < 2; ++sweep) {
-      for (int param = 0; param < 4; ++param) {
-        const std::vector<int> &cand = param == 0 ? lds : param == 1 ? waves : param == 2 ? tiled : remap;
+      for (int param = 0; param < 5; ++param) {
+        const std::vector<int> &cand = cands[param];
         for (int v : cand) {
           Tuning t = best;
-          int &slot = param == 0 ? t.lds_elems : param == 1 ? t.pair_waves : param == 2 ? t.spread_tiled : t.xcd_remap;
+          int *slots[5] = {&t.lds_elems, &t.pair_waves, &t.spread_tiled, &t.xcd_remap, &t.cell_order};
+          int &slot = *slots[param];
           if (slot == v) continue;
           slot = v;
           b.tu = t;

@@ -65,6 +65,29 @@ __device__ inline bool is_excluded(const int *ex_off, const int *ex_idx, int a, 
   return false;
 }
 
+// Pair arithmetic built on one reciprocal square root per pair.  FP64: exact
+// divide and sqrt (results agree with the reference's expressions to a few
+// ulp).  FP32 (mixed precision): the hardware reciprocal, reciprocal square
+// root and exponential, 1-2 ulp each.
+template <typename T>
+__device__ inline T pair_rsqrt(T x);
+template <>
+__device__ inline double pair_rsqrt<double>(double x) { return 1.0 / sqrt(x); }
+template <>
+__device__ inline float pair_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <typename T>
+__device__ inline T pair_rcp(T x);
+template <>
+__device__ inline double pair_rcp<double>(double x) { return 1.0 / x; }
+template <>
+__device__ inline float pair_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <typename T>
+__device__ inline T pair_exp(T x);
+template <>
+__device__ inline double pair_exp<double>(double x) { return exp(x); }
+template <>
+__device__ inline float pair_exp<float>(float x) { return __expf(x); }
+
 __device__ inline void lj_params(const PairDesc &d, int ti, int tj, double &c12, double &c6) {
   if (d.lj_dense) {
     double2 v = d.lj_dense[(size_t)ti * d.ntypes + tj];
@@ -82,6 +105,7 @@ template <typename T, int W>
 __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D, int remap) {
   __shared__ double red[PAIR_WAVES][16];
   __shared__ double facc[PAIR_WAVES][64][3];
+  __shared__ double acc[PAIR_WAVES][15][64];  // per-lane energy and virial sums (lane-private slots)
   int sys = blockIdx.y, blk = blockIdx.x;
   if (remap) {
     const int l = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -93,8 +117,10 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int t = blk * PAIR_WAVES + wid;  // owner group, sorted position
   double fxa = 0.0, fya = 0.0, fza = 0.0;
-  double en[6] = {0, 0, 0, 0, 0, 0};  // uconf[typeij], ucoul[typeij]
-  double st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double *en = &acc[wid][0][0];  // en[k * 64 + lane]: uconf[typeij], ucoul[typeij]
+  double *st = &acc[wid][6][0];  // st[k * 64 + lane]
+#pragma unroll
+  for (int k = 0; k < 15; ++k) acc[wid][k][lane] = 0.0;
   int ia0 = 0, nI = 1, nslot = 64;  // ia0: owner's first atom, original numbering
   if (t < d.ngrp) {
     const int4 gi = d.sgi[t];
@@ -113,7 +139,14 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
     const double4 gpi = d.sgp[t];
     const double xpgi = gpi.x, ypgi = gpi.y, zpgi = gpi.z;
     const double co0 = d.co[0], co3 = d.co[3], co6 = d.co[6], co4 = d.co[4], co7 = d.co[7], co8 = d.co[8];
-    const int nb0 = d.nb_off[t], nb1 = d.nb_off[t + 1];
+    int nb0, nb1;
+    if (d.nb_slab > 0) {
+      nb0 = t * d.nb_slab;
+      nb1 = nb0 + min(d.nb_cnt[t], d.nb_slab);
+    } else {
+      nb0 = d.nb_off[t];
+      nb1 = d.nb_off[t + 1];
+    }
     int tjn = (active && nb0 + slot < nb1) ? d.nb_idx[nb0 + slot] : t;
     for (int base = nb0; base < nb1; base += nslot) {
       const int jj = base + slot;
@@ -198,7 +231,8 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
           const T yc = T(co4 * yg + co7 * zg);
           const T zc = T(co8 * zg);
           const T rsq = xc * xc + yc * yc + zc * zc;
-          const T rsqi = T(1) / rsq;
+          const T rinv = pair_rsqrt<T>(rsq);
+          const T rsqi = rinv * rinv;
           T qforce = T(0);
           double vpair = 0.0;
           if (c6 > 1e-12) {
@@ -207,26 +241,25 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
             double conf = (double)(T(c12) * r12 - T(c6) * r6);
             vpair += conf;
             if (cls == 1) uconfa += conf;
-            else if (fwd) en[typeij] += sw * conf;
+            else if (fwd) en[typeij * 64 + lane] += sw * conf;
           }
           if (!d.lskip_ewald) {
-            T rsp = sqrt(rsq);
-            T rspqi = rsqi / rsp;
+            T rsp = rsq * rinv;
+            T rspqi = rsqi * rinv;
             T alphar = T(d.alpha) * rsp;
-            T qt = T(1) / (T(1) + T(AS_QP) * alphar);
-            T expcst = exp(-(alphar * alphar));
+            T qt = pair_rcp<T>(T(1) + T(AS_QP) * alphar);
+            T expcst = pair_exp<T>(-(alphar * alphar));
             T erfcst = ((((T(AS_A5) * qt + T(AS_A4)) * qt + T(AS_A3)) * qt + T(AS_A2)) * qt + T(AS_A1)) * qt * expcst;
             T furpar = T(pa4.w * pb4.w);
-            double vel = (double)(furpar * erfcst / rsp);
+            double vel = (double)(furpar * erfcst * rinv);
             vpair += vel;
             if (cls == 1) ucoula += vel;
-            else if (fwd) en[3 + typeij] += sw * vel;
+            else if (fwd) en[(3 + typeij) * 64 + lane] += sw * vel;
             qforce += furpar * (erfcst + T(d.twrtpi) * alphar * expcst) * rspqi;
           } else if (!d.ewald) {
-            T rsp = sqrt(rsq);
-            T ssv = T(pa4.w * pb4.w) / rsp;
+            T ssv = T(pa4.w * pb4.w) * rinv;
             vpair += (double)ssv;
-            if (fwd) en[3 + typeij] += (cls == 1 ? 1.0 : sw) * (double)ssv;
+            if (fwd) en[(3 + typeij) * 64 + lane] += (cls == 1 ? 1.0 : sw) * (double)ssv;
             qforce += ssv * rsqi;
           }
           double qf = (double)qforce;
@@ -243,8 +276,8 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
         fya += fgy;
         fza += fgz;
         if (fwd && cls == 1) {
-          en[typeij] += uconfa;
-          en[3 + typeij] += ucoula;
+          en[typeij * 64 + lane] += uconfa;
+          en[(3 + typeij) * 64 + lane] += ucoula;
         }
       }
       // total V of the group pair over the slot's lanes (every lane shuffles)
@@ -263,9 +296,9 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
         }
       }
       if (fwd && shifted) {
-        st[0] -= fgx * sxm; st[1] -= fgx * sym; st[2] -= fgx * szm;
-        st[3] -= fgy * sxm; st[4] -= fgy * sym; st[5] -= fgy * szm;
-        st[6] -= fgz * sxm; st[7] -= fgz * sym; st[8] -= fgz * szm;
+        st[0 * 64 + lane] -= fgx * sxm; st[1 * 64 + lane] -= fgx * sym; st[2 * 64 + lane] -= fgx * szm;
+        st[3 * 64 + lane] -= fgy * sxm; st[4 * 64 + lane] -= fgy * sym; st[5 * 64 + lane] -= fgy * szm;
+        st[6 * 64 + lane] -= fgz * sxm; st[7 * 64 + lane] -= fgz * sym; st[8 * 64 + lane] -= fgz * szm;
       }
     }
   }
@@ -287,20 +320,15 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
     d.fz[a] = sz;
     const int pa = d.atomp[a];
     const double cx = d.xpcm[pa], cy = d.ypcm[pa], cz = d.zpcm[pa];
-    st[0] += sx * cx; st[1] += sx * cy; st[2] += sx * cz;
-    st[3] += sy * cx; st[4] += sy * cy; st[5] += sy * cz;
-    st[6] += sz * cx; st[7] += sz * cy; st[8] += sz * cz;
+    st[0 * 64 + lane] += sx * cx; st[1 * 64 + lane] += sx * cy; st[2 * 64 + lane] += sx * cz;
+    st[3 * 64 + lane] += sy * cx; st[4 * 64 + lane] += sy * cy; st[5 * 64 + lane] += sy * cz;
+    st[6 * 64 + lane] += sz * cx; st[7 * 64 + lane] += sz * cy; st[8 * 64 + lane] += sz * cz;
   }
   // energies and virial: wave then block, fixed order
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    double v = wave_sum(en[k]);
+  for (int k = 0; k < 15; ++k) {
+    double v = wave_sum(acc[wid][k][lane]);
     if (lane == 0) red[wid][k] = v;
-  }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    double v = wave_sum(st[k]);
-    if (lane == 0) red[wid][6 + k] = v;
   }
   __syncthreads();
   if (threadIdx.x < 15) {

@@ -17,6 +17,7 @@ struct Tuning {
   int pair_waves = 0;   // occupancy target of the pair kernel (0: default for the precision)
   int spread_tiled = 1; // LDS-tiled spread where the grid allows it, else global atomics
   int xcd_remap = 1;    // keep each system's pair blocks on one XCD (L2 locality)
+  int cell_order = 0;   // pair kernel owners and neighbour records in cell order (1) or group order (0)
 };
 
 // ---------------------------------------------------------------- reciprocal space
@@ -51,7 +52,9 @@ struct PairDesc {
   const unsigned char *ex_inter;    // atom has an excluded partner outside its own group
   int any_heavy;
   double massinfty;
-  const int *nb_off, *nb_idx;       // full neighbour list (CSR): owners and entries are sorted positions
+  const int *nb_off, *nb_idx;       // full neighbour list (CSR), owners and entries in the list's order
+  const int *nb_cnt;                // slab layout (nb_slab > 0): owner t's entries at nb_idx[t * nb_slab],
+  int nb_slab;                      //   nb_cnt[t] of them (at most nb_slab are stored)
   double co[9];
   double alpha, twrtpi;
   double r2inn, r2out, rinn0, r2inn0, rout0, r2out0, arsout1, arsout2, arsinn1, arsinn2;
@@ -103,14 +106,19 @@ struct ListDesc {
   int *off;                  // [ngrp+1] CSR offsets (mode 0) / prefix of count (mode 1)
   int *out;                  // list entries
   long long cap;             // capacity of out (ints, count slots included in mode 1)
-  int *total;                // [0] entries (count slots excluded), [1] overflow flag
+  int *total;                // [0] entries (count slots excluded), [1] overflow flag, [2] largest count
   int mode;                  // 0: CSR, 0-based; 1: reference layout per group [nv, l1..lnv], 1-based
   const int *step;           // optional step counter of the system: the build runs only
   int every;                 //   when *step % every == 0 (device-side update schedule)
-  // mode 0 only: owners are taken in cell order (cell_items) and entries are
-  // cell-order positions; soff[t] = first atom of the t-th group in that order
+  // mode 0 only.  cellorder = 1: owners are taken in cell order (cell_items)
+  // and entries are cell-order positions, soff[t] = first atom of the t-th
+  // group in that order; cellorder = 0: owners and entries are group numbers.
+  int cellorder;
   const int *gsize;          // [ngrp] atoms per group (original numbering)
   int *soff;
+  // mode 0 only.  slab > 0: one pass writes owner o's entries at out[o * slab]
+  // and its count at count[o]; a count above slab sets the overflow flag.
+  int slab;
 };
 enum ListPhase { LIST_COUNT = 1, LIST_WRITE = 2 };
 // LIST_COUNT: cells, per-group counts, offsets and the capacity check;

@@ -177,13 +177,13 @@ __global__ void __launch_bounds__(256) k_cell_pack(const ListDesc *__restrict__ 
   if (t >= d.ngrp || !list_due(d)) return;
   const int g = d.cell_items[t];
   d.cg.cpos[t] = make_double4(d.xpg[g], d.ypg[g], d.zpg[g], (double)g);
-  if (d.soff) d.soff[t] = d.gsize[g];  // group sizes in cell order; scanned in place below
+  if (d.soff && d.cellorder) d.soff[t] = d.gsize[g];  // group sizes in cell order; scanned in place below
 }
 
 // first atom of each group in cell order (exclusive prefix of the sizes), one block per system
 __global__ void __launch_bounds__(1024) k_soff_scan(const ListDesc *__restrict__ D) {
   const ListDesc &d = D[blockIdx.x];
-  if (!d.soff || !list_due(d)) return;
+  if (!d.soff || !d.cellorder || !list_due(d)) return;
   block_exclusive_scan(d.soff, d.soff, d.ngrp);
 }
 
@@ -236,9 +236,9 @@ __device__ inline int wave_excl_scan_int(int v, int lane) {
 // stencil cells l, l+64, ...; write positions come from a wave prefix sum, so
 // the output order is the reference's: stencil order, then descending group
 // index inside a cell.  half: only l > n (the reference list); else every l != n.
-// Entries are group numbers + 1 (sorted = false) or cell-order positions.
+// Entries: group number + 1 (entry 0), cell-order position (1), group number (2).
 template <bool WRITE>
-__device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, bool sorted) {
+__device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, int entry, int cap = 0x7fffffff) {
   const CellGrid &cg = d.cg;
   const int lane = threadIdx.x & 63;
   const double x1 = d.xpg[n], y1 = d.ypg[n], z1 = d.zpg[n];
@@ -277,13 +277,16 @@ __device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, bool so
         int l;
         if (r < 64) {
           hit = (hits >> r) & 1ull;
-          l = (hit && !sorted) ? (int)cg.cpos[t].w : 0;
+          l = (hit && entry != 1) ? (int)cg.cpos[t].w : 0;
         } else {
           const double4 p = cg.cpos[t];
           l = (int)p.w;
           hit = !(cg.half ? l <= n : l == n) && list_hit(x1, y1, z1, p, cg.co, cg.sqcut);
         }
-        if (hit) out[pos++] = sorted ? t : l + 1;
+        if (hit) {
+          if (pos < cap) out[pos] = entry == 1 ? t : (entry == 0 ? l + 1 : l);
+          ++pos;
+        }
       }
     }
     base += wave_sum_int(cnt);
@@ -291,13 +294,13 @@ __device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, bool so
   return base;
 }
 
-// owners: group order in mode 1, cell order in mode 0 (owner o is group cell_items[o])
+// owners: group order, or cell order in mode 0 with cellorder (owner o is group cell_items[o])
 __global__ void __launch_bounds__(256) k_list_count(const ListDesc *__restrict__ D) {
   const ListDesc &d = D[blockIdx.y];
   const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (o >= d.ngrp || !list_due(d)) return;
-  const int n = d.mode == 0 ? d.cell_items[o] : o;
-  const int c = list_walk_wave<false>(n, d, nullptr, false);
+  const int n = (d.mode == 0 && d.cellorder) ? d.cell_items[o] : o;
+  const int c = list_walk_wave<false>(n, d, nullptr, 0);
   if ((threadIdx.x & 63) == 0) d.count[o] = c;
 }
 
@@ -326,10 +329,39 @@ __global__ void __launch_bounds__(256) k_list_write(const ListDesc *__restrict__
   if (o >= d.ngrp || d.total[1] || !list_due(d)) return;
   if (d.mode == 1) {
     const int w = d.off[o] + o;  // reference layout: one count slot per preceding group
-    const int nv = list_walk_wave<true>(o, d, d.out + w + 1, false);
+    const int nv = list_walk_wave<true>(o, d, d.out + w + 1, 0);
     if ((threadIdx.x & 63) == 0) d.out[w] = nv;
+  } else if (d.slab > 0) {
+    const int c = list_walk_wave<true>(d.cellorder ? d.cell_items[o] : o, d, d.out + (size_t)o * d.slab,
+                                       d.cellorder ? 1 : 2, d.slab);
+    if ((threadIdx.x & 63) == 0) d.count[o] = c;
+  } else if (d.cellorder) {
+    list_walk_wave<true>(d.cell_items[o], d, d.out + d.off[o], 1);
   } else {
-    list_walk_wave<true>(d.cell_items[o], d, d.out + d.off[o], true);
+    list_walk_wave<true>(o, d, d.out + d.off[o], 2);
+  }
+}
+
+// slab layout, one block per system: total and largest count (fixed order),
+// overflow when a count exceeds the slab
+__global__ void __launch_bounds__(1024) k_list_total(const ListDesc *__restrict__ D) {
+  __shared__ int wmax[16];
+  const ListDesc &d = D[blockIdx.x];
+  if (!list_due(d)) return;
+  const int n = d.ngrp;
+  const int tot = block_exclusive_scan(d.count, d.off, n);
+  int mx = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) mx = max(mx, d.count[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < 16; ++w) mx = max(mx, wmax[w]);
+    d.off[n] = tot;
+    d.total[0] = tot;
+    d.total[1] = mx > d.slab ? 1 : 0;
+    d.total[2] = mx;
   }
 }
 
@@ -341,6 +373,10 @@ void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int 
   }
   const dim3 b(256), gg((max_grp + 255) / 256, nsys), gc((max_cell + 255) / 256, nsys),
       gw((max_grp + 3) / 4, nsys);
+  // a batch uses one layout: all slab (resident systems) or none (plain lc_list)
+  const bool slab = h[0].slab > 0;
+  for (int s = 1; s < nsys; ++s)
+    if ((h[s].slab > 0) != slab) throw Error(E_STATE, "list layouts differ within a batch");
   if (phases & LIST_COUNT) {
     hipLaunchKernelGGL(k_list_zero, dim3(std::min(64, (max_cell + 255) / 256), nsys), b, 0, st, d_desc);
     hipLaunchKernelGGL(k_cell_assign, gg, b, 0, st, d_desc);
@@ -349,10 +385,19 @@ void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int 
     hipLaunchKernelGGL(k_cell_sort, gc, b, 0, st, d_desc);
     hipLaunchKernelGGL(k_cell_pack, gg, b, 0, st, d_desc);
     hipLaunchKernelGGL(k_soff_scan, dim3(nsys), dim3(1024), 0, st, d_desc);
-    hipLaunchKernelGGL(k_list_count, gw, b, 0, st, d_desc);
-    hipLaunchKernelGGL(k_list_offsets, dim3(nsys), dim3(1024), 0, st, d_desc);
+    if (!slab || !(phases & LIST_WRITE)) {
+      // counts only (or the compact layout's first pass)
+      hipLaunchKernelGGL(k_list_count, gw, b, 0, st, d_desc);
+      if (slab)
+        hipLaunchKernelGGL(k_list_total, dim3(nsys), dim3(1024), 0, st, d_desc);
+      else
+        hipLaunchKernelGGL(k_list_offsets, dim3(nsys), dim3(1024), 0, st, d_desc);
+    }
   }
-  if (phases & LIST_WRITE) hipLaunchKernelGGL(k_list_write, gw, b, 0, st, d_desc);
+  if (phases & LIST_WRITE) {
+    hipLaunchKernelGGL(k_list_write, gw, b, 0, st, d_desc);
+    if (slab) hipLaunchKernelGGL(k_list_total, dim3(nsys), dim3(1024), 0, st, d_desc);
+  }
 }
 
 // ---------------------------------------------------------------- half -> full

@@ -509,17 +509,20 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(const PmeDesc *__restrict__ D
 
 // ---------------------------------------------------------------- gather
 
-// Eight lanes per atom: lane i3 < order sums the z-plane i3 of the atom's
-// order^3 stencil; the partial f1, f2, f3 are reduced over the 8-lane group
-// with xor-shuffles (fixed order).  32 atoms per 256-thread block.
+// `order` lanes per atom, 64 / order atoms per wavefront: lane i3 sums the
+// z-plane i3 of the atom's order^3 stencil, and the partial f1, f2, f3 are
+// summed over the atom's lanes in plane order (fixed order).
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather(const PmeDesc *__restrict__ D) {
   __shared__ double red[8][9];
   const PmeDesc &d = D[blockIdx.y];
   if ((int)blockIdx.x >= d.gather_nblk) return;
   const int order = d.order;
-  const int sub = threadIdx.x & 7;
-  int a = blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3);
+  const int apw = 64 / order;
+  const int lane = threadIdx.x & 63;
+  const int slot = lane / order, sub = lane - slot * order;
+  int a = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * apw + slot;
+  if (slot >= apw) a = d.natoms;  // lanes past the last whole atom of the wave
   double stc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double xp = 0, yp = 0, zp = 0, fr1 = -1, fr2 = -1, fr3 = -1;
   if (a < d.natoms) {
@@ -536,7 +539,7 @@ __global__ void __launch_bounds__(256) k_gather(const PmeDesc *__restrict__ D) {
       a < d.natoms && fr1 >= 0.0 && fr1 <= d.K1 && fr2 >= 0.0 && fr2 <= d.K2 && fr3 >= 0.0 && fr3 <= d.K3;
   double f1 = 0.0, f2 = 0.0, f3 = 0.0;
   int b1 = 0, b2 = 0, b3 = 0;
-  if (valid && sub < order) {
+  if (valid) {
     b1 = (int)fr1;
     b2 = (int)fr2;
     b3 = (int)fr3;
@@ -562,11 +565,18 @@ __global__ void __launch_bounds__(256) k_gather(const PmeDesc *__restrict__ D) {
       f3 -= (double)(s23 * th2[i2] * dt3[i3]);
     }
   }
-#pragma unroll
-  for (int off = 4; off > 0; off >>= 1) {
-    f1 += __shfl_xor(f1, off, 64);
-    f2 += __shfl_xor(f2, off, 64);
-    f3 += __shfl_xor(f3, off, 64);
+  {
+    // every lane takes part in the shuffles; the atom's lanes are slot*order + k
+    const int src0 = (slot < apw ? slot : 0) * order;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int k = 0; k < order; ++k) {
+      s1 += __shfl(f1, src0 + k, 64);
+      s2 += __shfl(f2, src0 + k, 64);
+      s3 += __shfl(f3, src0 + k, 64);
+    }
+    f1 = s1;
+    f2 = s2;
+    f3 = s3;
   }
   if (valid && sub == 0) {
     const double q = d.q[a];
@@ -598,7 +608,7 @@ __global__ void __launch_bounds__(256) k_gather(const PmeDesc *__restrict__ D) {
       stc[6] = -gx * fz; stc[7] = -gy * fz; stc[8] = -gz * fz;
     }
   }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     double v = wave_sum(stc[k]);
@@ -661,7 +671,8 @@ void pme_set_tiles(PmeDesc &d, int lds_elems) {
   d.gy = (H * d.K3 + d.tpy - 1) / d.tpy;
   d.tpz = tile_pencils(d.K3, H * d.K2, lds_elems);
   d.conv_nblk = (H * d.K2 + d.tpz - 1) / d.tpz;
-  d.gather_nblk = (d.natoms + 31) / 32;  // k_gather: 8 lanes per atom, 32 atoms per block
+  const int apb = 4 * (64 / d.order);     // k_gather: order lanes per atom, 4 waves per block
+  d.gather_nblk = (d.natoms + apb - 1) / apb;
 }
 
 template <typename T>

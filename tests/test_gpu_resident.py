@@ -30,8 +30,6 @@ def oracle_total(s, corr_pairs, corr_fudge, nthreads=4):
     p = O.fft_pme(s, variant="fftw", nthreads=nthreads)
     f = d["f"] + p["f"]
     cs = cv = 0.0
-    for k in range(corr_pairs.shape[1]):
-        pass
     if corr_pairs.shape[1]:
         # group by fudge value so each ferrf call has one fudge
         for fud in np.unique(corr_fudge):
