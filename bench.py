@@ -111,17 +111,21 @@ def time_single(ctx, s, precision, steps, warmup):
     return dt, res, stages
 
 
+GRAPH = True  # replay batched and dynamics steps from HIP graphs (--no-graph: eager launches)
+
+
 def time_batched(ctx, reps, precision, steps, warmup, tune=True):
     from orac_hip.api import Batch
     systems = [resident(ctx, s, precision) for s in reps]
     b = Batch(ctx, systems)
-    tuning, tune_ms = (b.autotune(reps=3) if tune else (b.tuning(), None))
+    # the autotuner times graph replays, so eager runs keep the default tuning
+    tuning, tune_ms = (b.autotune(reps=3) if tune and GRAPH else (b.tuning(), None))
     for _ in range(warmup):
-        b.eval(rebuild_list=True, graph=True)
+        b.eval(rebuild_list=True, graph=GRAPH)
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        b.eval(rebuild_list=True, graph=True)
+        b.eval(rebuild_list=True, graph=GRAPH)
     ctx.sync()
     dt = time.perf_counter() - t0
     res0 = systems[0].results()
@@ -150,10 +154,10 @@ def prepare_md(ctx, systems, precision, relax=300, equil=200, temperature=300.0,
 
 def time_md(ctx, b, nsteps, warmup, dt=2.0, ninner=4, list_every=10):
     from orac_hip import md
-    md.run_batch(b, warmup, dt=dt, ninner=ninner, list_every=list_every)
+    md.run_batch(b, warmup, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH)
     ctx.sync()
     t0 = time.perf_counter()
-    md.run_batch(b, nsteps, dt=dt, ninner=ninner, list_every=list_every)
+    md.run_batch(b, nsteps, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH)
     ctx.sync()
     return time.perf_counter() - t0
 
@@ -162,7 +166,7 @@ def nve_drift(ctx, b, dyn, nsteps, dt=2.0, ninner=4, list_every=10):
     """Total-energy change over nsteps (per ns, kJ/mol) of the first system."""
     from orac_hip import md, synth
     e0 = dyn.energies()
-    md.run_batch(b, nsteps, dt=dt, ninner=ninner, list_every=list_every)
+    md.run_batch(b, nsteps, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH)
     ctx.sync()
     e1 = dyn.energies()
     kj = synth.EFACT / 1000.0
@@ -194,9 +198,9 @@ def serving_leg(ctx, nreq=16, rate=200.0, nsteps=500, budget_gib=64, seed=4242):
     return st
 
 
-def c0_leg(ctx, steps):
+def c0_leg(ctx, steps, md=True):
     """The 648-atom water box: one FP64 evaluation against the mixed one, and
-    100 outer r-RESPA steps in FP64 (ns/day, NVE drift)."""
+    (md) 100 outer r-RESPA steps in FP64 (ns/day, NVE drift)."""
     from orac_hip import synth
     from orac_hip.api import ResidentSystem
     w = synth.config("water216")
@@ -211,15 +215,18 @@ def c0_leg(ctx, steps):
         ctx.sync()
         res[prec] = (time.perf_counter() - t0, rs.results())
         rs.close()
+    out = {"workload": "216 rigid waters (648 atoms), L=18.64 A, PME 20^3 order 5, alpha 0.40, rc 8.5 A",
+           "fp64_evals_per_s": steps / res[0][0], "mixed_evals_per_s": steps / res[1][0],
+           "force_relrms_mixed_vs_f64": relrms(res[1][1]["f"], res[0][1]["f"])}
+    if not md:
+        return out
     rss, dyns, b = prepare_md(ctx, [w], 0, equil=100)
     wall = time_md(ctx, b, 100, 5, list_every=5)
     drift = nve_drift(ctx, b, dyns[0], 100, list_every=5)
     b.close()
     rss[0].close()
-    return {"workload": "216 rigid waters (648 atoms), L=18.64 A, PME 20^3 order 5, alpha 0.40, rc 8.5 A",
-            "fp64_evals_per_s": steps / res[0][0], "mixed_evals_per_s": steps / res[1][0],
-            "force_relrms_mixed_vs_f64": relrms(res[1][1]["f"], res[0][1]["f"]),
-            "md_100_steps": {"ns_per_day": ns_per_day(100, wall), "ms_per_step": 10 * wall, "nve": drift}}
+    out["md_100_steps"] = {"ns_per_day": ns_per_day(100, wall), "ms_per_step": 10 * wall, "nve": drift}
+    return out
 
 
 def cpu_baseline(s, corr, nthreads):
@@ -285,10 +292,13 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=8, help="requests of the serving leg (0 skips it)")
     ap.add_argument("--serve-steps", type=int, default=500, help="outer steps per served request")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-graph", action="store_true", help="launch batched and dynamics steps eagerly")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
+    global GRAPH
+    GRAPH = not args.no_graph
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (one system or one batch per device); --gpus {args.gpus} is not "
               "supported: sharding across GPUs is not part of this build.", file=sys.stderr)
@@ -332,8 +342,9 @@ def main():
             log(f"batched fp64 x{args.replicas}: {1e3 * bd['dt'] / bsteps:.3f} ms/step")
             batched = dict(bm=bm, bd=bd, bsteps=bsteps, err=relrms(bm["res0"]["f"], f0),
                            err64=relrms(bd["res0"]["f"], f0))
-        c0 = c0_leg(ctx, args.steps)
-        log(f"water216: {c0['fp64_evals_per_s']:.0f} FP64 evals/s, md {c0['md_100_steps']['ms_per_step']:.3f} ms/step")
+        c0 = c0_leg(ctx, args.steps, md=args.md_steps > 0)
+        log(f"water216: {c0['fp64_evals_per_s']:.0f} FP64 evals/s"
+            + (f", md {c0['md_100_steps']['ms_per_step']:.3f} ms/step" if "md_100_steps" in c0 else ""))
         dyn = None
         if args.md_steps > 0:
             K = args.md_steps

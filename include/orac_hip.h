@@ -199,8 +199,10 @@ int orac_hip_batch_stage_ms(orac_hip_ctx *ctx, int32_t batch, float *ms, int32_t
  * [1] pair-kernel occupancy target in waves/SIMD (0 default, 2, 3, 4),
  * [2] LDS-tiled spread (1) or global atomics (0), [3] XCD-grouped pair
  * blocks (1) or hardware order (0), [4] pair-kernel owners and neighbour
- * records in linked-cell order (1) or group order (0).  No choice changes
- * forces; [0] changes energies only in their last bits. */
+ * records in linked-cell order (1) or group order (0), [5] pair kernel as
+ * one persistent grid walking all work items of the batch (1) or one
+ * workgroup per item (0).  No choice changes forces; [0] and [4] change
+ * energies only in their last bits. */
 int orac_hip_batch_set_tuning(orac_hip_ctx *ctx, int32_t batch, const int32_t *tuning, int32_t n);
 int orac_hip_batch_get_tuning(orac_hip_ctx *ctx, int32_t batch, int32_t *tuning, int32_t n);
 /* Times the batch's step for the tuning candidates (reps evaluations each)

@@ -568,8 +568,9 @@ static int solo_batch(orac_hip_ctx *ctx, int32_t sys) {
 }
 
 // Size the neighbour list from the counts at the current coordinates: room
-// per owner for 1.25 x the largest count + 32 (at least the creation estimate).
-// Run whenever the host sets coordinates.
+// per owner for 1.5 x the largest count + 64 (at least the creation estimate),
+// so that density fluctuations during dynamics stay within the slab.  Clears
+// the sticky overflow flag.  Run whenever the host sets coordinates.
 static void size_list(orac_hip_ctx *ctx, int32_t id) {
   System &S = get_sys(ctx, id);
   Batch &b = get_batch(ctx, solo_batch(ctx, id));
@@ -581,8 +582,9 @@ static void size_list(orac_hip_ctx *ctx, int32_t id) {
   ORAC_HIP(hipMemcpyAsync(tot, S.list.total, sizeof tot, hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipStreamSynchronize(st));
   const int ng = S.ngrp;
-  const int need = std::min(ng, ((int)(1.25 * tot[2]) + 32 + 31) / 32 * 32);
+  const int need = std::min(ng, ((int)(1.5 * tot[2]) + 64 + 31) / 32 * 32);
   S.have_list = false;  // the count pass rewrote the counts
+  launch_zero_u64(reinterpret_cast<unsigned long long *>(S.list.total + 2), 1, st);
   if (need > S.slab) {
     ORAC_REQUIRE((size_t)ng * (size_t)need < (size_t)INT32_MAX, E_CAPACITY,
                  "neighbour list larger than 2^31 entries");
@@ -595,6 +597,19 @@ static void size_list(orac_hip_ctx *ctx, int32_t id) {
     S.pair.nb_slab = need;
     S.gen++;
   }
+}
+
+// Systems of b whose neighbour list overflowed its slab at some build since
+// their list was last sized (the sticky flag total[3]); synchronises.
+static std::vector<int> list_overflows(orac_hip_ctx *ctx, Batch &b) {
+  const size_t n = b.sys.size();
+  std::vector<int> flag(n, 0), bad;
+  for (size_t k = 0; k < n; ++k)
+    ORAC_HIP(hipMemcpyAsync(&flag[k], b.sys[k]->list.total + 3, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  ORAC_HIP(hipStreamSynchronize(ctx->stream));
+  for (size_t k = 0; k < n; ++k)
+    if (flag[k]) bad.push_back(b.ids[k]);
+  return bad;
 }
 
 // time of one evaluation (ms, mean over reps after one warm-up) of a batch
@@ -854,6 +869,12 @@ static void md_run(orac_hip_ctx *ctx, Batch &b, int nsteps, double dt, int ninne
   }
   for (System *S : b.sys) S->have_list = true;
   ORAC_HIP(hipGetLastError());
+  // a list cut short by its slab leaves that step's forces wrong: fail loudly
+  const std::vector<int> bad = list_overflows(ctx, b);
+  ORAC_REQUIRE(bad.empty(), E_CAPACITY,
+               "neighbour list of system " + std::to_string(bad.empty() ? -1 : bad[0]) +
+                   " outgrew its slab during dynamics; its trajectory is invalid from that step on "
+                   "(setting its positions again resizes the list)");
 }
 
 }  // namespace orac
@@ -1044,6 +1065,7 @@ static Tuning tuning_from(const int32_t *t, int32_t n, Tuning base) {
   if (n > 2) base.spread_tiled = t[2];
   if (n > 3) base.xcd_remap = t[3];
   if (n > 4) base.cell_order = t[4];
+  if (n > 5) base.persistent = t[5];
   ORAC_REQUIRE(base.lds_elems >= 64 && base.lds_elems <= 2048 && (base.lds_elems & (base.lds_elems - 1)) == 0, E_ARG,
                "lds_elems must be a power of two in [64, 2048]");
   ORAC_REQUIRE(base.pair_waves >= 0 && base.pair_waves <= 4 && base.pair_waves != 1, E_ARG,
@@ -1052,8 +1074,8 @@ static Tuning tuning_from(const int32_t *t, int32_t n, Tuning base) {
 }
 
 static void tuning_to(const Tuning &tu, int32_t *t, int32_t n) {
-  const int32_t v[5] = {tu.lds_elems, tu.pair_waves, tu.spread_tiled, tu.xcd_remap, tu.cell_order};
-  for (int k = 0; k < n && k < 5; ++k) t[k] = v[k];
+  const int32_t v[6] = {tu.lds_elems, tu.pair_waves, tu.spread_tiled, tu.xcd_remap, tu.cell_order, tu.persistent};
+  for (int k = 0; k < n && k < 6; ++k) t[k] = v[k];
 }
 
 extern "C" int orac_hip_batch_set_tuning(orac_hip_ctx *ctx, int32_t batch, const int32_t *tuning, int32_t n) {
@@ -1090,13 +1112,14 @@ extern "C" int orac_hip_batch_autotune(orac_hip_ctx *ctx, int32_t batch, int32_t
     Tuning best = b.tu;
     batch_upload(ctx, b);
     double tbest = batch_time(ctx, b, r, rebuild);
-    const std::vector<int> cands[5] = {{256, 512, 1024}, {2, 3, 4}, {1, 0}, {1, 0}, {0, 1}};
+    const std::vector<int> cands[6] = {{256, 512, 1024}, {2, 3, 4}, {1, 0}, {1, 0}, {0, 1}, {0, 1}};
     for (int sweep = 0; sweep < 2; ++sweep) {
-      for (int param = 0; param < 5; ++param) {
+      for (int param = 0; param < 6; ++param) {
         const std::vector<int> &cand = cands[param];
         for (int v : cand) {
           Tuning t = best;
-          int *slots[5] = {&t.lds_elems, &t.pair_waves, &t.spread_tiled, &t.xcd_remap, &t.cell_order};
+          int *slots[6] = {&t.lds_elems, &t.pair_waves, &t.spread_tiled, &t.xcd_remap, &t.cell_order,
+                           &t.persistent};
           int &slot = *slots[param];
           if (slot == v) continue;
           slot = v;
@@ -1200,12 +1223,19 @@ extern "C" int orac_hip_batch_md_relax(orac_hip_ctx *ctx, int32_t batch, int32_t
     md_upload(ctx, b, b.md_every > 0 ? b.md_every : 1);
     const int n = (int)b.sys.size();
     for (int it = 0; it < niter; ++it) {
+      if (it % list_every == 0 && it > 0) {
+        // the positions move far while relaxing: a list that outgrew its slab
+        // is resized at the current positions before the next rebuild
+        for (int id : list_overflows(ctx, b)) size_list(ctx, id);
+        md_upload(ctx, b, b.md_every > 0 ? b.md_every : 1);
+      }
       launch_md_fast_forces(b.dm, b.hm.data(), n, ctx->stream);
       batch_eval(ctx, b, it % list_every == 0, false);
       launch_md_relax(b.dm, b.hm.data(), n, hstep, dmax, ctx->stream);
     }
     ORAC_HIP(hipGetLastError());
     ORAC_HIP(hipStreamSynchronize(ctx->stream));
+    for (int id : list_overflows(ctx, b)) size_list(ctx, id);
     for (System *S : b.sys) S->md_primed = false;
   });
 }

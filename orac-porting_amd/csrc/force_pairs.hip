@@ -100,20 +100,19 @@ __device__ inline void lj_params(const PairDesc &d, int ti, int tj, double &c12,
   }
 }
 
-// W = minimum waves per SIMD the register allocation must allow (occupancy target)
-template <typename T, int W>
-__global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D, int remap) {
-  __shared__ double red[PAIR_WAVES][16];
-  __shared__ double facc[PAIR_WAVES][64][3];
-  __shared__ double acc[PAIR_WAVES][15][64];  // per-lane energy and virial sums (lane-private slots)
-  int sys = blockIdx.y, blk = blockIdx.x;
-  if (remap) {
-    const int l = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    sys = l / gridDim.x;
-    blk = l - sys * gridDim.x;
-  }
-  const PairDesc &d = D[sys];
+struct PairShared {
+  double red[PAIR_WAVES][16];
+  double facc[PAIR_WAVES][64][3];
+  double acc[PAIR_WAVES][15][64];  // per-lane energy and virial sums (lane-private slots)
+};
+
+// One work item: block blk of system d (PAIR_WAVES owner groups).
+template <typename T>
+__device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShared &sh) {
   if (blk >= d.nblk) return;
+  double(&red)[PAIR_WAVES][16] = sh.red;
+  double(&facc)[PAIR_WAVES][64][3] = sh.facc;
+  double(&acc)[PAIR_WAVES][15][64] = sh.acc;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int t = blk * PAIR_WAVES + wid;  // owner group, sorted position
   double fxa = 0.0, fya = 0.0, fza = 0.0;
@@ -338,6 +337,36 @@ __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D
   }
 }
 
+// item l of nitems = nsys x nblkmax (system l / nblkmax), optionally XCD-grouped
+__device__ __forceinline__ void pairs_item_index(int h, int nitems, int nblkmax, int remap, int &sys, int &blk) {
+  const int l = remap ? xcd_remap(h, nitems) : h;
+  sys = l / nblkmax;
+  blk = l - sys * nblkmax;
+}
+
+// W = minimum waves per SIMD the register allocation must allow (occupancy target)
+template <typename T, int W>
+__global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D, int remap) {
+  __shared__ PairShared sh;
+  int sys, blk;
+  pairs_item_index(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, gridDim.x, remap, sys, blk);
+  pairs_item<T>(D[sys], blk, sh);
+}
+
+// Persistent form: a fixed grid (a multiple of 8 workgroups, so each keeps to
+// one XCD) walks all items of the batch with a grid stride.
+template <typename T, int W>
+__global__ void __launch_bounds__(256, W) k_pairs_persistent(const PairDesc *__restrict__ D, int remap, int nitems,
+                                                             int nblkmax) {
+  __shared__ PairShared sh;
+  for (int h = blockIdx.x; h < nitems; h += gridDim.x) {
+    int sys, blk;
+    pairs_item_index(h, nitems, nblkmax, remap, sys, blk);
+    pairs_item<T>(D[sys], blk, sh);
+    __syncthreads();
+  }
+}
+
 // d.out[15]: uconf[3], ucoul[3] (slt, slt-slv, slv order of typeij), st[9] row-major;
 // block k reduces component k over the per-block partials in fixed order
 __global__ void __launch_bounds__(256) k_pairs_finalize(const PairDesc *__restrict__ D) {
@@ -383,11 +412,32 @@ void launch_pairs(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream
   hipLaunchKernelGGL(k_pack_sorted, dim3((max_grp + 255) / 256, nsys), dim3(256), 0, st, d_desc);
   dim3 grd(max_blk, nsys), blk(64 * PAIR_WAVES);
   const int remap = tu.xcd_remap ? 1 : 0;
+  const int waves = pair_occupancy<T>(tu.pair_waves);
+  const int nitems = max_blk * nsys;
   if (kbeg) ORAC_HIP(hipEventRecord(kbeg, st));
-  switch (pair_occupancy<T>(tu.pair_waves)) {
-    case 3: hipLaunchKernelGGL((k_pairs<T, 3>), grd, blk, 0, st, d_desc, remap); break;
-    case 4: hipLaunchKernelGGL((k_pairs<T, 4>), grd, blk, 0, st, d_desc, remap); break;
-    default: hipLaunchKernelGGL((k_pairs<T, 2>), grd, blk, 0, st, d_desc, remap); break;
+  if (tu.persistent) {
+    // CUs x workgroups per CU at the occupancy target (4 waves per workgroup)
+    static int ncu = [] {
+      int dev = 0, n = 0;
+      if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                   hipSuccess)
+        n = 256;
+      return n > 0 ? n : 256;
+    }();
+    const int g = std::max(8, std::min(nitems, ncu * waves) / 8 * 8);
+    switch (waves) {
+      case 3: hipLaunchKernelGGL((k_pairs_persistent<T, 3>), dim3(g), blk, 0, st, d_desc, remap, nitems, max_blk); break;
+      case 4: hipLaunchKernelGGL((k_pairs_persistent<T, 4>), dim3(g), blk, 0, st, d_desc, remap, nitems, max_blk); break;
+      default:
+        hipLaunchKernelGGL((k_pairs_persistent<T, 2>), dim3(g), blk, 0, st, d_desc, remap, nitems, max_blk);
+        break;
+    }
+  } else {
+    switch (waves) {
+      case 3: hipLaunchKernelGGL((k_pairs<T, 3>), grd, blk, 0, st, d_desc, remap); break;
+      case 4: hipLaunchKernelGGL((k_pairs<T, 4>), grd, blk, 0, st, d_desc, remap); break;
+      default: hipLaunchKernelGGL((k_pairs<T, 2>), grd, blk, 0, st, d_desc, remap); break;
+    }
   }
   if (kend) ORAC_HIP(hipEventRecord(kend, st));
   hipLaunchKernelGGL(k_pairs_finalize, dim3(15, nsys), dim3(256), 0, st, d_desc);

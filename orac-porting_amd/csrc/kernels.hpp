@@ -18,6 +18,7 @@ struct Tuning {
   int spread_tiled = 1; // LDS-tiled spread where the grid allows it, else global atomics
   int xcd_remap = 1;    // keep each system's pair blocks on one XCD (L2 locality)
   int cell_order = 0;   // pair kernel owners and neighbour records in cell order (1) or group order (0)
+  int persistent = 0;   // pair kernel as a fixed grid looping over the batch's work items
 };
 
 // ---------------------------------------------------------------- reciprocal space

@@ -320,6 +320,7 @@ __global__ void __launch_bounds__(1024) k_list_offsets(const ListDesc *__restric
     d.off[n] = over ? 0 : tot;
     d.total[0] = tot;
     d.total[1] = over ? 1 : 0;
+    if (over) d.total[3] = 1;  // sticky: cleared only by the host
   }
 }
 
@@ -362,6 +363,7 @@ __global__ void __launch_bounds__(1024) k_list_total(const ListDesc *__restrict_
     d.total[0] = tot;
     d.total[1] = mx > d.slab ? 1 : 0;
     d.total[2] = mx;
+    if (mx > d.slab) d.total[3] = 1;  // sticky: cleared only by the host
   }
 }
 

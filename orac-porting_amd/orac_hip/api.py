@@ -64,7 +64,7 @@ EXPORTS = [
 
 # stage_ms order of the resident evaluation
 STAGES = ["centres", "list", "pairs", "pme", "corrections", "pair_kernel"]
-TUNING_FIELDS = ["lds_elems", "pair_waves", "spread_tiled", "xcd_remap", "cell_order"]
+TUNING_FIELDS = ["lds_elems", "pair_waves", "spread_tiled", "xcd_remap", "cell_order", "persistent"]
 
 
 def _check(rc):

@@ -1,0 +1,80 @@
+"""The reference's test1 system (solvated FKBP12 with its ligand, 14,900 atoms)
+as a fixture, with the reference's own results on it.
+
+Reads test1/1j4i.tpgprm (topology-parameter dump) and test1/md.rst (restart)
+from the reference tree with orac_hip.formats, builds the system with the
+test's own parameters (md.in: PME alpha 0.37, grid 48^3, order 4, outer
+cutoff 10 A, list skin 1.8 A, 18^3 linked cells) and writes:
+
+* the system's arrays (inputs of every routine of the path);
+* the reference Fortran's results on it (oracle/_ref, compiled in place):
+  the linked-cell list, force() on the outer shell, fft_pme (fftpack branch)
+  and ferrf on the solute's bonds and 1-3 pairs.  Exact arrays are stored as
+  SHA-256 digests of their bytes; energies, stress and a sample of the PME
+  forces as values.
+
+    python tests/golden/make_fkbp12.py
+"""
+import hashlib
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+for p in (ROOT, os.path.join(ROOT, "orac-porting_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+import oracle as O  # noqa: E402
+from orac_hip import formats  # noqa: E402
+
+TEST1 = "/root/reference/test1"
+PARAMS = dict(alpha=0.37, nfft=(48, 48, 48), order=4, rcut=10.0, rtol=0.3, rnei=1.8)
+NCELL = 18
+E6 = ("uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss")
+
+
+def digest(a):
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(), dtype=np.uint8)
+
+
+def main():
+    if not O.ref_available():
+        sys.exit("oracle/_ref/libref.so is missing: build it with `make -C oracle ref` where the reference exists")
+    top = formats.read_tpgprm(os.path.join(TEST1, "1j4i.tpgprm"))
+    rst = formats.read_restart(os.path.join(TEST1, "md.rst"))
+    s = formats.reference_system(top, rst, name="fkbp12", **PARAMS)
+    out = dict(
+        xp0=s.xp0, yp0=s.yp0, zp0=s.zp0, charge=s.charge, mass=s.mass, nbtype=s.nbtype.astype(np.int16),
+        type_table=s.type_table.astype(np.int16), ecc12=s.ecc12, ecc6=s.ecc6, grppt=s.grppt.astype(np.int16),
+        atomp=s.atomp.astype(np.int16), ss_index=s.ss_index.astype(np.int8), co=s.co, oc=s.oc,
+        constraints=s.constraints.astype(np.int16), bonds=s.bonds.astype(np.int16),
+        bends=s.bends.astype(np.int16), params=np.array([PARAMS["alpha"], PARAMS["rcut"], PARAMS["rtol"],
+                                                          PARAMS["rnei"], PARAMS["order"], NCELL] +
+                                                         list(PARAMS["nfft"]), dtype=np.float64))
+    rc = s.rcut + s.rtol + s.rnei
+    fa = O.ForceArgs(s)
+    st = O.ref_lc_index(NCELL, NCELL, NCELL, rc, s.co)
+    nn = O.ref_lc_list(NCELL, NCELL, NCELL, st, rc, s.co, fa.xpg, fa.ypg, fa.zpg)
+    out["ref_nnlpp_len"], out["ref_nnlpp_sha"] = np.array(len(nn)), digest(nn.astype(np.int32))
+    f = O.ref_force(s, nn, rinn=0.0, rout=s.rcut, rtolinn=0.0, rtolout=s.rtol, rneigh=0.0, fa=fa)
+    out["ref_force_e"] = np.array([f[k] for k in E6])
+    out["ref_force_stress"] = f["stress"]
+    out["ref_force_f_sha"] = digest(f["f"])
+    out["ref_force_f_rms"] = np.array(np.sqrt((f["f"] ** 2).mean()))
+    p = O.ref_fft_pme(s)
+    out["ref_pme_eer"], out["ref_pme_stress"] = np.array(p["eer"]), p["stress"]
+    idx = np.arange(0, s.natoms, 97)
+    out["ref_pme_f_idx"], out["ref_pme_f_sample"] = idx.astype(np.int32), p["f"][:, idx]
+    pairs = np.concatenate([s.bonds, s.bends], axis=1)
+    c = O.ref_ferrf(s, pairs, 1, 1.0)
+    out["ref_ferrf_e"], out["ref_ferrf_f_sha"] = np.array([c["slt"], c["slv"]]), digest(c["f"])
+    path = os.path.join(HERE, "fkbp12.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,121 @@
+"""The GPU path on the reference's own test system (test1: solvated FKBP12 with
+its ligand, 14,900 atoms) against the compiled reference Fortran's results
+(tests/golden/fkbp12.npz) and the oracle.
+
+Plain entry points: the list must be the reference's bit for bit (SHA-256 of
+its bytes); force() and fft_pme within the FP64 tolerances (forces relative
+RMS <= 1e-9, energies and virial <= 1e-10).  The resident path (list, pairs,
+PME and corrections in one evaluation) is checked in FP64 and in mixed
+precision (forces relative RMS <= 5e-6).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel, relrms
+
+pytestmark = pytest.mark.gpu
+
+E6 = ("uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss")
+
+
+def digest(a):
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(), dtype=np.uint8)
+
+
+@pytest.fixture(scope="module")
+def fk():
+    from orac_hip import formats
+    s, ncell = formats.fkbp12()
+    return s, ncell, np.load(formats.FKBP12_PATH)
+
+
+@pytest.fixture(scope="module")
+def fk_list(fk):
+    s, ncell, _ = fk
+    fa = O.ForceArgs(s)
+    rc = s.rcut + s.rtol + s.rnei
+    st = O.lc_index(ncell, ncell, ncell, rc, s.co)
+    return fa, st, O.lc_list(ncell, ncell, ncell, st, rc, s.co, fa.xpg, fa.ypg, fa.zpg)
+
+
+def test_lc_list_is_reference(gpu_ctx, fk, fk_list):
+    s, nc, g = fk
+    fa, _, _ = fk_list
+    rc = s.rcut + s.rtol + s.rnei
+    nind, ii, jj, kk = gpu_ctx.lc_index(2500, nc, nc, nc, rc, s.co)
+    nn = np.zeros(int(g["ref_nnlpp_len"]) + 1024, np.int32)
+    n = gpu_ctx.lc_list(nc, nc, nc, nind, ii, jj, kk, rc, s.co, fa.xpg, fa.ypg, fa.zpg, s.ngrp, nn)
+    assert n == int(g["ref_nnlpp_len"])
+    assert np.array_equal(digest(nn[:n]), g["ref_nnlpp_sha"])
+
+
+def test_force_matches_reference(gpu_ctx, fk, fk_list):
+    from test_gpu_parity import _force_gpu, _stress_close
+    s, _, g = fk
+    fa, _, nn = fk_list
+    kw = dict(rinn=0.0, rout=s.rcut, rtolinn=0.0, rtolout=s.rtol, rneigh=0.0)
+    r = _force_gpu(gpu_ctx, s, nn, fa, **kw)
+    o = O.force(s, nn, fa=fa, **kw)
+    assert np.array_equal(digest(o["f"]), g["ref_force_f_sha"])  # the oracle is the reference here
+    err = relrms(r["f"], o["f"])
+    print("fkbp12 force relrms", err)
+    assert err < 1e-9
+    for k, e in zip(E6, g["ref_force_e"]):
+        assert rel(r[k], float(e)) < 1e-10, k
+    assert _stress_close(r["stress"], g["ref_force_stress"])
+
+
+def test_fft_pme_matches_reference(gpu_ctx, fk):
+    from test_gpu_parity import _pme_gpu, _stress_close
+    s, _, g = fk
+    r = _pme_gpu(gpu_ctx, s)
+    print("fkbp12 eer", r["eer"], float(g["ref_pme_eer"]))
+    assert rel(r["eer"], float(g["ref_pme_eer"])) < 1e-10
+    assert _stress_close(r["stress"], g["ref_pme_stress"])
+    assert relrms(r["f"][:, g["ref_pme_f_idx"]], g["ref_pme_f_sample"]) < 1e-9
+    o = O.fft_pme(s, nthreads=4)
+    assert relrms(r["f"], o["f"]) < 1e-9
+
+
+def test_ferrf_matches_reference(gpu_ctx, fk):
+    s, _, g = fk
+    pairs = np.concatenate([s.bonds, s.bends], axis=1)
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+    a, b = gpu_ctx.ferrf(s.ss_index, s.alpha, s.charge, 1.0, x, y, z, 1, pairs, pairs.shape[1], fpx, fpy, fpz)
+    assert rel(a, float(g["ref_ferrf_e"][0])) < 1e-10
+    assert b == float(g["ref_ferrf_e"][1]) == 0.0  # the solute's bonds and 1-3 pairs are solute-solute
+    o = O.ferrf(s, pairs, 1, 1.0)
+    assert np.array_equal(digest(o["f"]), g["ref_ferrf_f_sha"])
+    assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-9
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_resident_matches_oracle(gpu_ctx, fk, precision):
+    from orac_hip.api import ResidentSystem
+    from test_gpu_resident import oracle_total
+    s, nc, _ = fk
+    cp = np.concatenate([s.bonds, s.bends], axis=1)
+    cf = np.ones(cp.shape[1])
+    rs = ResidentSystem(gpu_ctx, s, precision=precision, ncell=nc, corr_pairs=cp, corr_fudge=cf)
+    rs.eval(rebuild_list=True)
+    r = rs.results()
+    o = oracle_total(s, cp, cf)
+    err = relrms(r["f"], o["f"])
+    print("fkbp12 resident precision", precision, "f relrms", err, "eer", rel(r["eer"], o["eer"]))
+    if precision == 0:
+        assert err < 1e-9
+        assert rel(r["eer"], o["eer"]) < 1e-10
+        for k in E6:
+            if o["d"][k] != 0.0:
+                assert rel(r[k], o["d"][k]) < 1e-10, k
+        assert rel(r["corr_slt"], o["corr_slt"]) < 1e-10
+        assert np.abs(r["stress"] - o["stress"]).max() < 1e-10 * np.abs(o["stress"]).max()
+    else:
+        assert err < 5e-6
+        assert rel(r["eer"], o["eer"]) < 1e-5
+    rs.close()

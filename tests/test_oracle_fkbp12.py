@@ -1,0 +1,86 @@
+"""The oracle on the reference's own test system (test1: solvated FKBP12 with
+its ligand, 14,900 atoms, read from 1j4i.tpgprm and md.rst), against the
+compiled reference Fortran's results on it, stored in tests/golden/fkbp12.npz
+by tests/golden/make_fkbp12.py.
+
+Serial oracle calls reproduce the Fortran's summation order, so the list,
+force() and ferrf results must agree bit for bit (compared as SHA-256 digests
+of the arrays' bytes); fft_pme's fftpack branch agrees to rounding.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+from orac_hip import formats
+
+E6 = ("uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss")
+
+
+def digest(a):
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(), dtype=np.uint8)
+
+
+@pytest.fixture(scope="module")
+def fk():
+    s, ncell = formats.fkbp12()
+    g = np.load(formats.FKBP12_PATH)
+    return s, ncell, g
+
+
+@pytest.fixture(scope="module")
+def fk_list(fk):
+    s, ncell, _ = fk
+    fa = O.ForceArgs(s)
+    rc = s.rcut + s.rtol + s.rnei
+    st = O.lc_index(ncell, ncell, ncell, rc, s.co)
+    return fa, O.lc_list(ncell, ncell, ncell, st, rc, s.co, fa.xpg, fa.ypg, fa.zpg)
+
+
+def test_fixture_is_the_test1_system(fk):
+    s, ncell, _ = fk
+    assert s.natoms == 14900 and ncell == 18 and tuple(s.nfft) == (48, 48, 48) and s.order == 4
+    # md.in: alpha 0.37, outer cutoff 10 A, update skin 1.8 A
+    assert (s.alpha, s.rcut, s.rnei) == (0.37, 10.0, 1.8)
+    # net charge of the neutralised system (charges are e / sqrt(unitc))
+    assert abs(s.charge.sum()) < 1e-6 * np.abs(s.charge).sum()
+    # rigid TIP3P waters (3 constraints each): O-H 0.9572 A, H-H 1.5139 A
+    x, y, z = s.cartesian()
+    c = s.constraints - 1
+    d = np.sqrt((x[c[0]] - x[c[1]]) ** 2 + (y[c[0]] - y[c[1]]) ** 2 + (z[c[0]] - z[c[1]]) ** 2)
+    assert c.shape[1] % 3 == 0
+    assert np.all(np.minimum(np.abs(d - 0.9572), np.abs(d - 1.5139)) < 1e-3)
+
+
+def test_list_matches_reference(fk, fk_list):
+    _, _, g = fk
+    _, nn = fk_list
+    assert len(nn) == int(g["ref_nnlpp_len"])
+    assert np.array_equal(digest(nn.astype(np.int32)), g["ref_nnlpp_sha"])
+
+
+def test_force_matches_reference(fk, fk_list):
+    s, _, g = fk
+    fa, nn = fk_list
+    f = O.force(s, nn, rinn=0.0, rout=s.rcut, rtolinn=0.0, rtolout=s.rtol, rneigh=0.0, fa=fa)
+    assert np.array_equal(np.array([f[k] for k in E6]), g["ref_force_e"])
+    assert np.array_equal(f["stress"], g["ref_force_stress"])
+    assert np.array_equal(digest(f["f"]), g["ref_force_f_sha"])
+
+
+def test_fft_pme_matches_reference(fk):
+    s, _, g = fk
+    p = O.fft_pme(s, variant="fftpack")
+    assert abs(p["eer"] - float(g["ref_pme_eer"])) <= 1e-13 * abs(float(g["ref_pme_eer"]))
+    assert np.abs(p["stress"] - g["ref_pme_stress"]).max() <= 1e-12 * np.abs(g["ref_pme_stress"]).max()
+    fs = p["f"][:, g["ref_pme_f_idx"]]
+    assert np.abs(fs - g["ref_pme_f_sample"]).max() <= 1e-11 * np.abs(g["ref_pme_f_sample"]).max()
+
+
+def test_ferrf_matches_reference(fk):
+    s, _, g = fk
+    pairs = np.concatenate([s.bonds, s.bends], axis=1)
+    c = O.ferrf(s, pairs, 1, 1.0)
+    assert np.array_equal(np.array([c["slt"], c["slv"]]), g["ref_ferrf_e"])
+    assert np.array_equal(digest(c["f"]), g["ref_ferrf_f_sha"])
