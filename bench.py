@@ -40,6 +40,9 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_PAIR = 89
+# HBM bytes per launch of k_pairs<double,3> on the 24,460-atom system, from the
+# committed FETCH_SIZE / WRITE_SIZE profile (rocprof_summary traffic)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bytes_k_pairs_fp64_solvated24k.txt")
 
 
 def log(msg):
@@ -148,7 +151,7 @@ def prepare_md(ctx, systems, precision, relax=300, equil=200, temperature=300.0,
     md.relax_batch(b, relax, dmax=0.03)
     for k, d in enumerate(dyns):
         d.maxwell_boltzmann(temperature, seed + k)
-    md.equilibrate(b, dyns, temperature, equil)
+    md.equilibrate(b, dyns, temperature, equil, graph=GRAPH)
     return rss, dyns, b
 
 
@@ -293,10 +296,18 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=500, help="outer steps per served request")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true", help="launch batched and dynamics steps eagerly")
+    ap.add_argument("--no-water", action="store_true", help="skip the 648-atom water box leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
+    # stdout carries only the result line: file descriptor 1 goes to stderr for
+    # the whole run (the reference Fortran of the CPU leg writes progress lines
+    # to unit 6, flushed as late as process exit) and the line is written to
+    # the saved descriptor
+    sys.stdout.flush()
+    result_fd = os.dup(1)
+    os.dup2(2, 1)
     global GRAPH
     GRAPH = not args.no_graph
     if args.gpus != 1:
@@ -315,14 +326,8 @@ def main():
     with Context(0) as ctx:
         dev = ctx.device_name()
         if args.copy_probe_mb:
-            import torch
-            n = args.copy_probe_mb * (1 << 20) // 8
-            a = torch.ones(n, dtype=torch.float64, device="cuda")
-            b = torch.empty_like(a)
-            b.copy_(a)
-            torch.cuda.synchronize()
-            del a, b
-            log(f"copy probe: {args.copy_probe_mb} MiB read + {args.copy_probe_mb} MiB written")
+            ctx.probe_copy(args.copy_probe_mb << 20)
+            log(f"copy probe (k_probe_copy): {args.copy_probe_mb} MiB read + {args.copy_probe_mb} MiB written")
         dt64, r64, st64 = time_single(ctx, s, 0, args.steps, args.warmup)
         log(f"fp64 single: {1e3 * dt64 / args.steps:.4f} ms/eval")
         dt32, r32, st32 = time_single(ctx, s, 1, args.steps, args.warmup)
@@ -342,9 +347,11 @@ def main():
             log(f"batched fp64 x{args.replicas}: {1e3 * bd['dt'] / bsteps:.3f} ms/step")
             batched = dict(bm=bm, bd=bd, bsteps=bsteps, err=relrms(bm["res0"]["f"], f0),
                            err64=relrms(bd["res0"]["f"], f0))
-        c0 = c0_leg(ctx, args.steps, md=args.md_steps > 0)
-        log(f"water216: {c0['fp64_evals_per_s']:.0f} FP64 evals/s"
-            + (f", md {c0['md_100_steps']['ms_per_step']:.3f} ms/step" if "md_100_steps" in c0 else ""))
+        c0 = None
+        if not args.no_water:
+            c0 = c0_leg(ctx, args.steps, md=args.md_steps > 0)
+            log(f"water216: {c0['fp64_evals_per_s']:.0f} FP64 evals/s"
+                + (f", md {c0['md_100_steps']['ms_per_step']:.3f} ms/step" if "md_100_steps" in c0 else ""))
         dyn = None
         if args.md_steps > 0:
             K = args.md_steps
@@ -358,7 +365,8 @@ def main():
             dyn = dict(fp64=dict(ns_per_day=ns_per_day(K, w64), ms_per_step=1e3 * w64 / K, nve=drift64))
             if args.replicas > 0:
                 rss, dyns, b = prepare_md(ctx, reps, 1)
-                b.autotune(reps=2, rebuild_list=False)
+                if GRAPH:  # the autotuner times graph replays
+                    b.autotune(reps=2, rebuild_list=False)
                 wm = time_md(ctx, b, K, 5)
                 driftm = nve_drift(ctx, b, dyns[0], K)
                 log(f"md mixed x{args.replicas}: {1e3 * wm / K:.3f} ms/step, NVE {driftm}")
@@ -382,9 +390,12 @@ def main():
     err32 = relrms(r32["f"], f64)
     kms = st64["pair_kernel"]
     achieved = FLOP_PER_PAIR * npairs / (kms * 1e-3) / 1e12
+    from orac_hip.rocprof_summary import read_traffic
+    traffic = read_traffic(TRAFFIC_PROFILE)
     roof = {"bound": "mfma", "kernel": "k_pairs<double,3>", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
             "per_launch": {"pairs": npairs, "flop_per_pair": FLOP_PER_PAIR, "ms": kms,
+                           "traffic_unit": "bytes (FETCH_SIZE + WRITE_SIZE, probe-scaled)",
                            "note": "FP64 VALU (no dense contraction): peak is the FP64 vector rate"}}
 
     def rnd(d):
@@ -423,7 +434,8 @@ def main():
             "speedup_mixed_batched_vs_plain_fp64": agg / evals64,
             "force_relrms_mixed_vs_f64": batched["err"], "force_relrms_batched_f64_vs_alone": batched["err64"],
             "tuning": bm["tuning"], "stage_ms_mixed": rnd(bm["stages"]), "device_bytes": bm["device_bytes"]}
-    out["water216"] = c0
+    if c0:
+        out["water216"] = c0
     if dyn:
         dyn["workload"] = ("NVE r-RESPA: 2 fs outer step, 4 inner steps for the solute harmonic terms, rigid waters "
                            "(SHAKE/RATTLE), list every 10 steps; after 200 steepest-descent steps and "
@@ -448,7 +460,8 @@ def main():
                 "sample": f"{ref[1]} evaluations by the reference Fortran compiled in place (serial, fftpack "
                           "branch; FFTW is not in the image)",
                 "gpu_vs_reference_force_relrms": relrms(f64, ref[2])}
-    print(json.dumps(out), flush=True)
+    sys.stdout.flush()
+    os.write(result_fd, (json.dumps(out) + "\n").encode())
 
 
 if __name__ == "__main__":

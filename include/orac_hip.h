@@ -38,6 +38,11 @@ int orac_hip_last_error(char *buf, int32_t len);
 int orac_hip_invalidate(orac_hip_ctx *ctx);
 /* Name of the device the context runs on, e.g. "gfx950". */
 int orac_hip_device_name(orac_hip_ctx *ctx, char *buf, int32_t len);
+/* No reference counterpart (a measurement aid): one plain device copy kernel
+ * (k_probe_copy) that reads nbytes and writes nbytes, so that profiler byte
+ * counters (FETCH_SIZE, WRITE_SIZE) can be scaled against a known size.
+ * Synchronous; allocates and frees its two buffers. */
+int orac_hip_probe_copy(orac_hip_ctx *ctx, int64_t nbytes);
 
 /* ---- reciprocal space ---------------------------------------------------- */
 /* Replaces fft_pme_init (fftw3_pme_init.f90:2-62): B-spline moduli
@@ -197,7 +202,8 @@ int orac_hip_batch_profile(orac_hip_ctx *ctx, int32_t batch, int32_t enable);
 int orac_hip_batch_stage_ms(orac_hip_ctx *ctx, int32_t batch, float *ms, int32_t n);
 /* Tuning vector: [0] LDS elements per FFT buffer (power of two, 64..2048),
  * [1] pair-kernel occupancy target in waves/SIMD (0 default, 2, 3, 4),
- * [2] LDS-tiled spread (1) or global atomics (0), [3] XCD-grouped pair
+ * [2] LDS-tiled spread with the largest tiles (1), the smallest tiles (2),
+ * or global atomics (0), [3] XCD-grouped pair
  * blocks (1) or hardware order (0), [4] pair-kernel owners and neighbour
  * records in linked-cell order (1) or group order (0), [5] pair kernel as
  * one persistent grid walking all work items of the batch (1) or one
@@ -231,7 +237,9 @@ int orac_hip_md_get_state(orac_hip_ctx *ctx, int32_t sys, double *x, double *y, 
                           double *vz);
 /* nsteps outer steps of dt fs for every system of the batch; the group
  * list is rebuilt every list_every steps; use_graph replays a captured HIP
- * graph of one outer step.  Asynchronous. */
+ * graph of one outer step.  Returns once the steps are done: status 2 if a
+ * system's list outgrew its slab at some step (that system's trajectory is
+ * invalid from there on; setting its positions resizes the list). */
 int orac_hip_batch_md_run(orac_hip_ctx *ctx, int32_t batch, int32_t nsteps, double dt, int32_t ninner,
                           int32_t list_every, int32_t use_graph);
 /* niter steepest-descent steps: every unit moves along its total force by

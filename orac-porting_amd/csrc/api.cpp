@@ -66,6 +66,28 @@ extern "C" int orac_hip_device_name(orac_hip_ctx *ctx, char *buf, int32_t len) {
   });
 }
 
+// plain copy of n doubles, one element per thread, for byte-counter calibration
+__global__ void __launch_bounds__(256) k_probe_copy(const double *__restrict__ a, double *__restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+extern "C" int orac_hip_probe_copy(orac_hip_ctx *ctx, int64_t nbytes) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(nbytes >= 8 && nbytes <= (int64_t)1 << 34, E_ARG, "probe size must be 8 B .. 16 GiB");
+    const size_t n = (size_t)nbytes / 8;
+    DevBuf a, b;
+    a.ensure(n * 8);
+    b.ensure(n * 8);
+    ORAC_HIP(hipMemsetAsync(a.p, 0, n * 8, ctx->stream));
+    hipLaunchKernelGGL(k_probe_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const double *)a.p, (double *)b.p, n);
+    ORAC_HIP(hipGetLastError());
+    ORAC_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 extern "C" int orac_hip_sync(orac_hip_ctx *ctx) {
   return guarded([&] {
     set_device(ctx);
@@ -212,6 +234,7 @@ extern "C" int orac_hip_lc_list(orac_hip_ctx *ctx, int32_t ncx, int32_t ncy, int
     L.cell_start = B.get<int>("l_start", ncell + 1);
     L.cell_cursor = B.get<int>("l_cur", ncell + 1);
     L.cell_items = B.get<int>("l_items", natp);
+    L.cell_fill = B.get<int>("l_fill", natp);
     L.count = B.get<int>("l_count", natp + 1);
     L.off = B.get<int>("l_off", natp + 1);
     L.total = B.get<int>("l_total", 2);
@@ -257,6 +280,7 @@ extern "C" int orac_hip_force(orac_hip_ctx *ctx, const int32_t *ss_index, const 
     (void)groupp;
     (void)atomg;
     ORAC_REQUIRE(nato >= 0 && ngrp >= 0 && ma >= 1, E_ARG, "bad sizes");
+    ORAC_REQUIRE(nato < (1 << 26), E_UNSUPPORTED, "more than 2^26 atoms");
     ORAC_REQUIRE(ss_index && xp0 && yp0 && zp0 && xpg && ypg && zpg && charge && nbtype && type && co && ecc12 &&
                      ecc6 && mapnl && grppt && rtolinn && ncount1 && nnlpp && gmass && mass,
                  E_ARG, "null array argument");
@@ -432,7 +456,8 @@ extern "C" int orac_hip_force(orac_hip_ctx *ctx, const int32_t *ss_index, const 
     p.fx = B.get<double>("f_fx", nato);
     p.fy = B.get<double>("f_fy", nato);
     p.fz = B.get<double>("f_fz", nato);
-    p.nblk = std::max(1, (ngrp + PAIR_WAVES - 1) / PAIR_WAVES);
+    p.split = pair_split_for(ngrp);
+    p.nblk = std::max(1, (ngrp + PAIR_WAVES / p.split - 1) / (PAIR_WAVES / p.split));
     p.part = B.get<double>("f_part", (size_t)p.nblk * 16);
     p.sgp = B.get<double4>("f_sgp", std::max(1, ngrp));
     p.sgi = B.get<int4>("f_sgi", std::max(1, ngrp));

@@ -81,11 +81,46 @@ struct PmeDesc {
   int2 *atom_bin;               // [natoms] {bin, position in bin}
   double4 *afr;                 // [natoms] {fr1, fr2, fr3, q}
   double4 *sfr;                 // [natoms] the same, sorted by bin
+  double *swt;                  // [natoms][3][order] B-spline weights of the sorted atoms (exact T values)
   int tiled;                    // 1: LDS-tiled spread, 0: global-atomic spread
   double *out;                  // [20]: eer, vir (3x3 col-major), stc1..stc9
 };
 
 // ---------------------------------------------------------------- helpers
+
+// Global-memory view of a device pointer.  Pointers a kernel reads out of a
+// descriptor are generic (flat) to the compiler; accesses through this view
+// are global ones: no aperture check, a uniform base in SGPRs, and waits on
+// the vector-memory counter only, so they overlap with LDS traffic.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
+// HIP vector types (double4, int4, ...) through the global view: their
+// member functions expect generic objects, so the native vector is moved.
+template <typename V>
+__device__ __forceinline__ V gload(const V *p) {
+  typedef typename V::Native_vec_ N;
+  const N n = *(__attribute__((address_space(1))) const N *)p;
+  V v;
+  __builtin_memcpy(&v, &n, sizeof(V));
+  return v;
+}
+// Element i of an array of HIP vector types as a uniform base plus a 32-bit
+// byte offset (i * sizeof(V) < 2^32), the form of a global load with a
+// scalar base and one offset VGPR.
+template <typename V>
+__device__ __forceinline__ V gload_at(const V *base, unsigned i) {
+  return gload(reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) + (i * (unsigned)sizeof(V))));
+}
+template <typename V>
+__device__ __forceinline__ void gstore(V *p, const V &v) {
+  typedef typename V::Native_vec_ N;
+  N n;
+  __builtin_memcpy(&n, &v, sizeof(V));
+  *(__attribute__((address_space(1))) N *)p = n;
+}
 
 __host__ __device__ inline int wrap_idx(int k0, int K) { return k0 >= 0 ? k0 : k0 + K; }
 
@@ -94,9 +129,25 @@ __host__ __device__ inline double fnint(double x) { return round(x); }
 
 // Cardinal B-spline weights and derivatives (Essmann recursion,
 // pme_fftw3.f90:537-576): derivative taken before the last pass.
+// 1 / (k - 1) for the passes k = 2..MAX_ORDER: the correctly rounded value,
+// as the division gives it, in either precision
+template <typename T>
+__host__ __device__ inline T recip_km1(int k) {
+  switch (k) {
+    case 2: return T(1.0);
+    case 3: return T(1.0 / 2.0);
+    case 4: return T(1.0 / 3.0);
+    case 5: return T(1.0 / 4.0);
+    case 6: return T(1.0 / 5.0);
+    case 7: return T(1.0 / 6.0);
+    case 8: return T(1.0 / 7.0);
+    default: return T(1) / T(k - 1);
+  }
+}
+
 template <typename T>
 __host__ __device__ inline void bspline_pass(T *c, T w, int k) {
-  T div = T(1) / T(k - 1);
+  T div = recip_km1<T>(k);
   c[k - 1] = div * w * c[k - 2];
   for (int j = 1; j <= k - 2; ++j) c[k - j - 1] = div * ((w + T(j)) * c[k - j - 2] + (T(k - j) - w) * c[k - j - 1]);
   c[0] = div * (T(1) - w) * c[0];

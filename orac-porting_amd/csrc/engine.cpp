@@ -133,6 +133,8 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   BufSet &B = S.B;
   const int n = d.natoms, ng = d.ngrp, ma = d.ntypes;
   ORAC_REQUIRE(n > 0 && ng > 0 && ma > 0 && d.nmol > 0, E_ARG, "system sizes must be positive");
+  // device kernels address atom and group records with 32-bit byte offsets
+  ORAC_REQUIRE(n < (1 << 26), E_UNSUPPORTED, "more than 2^26 atoms in one system");
   ORAC_REQUIRE(d.charge && d.mass && d.nbtype && d.type_table && d.ecc12 && d.ecc6 && d.grppt && d.atomp &&
                    d.ss_index && d.mapnl && d.co,
                E_ARG, "null array in system description");
@@ -230,6 +232,14 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   p.nmol = d.nmol;
   p.mfirst = B.put_vec("mfirst", mfirst, st, false);
   p.mlast = B.put_vec("mlast", mlast, st, false);
+  {
+    std::vector<int> big;
+    for (int m = 0; m < d.nmol; ++m)
+      if (mlast[m] - mfirst[m] + 1 > MOL_SMALL) big.push_back(m);
+    p.nbig = (int)big.size();
+    if (big.empty()) big.push_back(0);
+    p.mbig = B.put_vec("mbig", big, st, false);
+  }
   p.gfirst = B.put_vec("gfirst", gfirst, st, false);
   p.glast = B.put_vec("glast", glast, st, false);
   p.gss = B.put_vec("gss", gss, st, false);
@@ -287,7 +297,8 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   p.fx = S.fx;
   p.fy = S.fy;
   p.fz = S.fz;
-  p.nblk = (ng + PAIR_WAVES - 1) / PAIR_WAVES;
+  p.split = pair_split_for(ng);
+  p.nblk = (ng + PAIR_WAVES / p.split - 1) / (PAIR_WAVES / p.split);
   p.part = B.get<double>("ppart", (size_t)p.nblk * 16);
   p.sgp = B.get<double4>("sgp", ng);
   p.sgi = B.get<int4>("sgi", ng);
@@ -329,6 +340,7 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
     L.cell_start = B.get<int>("cstart", ncell + 1);
     L.cell_cursor = B.get<int>("ccur", ncell + 1);
     L.cell_items = B.get<int>("citems", ng);
+    L.cell_fill = B.get<int>("cfill", ng);
     L.count = B.get<int>("lcount", ng + 1);
     L.off = B.get<int>("nboff", ng + 1);
     L.out = B.get<int>("nbidx", S.list_cap);
@@ -1070,6 +1082,8 @@ static Tuning tuning_from(const int32_t *t, int32_t n, Tuning base) {
                "lds_elems must be a power of two in [64, 2048]");
   ORAC_REQUIRE(base.pair_waves >= 0 && base.pair_waves <= 4 && base.pair_waves != 1, E_ARG,
                "pair_waves must be 0 (default), 2, 3 or 4");
+  ORAC_REQUIRE(base.spread_tiled >= 0 && base.spread_tiled <= 2, E_ARG,
+               "spread_tiled must be 0 (atomics), 1 (largest tiles) or 2 (smallest tiles)");
   return base;
 }
 
@@ -1112,7 +1126,7 @@ extern "C" int orac_hip_batch_autotune(orac_hip_ctx *ctx, int32_t batch, int32_t
     Tuning best = b.tu;
     batch_upload(ctx, b);
     double tbest = batch_time(ctx, b, r, rebuild);
-    const std::vector<int> cands[6] = {{256, 512, 1024}, {2, 3, 4}, {1, 0}, {1, 0}, {0, 1}, {0, 1}};
+    const std::vector<int> cands[6] = {{256, 512, 1024}, {2, 3, 4}, {1, 2, 0}, {1, 0}, {0, 1}, {0, 1}};
     for (int sweep = 0; sweep < 2; ++sweep) {
       for (int param = 0; param < 6; ++param) {
         const std::vector<int> &cand = cands[param];

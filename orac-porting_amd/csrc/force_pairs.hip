@@ -39,19 +39,20 @@ __global__ void __launch_bounds__(256) k_pack_sorted(const PairDesc *__restrict_
   const PairDesc &d = D[blockIdx.y];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= d.ngrp) return;
-  const int g = d.perm ? d.perm[t] : t;
-  const int f0 = d.gfirst[g], n = d.glast[g] - f0 + 1;
-  const int sf = d.soff ? d.soff[t] : f0;
-  d.sgp[t] = make_double4(d.xpg[g], d.ypg[g], d.zpg[g], 0.0);
-  d.sgi[t] = make_int4(sf, f0, n | (d.gss[g] << 8), g);
+  const int g = d.perm ? gmem(d.perm)[t] : t;
+  const int f0 = gmem(d.gfirst)[g], n = gmem(d.glast)[g] - f0 + 1;
+  const int sf = d.soff ? gmem(d.soff)[t] : f0;
+  gstore(d.sgp + t, make_double4(gmem(d.xpg)[g], gmem(d.ypg)[g], gmem(d.zpg)[g], 0.0));
+  gstore(d.sgi + t, make_int4(sf, f0, n | (gmem(d.gss)[g] << 8), g));
   for (int k = 0; k < n; ++k) {
     const int a = f0 + k;
-    d.sap[sf + k] = make_double4(d.xp[a], d.yp[a], d.zp[a], d.q[a]);
-    d.sai[sf + k] = make_int2(d.type[a], d.ex_inter[a] ? 1 : 0);
+    gstore(d.sap + sf + k, make_double4(gmem(d.xp)[a], gmem(d.yp)[a], gmem(d.zp)[a], gmem(d.q)[a]));
+    gstore(d.sai + sf + k, make_int2(gmem(d.type)[a], gmem(d.ex_inter)[a] ? 1 : 0));
   }
 }
 
-__device__ inline bool has_partner_in(const int *ex_off, const int *ex_idx, int a, int lo, int hi) {
+template <typename P>
+__device__ inline bool has_partner_in(P ex_off, P ex_idx, int a, int lo, int hi) {
   for (int t = ex_off[a]; t < ex_off[a + 1]; ++t) {
     int p = ex_idx[t];
     if (p >= lo && p <= hi) return true;
@@ -59,26 +60,43 @@ __device__ inline bool has_partner_in(const int *ex_off, const int *ex_idx, int 
   return false;
 }
 
-__device__ inline bool is_excluded(const int *ex_off, const int *ex_idx, int a, int b) {
+template <typename P>
+__device__ inline bool is_excluded(P ex_off, P ex_idx, int a, int b) {
   for (int t = ex_off[a]; t < ex_off[a + 1]; ++t)
     if (ex_idx[t] == b) return true;
   return false;
 }
 
-// Pair arithmetic built on one reciprocal square root per pair.  FP64: exact
-// divide and sqrt (results agree with the reference's expressions to a few
-// ulp).  FP32 (mixed precision): the hardware reciprocal, reciprocal square
-// root and exponential, 1-2 ulp each.
+// Pair arithmetic built on one reciprocal square root per pair.  FP64: the
+// hardware reciprocal square root and reciprocal (half-precision-class
+// seeds) refined by two Newton-Raphson steps each, which leaves a few ulp of
+// error (the reference's 1/sqrt and divides are correctly rounded; the
+// difference is ~1e-16 relative, far below the force tolerance) for a
+// fraction of the instructions of an IEEE divide and square root.  FP32
+// (mixed precision): the hardware reciprocal, reciprocal square root and
+// exponential, 1-2 ulp each.  Pair distances are positive and finite, so no
+// special cases arise.
 template <typename T>
 __device__ inline T pair_rsqrt(T x);
 template <>
-__device__ inline double pair_rsqrt<double>(double x) { return 1.0 / sqrt(x); }
+__device__ inline double pair_rsqrt<double>(double x) {
+  const double hx = 0.5 * x;
+  double r = __builtin_amdgcn_rsq(x);
+  r = fma(r, fma(-hx, r * r, 0.5), r);
+  r = fma(r, fma(-hx, r * r, 0.5), r);
+  return r;
+}
 template <>
 __device__ inline float pair_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 template <typename T>
 __device__ inline T pair_rcp(T x);
 template <>
-__device__ inline double pair_rcp<double>(double x) { return 1.0 / x; }
+__device__ inline double pair_rcp<double>(double x) {
+  double q = __builtin_amdgcn_rcp(x);
+  q = fma(q, fma(-x, q, 1.0), q);
+  q = fma(q, fma(-x, q, 1.0), q);
+  return q;
+}
 template <>
 __device__ inline float pair_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <typename T>
@@ -88,16 +106,110 @@ __device__ inline double pair_exp<double>(double x) { return exp(x); }
 template <>
 __device__ inline float pair_exp<float>(float x) { return __expf(x); }
 
+// The pair (a, b) is masked: excluded (the symmetric exclusion list), or
+// both atoms at or above massinfty (force.f:488-489).  For the reverse
+// direction the owner group must first hold one of b's partners.
+__device__ __noinline__ bool pair_masked(const PairDesc &d, int a, int b, int ia0, int nI, bool fwd) {
+  if (!fwd && !has_partner_in(gmem(d.ex_off), gmem(d.ex_idx), b, ia0, ia0 + nI - 1)) return false;
+  if (is_excluded(gmem(d.ex_off), gmem(d.ex_idx), a, b)) return true;
+  return d.any_heavy && !(gmem(d.mass)[b] < d.massinfty || gmem(d.mass)[a] < d.massinfty);
+}
+
 __device__ inline void lj_params(const PairDesc &d, int ti, int tj, double &c12, double &c6) {
   if (d.lj_dense) {
-    double2 v = d.lj_dense[(size_t)ti * d.ntypes + tj];
+    double2 v = gload_at(d.lj_dense, ti * d.ntypes + tj);
     c12 = v.x;
     c6 = v.y;
   } else {
-    int lij = d.type_table[ti + (size_t)d.ntypes * tj];
-    c12 = d.ecc12[lij];
-    c6 = d.ecc6[lij];
+    int lij = gmem(d.type_table)[ti + (size_t)d.ntypes * tj];
+    c12 = gmem(d.ecc12)[lij];
+    c6 = gmem(d.ecc6)[lij];
   }
+}
+
+// Group-pair geometry (force.f:265-292): the image shift s (even integers in
+// reduced units) and the centre separation after it, without contraction.
+struct GroupGeom {
+  double sxm, sym, szm, xcc, ycc, zcc, drj;
+};
+__device__ __forceinline__ GroupGeom group_geom(double xpgi, double ypgi, double zpgi, const double4 &gpj,
+                                                const double *co) {
+#pragma clang fp contract(off)
+  GroupGeom g;
+  double xgg = xpgi - gpj.x, ygg = ypgi - gpj.y, zgg = zpgi - gpj.z;
+  g.sxm = 2.0 * fnint(0.5 * xgg);
+  g.sym = 2.0 * fnint(0.5 * ygg);
+  g.szm = 2.0 * fnint(0.5 * zgg);
+  xgg = xgg - g.sxm;
+  ygg = ygg - g.sym;
+  zgg = zgg - g.szm;
+  g.xcc = co[0] * xgg + co[3] * ygg + co[6] * zgg;
+  g.ycc = co[4] * ygg + co[7] * zgg;
+  g.zcc = co[8] * zgg;
+  g.drj = g.xcc * g.xcc + g.ycc * g.ycc + g.zcc * g.zcc;
+  return g;
+}
+
+// the group pair is inside the shell: normal [rinn0, rout) or switched
+// [rout, rout0) and [rinn, rinn0) (force.f:293-335)
+__device__ __forceinline__ bool in_shell(const PairDesc &d, double drj) {
+  return (drj >= d.r2inn0 && drj < d.r2out) || (drj >= d.r2out && drj < d.r2out0) ||
+         (drj >= d.r2inn && drj < d.r2inn0);
+}
+
+// One atom pair of a group pair (force.f:420-560): the LJ term when the pair
+// has one (c6 > 1e-12), the A&S erfc Ewald term or plain Coulomb, as the
+// scalar force factor qf (force = qf * r), the separation and the two energies.
+struct PairTerms {
+  double qf, xc, yc, zc, conf, cou;
+  __device__ __forceinline__ void add(double &uconf, double &ucoul, double &fx, double &fy, double &fz) const {
+    uconf += conf;
+    ucoul += cou;
+    fx += qf * xc;
+    fy += qf * yc;
+    fz += qf * zc;
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ PairTerms pair_terms(const PairDesc &d, double pxs, double pys, double pzs, double qa,
+                                                const double *co, const double4 &pb4, double c12, double c6) {
+  const double xg = pxs - pb4.x, yg = pys - pb4.y, zg = pzs - pb4.z;
+  const T xc = T(co[0] * xg + co[3] * yg + co[6] * zg);
+  const T yc = T(co[4] * yg + co[7] * zg);
+  const T zc = T(co[8] * zg);
+  const T rsq = xc * xc + yc * yc + zc * zc;
+  const T rinv = pair_rsqrt<T>(rsq);
+  const T rsqi = rinv * rinv;
+  // LJ by predication: a pair without LJ takes zero parameters
+  const bool lj = c6 > 1e-12;
+  const T c12t = lj ? T(c12) : T(0), c6t = lj ? T(c6) : T(0);
+  const T c12f = lj ? T(12.0 * c12) : T(0), c6f = lj ? T(6.0 * c6) : T(0);
+  const T r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+  T qforce = (c12f * r12 - c6f * r6) * rsqi;
+  PairTerms r;
+  r.conf = lj ? (double)(c12t * r12 - c6t * r6) : 0.0;
+  r.cou = 0.0;
+  if (!d.lskip_ewald) {
+    const T rsp = rsq * rinv;
+    const T rspqi = rsqi * rinv;
+    const T alphar = T(d.alpha) * rsp;
+    const T qt = pair_rcp<T>(T(1) + T(AS_QP) * alphar);
+    const T expcst = pair_exp<T>(-(alphar * alphar));
+    const T erfcst = ((((T(AS_A5) * qt + T(AS_A4)) * qt + T(AS_A3)) * qt + T(AS_A2)) * qt + T(AS_A1)) * qt * expcst;
+    const T furpar = T(qa * pb4.w);
+    r.cou = (double)(furpar * erfcst * rinv);
+    qforce += furpar * (erfcst + T(d.twrtpi) * alphar * expcst) * rspqi;
+  } else if (!d.ewald) {
+    const T ssv = T(qa * pb4.w) * rinv;
+    r.cou = (double)ssv;
+    qforce += ssv * rsqi;
+  }
+  r.qf = (double)qforce;
+  r.xc = (double)xc;
+  r.yc = (double)yc;
+  r.zc = (double)zc;
+  return r;
 }
 
 struct PairShared {
@@ -114,7 +226,9 @@ __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShare
   double(&facc)[PAIR_WAVES][64][3] = sh.facc;
   double(&acc)[PAIR_WAVES][15][64] = sh.acc;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int t = blk * PAIR_WAVES + wid;  // owner group, sorted position
+  // S waves share one owner group, each taking a contiguous part of its list
+  const int S = d.split, part = wid % S;
+  const int t = blk * (PAIR_WAVES / S) + wid / S;  // owner group, sorted position
   double fxa = 0.0, fya = 0.0, fza = 0.0;
   double *en = &acc[wid][0][0];  // en[k * 64 + lane]: uconf[typeij], ucoul[typeij]
   double *st = &acc[wid][6][0];  // st[k * 64 + lane]
@@ -122,7 +236,7 @@ __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShare
   for (int k = 0; k < 15; ++k) acc[wid][k][lane] = 0.0;
   int ia0 = 0, nI = 1, nslot = 64;  // ia0: owner's first atom, original numbering
   if (t < d.ngrp) {
-    const int4 gi = d.sgi[t];
+    const int4 gi = gload(d.sgi + t);
     const int g = gi.w;
     ia0 = gi.y;
     nI = gi.z & 255;
@@ -131,194 +245,165 @@ __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShare
     const bool active = slot < nslot;
     const int a = ia0 + (active ? ai : 0);        // original index
     const int as = gi.x + (active ? ai : 0);      // sorted index
-    const double4 pa4 = d.sap[as];
-    const int2 ia4 = d.sai[as];
-    const double ma = d.gmass[a];
+    const double4 pa4 = gload(d.sap + as);
+    const int2 ia4 = gload(d.sai + as);
+    const double ma = gmem(d.gmass)[a];
     const int typei = gi.z >> 8;
-    const double4 gpi = d.sgp[t];
+    const double4 gpi = gload(d.sgp + t);
     const double xpgi = gpi.x, ypgi = gpi.y, zpgi = gpi.z;
-    const double co0 = d.co[0], co3 = d.co[3], co6 = d.co[6], co4 = d.co[4], co7 = d.co[7], co8 = d.co[8];
+    double co[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) co[k] = d.co[k];
+    const double co0 = co[0], co3 = co[3], co6 = co[6], co4 = co[4], co7 = co[7], co8 = co[8];
     int nb0, nb1;
     if (d.nb_slab > 0) {
       nb0 = t * d.nb_slab;
-      nb1 = nb0 + min(d.nb_cnt[t], d.nb_slab);
+      nb1 = nb0 + min(gmem(d.nb_cnt)[t], d.nb_slab);
     } else {
-      nb0 = d.nb_off[t];
-      nb1 = d.nb_off[t + 1];
+      nb0 = gmem(d.nb_off)[t];
+      nb1 = gmem(d.nb_off)[t + 1];
     }
-    int tjn = (active && nb0 + slot < nb1) ? d.nb_idx[nb0 + slot] : t;
-    for (int base = nb0; base < nb1; base += nslot) {
-      const int jj = base + slot;
-      const bool have = active && jj < nb1;
-      const int tj = tjn;
-      // the next neighbour's index is fetched before this one is worked on
-      tjn = (active && jj + nslot < nb1) ? d.nb_idx[jj + nslot] : t;
-      const double4 gpj = d.sgp[tj];
-      const int4 gij = d.sgi[tj];
-      const int j = gij.w;
-      // group classification (force.f:265-335), no contraction
-      double sxm, sym, szm, xcc, ycc, zcc, drj;
-      {
-#pragma clang fp contract(off)
-        double xgg = xpgi - gpj.x, ygg = ypgi - gpj.y, zgg = zpgi - gpj.z;
-        sxm = 2.0 * fnint(0.5 * xgg);
-        sym = 2.0 * fnint(0.5 * ygg);
-        szm = 2.0 * fnint(0.5 * zgg);
-        xgg = xgg - sxm;
-        ygg = ygg - sym;
-        zgg = zgg - szm;
-        xcc = co0 * xgg + co3 * ygg + co6 * zgg;
-        ycc = co4 * ygg + co7 * zgg;
-        zcc = co8 * zgg;
-        drj = xcc * xcc + ycc * ycc + zcc * zcc;
+    if (S > 1) {
+      const int len = nb1 - nb0;
+      nb1 = nb0 + (len * (part + 1)) / S;
+      nb0 = nb0 + (len * part) / S;
+    }
+    // The list reaches rc + rtol + rnei; only groups inside the shell do work.
+    // Entries are classified 64 at a time and the survivors compacted, in list
+    // order, into this wave's part of facc (unused until the force reduction),
+    // so every slot of the pair loop is busy.
+    int *cbuf = reinterpret_cast<int *>(&facc[wid][0][0]);
+    constexpr int CBUF = (int)(sizeof(facc[0]) / sizeof(int));
+    for (int c0 = nb0; c0 < nb1;) {
+      int nc = 0;
+      for (; c0 < nb1 && nc <= CBUF - 64; c0 += 64) {
+        const int jj = c0 + lane;
+        const bool in = jj < nb1;
+        const int tj = in ? gmem(d.nb_idx)[jj] : t;
+        const GroupGeom gg = group_geom(xpgi, ypgi, zpgi, gload_at(d.sgp, tj), co);
+        const bool keep = in && in_shell(d, gg.drj);
+        const unsigned long long m = __ballot(keep);
+        if (keep) cbuf[nc + __popcll(m & ((1ull << lane) - 1ull))] = tj;
+        nc += __popcll(m);
       }
-      int cls = 0;  // 0 none, 1 normal, 2 switched
-      double sw = 1.0, dsw = 0.0;
-      if (have) {
-        if (drj >= d.r2inn0 && drj < d.r2out) {
-          cls = 1;
-        } else if (drj >= d.r2out && drj < d.r2out0) {
-          cls = 2;
-          double rsp = sqrt(drj);
-          double auxa = (d.arsout1 + 2.0 * rsp) / d.arsout2;
-          double auxb = d.rout0 - rsp;
-          sw = auxa * (auxb * auxb);
-          dsw = (-(2.0 * auxa * auxb) + 2.0 * (auxb * auxb) / d.arsout2) / rsp;
-        } else if (drj >= d.r2inn && drj < d.r2inn0) {
-          cls = 2;
-          double rsp = sqrt(drj);
-          double auxa = (d.arsinn1 + 2.0 * rsp) / d.arsinn2;
-          double auxb = d.rinn0 - rsp;
-          sw = 1.0 - auxa * (auxb * auxb);
-          dsw = (2.0 * auxa * auxb - 2.0 * (auxb * auxb) / d.arsinn2) / rsp;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int base = 0; base < nc; base += nslot) {
+        const int jj = base + slot;
+        const bool have = active && jj < nc;
+        const int tj = have ? cbuf[jj] : t;
+        const double4 gpj = gload_at(d.sgp, tj);
+        const int4 gij = gload_at(d.sgi, tj);
+        const int j = gij.w;
+        const GroupGeom gg = group_geom(xpgi, ypgi, zpgi, gpj, co);
+        const double sxm = gg.sxm, sym = gg.sym, szm = gg.szm, xcc = gg.xcc, ycc = gg.ycc, zcc = gg.zcc,
+                     drj = gg.drj;
+        int cls = 0;  // 0 none, 1 normal, 2 switched
+        double sw = 1.0, dsw = 0.0;
+        if (have) {
+          if (drj >= d.r2inn0 && drj < d.r2out) {
+            cls = 1;
+          } else if (drj >= d.r2out && drj < d.r2out0) {
+            cls = 2;
+            double rsp = sqrt(drj);
+            double auxa = (d.arsout1 + 2.0 * rsp) / d.arsout2;
+            double auxb = d.rout0 - rsp;
+            sw = auxa * (auxb * auxb);
+            dsw = (-(2.0 * auxa * auxb) + 2.0 * (auxb * auxb) / d.arsout2) / rsp;
+          } else if (drj >= d.r2inn && drj < d.r2inn0) {
+            cls = 2;
+            double rsp = sqrt(drj);
+            double auxa = (d.arsinn1 + 2.0 * rsp) / d.arsinn2;
+            double auxb = d.rinn0 - rsp;
+            sw = 1.0 - auxa * (auxb * auxb);
+            dsw = (2.0 * auxa * auxb - 2.0 * (auxb * auxb) / d.arsinn2) / rsp;
+          }
+        }
+        const bool fwd = j > g;
+        const bool shifted = sxm != 0.0 || sym != 0.0 || szm != 0.0;
+        double cm = 0.0;                      // this lane's part of V (switched pairs)
+        double fgx = 0.0, fgy = 0.0, fgz = 0.0;  // this lane's pair forces for the group pair
+        if (cls != 0) {
+          const int nJ = gij.z & 255;
+          const int jb0 = gij.y, jb1 = jb0 + nJ - 1, jbs = gij.x;  // original range, sorted start
+          const int typeij = typei + (gij.z >> 8) - 2;  // 0..2
+          const bool mask_a = fwd && (ia4.y & 1) && has_partner_in(gmem(d.ex_off), gmem(d.ex_idx), a, jb0, jb1);
+          // owner position minus the image shift: one association for every class
+          const double pxs = pa4.x - sxm, pys = pa4.y - sym, pzs = pa4.z - szm;
+          // unswitched sums over the neighbour's atoms; the switch (a function of
+          // the group distance only) scales them once below
+          double ucoula = 0.0, uconfa = 0.0;
+          for (int k = 0; k < nJ; ++k) {
+            const int2 ib0 = gload_at(d.sai, jbs + k);
+            const double4 pb0 = gload_at(d.sap, jbs + k);
+            double c12a, c6a;
+            lj_params(d, ia4.x, ib0.x, c12a, c6a);
+            bool skip0 = d.lskip_ewald && c6a < 1e-12 && d.ewald;
+            if (fwd ? mask_a : (ib0.y & 1) != 0) skip0 = skip0 || pair_masked(d, a, jb0 + k, ia0, nI, fwd);
+            const PairTerms r0 = pair_terms<T>(d, pxs, pys, pzs, pa4.w, co, pb0, c12a, c6a);
+            if (!skip0) r0.add(uconfa, ucoula, fgx, fgy, fgz);
+          }
+          const double s = cls == 2 ? sw : 1.0;
+          fgx *= s;
+          fgy *= s;
+          fgz *= s;
+          fxa += fgx;
+          fya += fgy;
+          fza += fgz;
+          if (fwd) {
+            en[typeij * 64 + lane] += s * uconfa;
+            en[(3 + typeij) * 64 + lane] += s * ucoula;
+          }
+          if (cls == 2) cm = uconfa + ucoula;
+        }
+        // total V of the group pair over the slot's lanes (every lane shuffles)
+        double cmt = 0.0;
+        for (int u = 0; u < nI; ++u) cmt += __shfl(cm, (slot * nI + u) & 63, 64);
+        if (cls == 2) {
+          // S'(r) V spread over group atoms by mass fraction (force.f:756-817)
+          const double x3 = -(dsw * cmt * xcc), y3 = -(dsw * cmt * ycc), z3 = -(dsw * cmt * zcc);
+          fxa += ma * x3;
+          fya += ma * y3;
+          fza += ma * z3;
+          if (fwd && shifted && ai == 0) {
+            fgx += x3;
+            fgy += y3;
+            fgz += z3;
+          }
+        }
+        if (fwd && shifted) {
+          st[0 * 64 + lane] -= fgx * sxm; st[1 * 64 + lane] -= fgx * sym; st[2 * 64 + lane] -= fgx * szm;
+          st[3 * 64 + lane] -= fgy * sxm; st[4 * 64 + lane] -= fgy * sym; st[5 * 64 + lane] -= fgy * szm;
+          st[6 * 64 + lane] -= fgz * sxm; st[7 * 64 + lane] -= fgz * sym; st[8 * 64 + lane] -= fgz * szm;
         }
       }
-      const bool fwd = j > g;
-      const bool shifted = sxm != 0.0 || sym != 0.0 || szm != 0.0;
-      double cm = 0.0;                      // this lane's part of V (switched pairs)
-      double fgx = 0.0, fgy = 0.0, fgz = 0.0;  // this lane's pair forces for the group pair
-      if (cls != 0) {
-        const int nJ = gij.z & 255;
-        const int jb0 = gij.y, jb1 = jb0 + nJ - 1, jbs = gij.x;  // original range, sorted start
-        const int typeij = typei + (gij.z >> 8) - 2;  // 0..2
-        const bool mask_a = fwd && (ia4.y & 1) && has_partner_in(d.ex_off, d.ex_idx, a, jb0, jb1);
-        double ucoula = 0.0, uconfa = 0.0;
-        for (int k = 0; k < nJ; ++k) {
-          const int b = jb0 + k;
-          const int2 ib4 = d.sai[jbs + k];
-          const bool masked =
-              fwd ? mask_a : ((ib4.y & 1) && has_partner_in(d.ex_off, d.ex_idx, b, ia0, ia0 + nI - 1));
-          if (masked) {
-            if (is_excluded(d.ex_off, d.ex_idx, a, b)) continue;
-            if (d.any_heavy && !(d.mass[b] < d.massinfty || d.mass[a] < d.massinfty)) continue;
-          }
-          double c12, c6;
-          lj_params(d, ia4.x, ib4.x, c12, c6);
-          if (d.lskip_ewald && c6 < 1e-12 && d.ewald) continue;
-          const double4 pb4 = d.sap[jbs + k];
-          double xg, yg, zg;
-          if (cls == 1) {
-            xg = (pa4.x - sxm) - pb4.x;
-            yg = (pa4.y - sym) - pb4.y;
-            zg = (pa4.z - szm) - pb4.z;
-          } else {
-            xg = (pa4.x - pb4.x) - sxm;
-            yg = (pa4.y - pb4.y) - sym;
-            zg = (pa4.z - pb4.z) - szm;
-          }
-          const T xc = T(co0 * xg + co3 * yg + co6 * zg);
-          const T yc = T(co4 * yg + co7 * zg);
-          const T zc = T(co8 * zg);
-          const T rsq = xc * xc + yc * yc + zc * zc;
-          const T rinv = pair_rsqrt<T>(rsq);
-          const T rsqi = rinv * rinv;
-          T qforce = T(0);
-          double vpair = 0.0;
-          if (c6 > 1e-12) {
-            T r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
-            qforce = (T(12.0 * c12) * r12 - T(6.0 * c6) * r6) * rsqi;
-            double conf = (double)(T(c12) * r12 - T(c6) * r6);
-            vpair += conf;
-            if (cls == 1) uconfa += conf;
-            else if (fwd) en[typeij * 64 + lane] += sw * conf;
-          }
-          if (!d.lskip_ewald) {
-            T rsp = rsq * rinv;
-            T rspqi = rsqi * rinv;
-            T alphar = T(d.alpha) * rsp;
-            T qt = pair_rcp<T>(T(1) + T(AS_QP) * alphar);
-            T expcst = pair_exp<T>(-(alphar * alphar));
-            T erfcst = ((((T(AS_A5) * qt + T(AS_A4)) * qt + T(AS_A3)) * qt + T(AS_A2)) * qt + T(AS_A1)) * qt * expcst;
-            T furpar = T(pa4.w * pb4.w);
-            double vel = (double)(furpar * erfcst * rinv);
-            vpair += vel;
-            if (cls == 1) ucoula += vel;
-            else if (fwd) en[(3 + typeij) * 64 + lane] += sw * vel;
-            qforce += furpar * (erfcst + T(d.twrtpi) * alphar * expcst) * rspqi;
-          } else if (!d.ewald) {
-            T ssv = T(pa4.w * pb4.w) * rinv;
-            vpair += (double)ssv;
-            if (fwd) en[(3 + typeij) * 64 + lane] += (cls == 1 ? 1.0 : sw) * (double)ssv;
-            qforce += ssv * rsqi;
-          }
-          double qf = (double)qforce;
-          if (cls == 2) {
-            qf *= sw;
-            cm += vpair;
-          }
-          const double fxv = qf * (double)xc, fyv = qf * (double)yc, fzv = qf * (double)zc;
-          fgx += fxv;
-          fgy += fyv;
-          fgz += fzv;
-        }
-        fxa += fgx;
-        fya += fgy;
-        fza += fgz;
-        if (fwd && cls == 1) {
-          en[typeij * 64 + lane] += uconfa;
-          en[(3 + typeij) * 64 + lane] += ucoula;
-        }
-      }
-      // total V of the group pair over the slot's lanes (every lane shuffles)
-      double cmt = 0.0;
-      for (int u = 0; u < nI; ++u) cmt += __shfl(cm, (slot * nI + u) & 63, 64);
-      if (cls == 2) {
-        // S'(r) V spread over group atoms by mass fraction (force.f:756-817)
-        const double x3 = -(dsw * cmt * xcc), y3 = -(dsw * cmt * ycc), z3 = -(dsw * cmt * zcc);
-        fxa += ma * x3;
-        fya += ma * y3;
-        fza += ma * z3;
-        if (fwd && shifted && ai == 0) {
-          fgx += x3;
-          fgy += y3;
-          fgz += z3;
-        }
-      }
-      if (fwd && shifted) {
-        st[0 * 64 + lane] -= fgx * sxm; st[1 * 64 + lane] -= fgx * sym; st[2 * 64 + lane] -= fgx * szm;
-        st[3 * 64 + lane] -= fgy * sxm; st[4 * 64 + lane] -= fgy * sym; st[5 * 64 + lane] -= fgy * szm;
-        st[6 * 64 + lane] -= fgz * sxm; st[7 * 64 + lane] -= fgz * sym; st[8 * 64 + lane] -= fgz * szm;
-      }
+      // every lane is done with this chunk before the next fill
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
-  // owner-atom forces: lanes ai, ai + nI, ... summed in lane order
+  // owner-atom forces: the owner's waves in part order, in each the lanes
+  // ai, ai + nI, ... in lane order
   facc[wid][lane][0] = fxa;
   facc[wid][lane][1] = fya;
   facc[wid][lane][2] = fza;
   __syncthreads();
-  if (t < d.ngrp && lane < nI) {
+  if (t < d.ngrp && part == 0 && lane < nI) {
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int s = 0; s < nslot; ++s) {
-      sx += facc[wid][s * nI + lane][0];
-      sy += facc[wid][s * nI + lane][1];
-      sz += facc[wid][s * nI + lane][2];
-    }
+    for (int p = 0; p < S; ++p)
+      for (int s = 0; s < nslot; ++s) {
+        sx += facc[wid + p][s * nI + lane][0];
+        sy += facc[wid + p][s * nI + lane][1];
+        sz += facc[wid + p][s * nI + lane][2];
+      }
     const int a = ia0 + lane;
-    d.fx[a] = sx;
-    d.fy[a] = sy;
-    d.fz[a] = sz;
-    const int pa = d.atomp[a];
-    const double cx = d.xpcm[pa], cy = d.ypcm[pa], cz = d.zpcm[pa];
+    gmem(d.fx)[a] = sx;
+    gmem(d.fy)[a] = sy;
+    gmem(d.fz)[a] = sz;
+    const int pa = gmem(d.atomp)[a];
+    const double cx = gmem(d.xpcm)[pa], cy = gmem(d.ypcm)[pa], cz = gmem(d.zpcm)[pa];
     st[0 * 64 + lane] += sx * cx; st[1 * 64 + lane] += sx * cy; st[2 * 64 + lane] += sx * cz;
     st[3 * 64 + lane] += sy * cx; st[4 * 64 + lane] += sy * cy; st[5 * 64 + lane] += sy * cz;
     st[6 * 64 + lane] += sz * cx; st[7 * 64 + lane] += sz * cy; st[8 * 64 + lane] += sz * cz;
@@ -333,7 +418,7 @@ __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShare
   if (threadIdx.x < 15) {
     double s = 0.0;
     for (int w = 0; w < PAIR_WAVES; ++w) s += red[w][threadIdx.x];
-    d.part[(size_t)blk * 16 + threadIdx.x] = s;
+    gmem(d.part)[(size_t)blk * 16 + threadIdx.x] = s;
   }
 }
 
@@ -367,6 +452,16 @@ __global__ void __launch_bounds__(256, W) k_pairs_persistent(const PairDesc *__r
   }
 }
 
+template <typename T, int W>
+static void launch_pair_kernel(bool persistent, dim3 grd, int g, hipStream_t st, const PairDesc *d_desc, int remap,
+                               int nitems, int max_blk) {
+  const dim3 blk(64 * PAIR_WAVES);
+  if (persistent)
+    hipLaunchKernelGGL((k_pairs_persistent<T, W>), dim3(g), blk, 0, st, d_desc, remap, nitems, max_blk);
+  else
+    hipLaunchKernelGGL((k_pairs<T, W>), grd, blk, 0, st, d_desc, remap);
+}
+
 // d.out[15]: uconf[3], ucoul[3] (slt, slt-slv, slv order of typeij), st[9] row-major;
 // block k reduces component k over the per-block partials in fixed order
 __global__ void __launch_bounds__(256) k_pairs_finalize(const PairDesc *__restrict__ D) {
@@ -374,7 +469,7 @@ __global__ void __launch_bounds__(256) k_pairs_finalize(const PairDesc *__restri
   const PairDesc &d = D[blockIdx.y];
   const int k = blockIdx.x;
   double s = 0.0;
-  for (int b = threadIdx.x; b < d.nblk; b += blockDim.x) s += d.part[(size_t)b * 16 + k];
+  for (int b = threadIdx.x; b < d.nblk; b += blockDim.x) s += gmem(d.part)[(size_t)b * 16 + k];
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -383,6 +478,15 @@ __global__ void __launch_bounds__(256) k_pairs_finalize(const PairDesc *__restri
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
     d.out[k] = t;
   }
+}
+
+int pair_split_for(int ngrp) {
+  // ORAC_HIP_PAIR_SPLIT=1|2|4 overrides the choice for systems created after it is set
+  const char *e = getenv("ORAC_HIP_PAIR_SPLIT");
+  const int v = e ? atoi(e) : 0;
+  if (v == 1 || v == 2 || v == 4) return v;
+  (void)ngrp;
+  return 1;  // measured: extra waves per owner cost more than the tail they trim
 }
 
 // Occupancy target of k_pairs: measured best on MI355X for the 24.5k-atom
@@ -410,11 +514,12 @@ void launch_pairs(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream
     max_blk = std::max(max_blk, h[s].nblk);
   }
   hipLaunchKernelGGL(k_pack_sorted, dim3((max_grp + 255) / 256, nsys), dim3(256), 0, st, d_desc);
-  dim3 grd(max_blk, nsys), blk(64 * PAIR_WAVES);
+  dim3 grd(max_blk, nsys);
   const int remap = tu.xcd_remap ? 1 : 0;
   const int waves = pair_occupancy<T>(tu.pair_waves);
   const int nitems = max_blk * nsys;
   if (kbeg) ORAC_HIP(hipEventRecord(kbeg, st));
+  int g = 0;
   if (tu.persistent) {
     // CUs x workgroups per CU at the occupancy target (4 waves per workgroup)
     static int ncu = [] {
@@ -424,20 +529,12 @@ void launch_pairs(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream
         n = 256;
       return n > 0 ? n : 256;
     }();
-    const int g = std::max(8, std::min(nitems, ncu * waves) / 8 * 8);
-    switch (waves) {
-      case 3: hipLaunchKernelGGL((k_pairs_persistent<T, 3>), dim3(g), blk, 0, st, d_desc, remap, nitems, max_blk); break;
-      case 4: hipLaunchKernelGGL((k_pairs_persistent<T, 4>), dim3(g), blk, 0, st, d_desc, remap, nitems, max_blk); break;
-      default:
-        hipLaunchKernelGGL((k_pairs_persistent<T, 2>), dim3(g), blk, 0, st, d_desc, remap, nitems, max_blk);
-        break;
-    }
-  } else {
-    switch (waves) {
-      case 3: hipLaunchKernelGGL((k_pairs<T, 3>), grd, blk, 0, st, d_desc, remap); break;
-      case 4: hipLaunchKernelGGL((k_pairs<T, 4>), grd, blk, 0, st, d_desc, remap); break;
-      default: hipLaunchKernelGGL((k_pairs<T, 2>), grd, blk, 0, st, d_desc, remap); break;
-    }
+    g = std::max(8, std::min(nitems, ncu * waves) / 8 * 8);
+  }
+  switch (waves) {
+    case 3: launch_pair_kernel<T, 3>(tu.persistent != 0, grd, g, st, d_desc, remap, nitems, max_blk); break;
+    case 4: launch_pair_kernel<T, 4>(tu.persistent != 0, grd, g, st, d_desc, remap, nitems, max_blk); break;
+    default: launch_pair_kernel<T, 2>(tu.persistent != 0, grd, g, st, d_desc, remap, nitems, max_blk); break;
   }
   if (kend) ORAC_HIP(hipEventRecord(kend, st));
   hipLaunchKernelGGL(k_pairs_finalize, dim3(15, nsys), dim3(256), 0, st, d_desc);

@@ -257,29 +257,44 @@ PmeDesc pme_desc_base(BufSet &B, const std::string &pfx, const PmePlan &P, int n
   // one K3 pencil per workgroup is the most blocks any tiling uses
   d.conv_part = B.get<double>(pfx + "conv_part", (size_t)H * P.K2 * 8);
   d.gather_part = B.get<double>(pfx + "gather_part", (size_t)std::max(1, (natoms + 31) / 32) * 9);
-  // LDS-tiled spread when every axis has a tile edge in [order-1, 8] dividing K
-  d.st1 = spread_tile_edge(P.K1, P.order);
-  d.st2 = spread_tile_edge(P.K2, P.order);
-  d.st3 = spread_tile_edge(P.K3, P.order);
-  if (!d.st1 || !d.st2 || !d.st3) d.st1 = d.st2 = d.st3 = 0;
-  if (d.st1) {
-    d.nb1 = P.K1 / d.st1;
-    d.nb2 = P.K2 / d.st2;
-    d.nb3 = P.K3 / d.st3;
-    const size_t nbin = (size_t)d.nb1 * d.nb2 * d.nb3;
+  // LDS-tiled spread when every axis has a tile edge in [order-1, 8] dividing K;
+  // the bins are sized for the smallest tiles (the most bins any tuning uses)
+  const int e1 = spread_tile_edge(P.K1, P.order, true), e2 = spread_tile_edge(P.K2, P.order, true),
+            e3 = spread_tile_edge(P.K3, P.order, true);
+  if (e1 && e2 && e3) {
+    const size_t nbin = (size_t)(P.K1 / e1) * (P.K2 / e2) * (P.K3 / e3);
     d.bin_cnt = B.get<int>(pfx + "bin_cnt", nbin);
     d.bin_start = B.get<int>(pfx + "bin_start", nbin);
     d.atom_bin = B.get<int2>(pfx + "atom_bin", std::max(1, natoms));
     d.afr = B.get<double4>(pfx + "afr", std::max(1, natoms));
     d.sfr = B.get<double4>(pfx + "sfr", std::max(1, natoms));
+    d.swt = B.get<double>(pfx + "swt", (size_t)std::max(1, natoms) * 3 * P.order);
   }
   pme_apply_tuning(d, tu);
   return d;
 }
 
+// Spread path of a tuning: 0 global atomics; 1 tiles of the largest edges
+// (the default: every atom's stencil reaches 8 tiles whatever the edge, and
+// large tiles give each workgroup the most work); 2 the smallest edges.
 void pme_apply_tuning(PmeDesc &d, const Tuning &tu) {
   pme_set_tiles(d, tu.lds_elems);
-  d.tiled = (d.st1 > 0 && tu.spread_tiled && !std::getenv("ORAC_HIP_SPREAD_ATOMIC")) ? 1 : 0;
+  d.tiled = 0;
+  if (!d.bin_cnt || !tu.spread_tiled || std::getenv("ORAC_HIP_SPREAD_ATOMIC")) return;
+  const int K[3] = {d.K1, d.K2, d.K3};
+  int big[3], small[3];
+  for (int a = 0; a < 3; ++a) {
+    big[a] = spread_tile_edge(K[a], d.order, false);
+    small[a] = spread_tile_edge(K[a], d.order, true);
+  }
+  const int *e = tu.spread_tiled == 2 ? small : big;
+  d.st1 = e[0];
+  d.st2 = e[1];
+  d.st3 = e[2];
+  d.nb1 = K[0] / e[0];
+  d.nb2 = K[1] / e[1];
+  d.nb3 = K[2] / e[2];
+  d.tiled = 1;
 }
 
 double erfc_as(double x) {

@@ -15,7 +15,8 @@ namespace orac {
 struct Tuning {
   int lds_elems = 512;  // complex elements per LDS buffer in the FFT kernels
   int pair_waves = 0;   // occupancy target of the pair kernel (0: default for the precision)
-  int spread_tiled = 1; // LDS-tiled spread where the grid allows it, else global atomics
+  int spread_tiled = 1; // LDS-tiled spread where the grid allows it (1 largest tiles, 2 smallest),
+                        // 0 global atomics
   int xcd_remap = 1;    // keep each system's pair blocks on one XCD (L2 locality)
   int cell_order = 0;   // pair kernel owners and neighbour records in cell order (1) or group order (0)
   int persistent = 0;   // pair kernel as a fixed grid looping over the batch's work items
@@ -27,7 +28,12 @@ void pme_set_tiles(PmeDesc &d, int lds_elems);
 template <typename T>
 void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st, const Tuning &tu);
 // tile edge of the LDS-tiled spread along an axis of K points (0: use the atomic spread)
-int spread_tile_edge(int K, int order);
+int spread_tile_edge(int K, int order, bool smallest);
+// Waves per owner group of the pair kernel for a system of ngrp groups (1
+// unless ORAC_HIP_PAIR_SPLIT says 2 or 4 when the system is created).  A
+// property of the system, never of a batch or a tuning: it fixes the
+// summation order.
+int pair_split_for(int ngrp);
 
 // ---------------------------------------------------------------- direct space
 
@@ -42,6 +48,8 @@ struct PairDesc {
   double *xpcm, *ypcm, *zpcm;       // molecule centres (reduced), written by the centre kernels
   int nmol;
   const int *mfirst, *mlast;        // atom range of each molecule
+  const int *mbig;                  // molecules of more than MOL_SMALL atoms (one wavefront each)
+  int nbig;
   const int *gfirst, *glast;        // atom range of each group
   const int *gss;                   // ss_index of the group's first atom (1 solute, 2 solvent)
   double *xpg, *ypg, *zpg;          // group centres, written by the centre kernels
@@ -63,6 +71,7 @@ struct PairDesc {
   double *fx, *fy, *fz;             // direct forces (written)
   double *part;                     // per-block partials [nblk][16]: uconf[3] ucoul[3] st[9]
   int nblk;
+  int split;                        // waves per owner group (1, 2 or 4), fixed per system
   // Cell-ordered copies written every evaluation by k_pack_sorted.  Sorted
   // position t holds group perm[t] (t itself without a permutation), whose
   // atoms start at sorted atom soff[t] (gfirst[t] without).
@@ -77,6 +86,7 @@ struct PairDesc {
 };
 
 constexpr int PAIR_WAVES = 4;       // owner groups per 256-thread block
+constexpr int MOL_SMALL = 16;       // molecules up to this size get one thread in k_mol_com
 
 // kbeg/kend (optional) are recorded right before and after the pair kernel itself
 template <typename T>
@@ -103,6 +113,7 @@ struct ListDesc {
   const double *xpg, *ypg, *zpg;
   CellGrid cg;
   int *cell_of, *cell_cnt, *cell_start, *cell_cursor, *cell_items;
+  int *cell_fill;            // [ngrp] cell members in arrival order, before the sort
   int *count;                // [ngrp] entries per group
   int *off;                  // [ngrp+1] CSR offsets (mode 0) / prefix of count (mode 1)
   int *out;                  // list entries

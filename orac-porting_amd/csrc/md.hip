@@ -39,6 +39,7 @@ __device__ inline bool shake3(double3 *q, const double3 *p, const double *im, co
   const int ci[3] = {0, 0, 1}, cj[3] = {1, 2, 2};
   for (int it = 0; it < maxit; ++it) {
     bool done = true;
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int i = ci[c], j = cj[c];
       const double sx = q[i].x - q[j].x, sy = q[i].y - q[j].y, sz = q[i].z - q[j].z;
@@ -64,6 +65,7 @@ __device__ inline bool rattle3(const double3 *r, double3 *v, const double *im, d
   const int ci[3] = {0, 0, 1}, cj[3] = {1, 2, 2};
   for (int it = 0; it < maxit; ++it) {
     bool done = true;
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int i = ci[c], j = cj[c];
       const double rx = r[i].x - r[j].x, ry = r[i].y - r[j].y, rz = r[i].z - r[j].z;
@@ -88,41 +90,42 @@ constexpr int CONSTRAINT_MAXIT = 500;
 
 // Kick (v += c F/m, F = ff (fast) or the slow forces), optionally a drift by h
 // with SHAKE, optionally RATTLE.  flags: 1 kick slow, 2 kick fast, 4 drift,
-// 8 rattle.  One thread per unit.
-__global__ void __launch_bounds__(256) k_md_units(const MdDesc *__restrict__ D, double cslow, double cfast,
-                                                  double h, int flags) {
-  const MdDesc &d = D[blockIdx.y];
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= d.nunits) return;
-  const int a0 = d.ufirst[u], na = d.urigid[u] ? 3 : 1;
-  double3 x[3], v[3], p[3], dv[3];
-  double im[3];
-  for (int k = 0; k < na; ++k) {
+// 8 rattle.  One thread per unit; NA atoms per unit as a template parameter
+// so the unit's arrays stay in registers.
+template <int NA>
+__device__ __forceinline__ void md_unit(const MdDesc &d, int u, int a0, double cslow, double cfast, double h,
+                                        int flags) {
+  double3 x[NA], v[NA], p[NA], dv[NA];
+  double im[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) {
     const int a = a0 + k;
-    im[k] = d.invm[a];
-    x[k] = make_double3(d.x[a], d.y[a], d.z[a]);
-    v[k] = make_double3(d.vx[a], d.vy[a], d.vz[a]);
+    im[k] = gmem(d.invm)[a];
+    x[k] = make_double3(gmem(d.x)[a], gmem(d.y)[a], gmem(d.z)[a]);
+    v[k] = make_double3(gmem(d.vx)[a], gmem(d.vy)[a], gmem(d.vz)[a]);
     if (flags & 1) {
-      v[k].x += cslow * im[k] * d.fx[a];
-      v[k].y += cslow * im[k] * d.fy[a];
-      v[k].z += cslow * im[k] * d.fz[a];
+      v[k].x += cslow * im[k] * gmem(d.fx)[a];
+      v[k].y += cslow * im[k] * gmem(d.fy)[a];
+      v[k].z += cslow * im[k] * gmem(d.fz)[a];
     }
     if (flags & 2) {
-      v[k].x += cfast * im[k] * d.bfx[a];
-      v[k].y += cfast * im[k] * d.bfy[a];
-      v[k].z += cfast * im[k] * d.bfz[a];
+      v[k].x += cfast * im[k] * gmem(d.bfx)[a];
+      v[k].y += cfast * im[k] * gmem(d.bfy)[a];
+      v[k].z += cfast * im[k] * gmem(d.bfz)[a];
     }
   }
   if (flags & 4) {
-    for (int k = 0; k < na; ++k) {
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
       p[k] = x[k];
       x[k].x += h * v[k].x;
       x[k].y += h * v[k].y;
       x[k].z += h * v[k].z;
       dv[k] = make_double3(0.0, 0.0, 0.0);
     }
-    if (na == 3) {
+    if constexpr (NA == 3) {
       if (!shake3(x, p, im, d.ud2 + 3 * (size_t)u, h, dv, SHAKE_TOL, CONSTRAINT_MAXIT)) atomicAdd(d.fail, 1);
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
         v[k].x += dv[k].x;
         v[k].y += dv[k].y;
@@ -130,27 +133,40 @@ __global__ void __launch_bounds__(256) k_md_units(const MdDesc *__restrict__ D, 
       }
     }
   }
-  if ((flags & 8) && na == 3)
-    if (!rattle3(x, v, im, RATTLE_TOL, CONSTRAINT_MAXIT)) atomicAdd(d.fail, 1);
-  for (int k = 0; k < na; ++k) {
+  if constexpr (NA == 3)
+    if (flags & 8)
+      if (!rattle3(x, v, im, RATTLE_TOL, CONSTRAINT_MAXIT)) atomicAdd(d.fail, 1);
+#pragma unroll
+  for (int k = 0; k < NA; ++k) {
     const int a = a0 + k;
-    d.vx[a] = v[k].x;
-    d.vy[a] = v[k].y;
-    d.vz[a] = v[k].z;
+    gmem(d.vx)[a] = v[k].x;
+    gmem(d.vy)[a] = v[k].y;
+    gmem(d.vz)[a] = v[k].z;
     if (flags & 4) {
-      d.x[a] = x[k].x;
-      d.y[a] = x[k].y;
-      d.z[a] = x[k].z;
+      gmem(d.x)[a] = x[k].x;
+      gmem(d.y)[a] = x[k].y;
+      gmem(d.z)[a] = x[k].z;
       double xp, yp, zp;
       to_reduced(d.oc, x[k].x, x[k].y, x[k].z, xp, yp, zp);
-      d.xp[a] = xp;
-      d.yp[a] = yp;
-      d.zp[a] = zp;
+      gmem(d.xp)[a] = xp;
+      gmem(d.yp)[a] = yp;
+      gmem(d.zp)[a] = zp;
     }
   }
 }
 
-// harmonic distance terms E = k (r - r0)^2: per-term force and energy
+__global__ void __launch_bounds__(256) k_md_units(const MdDesc *__restrict__ D, double cslow, double cfast,
+                                                  double h, int flags) {
+  const MdDesc &d = D[blockIdx.y];
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= d.nunits) return;
+  const int a0 = gmem(d.ufirst)[u];
+  if (gmem(d.urigid)[u])
+    md_unit<3>(d, u, a0, cslow, cfast, h, flags);
+  else
+    md_unit<1>(d, u, a0, cslow, cfast, h, flags);
+}
+
 __global__ void __launch_bounds__(256) k_md_bonds(const MdDesc *__restrict__ D) {
   const MdDesc &d = D[blockIdx.y];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -215,18 +231,17 @@ __global__ void __launch_bounds__(256) k_md_energy(const MdDesc *__restrict__ D)
 
 // steepest-descent relaxation step: each unit moves along its force by at
 // most dmax (per atom h F/m capped at dmax), then SHAKE; velocities are zeroed
-__global__ void __launch_bounds__(256) k_md_relax(const MdDesc *__restrict__ D, double hstep, double dmax) {
-  const MdDesc &d = D[blockIdx.y];
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= d.nunits) return;
-  const int a0 = d.ufirst[u], na = d.urigid[u] ? 3 : 1;
-  double3 x[3], p[3], dv[3];
-  double im[3];
-  for (int k = 0; k < na; ++k) {
+template <int NA>
+__device__ __forceinline__ void md_relax_unit(const MdDesc &d, int u, int a0, double hstep, double dmax) {
+  double3 x[NA], p[NA], dv[NA];
+  double im[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) {
     const int a = a0 + k;
-    im[k] = d.invm[a];
-    p[k] = make_double3(d.x[a], d.y[a], d.z[a]);
-    double fx = d.fx[a] + d.bfx[a], fy = d.fy[a] + d.bfy[a], fz = d.fz[a] + d.bfz[a];
+    im[k] = gmem(d.invm)[a];
+    p[k] = make_double3(gmem(d.x)[a], gmem(d.y)[a], gmem(d.z)[a]);
+    double fx = gmem(d.fx)[a] + gmem(d.bfx)[a], fy = gmem(d.fy)[a] + gmem(d.bfy)[a],
+           fz = gmem(d.fz)[a] + gmem(d.bfz)[a];
     double sx = hstep * im[k] * fx, sy = hstep * im[k] * fy, sz = hstep * im[k] * fz;
     const double s = sqrt(sx * sx + sy * sy + sz * sz);
     if (!(s <= dmax)) {  // also catches a non-finite step
@@ -238,20 +253,36 @@ __global__ void __launch_bounds__(256) k_md_relax(const MdDesc *__restrict__ D, 
     x[k] = make_double3(p[k].x + sx, p[k].y + sy, p[k].z + sz);
     dv[k] = make_double3(0.0, 0.0, 0.0);
   }
-  if (na == 3)
+  if constexpr (NA == 3)
     if (!shake3(x, p, im, d.ud2 + 3 * (size_t)u, 1.0, dv, SHAKE_TOL, CONSTRAINT_MAXIT)) atomicAdd(d.fail, 1);
-  for (int k = 0; k < na; ++k) {
+#pragma unroll
+  for (int k = 0; k < NA; ++k) {
     const int a = a0 + k;
-    d.x[a] = x[k].x;
-    d.y[a] = x[k].y;
-    d.z[a] = x[k].z;
-    d.vx[a] = d.vy[a] = d.vz[a] = 0.0;
+    gmem(d.x)[a] = x[k].x;
+    gmem(d.y)[a] = x[k].y;
+    gmem(d.z)[a] = x[k].z;
+    gmem(d.vx)[a] = 0.0;
+    gmem(d.vy)[a] = 0.0;
+    gmem(d.vz)[a] = 0.0;
     double xp, yp, zp;
     to_reduced(d.oc, x[k].x, x[k].y, x[k].z, xp, yp, zp);
-    d.xp[a] = xp;
-    d.yp[a] = yp;
-    d.zp[a] = zp;
+    gmem(d.xp)[a] = xp;
+    gmem(d.yp)[a] = yp;
+    gmem(d.zp)[a] = zp;
   }
+}
+
+// one steepest-descent step of every unit: displacement hstep F/m capped at
+// dmax per atom, rigid units put back on their constraints, velocities zeroed
+__global__ void __launch_bounds__(256) k_md_relax(const MdDesc *__restrict__ D, double hstep, double dmax) {
+  const MdDesc &d = D[blockIdx.y];
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= d.nunits) return;
+  const int a0 = gmem(d.ufirst)[u];
+  if (gmem(d.urigid)[u])
+    md_relax_unit<3>(d, u, a0, hstep, dmax);
+  else
+    md_relax_unit<1>(d, u, a0, hstep, dmax);
 }
 
 // v *= scale (per system), e.g. for velocity rescaling during equilibration

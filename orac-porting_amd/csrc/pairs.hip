@@ -51,37 +51,64 @@ __global__ void __launch_bounds__(256) k_group_centres(const PairDesc *__restric
   d.zpg[g] = sz;
 }
 
-// one wavefront per molecule: lanes stride over its atoms, fixed-order butterfly sum
-__global__ void __launch_bounds__(256) k_mol_com(const PairDesc *__restrict__ D) {
+// Molecule centres of mass.  Blocks [0, nbb) give each large molecule (more
+// than MOL_SMALL atoms) a wavefront: lanes stride over its atoms, fixed-order
+// butterfly sum.  The blocks after them give every small molecule (a water) a
+// thread that sums its atoms in order.  Each molecule has one fixed path.
+__global__ void __launch_bounds__(256) k_mol_com(const PairDesc *__restrict__ D, int nbb) {
   const PairDesc &d = D[blockIdx.y];
-  const int lane = threadIdx.x & 63;
-  const int m = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if ((int)blockIdx.x < nbb) {
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (k >= d.nbig) return;
+    const int m = gmem(d.mbig)[k];
+    double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
+    for (int a = gmem(d.mfirst)[m] + lane; a <= gmem(d.mlast)[m]; a += 64) {
+      const double w = gmem(d.mass)[a];
+      sx += w * gmem(d.xp)[a];
+      sy += w * gmem(d.yp)[a];
+      sz += w * gmem(d.zp)[a];
+      sm += w;
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    sz = wave_sum(sz);
+    sm = wave_sum(sm);
+    if (lane == 0) {
+      gmem(d.xpcm)[m] = sx / sm;
+      gmem(d.ypcm)[m] = sy / sm;
+      gmem(d.zpcm)[m] = sz / sm;
+    }
+    return;
+  }
+  const int m = (blockIdx.x - nbb) * blockDim.x + threadIdx.x;
   if (m >= d.nmol) return;
+  const int a0 = gmem(d.mfirst)[m], a1 = gmem(d.mlast)[m];
+  if (a1 - a0 + 1 > MOL_SMALL) return;
   double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
-  for (int a = d.mfirst[m] + lane; a <= d.mlast[m]; a += 64) {
-    sx += d.mass[a] * d.xp[a];
-    sy += d.mass[a] * d.yp[a];
-    sz += d.mass[a] * d.zp[a];
-    sm += d.mass[a];
+  for (int a = a0; a <= a1; ++a) {
+    const double w = gmem(d.mass)[a];
+    sx += w * gmem(d.xp)[a];
+    sy += w * gmem(d.yp)[a];
+    sz += w * gmem(d.zp)[a];
+    sm += w;
   }
-  sx = wave_sum(sx);
-  sy = wave_sum(sy);
-  sz = wave_sum(sz);
-  sm = wave_sum(sm);
-  if (lane == 0) {
-    d.xpcm[m] = sx / sm;
-    d.ypcm[m] = sy / sm;
-    d.zpcm[m] = sz / sm;
-  }
+  gmem(d.xpcm)[m] = sx / sm;
+  gmem(d.ypcm)[m] = sy / sm;
+  gmem(d.zpcm)[m] = sz / sm;
 }
 
 void launch_centres(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream_t st) {
-  int max_mol = 0, max_grp = 0;
+  int max_mol = 0, max_grp = 0, max_big = 0;
   for (int s = 0; s < nsys; ++s) {
     max_mol = std::max(max_mol, h[s].mfirst ? h[s].nmol : 0);
+    max_big = std::max(max_big, h[s].mfirst ? h[s].nbig : 0);
     max_grp = std::max(max_grp, h[s].ngrp);
   }
-  if (max_mol > 0) hipLaunchKernelGGL(k_mol_com, dim3((max_mol + 3) / 4, nsys), dim3(256), 0, st, d_desc);
+  if (max_mol > 0) {
+    const int nbb = (max_big + 3) / 4;
+    hipLaunchKernelGGL(k_mol_com, dim3(nbb + (max_mol + 255) / 256, nsys), dim3(256), 0, st, d_desc, nbb);
+  }
   if (max_grp > 0) hipLaunchKernelGGL(k_group_centres, dim3((max_grp + 255) / 256, nsys), dim3(256), 0, st, d_desc);
 }
 
@@ -147,25 +174,27 @@ __global__ void __launch_bounds__(256) k_cell_fill(const ListDesc *__restrict__ 
   if (g >= d.ngrp || !list_due(d)) return;
   const int c = d.cell_of[g];
   const int pos = atomicAdd(&d.cell_cursor[c], 1);
-  d.cell_items[d.cell_start[c] + pos] = g;
+  d.cell_fill[d.cell_start[c] + pos] = g;
 }
 
 // each cell's members in descending group order: the order of the
-// reference's head-inserted chains (lc_list.f:104-108)
+// reference's head-inserted chains (lc_list.f:104-108).  One wavefront per
+// cell ranks its members (group indices are distinct): a member's place is
+// the number of members above it, so the cost per cell is n^2 / 64 compares
+// and one cell holding every group (the single-cell fallback of a small box)
+// is as cheap as many small ones.
 __global__ void __launch_bounds__(256) k_cell_sort(const ListDesc *__restrict__ D) {
   const ListDesc &d = D[blockIdx.y];
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (c > d.cg.ncx * d.cg.ncy * d.cg.ncz || !list_due(d)) return;
-  int *items = d.cell_items;
-  const int s = d.cell_start[c], n = d.cell_cnt[c];
-  for (int i = 1; i < n; ++i) {
-    int v = items[s + i];
-    int j = i - 1;
-    while (j >= 0 && items[s + j] < v) {
-      items[s + j + 1] = items[s + j];
-      --j;
-    }
-    items[s + j + 1] = v;
+  const int s = gmem(d.cell_start)[c], n = gmem(d.cell_cnt)[c];
+  const auto in = gmem(d.cell_fill + s);
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + lane;
+    const int v = i < n ? in[i] : -1;
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += in[j] > v ? 1 : 0;
+    if (i < n) d.cell_items[s + rank] = v;
   }
 }
 
@@ -176,8 +205,8 @@ __global__ void __launch_bounds__(256) k_cell_pack(const ListDesc *__restrict__ 
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= d.ngrp || !list_due(d)) return;
   const int g = d.cell_items[t];
-  d.cg.cpos[t] = make_double4(d.xpg[g], d.ypg[g], d.zpg[g], (double)g);
-  if (d.soff && d.cellorder) d.soff[t] = d.gsize[g];  // group sizes in cell order; scanned in place below
+  d.cg.cpos[t] = make_double4(gmem(d.xpg)[g], gmem(d.ypg)[g], gmem(d.zpg)[g], (double)g);
+  if (d.soff && d.cellorder) d.soff[t] = gmem(d.gsize)[g];  // group sizes in cell order; scanned in place below
 }
 
 // first atom of each group in cell order (exclusive prefix of the sizes), one block per system
@@ -241,27 +270,55 @@ template <bool WRITE>
 __device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, int entry, int cap = 0x7fffffff) {
   const CellGrid &cg = d.cg;
   const int lane = threadIdx.x & 63;
-  const double x1 = d.xpg[n], y1 = d.ypg[n], z1 = d.zpg[n];
-  const int c = d.cell_of[n];
+  const double x1 = gmem(d.xpg)[n], y1 = gmem(d.ypg)[n], z1 = gmem(d.zpg)[n];
+  const int c = gmem(d.cell_of)[n];
   if (c >= cg.ncx * cg.ncy * cg.ncz) return 0;  // parked (non-finite centre)
   const int i = c / (cg.ncy * cg.ncz), j = (c / cg.ncz) % cg.ncy, k = c % cg.ncz;
   int base = 0;
+  if (cg.nsten < 64) {
+    // few, crowded stencil cells (small grids, the single-cell fallback of a
+    // small box): lanes take the members of one cell at a time, same order
+    for (int m = 0; m < cg.nsten; ++m) {
+      const int nx = fmodp(i + gmem(cg.sten)[3 * m], cg.ncx);
+      const int ny = fmodp(j + gmem(cg.sten)[3 * m + 1], cg.ncy);
+      const int nz = fmodp(k + gmem(cg.sten)[3 * m + 2], cg.ncz);
+      const int cc = nz + cg.ncz * (ny + cg.ncy * nx);
+      const int s = gmem(d.cell_start)[cc], e = s + gmem(d.cell_cnt)[cc];
+      for (int t0 = s; t0 < e; t0 += 64) {
+        const int t = t0 + lane;
+        bool hit = false;
+        int l = 0;
+        if (t < e) {
+          const double4 p = gload(cg.cpos + t);
+          l = (int)p.w;
+          hit = !(cg.half ? l <= n : l == n) && list_hit(x1, y1, z1, p, cg.co, cg.sqcut);
+        }
+        const unsigned long long mask = __ballot(hit);
+        if (WRITE && hit) {
+          const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+          if (pos < cap) out[pos] = entry == 1 ? t : (entry == 0 ? l + 1 : l);
+        }
+        base += __popcll(mask);
+      }
+    }
+    return base;
+  }
   for (int m0 = 0; m0 < cg.nsten; m0 += 64) {
     const int m = m0 + lane;
     int s = 0, e = 0;
     if (m < cg.nsten) {
-      int nx = fmodp(i + cg.sten[3 * m], cg.ncx);
-      int ny = fmodp(j + cg.sten[3 * m + 1], cg.ncy);
-      int nz = fmodp(k + cg.sten[3 * m + 2], cg.ncz);
+      int nx = fmodp(i + gmem(cg.sten)[3 * m], cg.ncx);
+      int ny = fmodp(j + gmem(cg.sten)[3 * m + 1], cg.ncy);
+      int nz = fmodp(k + gmem(cg.sten)[3 * m + 2], cg.ncz);
       int cc = nz + cg.ncz * (ny + cg.ncy * nx);
-      s = d.cell_start[cc];
-      e = s + d.cell_cnt[cc];
+      s = gmem(d.cell_start)[cc];
+      e = s + gmem(d.cell_cnt)[cc];
     }
     // hits of this lane's cell as a bit mask over its (at most 64) first members
     int cnt = 0;
     unsigned long long hits = 0ull;
     for (int t = s; t < e; ++t) {
-      const double4 p = cg.cpos[t];
+      const double4 p = gload(cg.cpos + t);
       const int l = (int)p.w;
       if (cg.half ? l <= n : l == n) continue;
       if (list_hit(x1, y1, z1, p, cg.co, cg.sqcut)) {
@@ -277,9 +334,9 @@ __device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, int ent
         int l;
         if (r < 64) {
           hit = (hits >> r) & 1ull;
-          l = (hit && entry != 1) ? (int)cg.cpos[t].w : 0;
+          l = (hit && entry != 1) ? (int)gload(cg.cpos + t).w : 0;
         } else {
-          const double4 p = cg.cpos[t];
+          const double4 p = gload(cg.cpos + t);
           l = (int)p.w;
           hit = !(cg.half ? l <= n : l == n) && list_hit(x1, y1, z1, p, cg.co, cg.sqcut);
         }
@@ -324,22 +381,26 @@ __global__ void __launch_bounds__(1024) k_list_offsets(const ListDesc *__restric
   }
 }
 
+// One instantiation per layout (a batch uses one), so each holds one walk:
+// LW_REF the reference's count-prefixed layout, LW_SLAB one slab per owner,
+// LW_CSR offsets from the count pass.
+enum { LW_REF = 0, LW_SLAB = 1, LW_CSR = 2 };
+template <int LAYOUT>
 __global__ void __launch_bounds__(256) k_list_write(const ListDesc *__restrict__ D) {
   const ListDesc &d = D[blockIdx.y];
   const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (o >= d.ngrp || d.total[1] || !list_due(d)) return;
-  if (d.mode == 1) {
-    const int w = d.off[o] + o;  // reference layout: one count slot per preceding group
+  if (LAYOUT == LW_REF) {
+    const int w = d.off[o] + o;  // one count slot per preceding group
     const int nv = list_walk_wave<true>(o, d, d.out + w + 1, 0);
     if ((threadIdx.x & 63) == 0) d.out[w] = nv;
-  } else if (d.slab > 0) {
-    const int c = list_walk_wave<true>(d.cellorder ? d.cell_items[o] : o, d, d.out + (size_t)o * d.slab,
-                                       d.cellorder ? 1 : 2, d.slab);
+  } else if (LAYOUT == LW_SLAB) {
+    const int n = d.cellorder ? d.cell_items[o] : o;
+    const int c = list_walk_wave<true>(n, d, d.out + (size_t)o * d.slab, d.cellorder ? 1 : 2, d.slab);
     if ((threadIdx.x & 63) == 0) d.count[o] = c;
-  } else if (d.cellorder) {
-    list_walk_wave<true>(d.cell_items[o], d, d.out + d.off[o], 1);
   } else {
-    list_walk_wave<true>(o, d, d.out + d.off[o], 2);
+    const int n = d.cellorder ? d.cell_items[o] : o;
+    list_walk_wave<true>(n, d, d.out + d.off[o], d.cellorder ? 1 : 2);
   }
 }
 
@@ -373,18 +434,18 @@ void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int 
     max_cell = std::max(max_cell, h[s].cg.ncx * h[s].cg.ncy * h[s].cg.ncz + 1);  // + the parking cell
     max_grp = std::max(max_grp, h[s].ngrp);
   }
-  const dim3 b(256), gg((max_grp + 255) / 256, nsys), gc((max_cell + 255) / 256, nsys),
-      gw((max_grp + 3) / 4, nsys);
+  const dim3 b(256), gg((max_grp + 255) / 256, nsys), gw((max_grp + 3) / 4, nsys);
   // a batch uses one layout: all slab (resident systems) or none (plain lc_list)
   const bool slab = h[0].slab > 0;
   for (int s = 1; s < nsys; ++s)
-    if ((h[s].slab > 0) != slab) throw Error(E_STATE, "list layouts differ within a batch");
+    if ((h[s].slab > 0) != slab || h[s].mode != h[0].mode) throw Error(E_STATE, "list layouts differ within a batch");
+  const int layout = h[0].mode == 1 ? LW_REF : (slab ? LW_SLAB : LW_CSR);
   if (phases & LIST_COUNT) {
     hipLaunchKernelGGL(k_list_zero, dim3(std::min(64, (max_cell + 255) / 256), nsys), b, 0, st, d_desc);
     hipLaunchKernelGGL(k_cell_assign, gg, b, 0, st, d_desc);
     hipLaunchKernelGGL(k_cell_scan, dim3(nsys), dim3(1024), 0, st, d_desc);
     hipLaunchKernelGGL(k_cell_fill, gg, b, 0, st, d_desc);
-    hipLaunchKernelGGL(k_cell_sort, gc, b, 0, st, d_desc);
+    hipLaunchKernelGGL(k_cell_sort, dim3((max_cell + 3) / 4, nsys), b, 0, st, d_desc);
     hipLaunchKernelGGL(k_cell_pack, gg, b, 0, st, d_desc);
     hipLaunchKernelGGL(k_soff_scan, dim3(nsys), dim3(1024), 0, st, d_desc);
     if (!slab || !(phases & LIST_WRITE)) {
@@ -397,7 +458,12 @@ void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int 
     }
   }
   if (phases & LIST_WRITE) {
-    hipLaunchKernelGGL(k_list_write, gw, b, 0, st, d_desc);
+    if (layout == LW_REF)
+      hipLaunchKernelGGL(k_list_write<LW_REF>, gw, b, 0, st, d_desc);
+    else if (layout == LW_SLAB)
+      hipLaunchKernelGGL(k_list_write<LW_SLAB>, gw, b, 0, st, d_desc);
+    else
+      hipLaunchKernelGGL(k_list_write<LW_CSR>, gw, b, 0, st, d_desc);
     if (slab) hipLaunchKernelGGL(k_list_total, dim3(nsys), dim3(1024), 0, st, d_desc);
   }
 }

@@ -244,28 +244,63 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const PmeDesc *__restrict__ D
   block_exclusive_scan(d.bin_cnt, d.bin_start, d.nb1 * d.nb2 * d.nb3);
 }
 
+// atoms into bin order, with their B-spline weights along the three axes
+// (computed once here instead of once per tile and stencil plane)
+template <typename T>
 __global__ void __launch_bounds__(256) k_bin_fill(const PmeDesc *__restrict__ D) {
   const PmeDesc &d = D[blockIdx.y];
   int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (!d.tiled || a >= d.natoms) return;
   int2 bp = d.atom_bin[a];
   if (bp.x < 0) return;
-  d.sfr[d.bin_start[bp.x] + bp.y] = d.afr[a];
+  const int s = d.bin_start[bp.x] + bp.y, order = d.order;
+  const double4 f = d.afr[a];
+  d.sfr[s] = f;
+  double *w = d.swt + (size_t)s * 3 * order;
+  T th[MAX_ORDER], dd[MAX_ORDER];
+  fill_bspline<T>(T(f.x - (int)f.x), order, th, dd);
+  for (int i = 0; i < order; ++i) w[i] = (double)th[i];
+  fill_bspline<T>(T(f.y - (int)f.y), order, th, dd);
+  for (int i = 0; i < order; ++i) w[order + i] = (double)th[i];
+  fill_bspline<T>(T(f.z - (int)f.z), order, th, dd);
+  for (int i = 0; i < order; ++i) w[2 * order + i] = (double)th[i];
 }
 
 constexpr int SPREAD_TILE_MAX = 8;
 
+// Stencil points i = 0..order-1 at tile-local offset (off + i) mod K: returns
+// (off mod K) and the range [ia, ib) of i whose point lies in [0, st).
+__device__ inline int tile_range(int off, int K, int st, int order, int &ia, int &ib) {
+  const int l0 = ((off % K) + K) % K;
+  if (st >= K) {  // one tile along the axis: every point, wrapped or not
+    ia = 0;
+    ib = order;
+  } else if (l0 < st) {  // (with two or more tiles, st >= order - 1 keeps wrapped points out)
+    ia = 0;
+    ib = min(order, st - l0);
+  } else {
+    ia = min(order, K - l0);  // the first point that wraps to 0
+    ib = min(order, K - l0 + st);
+  }
+  return l0;
+}
+
+// The tile is held in LDS with padded strides (st1 + 1, st2 + 1): z-planes and
+// y-rows of one column then fall on different banks, which the int64 atomics
+// of neighbouring stencil points would otherwise serialise on.
+constexpr int SPREAD_THREADS = 512;  // per tile: ~2 work items per thread at 8^3 tiles, order 5
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_spread_tile(const PmeDesc *__restrict__ D) {
-  __shared__ unsigned long long tile[SPREAD_TILE_MAX * SPREAD_TILE_MAX * SPREAD_TILE_MAX];
+__global__ void __launch_bounds__(SPREAD_THREADS) k_spread_tile(const PmeDesc *__restrict__ D) {
+  __shared__ unsigned long long tile[(SPREAD_TILE_MAX + 1) * (SPREAD_TILE_MAX + 1) * SPREAD_TILE_MAX];
   const PmeDesc &d = D[blockIdx.y];
   if (!d.tiled || (int)blockIdx.x >= d.nb1 * d.nb2 * d.nb3) return;
   const int order = d.order;
   const int T1 = blockIdx.x % d.nb1, T2 = (blockIdx.x / d.nb1) % d.nb2, T3 = blockIdx.x / (d.nb1 * d.nb2);
-  const int st1 = d.st1, st2 = d.st2, st3 = d.st3;
+  const int st1 = d.st1, st2 = d.st2, st3 = d.st3, P1 = st1 + 1, P2 = st2 + 1;
   const int t01 = T1 * st1, t02 = T2 * st2, t03 = T3 * st3;
   const int npts = st1 * st2 * st3;
-  for (int i = threadIdx.x; i < npts; i += blockDim.x) tile[i] = 0ull;
+  for (int i = threadIdx.x; i < P1 * P2 * st3; i += blockDim.x) tile[i] = 0ull;
   __syncthreads();
   const double scale = d.fixscale;
   for (int sb = 0; sb < 8; ++sb) {
@@ -282,19 +317,22 @@ __global__ void __launch_bounds__(256) k_spread_tile(const PmeDesc *__restrict__
       const int kk = b3 - order + i3 + 1;
       const int lz = ((kk - t03) % d.K3 + d.K3) % d.K3;
       if (lz >= st3) continue;
-      T th1[MAX_ORDER], th2[MAX_ORDER], th3[MAX_ORDER], dd[MAX_ORDER];
-      fill_bspline<T>(T(f.x - b1), order, th1, dd);
-      fill_bspline<T>(T(f.y - b2), order, th2, dd);
-      fill_bspline<T>(T(f.z - b3), order, th3, dd);
-      const T q3 = th3[i3] * T(f.w);
-      for (int i2 = 0; i2 < order; ++i2) {
-        const int ly = ((b2 - order + i2 + 1 - t02) % d.K2 + d.K2) % d.K2;
-        if (ly >= st2) continue;
-        const T prod = th2[i2] * q3;
-        for (int i1 = 0; i1 < order; ++i1) {
-          const int lx = ((b1 - order + i1 + 1 - t01) % d.K1 + d.K1) % d.K1;
-          if (lx >= st1) continue;
-          atomicAdd(&tile[lx + st1 * (ly + st2 * lz)], to_fix((double)(th1[i1] * prod), scale));
+      const double *wt = d.swt + (size_t)(s + at) * 3 * order;
+      const T q3 = T(wt[2 * order + i3]) * T(f.w);
+      // tile-local x and y of the first stencil point; the next ones step by
+      // one and wrap once (order <= K), so the points inside the tile are one
+      // contiguous range of i1 (and of i2)
+      int i1a, i1b, i2a, i2b;
+      const int lx0 = tile_range(b1 - order + 1 - t01, d.K1, st1, order, i1a, i1b);
+      const int ly0 = tile_range(b2 - order + 1 - t02, d.K2, st2, order, i2a, i2b);
+      for (int i2 = i2a; i2 < i2b; ++i2) {
+        int ly = ly0 + i2;
+        if (ly >= d.K2) ly -= d.K2;
+        const T prod = T(wt[order + i2]) * q3;
+        for (int i1 = i1a; i1 < i1b; ++i1) {
+          int lx = lx0 + i1;
+          if (lx >= d.K1) lx -= d.K1;
+          atomicAdd(&tile[lx + P1 * (ly + P2 * lz)], to_fix((double)(T(wt[i1]) * prod), scale));
         }
       }
     }
@@ -302,13 +340,21 @@ __global__ void __launch_bounds__(256) k_spread_tile(const PmeDesc *__restrict__
   __syncthreads();
   for (int i = threadIdx.x; i < npts; i += blockDim.x) {
     const int lx = i % st1, ly = (i / st1) % st2, lz = i / (st1 * st2);
-    d.Qfix[(size_t)(t01 + lx) + (size_t)d.K1 * ((t02 + ly) + (size_t)d.K2 * (t03 + lz))] = tile[i];
+    d.Qfix[(size_t)(t01 + lx) + (size_t)d.K1 * ((t02 + ly) + (size_t)d.K2 * (t03 + lz))] =
+        tile[lx + P1 * (ly + P2 * lz)];
   }
 }
 
-// tile edge along one axis: a divisor of K in [order - 1, 8], largest first; 0 if none
-int spread_tile_edge(int K, int order) {
-  for (int s = SPREAD_TILE_MAX; s >= 4 && s >= order - 1; --s)
+// tile edge along one axis: a divisor of K in [max(order - 1, 2), 8], the
+// largest or the smallest; 0 if none
+int spread_tile_edge(int K, int order, bool smallest) {
+  const int lo = std::max(2, order - 1);
+  if (smallest) {
+    for (int s = lo; s <= SPREAD_TILE_MAX; ++s)
+      if (K % s == 0) return s;
+    return 0;
+  }
+  for (int s = SPREAD_TILE_MAX; s >= lo; --s)
     if (K % s == 0) return s;
   return 0;
 }
@@ -706,8 +752,8 @@ void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t s
     hipLaunchKernelGGL(k_bin_zero, dim3((max_bins + 255) / 256, nsys), blk, 0, st, d_desc);
     hipLaunchKernelGGL(k_bin_count, ga, blk, 0, st, d_desc);
     hipLaunchKernelGGL(k_bin_scan, dim3(nsys), dim3(1024), 0, st, d_desc);
-    hipLaunchKernelGGL(k_bin_fill, ga, blk, 0, st, d_desc);
-    hipLaunchKernelGGL(k_spread_tile<T>, dim3(max_bins, nsys), blk, 0, st, d_desc);
+    hipLaunchKernelGGL(k_bin_fill<T>, ga, blk, 0, st, d_desc);
+    hipLaunchKernelGGL(k_spread_tile<T>, dim3(max_bins, nsys), dim3(SPREAD_THREADS), 0, st, d_desc);
   }
   if (any_atomic) {
     unsigned gzero = (unsigned)std::min<size_t>((max_grid + 255) / 256, 4096);

@@ -51,6 +51,7 @@ def lib():
 
 EXPORTS = [
     "orac_hip_create", "orac_hip_destroy", "orac_hip_last_error", "orac_hip_invalidate", "orac_hip_device_name",
+    "orac_hip_probe_copy",
     "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
     "orac_hip_decide_force", "orac_hip_ferrf", "orac_hip_cself", "orac_hip_fnbgrp", "orac_hip_system_create",
     "orac_hip_system_destroy", "orac_hip_system_set_coords", "orac_hip_system_device_ptrs", "orac_hip_system_eval",
@@ -124,6 +125,10 @@ class Context:
         buf = C.create_string_buffer(64)
         _check(lib().orac_hip_device_name(self._h, buf, 64))
         return buf.value.decode()
+
+    def probe_copy(self, nbytes):
+        """One device copy kernel of nbytes (read) + nbytes (written); calibrates byte counters."""
+        _check(lib().orac_hip_probe_copy(self._h, C.c_int64(nbytes)))
 
     def invalidate(self):
         _check(lib().orac_hip_invalidate(self._h))

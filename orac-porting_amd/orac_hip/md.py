@@ -119,10 +119,10 @@ def scale_velocities(batch, scale):
     _check(lib().orac_hip_batch_md_scale_velocities(batch.ctx._h, batch.id, _p(sc)))
 
 
-def equilibrate(batch, dyns, temperature, nsteps, every=10, dt=2.0, ninner=4, list_every=10):
+def equilibrate(batch, dyns, temperature, nsteps, every=10, dt=2.0, ninner=4, list_every=10, graph=True):
     """Velocity rescaling to temperature every `every` steps (preparation only)."""
     for _ in range(max(1, nsteps // every)):
-        run_batch(batch, every, dt=dt, ninner=ninner, list_every=list_every)
+        run_batch(batch, every, dt=dt, ninner=ninner, list_every=list_every, graph=graph)
         batch.ctx.sync()
         sc = [np.sqrt(temperature / max(d.energies()["temperature"], 1e-6)) for d in dyns]
         scale_velocities(batch, sc)

@@ -2,11 +2,15 @@
 
     python -m orac_hip.rocprof_summary stats <kernel_stats.csv> <out.txt> "<profiled command>"
     python -m orac_hip.rocprof_summary pmc <counter_collection.csv> <out.txt> "<profiled command>"
+    python -m orac_hip.rocprof_summary traffic <fetch.csv> <write.csv> "<kernel name part>" <probe bytes> \
+        <out.txt> "<FETCH_SIZE command>" "<WRITE_SIZE command>"
 
 ``stats`` lists every kernel's calls and average/min/max/total duration from
 ``rocprofv3 --kernel-trace --stats --output-format csv``.  ``pmc`` averages
 each counter per dispatch and kernel from ``rocprofv3 --pmc ... --output-format
-csv`` (FETCH_SIZE and WRITE_SIZE are in KiB per dispatch).  Column names are
+csv`` (FETCH_SIZE and WRITE_SIZE are in KiB per dispatch).  ``traffic`` turns
+the FETCH_SIZE and WRITE_SIZE runs of one kernel into HBM bytes per launch,
+scaled by the counters' reading on k_probe_copy.  Column names are
 matched loosely, so minor CSV layout changes between rocprofv3 versions do
 not matter.
 """
@@ -76,7 +80,71 @@ def pmc(path, out, cmd):
             f.write(f"{len(vals):10d} {sum(vals) / len(vals):14.1f}  {cn:10s} {name[:100]}\n")
 
 
+def _pmc_avg(path, kernel_part, counter):
+    """Average per dispatch of `counter` for the kernels whose name contains kernel_part."""
+    with open(path, newline="") as f:
+        rows = list(csv.reader(f))
+    h, body = rows[0], rows[1:]
+    iname = _col(h, "kernel_name", "kernel-name", "name", exclude=("counter",))
+    icn = _col(h, "counter_name", "counter-name")
+    icv = _col(h, "counter_value", "counter-value", "value")
+    idisp = _col(h, "dispatch_id", "dispatch-id", "correlation")
+    per = defaultdict(float)
+    for r in body:
+        if kernel_part in r[iname] and r[icn] == counter:
+            per[r[idisp]] += float(r[icv])
+    if not per:
+        raise KeyError(f"no {counter} rows for a kernel like {kernel_part!r} in {path}")
+    return sum(per.values()) / len(per), len(per)
+
+
+def traffic(fetch_csv, write_csv, kernel, probe_bytes, out, cmd_fetch, cmd_write):
+    """HBM bytes per launch of `kernel`: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) each scaled by what the counter reads for k_probe_copy, a plain
+    copy kernel that reads and writes probe_bytes."""
+    f_k, nf = _pmc_avg(fetch_csv, kernel, "FETCH_SIZE")
+    w_k, nw = _pmc_avg(write_csv, kernel, "WRITE_SIZE")
+    f_p, _ = _pmc_avg(fetch_csv, "k_probe_copy", "FETCH_SIZE")
+    w_p, _ = _pmc_avg(write_csv, "k_probe_copy", "WRITE_SIZE")
+    fs = probe_bytes / (f_p * 1024.0)
+    ws = probe_bytes / (w_p * 1024.0)
+    fetched, written = f_k * 1024.0 * fs, w_k * 1024.0 * ws
+    with open(out, "w") as f:
+        f.write(f"command (FETCH_SIZE): {cmd_fetch}\n")
+        f.write(f"command (WRITE_SIZE): {cmd_write}\n")
+        f.write("HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the\n")
+        f.write("same counter on k_probe_copy, a plain copy kernel reading and writing probe_copy_bytes\n\n")
+        f.write(f"kernel: {kernel}\n")
+        f.write(f"dispatches: {nf} (FETCH_SIZE), {nw} (WRITE_SIZE)\n")
+        f.write(f"fetch_kib_per_dispatch: {f_k:.1f}\n")
+        f.write(f"write_kib_per_dispatch: {w_k:.1f}\n")
+        f.write(f"probe_copy_bytes: {probe_bytes}\n")
+        f.write(f"probe_fetch_kib: {f_p:.1f}\n")
+        f.write(f"probe_write_kib: {w_p:.1f}\n")
+        f.write(f"fetch_scale: {fs:.4f}\n")
+        f.write(f"write_scale: {ws:.4f}\n")
+        f.write(f"fetched_bytes_per_launch: {fetched:.0f}\n")
+        f.write(f"written_bytes_per_launch: {written:.0f}\n")
+        f.write(f"traffic_bytes_per_launch: {fetched + written:.0f}\n")
+
+
+def read_traffic(path):
+    """traffic_bytes_per_launch from a summary written by traffic(), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def main(argv):
+    if len(argv) == 9 and argv[1] == "traffic":
+        traffic(argv[2], argv[3], argv[4], int(argv[5]), argv[6], argv[7], argv[8])
+        print(open(argv[6]).read())
+        return
     if len(argv) != 5 or argv[1] not in ("stats", "pmc"):
         sys.exit(__doc__)
     (stats if argv[1] == "stats" else pmc)(argv[2], argv[3], argv[4])

@@ -241,6 +241,21 @@ def ferrf(s, pairs, iz, fudge=1.0, fp=None):
     return dict(slt=a.value, slv=b.value, f=np.stack([fpx, fpy, fpz]))
 
 
+def fnbgrp(s, ingrp, typ, ewald):
+    """fnbgrp.f on the Cartesian coordinates of System s: pairs ingrp (2, n)
+    1-based, per-atom LJ types typ (1-based, packed j(j-1)/2 + i tables)."""
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    e = [C.c_double() for _ in range(4)]
+    lst = _i(ingrp)
+    lib().orc_fnbgrp(_p(_i(s.ss_index)), _p(_f(x)), _p(_f(y)), _p(_f(z)), _p(_f(s.charge)), _p(_i(typ)),
+                     _p(_f(s.ecc12)), _p(_f(s.ecc6)), C.c_int32(int(ewald)), C.c_double(s.alpha), _p(lst),
+                     C.c_int(lst.size // 2), _p(fpx), _p(fpy), _p(fpz), *[C.byref(v) for v in e])
+    return dict(uconf_slt=e[0].value, uconf_slv=e[1].value, ucoul_slt=e[2].value, ucoul_slv=e[3].value,
+                f=np.stack([fpx, fpy, fpz]))
+
+
 def cself(s, kcut=1e9):
     a, b = C.c_double(), C.c_double()
     lib().orc_cself(_p(_i(s.ss_index)), C.c_int(s.natoms), C.c_double(s.alpha), C.c_double(kcut), _p(_f(s.charge)),
@@ -388,6 +403,20 @@ def ref_ferrf(s, pairs, iz, fudge=1.0):
              _p(lst), _ri(lst.size // 2), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _ri(0), _p(dummy),
              _rd(0.0), _rd(0.0), _ri(0), _ri(n))
     return dict(slt=a.value, slv=b.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def ref_fnbgrp(s, ingrp, typ, ewald):
+    """The reference fnbgrp (serial) with the arguments of fnbgrp() above."""
+    R = ref()
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    e = [C.c_double() for _ in range(4)]
+    keep = [_i(s.ss_index), _f(x), _f(y), _f(z), _f(s.charge), _i(typ), _f(s.ecc12), _f(s.ecc6), _i(ingrp)]
+    R.fnbgrp_(*[_p(k) for k in keep[:8]], _ri(int(ewald)), _rd(s.alpha), _p(keep[8]), _ri(keep[8].size // 2),
+              _p(fpx), _p(fpy), _p(fpz), *[C.byref(v) for v in e], _ri(n), _ri(1))
+    return dict(uconf_slt=e[0].value, uconf_slv=e[1].value, ucoul_slt=e[2].value, ucoul_slv=e[3].value,
+                f=np.stack([fpx, fpy, fpz]))
 
 
 def ref_cself(s, kcut=1e9):

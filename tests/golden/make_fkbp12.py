@@ -8,8 +8,9 @@ cutoff 10 A, list skin 1.8 A, 18^3 linked cells) and writes:
 
 * the system's arrays (inputs of every routine of the path);
 * the reference Fortran's results on it (oracle/_ref, compiled in place):
-  the linked-cell list, force() on the outer shell, fft_pme (fftpack branch)
-  and ferrf on the solute's bonds and 1-3 pairs.  Exact arrays are stored as
+  the linked-cell list, force() on the outer shell, fft_pme (fftpack branch),
+  ferrf on the solute's bonds and 1-3 pairs, and fnbgrp (with and without
+  Ewald) on the intra-group pairs and on the bond + 1-3 lists.  Exact arrays are stored as
   SHA-256 digests of their bytes; energies, stress and a sample of the PME
   forces as values.
 
@@ -34,6 +35,26 @@ TEST1 = "/root/reference/test1"
 PARAMS = dict(alpha=0.37, nfft=(48, 48, 48), order=4, rcut=10.0, rtol=0.3, rnei=1.8)
 NCELL = 18
 E6 = ("uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss")
+
+
+FNB_E = ("uconf_slt", "uconf_slv", "ucoul_slt", "ucoul_slv")
+
+
+def intra_group_pairs(s):
+    """Atom pairs (i < j, 1-based) inside one group that are not excluded
+    (exclusions: constraints, bonds and 1-3 pairs), as (2, n)."""
+    ex = set()
+    for p in (s.constraints, s.bonds, s.bends):
+        for a, b in p.T:
+            ex.add((min(a, b), max(a, b)))
+    out = []
+    for g in range(s.grppt.shape[1]):
+        f0, f1 = int(s.grppt[0, g]), int(s.grppt[1, g])
+        for i in range(f0, f1 + 1):
+            for j in range(i + 1, f1 + 1):
+                if (i, j) not in ex:
+                    out.append((i, j))
+    return np.array(out, dtype=np.int32).T.reshape(2, -1)
 
 
 def digest(a):
@@ -71,6 +92,15 @@ def main():
     pairs = np.concatenate([s.bonds, s.bends], axis=1)
     c = O.ref_ferrf(s, pairs, 1, 1.0)
     out["ref_ferrf_e"], out["ref_ferrf_f_sha"] = np.array([c["slt"], c["slv"]]), digest(c["f"])
+    # fnbgrp: the solute's intra-group pairs that are not excluded (what the
+    # reference hands it), and the bond + 1-3 lists as a larger pair set
+    intra = intra_group_pairs(s)
+    out["fnb_intra"] = intra.astype(np.int16)
+    for name, lst in (("intra", intra), ("bb", pairs)):
+        for ew in (1, 0):
+            r = O.ref_fnbgrp(s, lst, s.nbtype, bool(ew))
+            out[f"ref_fnbgrp_{name}{ew}_e"] = np.array([r[k] for k in FNB_E])
+            out[f"ref_fnbgrp_{name}{ew}_f_sha"] = digest(r["f"])
     path = os.path.join(HERE, "fkbp12.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")

@@ -114,7 +114,8 @@ def test_autotune_keeps_forces(gpu_ctx, mixed_sizes):
     assert ms > 0 and tun["lds_elems"] in (256, 512, 1024) and tun["pair_waves"] in (0, 2, 3, 4)
     for t in ({"lds_elems": 256, "pair_waves": 2, "spread_tiled": 0, "xcd_remap": 0, "cell_order": 1, "persistent": 1},
               {"lds_elems": 1024, "pair_waves": 4, "spread_tiled": 1, "xcd_remap": 1, "cell_order": 0, "persistent": 0},
-              {"lds_elems": 512, "pair_waves": 3, "spread_tiled": 1, "xcd_remap": 1, "cell_order": 1, "persistent": 1},
+              {"lds_elems": 512, "pair_waves": 3, "spread_tiled": 2, "xcd_remap": 1, "cell_order": 1, "persistent": 1},
+              {"lds_elems": 512, "pair_waves": 4, "spread_tiled": 1, "xcd_remap": 1, "cell_order": 0, "persistent": 0},
               tun):
         b.set_tuning(**t)
         b.eval(rebuild_list=True)

@@ -69,15 +69,23 @@ def test_force_matches_reference(gpu_ctx, fk, fk_list):
 
 
 def test_fft_pme_matches_reference(gpu_ctx, fk):
+    """The GPU follows the reference's FFTW branch (the oracle's 'fftw'
+    variant) to the FP64 tolerances.  The stored reference values come from
+    its fftpack branch, which differs from the FFTW branch by ~3e-8 in energy
+    and ~1e-7 in stress on this system (DFTMOD's REAL(4) 2 pi makes |b(m)|^2
+    and |b(-m)|^2 differ; the FFTW branch weights mirror pairs by the +m
+    modulus), so against those the bound is 3e-7."""
     from test_gpu_parity import _pme_gpu, _stress_close
     s, _, g = fk
     r = _pme_gpu(gpu_ctx, s)
-    print("fkbp12 eer", r["eer"], float(g["ref_pme_eer"]))
-    assert rel(r["eer"], float(g["ref_pme_eer"])) < 1e-10
-    assert _stress_close(r["stress"], g["ref_pme_stress"])
-    assert relrms(r["f"][:, g["ref_pme_f_idx"]], g["ref_pme_f_sample"]) < 1e-9
     o = O.fft_pme(s, nthreads=4)
+    print("fkbp12 eer", r["eer"], o["eer"], float(g["ref_pme_eer"]))
+    assert rel(r["eer"], o["eer"]) < 1e-10
+    assert _stress_close(r["stress"], o["stress"])
     assert relrms(r["f"], o["f"]) < 1e-9
+    assert rel(r["eer"], float(g["ref_pme_eer"])) < 3e-7
+    assert _stress_close(r["stress"], g["ref_pme_stress"], tol=3e-7)
+    assert relrms(r["f"][:, g["ref_pme_f_idx"]], g["ref_pme_f_sample"]) < 3e-7
 
 
 def test_ferrf_matches_reference(gpu_ctx, fk):
@@ -119,3 +127,20 @@ def test_resident_matches_oracle(gpu_ctx, fk, precision):
         assert err < 5e-6
         assert rel(r["eer"], o["eer"]) < 1e-5
     rs.close()
+
+
+@pytest.mark.parametrize("name", ["intra", "bb"])
+def test_fnbgrp_matches_oracle(gpu_ctx, fk, name):
+    s, _, g = fk
+    lst = g["fnb_intra"].astype(np.int32) if name == "intra" else np.concatenate([s.bonds, s.bends], axis=1)
+    lst = np.asfortranarray(lst)
+    x, y, z = s.cartesian()
+    n = s.natoms
+    for ewald in (True, False):
+        fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+        e = gpu_ctx.fnbgrp(s.ss_index, x, y, z, s.charge, s.nbtype, s.ecc12, s.ecc6, ewald, s.alpha, lst,
+                           lst.shape[1], fpx, fpy, fpz)
+        o = O.fnbgrp(s, lst, s.nbtype, ewald)
+        assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
+        for got, k in zip(e, ("uconf_slt", "uconf_slv", "ucoul_slt", "ucoul_slv")):
+            assert (got == 0.0 and o[k] == 0.0) or rel(got, o[k]) < 1e-12, k

@@ -138,6 +138,25 @@ def test_lc_list_exact(gpu_ctx, sysname, request):
     assert np.array_equal(out[:n], ref)
 
 
+@pytest.mark.parametrize("sysname", ["small_solute", "solvated24k"])
+def test_lc_list_exact_coarse_cells(gpu_ctx, sysname, request):
+    """3 cells per side: the stencil has fewer than 64 cells and each cell
+    many groups, so the walk takes the members of one cell across the lanes;
+    the list must still be the reference's, entry for entry."""
+    s = request.getfixturevalue(sysname)
+    rc = s.rcut + s.rtol + s.rnei
+    fa = O.ForceArgs(s)
+    nc = 3
+    so = O.lc_index(nc, nc, nc, rc, s.co)
+    assert so[0][1] < 64
+    nind, ii, jj, kk = gpu_ctx.lc_index(2500, nc, nc, nc, rc, s.co)
+    ref = O.lc_list(nc, nc, nc, so, rc, s.co, fa.xpg, fa.ypg, fa.zpg)
+    out = np.zeros(len(ref) + 100, np.int32)
+    n = gpu_ctx.lc_list(nc, nc, nc, nind, ii, jj, kk, rc, s.co, fa.xpg, fa.ypg, fa.zpg, s.ngrp, out)
+    assert n == len(ref)
+    assert np.array_equal(out[:n], ref)
+
+
 def test_lc_list_capacity_error(gpu_ctx, small_solute):
     from orac_hip.api import OracHipError
     s = small_solute

@@ -84,3 +84,18 @@ def test_ferrf_matches_reference(fk):
     c = O.ferrf(s, pairs, 1, 1.0)
     assert np.array_equal(np.array([c["slt"], c["slv"]]), g["ref_ferrf_e"])
     assert np.array_equal(digest(c["f"]), g["ref_ferrf_f_sha"])
+
+
+FNB_E = ("uconf_slt", "uconf_slv", "ucoul_slt", "ucoul_slv")
+
+
+@pytest.mark.parametrize("name", ["intra", "bb"])
+@pytest.mark.parametrize("ewald", [1, 0])
+def test_fnbgrp_matches_reference(fk, name, ewald):
+    """fnbgrp on the solute's non-excluded intra-group pairs (the 1-4 pairs
+    that share a group) and on the bond + 1-3 lists, with and without Ewald."""
+    s, _, g = fk
+    lst = g["fnb_intra"].astype(np.int32) if name == "intra" else np.concatenate([s.bonds, s.bends], axis=1)
+    r = O.fnbgrp(s, lst, s.nbtype, bool(ewald))
+    assert np.array_equal(np.array([r[k] for k in FNB_E]), g[f"ref_fnbgrp_{name}{ewald}_e"])
+    assert np.array_equal(digest(r["f"]), g[f"ref_fnbgrp_{name}{ewald}_f_sha"])
