@@ -390,7 +390,7 @@ extern "C" int orac_hip_force(orac_hip_ctx *ctx, const int32_t *ss_index, const 
     p.zpg = d_zpg;
     p.ntypes = ma;
     const size_t npk = (size_t)ma * (ma + 1) / 2;
-    if (ma <= 256) {
+    if (ma <= LJ_DENSE_MAX) {
       std::vector<double2> dense((size_t)ma * ma);
       for (int i = 0; i < ma; ++i)
         for (int j = 0; j < ma; ++j) {

@@ -248,7 +248,7 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   p.zpg = zpg;
   p.ntypes = ma;
   const size_t npk = (size_t)ma * (ma + 1) / 2;
-  if (ma <= 256) {
+  if (ma <= LJ_DENSE_MAX) {
     std::vector<double2> dense((size_t)ma * ma);
     for (int i = 0; i < ma; ++i)
       for (int j = 0; j < ma; ++j) {

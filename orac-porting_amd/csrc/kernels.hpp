@@ -87,6 +87,10 @@ struct PairDesc {
 
 constexpr int PAIR_WAVES = 4;       // owner groups per 256-thread block
 constexpr int MOL_SMALL = 16;       // molecules up to this size get one thread in k_mol_com
+// LJ parameters as a dense ntypes x ntypes table of {c12, c6} (one gather per
+// pair) up to this many types (64 MB); beyond it the packed table and
+// type_table of the reference (two dependent gathers per pair)
+constexpr int LJ_DENSE_MAX = 2048;
 
 // kbeg/kend (optional) are recorded right before and after the pair kernel itself
 template <typename T>

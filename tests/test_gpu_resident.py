@@ -129,3 +129,41 @@ def test_pair_split_keeps_forces(gpu_ctx, solvated24k, split, monkeypatch):
     for k in ("uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"):
         if o["d"][k] != 0.0:
             assert rel(g[k], o["d"][k]) < 1e-10, k
+
+
+def test_packed_lj_table_path(gpu_ctx, small_solute):
+    """More LJ types than the dense table takes (2,049+): the pair kernel reads
+    the reference's packed type_table and ecc12/ecc6.  The system's own types
+    are moved to the end of a 2,100-type table with random other entries."""
+    import dataclasses
+    from orac_hip import synth
+    from orac_hip.api import ResidentSystem
+    s = small_solute
+    nt0, nt = s.ntypes, 2100
+    rng = np.random.default_rng(5)
+    sig, eps = rng.uniform(2.0, 3.5, nt), rng.uniform(0.01, 0.3, nt)
+    tt, e12, e6 = synth._lj_tables(sig, eps)
+    # types t (1-based) of the system become nt - nt0 + t; copy their pair parameters
+    off = nt - nt0
+    for i in range(nt0):
+        for j in range(nt0):
+            src = s.type_table[i, j] - 1
+            dst = tt[off + i, off + j] - 1
+            e12[dst], e6[dst] = s.ecc12[src], s.ecc6[src]
+    s2 = dataclasses.replace(s, ntypes=nt, type_table=tt, ecc12=e12, ecc6=e6,
+                             nbtype=(s.nbtype + off).astype(np.int32))
+    cp, cf = _corr(s2)
+    rs = ResidentSystem(gpu_ctx, s2, precision=0, corr_pairs=cp, corr_fudge=cf)
+    rs.eval(rebuild_list=True)
+    g = rs.results()
+    rs.close()
+    o = oracle_total(s2, cp, cf)
+    assert relrms(g["f"], o["f"]) < 1e-9
+    for k in ("uconf_slt", "uconf_slv", "uconf_ss"):
+        if o["d"][k] != 0.0:
+            assert rel(g[k], o["d"][k]) < 1e-10, k
+    # and the same physics as with the system's own small table
+    rs = ResidentSystem(gpu_ctx, s, precision=0, corr_pairs=cp, corr_fudge=cf)
+    rs.eval(rebuild_list=True)
+    assert relrms(rs.results()["f"], g["f"]) < 1e-12
+    rs.close()
