@@ -410,3 +410,113 @@ def small_solvated(seed=5, L=24.0, n_solute=40, radius=5.0, n_water=380, nfft=(2
     """A small solute-in-water box for parity tests that exercise solute exclusions."""
     return solvated_solute(seed=seed, L=L, n_solute=n_solute, radius=radius, n_water=n_water, alpha=alpha,
                            nfft=nfft, order=order, rcut=rcut)
+
+
+# ---- alchemy ---------------------------------------------------------------
+
+@dataclasses.dataclass
+class Alchemy:
+    """Alchemical description of a System, in force_alchemy.f's terms.
+
+    ss_alchemy: per atom 1 (normal), 2 (added), -2 (removed) (cpropar,
+    initialize_lambda.f:35,49).  rmin, epsm: per-atom soft-core LJ parameters,
+    sigma_ij = rmin_i + rmin_j (2 sqrt(rmin_i rmin_j) for OPLS) and
+    epsmin = 4 sqrt(epsm_i epsm_j) in internal energy units
+    (force_alchemy.f:431-437).  qsoft, ljsoft: Beutler soft-core parameters
+    (defaults 0.5, blkdta.f90:477-478); intramol, ljintra: the INTRAMOL switches
+    of alch_set (defaults false, blkdta.f90:473-474).
+    """
+    ss_alchemy: np.ndarray
+    rmin: np.ndarray
+    epsm: np.ndarray
+    qsoft: float = 0.5
+    ljsoft: float = 0.5
+    intramol: bool = False
+    ljintra: bool = False
+    opls: bool = False
+
+
+def lambda_state(lrm, lrm0=None, lrmq=None, lrmq0=None, ladd=1.0, ladd0=None, laddq=None, laddq0=None):
+    """The eight lambdas force_alchemy.f reads, in the order the library and
+    the oracle take them: ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0.
+    Unset lambda0 values equal lambda; unset charge lambdas (and their
+    lambda0) equal the LJ ones."""
+    lrm0 = lrm if lrm0 is None else lrm0
+    ladd0 = ladd if ladd0 is None else ladd0
+    if lrmq0 is None:
+        lrmq0 = lrm0 if lrmq is None else lrmq
+    if laddq0 is None:
+        laddq0 = ladd0 if laddq is None else laddq
+    lrmq = lrm if lrmq is None else lrmq
+    laddq = ladd if laddq is None else laddq
+    return np.array([ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0], dtype=np.float64)
+
+
+def alchemy_charges(charge, ss_alchemy, laddq, lrmq):
+    """Charges the reciprocal sum and the corrections see at (laddq, lrmq):
+    per-atom lambdaq = laddq (added), -lrmq (removed), 0 (normal)
+    (mtsmd.f:748-757), charge (1 - |lambdaq|) q with the end points exact
+    (mts_furier_alchemy.f:98-111)."""
+    lq = np.where(ss_alchemy == 2, laddq, np.where(ss_alchemy == -2, -lrmq, 0.0)).astype(np.float64)
+    a = np.abs(lq)
+    return np.where(a == 1.0, 0.0, np.where(a == 0.0, charge, (1.0 - a) * charge))
+
+
+def alchemy_for(s, alchemical_atoms, removed=True, **kw):
+    """An Alchemy for System s whose atoms alchemical_atoms (0-based) are
+    removed (or added): rmin = sigma / 2 and epsm = eps (internal units) of
+    each atom's LJ type, so that at full coupling the soft-core pair is the
+    system's own Lorentz-Berthelot LJ pair; atoms without LJ get rmin 0.1
+    (mtsmd.f:1300-1303)."""
+    n = s.natoms
+    ss = np.ones(n, dtype=np.int32)
+    ss[np.asarray(alchemical_atoms, dtype=np.int64)] = -2 if removed else 2
+    t = s.nbtype - 1
+    lij = s.type_table[t, t] - 1                      # diagonal (i, i) entries
+    c12, c6 = s.ecc12[lij], s.ecc6[lij]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        sig = np.where(c6 > 0, (c12 / np.where(c6 > 0, c6, 1.0)) ** (1.0 / 6.0), 0.0)
+        eps4 = np.where(c12 > 0, c6 * c6 / np.where(c12 > 0, c12, 1.0), 0.0)   # 4 eps
+    epsm = eps4 / 4.0
+    rmin = np.where(epsm == 0.0, 0.1, sig / 2.0)
+    return Alchemy(ss_alchemy=ss, rmin=rmin, epsm=epsm, **kw)
+
+
+def alchemy_windows(n_windows=16, seed=3001, L=40.0, n_ligand=48, radius=5.5, n_water=2117, alpha=0.35,
+                    nfft=(48, 48, 48), order=5, rcut=9.0):
+    """lambda-windows of one ligand decoupled in water: a 48-atom ligand
+    (neutral charges ~N(0, 0.3) e, random spanning-chain exclusions) in a
+    40 A box of rigid waters (6,399 atoms), grid 48^3, order 5, cutoff 9 A.
+    Window w removes the ligand at lambda_w = w / (n - 1) for both LJ and
+    charge; its lambda0 is the next window's (the last window's, the one
+    before), so each window's dwrk is the energy step to its neighbour.
+    Returns (system, alchemy, [lambda_state per window])."""
+    rng = np.random.default_rng(seed)
+    pos = _rsa_sphere(rng, n_ligand, radius, 1.2)
+    q = rng.normal(0.0, 0.3, n_ligand)
+    q = q - q.mean()
+    sig = rng.uniform(2.0, 3.8, n_ligand)
+    eps = rng.uniform(0.02, 0.2, n_ligand)
+    bonds, bends = _chain_exclusions(pos)
+    nside = int(math.ceil(L / 3.1067))
+    while True:
+        a = L / nside
+        g = (np.arange(nside) + 0.5) * a - L / 2.0
+        o = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+        far = np.sqrt(np.sum(o * o, axis=1)) > radius + 1.5
+        if far.sum() >= n_water:
+            break
+        nside += 1
+    cand = o[far]
+    pick = np.sort(rng.choice(len(cand), n_water, replace=False))
+    o = cand[pick] + rng.uniform(-0.3, 0.3, size=(n_water, 3))
+    sites = _place_waters(rng, o)
+    s = _assemble(L, pos, q, sig, eps, sites, alpha=alpha, nfft=nfft, order=order, rcut=rcut, solute_bonds=bonds,
+                  solute_bends=bends, name=f"ligand{n_ligand}+water{n_water}")
+    al = alchemy_for(s, np.arange(n_ligand))
+    lam = np.linspace(0.0, 1.0, n_windows) if n_windows > 1 else np.zeros(1)
+    states = []
+    for w in range(n_windows):
+        l0 = lam[w + 1] if w + 1 < n_windows else (lam[w - 1] if w > 0 else lam[w])
+        states.append(lambda_state(float(lam[w]), lrm0=float(l0)))
+    return s, al, states

@@ -424,3 +424,171 @@ def ref_cself(s, kcut=1e9):
     a, b = C.c_double(), C.c_double()
     R.cself_(_p(_i(s.ss_index)), _ri(s.natoms), _rd(s.alpha), _rd(kcut), _p(_f(s.charge)), C.byref(a), C.byref(b))
     return a.value, b.value
+
+
+# ---------------------------------------------------------------- alchemy
+
+E6 = ("uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss")
+
+
+def alch_set(ladd, lrm, intramol=False, ljintra=False, qq=False):
+    """alch_fact(1:2, -4:4) as an (9, 2) array indexed [typeaij + 4]."""
+    f = np.zeros(18)
+    lib().orc_alch_set(_p(f), C.c_double(ladd), C.c_double(lrm), C.c_int(int(intramol)), C.c_int(int(ljintra)),
+                       C.c_int(int(qq)))
+    return f.reshape(9, 2)
+
+
+def force_alchemy(s, al, lam, nnlpp, *, rinn, rout, rtolinn, rtolout, rneigh, iz=1, lskip_ewald=False,
+                  ewald=True, fa=None, massinfty=1.0e20, dwrk=0.0):
+    """force_alchemy.f (serial) on System s with Alchemy al and the eight
+    lambdas lam (synth.lambda_state order)."""
+    fa = fa or ForceArgs(s)
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    st = np.zeros(9)
+    e6 = np.zeros(6)
+    cap = len(nnlpp) + s.ngrp + 16 if iz == 1 else s.ngrp * (s.ngrp // 2 + 2)
+    nn1 = np.zeros(cap, dtype=np.int32)
+    nc1 = C.c_int32()
+    rti = C.c_double(rtolinn)
+    dw = C.c_double(dwrk)
+    rc = lib().orc_force_alchemy(
+        _p(_i(s.ss_index)), _p(_f(s.xp0)), _p(_f(s.yp0)), _p(_f(s.zp0)), _p(fa.xpg), _p(fa.ypg), _p(fa.zpg),
+        _p(_f(s.charge)), _p(_i(al.ss_alchemy)), _p(_f(al.rmin)), _p(_f(al.epsm)), C.c_int(n), _p(fa.xpcm),
+        _p(fa.ypcm), _p(fa.zpcm), _p(_i(s.atomp)), _p(_f(s.co)), C.c_double(s.alpha), _p(fa.mapnl),
+        C.c_int(s.ngrp), _p(fa.grppt), _p(e6), _p(fpx), _p(fpy), _p(fpz), _p(st), _p(_i(nnlpp)), _p(nn1),
+        C.c_long(cap), C.c_double(rneigh), C.c_double(rinn), C.c_double(rout), C.byref(rti), C.c_double(rtolout),
+        _p(_f(s.pmass)), C.c_int(iz), C.byref(nc1), _p(_f(s.mass)), C.c_double(massinfty), _p(_f(lam)),
+        C.c_int(int(al.intramol)), C.c_int(int(al.ljintra)), C.c_int(int(al.opls)), C.byref(dw),
+        C.c_int32(int(lskip_ewald)), C.c_int32(int(ewald)), C.c_double(al.qsoft), C.c_double(al.ljsoft))
+    if rc != 0:
+        raise RuntimeError("orc_force_alchemy: nnlpp1 capacity exceeded")
+    out = dict(zip(E6, e6.tolist()))
+    out.update(f=np.stack([fpx, fpy, fpz]), stress=st.reshape(3, 3, order="F"), dwrk=dw.value,
+               nnlpp1=nn1[:nc1.value].copy(), ncount1=nc1.value)
+    return out
+
+
+def ferrf_alchemy(s, charge, charge0, pairs, iz, fudge=1.0):
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b, dw = C.c_double(), C.c_double(), C.c_double()
+    lst = _i(pairs)
+    lib().orc_ferrf_alchemy(_p(_i(s.ss_index)), C.c_double(s.alpha), _p(_f(charge)), _p(_f(charge0)),
+                            C.c_double(fudge), _p(_f(x)), _p(_f(y)), _p(_f(z)), C.c_int(iz), _p(lst),
+                            C.c_int(lst.size // 2), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), C.c_int(0),
+                            C.byref(dw))
+    return dict(slt=a.value, slv=b.value, dwrk=dw.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def cself_alchemy(s, charge, charge0, kcut=1e9):
+    a, dw = C.c_double(), C.c_double()
+    lib().orc_cself_alchemy(_p(_i(s.ss_index)), C.c_int(s.natoms), C.c_double(s.alpha), C.c_double(kcut),
+                            _p(_f(charge)), _p(_f(charge0)), C.byref(a), C.byref(dw))
+    return a.value, dw.value
+
+
+def fft_pme_alchemy(s, charge, charge0, variant="fftw", nthreads=1, rkcut=1e9):
+    """do_pmesh_alchemy.f90: the reciprocal sum on `charge` (energy, forces,
+    virial) and the energy eer0 on `charge0` from the same B-spline weights and
+    influence function (scalar_sum_alchemy.f90: the same terms summed in the
+    same order as scalar_sum); dwrk = eer - eer0."""
+    import dataclasses
+    r = fft_pme(dataclasses.replace(s, charge=np.asarray(charge, float)), nthreads=nthreads, rkcut=rkcut,
+                variant=variant)
+    r0 = fft_pme(dataclasses.replace(s, charge=np.asarray(charge0, float)), nthreads=nthreads, rkcut=rkcut,
+                 variant=variant)
+    r.update(eer0=r0["eer"], dwrk=r["eer"] - r0["eer"])
+    return r
+
+
+def ref_set_alchemy(al, lam):
+    ref().refglue_set_alchemy(C.c_int(len(al.ss_alchemy)), _p(_i(al.ss_alchemy)), _p(_f(lam)),
+                              C.c_int(int(al.intramol)), C.c_int(int(al.ljintra)), C.c_int(int(al.opls)))
+
+
+def ref_force_alchemy(s, al, lam, nnlpp, *, rinn, rout, rtolinn, rtolout, rneigh, iz=1, lskip_ewald=False,
+                      ewald=True, massinfty=1.0e20, fa=None, dwrk=0.0):
+    """The reference force_alchemy (serial build) with the arguments of force_alchemy() above."""
+    R = ref()
+    ref_set_alchemy(al, lam)
+    fa = fa or ForceArgs(s)
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    st = np.zeros(9)
+    e = [C.c_double() for _ in range(6)]
+    cap = len(nnlpp) + s.ngrp + 16 if iz == 1 else s.ngrp * (s.ngrp // 2 + 2)
+    nn1 = np.zeros(cap, dtype=np.int32)
+    nc1 = C.c_int32()
+    rti = C.c_double(rtolinn)
+    dw = C.c_double(dwrk)
+    lam_dummy = np.zeros(n)
+    keep = [_i(s.ss_index), _f(s.xp0), _f(s.yp0), _f(s.zp0), _f(s.charge), _i(s.nbtype), _i(s.atomg),
+            _i(s.atomp), _f(s.co), _f(s.ecc12), _f(s.ecc6), _i(nnlpp), _f(s.pmass), _f(s.mass),
+            np.zeros(s.ngrp, dtype=np.int32), _f(al.rmin), _f(al.epsm)]
+    (ss, x, y, z, q, nbt, atg, atp, co, e12, e6, nl, pm, ms, groupp, rmin, epsm) = keep
+
+    def call():
+        R.force_alchemy_(_p(ss), _p(x), _p(y), _p(z), _p(fa.xpg), _p(fa.ypg), _p(fa.zpg), _p(q), _p(nbt),
+                         _p(fa.tt), _ri(s.ntypes), _ri(n), _p(atg), _p(fa.xpcm), _p(fa.ypcm), _p(fa.zpcm),
+                         _p(groupp), _p(atp), _p(co), _p(e12), _p(e6), _rd(s.alpha), _p(fa.mapnl), _ri(s.ngrp),
+                         _p(fa.grppt), *[C.byref(v) for v in e], _p(fpx), _p(fpy), _p(fpz), _p(st), _p(nl),
+                         _p(nn1), _rd(rneigh), _rd(rinn), _rd(rout), C.byref(rti), _rd(rtolout), _p(pm), _ri(iz),
+                         C.byref(nc1), _p(ms), _rd(massinfty), _p(lam_dummy), _p(lam_dummy), _p(lam_dummy),
+                         _p(lam_dummy), _p(rmin), _p(epsm), C.byref(dw), _ri(int(lskip_ewald)), _ri(int(ewald)),
+                         _rd(al.qsoft), _rd(al.ljsoft))
+
+    _big_stack(call)
+    out = {k: v.value for k, v in zip(E6, e)}
+    out.update(f=np.stack([fpx, fpy, fpz]), stress=st.reshape(3, 3, order="F"), dwrk=dw.value,
+               nnlpp1=nn1[:nc1.value].copy(), ncount1=nc1.value)
+    return out
+
+
+def ref_ferrf_alchemy(s, charge, charge0, pairs, iz, fudge=1.0):
+    R = ref()
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b, dw = C.c_double(), C.c_double(), C.c_double()
+    lst = _i(pairs)
+    dummy = np.zeros(8)
+    keep = [_i(s.ss_index), _f(charge), _f(charge0), _f(x), _f(y), _f(z)]
+    R.ferrf_alchemy_(_p(keep[0]), _rd(s.alpha), _p(keep[1]), _p(keep[2]), _rd(fudge), _p(keep[3]), _p(keep[4]),
+                     _p(keep[5]), _ri(iz), _p(lst), _ri(lst.size // 2), C.byref(a), C.byref(b), _p(fpx), _p(fpy),
+                     _p(fpz), _ri(0), _p(dummy), _rd(0.0), _rd(0.0), _ri(0), C.byref(dw), _ri(n))
+    return dict(slt=a.value, slv=b.value, dwrk=dw.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def ref_cself_alchemy(s, charge, charge0, kcut=1e9):
+    R = ref()
+    a, dw = C.c_double(), C.c_double()
+    R.cself_alchemy_(_p(_i(s.ss_index)), _ri(s.natoms), _rd(s.alpha), _rd(kcut), _p(_f(charge)), _p(_f(charge0)),
+                     C.byref(a), C.byref(dw))
+    return a.value, dw.value
+
+
+def ref_fft_pme_alchemy(s, charge, charge0, pressure=True, rkcut=1e9):
+    """The reference fft_pme_alchemy (fftpack branch of do_pmesh_alchemy)."""
+    R = ref()
+    n = s.natoms
+    R.refglue_pme_init(C.c_int(n), C.c_int(s.nfft[0]), C.c_int(s.nfft[1]), C.c_int(s.nfft[2]), C.c_int(s.order))
+    cm = s.molecule_coms()
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    st = np.zeros(9)
+    eer, dw = C.c_double(), C.c_double()
+    arrs = [_f(s.xp0), _f(s.yp0), _f(s.zp0), _f(cm[0]), _f(cm[1]), _f(cm[2]), _f(charge), _f(charge0), _f(s.co),
+            _f(s.oc)]
+    atomp, grppt = _i(s.atomp), _i(s.grppt)
+
+    def call():
+        R.fft_pme_alchemy_(_ri(n), *[_p(a) for a in arrs[:8]], _p(arrs[8]), _p(arrs[9]), _rd(s.volume),
+                           _rd(s.alpha), _ri(s.order), _ri(s.nfft[0]), _ri(s.nfft[1]), _ri(s.nfft[2]),
+                           C.byref(eer), _p(fpx), _p(fpy), _p(fpz), _p(st), _p(atomp), _p(grppt),
+                           _ri(1 if pressure else 0), _rd(rkcut), C.byref(dw))
+
+    _big_stack(call)
+    return dict(eer=eer.value, dwrk=dw.value, eer0=eer.value - dw.value, f=np.stack([fpx, fpy, fpz]),
+                stress=st.reshape(3, 3, order="F"))

@@ -1259,3 +1259,342 @@ void orc_fnbgrp(const int32_t *ss_index, const double *xp0, const double *yp0, c
   *uconf_slt = uconf[0]; *uconf_slv = uconf[1];
   *ucoul_slt = ucoul[0]; *ucoul_slv = ucoul[1];
 }
+
+/* ------------------------------------------------------------------ */
+/* Alchemy: force_alchemy.f (soft-core GROMOS), alch_set                */
+/* (force_alchemy_shifted.f:732-790), ferrf_alchemy.f, cself_alchemy.f  */
+/* ------------------------------------------------------------------ */
+
+/* alch_set: fact[2 * (typeaij + 4) + {0, 1}] = alch_fact(1:2, typeaij) for
+ * typeaij = s_i + s_j (s = 1 normal, 2 added, -2 removed); entries the
+ * reference leaves untouched stay 0 (force_alchemy.f:193-194 zeroes the LJ
+ * tables before the call). */
+void orc_alch_set(double *fact, double ladd, double lrm, int intramol, int ljintra, int qq) {
+  memset(fact, 0, sizeof(double) * 18);
+#define AF(t, k) fact[2 * ((t) + 4) + (k)]
+  const double a = fabs(ladd), r = fabs(lrm);
+  AF(2, 0) = 0.0; AF(2, 1) = 1.0;
+  AF(3, 0) = a; AF(3, 1) = 1.0 - a;
+  AF(-1, 0) = r; AF(-1, 1) = 1.0 - r;
+  if (intramol && qq) {
+    AF(4, 0) = a * a; AF(4, 1) = 1.0 - a * a;
+    AF(-4, 0) = r * r; AF(-4, 1) = 1.0 - r * r;
+  } else if (intramol && !ljintra) {
+    AF(4, 0) = a; AF(4, 1) = 1.0 - a;
+    AF(-4, 0) = r; AF(-4, 1) = 1.0 - r;
+  } else {
+    AF(4, 0) = 0.0; AF(4, 1) = 1.0;
+    AF(-4, 0) = 0.0; AF(-4, 1) = 1.0;
+  }
+  AF(0, 0) = 0.0; AF(0, 1) = 0.0;
+#undef AF
+}
+
+/* force_alchemy.f:147-730, serial build.  The module state the routine reads
+ * arrives as arguments: ss_alchemy (cpropar), lam = {ladd, lrm, ladd0, lrm0,
+ * laddq, lrmq, laddq0, lrmq0}, intramol, ljintra, opls (ffield_type = 'opls').
+ * e6 = uconf_slt, uconf_slv, uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss.  As in
+ * the serial build, dwrk is added to (not reset) and energies are written only
+ * when iz = 1 and rinn0 < rout. */
+int orc_force_alchemy(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                      const double *xpg, const double *ypg, const double *zpg, const double *charge,
+                      const int32_t *ss_alchemy, const double *rmin, const double *epsm, int nato,
+                      const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *atomp,
+                      const double *co, double alphal, const int32_t *mapnl, int ngrp, const int32_t *grppt,
+                      double *e6, double *fpx, double *fpy, double *fpz, double *stress, const int32_t *nnlpp,
+                      int32_t *nnlpp1, long nnlpp1_cap, double rneigh, double rinn, double rout, double *rtolinn,
+                      double rtolout, const double *gmass, int iz, int32_t *ncount1, const double *mass,
+                      double massinfty, const double *lam, int intramol, int ljintra, int opls, double *dwrk,
+                      int32_t lskip_ewald, int32_t ewald, double qsoft, double ljsoft) {
+  const double r2inn = rinn * rinn, r2out = rout * rout;
+  const double rinn0 = rinn + *rtolinn, r2inn0 = rinn0 * rinn0;
+  if (rinn < 1.0e-5) *rtolinn = 0.0;
+  const double rout0 = rout + rtolout, r2out0 = rout0 * rout0;
+  const double r2neigh = (rinn0 + rneigh) * (rinn0 + rneigh);
+  const double twrtpi = 2.0 / sqrt(4.0 * atan(1.0));
+  const double arsout1 = rout0 - 3.0 * rout, arsout2 = rtolout * rtolout * rtolout;
+  const double arsinn1 = rinn0 - 3.0 * rinn, arsinn2 = *rtolinn * *rtolinn * *rtolinn;
+  double fl[18], fl0[18], fq[18], fq0[18];
+  orc_alch_set(fl, lam[0], lam[1], intramol, ljintra, 0);
+  orc_alch_set(fl0, lam[2], lam[3], intramol, ljintra, 0);
+  orc_alch_set(fq, lam[4], lam[5], intramol, ljintra, 1);
+  orc_alch_set(fq0, lam[6], lam[7], intramol, ljintra, 1);
+  double st[9] = {0}, uconf[3] = {0}, ucoul[3] = {0}, uconf0[3] = {0}, ucoul0[3] = {0};
+  int cap = 1;
+  {
+    long n = 0;
+    for (int i = 1; i <= ngrp; ++i) {
+      int m = nnlpp[n];
+      if (m + 1 > cap) cap = m + 1;
+      if (iz == 1) n += m + 1; else break;
+    }
+  }
+  int *index1 = malloc(sizeof(int) * cap);
+  double *xm1 = malloc(sizeof(double) * cap * 9);
+  double *ym1 = xm1 + cap, *zm1 = ym1 + cap, *xm2 = zm1 + cap, *ym2 = xm2 + cap, *zm2 = ym2 + cap;
+  double *dr = zm2 + cap, *swrs = dr + cap, *dswrs = swrs + cap;
+  double *cmap2 = malloc(sizeof(double) * cap);
+  char *maplg = malloc((size_t)nato + 1);
+  memset(maplg, 1, (size_t)nato + 1);
+  long n = 0, na = 0, c1 = 0;
+  int mbeg = 1, status = 0;
+  for (int i = 1; i <= ngrp; ++i) {
+    const double xpgi = xpg[i - 1], ypgi = ypg[i - 1], zpgi = zpg[i - 1];
+    const int m = nnlpp[n];
+    int map = 0, mp = 0;
+    const int typei = ss_index[grppt[2 * (i - 1)] - 1];
+    if (iz == 0) mbeg = i + 1;
+    if (r2inn > 1.0e-5 && c1 + 1 + m > nnlpp1_cap) { status = 1; break; }
+    for (int jj = mbeg; jj <= m; ++jj) {
+      const int j = nnlpp[jj + n];
+      double xgg = xpgi - xpg[j - 1], ygg = ypgi - ypg[j - 1], zgg = zpgi - zpg[j - 1];
+      const double sx = 2.0 * fnint(0.5 * xgg), sy = 2.0 * fnint(0.5 * ygg), sz = 2.0 * fnint(0.5 * zgg);
+      xgg = xgg - sx; ygg = ygg - sy; zgg = zgg - sz;
+      const double xc = co[0] * xgg + co[3] * ygg + co[6] * zgg;
+      const double yc = co[4] * ygg + co[7] * zgg;
+      const double zc = co[8] * zgg;
+      const double drj = xc * xc + yc * yc + zc * zc;
+      if (r2inn > 1.0e-5 && drj < r2neigh) {
+        map += 1;
+        nnlpp1[c1 + map] = j;
+      }
+      if (rinn0 > rout) continue;
+      if (drj >= r2inn && drj < r2out0) {
+        xm1[mp] = sx; ym1[mp] = sy; zm1[mp] = sz;
+        xm2[mp] = xc; ym2[mp] = yc; zm2[mp] = zc;
+        dr[mp] = drj;
+        index1[mp] = j;
+        mp += 1;
+      }
+    }
+    if (rinn > 1.0e-5) {
+      nnlpp1[c1] = map;
+      c1 += map + 1;
+    }
+    const int mapb = mp;
+    /* force_alchemy.f:344-368 */
+    for (int jj = 0; jj < mapb; ++jj) {
+      if (dr[jj] < r2out && dr[jj] >= r2inn0) {
+        swrs[jj] = 1.0;
+        dswrs[jj] = 0.0;
+      } else if (dr[jj] >= r2out) {
+        const double rsp = sqrt(dr[jj]);
+        const double auxa = (arsout1 + 2.0 * rsp) / arsout2, auxb = rout0 - rsp;
+        swrs[jj] = auxa * (auxb * auxb);
+        dswrs[jj] = (-2.0 * auxa * auxb + 2.0 * (auxb * auxb) / arsout2) / rsp;
+      } else {
+        const double rsp = sqrt(dr[jj]);
+        const double auxa = (arsinn1 + 2.0 * rsp) / arsinn2, auxb = rinn0 - rsp;
+        swrs[jj] = 1.0 - auxa * (auxb * auxb);
+        dswrs[jj] = (2.0 * auxa * auxb - 2.0 * (auxb * auxb) / arsinn2) / rsp;
+      }
+    }
+    for (int jj = 0; jj < mapb; ++jj) cmap2[jj] = 0.0;
+    const int ga = grppt[2 * (i - 1)], gb = grppt[2 * (i - 1) + 1];
+    if (mapb == 0) {
+      for (int i1 = ga; i1 <= gb; ++i1) na += mapnl[na] + 1;
+    } else {
+      for (int i1 = ga; i1 <= gb; ++i1) {
+        const double xpi = xp0[i1 - 1], ypi = yp0[i1 - 1], zpi = zp0[i1 - 1];
+        const double chrgei = charge[i1 - 1];
+        const int p1 = atomp[i1 - 1];
+        const double xpcmp1 = xpcm[p1 - 1], ypcmp1 = ypcm[p1 - 1], zpcmp1 = zpcm[p1 - 1];
+        const int la = mapnl[na];
+        for (int t = 1; t <= la; ++t) maplg[mapnl[na + t]] = 0;
+        /* pairs in group-then-atom order (force_alchemy.f:411-424), computed in place */
+        for (int jj = 0; jj < mapb; ++jj) {
+          const int jg = index1[jj];
+          const int typej = ss_index[grppt[2 * (jg - 1)] - 1];
+          const int typeij = typei + typej - 1;
+          for (int j = grppt[2 * (jg - 1)]; j <= grppt[2 * (jg - 1) + 1]; ++j) {
+            if (!(maplg[j] && (mass[j - 1] < massinfty || mass[i1 - 1] < massinfty))) continue;
+            const double sigma = opls ? 2.0 * pow(rmin[j - 1] * rmin[i1 - 1], 0.5) : rmin[j - 1] + rmin[i1 - 1];
+            const double epsmin = 4.0 * sqrt(epsm[i1 - 1] * epsm[j - 1]);
+            if (epsmin < 1.0e-12 && lskip_ewald && ewald) continue;
+            const int ta = ss_alchemy[i1 - 1] + ss_alchemy[j - 1];
+            const double *F = fl + 2 * (ta + 4), *F0 = fl0 + 2 * (ta + 4);
+            const double *Q = fq + 2 * (ta + 4), *Q0 = fq0 + 2 * (ta + 4);
+            const int p2 = atomp[j - 1];
+            const double xxcm = xpcmp1 - xpcm[p2 - 1] - xm1[jj];
+            const double yycm = ypcmp1 - ypcm[p2 - 1] - ym1[jj];
+            const double zzcm = zpcmp1 - zpcm[p2 - 1] - zm1[jj];
+            const double xg = xpi - xp0[j - 1] - xm1[jj];
+            const double yg = ypi - yp0[j - 1] - ym1[jj];
+            const double zg = zpi - zp0[j - 1] - zm1[jj];
+            const double xc = co[0] * xg + co[3] * yg + co[6] * zg;
+            const double yc = co[4] * yg + co[7] * zg;
+            const double zc = co[8] * zg;
+            const double rsq = xc * xc + yc * yc + zc * zc;
+            const double auxa = 1.0 / sqrt(rsq);
+            const double auxaa = 1.0 / sqrt(qsoft * pow(Q[0], 2.0) + rsq);
+            const double auxa0 = 1.0 / sqrt(qsoft * pow(Q0[0], 2.0) + rsq);
+            const double auxbb = auxaa * auxaa * auxaa;
+            const double sw = swrs[jj];
+            double qforce;
+            if (epsmin > 1.0e-12) {
+              const double r6 = rsq * rsq * rsq / pow(sigma, 6.0);
+              const double gp = ljsoft * F[0] + r6, gp1 = 1.0 / gp, gp2 = gp1 * gp1;
+              const double gp0 = ljsoft * F0[0] + r6, gp10 = 1.0 / gp0, gp20 = gp10 * gp10;
+              const double gr = 6.0 * r6 * auxa;
+              const double ssvir = F[1] * epsmin * (2.0 * gp1 * gp2 - gp2);
+              const double ssvirr = gr * ssvir;
+              const double auxc = epsmin * (gp2 - gp1), auxc0 = epsmin * (gp20 - gp10);
+              const double ucon = F[1] * auxc, ucon0 = F0[1] * auxc0;
+              qforce = ssvirr * sw * auxa;
+              cmap2[jj] = cmap2[jj] + ucon;
+              uconf[typeij - 1] = uconf[typeij - 1] + sw * ucon;
+              uconf0[typeij - 1] = uconf0[typeij - 1] + sw * ucon0;
+            } else {
+              qforce = 0.0;
+            }
+            if (!lskip_ewald) {
+              const double alphar = alphal / auxa;
+              const double qt = 1.0 / (1.0 + QP * alphar);
+              const double expcst = exp(-alphar * alphar);
+              const double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+              const double furpar = Q[1] * chrgei * charge[j - 1];
+              const double furpar0 = Q0[1] * chrgei * charge[j - 1];
+              ucoul[typeij - 1] = ucoul[typeij - 1] + sw * furpar * erfcst * auxaa;
+              ucoul0[typeij - 1] = ucoul0[typeij - 1] + sw * furpar0 * erfcst * auxa0;
+              cmap2[jj] = cmap2[jj] + furpar * erfcst * auxaa;
+              qforce = qforce + furpar * erfcst * auxbb * sw +
+                       furpar * twrtpi * alphar * expcst * auxa * auxa * auxaa * sw;
+            } else if (!ewald) {
+              const double auxc = chrgei * charge[j - 1] * auxaa;
+              const double ssvir = Q[1] * auxc;
+              const double ssvir0 = Q0[1] * chrgei * charge[j - 1] * auxa0;
+              cmap2[jj] = cmap2[jj] + ssvir;
+              ucoul[typeij - 1] = ucoul[typeij - 1] + ssvir * sw;
+              ucoul0[typeij - 1] = ucoul0[typeij - 1] + ssvir0 * sw;
+              qforce = qforce + ssvir * sw * auxaa * auxaa;
+            }
+            const double emvir = qforce;
+            fpx[i1 - 1] = fpx[i1 - 1] + qforce * xc;
+            fpy[i1 - 1] = fpy[i1 - 1] + qforce * yc;
+            fpz[i1 - 1] = fpz[i1 - 1] + qforce * zc;
+            fpx[j - 1] = fpx[j - 1] - qforce * xc;
+            fpy[j - 1] = fpy[j - 1] - qforce * yc;
+            fpz[j - 1] = fpz[j - 1] - qforce * zc;
+            st[0] = st[0] + emvir * xc * xxcm; st[1] = st[1] + emvir * xc * yycm; st[2] = st[2] + emvir * xc * zzcm;
+            st[3] = st[3] + emvir * yc * xxcm; st[4] = st[4] + emvir * yc * yycm; st[5] = st[5] + emvir * yc * zzcm;
+            st[6] = st[6] + emvir * zc * xxcm; st[7] = st[7] + emvir * zc * yycm; st[8] = st[8] + emvir * zc * zzcm;
+          }
+        }
+        for (int t = 1; t <= la; ++t) maplg[mapnl[na + t]] = 1;
+        na += la + 1;
+      }
+      /* S'(r) V by mass fraction, force_alchemy.f:559-622 */
+      for (int jj = 0; jj < mapb; ++jj) {
+        const double x3 = -dswrs[jj] * cmap2[jj] * xm2[jj];
+        const double y3 = -dswrs[jj] * cmap2[jj] * ym2[jj];
+        const double z3 = -dswrs[jj] * cmap2[jj] * zm2[jj];
+        xm2[jj] = x3; ym2[jj] = y3; zm2[jj] = z3;  /* xmap3 (xmap2 is not read again) */
+      }
+      for (int i1 = ga; i1 <= gb; ++i1) {
+        const double massi = gmass[i1 - 1];
+        const int p1 = atomp[i1 - 1];
+        const double xq = xpcm[p1 - 1], yq = ypcm[p1 - 1], zq = zpcm[p1 - 1];
+        for (int jj = 0; jj < mapb; ++jj) {
+          fpx[i1 - 1] = fpx[i1 - 1] + massi * xm2[jj];
+          fpy[i1 - 1] = fpy[i1 - 1] + massi * ym2[jj];
+          fpz[i1 - 1] = fpz[i1 - 1] + massi * zm2[jj];
+          st[0] = st[0] + massi * xm2[jj] * xq; st[1] = st[1] + massi * xm2[jj] * yq;
+          st[2] = st[2] + massi * xm2[jj] * zq; st[3] = st[3] + massi * ym2[jj] * xq;
+          st[4] = st[4] + massi * ym2[jj] * yq; st[5] = st[5] + massi * ym2[jj] * zq;
+          st[6] = st[6] + massi * zm2[jj] * xq; st[7] = st[7] + massi * zm2[jj] * yq;
+          st[8] = st[8] + massi * zm2[jj] * zq;
+        }
+      }
+      for (int jj = 0; jj < mapb; ++jj) {
+        const int jg = index1[jj];
+        for (int j1 = grppt[2 * (jg - 1)]; j1 <= grppt[2 * (jg - 1) + 1]; ++j1) {
+          const double massj = gmass[j1 - 1];
+          fpx[j1 - 1] = fpx[j1 - 1] - massj * xm2[jj];
+          fpy[j1 - 1] = fpy[j1 - 1] - massj * ym2[jj];
+          fpz[j1 - 1] = fpz[j1 - 1] - massj * zm2[jj];
+          const int p2 = atomp[j1 - 1];
+          const double xq = -(xpcm[p2 - 1] + xm1[jj]) * massj;
+          const double yq = -(ypcm[p2 - 1] + ym1[jj]) * massj;
+          const double zq = -(zpcm[p2 - 1] + zm1[jj]) * massj;
+          st[0] = st[0] + xm2[jj] * xq; st[1] = st[1] + xm2[jj] * yq; st[2] = st[2] + xm2[jj] * zq;
+          st[3] = st[3] + ym2[jj] * xq; st[4] = st[4] + ym2[jj] * yq; st[5] = st[5] + ym2[jj] * zq;
+          st[6] = st[6] + zm2[jj] * xq; st[7] = st[7] + zm2[jj] * yq; st[8] = st[8] + zm2[jj] * zq;
+        }
+      }
+    }
+    if (iz == 1) n += m + 1;
+  }
+  *ncount1 = (int32_t)c1;
+  free(index1); free(xm1); free(cmap2); free(maplg);
+  if (status || iz == 0 || rinn0 >= rout) return status;
+  e6[0] = uconf[0]; e6[1] = uconf[2]; e6[2] = uconf[1];
+  e6[3] = ucoul[0]; e6[4] = ucoul[2]; e6[5] = ucoul[1];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < 3; ++k) stress[r + 3 * c] = stress[r + 3 * c] + st[3 * r + k] * co[c + 3 * k];
+  for (int k = 0; k < 3; ++k) {
+    const double auxa = ucoul[k] - ucoul0[k], auxb = uconf[k] - uconf0[k];
+    *dwrk = *dwrk + auxa + auxb;
+  }
+  return 0;
+}
+
+/* ferrf_alchemy.f:1-204 (the ERF_CORR spline branches are not restated):
+ * ferrf on charge, plus en0 on charge0; dwrk = slt + slv - en0. */
+int orc_ferrf_alchemy(const int32_t *ss_index, double alphal, const double *charge, const double *charge0,
+                      double fudge, const double *x0, const double *y0, const double *z0, int iz,
+                      const int32_t *list, int nlist, double *fs_slt, double *fs_slv, double *fpx, double *fpy,
+                      double *fpz, int nbonds_added, double *dwrk) {
+  const double twrtpi = 2.0 / sqrt(4.0 * atan(1.0));
+  double fs[2] = {0.0, 0.0}, en0 = 0.0;
+  for (int i = 0; i < nlist - nbonds_added; ++i) {
+    const int ia = list[2 * i], ib = list[2 * i + 1];
+    const int type = ss_index[ia - 1];
+    /* ferrf_alchemy.f:144-145 (iz = 1) and 190-191, 232-233 (iz = 0) */
+    const double furpar = iz == 1 ? fudge * charge[ia - 1] * charge[ib - 1] : charge[ia - 1] * charge[ib - 1];
+    const double furpar0 = iz == 1 ? fudge * charge0[ia - 1] * charge0[ib - 1] : charge0[ia - 1] * charge0[ib - 1];
+    const double xab = x0[ia - 1] - x0[ib - 1], yab = y0[ia - 1] - y0[ib - 1], zab = z0[ia - 1] - z0[ib - 1];
+    const double rsq = xab * xab + yab * yab + zab * zab;
+    const double rsp = sqrt(rsq);
+    const double alphar = alphal * rsp;
+    const double qt = 1.0 / (1.0 + QP * alphar);
+    const double expcst = exp(-alphar * alphar);
+    const double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+    const double erfst = 1.0 - erfcst;
+    fs[type - 1] = fs[type - 1] - furpar * erfst / rsp;
+    en0 = en0 - furpar0 * erfst / rsp;
+    if (iz == 1) {
+      const double qforce = -furpar * (erfst - twrtpi * alphar * expcst) / (rsp * rsq);
+      fpx[ia - 1] = fpx[ia - 1] + qforce * xab;
+      fpy[ia - 1] = fpy[ia - 1] + qforce * yab;
+      fpz[ia - 1] = fpz[ia - 1] + qforce * zab;
+      fpx[ib - 1] = fpx[ib - 1] - qforce * xab;
+      fpy[ib - 1] = fpy[ib - 1] - qforce * yab;
+      fpz[ib - 1] = fpz[ib - 1] - qforce * zab;
+    }
+  }
+  *fs_slt = fs[0];
+  *fs_slv = fs[1];
+  *dwrk = fs[0] + fs[1] - en0;
+  return 0;
+}
+
+/* cself_alchemy.f:1-66: solute atoms only */
+void orc_cself_alchemy(const int32_t *ss_index, int nato, double alphal, double kcut, const double *charge,
+                       const double *charge0, double *self_slt, double *dwrk) {
+  double fs = 0.0, en0 = 0.0;
+  const double pi = 4.0 * atan(1.0);
+  for (int n = 0; n < nato; ++n)
+    if (ss_index[n] == 1) {
+      fs = fs + charge[n] * charge[n];
+      en0 = en0 + charge0[n] * charge0[n];
+    }
+  const double alphar = kcut / (2.0 * alphal);
+  const double qt = 1.0 / (1.0 + QP * alphar);
+  const double expcst = exp(-alphar * alphar);
+  const double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+  fs = -fs * alphal / sqrt(pi) * (1.0 - erfcst);
+  en0 = -en0 * alphal / sqrt(pi) * (1.0 - erfcst);
+  *self_slt = fs;
+  *dwrk = fs - en0;
+}

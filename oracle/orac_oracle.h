@@ -110,3 +110,22 @@ void orc_fnbgrp(const int32_t *ss_index, const double *xp0, const double *yp0, c
 }
 #endif
 #endif
+
+/* alchemy (force_alchemy.f, force_alchemy_shifted.f:732, ferrf_alchemy.f, cself_alchemy.f) */
+void orc_alch_set(double *fact, double ladd, double lrm, int intramol, int ljintra, int qq);
+int orc_force_alchemy(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                      const double *xpg, const double *ypg, const double *zpg, const double *charge,
+                      const int32_t *ss_alchemy, const double *rmin, const double *epsm, int nato,
+                      const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *atomp,
+                      const double *co, double alphal, const int32_t *mapnl, int ngrp, const int32_t *grppt,
+                      double *e6, double *fpx, double *fpy, double *fpz, double *stress, const int32_t *nnlpp,
+                      int32_t *nnlpp1, long nnlpp1_cap, double rneigh, double rinn, double rout, double *rtolinn,
+                      double rtolout, const double *gmass, int iz, int32_t *ncount1, const double *mass,
+                      double massinfty, const double *lam, int intramol, int ljintra, int opls, double *dwrk,
+                      int32_t lskip_ewald, int32_t ewald, double qsoft, double ljsoft);
+int orc_ferrf_alchemy(const int32_t *ss_index, double alphal, const double *charge, const double *charge0,
+                      double fudge, const double *x0, const double *y0, const double *z0, int iz,
+                      const int32_t *list, int nlist, double *fs_slt, double *fs_slv, double *fpx, double *fpy,
+                      double *fpz, int nbonds_added, double *dwrk);
+void orc_cself_alchemy(const int32_t *ss_index, int nato, double alphal, double kcut, const double *charge,
+                       const double *charge0, double *self_slt, double *dwrk);

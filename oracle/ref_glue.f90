@@ -36,3 +36,26 @@ subroutine refglue_get_bsp(b1, b2, b3, nfft1, nfft2, nfft3) bind(C, name="refglu
   b2(1:nfft2) = bsp_mod2(1:nfft2)
   b3(1:nfft3) = bsp_mod3(1:nfft3)
 end subroutine refglue_get_bsp
+
+! refglue_set_alchemy: the module state force_alchemy.f reads (cpropar:
+! ss_alchemy, ladd/lrm and their charge and lambda0 twins, intramol,
+! ljintra; unit: ffield_type).  lam = ladd, lrm, ladd0, lrm0, laddq, lrmq,
+! laddq0, lrmq0.
+subroutine refglue_set_alchemy(n, ss, lam, intra, ljin, opls) bind(C, name="refglue_set_alchemy")
+  use unit, only: ffield_type
+  use cpropar, only: ss_alchemy, ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0, intramol, ljintra
+  implicit none
+  integer, value :: n, intra, ljin, opls
+  integer :: ss(n)
+  real(8) :: lam(8)
+  ss_alchemy(1:n) = ss(1:n)
+  ladd = lam(1); lrm = lam(2); ladd0 = lam(3); lrm0 = lam(4)
+  laddq = lam(5); lrmq = lam(6); laddq0 = lam(7); lrmq0 = lam(8)
+  intramol = intra /= 0
+  ljintra = ljin /= 0
+  if (opls /= 0) then
+    ffield_type = 'opls'
+  else
+    ffield_type = 'amber'
+  end if
+end subroutine refglue_set_alchemy
