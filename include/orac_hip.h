@@ -101,6 +101,30 @@ int orac_hip_force(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0
                    double rtolout, const double *gmass, int32_t iz, int32_t *ncount1, const double *mass,
                    double massinfty, int32_t lskip_ewald, int32_t ewald);
 
+/* Replaces force_alchemy (force_alchemy.f:1-759, the soft-core GROMOS form)
+ * as mts_forces.f calls it for an alchemical run.  Arguments as
+ * orac_hip_force, plus what force_alchemy reads from module cpropar and its
+ * own argument list: ss_alchemy (per atom 1 normal, 2 added, -2 removed),
+ * lam = {ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0}, intramol,
+ * ljintra, opls (ffield_type 'opls'), per-atom rmin and epsm, qsoft and
+ * ljsoft.  dwrk is in/out: += U(lambda) - U(lambda0) over the shell, as the
+ * serial reference does.  The per-atom lambda arrays of the Fortran
+ * signature are not read by force_alchemy and are not taken here. */
+int orac_hip_force_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                           const double *zp0, const double *xpg, const double *ypg, const double *zpg,
+                           const double *charge, const int32_t *nbtype, const int32_t *type, int32_t ma,
+                           int32_t nato, const int32_t *atomg, const double *xpcm, const double *ypcm,
+                           const double *zpcm, const int32_t *groupp, const int32_t *atomp, const double *co,
+                           const double *ecc12, const double *ecc6, double alphal, const int32_t *mapnl,
+                           int32_t ngrp, const int32_t *grppt, double *uconf_slt, double *uconf_slv,
+                           double *uconf_ss, double *ucoul_slt, double *ucoul_slv, double *ucoul_ss, double *fpx,
+                           double *fpy, double *fpz, double *stress, const int32_t *nnlpp, int32_t *nnlpp1,
+                           int64_t nnlpp1_capacity, double rneigh, double rinn, double rout, double *rtolinn,
+                           double rtolout, const double *gmass, int32_t iz, int32_t *ncount1, const double *mass,
+                           double massinfty, const int32_t *ss_alchemy, const double *lam, int32_t intramol,
+                           int32_t ljintra, int32_t opls, const double *rmin, const double *epsm, double *dwrk,
+                           int32_t lskip_ewald, int32_t ewald, double qsoft, double ljsoft);
+
 /* Replaces decide_force (mts_forces.f:212-241): lskip_ewald = erfc_AS(alpha*rinn) < 1e-3. */
 int orac_hip_decide_force(double rinn, double alphal, int32_t *lskip_ewald);
 
@@ -184,6 +208,32 @@ int orac_hip_system_bytes(orac_hip_ctx *ctx, int32_t sys, int64_t *bytes);
 int orac_hip_sync(orac_hip_ctx *ctx);
 /* Stream handle of the context, as a void* (a hipStream_t). */
 int orac_hip_stream(orac_hip_ctx *ctx, void **stream);
+
+/* ---- alchemy on a resident system ------------------------------------------ */
+/* The alchemical force step of the reference (mts_furier_alchemy.f:80-190
+ * with force_alchemy.f in the direct shell) on a resident system:
+ * soft-core pairs on the created charges scaled by the charge factors;
+ * reciprocal sum, ferrf corrections and self term on the lambda charges
+ * (1 - |lambdaq|) q (mts_furier_alchemy.f:98-111); and the lambda0 charges
+ * on a second, energy-only grid (do_pmesh_alchemy.f90), in the corrections
+ * and in the self term, for dwrk = U(lambda) - U(lambda0).  ss_alchemy,
+ * rmin, epsm, qsoft, ljsoft, intramol, ljintra and opls as in
+ * orac_hip_force_alchemy.  Once set the system stays alchemical; its
+ * energies record reports the self energy of the solute only
+ * (cself_alchemy.f), slv 0. */
+typedef struct orac_hip_alchemy_desc {
+  const int32_t *ss_alchemy;          /* natoms: 1 normal, 2 added, -2 removed */
+  const double *rmin, *epsm;          /* natoms */
+  double qsoft, ljsoft;
+  int32_t intramol, ljintra, opls;
+} orac_hip_alchemy_desc;
+int orac_hip_system_set_alchemy(orac_hip_ctx *ctx, int32_t sys, const orac_hip_alchemy_desc *desc);
+/* lam = ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0 (initially 1, 0, 1, 0, 1, 0, 1, 0). */
+int orac_hip_system_set_lambda(orac_hip_ctx *ctx, int32_t sys, const double *lam);
+/* After an eval: out[0] dwrk = U(lambda) - U(lambda0) of the force step,
+ * [1] direct pairs, [2] reciprocal eer - eer0, [3] corrections, [4] self,
+ * [5] eer0.  Needs room for 6 doubles. */
+int orac_hip_system_alchemy_results(orac_hip_ctx *ctx, int32_t sys, double *out);
 
 /* ---- batches (no reference counterpart) ------------------------------------ */
 /* An ordered set of systems (replicas, windows, requests; sizes may differ)

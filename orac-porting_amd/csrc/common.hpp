@@ -84,6 +84,7 @@ struct PmeDesc {
   double *swt;                  // [natoms][3][order] B-spline weights of the sorted atoms (exact T values)
   int tiled;                    // 1: LDS-tiled spread, 0: global-atomic spread
   double *out;                  // [20]: eer, vir (3x3 col-major), stc1..stc9
+  int energy_only;              // 1: spread, forward FFT and energy only (a lambda0 charge grid)
 };
 
 // ---------------------------------------------------------------- helpers

@@ -36,8 +36,21 @@ struct System {
   int slab = 0;           // neighbour-list entries per owner (slab layout)
   double *xp = nullptr, *yp = nullptr, *zp = nullptr;
   double *fx = nullptr, *fy = nullptr, *fz = nullptr;
-  double *out = nullptr;  // device: pme[20], pairs[15] at 20, corrections[4] at 36
+  double *out = nullptr;  // device: pme[20], pairs[16] at 20, corrections[4] at 36, lambda0 pme[20] at 48
   double self_e[2] = {0, 0};
+  std::vector<double> q_host;      // charges as created
+  std::vector<int32_t> ss_host;    // ss_index
+  // alchemy (orac_hip_system_set_alchemy): soft-core pairs on the created
+  // charges, reciprocal sum and corrections on the lambda charges, and an
+  // energy-only grid on the lambda0 charges
+  bool alch = false;
+  std::vector<int32_t> ss_alch;
+  bool intramol = false, ljintra = false;
+  double qsoft = 0.5, ljsoft = 0.5;
+  double lam[8] = {1, 0, 1, 0, 1, 0, 1, 0};
+  double self_dw = 0.0;
+  double *qc = nullptr, *q0 = nullptr, *afac = nullptr;
+  PmeDesc pme0;
   bool have_list = false;
   int list_fmt = -1;      // cell_order tuning the current list was built with
   int solo = 0;           // id of this system's own one-system batch (0: none yet)
@@ -210,8 +223,10 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   double *xpcm = B.get<double>("xpcm", d.nmol), *ypcm = B.get<double>("ypcm", d.nmol),
          *zpcm = B.get<double>("zpcm", d.nmol);
   double *xpg = B.get<double>("xpg", ng), *ypg = B.get<double>("ypg", ng), *zpg = B.get<double>("zpg", ng);
-  S.out = B.get<double>("out", 48);
-  launch_zero_f64(S.out, 48, st);
+  S.out = B.get<double>("out", 72);
+  launch_zero_f64(S.out, 72, st);
+  S.q_host.assign(d.charge, d.charge + n);
+  S.ss_host.assign(d.ss_index, d.ss_index + n);
 
   // ---- direct space
   PairDesc &p = S.pair;
@@ -466,6 +481,12 @@ static void batch_upload(orac_hip_ctx *ctx, Batch &b) {
     b.hc[k] = S.corr;
     if (S.corr.npairs > 0) b.any_corr = true;
   }
+  // the energy-only lambda0 grids of alchemical systems follow the systems' own
+  for (size_t k = 0; k < n; ++k)
+    if (b.sys[k]->alch) {
+      b.hp.push_back(b.sys[k]->pme0);
+      pme_apply_tuning(b.hp.back(), b.tu);
+    }
   b.dp = b.B.put_vec("dp", b.hp, st, false);
   b.dq = b.B.put_vec("dq", b.hq, st, false);
   b.dl = b.B.put_vec("dl", b.hl, st, false);
@@ -531,10 +552,11 @@ static void batch_enqueue(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool events
   else
     launch_pairs<float>(b.dq, b.hq.data(), n, st, b.tu, kb, ke);
   mark(ST_PAIRS);
+  const int npme = (int)b.hp.size();
   if (b.precision == 0)
-    launch_pme<double>(b.dp, b.hp.data(), n, st, b.tu);
+    launch_pme<double>(b.dp, b.hp.data(), npme, st, b.tu);
   else
-    launch_pme<float>(b.dp, b.hp.data(), n, st, b.tu);
+    launch_pme<float>(b.dp, b.hp.data(), npme, st, b.tu);
   mark(ST_PME);
   if (b.any_corr) launch_corr(b.dc, b.hc.data(), n, st);
   mark(ST_END);
@@ -1031,6 +1053,120 @@ extern "C" int orac_hip_system_results(orac_hip_ctx *ctx, int32_t sys, double *f
       energies[19] = S.self_e[0];
       energies[20] = S.self_e[1];
     }
+  });
+}
+
+// ================================================================ C ABI: alchemy
+
+// charges, soft-core factors and self terms at the system's lambdas
+static void alch_apply_lambda(orac_hip_ctx *ctx, System &S) {
+  hipStream_t st = ctx->stream;
+  const int n = S.natoms;
+  double fac[72];
+  alch_factors(S.lam, S.intramol, S.ljintra, S.qsoft, S.ljsoft, fac);
+  std::vector<double> qc(n), q0(n);
+  alch_charges(n, S.q_host.data(), S.ss_alch.data(), S.lam[4], S.lam[5], qc.data());
+  alch_charges(n, S.q_host.data(), S.ss_alch.data(), S.lam[6], S.lam[7], q0.data());
+  ORAC_HIP(hipMemcpyAsync(S.afac, fac, sizeof fac, hipMemcpyHostToDevice, st));
+  ORAC_HIP(hipMemcpyAsync(S.qc, qc.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+  ORAC_HIP(hipMemcpyAsync(S.q0, q0.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+  // cself_alchemy.f: solute atoms only, kcut = 1e9 (erfc term 0)
+  double fs = 0.0, en0 = 0.0;
+  for (int a = 0; a < n; ++a)
+    if (S.ss_host[a] == 1) {
+      fs = fs + qc[a] * qc[a];
+      en0 = en0 + q0[a] * q0[a];
+    }
+  const double pi = 4.0 * std::atan(1.0);
+  const double e = erfc_as(1.0e9 / (2.0 * S.alpha));
+  fs = -fs * S.alpha / std::sqrt(pi) * (1.0 - e);
+  en0 = -en0 * S.alpha / std::sqrt(pi) * (1.0 - e);
+  S.self_e[0] = fs;
+  S.self_e[1] = 0.0;
+  S.self_dw = fs - en0;
+  ORAC_HIP(hipStreamSynchronize(st));
+}
+
+extern "C" int orac_hip_system_set_alchemy(orac_hip_ctx *ctx, int32_t sys, const orac_hip_alchemy_desc *a) {
+  return guarded([&] {
+    set_device(ctx);
+    System &S = get_sys(ctx, sys);
+    ORAC_REQUIRE(a && a->ss_alchemy && a->rmin && a->epsm, E_ARG, "null alchemy description");
+    ORAC_REQUIRE(!S.alch, E_STATE, "the system is already alchemical");
+    hipStream_t st = ctx->stream;
+    BufSet &B = S.B;
+    const int n = S.natoms;
+    AlchTables A = alch_tables(n, a->ss_alchemy, a->rmin, a->epsm, a->opls);
+    S.ss_alch.assign(a->ss_alchemy, a->ss_alchemy + n);
+    S.intramol = a->intramol != 0;
+    S.ljintra = a->ljintra != 0;
+    S.qsoft = a->qsoft;
+    S.ljsoft = a->ljsoft;
+    S.pair.type = B.put_vec("atype", A.cls, st, false);
+    S.pair.ntypes = A.ncls;
+    S.pair.alch_pair = B.put_vec("apair", A.pair, st, false);
+    S.afac = B.get<double>("afac", 72);
+    S.pair.alch_fac = S.afac;
+    S.qc = B.get<double>("qc", n);
+    S.q0 = B.get<double>("q0", n);
+    S.pme.q = S.qc;
+    S.corr.q = S.qc;
+    S.corr.q0 = S.q0;
+    PmeDesc &e = S.pme0;
+    e = pme_desc_base(B, "a0_", S.plan, n, S.oc, S.volume, S.alpha, 1.0e9, S.precision == 0, ctx->tuning, st);
+    e.xp = S.pme.xp;
+    e.yp = S.pme.yp;
+    e.zp = S.pme.zp;
+    e.co = S.pme.co;
+    e.q = S.q0;
+    e.fixscale = S.pme.fixscale;
+    e.fixinv = S.pme.fixinv;
+    e.fx = S.fx;
+    e.fy = S.fy;
+    e.fz = S.fz;
+    e.accumulate = 1;
+    e.atomp = S.pme.atomp;
+    e.xpcm = S.pme.xpcm;
+    e.ypcm = S.pme.ypcm;
+    e.zpcm = S.pme.zpcm;
+    e.out = S.out + 48;
+    e.energy_only = 1;
+    S.alch = true;
+    alch_apply_lambda(ctx, S);
+    S.gen++;
+  });
+}
+
+extern "C" int orac_hip_system_set_lambda(orac_hip_ctx *ctx, int32_t sys, const double *lam) {
+  return guarded([&] {
+    set_device(ctx);
+    System &S = get_sys(ctx, sys);
+    ORAC_REQUIRE(S.alch, E_STATE, "the system is not alchemical (orac_hip_system_set_alchemy)");
+    ORAC_REQUIRE(lam, E_ARG, "null lambda array");
+    for (int k = 0; k < 8; ++k) {
+      ORAC_REQUIRE(std::isfinite(lam[k]) && std::fabs(lam[k]) <= 1.0, E_ARG, "lambdas must lie in [-1, 1]");
+      S.lam[k] = lam[k];
+    }
+    alch_apply_lambda(ctx, S);
+  });
+}
+
+extern "C" int orac_hip_system_alchemy_results(orac_hip_ctx *ctx, int32_t sys, double *out) {
+  return guarded([&] {
+    set_device(ctx);
+    System &S = get_sys(ctx, sys);
+    ORAC_REQUIRE(S.alch, E_STATE, "the system is not alchemical");
+    ORAC_REQUIRE(out, E_ARG, "null output");
+    double o[72];
+    ORAC_HIP(hipMemcpyAsync(o, S.out, sizeof o, hipMemcpyDeviceToHost, ctx->stream));
+    ORAC_HIP(hipStreamSynchronize(ctx->stream));
+    const double direct = o[35], recip = o[0] - o[48], corr = o[36] + o[37] - o[38];
+    out[1] = direct;
+    out[2] = recip;
+    out[3] = corr;
+    out[4] = S.self_dw;
+    out[0] = direct + recip + corr + S.self_dw;
+    out[5] = o[48];
   });
 }
 

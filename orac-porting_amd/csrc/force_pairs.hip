@@ -212,19 +212,96 @@ __device__ __forceinline__ PairTerms pair_terms(const PairDesc &d, double pxs, d
   return r;
 }
 
-struct PairShared {
-  double red[PAIR_WAVES][16];
-  double facc[PAIR_WAVES][64][3];
-  double acc[PAIR_WAVES][15][64];  // per-lane energy and virial sums (lane-private slots)
+// One atom pair in soft-core form (force_alchemy.f:439-523): F = the pair's
+// row of alch_fac, ap = {1 / sigma^6, epsmin, typeaij + 4} of its classes.
+// LJ eps (gp^-2 - gp^-1) with gp = ljsoft f1 + r^6 / sigma^6, scaled by f2;
+// Coulomb q_a q_b fq2 erfc(alpha r) / sqrt(qsoft fq1^2 + r^2) (or without
+// erfc, plain); the energies at lambda0 come from the same distances with
+// the lambda0 factors.  dw = U(lambda) - U(lambda0) of the pair.
+struct AlchTerms {
+  double qf, xc, yc, zc, conf, cou, dw;
+  __device__ __forceinline__ void add(double &uconf, double &ucoul, double &dwa, double &fx, double &fy,
+                                      double &fz) const {
+    uconf += conf;
+    ucoul += cou;
+    dwa += dw;
+    fx += qf * xc;
+    fy += qf * yc;
+    fz += qf * zc;
+  }
 };
 
-// One work item: block blk of system d (PAIR_WAVES owner groups).
 template <typename T>
+__device__ __forceinline__ AlchTerms alch_terms(const PairDesc &d, double pxs, double pys, double pzs, double qa,
+                                                const double *co, const double4 &pb4, const double4 &ap,
+                                                const double *F) {
+  const double xg = pxs - pb4.x, yg = pys - pb4.y, zg = pzs - pb4.z;
+  const T xc = T(co[0] * xg + co[3] * yg + co[6] * zg);
+  const T yc = T(co[4] * yg + co[7] * zg);
+  const T zc = T(co[8] * zg);
+  const T rsq = xc * xc + yc * yc + zc * zc;
+  const T rinv = pair_rsqrt<T>(rsq);
+  const T auxaa = pair_rsqrt<T>(T(F[4]) + rsq);
+  const T auxa0 = pair_rsqrt<T>(T(F[6]) + rsq);
+  T qforce = T(0), conf = T(0), conf0 = T(0), cou = T(0), cou0 = T(0);
+  if (ap.y > 1e-12) {
+    const T r6 = rsq * rsq * rsq * T(ap.x);
+    const T gp1 = pair_rcp<T>(T(F[0]) + r6), gp2 = gp1 * gp1;
+    const T gp10 = pair_rcp<T>(T(F[2]) + r6), gp20 = gp10 * gp10;
+    const T eps = T(ap.y);
+    const T ssvir = T(F[1]) * eps * (T(2) * gp1 * gp2 - gp2);
+    qforce = T(6) * r6 * ssvir * (rinv * rinv);
+    conf = T(F[1]) * (eps * (gp2 - gp1));
+    conf0 = T(F[3]) * (eps * (gp20 - gp10));
+  }
+  const T qq = T(qa * pb4.w);
+  if (!d.lskip_ewald) {
+    const T alphar = T(d.alpha) * (rsq * rinv);
+    const T qt = pair_rcp<T>(T(1) + T(AS_QP) * alphar);
+    const T expcst = pair_exp<T>(-(alphar * alphar));
+    const T erfcst = ((((T(AS_A5) * qt + T(AS_A4)) * qt + T(AS_A3)) * qt + T(AS_A2)) * qt + T(AS_A1)) * qt * expcst;
+    const T furpar = T(F[5]) * qq, furpar0 = T(F[7]) * qq;
+    cou = furpar * erfcst * auxaa;
+    cou0 = furpar0 * erfcst * auxa0;
+    qforce += furpar * auxaa * (erfcst * (auxaa * auxaa) + T(d.twrtpi) * alphar * expcst * (rinv * rinv));
+  } else if (!d.ewald) {
+    cou = T(F[5]) * qq * auxaa;
+    cou0 = T(F[7]) * qq * auxa0;
+    qforce += cou * (auxaa * auxaa);
+  }
+  AlchTerms r;
+  r.qf = (double)qforce;
+  r.xc = (double)xc;
+  r.yc = (double)yc;
+  r.zc = (double)zc;
+  r.conf = (double)conf;
+  r.cou = (double)cou;
+  r.dw = (double)(conf - conf0) + (double)(cou - cou0);
+  return r;
+}
+
+constexpr int PAIR_NACC = 16;      // per-lane sums: uconf[3], ucoul[3], st[9], dwrk
+
+struct PairShared {
+  double red[PAIR_WAVES][PAIR_NACC];
+  double facc[PAIR_WAVES][64][3];
+  double acc[PAIR_WAVES][PAIR_NACC][64];  // per-lane energy and virial sums (lane-private slots)
+  double alch[9][8];                      // the system's soft-core factors (alchemical kernels)
+};
+
+// One work item: block blk of system d (PAIR_WAVES owner groups).  ALCH:
+// the kernel also takes the soft-core form for systems that carry alch_pair.
+template <typename T, bool ALCH>
 __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShared &sh) {
   if (blk >= d.nblk) return;
-  double(&red)[PAIR_WAVES][16] = sh.red;
+  double(&red)[PAIR_WAVES][PAIR_NACC] = sh.red;
   double(&facc)[PAIR_WAVES][64][3] = sh.facc;
-  double(&acc)[PAIR_WAVES][15][64] = sh.acc;
+  double(&acc)[PAIR_WAVES][PAIR_NACC][64] = sh.acc;
+  const bool alch = ALCH && d.alch_pair != nullptr;
+  if (alch) {
+    if (threadIdx.x < 72) sh.alch[threadIdx.x >> 3][threadIdx.x & 7] = gmem(d.alch_fac)[threadIdx.x];
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   // S waves share one owner group, each taking a contiguous part of its list
   const int S = d.split, part = wid % S;
@@ -233,7 +310,7 @@ __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShare
   double *en = &acc[wid][0][0];  // en[k * 64 + lane]: uconf[typeij], ucoul[typeij]
   double *st = &acc[wid][6][0];  // st[k * 64 + lane]
 #pragma unroll
-  for (int k = 0; k < 15; ++k) acc[wid][k][lane] = 0.0;
+  for (int k = 0; k < PAIR_NACC; ++k) acc[wid][k][lane] = 0.0;
   int ia0 = 0, nI = 1, nslot = 64;  // ia0: owner's first atom, original numbering
   if (t < d.ngrp) {
     const int4 gi = gload(d.sgi + t);
@@ -333,16 +410,28 @@ __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShare
           const double pxs = pa4.x - sxm, pys = pa4.y - sym, pzs = pa4.z - szm;
           // unswitched sums over the neighbour's atoms; the switch (a function of
           // the group distance only) scales them once below
-          double ucoula = 0.0, uconfa = 0.0;
-          for (int k = 0; k < nJ; ++k) {
-            const int2 ib0 = gload_at(d.sai, jbs + k);
-            const double4 pb0 = gload_at(d.sap, jbs + k);
-            double c12a, c6a;
-            lj_params(d, ia4.x, ib0.x, c12a, c6a);
-            bool skip0 = d.lskip_ewald && c6a < 1e-12 && d.ewald;
-            if (fwd ? mask_a : (ib0.y & 1) != 0) skip0 = skip0 || pair_masked(d, a, jb0 + k, ia0, nI, fwd);
-            const PairTerms r0 = pair_terms<T>(d, pxs, pys, pzs, pa4.w, co, pb0, c12a, c6a);
-            if (!skip0) r0.add(uconfa, ucoula, fgx, fgy, fgz);
+          double ucoula = 0.0, uconfa = 0.0, dwa = 0.0;
+          if (alch) {
+            for (int k = 0; k < nJ; ++k) {
+              const int2 ib0 = gload_at(d.sai, jbs + k);
+              const double4 pb0 = gload_at(d.sap, jbs + k);
+              const double4 ap = gload_at(d.alch_pair, ia4.x * d.ntypes + ib0.x);
+              bool skip0 = d.lskip_ewald && ap.y < 1e-12 && d.ewald;
+              if (fwd ? mask_a : (ib0.y & 1) != 0) skip0 = skip0 || pair_masked(d, a, jb0 + k, ia0, nI, fwd);
+              const AlchTerms r0 = alch_terms<T>(d, pxs, pys, pzs, pa4.w, co, pb0, ap, sh.alch[(int)ap.z]);
+              if (!skip0) r0.add(uconfa, ucoula, dwa, fgx, fgy, fgz);
+            }
+          } else {
+            for (int k = 0; k < nJ; ++k) {
+              const int2 ib0 = gload_at(d.sai, jbs + k);
+              const double4 pb0 = gload_at(d.sap, jbs + k);
+              double c12a, c6a;
+              lj_params(d, ia4.x, ib0.x, c12a, c6a);
+              bool skip0 = d.lskip_ewald && c6a < 1e-12 && d.ewald;
+              if (fwd ? mask_a : (ib0.y & 1) != 0) skip0 = skip0 || pair_masked(d, a, jb0 + k, ia0, nI, fwd);
+              const PairTerms r0 = pair_terms<T>(d, pxs, pys, pzs, pa4.w, co, pb0, c12a, c6a);
+              if (!skip0) r0.add(uconfa, ucoula, fgx, fgy, fgz);
+            }
           }
           const double s = cls == 2 ? sw : 1.0;
           fgx *= s;
@@ -354,6 +443,7 @@ __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShare
           if (fwd) {
             en[typeij * 64 + lane] += s * uconfa;
             en[(3 + typeij) * 64 + lane] += s * ucoula;
+            if (ALCH) acc[wid][15][lane] += s * dwa;
           }
           if (cls == 2) cm = uconfa + ucoula;
         }
@@ -410,12 +500,12 @@ __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShare
   }
   // energies and virial: wave then block, fixed order
 #pragma unroll
-  for (int k = 0; k < 15; ++k) {
+  for (int k = 0; k < PAIR_NACC; ++k) {
     double v = wave_sum(acc[wid][k][lane]);
     if (lane == 0) red[wid][k] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 15) {
+  if (threadIdx.x < PAIR_NACC) {
     double s = 0.0;
     for (int w = 0; w < PAIR_WAVES; ++w) s += red[w][threadIdx.x];
     gmem(d.part)[(size_t)blk * 16 + threadIdx.x] = s;
@@ -430,40 +520,40 @@ __device__ __forceinline__ void pairs_item_index(int h, int nitems, int nblkmax,
 }
 
 // W = minimum waves per SIMD the register allocation must allow (occupancy target)
-template <typename T, int W>
+template <typename T, int W, bool ALCH>
 __global__ void __launch_bounds__(256, W) k_pairs(const PairDesc *__restrict__ D, int remap) {
   __shared__ PairShared sh;
   int sys, blk;
   pairs_item_index(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, gridDim.x, remap, sys, blk);
-  pairs_item<T>(D[sys], blk, sh);
+  pairs_item<T, ALCH>(D[sys], blk, sh);
 }
 
 // Persistent form: a fixed grid (a multiple of 8 workgroups, so each keeps to
 // one XCD) walks all items of the batch with a grid stride.
-template <typename T, int W>
+template <typename T, int W, bool ALCH>
 __global__ void __launch_bounds__(256, W) k_pairs_persistent(const PairDesc *__restrict__ D, int remap, int nitems,
                                                              int nblkmax) {
   __shared__ PairShared sh;
   for (int h = blockIdx.x; h < nitems; h += gridDim.x) {
     int sys, blk;
     pairs_item_index(h, nitems, nblkmax, remap, sys, blk);
-    pairs_item<T>(D[sys], blk, sh);
+    pairs_item<T, ALCH>(D[sys], blk, sh);
     __syncthreads();
   }
 }
 
-template <typename T, int W>
+template <typename T, int W, bool ALCH>
 static void launch_pair_kernel(bool persistent, dim3 grd, int g, hipStream_t st, const PairDesc *d_desc, int remap,
                                int nitems, int max_blk) {
   const dim3 blk(64 * PAIR_WAVES);
   if (persistent)
-    hipLaunchKernelGGL((k_pairs_persistent<T, W>), dim3(g), blk, 0, st, d_desc, remap, nitems, max_blk);
+    hipLaunchKernelGGL((k_pairs_persistent<T, W, ALCH>), dim3(g), blk, 0, st, d_desc, remap, nitems, max_blk);
   else
-    hipLaunchKernelGGL((k_pairs<T, W>), grd, blk, 0, st, d_desc, remap);
+    hipLaunchKernelGGL((k_pairs<T, W, ALCH>), grd, blk, 0, st, d_desc, remap);
 }
 
-// d.out[15]: uconf[3], ucoul[3] (slt, slt-slv, slv order of typeij), st[9] row-major;
-// block k reduces component k over the per-block partials in fixed order
+// d.out[16]: uconf[3], ucoul[3] (slt, slt-slv, slv order of typeij), st[9] row-major,
+// dwrk; block k reduces component k over the per-block partials in fixed order
 __global__ void __launch_bounds__(256) k_pairs_finalize(const PairDesc *__restrict__ D) {
   __shared__ double red[4];
   const PairDesc &d = D[blockIdx.y];
@@ -507,7 +597,9 @@ template <typename T>
 void launch_pairs(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream_t st, const Tuning &tu,
                   hipEvent_t kbeg, hipEvent_t kend) {
   int max_grp = 1, max_blk = 1;
+  bool any_alch = false;
   for (int s = 0; s < nsys; ++s) {
+    any_alch = any_alch || h[s].alch_pair != nullptr;
     if (h[s].max_group > 64)
       throw Error(E_UNSUPPORTED, "groups larger than 64 atoms are not supported by the pair kernel");
     max_grp = std::max(max_grp, h[s].ngrp);
@@ -531,13 +623,19 @@ void launch_pairs(const PairDesc *d_desc, const PairDesc *h, int nsys, hipStream
     }();
     g = std::max(8, std::min(nitems, ncu * waves) / 8 * 8);
   }
-  switch (waves) {
-    case 3: launch_pair_kernel<T, 3>(tu.persistent != 0, grd, g, st, d_desc, remap, nitems, max_blk); break;
-    case 4: launch_pair_kernel<T, 4>(tu.persistent != 0, grd, g, st, d_desc, remap, nitems, max_blk); break;
-    default: launch_pair_kernel<T, 2>(tu.persistent != 0, grd, g, st, d_desc, remap, nitems, max_blk); break;
+  const bool pers = tu.persistent != 0;
+  if (any_alch) {
+    // the soft-core form at the default occupancy target only
+    launch_pair_kernel<T, sizeof(T) == 8 ? 3 : 4, true>(pers, grd, g, st, d_desc, remap, nitems, max_blk);
+  } else {
+    switch (waves) {
+      case 3: launch_pair_kernel<T, 3, false>(pers, grd, g, st, d_desc, remap, nitems, max_blk); break;
+      case 4: launch_pair_kernel<T, 4, false>(pers, grd, g, st, d_desc, remap, nitems, max_blk); break;
+      default: launch_pair_kernel<T, 2, false>(pers, grd, g, st, d_desc, remap, nitems, max_blk); break;
+    }
   }
   if (kend) ORAC_HIP(hipEventRecord(kend, st));
-  hipLaunchKernelGGL(k_pairs_finalize, dim3(15, nsys), dim3(256), 0, st, d_desc);
+  hipLaunchKernelGGL(k_pairs_finalize, dim3(PAIR_NACC, nsys), dim3(256), 0, st, d_desc);
 }
 
 template void launch_pairs<double>(const PairDesc *, const PairDesc *, int, hipStream_t, const Tuning &, hipEvent_t,

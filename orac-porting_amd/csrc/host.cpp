@@ -3,6 +3,9 @@
 #include <cmath>
 #include <cstdlib>
 
+#include <map>
+#include <tuple>
+
 #include "host.hpp"
 
 namespace orac {
@@ -302,6 +305,89 @@ double erfc_as(double x) {
   double qt = 1.0 / (1.0 + qp * x);
   double expcst = std::exp(-(x * x));
   return ((((a5 * qt + a4) * qt + a3) * qt + a2) * qt + a1) * qt * expcst;
+}
+
+// ---------------------------------------------------------------- alchemy
+
+AlchTables alch_tables(int n, const int32_t *ss_alchemy, const double *rmin, const double *epsm, int opls) {
+  AlchTables T;
+  T.cls.resize(n);
+  std::map<std::tuple<double, double, int>, int> ids;
+  std::vector<double> cr, ce;
+  std::vector<int> cs;
+  for (int a = 0; a < n; ++a) {
+    const int s = ss_alchemy[a];
+    ORAC_REQUIRE(s == 1 || s == 2 || s == -2, E_ARG, "ss_alchemy must be 1, 2 or -2");
+    ORAC_REQUIRE(epsm[a] >= 0.0 && std::isfinite(rmin[a]), E_ARG, "bad rmin or epsm");
+    auto key = std::make_tuple(rmin[a], epsm[a], s);
+    auto it = ids.find(key);
+    if (it == ids.end()) {
+      it = ids.emplace(key, (int)cr.size()).first;
+      cr.push_back(rmin[a]);
+      ce.push_back(epsm[a]);
+      cs.push_back(s);
+    }
+    T.cls[a] = it->second;
+  }
+  T.ncls = (int)cr.size();
+  ORAC_REQUIRE(T.ncls <= LJ_DENSE_MAX, E_UNSUPPORTED, "more than 2048 alchemical classes");
+  T.pair.resize((size_t)T.ncls * T.ncls);
+  for (int i = 0; i < T.ncls; ++i)
+    for (int j = 0; j < T.ncls; ++j) {
+      // force_alchemy.f:432-437
+      const double sigma = opls ? 2.0 * std::pow(cr[j] * cr[i], 0.5) : cr[j] + cr[i];
+      const double epsmin = 4.0 * std::sqrt(ce[i] * ce[j]);
+      const double s6 = std::pow(sigma, 6.0);
+      T.pair[(size_t)i * T.ncls + j] = make_double4(s6 > 0.0 ? 1.0 / s6 : 0.0, epsmin, cs[i] + cs[j] + 4, 0.0);
+    }
+  return T;
+}
+
+void alch_set_host(double fact[18], double ladd, double lrm, bool intramol, bool ljintra, bool qq) {
+  for (int k = 0; k < 18; ++k) fact[k] = 0.0;
+  auto F = [&](int t, int k) -> double & { return fact[2 * (t + 4) + k]; };
+  const double a = std::fabs(ladd), r = std::fabs(lrm);
+  F(2, 0) = 0.0; F(2, 1) = 1.0;    // normal-normal
+  F(3, 0) = a; F(3, 1) = 1.0 - a;  // normal-added
+  F(-1, 0) = r; F(-1, 1) = 1.0 - r;  // normal-removed
+  if (intramol && qq) {
+    F(4, 0) = a * a; F(4, 1) = 1.0 - a * a;
+    F(-4, 0) = r * r; F(-4, 1) = 1.0 - r * r;
+  } else if (intramol && !ljintra) {
+    F(4, 0) = a; F(4, 1) = 1.0 - a;
+    F(-4, 0) = r; F(-4, 1) = 1.0 - r;
+  } else {
+    F(4, 0) = 0.0; F(4, 1) = 1.0;
+    F(-4, 0) = 0.0; F(-4, 1) = 1.0;
+  }
+  F(0, 0) = 0.0; F(0, 1) = 0.0;    // added-removed
+}
+
+void alch_factors(const double lam[8], bool intramol, bool ljintra, double qsoft, double ljsoft, double out[72]) {
+  double fl[18], fl0[18], fq[18], fq0[18];
+  alch_set_host(fl, lam[0], lam[1], intramol, ljintra, false);
+  alch_set_host(fl0, lam[2], lam[3], intramol, ljintra, false);
+  alch_set_host(fq, lam[4], lam[5], intramol, ljintra, true);
+  alch_set_host(fq0, lam[6], lam[7], intramol, ljintra, true);
+  for (int t = 0; t < 9; ++t) {
+    double *o = out + 8 * t;
+    o[0] = ljsoft * fl[2 * t];
+    o[1] = fl[2 * t + 1];
+    o[2] = ljsoft * fl0[2 * t];
+    o[3] = fl0[2 * t + 1];
+    o[4] = qsoft * (fq[2 * t] * fq[2 * t]);
+    o[5] = fq[2 * t + 1];
+    o[6] = qsoft * (fq0[2 * t] * fq0[2 * t]);
+    o[7] = fq0[2 * t + 1];
+  }
+}
+
+void alch_charges(int n, const double *q, const int32_t *ss_alchemy, double laddq, double lrmq, double *out) {
+  for (int a = 0; a < n; ++a) {
+    const double l = ss_alchemy[a] == 2 ? laddq : (ss_alchemy[a] == -2 ? -lrmq : 0.0);
+    const double m = std::fabs(l);
+    out[a] = m == 1.0 ? 0.0 : (m == 0.0 ? q[a] : (1. - m) * q[a]);
+  }
 }
 
 }  // namespace orac

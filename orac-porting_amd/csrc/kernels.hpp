@@ -81,8 +81,14 @@ struct PairDesc {
   int4 *sgi;                        // [ngrp] {sorted first atom, first atom, natoms | gss << 8, group}
   double4 *sap;                     // [natoms] {xp, yp, zp, q}
   int2 *sai;                        // [natoms] {type, flags}; flags bit 0 = ex_inter
-  double *out;                      // [15]: uconf[3], ucoul[3] by typeij, st[9] row-major
+  double *out;                      // [16]: uconf[3], ucoul[3] by typeij, st[9] row-major, dwrk
   int max_group;                    // largest group (atoms)
+  // Alchemy (force_alchemy.f): with alch_pair set, `type` is each atom's
+  // alchemical class (distinct rmin, epsm, ss_alchemy) and the pairs take the
+  // soft-core form.
+  const double4 *alch_pair;         // [ntypes x ntypes] {1 / sigma^6, epsmin, typeaij + 4, 0}
+  const double *alch_fac;           // [9][8] by typeaij + 4: ljsoft f1, f2, ljsoft f1_0, f2_0,
+                                    //   qsoft fq1^2, fq2, qsoft fq1_0^2, fq2_0
 };
 
 constexpr int PAIR_WAVES = 4;       // owner groups per 256-thread block
@@ -175,6 +181,7 @@ struct CorrDesc {
   int natoms;
   double *fx, *fy, *fz;               // accumulated (+=)
   double *out;                        // [4]: coul slt, coul slv, lj slt, lj slv
+  const double *q0;                   // ferrf_alchemy: lambda0 charges; en0 goes to the lj slt column
 };
 void launch_corr(const CorrDesc *d_desc, const CorrDesc *h, int nsys, hipStream_t st);
 

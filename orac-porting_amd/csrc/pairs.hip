@@ -615,6 +615,11 @@ __global__ void __launch_bounds__(256) k_corr_pairs(const CorrDesc *__restrict__
     double erfst = 1.0 - erfcst;
     ecoul = -(furpar * erfst / rsp);
     if (d.iz == 1) qforce = -(furpar * (erfst - d.twrtpi * alphar * expcst) / (rsp * rsq));
+    if (d.q0) {
+      // ferrf_alchemy.f:144-145,190-191: the same term on the lambda0 charges (en0)
+      double furpar0 = d.iz == 1 ? fud * d.q0[ia] * d.q0[ib] : d.q0[ia] * d.q0[ib];
+      elj = -(furpar0 * erfst / rsp);
+    }
   } else {
     // fnbgrp.f:120-202 (reduced coordinates, no PBC)
     xab = d.xp[ia] - d.xp[ib];
@@ -651,8 +656,13 @@ __global__ void __launch_bounds__(256) k_corr_pairs(const CorrDesc *__restrict__
   double *pe = d.pe + 4 * (size_t)p;
   pe[0] = c == 0 ? ecoul : 0.0;
   pe[1] = c == 1 ? ecoul : 0.0;
-  pe[2] = c == 0 ? elj : 0.0;
-  pe[3] = c == 1 ? elj : 0.0;
+  if (d.kind == 0 && d.q0) {  // en0 of every pair in one column
+    pe[2] = elj;
+    pe[3] = 0.0;
+  } else {
+    pe[2] = c == 0 ? elj : 0.0;
+    pe[3] = c == 1 ? elj : 0.0;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_corr_gather(const CorrDesc *__restrict__ D) {

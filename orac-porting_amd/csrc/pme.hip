@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(256) k_fft_pencil(const PmeDesc *__restrict__ 
   extern __shared__ __align__(16) unsigned char smem[];
   const PmeDesc &d = D[blockIdx.y];
   const int H = d.K1 / 2 + 1, TP = d.tpy;
-  if ((int)blockIdx.x >= d.gy) return;
+  if ((int)blockIdx.x >= d.gy || (S > 0 && d.energy_only)) return;
   const FftAxis &ax = axis == 2 ? d.ax2 : d.ax3;
   const int N = ax.n;
   const int npen = axis == 2 ? H * d.K3 : H * d.K2;
@@ -501,11 +501,13 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(const PmeDesc *__restrict__ 
       r[t * N + e] = {T((double)qv.x * eterm), T((double)qv.y * eterm)};
     }
     __syncthreads();
-    cx<T> *o = lds_fft<T, +1>(r, s, N, TP, d.ax3);
-    for (int i = threadIdx.x; i < TP * N; i += blockDim.x) {
-      int e = i / TP, t = i - e * TP;
-      int p = p0 + t;
-      if (p < npen) FQ[(size_t)p + (size_t)e * Se] = o[t * N + e];
+    if (!d.energy_only) {
+      cx<T> *o = lds_fft<T, +1>(r, s, N, TP, d.ax3);
+      for (int i = threadIdx.x; i < TP * N; i += blockDim.x) {
+        int e = i / TP, t = i - e * TP;
+        int p = p0 + t;
+        if (p < npen) FQ[(size_t)p + (size_t)e * Se] = o[t * N + e];
+      }
     }
   }
   // fixed-order block reduction of the 7 partial sums
@@ -529,7 +531,7 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(const PmeDesc *__restrict__ D
   extern __shared__ __align__(16) unsigned char smem[];
   const PmeDesc &d = D[blockIdx.y];
   const int N = d.K1, H = N / 2 + 1, rows = d.K2 * d.K3, TP = d.tpx;
-  if ((int)blockIdx.x >= d.gx) return;
+  if ((int)blockIdx.x >= d.gx || d.energy_only) return;
   cx<T> *a = reinterpret_cast<cx<T> *>(smem);
   cx<T> *b = a + TP * N;
   const int row0 = blockIdx.x * TP;
@@ -562,7 +564,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_gather(const PmeDesc *__restrict__ D) {
   __shared__ double red[8][9];
   const PmeDesc &d = D[blockIdx.y];
-  if ((int)blockIdx.x >= d.gather_nblk) return;
+  if ((int)blockIdx.x >= d.gather_nblk || d.energy_only) return;
   const int order = d.order;
   const int apw = 64 / order;
   const int lane = threadIdx.x & 63;
@@ -676,6 +678,10 @@ __global__ void __launch_bounds__(256) k_pme_finalize(const PmeDesc *__restrict_
   __shared__ double red[4];
   const PmeDesc &d = D[blockIdx.y];
   const int k = blockIdx.x;
+  if (k >= 7 && d.energy_only) {  // no gather: no molecular-virial terms
+    if (threadIdx.x == 0) d.out[10 + (k - 7)] = 0.0;
+    return;
+  }
   const double *src = k < 7 ? d.conv_part + k : d.gather_part + (k - 7);
   const int nrow = k < 7 ? d.conv_nblk : d.gather_nblk, stride = k < 7 ? 8 : 9;
   double s = 0.0;

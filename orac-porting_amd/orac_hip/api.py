@@ -53,6 +53,7 @@ EXPORTS = [
     "orac_hip_create", "orac_hip_destroy", "orac_hip_last_error", "orac_hip_invalidate", "orac_hip_device_name",
     "orac_hip_probe_copy",
     "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
+    "orac_hip_force_alchemy",
     "orac_hip_decide_force", "orac_hip_ferrf", "orac_hip_cself", "orac_hip_fnbgrp", "orac_hip_system_create",
     "orac_hip_system_destroy", "orac_hip_system_set_coords", "orac_hip_system_device_ptrs", "orac_hip_system_eval",
     "orac_hip_system_results", "orac_hip_system_profile", "orac_hip_system_stage_ms", "orac_hip_system_bytes",
@@ -60,7 +61,8 @@ EXPORTS = [
     "orac_hip_batch_profile", "orac_hip_batch_stage_ms", "orac_hip_batch_set_tuning", "orac_hip_batch_get_tuning",
     "orac_hip_batch_autotune", "orac_hip_md_setup", "orac_hip_md_set_state", "orac_hip_md_get_state",
     "orac_hip_batch_md_run", "orac_hip_batch_md_relax", "orac_hip_batch_md_scale_velocities",
-    "orac_hip_system_md_energies",
+    "orac_hip_system_md_energies", "orac_hip_system_set_alchemy", "orac_hip_system_set_lambda",
+    "orac_hip_system_alchemy_results",
 ]
 
 # stage_ms order of the resident evaluation
@@ -179,7 +181,7 @@ class Context:
     # -- force.f:1
     def force(self, ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, ypcm, zpcm,
               groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, fpx, fpy, fpz, stress, nnlpp, nnlpp1,
-              rneigh, rinn, rout, rtolinn, rtolout, gmass, iz, mass, massinfty, lskip_ewald, ewald):
+              rneigh, rinn, rout, rtolinn, rtolout, gmass, iz, mass, massinfty, lskip_ewald, ewald, alchemy=None):
         """Returns a dict with the six energies, ncount1 and the updated rtolinn.
 
         fpx/fpy/fpz and stress are accumulated in place, nnlpp1 filled.
@@ -194,17 +196,38 @@ class Context:
                 _i(atomg), _f(xpcm), _f(ypcm), _f(zpcm), _i(groupp) if groupp is not None else np.zeros(1, np.int32),
                 _i(atomp), _f(co), _f(ecc12), _f(ecc6), _i(mapnl), _i(grppt), _i(nnlpp), _f(gmass), _f(mass)]
         (ss, x, y, z, xg, yg, zg, q, nbt, tt, atg, xc, yc, zc, grp, atp, co_, e12, e6, mp, gpt, nl, gm, ms) = keep
-        _check(lib().orac_hip_force(
+        common = [
             self._h, _p(ss), _p(x), _p(y), _p(z), _p(xg), _p(yg), _p(zg), _p(q), _p(nbt), _p(tt), C.c_int32(ma),
             C.c_int32(nato), _p(atg), _p(xc), _p(yc), _p(zc), _p(grp), _p(atp), _p(co_), _p(e12), _p(e6),
             C.c_double(alphal), _p(mp), C.c_int32(ngrp), _p(gpt), *[C.byref(v) for v in e], _p(fpx), _p(fpy),
             _p(fpz), _p(stress), _p(nl), _p(nnlpp1), C.c_int64(nnlpp1.size), C.c_double(rneigh), C.c_double(rinn),
             C.c_double(rout), C.byref(rti), C.c_double(rtolout), _p(gm), C.c_int32(iz), C.byref(nc1), _p(ms),
-            C.c_double(massinfty), C.c_int32(int(bool(lskip_ewald))), C.c_int32(int(bool(ewald)))))
+            C.c_double(massinfty)]
+        flags = [C.c_int32(int(bool(lskip_ewald))), C.c_int32(int(bool(ewald)))]
+        dw = None
+        if alchemy is None:
+            _check(lib().orac_hip_force(*common, *flags))
+        else:
+            al, lam, dwrk = alchemy
+            dw = C.c_double(dwrk)
+            arrs = [_i(al.ss_alchemy), _f(lam), _f(al.rmin), _f(al.epsm)]
+            _check(lib().orac_hip_force_alchemy(
+                *common, _p(arrs[0]), _p(arrs[1]), C.c_int32(int(al.intramol)), C.c_int32(int(al.ljintra)),
+                C.c_int32(int(al.opls)), _p(arrs[2]), _p(arrs[3]), C.byref(dw), *flags, C.c_double(al.qsoft),
+                C.c_double(al.ljsoft)))
         names = ["uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"]
         out = {k: v.value for k, v in zip(names, e)}
         out.update(ncount1=nc1.value, rtolinn=rti.value)
+        if dw is not None:
+            out["dwrk"] = dw.value
         return out
+
+    def force_alchemy(self, *args, alchemy, lam, dwrk=0.0, **kw):
+        """force_alchemy.f: the arguments of force(), plus the Alchemy
+        description (orac_hip.synth.Alchemy), the eight lambdas (ladd, lrm,
+        ladd0, lrm0, laddq, lrmq, laddq0, lrmq0) and the incoming dwrk.  The
+        returned dict adds dwrk (incoming value + U(lambda) - U(lambda0))."""
+        return self.force(*args, alchemy=(alchemy, lam, dwrk), **kw)
 
     # -- mts_forces.f:212
     @staticmethod
@@ -267,6 +290,12 @@ ENERGY_FIELDS = ["eer", "uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucou
                  "corr_slv"]
 
 
+class _AlchDesc(C.Structure):
+    """orac_hip_alchemy_desc (include/orac_hip.h)."""
+    _fields_ = [("ss_alchemy", C.c_void_p), ("rmin", C.c_void_p), ("epsm", C.c_void_p), ("qsoft", C.c_double),
+                ("ljsoft", C.c_double), ("intramol", C.c_int32), ("ljintra", C.c_int32), ("opls", C.c_int32)]
+
+
 class ResidentSystem:
     """A system kept in HBM: topology once, coordinates per step, forces on the device.
 
@@ -314,6 +343,28 @@ class ResidentSystem:
         out.update(f=np.stack([fx, fy, fz]), stress=e[9:18].reshape(3, 3, order="F"), list_entries=int(e[18]),
                    self_slt=e[19], self_slv=e[20])
         return out
+
+    def set_alchemy(self, al, lam=None):
+        """Make the system alchemical with orac_hip.synth.Alchemy al, then set
+        the eight lambdas lam (synth.lambda_state order) if given."""
+        self._alch = dict(ss=_i(al.ss_alchemy), rmin=_f(al.rmin), epsm=_f(al.epsm))
+        d = _AlchDesc(ss_alchemy=self._alch["ss"].ctypes.data, rmin=self._alch["rmin"].ctypes.data,
+                      epsm=self._alch["epsm"].ctypes.data, qsoft=al.qsoft, ljsoft=al.ljsoft,
+                      intramol=int(al.intramol), ljintra=int(al.ljintra), opls=int(al.opls))
+        _check(lib().orac_hip_system_set_alchemy(self.ctx._h, self.id, C.byref(d)))
+        if lam is not None:
+            self.set_lambda(lam)
+
+    def set_lambda(self, lam):
+        lam = _f(lam)
+        assert lam.size == 8
+        _check(lib().orac_hip_system_set_lambda(self.ctx._h, self.id, _p(lam)))
+
+    def alchemy_results(self):
+        """dwrk = U(lambda) - U(lambda0) of the last eval and its parts."""
+        o = np.zeros(6)
+        _check(lib().orac_hip_system_alchemy_results(self.ctx._h, self.id, _p(o)))
+        return dict(zip(("dwrk", "direct", "recip", "corr", "self", "eer0"), o.tolist()))
 
     def profile(self, enable=True):
         _check(lib().orac_hip_system_profile(self.ctx._h, self.id, C.c_int32(1 if enable else 0)))
