@@ -232,6 +232,73 @@ def c0_leg(ctx, steps, md=True):
     return out
 
 
+def alchemy_leg(ctx, steps, md_steps=0):
+    """16 lambda-windows of a 48-atom ligand decoupled in a 40 A water box
+    (6,399 atoms, PME 48^3 order 5, rc 9 A), every window resident with its
+    own charge grids, one batch: soft-core direct pairs, reciprocal sum on the
+    window's charges and the energy-only grid of its neighbour window's
+    charges (dwrk = U(lambda_w) - U(lambda_w')).  Mixed against FP64: forces
+    and per-window dU/dlambda ~ dwrk / (lambda_w - lambda_w')."""
+    from orac_hip import md, synth
+    from orac_hip.api import Batch, ResidentSystem
+    s, al, lams = synth.alchemy_windows()
+    cp, cf = corrections(s)
+    res = {}
+    for prec in (1, 0):
+        rss = []
+        for lam in lams:
+            rs = ResidentSystem(ctx, s, precision=prec, corr_pairs=cp, corr_fudge=cf)
+            rs.set_alchemy(al, lam)
+            rss.append(rs)
+        b = Batch(ctx, rss)
+        tuning = b.autotune(reps=3)[0] if (prec == 1 and GRAPH) else b.tuning()
+        for _ in range(2):
+            b.eval(rebuild_list=True, graph=GRAPH)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.eval(rebuild_list=True, graph=GRAPH)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        win = [(rs.results()["f"], rs.alchemy_results()) for rs in rss]
+        stages = profile_stages(b, 3)
+        r = dict(dt=dt, win=win, stages=stages, tuning=tuning)
+        if md_steps and prec == 1:
+            dyns = [md.Dynamics(rs, s) for rs in rss]
+            md.relax_batch(b, 200, dmax=0.03)
+            for k, d in enumerate(dyns):
+                d.maxwell_boltzmann(300.0, 77 + k)
+            md.equilibrate(b, dyns, 300.0, 100, graph=GRAPH)
+            r["md_wall"] = time_md(ctx, b, md_steps, 5)
+        b.close()
+        for rs in rss:
+            rs.close()
+        res[prec] = r
+    n = len(lams)
+    dl = np.array([lam[1] - lam[3] for lam in lams])
+    kj = synth.EFACT / 1000.0  # kJ/mol per internal energy unit
+    du64 = np.array([w[1]["dwrk"] for w in res[0]["win"]]) / dl * kj
+    du32 = np.array([w[1]["dwrk"] for w in res[1]["win"]]) / dl * kj
+    ferr = max(relrms(a[0], b[0]) for a, b in zip(res[1]["win"], res[0]["win"]))
+    out = {"workload": "16 lambda-windows (charge and soft-core LJ lambda 0->1) of a 48-atom ligand in a 40 A water box "
+                       "(6,399 atoms), PME 48^3 order 5, alpha 0.35, rc 9 A; per window its own charge grid and the "
+                       "energy-only grid of the next window's charges; one batch, list rebuilt every evaluation",
+           "windows": n, "natoms": s.natoms, "steps": steps,
+           "mixed_evals_per_s": n * steps / res[1]["dt"], "fp64_evals_per_s": n * steps / res[0]["dt"],
+           "mixed_ms_per_step": 1e3 * res[1]["dt"] / steps, "fp64_ms_per_step": 1e3 * res[0]["dt"] / steps,
+           "lambda": [float(lam[1]) for lam in lams], "lambda0": [float(lam[3]) for lam in lams],
+           "dU_dlambda_fp64": du64.tolist(), "dU_dlambda_mixed": du32.tolist(),
+           "dU_dlambda_unit": "kJ/mol per unit lambda",
+           "dU_dlambda_max_abs_err_mixed": float(np.abs(du32 - du64).max()),
+           "dU_dlambda_max_rel_err_mixed": float(np.abs(du32 - du64).max() / np.abs(du64).max()),
+           "force_relrms_mixed_vs_f64_max": ferr, "tuning": res[1]["tuning"],
+           "stage_ms_mixed": {k: round(float(v), 4) for k, v in res[1]["stages"].items()}}
+    if "md_wall" in res[1]:
+        out["md_mixed"] = {"steps": md_steps, "ms_per_step": 1e3 * res[1]["md_wall"] / md_steps,
+                           "ns_per_day_aggregate": ns_per_day(md_steps * n, res[1]["md_wall"])}
+    return out
+
+
 def cpu_baseline(s, corr, nthreads):
     import oracle as O
     rc = s.rcut + s.rtol + s.rnei
@@ -297,6 +364,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true", help="launch batched and dynamics steps eagerly")
     ap.add_argument("--no-water", action="store_true", help="skip the 648-atom water box leg")
+    ap.add_argument("--no-alchemy", action="store_true", help="skip the lambda-window leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one device copy of this size first (calibrates HBM byte counters)")
@@ -352,6 +420,11 @@ def main():
             c0 = c0_leg(ctx, args.steps, md=args.md_steps > 0)
             log(f"water216: {c0['fp64_evals_per_s']:.0f} FP64 evals/s"
                 + (f", md {c0['md_100_steps']['ms_per_step']:.3f} ms/step" if "md_100_steps" in c0 else ""))
+        alch = None
+        if not args.no_alchemy:
+            alch = alchemy_leg(ctx, max(5, args.steps // 10), md_steps=args.md_steps)
+            log(f"alchemy x{alch['windows']}: mixed {alch['mixed_ms_per_step']:.3f} ms/step, fp64 "
+                f"{alch['fp64_ms_per_step']:.3f} ms/step, dU/dl rel err {alch['dU_dlambda_max_rel_err_mixed']:.2e}")
         dyn = None
         if args.md_steps > 0:
             K = args.md_steps
@@ -436,6 +509,8 @@ def main():
             "tuning": bm["tuning"], "stage_ms_mixed": rnd(bm["stages"]), "device_bytes": bm["device_bytes"]}
     if c0:
         out["water216"] = c0
+    if alch:
+        out["alchemy"] = alch
     if dyn:
         dyn["workload"] = ("NVE r-RESPA: 2 fs outer step, 4 inner steps for the solute harmonic terms, rigid waters "
                            "(SHAKE/RATTLE), list every 10 steps; after 200 steepest-descent steps and "
