@@ -40,7 +40,7 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_PAIR = 89
-# HBM bytes per launch of k_pairs<double,3> on the 24,460-atom system, from the
+# HBM bytes per launch of k_pairs<double,3,false> on the 24,460-atom system, from the
 # committed FETCH_SIZE / WRITE_SIZE profile (rocprof_summary traffic)
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bytes_k_pairs_fp64_solvated24k.txt")
 
@@ -365,6 +365,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch batched and dynamics steps eagerly")
     ap.add_argument("--no-water", action="store_true", help="skip the 648-atom water box leg")
     ap.add_argument("--no-alchemy", action="store_true", help="skip the lambda-window leg")
+    ap.add_argument("--only-alchemy", action="store_true",
+                    help="run the lambda-window leg alone and print its record (for profiling that workload)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one device copy of this size first (calibrates HBM byte counters)")
@@ -390,6 +392,13 @@ def main():
     corr = corrections(s)
     npairs = evaluated_pairs(s)
     log(f"system: {s.natoms} atoms, {s.ngrp} groups, {npairs} evaluated pairs")
+
+    if args.only_alchemy:
+        with Context(0) as ctx:
+            rec = alchemy_leg(ctx, args.steps, md_steps=args.md_steps)
+        sys.stdout.flush()
+        os.write(result_fd, (json.dumps(rec) + "\n").encode())
+        return
 
     with Context(0) as ctx:
         dev = ctx.device_name()
@@ -465,7 +474,7 @@ def main():
     achieved = FLOP_PER_PAIR * npairs / (kms * 1e-3) / 1e12
     from orac_hip.rocprof_summary import read_traffic
     traffic = read_traffic(TRAFFIC_PROFILE)
-    roof = {"bound": "mfma", "kernel": "k_pairs<double,3>", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+    roof = {"bound": "mfma", "kernel": "k_pairs<double,3,false>", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
             "per_launch": {"pairs": npairs, "flop_per_pair": FLOP_PER_PAIR, "ms": kms,
                            "traffic_unit": "bytes (FETCH_SIZE + WRITE_SIZE, probe-scaled)",

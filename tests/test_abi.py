@@ -53,3 +53,22 @@ def test_error_path_is_a_status_not_an_exception():
     with pytest.raises(OracHipError) as e:
         Context.lc_index(2500, 6, 6, 6, 9.1, co)   # a cell would be visited twice
     assert e.value.code == 2
+
+
+def test_alchemy_shim_compiles_against_the_reference_modules(tmp_path):
+    """fortran/orac_hip_shim_alchemy.f90 (the drop-in force_alchemy for ORAC's
+    own build) compiles against the modules of the reference build in
+    oracle/_ref and calls orac_hip_force_alchemy."""
+    import shutil
+    import subprocess
+    flang = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    mods = os.path.join(ROOT, "oracle", "_ref", "build")
+    if not (os.path.exists(flang) and os.path.exists(os.path.join(mods, "cpropar.mod"))):
+        pytest.skip("flang or the reference modules (make -C oracle ref) not available")
+    src = os.path.join(ROOT, "orac-porting_amd", "fortran")
+    subprocess.run([flang, "-c", "-fPIC", "-module-dir", str(tmp_path), os.path.join(src, "orac_hip_shim.f90"),
+                    "-o", str(tmp_path / "shim.o")], check=True)
+    subprocess.run([flang, "-c", "-fPIC", "-I", str(tmp_path), "-I", mods,
+                    os.path.join(src, "orac_hip_shim_alchemy.f90"), "-o", str(tmp_path / "alch.o")], check=True)
+    syms = subprocess.run(["nm", str(tmp_path / "alch.o")], check=True, capture_output=True, text=True).stdout
+    assert re.search(r"\bT force_alchemy_\b", syms) and "orac_hip_force_alchemy" in syms
