@@ -125,6 +125,26 @@ int orac_hip_force_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, const dou
                            int32_t ljintra, int32_t opls, const double *rmin, const double *epsm, double *dwrk,
                            int32_t lskip_ewald, int32_t ewald, double qsoft, double ljsoft);
 
+/* Replaces force_alchemy_shift (force_alchemy_shifted.f:1-730), the
+ * shifted-potential form mts_forces.f:130 selects with SHIFT-POT: arguments
+ * as orac_hip_force_alchemy with qshift and ljshift in place of qsoft and
+ * ljsoft. */
+int orac_hip_force_alchemy_shift(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                                 const double *zp0, const double *xpg, const double *ypg, const double *zpg,
+                                 const double *charge, const int32_t *nbtype, const int32_t *type, int32_t ma,
+                                 int32_t nato, const int32_t *atomg, const double *xpcm, const double *ypcm,
+                                 const double *zpcm, const int32_t *groupp, const int32_t *atomp, const double *co,
+                                 const double *ecc12, const double *ecc6, double alphal, const int32_t *mapnl,
+                                 int32_t ngrp, const int32_t *grppt, double *uconf_slt, double *uconf_slv,
+                                 double *uconf_ss, double *ucoul_slt, double *ucoul_slv, double *ucoul_ss,
+                                 double *fpx, double *fpy, double *fpz, double *stress, const int32_t *nnlpp,
+                                 int32_t *nnlpp1, int64_t nnlpp1_capacity, double rneigh, double rinn, double rout,
+                                 double *rtolinn, double rtolout, const double *gmass, int32_t iz, int32_t *ncount1,
+                                 const double *mass, double massinfty, const int32_t *ss_alchemy, const double *lam,
+                                 int32_t intramol, int32_t ljintra, int32_t opls, const double *rmin,
+                                 const double *epsm, double *dwrk, int32_t lskip_ewald, int32_t ewald,
+                                 double qshift, double ljshift);
+
 /* Replaces decide_force (mts_forces.f:212-241): lskip_ewald = erfc_AS(alpha*rinn) < 1e-3. */
 int orac_hip_decide_force(double rinn, double alphal, int32_t *lskip_ewald);
 
@@ -226,6 +246,8 @@ typedef struct orac_hip_alchemy_desc {
   const double *rmin, *epsm;          /* natoms */
   double qsoft, ljsoft;
   int32_t intramol, ljintra, opls;
+  int32_t shifted;                    /* 1: shifted potential (force_alchemy_shifted.f) with qshift, ljshift */
+  double qshift, ljshift;
 } orac_hip_alchemy_desc;
 int orac_hip_system_set_alchemy(orac_hip_ctx *ctx, int32_t sys, const orac_hip_alchemy_desc *desc);
 /* lam = ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0 (initially 1, 0, 1, 0, 1, 0, 1, 0). */

@@ -270,7 +270,8 @@ struct AlchIn {
   int intramol, ljintra, opls;
   const double *rmin, *epsm;
   double *dwrk;
-  double qsoft, ljsoft;
+  double qsoft, ljsoft;  // (qshift, ljshift) when shifted
+  int shifted;
 };
 
 static int force_impl(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
@@ -404,7 +405,8 @@ static int force_impl(orac_hip_ctx *ctx, const int32_t *ss_index, const double *
       ORAC_REQUIRE(al->ss_alchemy && al->lam && al->rmin && al->epsm && al->dwrk, E_ARG, "null alchemy argument");
       AlchTables A = alch_tables(nato, al->ss_alchemy, al->rmin, al->epsm, al->opls);
       double fac[72];
-      alch_factors(al->lam, al->intramol != 0, al->ljintra != 0, al->qsoft, al->ljsoft, fac);
+      alch_factors(al->lam, al->intramol != 0, al->ljintra != 0, al->qsoft, al->ljsoft, al->shifted != 0, fac);
+      p.alch_shifted = al->shifted ? 1 : 0;
       p.ntypes = A.ncls;
       p.type = B.put_vec("f_atype", A.cls, st, true);
       p.alch_pair = B.put_vec("f_apair", A.pair, st, true);
@@ -552,7 +554,26 @@ extern "C" int orac_hip_force_alchemy(
     int32_t *ncount1, const double *mass, double massinfty, const int32_t *ss_alchemy, const double *lam,
     int32_t intramol, int32_t ljintra, int32_t opls, const double *rmin, const double *epsm, double *dwrk,
     int32_t lskip_ewald, int32_t ewald, double qsoft, double ljsoft) {
-  AlchIn al{ss_alchemy, lam, intramol, ljintra, opls, rmin, epsm, dwrk, qsoft, ljsoft};
+  AlchIn al{ss_alchemy, lam, intramol, ljintra, opls, rmin, epsm, dwrk, qsoft, ljsoft, 0};
+  return force_impl(ctx, ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, ypcm,
+                    zpcm, groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, uconf_ss,
+                    ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, nnlpp1_capacity, rneigh,
+                    rinn, rout, rtolinn, rtolout, gmass, iz, ncount1, mass, massinfty, lskip_ewald, ewald, &al);
+}
+
+extern "C" int orac_hip_force_alchemy_shift(
+    orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+    const double *xpg, const double *ypg, const double *zpg, const double *charge, const int32_t *nbtype,
+    const int32_t *type, int32_t ma, int32_t nato, const int32_t *atomg, const double *xpcm, const double *ypcm,
+    const double *zpcm, const int32_t *groupp, const int32_t *atomp, const double *co, const double *ecc12,
+    const double *ecc6, double alphal, const int32_t *mapnl, int32_t ngrp, const int32_t *grppt, double *uconf_slt,
+    double *uconf_slv, double *uconf_ss, double *ucoul_slt, double *ucoul_slv, double *ucoul_ss, double *fpx,
+    double *fpy, double *fpz, double *stress, const int32_t *nnlpp, int32_t *nnlpp1, int64_t nnlpp1_capacity,
+    double rneigh, double rinn, double rout, double *rtolinn, double rtolout, const double *gmass, int32_t iz,
+    int32_t *ncount1, const double *mass, double massinfty, const int32_t *ss_alchemy, const double *lam,
+    int32_t intramol, int32_t ljintra, int32_t opls, const double *rmin, const double *epsm, double *dwrk,
+    int32_t lskip_ewald, int32_t ewald, double qshift, double ljshift) {
+  AlchIn al{ss_alchemy, lam, intramol, ljintra, opls, rmin, epsm, dwrk, qshift, ljshift, 1};
   return force_impl(ctx, ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, ypcm,
                     zpcm, groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, uconf_ss,
                     ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, nnlpp1_capacity, rneigh,

@@ -45,8 +45,8 @@ struct System {
   // energy-only grid on the lambda0 charges
   bool alch = false;
   std::vector<int32_t> ss_alch;
-  bool intramol = false, ljintra = false;
-  double qsoft = 0.5, ljsoft = 0.5;
+  bool intramol = false, ljintra = false, shifted = false;
+  double qsoft = 0.5, ljsoft = 0.5;  // (qshift, ljshift) when shifted
   double lam[8] = {1, 0, 1, 0, 1, 0, 1, 0};
   double self_dw = 0.0;
   double *qc = nullptr, *q0 = nullptr, *afac = nullptr;
@@ -1063,7 +1063,7 @@ static void alch_apply_lambda(orac_hip_ctx *ctx, System &S) {
   hipStream_t st = ctx->stream;
   const int n = S.natoms;
   double fac[72];
-  alch_factors(S.lam, S.intramol, S.ljintra, S.qsoft, S.ljsoft, fac);
+  alch_factors(S.lam, S.intramol, S.ljintra, S.qsoft, S.ljsoft, S.shifted, fac);
   std::vector<double> qc(n), q0(n);
   alch_charges(n, S.q_host.data(), S.ss_alch.data(), S.lam[4], S.lam[5], qc.data());
   alch_charges(n, S.q_host.data(), S.ss_alch.data(), S.lam[6], S.lam[7], q0.data());
@@ -1100,8 +1100,10 @@ extern "C" int orac_hip_system_set_alchemy(orac_hip_ctx *ctx, int32_t sys, const
     S.ss_alch.assign(a->ss_alchemy, a->ss_alchemy + n);
     S.intramol = a->intramol != 0;
     S.ljintra = a->ljintra != 0;
-    S.qsoft = a->qsoft;
-    S.ljsoft = a->ljsoft;
+    S.shifted = a->shifted != 0;
+    S.qsoft = S.shifted ? a->qshift : a->qsoft;
+    S.ljsoft = S.shifted ? a->ljshift : a->ljsoft;
+    S.pair.alch_shifted = S.shifted ? 1 : 0;
     S.pair.type = B.put_vec("atype", A.cls, st, false);
     S.pair.ntypes = A.ncls;
     S.pair.alch_pair = B.put_vec("apair", A.pair, st, false);

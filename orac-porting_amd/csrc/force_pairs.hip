@@ -241,33 +241,63 @@ __device__ __forceinline__ AlchTerms alch_terms(const PairDesc &d, double pxs, d
   const T zc = T(co[8] * zg);
   const T rsq = xc * xc + yc * yc + zc * zc;
   const T rinv = pair_rsqrt<T>(rsq);
-  const T auxaa = pair_rsqrt<T>(T(F[4]) + rsq);
-  const T auxa0 = pair_rsqrt<T>(T(F[6]) + rsq);
   T qforce = T(0), conf = T(0), conf0 = T(0), cou = T(0), cou0 = T(0);
-  if (ap.y > 1e-12) {
-    const T r6 = rsq * rsq * rsq * T(ap.x);
-    const T gp1 = pair_rcp<T>(T(F[0]) + r6), gp2 = gp1 * gp1;
-    const T gp10 = pair_rcp<T>(T(F[2]) + r6), gp20 = gp10 * gp10;
-    const T eps = T(ap.y);
-    const T ssvir = T(F[1]) * eps * (T(2) * gp1 * gp2 - gp2);
-    qforce = T(6) * r6 * ssvir * (rinv * rinv);
-    conf = T(F[1]) * (eps * (gp2 - gp1));
-    conf0 = T(F[3]) * (eps * (gp20 - gp10));
-  }
   const T qq = T(qa * pb4.w);
-  if (!d.lskip_ewald) {
-    const T alphar = T(d.alpha) * (rsq * rinv);
-    const T qt = pair_rcp<T>(T(1) + T(AS_QP) * alphar);
-    const T expcst = pair_exp<T>(-(alphar * alphar));
-    const T erfcst = ((((T(AS_A5) * qt + T(AS_A4)) * qt + T(AS_A3)) * qt + T(AS_A2)) * qt + T(AS_A1)) * qt * expcst;
-    const T furpar = T(F[5]) * qq, furpar0 = T(F[7]) * qq;
-    cou = furpar * erfcst * auxaa;
-    cou0 = furpar0 * erfcst * auxa0;
-    qforce += furpar * auxaa * (erfcst * (auxaa * auxaa) + T(d.twrtpi) * alphar * expcst * (rinv * rinv));
-  } else if (!d.ewald) {
-    cou = T(F[5]) * qq * auxaa;
-    cou0 = T(F[7]) * qq * auxa0;
-    qforce += cou * (auxaa * auxaa);
+  if (d.alch_shifted) {
+    // force_alchemy_shifted.f:469-509: LJ at r + ljshift 2^(1/6) sigma f1, Coulomb over
+    // r + qshift fq1 (erfc of the real distance)
+    const T r = rsq * rinv;
+    if (ap.y > 1e-12) {
+      const T sig = T(ap.w), eps = T(ap.y);
+      const T irl = pair_rcp<T>(r + T(F[0]) * sig), irl0 = pair_rcp<T>(r + T(F[2]) * sig);
+      const T x = sig * irl, x2 = x * x, r6 = x2 * x2 * x2;
+      const T y = sig * irl0, y2 = y * y, r60 = y2 * y2 * y2;
+      qforce = T(F[1]) * eps * (T(12) * r6 * r6 - T(6) * r6) * (rinv * irl);
+      conf = T(F[1]) * (eps * (r6 * r6 - r6));
+      conf0 = T(F[3]) * (eps * (r60 * r60 - r60));
+    }
+    const T irq = pair_rcp<T>(r + T(F[4])), irq0 = pair_rcp<T>(r + T(F[6]));
+    if (!d.lskip_ewald) {
+      const T alphar = T(d.alpha) * r;
+      const T qt = pair_rcp<T>(T(1) + T(AS_QP) * alphar);
+      const T expcst = pair_exp<T>(-(alphar * alphar));
+      const T erfcst = ((((T(AS_A5) * qt + T(AS_A4)) * qt + T(AS_A3)) * qt + T(AS_A2)) * qt + T(AS_A1)) * qt * expcst;
+      const T furpar = T(F[5]) * qq;
+      cou = furpar * erfcst * irq;
+      cou0 = T(F[7]) * qq * erfcst * irq0;
+      qforce += furpar * (rinv * irq) * (erfcst * irq + T(d.twrtpi) * alphar * expcst * rinv);
+    } else if (!d.ewald) {
+      cou = T(F[5]) * qq * irq;
+      cou0 = T(F[7]) * qq * irq0;
+      qforce += cou * (irq * rinv);
+    }
+  } else {
+    const T auxaa = pair_rsqrt<T>(T(F[4]) + rsq);
+    const T auxa0 = pair_rsqrt<T>(T(F[6]) + rsq);
+    if (ap.y > 1e-12) {
+      const T r6 = rsq * rsq * rsq * T(ap.x);
+      const T gp1 = pair_rcp<T>(T(F[0]) + r6), gp2 = gp1 * gp1;
+      const T gp10 = pair_rcp<T>(T(F[2]) + r6), gp20 = gp10 * gp10;
+      const T eps = T(ap.y);
+      const T ssvir = T(F[1]) * eps * (T(2) * gp1 * gp2 - gp2);
+      qforce = T(6) * r6 * ssvir * (rinv * rinv);
+      conf = T(F[1]) * (eps * (gp2 - gp1));
+      conf0 = T(F[3]) * (eps * (gp20 - gp10));
+    }
+    if (!d.lskip_ewald) {
+      const T alphar = T(d.alpha) * (rsq * rinv);
+      const T qt = pair_rcp<T>(T(1) + T(AS_QP) * alphar);
+      const T expcst = pair_exp<T>(-(alphar * alphar));
+      const T erfcst = ((((T(AS_A5) * qt + T(AS_A4)) * qt + T(AS_A3)) * qt + T(AS_A2)) * qt + T(AS_A1)) * qt * expcst;
+      const T furpar = T(F[5]) * qq, furpar0 = T(F[7]) * qq;
+      cou = furpar * erfcst * auxaa;
+      cou0 = furpar0 * erfcst * auxa0;
+      qforce += furpar * auxaa * (erfcst * (auxaa * auxaa) + T(d.twrtpi) * alphar * expcst * (rinv * rinv));
+    } else if (!d.ewald) {
+      cou = T(F[5]) * qq * auxaa;
+      cou0 = T(F[7]) * qq * auxa0;
+      qforce += cou * (auxaa * auxaa);
+    }
   }
   AlchTerms r;
   r.qf = (double)qforce;

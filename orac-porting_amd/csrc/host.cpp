@@ -338,7 +338,7 @@ AlchTables alch_tables(int n, const int32_t *ss_alchemy, const double *rmin, con
       const double sigma = opls ? 2.0 * std::pow(cr[j] * cr[i], 0.5) : cr[j] + cr[i];
       const double epsmin = 4.0 * std::sqrt(ce[i] * ce[j]);
       const double s6 = std::pow(sigma, 6.0);
-      T.pair[(size_t)i * T.ncls + j] = make_double4(s6 > 0.0 ? 1.0 / s6 : 0.0, epsmin, cs[i] + cs[j] + 4, 0.0);
+      T.pair[(size_t)i * T.ncls + j] = make_double4(s6 > 0.0 ? 1.0 / s6 : 0.0, epsmin, cs[i] + cs[j] + 4, sigma);
     }
   return T;
 }
@@ -363,21 +363,24 @@ void alch_set_host(double fact[18], double ladd, double lrm, bool intramol, bool
   F(0, 0) = 0.0; F(0, 1) = 0.0;    // added-removed
 }
 
-void alch_factors(const double lam[8], bool intramol, bool ljintra, double qsoft, double ljsoft, double out[72]) {
+void alch_factors(const double lam[8], bool intramol, bool ljintra, double qx, double ljx, bool shifted,
+                  double out[72]) {
   double fl[18], fl0[18], fq[18], fq0[18];
   alch_set_host(fl, lam[0], lam[1], intramol, ljintra, false);
   alch_set_host(fl0, lam[2], lam[3], intramol, ljintra, false);
   alch_set_host(fq, lam[4], lam[5], intramol, ljintra, true);
   alch_set_host(fq0, lam[6], lam[7], intramol, ljintra, true);
+  // two6 = 2**(1./6.) with the REAL(4) exponent (force_alchemy_shifted.f:178)
+  const double lj = shifted ? ljx * std::pow(2.0, (double)(1.0f / 6.0f)) : ljx;
   for (int t = 0; t < 9; ++t) {
     double *o = out + 8 * t;
-    o[0] = ljsoft * fl[2 * t];
+    o[0] = lj * fl[2 * t];
     o[1] = fl[2 * t + 1];
-    o[2] = ljsoft * fl0[2 * t];
+    o[2] = lj * fl0[2 * t];
     o[3] = fl0[2 * t + 1];
-    o[4] = qsoft * (fq[2 * t] * fq[2 * t]);
+    o[4] = shifted ? qx * fq[2 * t] : qx * (fq[2 * t] * fq[2 * t]);
     o[5] = fq[2 * t + 1];
-    o[6] = qsoft * (fq0[2 * t] * fq0[2 * t]);
+    o[6] = shifted ? qx * fq0[2 * t] : qx * (fq0[2 * t] * fq0[2 * t]);
     o[7] = fq0[2 * t + 1];
   }
 }

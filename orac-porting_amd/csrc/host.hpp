@@ -141,7 +141,7 @@ int guarded(F &&f) {
 
 // ---- alchemy (force_alchemy.f, alch_set in force_alchemy_shifted.f:732-790)
 // Atoms with equal (rmin, epsm, ss_alchemy) share a class; the pair table
-// holds {1 / sigma^6, epsmin, typeaij + 4, 0} for every class pair.
+// holds {1 / sigma^6, epsmin, typeaij + 4, sigma} for every class pair.
 struct AlchTables {
   std::vector<int> cls;  // class of each atom
   int ncls = 0;
@@ -150,8 +150,10 @@ struct AlchTables {
 AlchTables alch_tables(int n, const int32_t *ss_alchemy, const double *rmin, const double *epsm, int opls);
 // alch_fact(1:2, -4:4) as fact[2 * (typeaij + 4) + k]
 void alch_set_host(double fact[18], double ladd, double lrm, bool intramol, bool ljintra, bool qq);
-// the pair kernel's [9][8] factor rows from lam = ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0
-void alch_factors(const double lam[8], bool intramol, bool ljintra, double qsoft, double ljsoft, double out[72]);
+// the pair kernel's [9][8] factor rows from lam = ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0;
+// (qx, ljx) are (qsoft, ljsoft), or (qshift, ljshift) for the shifted potential
+void alch_factors(const double lam[8], bool intramol, bool ljintra, double qx, double ljx, bool shifted,
+                  double out[72]);
 // charges at (laddq, lrmq): (1 - |lambdaq|) q, end points exact (mts_furier_alchemy.f:98-111)
 void alch_charges(int n, const double *q, const int32_t *ss_alchemy, double laddq, double lrmq, double *out);
 

@@ -86,9 +86,11 @@ struct PairDesc {
   // Alchemy (force_alchemy.f): with alch_pair set, `type` is each atom's
   // alchemical class (distinct rmin, epsm, ss_alchemy) and the pairs take the
   // soft-core form.
-  const double4 *alch_pair;         // [ntypes x ntypes] {1 / sigma^6, epsmin, typeaij + 4, 0}
+  const double4 *alch_pair;         // [ntypes x ntypes] {1 / sigma^6, epsmin, typeaij + 4, sigma}
   const double *alch_fac;           // [9][8] by typeaij + 4: ljsoft f1, f2, ljsoft f1_0, f2_0,
-                                    //   qsoft fq1^2, fq2, qsoft fq1_0^2, fq2_0
+                                    //   qsoft fq1^2, fq2, qsoft fq1_0^2, fq2_0 (soft core); shifted:
+                                    //   ljshift 2^(1/6) f1, f2, (lambda0), qshift fq1, fq2, (lambda0)
+  int alch_shifted;                 // 1: shifted potential (force_alchemy_shifted.f)
 };
 
 constexpr int PAIR_WAVES = 4;       // owner groups per 256-thread block

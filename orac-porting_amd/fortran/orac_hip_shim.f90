@@ -96,6 +96,23 @@ module orac_hip_iface
       real(c_double), value :: alphal, rneigh, rinn, rout, rtolout, massinfty, qsoft, ljsoft
       integer(c_int64_t), value :: cap
     end function
+    integer(c_int) function c_force_alchemy_shift(ctx, ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, &
+         nato, atomg, xpcm, ypcm, zpcm, groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, &
+         uconf_slv, uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, cap, rneigh, &
+         rinn, rout, rtolinn, rtolout, gmass, iz, ncount1, mass, massinfty, ss_alchemy, lam, intramol, ljintra, &
+         opls, rmin, epsm, dwrk, lskip_ewald, ewald, qsoft, ljsoft) bind(C, name="orac_hip_force_alchemy_shift")
+      import :: c_ptr, c_int, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), nbtype(*), type(*), atomg(*), groupp(*), atomp(*), mapnl(*), grppt(*)
+      integer(c_int32_t) :: nnlpp(*), nnlpp1(*), ncount1, ss_alchemy(*)
+      integer(c_int32_t), value :: ma, nato, ngrp, iz, lskip_ewald, ewald, intramol, ljintra, opls
+      real(c_double) :: xp0(*), yp0(*), zp0(*), xpg(*), ypg(*), zpg(*), charge(*), xpcm(*), ypcm(*), zpcm(*)
+      real(c_double) :: co(*), ecc12(*), ecc6(*), gmass(*), mass(*), fpx(*), fpy(*), fpz(*), stress(*)
+      real(c_double) :: uconf_slt, uconf_slv, uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss, rtolinn, dwrk
+      real(c_double) :: lam(8), rmin(*), epsm(*)
+      real(c_double), value :: alphal, rneigh, rinn, rout, rtolout, massinfty, qsoft, ljsoft
+      integer(c_int64_t), value :: cap
+    end function
     integer(c_int) function c_ferrf(ctx, ss_index, alphal, charge, fudge, x0, y0, z0, iz, list, nlist, fs_slt, &
          fs_slv, fpx, fpy, fpz, erf_corr, nbonds_added, mb) bind(C, name="orac_hip_ferrf")
       import :: c_ptr, c_int, c_int32_t, c_double

@@ -53,7 +53,7 @@ EXPORTS = [
     "orac_hip_create", "orac_hip_destroy", "orac_hip_last_error", "orac_hip_invalidate", "orac_hip_device_name",
     "orac_hip_probe_copy",
     "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
-    "orac_hip_force_alchemy",
+    "orac_hip_force_alchemy", "orac_hip_force_alchemy_shift",
     "orac_hip_decide_force", "orac_hip_ferrf", "orac_hip_cself", "orac_hip_fnbgrp", "orac_hip_system_create",
     "orac_hip_system_destroy", "orac_hip_system_set_coords", "orac_hip_system_device_ptrs", "orac_hip_system_eval",
     "orac_hip_system_results", "orac_hip_system_profile", "orac_hip_system_stage_ms", "orac_hip_system_bytes",
@@ -211,10 +211,12 @@ class Context:
             al, lam, dwrk = alchemy
             dw = C.c_double(dwrk)
             arrs = [_i(al.ss_alchemy), _f(lam), _f(al.rmin), _f(al.epsm)]
-            _check(lib().orac_hip_force_alchemy(
+            fn = lib().orac_hip_force_alchemy_shift if al.shifted else lib().orac_hip_force_alchemy
+            qx, ljx = al.soft_params()
+            _check(fn(
                 *common, _p(arrs[0]), _p(arrs[1]), C.c_int32(int(al.intramol)), C.c_int32(int(al.ljintra)),
-                C.c_int32(int(al.opls)), _p(arrs[2]), _p(arrs[3]), C.byref(dw), *flags, C.c_double(al.qsoft),
-                C.c_double(al.ljsoft)))
+                C.c_int32(int(al.opls)), _p(arrs[2]), _p(arrs[3]), C.byref(dw), *flags, C.c_double(qx),
+                C.c_double(ljx)))
         names = ["uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"]
         out = {k: v.value for k, v in zip(names, e)}
         out.update(ncount1=nc1.value, rtolinn=rti.value)
@@ -293,7 +295,8 @@ ENERGY_FIELDS = ["eer", "uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucou
 class _AlchDesc(C.Structure):
     """orac_hip_alchemy_desc (include/orac_hip.h)."""
     _fields_ = [("ss_alchemy", C.c_void_p), ("rmin", C.c_void_p), ("epsm", C.c_void_p), ("qsoft", C.c_double),
-                ("ljsoft", C.c_double), ("intramol", C.c_int32), ("ljintra", C.c_int32), ("opls", C.c_int32)]
+                ("ljsoft", C.c_double), ("intramol", C.c_int32), ("ljintra", C.c_int32), ("opls", C.c_int32),
+                ("shifted", C.c_int32), ("qshift", C.c_double), ("ljshift", C.c_double)]
 
 
 class ResidentSystem:
@@ -350,7 +353,8 @@ class ResidentSystem:
         self._alch = dict(ss=_i(al.ss_alchemy), rmin=_f(al.rmin), epsm=_f(al.epsm))
         d = _AlchDesc(ss_alchemy=self._alch["ss"].ctypes.data, rmin=self._alch["rmin"].ctypes.data,
                       epsm=self._alch["epsm"].ctypes.data, qsoft=al.qsoft, ljsoft=al.ljsoft,
-                      intramol=int(al.intramol), ljintra=int(al.ljintra), opls=int(al.opls))
+                      intramol=int(al.intramol), ljintra=int(al.ljintra), opls=int(al.opls),
+                      shifted=int(al.shifted), qshift=al.qshift, ljshift=al.ljshift)
         _check(lib().orac_hip_system_set_alchemy(self.ctx._h, self.id, C.byref(d)))
         if lam is not None:
             self.set_lambda(lam)

@@ -424,7 +424,10 @@ class Alchemy:
     epsmin = 4 sqrt(epsm_i epsm_j) in internal energy units
     (force_alchemy.f:431-437).  qsoft, ljsoft: Beutler soft-core parameters
     (defaults 0.5, blkdta.f90:477-478); intramol, ljintra: the INTRAMOL switches
-    of alch_set (defaults false, blkdta.f90:473-474).
+    of alch_set (defaults false, blkdta.f90:473-474).  shifted selects the
+    shifted potential (force_alchemy_shifted.f, SHIFT-POT) with qshift, ljshift
+    (defaults 4.0, 0.35, blkdta.f90:479-480) in place of the soft core
+    (mts_forces.f:130).
     """
     ss_alchemy: np.ndarray
     rmin: np.ndarray
@@ -434,6 +437,13 @@ class Alchemy:
     intramol: bool = False
     ljintra: bool = False
     opls: bool = False
+    shifted: bool = False
+    qshift: float = 4.0
+    ljshift: float = 0.35
+
+    def soft_params(self):
+        """(q, lj) parameters of the selected form."""
+        return (self.qshift, self.ljshift) if self.shifted else (self.qsoft, self.ljsoft)
 
 
 def lambda_state(lrm, lrm0=None, lrmq=None, lrmq0=None, ladd=1.0, ladd0=None, laddq=None, laddq0=None):

@@ -461,7 +461,8 @@ def force_alchemy(s, al, lam, nnlpp, *, rinn, rout, rtolinn, rtolout, rneigh, iz
         C.c_long(cap), C.c_double(rneigh), C.c_double(rinn), C.c_double(rout), C.byref(rti), C.c_double(rtolout),
         _p(_f(s.pmass)), C.c_int(iz), C.byref(nc1), _p(_f(s.mass)), C.c_double(massinfty), _p(_f(lam)),
         C.c_int(int(al.intramol)), C.c_int(int(al.ljintra)), C.c_int(int(al.opls)), C.byref(dw),
-        C.c_int32(int(lskip_ewald)), C.c_int32(int(ewald)), C.c_double(al.qsoft), C.c_double(al.ljsoft))
+        C.c_int32(int(lskip_ewald)), C.c_int32(int(ewald)), C.c_double(al.soft_params()[0]),
+        C.c_double(al.soft_params()[1]), C.c_int(int(al.shifted)))
     if rc != 0:
         raise RuntimeError("orc_force_alchemy: nnlpp1 capacity exceeded")
     out = dict(zip(E6, e6.tolist()))
@@ -530,15 +531,18 @@ def ref_force_alchemy(s, al, lam, nnlpp, *, rinn, rout, rtolinn, rtolout, rneigh
             np.zeros(s.ngrp, dtype=np.int32), _f(al.rmin), _f(al.epsm)]
     (ss, x, y, z, q, nbt, atg, atp, co, e12, e6, nl, pm, ms, groupp, rmin, epsm) = keep
 
+    fn = R.force_alchemy_shift_ if al.shifted else R.force_alchemy_
+    qx, ljx = al.soft_params()
+
     def call():
-        R.force_alchemy_(_p(ss), _p(x), _p(y), _p(z), _p(fa.xpg), _p(fa.ypg), _p(fa.zpg), _p(q), _p(nbt),
+        fn(_p(ss), _p(x), _p(y), _p(z), _p(fa.xpg), _p(fa.ypg), _p(fa.zpg), _p(q), _p(nbt),
                          _p(fa.tt), _ri(s.ntypes), _ri(n), _p(atg), _p(fa.xpcm), _p(fa.ypcm), _p(fa.zpcm),
                          _p(groupp), _p(atp), _p(co), _p(e12), _p(e6), _rd(s.alpha), _p(fa.mapnl), _ri(s.ngrp),
                          _p(fa.grppt), *[C.byref(v) for v in e], _p(fpx), _p(fpy), _p(fpz), _p(st), _p(nl),
                          _p(nn1), _rd(rneigh), _rd(rinn), _rd(rout), C.byref(rti), _rd(rtolout), _p(pm), _ri(iz),
                          C.byref(nc1), _p(ms), _rd(massinfty), _p(lam_dummy), _p(lam_dummy), _p(lam_dummy),
                          _p(lam_dummy), _p(rmin), _p(epsm), C.byref(dw), _ri(int(lskip_ewald)), _ri(int(ewald)),
-                         _rd(al.qsoft), _rd(al.ljsoft))
+                         _rd(qx), _rd(ljx))
 
     _big_stack(call)
     out = {k: v.value for k, v in zip(E6, e)}

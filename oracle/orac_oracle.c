@@ -1290,13 +1290,18 @@ void orc_alch_set(double *fact, double ladd, double lrm, int intramol, int ljint
 #undef AF
 }
 
+/* x**6 with an integer exponent, as the compiled Fortran evaluates it (five multiplications, left to right) */
+static double ipow6(double x) {
+  return x * x * x * x * x * x;
+}
+
 /* force_alchemy.f:147-730, serial build.  The module state the routine reads
  * arrives as arguments: ss_alchemy (cpropar), lam = {ladd, lrm, ladd0, lrm0,
  * laddq, lrmq, laddq0, lrmq0}, intramol, ljintra, opls (ffield_type = 'opls').
  * e6 = uconf_slt, uconf_slv, uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss.  As in
  * the serial build, dwrk is added to (not reset) and energies are written only
  * when iz = 1 and rinn0 < rout. */
-int orc_force_alchemy(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+static int force_alchemy_impl(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
                       const double *xpg, const double *ypg, const double *zpg, const double *charge,
                       const int32_t *ss_alchemy, const double *rmin, const double *epsm, int nato,
                       const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *atomp,
@@ -1305,7 +1310,11 @@ int orc_force_alchemy(const int32_t *ss_index, const double *xp0, const double *
                       int32_t *nnlpp1, long nnlpp1_cap, double rneigh, double rinn, double rout, double *rtolinn,
                       double rtolout, const double *gmass, int iz, int32_t *ncount1, const double *mass,
                       double massinfty, const double *lam, int intramol, int ljintra, int opls, double *dwrk,
-                      int32_t lskip_ewald, int32_t ewald, double qsoft, double ljsoft) {
+                      int32_t lskip_ewald, int32_t ewald, double qsoft, double ljsoft, int shifted) {
+  /* shifted: force_alchemy_shift (force_alchemy_shifted.f:1-730), with qsoft/ljsoft
+   * standing for qshift/ljshift; two6 = 2**(1./6.) with the REAL(4) exponent
+   * (force_alchemy_shifted.f:178) */
+  const double two6 = pow(2.0, (double)(1.0f / 6.0f));
   const double r2inn = rinn * rinn, r2out = rout * rout;
   const double rinn0 = rinn + *rtolinn, r2inn0 = rinn0 * rinn0;
   if (rinn < 1.0e-5) *rtolinn = 0.0;
@@ -1425,12 +1434,51 @@ int orc_force_alchemy(const int32_t *ss_index, const double *xp0, const double *
             const double yc = co[4] * yg + co[7] * zg;
             const double zc = co[8] * zg;
             const double rsq = xc * xc + yc * yc + zc * zc;
+            const double sw = swrs[jj];
+            double qforce;
+            if (shifted) {
+              /* force_alchemy_shifted.f:469-509 */
+              const double r = sqrt(rsq);
+              if (epsmin > 1.0e-12) {
+                const double rl = r + ljsoft * sigma * two6 * F[0];
+                const double rl0 = r + ljsoft * sigma * two6 * F0[0];
+                const double r6 = ipow6(sigma / rl), r60 = ipow6(sigma / rl0);
+                const double ssvirr = F[1] * epsmin * (12.0 * r6 * r6 - 6.0 * r6);
+                const double auxc = epsmin * (r6 * r6 - r6), auxc0 = epsmin * (r60 * r60 - r60);
+                const double ucon = F[1] * auxc, ucon0 = F0[1] * auxc0;
+                qforce = ssvirr * sw / (r * rl);
+                cmap2[jj] = cmap2[jj] + ucon;
+                uconf[typeij - 1] = uconf[typeij - 1] + sw * ucon;
+                uconf0[typeij - 1] = uconf0[typeij - 1] + sw * ucon0;
+              } else {
+                qforce = 0.0;
+              }
+              const double rq = r + qsoft * Q[0], rq0 = r + qsoft * Q0[0];
+              if (!lskip_ewald) {
+                const double alphar = alphal * r;
+                const double qt = 1.0 / (1.0 + QP * alphar);
+                const double expcst = exp(-alphar * alphar);
+                const double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+                const double furpar = Q[1] * chrgei * charge[j - 1];
+                const double furpar0 = Q0[1] * chrgei * charge[j - 1];
+                ucoul[typeij - 1] = ucoul[typeij - 1] + sw * furpar * erfcst / rq;
+                ucoul0[typeij - 1] = ucoul0[typeij - 1] + sw * furpar0 * erfcst / rq0;
+                cmap2[jj] = cmap2[jj] + furpar * erfcst / rq;
+                qforce = qforce + furpar * (erfcst * sw / (rq * rq * r) + twrtpi * alphar * expcst * sw / (r * r * rq));
+              } else if (!ewald) {
+                const double ssvir = Q[1] * chrgei * charge[j - 1] / rq;
+                const double ssvir0 = Q0[1] * chrgei * charge[j - 1] / rq0;
+                cmap2[jj] = cmap2[jj] + ssvir;
+                ucoul[typeij - 1] = ucoul[typeij - 1] + ssvir * sw;
+                ucoul0[typeij - 1] = ucoul0[typeij - 1] + ssvir0 * sw;
+                qforce = qforce + ssvir * sw / (rq * r);
+              }
+              goto accumulate;
+            }
             const double auxa = 1.0 / sqrt(rsq);
             const double auxaa = 1.0 / sqrt(qsoft * pow(Q[0], 2.0) + rsq);
             const double auxa0 = 1.0 / sqrt(qsoft * pow(Q0[0], 2.0) + rsq);
             const double auxbb = auxaa * auxaa * auxaa;
-            const double sw = swrs[jj];
-            double qforce;
             if (epsmin > 1.0e-12) {
               const double r6 = rsq * rsq * rsq / pow(sigma, 6.0);
               const double gp = ljsoft * F[0] + r6, gp1 = 1.0 / gp, gp2 = gp1 * gp1;
@@ -1468,6 +1516,7 @@ int orc_force_alchemy(const int32_t *ss_index, const double *xp0, const double *
               ucoul0[typeij - 1] = ucoul0[typeij - 1] + ssvir0 * sw;
               qforce = qforce + ssvir * sw * auxaa * auxaa;
             }
+          accumulate:;
             const double emvir = qforce;
             fpx[i1 - 1] = fpx[i1 - 1] + qforce * xc;
             fpy[i1 - 1] = fpy[i1 - 1] + qforce * yc;
@@ -1537,6 +1586,23 @@ int orc_force_alchemy(const int32_t *ss_index, const double *xp0, const double *
     *dwrk = *dwrk + auxa + auxb;
   }
   return 0;
+}
+
+int orc_force_alchemy(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                      const double *xpg, const double *ypg, const double *zpg, const double *charge,
+                      const int32_t *ss_alchemy, const double *rmin, const double *epsm, int nato,
+                      const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *atomp,
+                      const double *co, double alphal, const int32_t *mapnl, int ngrp, const int32_t *grppt,
+                      double *e6, double *fpx, double *fpy, double *fpz, double *stress, const int32_t *nnlpp,
+                      int32_t *nnlpp1, long nnlpp1_cap, double rneigh, double rinn, double rout, double *rtolinn,
+                      double rtolout, const double *gmass, int iz, int32_t *ncount1, const double *mass,
+                      double massinfty, const double *lam, int intramol, int ljintra, int opls, double *dwrk,
+                      int32_t lskip_ewald, int32_t ewald, double qsoft, double ljsoft, int shifted) {
+  return force_alchemy_impl(ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, ss_alchemy, rmin, epsm, nato, xpcm,
+                            ypcm, zpcm, atomp, co, alphal, mapnl, ngrp, grppt, e6, fpx, fpy, fpz, stress, nnlpp,
+                            nnlpp1, nnlpp1_cap, rneigh, rinn, rout, rtolinn, rtolout, gmass, iz, ncount1, mass,
+                            massinfty, lam, intramol, ljintra, opls, dwrk, lskip_ewald, ewald, qsoft, ljsoft,
+                            shifted);
 }
 
 /* ferrf_alchemy.f:1-204 (the ERF_CORR spline branches are not restated):

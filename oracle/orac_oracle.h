@@ -122,7 +122,7 @@ int orc_force_alchemy(const int32_t *ss_index, const double *xp0, const double *
                       int32_t *nnlpp1, long nnlpp1_cap, double rneigh, double rinn, double rout, double *rtolinn,
                       double rtolout, const double *gmass, int iz, int32_t *ncount1, const double *mass,
                       double massinfty, const double *lam, int intramol, int ljintra, int opls, double *dwrk,
-                      int32_t lskip_ewald, int32_t ewald, double qsoft, double ljsoft);
+                      int32_t lskip_ewald, int32_t ewald, double qsoft, double ljsoft, int shifted);
 int orc_ferrf_alchemy(const int32_t *ss_index, double alphal, const double *charge, const double *charge0,
                       double fudge, const double *x0, const double *y0, const double *z0, int iz,
                       const int32_t *list, int nlist, double *fs_slt, double *fs_slv, double *fpx, double *fpy,

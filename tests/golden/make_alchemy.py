@@ -3,8 +3,8 @@
 The system is orac_hip.synth.small_solvated() (40-atom solute in 380 waters)
 with its first 12 solute atoms removed alchemically (synth.alchemy_for).  For
 each lambda state below the script runs the reference Fortran compiled in
-place (oracle/_ref): force_alchemy (serial build) on the single outer shell
-and on a switched inner + outer shell pair, ferrf_alchemy on the solute's
+place (oracle/_ref): force_alchemy and force_alchemy_shift (serial build) on
+the single outer shell and on a switched inner + outer shell pair, ferrf_alchemy on the solute's
 bond and 1-3 pairs (iz = 1 and 0), cself_alchemy, and fft_pme_alchemy
 (fftpack branch), and writes tests/golden/alchemy.npz: energies, dwrk and
 stress as values, forces of the direct routines as SHA-256 digests of their
@@ -12,6 +12,7 @@ bytes, PME forces as values.
 
     python tests/golden/make_alchemy.py
 """
+import dataclasses
 import hashlib
 import os
 import sys
@@ -74,13 +75,14 @@ def main():
     for name, *_ in STATES:
         lam, intra, ljin = state(name)
         al = synth.alchemy_for(s, atoms, intramol=intra, ljintra=ljin)
-        for sh, kw in SHELLS:
-            kw = dict(kw)
-            kw.setdefault("rout", s.rcut)
-            r = O.ref_force_alchemy(s, al, lam, nn, fa=fa, **kw)
-            out[f"force_{name}_{sh}_e"] = np.array([r[k] for k in E6] + [r["dwrk"]])
-            out[f"force_{name}_{sh}_stress"] = r["stress"]
-            out[f"force_{name}_{sh}_f_sha"] = digest(r["f"])
+        for form, alf in (("force", al), ("shift", dataclasses.replace(al, shifted=True))):
+            for sh, kw in SHELLS:
+                kw = dict(kw)
+                kw.setdefault("rout", s.rcut)
+                r = O.ref_force_alchemy(s, alf, lam, nn, fa=fa, **kw)
+                out[f"{form}_{name}_{sh}_e"] = np.array([r[k] for k in E6] + [r["dwrk"]])
+                out[f"{form}_{name}_{sh}_stress"] = r["stress"]
+                out[f"{form}_{name}_{sh}_f_sha"] = digest(r["f"])
         q = synth.alchemy_charges(s.charge, al.ss_alchemy, lam[4], lam[5])
         q0 = synth.alchemy_charges(s.charge, al.ss_alchemy, lam[6], lam[7])
         for iz in (1, 0):

@@ -72,3 +72,4 @@ def test_alchemy_shim_compiles_against_the_reference_modules(tmp_path):
                     os.path.join(src, "orac_hip_shim_alchemy.f90"), "-o", str(tmp_path / "alch.o")], check=True)
     syms = subprocess.run(["nm", str(tmp_path / "alch.o")], check=True, capture_output=True, text=True).stdout
     assert re.search(r"\bT force_alchemy_\b", syms) and "orac_hip_force_alchemy" in syms
+    assert re.search(r"\bT force_alchemy_shift_\b", syms) and "orac_hip_force_alchemy_shift" in syms

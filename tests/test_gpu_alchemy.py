@@ -46,12 +46,14 @@ def _gpu_force_alchemy(ctx, s, al, lam, nnlpp, fa, *, rinn, rout, rtolinn, rtolo
     return r
 
 
+@pytest.mark.parametrize("shifted", [False, True])
 @pytest.mark.parametrize("sh", ["outer", "switched"])
 @pytest.mark.parametrize("name", [st[0] for st in G.STATES])
-def test_force_alchemy_entry(gpu_ctx, alch_sys, name, sh):
+def test_force_alchemy_entry(gpu_ctx, alch_sys, name, sh, shifted):
+    """orac_hip_force_alchemy (soft core) and orac_hip_force_alchemy_shift."""
     s, atoms, fa, nn, synth = alch_sys
     lam, intra, ljin = G.state(name)
-    al = synth.alchemy_for(s, atoms, intramol=intra, ljintra=ljin)
+    al = synth.alchemy_for(s, atoms, intramol=intra, ljintra=ljin, shifted=shifted)
     kw = dict(dict(G.SHELLS)[sh])
     kw.setdefault("rout", s.rcut)
     g = _gpu_force_alchemy(gpu_ctx, s, al, lam, nn, fa, **kw)
@@ -88,13 +90,13 @@ def oracle_alchemy_total(s, al, lam, cp, cf, synth):
                 dwrk=d["dwrk"] + p["dwrk"] + cdw + sdw)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_resident_alchemy_matches_oracle(gpu_ctx, alch_sys, precision):
+@pytest.mark.parametrize("precision,shifted", [(0, False), (1, False), (0, True), (1, True)])
+def test_resident_alchemy_matches_oracle(gpu_ctx, alch_sys, precision, shifted):
     from orac_hip.api import ResidentSystem
     s, atoms, _, _, synth = alch_sys
     cp = np.concatenate([s.bonds, s.bends], axis=1)
     cf = np.ones(cp.shape[1])
-    al = synth.alchemy_for(s, atoms)
+    al = synth.alchemy_for(s, atoms, shifted=shifted)
     rs = ResidentSystem(gpu_ctx, s, precision=precision, ncell=2, corr_pairs=cp, corr_fudge=cf)
     rs.set_alchemy(al)
     for lrm, lrm0, lrmq, lrmq0 in [(0.35, 0.4, 0.6, 0.55), (0.0, 0.05, 0.0, 0.05), (1.0, 0.9, 1.0, 0.9)]:

@@ -55,6 +55,19 @@ def test_force_alchemy_matches_reference(ctx, name, sh):
     assert np.array_equal(digest(r["f"]), g[f"force_{name}_{sh}_f_sha"])
 
 
+@pytest.mark.parametrize("sh", ["outer", "switched"])
+@pytest.mark.parametrize("name", [st[0] for st in G.STATES])
+def test_force_alchemy_shift_matches_reference(ctx, name, sh):
+    """The shifted-potential form (force_alchemy_shift, SHIFT-POT)."""
+    s, atoms, fa, nn, g = ctx
+    lam, intra, ljin = G.state(name)
+    al = synth.alchemy_for(s, atoms, intramol=intra, ljintra=ljin, shifted=True)
+    r = O.force_alchemy(s, al, lam, nn, fa=fa, **_shell(s, sh))
+    assert np.array_equal(np.array([r[k] for k in G.E6] + [r["dwrk"]]), g[f"shift_{name}_{sh}_e"])
+    assert np.array_equal(r["stress"], g[f"shift_{name}_{sh}_stress"])
+    assert np.array_equal(digest(r["f"]), g[f"shift_{name}_{sh}_f_sha"])
+
+
 @pytest.mark.parametrize("name", [st[0] for st in G.STATES])
 def test_corrections_and_pme_match_reference(ctx, name):
     s, atoms, fa, nn, g = ctx
@@ -90,12 +103,13 @@ def test_alch_set_tables():
     assert tuple(fj[-4 + 4]) == (0.0, 1.0)
 
 
-def test_full_coupling_is_the_plain_pair(ctx):
-    """At lambda = 0 every factor is (0, 1): the soft-core pair is force.f's
-    LJ + erfc pair (sigma = rmin_i + rmin_j, 4 sqrt(eps_i eps_j) reproduce the
-    Lorentz-Berthelot tables), up to rounding."""
+@pytest.mark.parametrize("shifted", [False, True])
+def test_full_coupling_is_the_plain_pair(ctx, shifted):
+    """At lambda = 0 every factor is (0, 1): the soft-core (or shifted) pair is
+    force.f's LJ + erfc pair (sigma = rmin_i + rmin_j, 4 sqrt(eps_i eps_j)
+    reproduce the Lorentz-Berthelot tables), up to rounding."""
     s, atoms, fa, nn, _ = ctx
-    al = synth.alchemy_for(s, atoms)
+    al = synth.alchemy_for(s, atoms, shifted=shifted)
     kw = _shell(s, "switched")
     a = O.force_alchemy(s, al, synth.lambda_state(0.0), nn, fa=fa, **kw)
     b = O.force(s, nn, fa=fa, **kw)
@@ -120,12 +134,13 @@ def test_dwrk_is_the_energy_step(ctx):
     assert abs(r["dwrk"] - du) <= 1e-12 * abs(u(0.3, 0.5))
 
 
-def test_forces_are_minus_the_energy_gradient(ctx):
-    """Central differences of the total direct energy (soft-core LJ and
-    Coulomb at lambda = 0.5, switched shells) against the forces on
+@pytest.mark.parametrize("shifted", [False, True])
+def test_forces_are_minus_the_energy_gradient(ctx, shifted):
+    """Central differences of the total direct energy (soft-core or shifted LJ
+    and Coulomb at lambda = 0.5, switched shells) against the forces on
     alchemical and plain atoms; the group switch makes the energy smooth."""
     s, atoms, fa, nn, _ = ctx
-    al = synth.alchemy_for(s, atoms)
+    al = synth.alchemy_for(s, atoms, shifted=shifted)
     lam = synth.lambda_state(0.5, lrmq=0.4)
     kw = _shell(s, "switched")
     base = O.force_alchemy(s, al, lam, nn, fa=fa, **kw)
