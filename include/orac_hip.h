@@ -162,6 +162,28 @@ int orac_hip_ferrf(orac_hip_ctx *ctx, const int32_t *ss_index, double alphal, co
 int orac_hip_cself(orac_hip_ctx *ctx, const int32_t *ss_index, int32_t nato, double alphal, double kcut,
                    const double *charge, double *self_slt, double *self_slv);
 
+/* Replaces fptors (fptors.f:1-345) as mts_intra_n1.f:118-119 calls it:
+ * proper torsions ltor(4, ltors) on the coordinates as given, pota the
+ * signed multiplicity (1-8, 12, 14) and potb the barrier; mb atoms.  Forces
+ * accumulate, energies (by the ss_index of each torsion's first atom) are
+ * overwritten.  A torsion with collinear bond angles returns 5 (the
+ * reference perturbs its atoms at random); a multiplicity outside the
+ * reference's table returns 4 (the reference stops). */
+int orac_hip_fptors(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *ltor, int32_t ltors, const double *xp0,
+                    const double *yp0, const double *zp0, const double *pota, const double *potb, double *utors_slt,
+                    double *utors_slv, double *fpx, double *fpy, double *fpz, int32_t mb);
+
+/* Replaces fnb14 (fnb14.f:1-219): the 1-4 pairs int14(2, int14p) on the
+ * coordinates as given (no PBC), LJ with the packed index type14(i) scaled
+ * by lj_fudge, Coulomb scaled by fudge, erfc (A&S 7.1.26) when ewald and
+ * cut are both set and plain Coulomb otherwise.  Forces accumulate,
+ * energies (by the ss_index of the pair's first atom) are overwritten. */
+int orac_hip_fnb14(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                   const double *zp0, const double *charge, int32_t nato, const double *ecc12, const double *ecc6,
+                   int32_t cut, int32_t ewald, double alphal, const int32_t *int14, int32_t int14p,
+                   const int32_t *type14, double fudge, double lj_fudge, double *fpx, double *fpy, double *fpz,
+                   double *uconf_slt, double *uconf_slv, double *ucoul_slt, double *ucoul_slv);
+
 /* Replaces fnbgrp (fnbgrp.f:1-213), serial layout: intra-group
  * non-excluded pairs ingrp(2, ingrpp), LJ + Ewald (or plain Coulomb). */
 int orac_hip_fnbgrp(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,

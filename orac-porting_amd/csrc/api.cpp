@@ -692,6 +692,135 @@ extern "C" int orac_hip_cself(orac_hip_ctx *ctx, const int32_t *ss_index, int32_
   });
 }
 
+extern "C" int orac_hip_fptors(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *ltor, int32_t ltors,
+                               const double *xp0, const double *yp0, const double *zp0, const double *pota,
+                               const double *potb, double *utors_slt, double *utors_slv, double *fpx, double *fpy,
+                               double *fpz, int32_t mb) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(utors_slt && utors_slv, E_ARG, "null output");
+    *utors_slt = *utors_slv = 0.0;
+    if (ltors == 0) return;
+    ORAC_REQUIRE(ss_index && ltor && xp0 && yp0 && zp0 && pota && potb && fpx && fpy && fpz && mb > 0 && ltors > 0,
+                 E_ARG, "null argument");
+    hipStream_t st = ctx->stream;
+    BufSet &B = ctx->B;
+    std::vector<int4> at(ltors);
+    std::vector<std::vector<int>> per(mb);
+    for (int i = 0; i < ltors; ++i) {
+      int l[4];
+      for (int k = 0; k < 4; ++k) {
+        l[k] = ltor[4 * i + k] - 1;
+        ORAC_REQUIRE(l[k] >= 0 && l[k] < mb, E_ARG, "fptors: torsion atom out of range");
+        per[l[k]].push_back(4 * i + k);
+      }
+      at[i] = make_int4(l[0], l[1], l[2], l[3]);
+    }
+    for (int a = 0; a < mb; ++a) ORAC_REQUIRE(ss_index[a] == 1 || ss_index[a] == 2, E_ARG, "ss_index must be 1 or 2");
+    std::vector<int> goff(mb + 1, 0), gs;
+    for (int a = 0; a < mb; ++a) {
+      for (int v : per[a]) gs.push_back(v);
+      goff[a + 1] = (int)gs.size();
+    }
+    TorsDesc h;
+    std::memset(&h, 0, sizeof h);
+    h.ntors = ltors;
+    h.natoms = mb;
+    h.atoms = B.put_vec("t_at", at, st, false);
+    h.pota = B.put("t_pa", pota, ltors, st, false);
+    h.potb = B.put("t_pb", potb, ltors, st, false);
+    h.x = B.put("t_x", xp0, mb, st, false);
+    h.y = B.put("t_y", yp0, mb, st, false);
+    h.z = B.put("t_z", zp0, mb, st, false);
+    h.ss = B.put("t_ss", ss_index, mb, st, true);
+    h.tf = B.get<double>("t_tf", 12 * (size_t)ltors);
+    h.te = B.get<double>("t_te", 2 * (size_t)ltors);
+    h.status = B.get<int>("t_status", 2);
+    launch_zero_u64(reinterpret_cast<unsigned long long *>(h.status), 1, st);
+    h.g_off = B.put_vec("t_goff", goff, st, false);
+    h.g_slot = B.put_vec("t_gs", gs, st, false);
+    h.fx = B.get<double>("t_fx", mb);
+    h.fy = B.get<double>("t_fy", mb);
+    h.fz = B.get<double>("t_fz", mb);
+    launch_zero_f64(h.fx, mb, st);
+    launch_zero_f64(h.fy, mb, st);
+    launch_zero_f64(h.fz, mb, st);
+    double *dout = B.get<double>("t_out", 2);
+    h.out = dout;
+    TorsDesc *dd = B.put("t_desc", &h, 1, st, false);
+    launch_tors(dd, h, st);
+    ORAC_HIP(hipGetLastError());
+    std::vector<double> fx(mb), fy(mb), fz(mb);
+    double e[2];
+    int status = 0;
+    ORAC_HIP(hipMemcpyAsync(fx.data(), h.fx, mb * sizeof(double), hipMemcpyDeviceToHost, st));
+    ORAC_HIP(hipMemcpyAsync(fy.data(), h.fy, mb * sizeof(double), hipMemcpyDeviceToHost, st));
+    ORAC_HIP(hipMemcpyAsync(fz.data(), h.fz, mb * sizeof(double), hipMemcpyDeviceToHost, st));
+    ORAC_HIP(hipMemcpyAsync(e, dout, sizeof e, hipMemcpyDeviceToHost, st));
+    ORAC_HIP(hipMemcpyAsync(&status, h.status, sizeof(int), hipMemcpyDeviceToHost, st));
+    ORAC_HIP(hipStreamSynchronize(st));
+    ORAC_REQUIRE(status != 2, E_UNSUPPORTED, "fptors: torsion multiplicity outside 1-8, 12, 14");
+    ORAC_REQUIRE(status != 1, E_STATE,
+                 "fptors: a torsion with collinear atoms (the reference perturbs it at random; not reproduced)");
+    for (int a = 0; a < mb; ++a) {
+      fpx[a] = fpx[a] + fx[a];
+      fpy[a] = fpy[a] + fy[a];
+      fpz[a] = fpz[a] + fz[a];
+    }
+    *utors_slt = e[0];
+    *utors_slv = e[1];
+  });
+}
+
+extern "C" int orac_hip_fnb14(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                              const double *zp0, const double *charge, int32_t nato, const double *ecc12,
+                              const double *ecc6, int32_t cut, int32_t ewald, double alphal, const int32_t *int14,
+                              int32_t int14p, const int32_t *type14, double fudge, double lj_fudge, double *fpx,
+                              double *fpy, double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt,
+                              double *ucoul_slv) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(uconf_slt && uconf_slv && ucoul_slt && ucoul_slv, E_ARG, "null output");
+    *uconf_slt = *uconf_slv = *ucoul_slt = *ucoul_slv = 0.0;
+    if (int14p == 0) return;  // fnb14.f:64
+    ORAC_REQUIRE(ss_index && xp0 && yp0 && zp0 && charge && ecc12 && ecc6 && int14 && type14 && nato > 0 &&
+                     int14p > 0,
+                 E_ARG, "null argument");
+    hipStream_t st = ctx->stream;
+    BufSet &B = ctx->B;
+    int lmax = 1;
+    for (int k = 0; k < int14p; ++k) {
+      ORAC_REQUIRE(type14[k] >= 1, E_ARG, "fnb14: type14 entries are 1-based packed indices");
+      lmax = std::max(lmax, type14[k]);
+    }
+    for (int a = 0; a < nato; ++a)
+      ORAC_REQUIRE(ss_index[a] == 1 || ss_index[a] == 2, E_ARG, "ss_index must be 1 or 2");
+    CorrDesc h;
+    std::memset(&h, 0, sizeof h);
+    h.xp = B.put("n_x", xp0, nato, st, false);
+    h.yp = B.put("n_y", yp0, nato, st, false);
+    h.zp = B.put("n_z", zp0, nato, st, false);
+    h.q = B.put("n_q", charge, nato, st, true);
+    h.ss = B.put("n_ss", ss_index, nato, st, true);
+    h.pair_type = B.put("n_t14", type14, int14p, st, true);
+    h.ecc12 = B.put("n_e12", ecc12, lmax, st, true);
+    h.ecc6 = B.put("n_e6", ecc6, lmax, st, true);
+    h.alpha = alphal;
+    h.twrtpi = 2.0 / std::sqrt(4.0 * std::atan(1.0));
+    h.fudge = fudge;
+    h.lj_fudge = lj_fudge;
+    h.iz = 1;
+    h.kind = 2;
+    h.ewald = (ewald && cut) ? 1 : 0;  // fnb14.f:67: plain Coulomb unless ewald and cut
+    double e[4];
+    run_corr_pairs(ctx, h, int14, int14p, nato, "n_", e, fpx, fpy, fpz);
+    *ucoul_slt = e[0];
+    *ucoul_slv = e[1];
+    *uconf_slt = e[2];
+    *uconf_slv = e[3];
+  });
+}
+
 extern "C" int orac_hip_fnbgrp(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
                                const double *zp0, const double *charge, const int32_t *type, const double *ecc12,
                                const double *ecc6, int32_t ewald, double alphal, const int32_t *ingrp,

@@ -173,7 +173,7 @@ struct CorrDesc {
   double alpha, twrtpi, fudge;
   const double *fudge_p;              // per-pair fudge (overrides fudge when set)
   int iz;                             // 1: forces + energy, 0: energy only
-  int kind;                           // 0: ferrf, 1: fnbgrp (LJ + Ewald/Coulomb)
+  int kind;                           // 0: ferrf, 1: fnbgrp (LJ + Ewald/Coulomb), 2: fnb14 (fudged 1-4 pairs)
   int ewald;
   const int *type;                    // fnbgrp: per-atom LJ type (1-based, as fnbgrp.f uses)
   const double *ecc12, *ecc6;
@@ -184,8 +184,27 @@ struct CorrDesc {
   double *fx, *fy, *fz;               // accumulated (+=)
   double *out;                        // [4]: coul slt, coul slv, lj slt, lj slv
   const double *q0;                   // ferrf_alchemy: lambda0 charges; en0 goes to the lj slt column
+  const int *pair_type;               // fnb14: packed LJ index of each pair (type14, 1-based)
+  double lj_fudge;                    // fnb14: LJ-FUDGE (fudge is QQ-FUDGE)
 };
 void launch_corr(const CorrDesc *d_desc, const CorrDesc *h, int nsys, hipStream_t st);
+
+// Proper torsions (fptors.f): one thread per torsion writes its four atoms'
+// forces and its energy; a per-atom gather sums them in a fixed order.
+struct TorsDesc {
+  int ntors, natoms;
+  const int4 *atoms;                  // 0-based l1..l4
+  const double *pota, *potb;          // signed multiplicity, barrier
+  const double *x, *y, *z;            // coordinates as given
+  const int *ss;                      // ss_index of each atom
+  double *tf;                         // [ntors][12] force components: x of l1..l4, y, z
+  double *te;                         // [ntors][2] energy by the class of l1
+  int *status;                        // [1]: 1 singular torsion, 2 unsupported multiplicity (sticky)
+  const int *g_off, *g_slot;          // per atom gather CSR: 4 * torsion + position
+  double *fx, *fy, *fz;               // accumulated (+=)
+  double *out;                        // [2]: utors slt, slv
+};
+void launch_tors(const TorsDesc *d_desc, const TorsDesc &h, hipStream_t st);
 
 // ---------------------------------------------------------------- integration
 

@@ -620,6 +620,31 @@ __global__ void __launch_bounds__(256) k_corr_pairs(const CorrDesc *__restrict__
       double furpar0 = d.iz == 1 ? fud * d.q0[ia] * d.q0[ib] : d.q0[ia] * d.q0[ib];
       elj = -(furpar0 * erfst / rsp);
     }
+  } else if (d.kind == 2) {
+    // fnb14.f:69-158: the coordinates as given, no PBC; LJ by the pair's packed
+    // type scaled by lj_fudge; charges by fudge; erfc only with ewald and cut
+    xab = d.xp[ia] - d.xp[ib];
+    yab = d.yp[ia] - d.yp[ib];
+    zab = d.zp[ia] - d.zp[ib];
+    rsq = xab * xab + yab * yab + zab * zab;
+    rsp = sqrt(rsq);
+    const int lij = d.pair_type[p] - 1;
+    const double rsqi = 1.0 / rsq;
+    const double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+    const double ssvir = 12.0 * d.ecc12[lij] * r12 - 6.0 * d.ecc6[lij] * r6;
+    elj = d.lj_fudge * (d.ecc12[lij] * r12 - d.ecc6[lij] * r6);
+    if (d.ewald) {
+      double alphar = d.alpha * rsp;
+      double qt = 1.0 / (1.0 + AS_QP * alphar);
+      double expcst = exp(-(alphar * alphar));
+      double erfcst = ((((AS_A5 * qt + AS_A4) * qt + AS_A3) * qt + AS_A2) * qt + AS_A1) * qt * expcst;
+      const double furpar = d.fudge * d.q[ia] * d.q[ib];
+      ecoul = furpar * erfcst / rsp;
+      qforce = d.lj_fudge * ssvir * rsqi + furpar * (erfcst + d.twrtpi * alphar * expcst) * (rsqi / rsp);
+    } else {
+      ecoul = d.fudge * d.q[ia] * d.q[ib] / rsp;
+      qforce = d.lj_fudge * ssvir * rsqi + ecoul * rsqi;
+    }
   } else {
     // fnbgrp.f:120-202 (reduced coordinates, no PBC)
     xab = d.xp[ia] - d.xp[ib];

@@ -53,7 +53,7 @@ EXPORTS = [
     "orac_hip_create", "orac_hip_destroy", "orac_hip_last_error", "orac_hip_invalidate", "orac_hip_device_name",
     "orac_hip_probe_copy",
     "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
-    "orac_hip_force_alchemy", "orac_hip_force_alchemy_shift",
+    "orac_hip_force_alchemy", "orac_hip_force_alchemy_shift", "orac_hip_fptors", "orac_hip_fnb14",
     "orac_hip_decide_force", "orac_hip_ferrf", "orac_hip_cself", "orac_hip_fnbgrp", "orac_hip_system_create",
     "orac_hip_system_destroy", "orac_hip_system_set_coords", "orac_hip_system_device_ptrs", "orac_hip_system_eval",
     "orac_hip_system_results", "orac_hip_system_profile", "orac_hip_system_stage_ms", "orac_hip_system_bytes",
@@ -230,6 +230,31 @@ class Context:
         ladd0, lrm0, laddq, lrmq, laddq0, lrmq0) and the incoming dwrk.  The
         returned dict adds dwrk (incoming value + U(lambda) - U(lambda0))."""
         return self.force(*args, alchemy=(alchemy, lam, dwrk), **kw)
+
+    # -- fptors.f
+    def fptors(self, ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, fpx, fpy, fpz, mb):
+        """Returns (utors_slt, utors_slv); fpx/fpy/fpz accumulated in place."""
+        a, b = C.c_double(), C.c_double()
+        fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
+        keep = [_i(ss_index), _i(ltor), _f(xp0), _f(yp0), _f(zp0), _f(pota), _f(potb)]
+        _check(lib().orac_hip_fptors(self._h, _p(keep[0]), _p(keep[1]), C.c_int32(ltors), _p(keep[2]), _p(keep[3]),
+                                     _p(keep[4]), _p(keep[5]), _p(keep[6]), C.byref(a), C.byref(b), _p(fpx), _p(fpy),
+                                     _p(fpz), C.c_int32(mb)))
+        return a.value, b.value
+
+    # -- fnb14.f
+    def fnb14(self, ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, cut, ewald, alphal, int14, int14p, type14,
+              fudge, lj_fudge, fpx, fpy, fpz):
+        """Returns (uconf_slt, uconf_slv, ucoul_slt, ucoul_slv); forces accumulated in place."""
+        e = [C.c_double() for _ in range(4)]
+        fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
+        keep = [_i(ss_index), _f(xp0), _f(yp0), _f(zp0), _f(charge), _f(ecc12), _f(ecc6), _i(int14), _i(type14)]
+        _check(lib().orac_hip_fnb14(self._h, _p(keep[0]), _p(keep[1]), _p(keep[2]), _p(keep[3]), _p(keep[4]),
+                                    C.c_int32(nato), _p(keep[5]), _p(keep[6]), C.c_int32(int(bool(cut))),
+                                    C.c_int32(int(bool(ewald))), C.c_double(alphal), _p(keep[7]), C.c_int32(int14p),
+                                    _p(keep[8]), C.c_double(fudge), C.c_double(lj_fudge), _p(fpx), _p(fpy), _p(fpz),
+                                    *[C.byref(v) for v in e]))
+        return tuple(v.value for v in e)
 
     # -- mts_forces.f:212
     @staticmethod

@@ -20,7 +20,7 @@ CSRC = os.path.join(ROOT, "csrc")
 LIBDIR = os.path.join(ROOT, "lib")
 LIB = os.path.join(LIBDIR, "liborac_hip.so")
 FLIB = os.path.join(LIBDIR, "liborac_hip_fortran.so")
-SOURCES = ["pme.hip", "pairs.hip", "force_pairs.hip", "md.hip", "host.cpp", "api.cpp", "engine.cpp"]
+SOURCES = ["pme.hip", "pairs.hip", "force_pairs.hip", "bonded.hip", "md.hip", "host.cpp", "api.cpp", "engine.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ORAC_HIP_ARCH", "gfx950")
 

@@ -76,7 +76,9 @@ def read_tpgprm(path, amphi=False):
     """Topology-parameter dump, readtp.f:45-150 (the non-amphi potential record
     unless amphi).  Arrays come back 1-based as stored, Fortran layouts as
     (rows, n): grppt (2, ngrp), lbnd (2, lbond), lbndg (3, lbend), int14 (2, int14p),
-    int13 (2, int13p).  Charges are in electrons, masses in amu."""
+    int13 (2, int13p), ltor (4, ltors); potto is (ltors, 2): barrier, then the signed
+    multiplicity (fptors receives potto(1,2) as pota and potto(1,1) as potb,
+    mts_intra_n1.f:118-119).  Charges are in electrons, masses in amu."""
     r = _ListDirected(path)
     h = dict(zip(_HEADER, r.values(len(_HEADER))))
     n = {k: int(v) for k, v in h.items() if v not in ("T", "F")}
@@ -93,7 +95,7 @@ def read_tpgprm(path, amphi=False):
     t["lbnd"] = r.ints(2 * n["lbond"]).reshape(2, n["lbond"])
     r.counted_lists(n["ntap"])                                     # concta
     t["lbndg"] = r.ints(3 * n["lbend"]).reshape(3, n["lbend"])
-    r.ints(4 * n["ltors"])
+    t["ltor"] = r.ints(4 * n["ltors"]).reshape(4, n["ltors"])      # ((ltor(j,i),i=1,ltors),j=1,4)
     r.ints(4 * n["litor"])
     v = r.ints(3 * n["int14p"] + 2 * n["int13p"] + n["lpnbd"] ** 2)
     t["int14"] = v[:2 * n["int14p"]].reshape(n["int14p"], 2).T
@@ -109,6 +111,9 @@ def read_tpgprm(path, amphi=False):
     npot = 2 * n["lbond"] + 7 * n["lbend"] + (4 if amphi else 2) * n["ltors"] + 3 * n["litor"] + \
         (6 if amphi else 4) * L + 2 * n["lphyd"]
     p = r.reals(npot)
+    if not amphi:                                                  # potto(ltors, 2): barrier, multiplicity
+        o = 2 * n["lbond"] + 7 * n["lbend"]
+        t["potto"] = p[o:o + 2 * n["ltors"]].reshape(2, n["ltors"]).T
     o = 2 * n["lbond"] + 7 * n["lbend"] + (4 if amphi else 2) * n["ltors"] + 3 * n["litor"]
     t["pnbd"] = p[o:o + 4 * L].reshape(4, L)                       # pnbd1..4 (readtp.f:140-143)
     if iz == 1:

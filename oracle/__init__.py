@@ -596,3 +596,63 @@ def ref_fft_pme_alchemy(s, charge, charge0, pressure=True, rkcut=1e9):
     _big_stack(call)
     return dict(eer=eer.value, dwrk=dw.value, eer0=eer.value - dw.value, f=np.stack([fpx, fpy, fpz]),
                 stress=st.reshape(3, 3, order="F"))
+
+
+# ---------------------------------------------------------------- 1-4 pairs
+
+FNB_E = ("uconf_slt", "uconf_slv", "ucoul_slt", "ucoul_slv")
+
+
+def fnb14(s, int14, type14, fudge, lj_fudge, ewald=True, cut=True):
+    """fnb14.f on the Cartesian coordinates of System s: pairs int14 (2, n)
+    1-based, packed LJ indices type14 (1-based)."""
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    e = [C.c_double() for _ in range(4)]
+    lst, t14 = _i(int14), _i(type14)
+    lib().orc_fnb14(_p(_i(s.ss_index)), _p(_f(x)), _p(_f(y)), _p(_f(z)), _p(_f(s.charge)), C.c_int(n),
+                    _p(_f(s.ecc12)), _p(_f(s.ecc6)), C.c_int(int(cut)), C.c_int(int(ewald)), C.c_double(s.alpha),
+                    _p(lst), C.c_int(lst.size // 2), _p(t14), C.c_double(fudge), C.c_double(lj_fudge), _p(fpx),
+                    _p(fpy), _p(fpz), *[C.byref(v) for v in e])
+    return dict(zip(FNB_E, [v.value for v in e]), f=np.stack([fpx, fpy, fpz]))
+
+
+def ref_fnb14(s, int14, type14, fudge, lj_fudge, ewald=True, cut=True):
+    R = ref()
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    e = [C.c_double() for _ in range(4)]
+    keep = [_i(s.ss_index), _f(x), _f(y), _f(z), _f(s.charge), _f(s.ecc12), _f(s.ecc6), _i(int14), _i(type14)]
+    R.fnb14_(_p(keep[0]), _p(keep[1]), _p(keep[2]), _p(keep[3]), _p(keep[4]), _ri(n), _p(keep[5]), _p(keep[6]),
+             _ri(int(cut)), _ri(int(ewald)), _rd(s.alpha), _p(keep[7]), _ri(keep[7].size // 2), _p(keep[8]),
+             _rd(fudge), _rd(lj_fudge), _p(fpx), _p(fpy), _p(fpz), *[C.byref(v) for v in e], _ri(1))
+    return dict(zip(FNB_E, [v.value for v in e]), f=np.stack([fpx, fpy, fpz]))
+
+
+def fptors(s, ltor, pota, potb, coords=None):
+    """fptors.f on the Cartesian coordinates of System s (or coords = (x, y, z)):
+    torsions ltor (4, n) 1-based, pota = signed multiplicity, potb = barrier.
+    Returns utors_slt, utors_slv, f and status (1: a singular torsion was
+    skipped, 2: unsupported multiplicity)."""
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b = C.c_double(), C.c_double()
+    lt = _i(ltor)
+    rc = lib().orc_fptors(_p(_i(s.ss_index)), _p(lt), C.c_int(lt.size // 4), _p(_f(x)), _p(_f(y)), _p(_f(z)),
+                          _p(_f(pota)), _p(_f(potb)), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz))
+    return dict(utors_slt=a.value, utors_slv=b.value, f=np.stack([fpx, fpy, fpz]), status=rc)
+
+
+def ref_fptors(s, ltor, pota, potb, coords=None):
+    R = ref()
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b = C.c_double(), C.c_double()
+    keep = [_i(s.ss_index), _i(ltor), _f(x), _f(y), _f(z), _f(pota), _f(potb)]
+    R.fptors_(_p(keep[0]), _p(keep[1]), _ri(keep[1].size // 4), _p(keep[2]), _p(keep[3]), _p(keep[4]), _p(keep[5]),
+              _p(keep[6]), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _ri(n), _ri(1))
+    return dict(utors_slt=a.value, utors_slv=b.value, f=np.stack([fpx, fpy, fpz]))

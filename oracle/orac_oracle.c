@@ -1664,3 +1664,204 @@ void orc_cself_alchemy(const int32_t *ss_index, int nato, double alphal, double 
   *self_slt = fs;
   *dwrk = fs - en0;
 }
+
+/* fnb14.f:1-219, serial build: the 1-4 pairs with LJ-FUDGE and QQ-FUDGE;
+ * erfc when ewald and cut, plain Coulomb otherwise.  Energies by the
+ * ss_index of the first atom. */
+void orc_fnb14(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+               const double *charge, int nato, const double *ecc12, const double *ecc6, int cut, int ewald,
+               double alphal, const int32_t *int14, int int14p, const int32_t *type14, double fudge, double lj_fudge,
+               double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt,
+               double *ucoul_slv) {
+  (void)nato;
+  double uconf[2] = {0.0, 0.0}, ucoul[2] = {0.0, 0.0};
+  *uconf_slt = *uconf_slv = *ucoul_slt = *ucoul_slv = 0.0;
+  if (int14p == 0) return;
+  const double twrtpi = 2.0 / sqrt(4.0 * atan(1.0));
+  const int erfc_branch = ewald && cut;
+  for (int i = 0; i < int14p; ++i) {
+    const int i1 = int14[2 * i], i2 = int14[2 * i + 1];
+    const int type = ss_index[i1 - 1];
+    const double xa = xp0[i1 - 1] - xp0[i2 - 1], ya = yp0[i1 - 1] - yp0[i2 - 1], za = zp0[i1 - 1] - zp0[i2 - 1];
+    const int lij = type14[i];
+    const double chrgei = charge[i1 - 1], chrgej = charge[i2 - 1];
+    const double rsq = xa * xa + ya * ya + za * za;
+    const double rsp = sqrt(rsq);
+    const double rsqi = 1.0 / rsq;
+    const double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+    const double ssvir = 12.0 * ecc12[lij - 1] * r12 - 6.0 * ecc6[lij - 1] * r6;
+    double qforce;
+    if (!erfc_branch) {
+      /* fnb14.f:69-105 */
+      qforce = lj_fudge * ssvir * rsqi;
+      const double elj = lj_fudge * (ecc12[lij - 1] * r12 - ecc6[lij - 1] * r6);
+      uconf[type - 1] = uconf[type - 1] + elj;
+      const double sv = fudge * chrgei * chrgej / rsp;
+      ucoul[type - 1] = ucoul[type - 1] + sv;
+      qforce = qforce + sv * rsqi;
+    } else {
+      /* fnb14.f:107-152 */
+      const double rspqi = rsqi / rsp;
+      const double ucon = lj_fudge * (ecc12[lij - 1] * r12 - ecc6[lij - 1] * r6);
+      uconf[type - 1] = uconf[type - 1] + ucon;
+      const double alphar = alphal * rsp;
+      const double qt = 1.0 / (1.0 + QP * alphar);
+      const double expcst = exp(-alphar * alphar);
+      const double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+      const double furpar = fudge * chrgei * chrgej;
+      const double ucou = furpar * erfcst / rsp;
+      ucoul[type - 1] = ucoul[type - 1] + ucou;
+      qforce = lj_fudge * ssvir * rsqi + furpar * (erfcst + twrtpi * alphar * expcst) * rspqi;
+    }
+    fpx[i1 - 1] = fpx[i1 - 1] + qforce * xa;
+    fpy[i1 - 1] = fpy[i1 - 1] + qforce * ya;
+    fpz[i1 - 1] = fpz[i1 - 1] + qforce * za;
+    fpx[i2 - 1] = fpx[i2 - 1] - qforce * xa;
+    fpy[i2 - 1] = fpy[i2 - 1] - qforce * ya;
+    fpz[i2 - 1] = fpz[i2 - 1] - qforce * za;
+  }
+  *uconf_slt = uconf[0];
+  *uconf_slv = uconf[1];
+  *ucoul_slt = ucoul[0];
+  *ucoul_slv = ucoul[1];
+}
+
+/* ------------------------------------------------------------------ */
+/* fptors.f:1-345, serial build: proper torsions                       */
+/* ------------------------------------------------------------------ */
+
+static double powl_ltr(double x, int n) { /* x**n for integer n >= 1, products left to right */
+  double r = x;
+  for (int k = 1; k < n; ++k) r = r * x;
+  return r;
+}
+
+static double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+/* Returns 0, or 1 at a torsion whose bond angles are singular (the reference
+ * then perturbs the four atoms at random, fptors.f:107-130, which is not
+ * restated), or 2 at an unsupported multiplicity (the reference stops). */
+int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const double *xp0, const double *yp0,
+               const double *zp0, const double *pota, const double *potb, double *utors_slt, double *utors_slv,
+               double *fpx, double *fpy, double *fpz) {
+  const double sin60 = (double)0.866025f, lim = (double)1.000001f;
+  double utors[2] = {0.0, 0.0};
+  int status = 0;
+  for (int i = 0; i < ltors; ++i) {
+    if (fabs(potb[i]) < 1.0e-12) continue;
+    const int l1 = ltor[4 * i], l2 = ltor[4 * i + 1], l3 = ltor[4 * i + 2], l4 = ltor[4 * i + 3];
+    const int type = ss_index[l1 - 1];
+    const double x21 = xp0[l2 - 1] - xp0[l1 - 1], y21 = yp0[l2 - 1] - yp0[l1 - 1], z21 = zp0[l2 - 1] - zp0[l1 - 1];
+    const double x32 = xp0[l3 - 1] - xp0[l2 - 1], y32 = yp0[l3 - 1] - yp0[l2 - 1], z32 = zp0[l3 - 1] - zp0[l2 - 1];
+    const double x43 = xp0[l4 - 1] - xp0[l3 - 1], y43 = yp0[l4 - 1] - yp0[l3 - 1], z43 = zp0[l4 - 1] - zp0[l3 - 1];
+    const double rsq21 = x21 * x21 + y21 * y21 + z21 * z21;
+    const double rsq32 = x32 * x32 + y32 * y32 + z32 * z32;
+    const double rsq43 = x43 * x43 + y43 * y43 + z43 * z43;
+    const double rsp21 = sqrt(rsq21), rsp32 = sqrt(rsq32), rsp43 = sqrt(rsq43);
+    const double cb1 = (x21 * x32 + y21 * y32 + z21 * z32) / (rsp21 * rsp32);
+    const double cb2 = (x43 * x32 + y43 * y32 + z43 * z32) / (rsp43 * rsp32);
+    const double cb3 = (x21 * x43 + y21 * y43 + z21 * z43) / (rsp21 * rsp43);
+    const double sb1 = sqrt(fabs(1.0 - cb1 * cb1)), sb2 = sqrt(fabs(1.0 - cb2 * cb2)),
+                 sb3 = sqrt(fabs(1.0 - cb3 * cb3));
+    const double aux = sb1 * sb2;
+    if (fabs(aux * sb3) < 1.0e-25) { status = 1; continue; }
+    double coa = (cb1 * cb2 - cb3) / aux;
+    const double pa = pota[i];
+    const int tmp = (int)(pa >= 0.0 ? floor(pa + 0.5) : -floor(-pa + 0.5)); /* IDNINT */
+    const int ntph = abs(tmp);
+    const double sig = (double)(tmp / ntph);
+    if (coa > 1.0 && coa < lim) coa = 1.0;
+    if (coa < -1.0 && coa > -lim) coa = -1.0;
+    double aux1 = cb1 * sb2, aux2 = sb1 * cb2;
+    const double auxa = -(aux1 * coa + aux2) / aux;
+    const double auxb = -(aux2 * coa + aux1) / aux;
+    const double auxc = sb3 / aux;
+    aux1 = rsp21 * rsp32 * sb1;
+    aux2 = rsp32 * rsp43 * sb2;
+    const double aux3 = rsp21 * rsp43 * sb3;
+    /* derivatives of the three bond-angle cosines (fptors.f:152-183) */
+    double d1[3][4], d2[3][4], d3[3][4]; /* [axis][atom] */
+    const double r21[3] = {x21, y21, z21}, r32[3] = {x32, y32, z32}, r43[3] = {x43, y43, z43};
+    for (int c = 0; c < 3; ++c) {
+      d1[c][0] = r32[c] / aux1 - cb1 * r21[c] / (rsq21 * sb1);
+      d1[c][2] = -r21[c] / aux1 + cb1 * r32[c] / (rsq32 * sb1);
+      d1[c][1] = -d1[c][2] - d1[c][0];
+      d2[c][1] = r43[c] / aux2 - cb2 * r32[c] / (rsq32 * sb2);
+      d2[c][3] = -r32[c] / aux2 + cb2 * r43[c] / (rsq43 * sb2);
+      d2[c][2] = -d2[c][1] - d2[c][3];
+      d3[c][0] = r43[c] / aux3 - cb3 * r21[c] / (rsq21 * sb3);
+      d3[c][2] = r21[c] / aux3 - cb3 * r43[c] / (rsq43 * sb3);
+      d3[c][1] = -d3[c][0];
+      d3[c][3] = -d3[c][2];
+    }
+    const double b = potb[i];
+    double enrg, qforce;
+#define P(n) powl_ltr(coa, n)
+    switch (ntph) {
+      case 1: enrg = sig * (fabs(b) - coa * b); qforce = sig * b; break;
+      case 2: enrg = sig * (fabs(b) - (2.0 * P(2) - 1.0) * b); qforce = sig * (4.0 * b * coa); break;
+      case 3: enrg = sig * (fabs(b) - (4.0 * P(3) - 3.0 * coa) * b); qforce = sig * ((12.0 * P(2) - 3.0) * b); break;
+      case 4:
+        enrg = sig * (fabs(b) - (8.0 * P(4) - 8.0 * P(2) + 1.0) * b);
+        qforce = sig * ((32.0 * P(3) - 16.0 * coa) * b);
+        break;
+      case 5:
+        enrg = sig * (fabs(b) - (16.0 * P(5) - 20.0 * P(3) + 5.0 * coa) * b);
+        qforce = sig * ((80.0 * P(4) - 60.0 * P(2) + 5.0) * b);
+        break;
+      case 6:
+        enrg = sig * (fabs(b) - b * (32.0 * P(6) - 48.0 * P(4) + 18.0 * P(2) - 1.0));
+        qforce = sig * (b * (192.0 * P(5) - 192.0 * P(3) + 36.0 * coa));
+        break;
+      case 7:
+        enrg = sig * (fabs(b) - b * (64.0 * P(7) - 112.0 * P(5) + 56.0 * P(3) - 7.0 * coa));
+        qforce = sig * (b * (448.0 * P(6) - 560.0 * P(4) + 168.0 * P(2) - 7.0));
+        break;
+      case 8:
+        enrg = sig * (fabs(b) - b * (128.0 * P(8) - 256.0 * P(6) + 160.0 * P(4) - 32.0 * P(2) + 1.0));
+        qforce = sig * (b * (1024.0 * P(7) - 1536.0 * P(5) + 640.0 * P(3) - 64.0 * coa));
+        break;
+      case 12:
+      case 14: {
+        /* fptors.f:251-290: the sine from the dihedral's orientation */
+        const double bb = acos(coa);
+        const double xn = y21 * z32 - y32 * z21, yn = z21 * x32 - z32 * x21, zn = x21 * y32 - x32 * y21;
+        const double aux4 = fsign(bb, xn * x43 + yn * y43 + zn * z43);
+        const double soa = fsign(sqrt(1.0 - pow(coa, 2.0)), aux4);
+        const double dsoa = -coa / soa;
+        double auxn, dauxn;
+        if (ntph == 14) {
+          enrg = b * (1.0 + 0.5 * (8.0 * P(4) - 8.0 * P(2) + 1.0));
+          auxn = b * sin60 * 4.0 * (soa * P(3) - coa * powl_ltr(soa, 3));
+          dauxn = b * sin60 * 4.0 * (dsoa * (P(3) - 3.0 * coa * powl_ltr(soa, 2)) - powl_ltr(soa, 3) + 3.0 * soa * P(2));
+          qforce = 0.5 * (32.0 * P(3) - 16.0 * coa) * b;
+        } else {
+          enrg = b * (1.0 + 0.5 * (2.0 * P(2) + 1.0));
+          auxn = b * sin60 * 2.0 * soa * coa;
+          dauxn = b * sin60 * 2.0 * (soa + dsoa * coa);
+          qforce = 2.0 * coa * b;
+        }
+        enrg = sig * (enrg + auxn);
+        qforce = sig * (-(qforce + dauxn));
+        break;
+      }
+      default: return 2;
+    }
+#undef P
+    utors[type - 1] = utors[type - 1] + enrg;
+    double u[3][4];
+    for (int c = 0; c < 3; ++c) {
+      u[c][0] = auxa * d1[c][0] + auxc * d3[c][0];
+      u[c][1] = auxa * d1[c][1] + auxb * d2[c][1] + auxc * d3[c][1];
+      u[c][2] = auxa * d1[c][2] + auxb * d2[c][2] + auxc * d3[c][2];
+      u[c][3] = auxb * d2[c][3] + auxc * d3[c][3];
+    }
+    const int at[4] = {l1, l2, l3, l4};
+    for (int k = 0; k < 4; ++k) fpx[at[k] - 1] = fpx[at[k] - 1] + qforce * u[0][k];
+    for (int k = 0; k < 4; ++k) fpy[at[k] - 1] = fpy[at[k] - 1] + qforce * u[1][k];
+    for (int k = 0; k < 4; ++k) fpz[at[k] - 1] = fpz[at[k] - 1] + qforce * u[2][k];
+  }
+  *utors_slt = utors[0];
+  *utors_slv = utors[1];
+  return status;
+}

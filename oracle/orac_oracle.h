@@ -129,3 +129,11 @@ int orc_ferrf_alchemy(const int32_t *ss_index, double alphal, const double *char
                       double *fpz, int nbonds_added, double *dwrk);
 void orc_cself_alchemy(const int32_t *ss_index, int nato, double alphal, double kcut, const double *charge,
                        const double *charge0, double *self_slt, double *dwrk);
+void orc_fnb14(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+               const double *charge, int nato, const double *ecc12, const double *ecc6, int cut, int ewald,
+               double alphal, const int32_t *int14, int int14p, const int32_t *type14, double fudge, double lj_fudge,
+               double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt,
+               double *ucoul_slv);
+int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const double *xp0, const double *yp0,
+               const double *zp0, const double *pota, const double *potb, double *utors_slt, double *utors_slv,
+               double *fpx, double *fpy, double *fpz);

@@ -9,8 +9,9 @@ cutoff 10 A, list skin 1.8 A, 18^3 linked cells) and writes:
 * the system's arrays (inputs of every routine of the path);
 * the reference Fortran's results on it (oracle/_ref, compiled in place):
   the linked-cell list, force() on the outer shell, fft_pme (fftpack branch),
-  ferrf on the solute's bonds and 1-3 pairs, and fnbgrp (with and without
-  Ewald) on the intra-group pairs and on the bond + 1-3 lists.  Exact arrays are stored as
+  ferrf on the solute's bonds and 1-3 pairs, fnbgrp (with and without
+  Ewald) on the intra-group pairs and on the bond + 1-3 lists, fnb14 on the
+  topology's 1-4 pairs (both branches) and fptors on its proper torsions.  Exact arrays are stored as
   SHA-256 digests of their bytes; energies, stress and a sample of the PME
   forces as values.
 
@@ -38,6 +39,7 @@ E6 = ("uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"
 
 
 FNB_E = ("uconf_slt", "uconf_slv", "ucoul_slt", "ucoul_slv")
+FUDGE14 = (0.83333, 0.5)
 
 
 def intra_group_pairs(s):
@@ -101,6 +103,17 @@ def main():
             r = O.ref_fnbgrp(s, lst, s.nbtype, bool(ew))
             out[f"ref_fnbgrp_{name}{ew}_e"] = np.array([r[k] for k in FNB_E])
             out[f"ref_fnbgrp_{name}{ew}_f_sha"] = digest(r["f"])
+    # bonded terms of the solute: the 1-4 pairs (QQ-FUDGE 0.83333 and LJ-FUDGE
+    # 0.5 of test3's alchemy.in; erfc and plain branches) and the proper torsions
+    out["int14"], out["type14"] = top["int14"].astype(np.int16), top["type14"].astype(np.int32)
+    out["ltor"], out["potto"] = top["ltor"].astype(np.int16), top["potto"]
+    for ew in (1, 0):
+        r = O.ref_fnb14(s, top["int14"], top["type14"], FUDGE14[0], FUDGE14[1], ewald=bool(ew), cut=True)
+        out[f"ref_fnb14_{ew}_e"] = np.array([r[k] for k in FNB_E])
+        out[f"ref_fnb14_{ew}_f_sha"] = digest(r["f"])
+    r = O.ref_fptors(s, top["ltor"], top["potto"][:, 1], top["potto"][:, 0])
+    out["ref_fptors_e"] = np.array([r["utors_slt"], r["utors_slv"]])
+    out["ref_fptors_f_sha"] = digest(r["f"])
     path = os.path.join(HERE, "fkbp12.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")

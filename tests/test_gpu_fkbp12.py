@@ -144,3 +144,51 @@ def test_fnbgrp_matches_oracle(gpu_ctx, fk, name):
         assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
         for got, k in zip(e, ("uconf_slt", "uconf_slv", "ucoul_slt", "ucoul_slv")):
             assert (got == 0.0 and o[k] == 0.0) or rel(got, o[k]) < 1e-12, k
+
+
+@pytest.mark.parametrize("ewald", [True, False])
+def test_fnb14_matches_oracle(gpu_ctx, fk, ewald):
+    s, _, g = fk
+    x, y, z = s.cartesian()
+    n = s.natoms
+    i14 = np.asfortranarray(g["int14"].astype(np.int32))
+    fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+    e = gpu_ctx.fnb14(s.ss_index, x, y, z, s.charge, n, s.ecc12, s.ecc6, True, ewald, s.alpha, i14, i14.shape[1],
+                      g["type14"], 0.83333, 0.5, fpx, fpy, fpz)
+    o = O.fnb14(s, i14, g["type14"], 0.83333, 0.5, ewald=ewald, cut=True)
+    assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
+    for got, k in zip(e, ("uconf_slt", "uconf_slv", "ucoul_slt", "ucoul_slv")):
+        assert (got == 0.0 and o[k] == 0.0) or rel(got, o[k]) < 1e-12, k
+
+
+def test_fptors_matches_oracle(gpu_ctx, fk):
+    s, _, g = fk
+    x, y, z = s.cartesian()
+    n = s.natoms
+    lt = np.asfortranarray(g["ltor"].astype(np.int32))
+    pt = g["potto"]
+    fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+    a, b = gpu_ctx.fptors(s.ss_index, lt, lt.shape[1], x, y, z, pt[:, 1], pt[:, 0], fpx, fpy, fpz, n)
+    o = O.fptors(s, lt, pt[:, 1], pt[:, 0])
+    assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
+    assert rel(a, o["utors_slt"]) < 1e-12 and b == o["utors_slv"] == 0.0
+
+
+def test_fptors_every_multiplicity(gpu_ctx):
+    """Synthetic torsions of every multiplicity the reference knows (1-8, 12, 14), both signs."""
+    from orac_hip import synth
+    s = synth.water_box()
+    rng = np.random.default_rng(11)
+    mults = [1, 2, 3, 4, 5, 6, 7, 8, 12, 14]
+    nt = 2 * len(mults)
+    lt = np.asfortranarray(np.arange(4 * nt, dtype=np.int32).reshape(nt, 4).T + 1)
+    pota = np.array([sg * m for m in mults for sg in (1, -1)], float)
+    potb = rng.uniform(0.2, 3.0, nt)
+    x, y, z = (rng.normal(0.0, 1.5, s.natoms) for _ in range(3))
+    n = s.natoms
+    fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+    a, b = gpu_ctx.fptors(s.ss_index, lt, nt, x, y, z, pota, potb, fpx, fpy, fpz, n)
+    o = O.fptors(s, lt, pota, potb, coords=(x, y, z))
+    assert o["status"] == 0
+    assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
+    assert rel(a + b, o["utors_slt"] + o["utors_slv"]) < 1e-12

@@ -99,3 +99,22 @@ def test_fnbgrp_matches_reference(fk, name, ewald):
     r = O.fnbgrp(s, lst, s.nbtype, bool(ewald))
     assert np.array_equal(np.array([r[k] for k in FNB_E]), g[f"ref_fnbgrp_{name}{ewald}_e"])
     assert np.array_equal(digest(r["f"]), g[f"ref_fnbgrp_{name}{ewald}_f_sha"])
+
+
+@pytest.mark.parametrize("ewald", [1, 0])
+def test_fnb14_matches_reference(fk, ewald):
+    """fnb14 on the topology's 4,500 1-4 pairs (QQ-FUDGE 0.83333, LJ-FUDGE 0.5)."""
+    s, _, g = fk
+    r = O.fnb14(s, g["int14"].astype(np.int32), g["type14"], 0.83333, 0.5, ewald=bool(ewald), cut=True)
+    assert np.array_equal(np.array([r[k] for k in FNB_E]), g[f"ref_fnb14_{ewald}_e"])
+    assert np.array_equal(digest(r["f"]), g[f"ref_fnb14_{ewald}_f_sha"])
+
+
+def test_fptors_matches_reference(fk):
+    """fptors on the topology's 5,732 proper torsions (multiplicities 1-6, both signs)."""
+    s, _, g = fk
+    pt = g["potto"]
+    r = O.fptors(s, g["ltor"].astype(np.int32), pt[:, 1], pt[:, 0])
+    assert r["status"] == 0
+    assert np.array_equal(np.array([r["utors_slt"], r["utors_slv"]]), g["ref_fptors_e"])
+    assert np.array_equal(digest(r["f"]), g["ref_fptors_f_sha"])
