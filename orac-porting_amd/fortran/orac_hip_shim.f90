@@ -113,6 +113,25 @@ module orac_hip_iface
       real(c_double), value :: alphal, rneigh, rinn, rout, rtolout, massinfty, qsoft, ljsoft
       integer(c_int64_t), value :: cap
     end function
+    integer(c_int) function c_fptors(ctx, ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, ut_slt, ut_slv, fpx, &
+         fpy, fpz, mb) bind(C, name="orac_hip_fptors")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), ltor(*)
+      integer(c_int32_t), value :: ltors, mb
+      real(c_double) :: xp0(*), yp0(*), zp0(*), pota(*), potb(*), ut_slt, ut_slv, fpx(*), fpy(*), fpz(*)
+    end function
+    integer(c_int) function c_fnb14(ctx, ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, cut, ewald, alphal, &
+         int14, int14p, type14, fudge, lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv) &
+         bind(C, name="orac_hip_fnb14")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), int14(*), type14(*)
+      integer(c_int32_t), value :: nato, cut, ewald, int14p
+      real(c_double) :: xp0(*), yp0(*), zp0(*), charge(*), ecc12(*), ecc6(*), fpx(*), fpy(*), fpz(*)
+      real(c_double), value :: alphal, fudge, lj_fudge
+      real(c_double) :: uconf_slt, uconf_slv, ucoul_slt, ucoul_slv
+    end function
     integer(c_int) function c_ferrf(ctx, ss_index, alphal, charge, fudge, x0, y0, z0, iz, list, nlist, fs_slt, &
          fs_slv, fpx, fpy, fpz, erf_corr, nbonds_added, mb) bind(C, name="orac_hip_ferrf")
       import :: c_ptr, c_int, c_int32_t, c_double
@@ -278,4 +297,27 @@ subroutine fnbgrp(ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alp
   logical :: ewald
   call check(c_fnbgrp(ctx(), ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, lg(ewald), alphal, ingrp, ingrpp, &
        fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, nato))
+end subroutine
+
+! fptors.f:1-2 (mts_intra_n1.f:118-119)
+subroutine fptors(ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, utors_slt, utors_slv, fpx, fpy, fpz, mb, itask)
+  use orac_hip_iface
+  implicit none
+  integer :: ltors, mb, itask, ltor(4,*), ss_index(*)
+  real(8) :: utors_slt, utors_slv, pota(*), potb(*), xp0(*), yp0(*), zp0(*), fpx(mb), fpy(mb), fpz(mb)
+  call check(c_fptors(ctx(), ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, utors_slt, utors_slv, fpx, fpy, fpz, &
+       mb))
+end subroutine
+
+! fnb14.f:1-3
+subroutine fnb14(ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, cut, ewald, alphal, int14, int14p, type14, &
+     fudge, lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, itask)
+  use orac_hip_iface
+  implicit none
+  integer :: nato, int14p, int14(2,*), type14(*), ss_index(*), itask
+  real(8) :: xp0(nato), yp0(nato), zp0(nato), fpx(nato), fpy(nato), fpz(nato), ecc6(*), ecc12(*), alphal, &
+       charge(nato), fudge, lj_fudge, ucoul_slt, ucoul_slv, uconf_slt, uconf_slv
+  logical :: ewald, cut
+  call check(c_fnb14(ctx(), ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, lg(cut), lg(ewald), alphal, int14, &
+       int14p, type14, fudge, lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv))
 end subroutine
