@@ -162,6 +162,27 @@ int orac_hip_ferrf(orac_hip_ctx *ctx, const int32_t *ss_index, double alphal, co
 int orac_hip_cself(orac_hip_ctx *ctx, const int32_t *ss_index, int32_t nato, double alphal, double kcut,
                    const double *charge, double *self_slt, double *self_slv);
 
+/* Replace fft_pme_alchemy (fft_pme_alchemy.f:1-117, do_pmesh_alchemy.f90),
+ * ferrf_alchemy (ferrf_alchemy.f:1-204, without ERF_CORR) and cself_alchemy
+ * (cself_alchemy.f:1-66) as mts_furier_alchemy.f:120-185 calls them: the
+ * routines of orac_hip_fft_pme / _ferrf / _cself on `charge`, plus the
+ * energy on the lambda0 charges charg0 (a second, energy-only grid for the
+ * reciprocal sum), returned as dwrk = U(charge) - U(charg0).  cself_alchemy
+ * sums solute atoms only and returns no solvent term, as the reference. */
+int orac_hip_fft_pme_alchemy(orac_hip_ctx *ctx, int32_t ntap, const double *xp, const double *yp, const double *zp,
+                             const double *xpcm, const double *ypcm, const double *zpcm, const double *pmechg,
+                             const double *charg0, const double *co, const double *oc, double volume, double alphal,
+                             int32_t order, int32_t nfft1, int32_t nfft2, int32_t nfft3, double *eer, double *fpx,
+                             double *fpy, double *fpz, double *stressc, const int32_t *atomp, const int32_t *grppt,
+                             int32_t pressure, double rkcut, double *dwrk);
+int orac_hip_ferrf_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, double alphal, const double *charge,
+                           const double *charge0, double fudge, const double *x0, const double *y0, const double *z0,
+                           int32_t iz, const int32_t *list, int32_t nlist, double *fscnstr_slt, double *fscnstr_slv,
+                           double *fpx, double *fpy, double *fpz, int32_t erf_corr, int32_t nbonds_added,
+                           double *dwrk, int32_t mb);
+int orac_hip_cself_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, int32_t nato, double alphal, double kcut,
+                           const double *charge, const double *charge0, double *self_slt, double *dwrk);
+
 /* Replaces fptors (fptors.f:1-345) as mts_intra_n1.f:118-119 calls it:
  * proper torsions ltor(4, ltors) on the coordinates as given, pota the
  * signed multiplicity (1-8, 12, 14) and potb the barrier; mb atoms.  Forces

@@ -126,12 +126,13 @@ extern "C" int orac_hip_fft_pme_init(orac_hip_ctx *ctx, int32_t numatoms, int32_
   });
 }
 
-extern "C" int orac_hip_fft_pme(orac_hip_ctx *ctx, int32_t ntap, const double *xp, const double *yp,
+static int fft_pme_impl(orac_hip_ctx *ctx, int32_t ntap, const double *xp, const double *yp,
                                 const double *zp, const double *xpcm, const double *ypcm, const double *zpcm,
                                 const double *pmechg, const double *co, const double *oc, double volume,
                                 double alphal, int32_t order, int32_t nfft1, int32_t nfft2, int32_t nfft3,
                                 double *eer, double *fpx, double *fpy, double *fpz, double *stressc,
-                                const int32_t *atomp, const int32_t *grppt, int32_t pressure, double rkcut) {
+                                const int32_t *atomp, const int32_t *grppt, int32_t pressure, double rkcut,
+                        const double *charg0, double *dwrk) {
   return guarded([&] {
     set_device(ctx);
     (void)grppt;
@@ -169,19 +170,40 @@ extern "C" int orac_hip_fft_pme(orac_hip_ctx *ctx, int32_t ntap, const double *x
       d.ypcm = B.put("p_ypcm", ypcm, nprot, st, false);
       d.zpcm = B.put("p_zpcm", zpcm, nprot, st, false);
     }
-    double *dout = B.get<double>("p_out", 20);
+    double *dout = B.get<double>("p_out", 40);
     d.out = dout;
-    PmeDesc *dd = B.put("p_desc", &d, 1, st, false);
-    launch_pme<double>(dd, &d, 1, st, ctx->tuning);
+    PmeDesc descs[2] = {d, d};
+    int ndesc = 1;
+    if (charg0) {
+      // do_pmesh_alchemy.f90: the lambda0 charges on a second grid, energy only
+      PmeDesc &e = descs[1];
+      e = pme_desc_base(B, "p0_", ctx->plan, ntap, oc, volume, alphal, rkcut, true, ctx->tuning, st);
+      e.xp = d.xp;
+      e.yp = d.yp;
+      e.zp = d.zp;
+      e.co = d.co;
+      e.q = B.put("p0_q", charg0, ntap, st, true);
+      e.fixscale = fix_scale_for(charg0, ntap);
+      e.fixinv = 1.0 / e.fixscale;
+      e.fx = d.fx;
+      e.fy = d.fy;
+      e.fz = d.fz;
+      e.out = dout + 20;
+      e.energy_only = 1;
+      ndesc = 2;
+    }
+    PmeDesc *dd = B.put("p_desc", descs, ndesc, st, false);
+    launch_pme<double>(dd, descs, ndesc, st, ctx->tuning);
     ORAC_HIP(hipGetLastError());
     std::vector<double> fx(ntap), fy(ntap), fz(ntap);
-    double out[20];
+    double out[40];
     ORAC_HIP(hipMemcpyAsync(fx.data(), d.fx, ntap * sizeof(double), hipMemcpyDeviceToHost, st));
     ORAC_HIP(hipMemcpyAsync(fy.data(), d.fy, ntap * sizeof(double), hipMemcpyDeviceToHost, st));
     ORAC_HIP(hipMemcpyAsync(fz.data(), d.fz, ntap * sizeof(double), hipMemcpyDeviceToHost, st));
     ORAC_HIP(hipMemcpyAsync(out, dout, sizeof out, hipMemcpyDeviceToHost, st));
     ORAC_HIP(hipStreamSynchronize(st));
     *eer = out[0];
+    if (charg0) *dwrk = out[0] - out[20];  // do_pmesh_alchemy.f90: dwrk = eer - eer0
     if (pressure) {
       // stressc(r,c) = stressc(r,c) - vir(r,c) + stc(3r+c)   (fft_pme.f:97-105)
       for (int r = 0; r < 3; ++r)
@@ -193,6 +215,28 @@ extern "C" int orac_hip_fft_pme(orac_hip_ctx *ctx, int32_t ntap, const double *x
       fpz[i] = fpz[i] + fz[i];
     }
   });
+}
+
+extern "C" int orac_hip_fft_pme(orac_hip_ctx *ctx, int32_t ntap, const double *xp, const double *yp,
+                                const double *zp, const double *xpcm, const double *ypcm, const double *zpcm,
+                                const double *pmechg, const double *co, const double *oc, double volume,
+                                double alphal, int32_t order, int32_t nfft1, int32_t nfft2, int32_t nfft3,
+                                double *eer, double *fpx, double *fpy, double *fpz, double *stressc,
+                                const int32_t *atomp, const int32_t *grppt, int32_t pressure, double rkcut) {
+  return fft_pme_impl(ctx, ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, co, oc, volume, alphal, order, nfft1, nfft2,
+                      nfft3, eer, fpx, fpy, fpz, stressc, atomp, grppt, pressure, rkcut, nullptr, nullptr);
+}
+
+extern "C" int orac_hip_fft_pme_alchemy(orac_hip_ctx *ctx, int32_t ntap, const double *xp, const double *yp,
+                                        const double *zp, const double *xpcm, const double *ypcm, const double *zpcm,
+                                        const double *pmechg, const double *charg0, const double *co,
+                                        const double *oc, double volume, double alphal, int32_t order, int32_t nfft1,
+                                        int32_t nfft2, int32_t nfft3, double *eer, double *fpx, double *fpy,
+                                        double *fpz, double *stressc, const int32_t *atomp, const int32_t *grppt,
+                                        int32_t pressure, double rkcut, double *dwrk) {
+  if (!charg0 || !dwrk) return guarded([] { ORAC_REQUIRE(false, E_ARG, "null charg0 or dwrk"); });
+  return fft_pme_impl(ctx, ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, co, oc, volume, alphal, order, nfft1, nfft2,
+                      nfft3, eer, fpx, fpy, fpz, stressc, atomp, grppt, pressure, rkcut, charg0, dwrk);
 }
 
 // ---------------------------------------------------------------- lists
@@ -689,6 +733,70 @@ extern "C" int orac_hip_cself(orac_hip_ctx *ctx, const int32_t *ss_index, int32_
     for (int n = 0; n < 2; ++n) s[n] = -s[n] * alphal / std::sqrt(pi) * (1.0 - erfcst);
     *self_slt = s[0];
     *self_slv = s[1];
+  });
+}
+
+extern "C" int orac_hip_ferrf_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, double alphal,
+                                      const double *charge, const double *charge0, double fudge, const double *x0,
+                                      const double *y0, const double *z0, int32_t iz, const int32_t *list,
+                                      int32_t nlist, double *fscnstr_slt, double *fscnstr_slv, double *fpx,
+                                      double *fpy, double *fpz, int32_t erf_corr, int32_t nbonds_added, double *dwrk,
+                                      int32_t mb) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(!erf_corr, E_UNSUPPORTED, "ferrf_alchemy: the ERF_CORR spline correction is not supported");
+    ORAC_REQUIRE(ss_index && charge && charge0 && x0 && y0 && z0 && fscnstr_slt && fscnstr_slv && dwrk, E_ARG,
+                 "null argument");
+    ORAC_REQUIRE(mb >= 0 && nlist >= 0 && nbonds_added >= 0 && nbonds_added <= nlist, E_ARG, "bad sizes");
+    int n = nlist - nbonds_added;
+    ORAC_REQUIRE(n == 0 || list, E_ARG, "null pair list");
+    hipStream_t st = ctx->stream;
+    BufSet &B = ctx->B;
+    CorrDesc h;
+    std::memset(&h, 0, sizeof h);
+    h.x = B.put("ca_x", x0, mb, st, false);
+    h.y = B.put("ca_y", y0, mb, st, false);
+    h.z = B.put("ca_z", z0, mb, st, false);
+    h.q = B.put("ca_q", charge, mb, st, false);
+    h.q0 = B.put("ca_q0", charge0, mb, st, false);
+    h.ss = B.put("ca_ss", ss_index, mb, st, true);
+    h.alpha = alphal;
+    h.twrtpi = 2.0 / std::sqrt(4.0 * std::atan(1.0));
+    h.fudge = fudge;
+    h.iz = iz == 1 ? 1 : 0;
+    h.kind = 0;
+    double e[4];
+    run_corr_pairs(ctx, h, list, n, mb, "ca_", e, fpx, fpy, fpz);
+    *fscnstr_slt = e[0];
+    *fscnstr_slv = e[1];
+    *dwrk = e[0] + e[1] - e[2];  // ferrf_alchemy.f: fsrtal(1) + fsrtal(2) - en0
+  });
+}
+
+extern "C" int orac_hip_cself_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, int32_t nato, double alphal,
+                                      double kcut, const double *charge, const double *charge0, double *self_slt,
+                                      double *dwrk) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(ss_index && charge && charge0 && self_slt && dwrk && nato >= 0, E_ARG, "bad argument");
+    hipStream_t st = ctx->stream;
+    BufSet &B = ctx->B;
+    int *ss = B.put("csa_ss", ss_index, nato, st, false);
+    double *q = B.put("csa_q", charge, nato, st, false);
+    double *q0 = B.put("csa_q0", charge0, nato, st, false);
+    double *dout = B.get<double>("csa_out", 4);
+    launch_cself(nato, q, ss, dout, st);
+    launch_cself(nato, q0, ss, dout + 2, st);
+    double s[4];
+    ORAC_HIP(hipMemcpyAsync(s, dout, sizeof s, hipMemcpyDeviceToHost, st));
+    ORAC_HIP(hipStreamSynchronize(st));
+    // cself_alchemy.f:36-56: solute atoms only
+    const double pi = 4.0 * std::atan(1.0);
+    const double erfcst = erfc_as(kcut / (2.0 * alphal));
+    const double fs = -s[0] * alphal / std::sqrt(pi) * (1.0 - erfcst);
+    const double en0 = -s[2] * alphal / std::sqrt(pi) * (1.0 - erfcst);
+    *self_slt = fs;
+    *dwrk = fs - en0;
   });
 }
 

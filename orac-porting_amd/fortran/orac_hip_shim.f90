@@ -113,6 +113,35 @@ module orac_hip_iface
       real(c_double), value :: alphal, rneigh, rinn, rout, rtolout, massinfty, qsoft, ljsoft
       integer(c_int64_t), value :: cap
     end function
+    integer(c_int) function c_fft_pme_alchemy(ctx, ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, charg0, co, oc, &
+         volume, alphal, order, n1, n2, n3, eer, fpx, fpy, fpz, stressc, atomp, grppt, pressure, rkcut, dwrk) &
+         bind(C, name="orac_hip_fft_pme_alchemy")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ntap, order, n1, n2, n3, pressure
+      real(c_double) :: xp(*), yp(*), zp(*), xpcm(*), ypcm(*), zpcm(*), pmechg(*), charg0(*), co(*), oc(*)
+      real(c_double), value :: volume, alphal, rkcut
+      real(c_double) :: eer, fpx(*), fpy(*), fpz(*), stressc(*), dwrk
+      integer(c_int32_t) :: atomp(*), grppt(*)
+    end function
+    integer(c_int) function c_ferrf_alchemy(ctx, ss_index, alphal, charge, charge0, fudge, x0, y0, z0, iz, list, &
+         nlist, fs_slt, fs_slv, fpx, fpy, fpz, erf_corr, nbonds_added, dwrk, mb) bind(C, name="orac_hip_ferrf_alchemy")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), list(*)
+      real(c_double), value :: alphal, fudge
+      real(c_double) :: charge(*), charge0(*), x0(*), y0(*), z0(*), fs_slt, fs_slv, fpx(*), fpy(*), fpz(*), dwrk
+      integer(c_int32_t), value :: iz, nlist, erf_corr, nbonds_added, mb
+    end function
+    integer(c_int) function c_cself_alchemy(ctx, ss_index, nato, alphal, kcut, charge, charge0, self_slt, dwrk) &
+         bind(C, name="orac_hip_cself_alchemy")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*)
+      integer(c_int32_t), value :: nato
+      real(c_double), value :: alphal, kcut
+      real(c_double) :: charge(*), charge0(*), self_slt, dwrk
+    end function
     integer(c_int) function c_fptors(ctx, ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, ut_slt, ut_slv, fpx, &
          fpy, fpz, mb) bind(C, name="orac_hip_fptors")
       import :: c_ptr, c_int, c_int32_t, c_double
@@ -320,4 +349,39 @@ subroutine fnb14(ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, cut, ewald,
   logical :: ewald, cut
   call check(c_fnb14(ctx(), ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, lg(cut), lg(ewald), alphal, int14, &
        int14p, type14, fudge, lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv))
+end subroutine
+
+! fft_pme_alchemy.f:1-3 (mts_furier_alchemy.f:125-128)
+subroutine fft_pme_alchemy(ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, charg0, co, oc, volume, alphal, order, nfft1, &
+     nfft2, nfft3, eer, fpx, fpy, fpz, stressc, atomp, grppt, pressure, rkcut, dwrk)
+  use orac_hip_iface
+  implicit none
+  integer :: ntap, order, nfft1, nfft2, nfft3, atomp(*), grppt(2,*)
+  real(8) :: xp(*), yp(*), zp(*), xpcm(*), ypcm(*), zpcm(*), pmechg(*), charg0(*), co(3,3), oc(3,3), volume, alphal
+  real(8) :: eer, fpx(*), fpy(*), fpz(*), stressc(3,3), rkcut, dwrk
+  logical :: pressure
+  call check(c_fft_pme_alchemy(ctx(), ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, charg0, co, oc, volume, alphal, &
+       order, nfft1, nfft2, nfft3, eer, fpx, fpy, fpz, stressc, atomp, grppt, lg(pressure), rkcut, dwrk))
+end subroutine
+
+! ferrf_alchemy.f:1-3
+subroutine ferrf_alchemy(ss_index, alphal, charge, charge0, fudge, x0, y0, z0, iz, list, nlist, fscnstr_slt, &
+     fscnstr_slv, fpx, fpy, fpz, erf_corr, erf_arr_corr, delew, rlew, nbonds_added, dwrk, nato)
+  use orac_hip_iface
+  implicit none
+  integer :: nato, iz, nlist, list(2,*), ss_index(*), nbonds_added
+  real(8) :: charge(*), charge0(*), x0(*), y0(*), z0(*), fpx(nato), fpy(nato), fpz(nato), fscnstr_slt, fscnstr_slv
+  real(8) :: erf_arr_corr(4,*), delew, rlew, dwrk, alphal, fudge
+  logical :: erf_corr
+  call check(c_ferrf_alchemy(ctx(), ss_index, alphal, charge, charge0, fudge, x0, y0, z0, iz, list, nlist, &
+       fscnstr_slt, fscnstr_slv, fpx, fpy, fpz, lg(erf_corr), nbonds_added, dwrk, nato))
+end subroutine
+
+! cself_alchemy.f:1-2
+subroutine cself_alchemy(ss_index, nato, alphal, kcut, charge, charge0, self_slt, dwrk)
+  use orac_hip_iface
+  implicit none
+  integer :: ss_index(*), nato
+  real(8) :: charge(*), charge0(*), alphal, kcut, self_slt, dwrk
+  call check(c_cself_alchemy(ctx(), ss_index, nato, alphal, kcut, charge, charge0, self_slt, dwrk))
 end subroutine

@@ -54,6 +54,7 @@ EXPORTS = [
     "orac_hip_probe_copy",
     "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
     "orac_hip_force_alchemy", "orac_hip_force_alchemy_shift", "orac_hip_fptors", "orac_hip_fnb14",
+    "orac_hip_fft_pme_alchemy", "orac_hip_ferrf_alchemy", "orac_hip_cself_alchemy",
     "orac_hip_decide_force", "orac_hip_ferrf", "orac_hip_cself", "orac_hip_fnbgrp", "orac_hip_system_create",
     "orac_hip_system_destroy", "orac_hip_system_set_coords", "orac_hip_system_device_ptrs", "orac_hip_system_eval",
     "orac_hip_system_results", "orac_hip_system_profile", "orac_hip_system_stage_ms", "orac_hip_system_bytes",
@@ -156,6 +157,20 @@ class Context:
                                       C.c_int32(nfft3), C.byref(eer), _p(fpx), _p(fpy), _p(fpz), _p(stressc),
                                       _p(ap), _p(gp), C.c_int32(1 if pressure else 0), C.c_double(rkcut)))
         return eer.value
+
+    # -- fft_pme_alchemy.f:1
+    def fft_pme_alchemy(self, ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, charg0, co, oc, volume, alphal, order,
+                        nfft1, nfft2, nfft3, fpx, fpy, fpz, stressc, atomp, grppt, pressure, rkcut):
+        """Returns (eer, dwrk); fpx/fpy/fpz (+=) and stressc (+=) are updated in place."""
+        eer, dw = C.c_double(), C.c_double()
+        args = [_f(xp), _f(yp), _f(zp), _f(xpcm), _f(ypcm), _f(zpcm), _f(pmechg), _f(charg0), _f(co), _f(oc)]
+        fpx, fpy, fpz, stressc = (_out(a, np.float64) for a in (fpx, fpy, fpz, stressc))
+        ap, gp = _i(atomp), _i(grppt)
+        _check(lib().orac_hip_fft_pme_alchemy(
+            self._h, C.c_int32(ntap), *[_p(a) for a in args], C.c_double(volume), C.c_double(alphal),
+            C.c_int32(order), C.c_int32(nfft1), C.c_int32(nfft2), C.c_int32(nfft3), C.byref(eer), _p(fpx), _p(fpy),
+            _p(fpz), _p(stressc), _p(ap), _p(gp), C.c_int32(1 if pressure else 0), C.c_double(rkcut), C.byref(dw)))
+        return eer.value, dw.value
 
     # -- lc_index.f:3
     @staticmethod
@@ -276,6 +291,30 @@ class Context:
                                     C.c_int32(nlist), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz),
                                     C.c_int32(int(bool(erf_corr))), C.c_int32(nbonds_added), C.c_int32(mb)))
         return a.value, b.value
+
+    # -- ferrf_alchemy.f:1
+    def ferrf_alchemy(self, ss_index, alphal, charge, charge0, fudge, x0, y0, z0, iz, list, nlist, fpx, fpy, fpz,
+                      nbonds_added=0, mb=None):
+        """Returns (fscnstr_slt, fscnstr_slv, dwrk); forces accumulated in place when iz = 1."""
+        a, b, dw = C.c_double(), C.c_double(), C.c_double()
+        fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
+        mb = len(fpx) if mb is None else mb
+        keep = [_i(ss_index), _f(charge), _f(charge0), _f(x0), _f(y0), _f(z0), _i(list)]
+        _check(lib().orac_hip_ferrf_alchemy(self._h, _p(keep[0]), C.c_double(alphal), _p(keep[1]), _p(keep[2]),
+                                            C.c_double(fudge), _p(keep[3]), _p(keep[4]), _p(keep[5]), C.c_int32(iz),
+                                            _p(keep[6]), C.c_int32(nlist), C.byref(a), C.byref(b), _p(fpx), _p(fpy),
+                                            _p(fpz), C.c_int32(0), C.c_int32(nbonds_added), C.byref(dw),
+                                            C.c_int32(mb)))
+        return a.value, b.value, dw.value
+
+    # -- cself_alchemy.f:1
+    def cself_alchemy(self, ss_index, nato, alphal, kcut, charge, charge0):
+        """Returns (self_slt, dwrk)."""
+        a, dw = C.c_double(), C.c_double()
+        _check(lib().orac_hip_cself_alchemy(self._h, _p(_i(ss_index)), C.c_int32(nato), C.c_double(alphal),
+                                            C.c_double(kcut), _p(_f(charge)), _p(_f(charge0)), C.byref(a),
+                                            C.byref(dw)))
+        return a.value, dw.value
 
     # -- cself.f:1
     def cself(self, ss_index, nato, alphal, kcut, charge):

@@ -152,3 +152,38 @@ def test_windows_batched_equal_alone(gpu_ctx):
     b.close()
     for rs in rss:
         rs.close()
+
+
+def test_plain_reciprocal_and_corrections(gpu_ctx, alch_sys):
+    """orac_hip_fft_pme_alchemy, _ferrf_alchemy and _cself_alchemy (the routines
+    mts_furier_alchemy.f calls) against the oracle."""
+    s, atoms, _, _, synth = alch_sys
+    al = synth.alchemy_for(s, atoms)
+    q = synth.alchemy_charges(s.charge, al.ss_alchemy, 1.0, 0.35)
+    q0 = synth.alchemy_charges(s.charge, al.ss_alchemy, 1.0, 0.4)
+    n = s.natoms
+    gpu_ctx.fft_pme_init(n, *s.nfft, s.order)
+    cm = s.molecule_coms()
+    fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+    st = np.zeros((3, 3), order="F")
+    eer, dw = gpu_ctx.fft_pme_alchemy(n, s.xp0, s.yp0, s.zp0, cm[0], cm[1], cm[2], q, q0, s.co, s.oc, s.volume,
+                                      s.alpha, s.order, *s.nfft, fpx, fpy, fpz, st, s.atomp, s.grppt, True, 1e9)
+    o = O.fft_pme_alchemy(s, q, q0, variant="fftw", nthreads=4)
+    assert rel(eer, o["eer"]) < 1e-10
+    assert abs(dw - o["dwrk"]) <= 1e-10 * abs(o["eer"])
+    assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-9
+    assert np.abs(st - o["stress"]).max() <= 1e-10 * np.abs(o["stress"]).max()
+    pairs = np.asfortranarray(np.concatenate([s.bonds, s.bends], axis=1))
+    x, y, z = s.cartesian()
+    for iz in (1, 0):
+        fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+        a, b, d = gpu_ctx.ferrf_alchemy(s.ss_index, s.alpha, q, q0, 0.8, x, y, z, iz, pairs, pairs.shape[1], fpx,
+                                        fpy, fpz)
+        r = O.ferrf_alchemy(s, q, q0, pairs, iz, 0.8)
+        assert rel(a, r["slt"]) < 1e-12 and b == r["slv"]
+        assert abs(d - r["dwrk"]) <= 1e-12 * abs(r["slt"])
+        if iz == 1:
+            assert relrms(np.stack([fpx, fpy, fpz]), r["f"]) < 1e-12
+    a, d = gpu_ctx.cself_alchemy(s.ss_index, n, s.alpha, 1e9, q, q0)
+    ra, rd = O.cself_alchemy(s, q, q0)
+    assert rel(a, ra) < 1e-13 and abs(d - rd) <= 1e-13 * abs(ra)
