@@ -986,7 +986,8 @@ extern "C" int orac_hip_system_device_ptrs(orac_hip_ctx *ctx, int32_t sys, doubl
 extern "C" int orac_hip_system_eval(orac_hip_ctx *ctx, int32_t sys, int32_t rebuild_list) {
   return guarded([&] {
     set_device(ctx);
-    batch_eval(ctx, get_batch(ctx, solo_batch(ctx, sys)), rebuild_list != 0, false);
+    // replayed from the system's own HIP graph (eager while profiling)
+    batch_eval(ctx, get_batch(ctx, solo_batch(ctx, sys)), rebuild_list != 0, true);
   });
 }
 

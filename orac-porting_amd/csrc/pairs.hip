@@ -410,16 +410,36 @@ __global__ void __launch_bounds__(1024) k_list_total(const ListDesc *__restrict_
   __shared__ int wmax[16];
   const ListDesc &d = D[blockIdx.x];
   if (!list_due(d)) return;
+  __shared__ int wsum[16];
   const int n = d.ngrp;
-  const int tot = block_exclusive_scan(d.count, d.off, n);
-  int mx = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) mx = max(mx, d.count[i]);
+  // the slab layout reads counts only: sum and largest count, no offsets
+  int tot = 0, mx = 0, s = 0;
+  if (d.slab > 0) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int c = d.count[i];
+      s += c;
+      mx = max(mx, c);
+    }
+  } else {
+    tot = block_exclusive_scan(d.count, d.off, n);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = max(mx, d.count[i]);
+  }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
-  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+  for (int off = 32; off > 0; off >>= 1) {
+    mx = max(mx, __shfl_xor(mx, off, 64));
+    s += __shfl_xor(s, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    wmax[threadIdx.x >> 6] = mx;
+    wsum[threadIdx.x >> 6] = s;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 0; w < 16; ++w) mx = max(mx, wmax[w]);
+    if (d.slab > 0) {
+      tot = 0;
+      for (int w = 0; w < 16; ++w) tot += wsum[w];
+    }
     d.off[n] = tot;
     d.total[0] = tot;
     d.total[1] = mx > d.slab ? 1 : 0;

@@ -213,28 +213,46 @@ __global__ void __launch_bounds__(256) k_bin_zero(const PmeDesc *__restrict__ D)
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) d.bin_cnt[i] = 0;
 }
 
+// Neighbouring atoms fall in the same tile, so a wavefront's atoms share a
+// handful of bins: lanes of one bin are counted with one atomic by their
+// lowest lane, and each lane takes base + its rank among them.
 __global__ void __launch_bounds__(256) k_bin_count(const PmeDesc *__restrict__ D) {
   const PmeDesc &d = D[blockIdx.y];
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (!d.tiled || a >= d.natoms) return;
-  double x, y, z;
-  cartesian(d.co, d.xp[a], d.yp[a], d.zp[a], x, y, z);
-  double fr1 = frac_coord(x, y, z, d.recip, 0, d.K1);
-  double fr2 = frac_coord(x, y, z, d.recip, 1, d.K2);
-  double fr3 = frac_coord(x, y, z, d.recip, 2, d.K3);
-  if (!(fr1 >= 0.0 && fr1 <= d.K1 && fr2 >= 0.0 && fr2 <= d.K2 && fr3 >= 0.0 && fr3 <= d.K3)) {
-    d.atom_bin[a] = make_int2(-1, 0);  // non-finite coordinates are not spread
-    return;
+  if (!d.tiled) return;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int bin = -1;
+  if (a < d.natoms) {
+    double x, y, z;
+    cartesian(d.co, d.xp[a], d.yp[a], d.zp[a], x, y, z);
+    double fr1 = frac_coord(x, y, z, d.recip, 0, d.K1);
+    double fr2 = frac_coord(x, y, z, d.recip, 1, d.K2);
+    double fr3 = frac_coord(x, y, z, d.recip, 2, d.K3);
+    if (!(fr1 >= 0.0 && fr1 <= d.K1 && fr2 >= 0.0 && fr2 <= d.K2 && fr3 >= 0.0 && fr3 <= d.K3)) {
+      d.atom_bin[a] = make_int2(-1, 0);  // non-finite coordinates are not spread
+    } else {
+      // fr = K (w - anint(w) = 0.5 exactly) is the grid origin: same spline weights
+      if (fr1 >= d.K1) fr1 -= d.K1;
+      if (fr2 >= d.K2) fr2 -= d.K2;
+      if (fr3 >= d.K3) fr3 -= d.K3;
+      d.afr[a] = make_double4(fr1, fr2, fr3, d.q[a]);
+      int b1 = (int)fr1, b2 = (int)fr2, b3 = (int)fr3;
+      bin = b1 / d.st1 + d.nb1 * (b2 / d.st2 + d.nb2 * (b3 / d.st3));
+    }
   }
-  // fr = K (w - anint(w) = 0.5 exactly) is the grid origin: same spline weights
-  if (fr1 >= d.K1) fr1 -= d.K1;
-  if (fr2 >= d.K2) fr2 -= d.K2;
-  if (fr3 >= d.K3) fr3 -= d.K3;
-  d.afr[a] = make_double4(fr1, fr2, fr3, d.q[a]);
-  int b1 = (int)fr1, b2 = (int)fr2, b3 = (int)fr3;
-  int bin = b1 / d.st1 + d.nb1 * (b2 / d.st2 + d.nb2 * (b3 / d.st3));
-  int pos = atomicAdd(&d.bin_cnt[bin], 1);
-  d.atom_bin[a] = make_int2(bin, pos);
+  unsigned long long todo = __ballot(bin >= 0);
+  int pos = 0;
+  while (todo) {  // the same on every lane of the wavefront
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lb = __shfl(bin, leader, 64);
+    const unsigned long long same = __ballot(bin == lb);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&d.bin_cnt[lb], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (bin == lb) pos = base + __popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  if (bin >= 0) d.atom_bin[a] = make_int2(bin, pos);
 }
 
 // one block per system: exclusive prefix of the bin counts, fixed order
