@@ -272,6 +272,11 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
         dense[(size_t)i * ma + j] = make_double2(d.ecc12[lij], d.ecc6[lij]);
       }
     p.lj_dense = B.put_vec("ljd", dense, st, false);
+    if (d.precision == 1) {
+      std::vector<float2> d32(dense.size());
+      for (size_t k = 0; k < dense.size(); ++k) d32[k] = make_float2((float)dense[k].x, (float)dense[k].y);
+      p.lj_dense32 = B.put_vec("ljd32", d32, st, false);
+    }
   } else {
     std::vector<int> tt((size_t)ma * ma);
     for (size_t k = 0; k < tt.size(); ++k) {
@@ -318,6 +323,7 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   p.sgp = B.get<double4>("sgp", ng);
   p.sgi = B.get<int4>("sgi", ng);
   p.sap = B.get<double4>("sap", n);
+  if (d.precision == 1) p.sapf = B.get<float4>("sapf", n);
   p.sai = B.get<int2>("sai", n);
   p.out = S.out + 20;
   p.max_group = S.max_group;

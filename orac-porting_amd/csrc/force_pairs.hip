@@ -44,10 +44,19 @@ __global__ void __launch_bounds__(256) k_pack_sorted(const PairDesc *__restrict_
   const int sf = d.soff ? gmem(d.soff)[t] : f0;
   gstore(d.sgp + t, make_double4(gmem(d.xpg)[g], gmem(d.ypg)[g], gmem(d.zpg)[g], 0.0));
   gstore(d.sgi + t, make_int4(sf, f0, n | (gmem(d.gss)[g] << 8), g));
+  const double gx = gmem(d.xpg)[g], gy = gmem(d.ypg)[g], gz = gmem(d.zpg)[g];
   for (int k = 0; k < n; ++k) {
     const int a = f0 + k;
-    gstore(d.sap + sf + k, make_double4(gmem(d.xp)[a], gmem(d.yp)[a], gmem(d.zp)[a], gmem(d.q)[a]));
+    const double x = gmem(d.xp)[a], y = gmem(d.yp)[a], z = gmem(d.zp)[a], q = gmem(d.q)[a];
+    gstore(d.sap + sf + k, make_double4(x, y, z, q));
     gstore(d.sai + sf + k, make_int2(gmem(d.type)[a], gmem(d.ex_inter)[a] ? 1 : 0));
+    if (d.sapf) {
+      // Cartesian offset from the group centre: a pair's separation is the
+      // group separation (FP64) plus the difference of two small offsets
+      const double dx = x - gx, dy = y - gy, dz = z - gz;
+      gstore(d.sapf + sf + k, make_float4((float)(d.co[0] * dx + d.co[3] * dy + d.co[6] * dz),
+                                          (float)(d.co[4] * dy + d.co[7] * dz), (float)(d.co[8] * dz), (float)q));
+    }
   }
 }
 
@@ -171,13 +180,20 @@ struct PairTerms {
   }
 };
 
+// The pair at Cartesian separation (xc, yc, zc) with charge product qq.
+template <typename T>
+__device__ __forceinline__ PairTerms pair_terms_c(const PairDesc &d, T xc, T yc, T zc, T qq, double c12, double c6);
+
 template <typename T>
 __device__ __forceinline__ PairTerms pair_terms(const PairDesc &d, double pxs, double pys, double pzs, double qa,
                                                 const double *co, const double4 &pb4, double c12, double c6) {
   const double xg = pxs - pb4.x, yg = pys - pb4.y, zg = pzs - pb4.z;
-  const T xc = T(co[0] * xg + co[3] * yg + co[6] * zg);
-  const T yc = T(co[4] * yg + co[7] * zg);
-  const T zc = T(co[8] * zg);
+  return pair_terms_c<T>(d, T(co[0] * xg + co[3] * yg + co[6] * zg), T(co[4] * yg + co[7] * zg), T(co[8] * zg),
+                         T(qa * pb4.w), c12, c6);
+}
+
+template <typename T>
+__device__ __forceinline__ PairTerms pair_terms_c(const PairDesc &d, T xc, T yc, T zc, T qq, double c12, double c6) {
   const T rsq = xc * xc + yc * yc + zc * zc;
   const T rinv = pair_rsqrt<T>(rsq);
   const T rsqi = rinv * rinv;
@@ -197,11 +213,11 @@ __device__ __forceinline__ PairTerms pair_terms(const PairDesc &d, double pxs, d
     const T qt = pair_rcp<T>(T(1) + T(AS_QP) * alphar);
     const T expcst = pair_exp<T>(-(alphar * alphar));
     const T erfcst = ((((T(AS_A5) * qt + T(AS_A4)) * qt + T(AS_A3)) * qt + T(AS_A2)) * qt + T(AS_A1)) * qt * expcst;
-    const T furpar = T(qa * pb4.w);
+    const T furpar = qq;
     r.cou = (double)(furpar * erfcst * rinv);
     qforce += furpar * (erfcst + T(d.twrtpi) * alphar * expcst) * rspqi;
   } else if (!d.ewald) {
-    const T ssv = T(qa * pb4.w) * rinv;
+    const T ssv = qq * rinv;
     r.cou = (double)ssv;
     qforce += ssv * rsqi;
   }
@@ -354,6 +370,9 @@ __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShare
     const int as = gi.x + (active ? ai : 0);      // sorted index
     const double4 pa4 = gload(d.sap + as);
     const int2 ia4 = gload(d.sai + as);
+    // mixed precision reads compact FP32 records (the soft-core form keeps the FP64 ones)
+    constexpr bool FREC = sizeof(T) == 4;
+    const float4 paf = (FREC && !alch) ? gload(d.sapf + as) : make_float4(0.f, 0.f, 0.f, 0.f);
     const double ma = gmem(d.gmass)[a];
     const int typei = gi.z >> 8;
     const double4 gpi = gload(d.sgp + t);
@@ -450,6 +469,26 @@ __device__ __forceinline__ void pairs_item(const PairDesc &d, int blk, PairShare
               if (fwd ? mask_a : (ib0.y & 1) != 0) skip0 = skip0 || pair_masked(d, a, jb0 + k, ia0, nI, fwd);
               const AlchTerms r0 = alch_terms<T>(d, pxs, pys, pzs, pa4.w, co, pb0, ap, sh.alch[(int)ap.z]);
               if (!skip0) r0.add(uconfa, ucoula, dwa, fgx, fgy, fgz);
+            }
+          } else if constexpr (FREC) {
+            // compact records: separation = group separation (FP64) + offset difference
+            for (int k = 0; k < nJ; ++k) {
+              const int2 ib0 = gload_at(d.sai, jbs + k);
+              const float4 pb0 = gload_at(d.sapf, jbs + k);
+              double c12a, c6a;
+              if (d.lj_dense32) {
+                const float2 lj = gload_at(d.lj_dense32, ia4.x * d.ntypes + ib0.x);
+                c12a = lj.x;
+                c6a = lj.y;
+              } else {
+                lj_params(d, ia4.x, ib0.x, c12a, c6a);
+              }
+              bool skip0 = d.lskip_ewald && c6a < 1e-12 && d.ewald;
+              if (fwd ? mask_a : (ib0.y & 1) != 0) skip0 = skip0 || pair_masked(d, a, jb0 + k, ia0, nI, fwd);
+              const PairTerms r0 =
+                  pair_terms_c<T>(d, T(gg.xcc + (double)(paf.x - pb0.x)), T(gg.ycc + (double)(paf.y - pb0.y)),
+                                  T(gg.zcc + (double)(paf.z - pb0.z)), T(paf.w * pb0.w), c12a, c6a);
+              if (!skip0) r0.add(uconfa, ucoula, fgx, fgy, fgz);
             }
           } else {
             for (int k = 0; k < nJ; ++k) {

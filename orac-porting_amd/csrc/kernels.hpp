@@ -80,6 +80,8 @@ struct PairDesc {
   double4 *sgp;                     // [ngrp] {xpg, ypg, zpg, 0}
   int4 *sgi;                        // [ngrp] {sorted first atom, first atom, natoms | gss << 8, group}
   double4 *sap;                     // [natoms] {xp, yp, zp, q}
+  float4 *sapf;                     // [natoms] {Cartesian offset from the group centre, q}: mixed-precision pairs
+  const float2 *lj_dense32;         // the dense LJ table in FP32 (mixed-precision pairs), or null
   int2 *sai;                        // [natoms] {type, flags}; flags bit 0 = ex_inter
   double *out;                      // [16]: uconf[3], ucoul[3] by typeij, st[9] row-major, dwrk
   int max_group;                    // largest group (atoms)
