@@ -149,14 +149,26 @@ int orac_hip_force_alchemy_shift(orac_hip_ctx *ctx, const int32_t *ss_index, con
 int orac_hip_decide_force(double rinn, double alphal, int32_t *lskip_ewald);
 
 /* ---- corrections ------------------------------------------------------------ */
-/* Replaces ferrf (ferrf.f:1-244) without the ERF_CORR spline branch
- * (erf_corr != 0 returns 4): reciprocal-space erf correction of the
+/* Replaces ferrf (ferrf.f:1-244): reciprocal-space erf correction of the
  * excluded pairs list(2, nlist) on cartesian x0/y0/z0; iz = 1 also adds
- * forces (fudge-scaled), iz = 0 gives the energy only. */
+ * forces (fudge-scaled), iz = 0 gives the energy only.  With erf_corr the
+ * reciprocal-cutoff correction is added from the spline table
+ * erf_arr_corr(4, *) on r = rlew + (i - 1) delew (ferrf.f:68-105,
+ * 170-193), as int_corr_erf_spline builds it; a pair below rlew returns 1. */
 int orac_hip_ferrf(orac_hip_ctx *ctx, const int32_t *ss_index, double alphal, const double *charge, double fudge,
                    const double *x0, const double *y0, const double *z0, int32_t iz, const int32_t *list,
                    int32_t nlist, double *fscnstr_slt, double *fscnstr_slv, double *fpx, double *fpy, double *fpz,
-                   int32_t erf_corr, int32_t nbonds_added, int32_t mb);
+                   int32_t erf_corr, const double *erf_arr_corr, double delew, double rlew, int32_t nbonds_added,
+                   int32_t mb);
+/* ERF_CORR table builders, called once at set-up (mtsmd.f:1595-1606):
+ * int_corr_erf_spline (int_corr_erf_spline.f:1-89, with cubspl of
+ * erfc_spline.f:71-193) fills erf_arr_corr(4, nbin + 1) and tau(nbin + 1)
+ * for r in [rmin, rmax]; erf_corr_cutoff (erf_corr_cutoff.f:1-54) sets
+ * rkcut from the cell and the grid (delew is accepted and not read). */
+int orac_hip_int_corr_erf_spline(double rmin, double rmax, int32_t nbin, double alpha, double kcut,
+                                 double *erf_arr_corr, double *tau);
+int orac_hip_erf_corr_cutoff(const double *oc, double *delew, double *rkcut, int32_t nfft1, int32_t nfft2,
+                             int32_t nfft3);
 
 /* Replaces cself (cself.f:1-64): Ewald self energy per class. */
 int orac_hip_cself(orac_hip_ctx *ctx, const int32_t *ss_index, int32_t nato, double alphal, double kcut,

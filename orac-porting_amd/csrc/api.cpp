@@ -680,14 +680,35 @@ static void run_corr_pairs(orac_hip_ctx *ctx, CorrDesc &h, const int32_t *list, 
     }
 }
 
+// ERF_CORR: the table rows the pair list reaches (ic of the largest separation),
+// uploaded for the correction kernel; a pair below the table's first node is an error
+static void erf_table_for(orac_hip_ctx *ctx, CorrDesc &h, const std::string &pfx, const double *tab, double delew,
+                          double rlew, const int32_t *list, int n, int mb, const double *x0, const double *y0,
+                          const double *z0) {
+  ORAC_REQUIRE(tab && delew > 0.0, E_ARG, "ERF_CORR needs its table and a positive spacing");
+  int icmax = 1;
+  for (int p = 0; p < n; ++p) {
+    const int a = list[2 * p] - 1, b = list[2 * p + 1] - 1;
+    ORAC_REQUIRE(a >= 0 && a < mb && b >= 0 && b < mb, E_ARG, "pair list out of range");
+    const double xab = x0[a] - x0[b], yab = y0[a] - y0[b], zab = z0[a] - z0[b];
+    const double rsp = std::sqrt(xab * xab + yab * yab + zab * zab);
+    const int ic = (int)(1.0 + (rsp - rlew) / delew);
+    ORAC_REQUIRE(ic >= 1, E_ARG, "ERF_CORR: a pair lies below the table's first node");
+    icmax = std::max(icmax, ic);
+  }
+  h.erf_tab = ctx->B.put(pfx + "etab", tab, 4 * (size_t)icmax, ctx->stream, true);
+  h.erf_n = icmax;
+  h.delew = delew;
+  h.rlew = rlew;
+}
+
 extern "C" int orac_hip_ferrf(orac_hip_ctx *ctx, const int32_t *ss_index, double alphal, const double *charge,
                               double fudge, const double *x0, const double *y0, const double *z0, int32_t iz,
                               const int32_t *list, int32_t nlist, double *fscnstr_slt, double *fscnstr_slv,
-                              double *fpx, double *fpy, double *fpz, int32_t erf_corr, int32_t nbonds_added,
-                              int32_t mb) {
+                              double *fpx, double *fpy, double *fpz, int32_t erf_corr, const double *erf_arr_corr,
+                              double delew, double rlew, int32_t nbonds_added, int32_t mb) {
   return guarded([&] {
     set_device(ctx);
-    ORAC_REQUIRE(!erf_corr, E_UNSUPPORTED, "ferrf: the ERF_CORR spline correction is not supported");
     ORAC_REQUIRE(ss_index && charge && x0 && y0 && z0 && fscnstr_slt && fscnstr_slv, E_ARG, "null argument");
     ORAC_REQUIRE(mb >= 0 && nlist >= 0 && nbonds_added >= 0 && nbonds_added <= nlist, E_ARG, "bad sizes");
     int n = nlist - nbonds_added;
@@ -696,6 +717,7 @@ extern "C" int orac_hip_ferrf(orac_hip_ctx *ctx, const int32_t *ss_index, double
     BufSet &B = ctx->B;
     CorrDesc h;
     std::memset(&h, 0, sizeof h);
+    if (erf_corr && n > 0) erf_table_for(ctx, h, "c_", erf_arr_corr, delew, rlew, list, n, mb, x0, y0, z0);
     h.x = B.put("c_x", x0, mb, st, false);
     h.y = B.put("c_y", y0, mb, st, false);
     h.z = B.put("c_z", z0, mb, st, false);
@@ -733,6 +755,23 @@ extern "C" int orac_hip_cself(orac_hip_ctx *ctx, const int32_t *ss_index, int32_
     for (int n = 0; n < 2; ++n) s[n] = -s[n] * alphal / std::sqrt(pi) * (1.0 - erfcst);
     *self_slt = s[0];
     *self_slv = s[1];
+  });
+}
+
+extern "C" int orac_hip_int_corr_erf_spline(double rmin, double rmax, int32_t nbin, double alpha, double kcut,
+                                            double *erf_arr_corr, double *tau) {
+  return guarded([&] {
+    ORAC_REQUIRE(nbin >= 1 && erf_arr_corr && tau && rmax > rmin, E_ARG, "bad table arguments");
+    int_corr_erf_spline(rmin, rmax, nbin, alpha, kcut, erf_arr_corr, tau);
+  });
+}
+
+extern "C" int orac_hip_erf_corr_cutoff(const double *oc, double *delew, double *rkcut, int32_t nfft1, int32_t nfft2,
+                                        int32_t nfft3) {
+  return guarded([&] {
+    (void)delew;
+    ORAC_REQUIRE(oc && rkcut, E_ARG, "null argument");
+    *rkcut = erf_corr_cutoff(oc, nfft1, nfft2, nfft3);
   });
 }
 

@@ -393,4 +393,161 @@ void alch_charges(int n, const double *q, const int32_t *ss_alchemy, double ladd
   }
 }
 
+// ---------------------------------------------------------------- ERF_CORR tables
+
+// Cubic spline interpolation coefficients, C. de Boor, "A Practical Guide to
+// Splines", routine CUBSPL (the form erfc_spline.f:71-193 carries).  c is
+// (4, n) column-major with the values in c(1, i) on entry; on exit c(2..4, i)
+// hold the first derivative, the second derivative and the third derivative
+// on [tau(i), tau(i+1)].  ibcbeg/ibcend: 0 not-a-knot, 1 slope given in
+// c(2, 1)/c(2, n), 2 second derivative given there.
+void cubspl(const double *tau, double *c, int n, int ibcbeg, int ibcend) {
+  auto C = [&](int k, int i) -> double & { return c[(k - 1) + 4 * (i - 1)]; };
+  auto T = [&](int i) { return tau[i - 1]; };
+  const int l = n - 1;
+  for (int m = 2; m <= n; ++m) {
+    C(3, m) = T(m) - T(m - 1);
+    C(4, m) = (C(1, m) - C(1, m - 1)) / C(3, m);
+  }
+  double g = 0.0;
+  bool skip_interior = false;  // n == 2: straight to the end condition
+  if (ibcbeg == 0) {
+    if (n > 2) {
+      C(4, 1) = C(3, 3);
+      C(3, 1) = C(3, 2) + C(3, 3);
+      C(2, 1) = ((C(3, 2) + 2.0 * C(3, 1)) * C(4, 2) * C(3, 3) + C(3, 2) * C(3, 2) * C(4, 3)) / C(3, 1);
+    } else {
+      C(4, 1) = 1.0;
+      C(3, 1) = 1.0;
+      C(2, 1) = 2.0 * C(4, 2);
+      skip_interior = true;
+    }
+  } else if (ibcbeg == 1) {
+    C(4, 1) = 1.0;
+    C(3, 1) = 0.0;
+    if (n == 2) skip_interior = true;
+  } else {
+    C(4, 1) = 2.0;
+    C(3, 1) = 1.0;
+    C(2, 1) = 3.0 * C(4, 2) - C(3, 2) / 2.0 * C(2, 1);
+    if (n == 2) skip_interior = true;
+  }
+  // 0: back substitution only; 1: g = -1 / c(4, n-1), then the last row; 2: the last row with g set
+  int last = 0;
+  if (!skip_interior) {
+    for (int m = 2; m <= l; ++m) {
+      g = -C(3, m + 1) / C(4, m - 1);
+      C(2, m) = g * C(2, m - 1) + 3.0 * (C(3, m) * C(4, m + 1) + C(3, m + 1) * C(4, m));
+      C(4, m) = g * C(3, m - 1) + 2.0 * (C(3, m) + C(3, m + 1));
+    }
+    if (ibcend == 0) {
+      if (n == 3 && ibcbeg == 0) {
+        C(2, n) = 2.0 * C(4, n);
+        C(4, n) = 1.0;
+        last = 1;
+      } else {
+        g = C(3, n - 1) + C(3, n);
+        C(2, n) = ((C(3, n) + 2.0 * g) * C(4, n) * C(3, n - 1) +
+                   C(3, n) * C(3, n) * (C(1, n - 1) - C(1, n - 2)) / C(3, n - 1)) / g;
+        g = -g / C(4, n - 1);
+        C(4, n) = C(3, n - 1);
+        last = 2;
+      }
+    } else if (ibcend == 2) {
+      C(2, n) = 3.0 * C(4, n) + C(3, n) / 2.0 * C(2, n);
+      C(4, n) = 2.0;
+      last = 1;
+    }
+  } else {
+    if (ibcend == 0) {
+      if (ibcbeg > 0) {
+        C(2, n) = 2.0 * C(4, n);
+        C(4, n) = 1.0;
+        last = 1;
+      } else {
+        C(2, n) = C(4, n);
+      }
+    } else if (ibcend == 2) {
+      C(2, n) = 3.0 * C(4, n) + C(3, n) / 2.0 * C(2, n);
+      C(4, n) = 2.0;
+      last = 1;
+    }
+  }
+  if (last == 1) g = -1.0 / C(4, n - 1);
+  if (last != 0) {
+    C(4, n) = g * C(3, n - 1) + C(4, n);
+    C(2, n) = (g * C(2, n - 1) + C(2, n)) / C(4, n);
+  }
+  for (int j = l; j > 0; --j) C(2, j) = (C(2, j) - C(3, j) * C(2, j + 1)) / C(4, j);
+  for (int i = 2; i <= n; ++i) {
+    const double dtau = C(3, i);
+    const double divdf1 = (C(1, i) - C(1, i - 1)) / dtau;
+    const double divdf3 = C(2, i - 1) + C(2, i) - 2.0 * divdf1;
+    C(3, i - 1) = 2.0 * (divdf1 - C(2, i - 1) - divdf3) / dtau;
+    C(4, i - 1) = (divdf3 / dtau) * (6.0 / dtau);
+  }
+}
+
+// Integrand of the reciprocal-cutoff correction (int_corr_erf_spline.f:70-80):
+// exp(-k^2 / 4 alpha^2) sin(k r) / (k r)
+static double erf_corr_func(double x, double alpha, double r) {
+  if (r < 1.0e-8) return std::exp(-(x * x) / (4.0 * (alpha * alpha)));
+  return std::exp(-(x * x) / (4.0 * (alpha * alpha))) * std::sin(r * x) / (r * x);
+}
+
+// Simpson's rule by refinement of the extended trapezoid rule (the published
+// routines qsimp/trapzd, Numerical Recipes section 4.2), as
+// int_corr_erf_spline.f:37-69 uses them: tolerance 1e-6, at most 20 levels.
+static double qsimp_erf(double a, double b, double alpha, double r) {
+  double st = 0.0, ost = -1.0e30, os = -1.0e30, s = 0.0;
+  long it = 1;
+  for (int j = 1; j <= 20; ++j) {
+    if (j == 1) {
+      st = 0.5 * (b - a) * (erf_corr_func(a, alpha, r) + erf_corr_func(b, alpha, r));
+      it = 1;
+    } else {
+      const double tnm = (double)it;
+      const double del = (b - a) / tnm;
+      double x = a + 0.5 * del, sum = 0.0;
+      for (long k = 1; k <= it; ++k) {
+        sum = sum + erf_corr_func(x, alpha, r);
+        x = x + del;
+      }
+      st = 0.5 * (st + (b - a) * sum / tnm);
+      it = 2 * it;
+    }
+    s = (4.0 * st - ost) / 3.0;
+    if (std::fabs(s - os) < 1.0e-6 * std::fabs(os)) return s;
+    os = s;
+    ost = st;
+  }
+  return s;
+}
+
+void int_corr_erf_spline(double rmin, double rmax, int nbin, double alpha, double kcut, double *erf_arr_corr,
+                         double *tau) {
+  const double del = (rmax - rmin) / (double)(float)nbin;
+  const double pi = 3.14159265358979323846;
+  const double kcut_max = 10.0 * 2.0 * alpha;
+  for (int i = 1; i <= nbin + 1; ++i) {
+    const double r = del * (i - 1) + rmin;
+    const double s = qsimp_erf(kcut, kcut_max, alpha, r);
+    tau[i - 1] = r;
+    erf_arr_corr[4 * (i - 1)] = 2.0 * s / pi;
+  }
+  cubspl(tau, erf_arr_corr, nbin + 1, 0, 0);
+}
+
+double erf_corr_cutoff(const double *oc, int nfft1, int nfft2, int nfft3) {
+  const double pi = 4.0 * std::atan(1.0);
+  const double aux = 0.5 * 0.5 * 2.0 * pi;
+  const double m1 = aux * (oc[0] * nfft1 + oc[1] * nfft2 + oc[2] * nfft3);
+  double rkcut = m1;
+  const double m2 = aux * (oc[3] * nfft1 + oc[4] * nfft2 + oc[5] * nfft3);
+  if (m2 < m1) rkcut = m2;
+  const double m3 = aux * (oc[6] * nfft1 + oc[7] * nfft2 + oc[8] * nfft3);
+  if (m3 < m2) rkcut = m3;
+  return (double)0.9f * rkcut;  // default-real literal (erf_corr_cutoff.f:49)
+}
+
 }  // namespace orac

@@ -157,6 +157,12 @@ void alch_factors(const double lam[8], bool intramol, bool ljintra, double qx, d
 // charges at (laddq, lrmq): (1 - |lambdaq|) q, end points exact (mts_furier_alchemy.f:98-111)
 void alch_charges(int n, const double *q, const int32_t *ss_alchemy, double laddq, double lrmq, double *out);
 
+// ---- ERF_CORR tables (int_corr_erf_spline.f, erf_corr_cutoff.f, cubspl)
+void cubspl(const double *tau, double *c, int n, int ibcbeg, int ibcend);
+void int_corr_erf_spline(double rmin, double rmax, int nbin, double alpha, double kcut, double *erf_arr_corr,
+                         double *tau);
+double erf_corr_cutoff(const double *oc, int nfft1, int nfft2, int nfft3);
+
 inline void set_device(orac_hip_ctx *c) {
   ORAC_REQUIRE(c != nullptr, E_ARG, "null context");
   ORAC_HIP(hipSetDevice(c->device));

@@ -186,6 +186,9 @@ struct CorrDesc {
   double *fx, *fy, *fz;               // accumulated (+=)
   double *out;                        // [4]: coul slt, coul slv, lj slt, lj slv
   const double *q0;                   // ferrf_alchemy: lambda0 charges; en0 goes to the lj slt column
+  const double *erf_tab;              // ferrf ERF_CORR: spline table (4, erf_n) over r = rlew + (i - 1) delew
+  int erf_n;
+  double delew, rlew;
   const int *pair_type;               // fnb14: packed LJ index of each pair (type14, 1-based)
   double lj_fudge;                    // fnb14: LJ-FUDGE (fudge is QQ-FUDGE)
 };

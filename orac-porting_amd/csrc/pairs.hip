@@ -633,8 +633,28 @@ __global__ void __launch_bounds__(256) k_corr_pairs(const CorrDesc *__restrict__
     double expcst = exp(-(alphar * alphar));
     double erfcst = ((((AS_A5 * qt + AS_A4) * qt + AS_A3) * qt + AS_A2) * qt + AS_A1) * qt * expcst;
     double erfst = 1.0 - erfcst;
-    ecoul = -(furpar * erfst / rsp);
-    if (d.iz == 1) qforce = -(furpar * (erfst - d.twrtpi * alphar * expcst) / (rsp * rsq));
+    if (d.erf_tab) {
+      // ERF_CORR (ferrf.f:68-105 and 170-193): the reciprocal-cutoff correction
+      // from its cubic spline table, interval ic with offset h
+      int ic = (int)(1.0 + (rsp - d.rlew) / d.delew);
+      ic = min(max(ic, 1), d.erf_n);
+      const double h = rsp - (d.rlew + (ic - 1) * d.delew);
+      const double *c = d.erf_tab + 4 * (size_t)(ic - 1);
+      const double poly = c[0] + h * (c[1] + h * (c[2] + h * c[3] / 3) / 2);
+      if (d.iz == 1) {
+        const double qq = d.q[ia] * d.q[ib];
+        furpar = fud * qq;
+        const double corr = qq * poly;
+        const double dcorr = qq * (c[1] + h * (c[2] + h * c[3] / 2)) / rsp;
+        qforce = -dcorr - furpar * (erfst - d.twrtpi * alphar * expcst) / (rsp * rsq);
+        ecoul = -(furpar * erfst / rsp - corr);
+      } else {
+        ecoul = -(furpar * (erfst / rsp - poly));
+      }
+    } else {
+      ecoul = -(furpar * erfst / rsp);
+      if (d.iz == 1) qforce = -(furpar * (erfst - d.twrtpi * alphar * expcst) / (rsp * rsq));
+    }
     if (d.q0) {
       // ferrf_alchemy.f:144-145,190-191: the same term on the lambda0 charges (en0)
       double furpar0 = d.iz == 1 ? fud * d.q0[ia] * d.q0[ib] : d.q0[ia] * d.q0[ib];

@@ -162,13 +162,25 @@ module orac_hip_iface
       real(c_double) :: uconf_slt, uconf_slv, ucoul_slt, ucoul_slv
     end function
     integer(c_int) function c_ferrf(ctx, ss_index, alphal, charge, fudge, x0, y0, z0, iz, list, nlist, fs_slt, &
-         fs_slv, fpx, fpy, fpz, erf_corr, nbonds_added, mb) bind(C, name="orac_hip_ferrf")
+         fs_slv, fpx, fpy, fpz, erf_corr, erf_arr_corr, delew, rlew, nbonds_added, mb) bind(C, name="orac_hip_ferrf")
       import :: c_ptr, c_int, c_int32_t, c_double
       type(c_ptr), value :: ctx
       integer(c_int32_t) :: ss_index(*), list(*)
-      real(c_double), value :: alphal, fudge
-      real(c_double) :: charge(*), x0(*), y0(*), z0(*), fs_slt, fs_slv, fpx(*), fpy(*), fpz(*)
+      real(c_double), value :: alphal, fudge, delew, rlew
+      real(c_double) :: charge(*), x0(*), y0(*), z0(*), fs_slt, fs_slv, fpx(*), fpy(*), fpz(*), erf_arr_corr(*)
       integer(c_int32_t), value :: iz, nlist, erf_corr, nbonds_added, mb
+    end function
+    integer(c_int) function c_int_corr_erf_spline(rmin, rmax, nbin, alpha, kcut, erf_arr_corr, tau) &
+         bind(C, name="orac_hip_int_corr_erf_spline")
+      import :: c_int, c_int32_t, c_double
+      real(c_double), value :: rmin, rmax, alpha, kcut
+      integer(c_int32_t), value :: nbin
+      real(c_double) :: erf_arr_corr(*), tau(*)
+    end function
+    integer(c_int) function c_erf_corr_cutoff(oc, delew, rkcut, n1, n2, n3) bind(C, name="orac_hip_erf_corr_cutoff")
+      import :: c_int, c_int32_t, c_double
+      real(c_double) :: oc(*), delew, rkcut
+      integer(c_int32_t), value :: n1, n2, n3
     end function
     integer(c_int) function c_cself(ctx, ss_index, nato, alphal, kcut, charge, s_slt, s_slv) &
          bind(C, name="orac_hip_cself")
@@ -303,7 +315,7 @@ subroutine ferrf(ss_index, alphal, charge, fudge, x0, y0, z0, iz, list, nlist, f
   real(8) :: erf_arr_corr(4,*), delew, rlew, alphal, fudge
   logical :: erf_corr
   call check(c_ferrf(ctx(), ss_index, alphal, charge, fudge, x0, y0, z0, iz, list, nlist, fscnstr_slt, fscnstr_slv, &
-       fpx, fpy, fpz, lg(erf_corr), nbonds_added, mb))
+       fpx, fpy, fpz, lg(erf_corr), erf_arr_corr, delew, rlew, nbonds_added, mb))
 end subroutine
 
 ! cself.f:1-2
@@ -384,4 +396,22 @@ subroutine cself_alchemy(ss_index, nato, alphal, kcut, charge, charge0, self_slt
   integer :: ss_index(*), nato
   real(8) :: charge(*), charge0(*), alphal, kcut, self_slt, dwrk
   call check(c_cself_alchemy(ctx(), ss_index, nato, alphal, kcut, charge, charge0, self_slt, dwrk))
+end subroutine
+
+! int_corr_erf_spline.f:1-2 (mtsmd.f:1597-1598)
+subroutine int_corr_erf_spline(rmin, rmax, nbin, alpha, kcut, erf_arr_corr, tau)
+  use orac_hip_iface
+  implicit none
+  integer :: nbin
+  real(8) :: rmin, rmax, alpha, kcut, erf_arr_corr(4,*), tau(*)
+  call check(c_int_corr_erf_spline(rmin, rmax, nbin, alpha, kcut, erf_arr_corr, tau))
+end subroutine
+
+! erf_corr_cutoff.f:1 (mtsmd.f:1596)
+subroutine erf_corr_cutoff(oc, delew, rkcut, nfft1, nfft2, nfft3)
+  use orac_hip_iface
+  implicit none
+  integer :: nfft1, nfft2, nfft3
+  real(8) :: oc(3,3), delew, rkcut
+  call check(c_erf_corr_cutoff(oc, delew, rkcut, nfft1, nfft2, nfft3))
 end subroutine

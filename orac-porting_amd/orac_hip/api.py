@@ -54,7 +54,8 @@ EXPORTS = [
     "orac_hip_probe_copy",
     "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
     "orac_hip_force_alchemy", "orac_hip_force_alchemy_shift", "orac_hip_fptors", "orac_hip_fnb14",
-    "orac_hip_fft_pme_alchemy", "orac_hip_ferrf_alchemy", "orac_hip_cself_alchemy",
+    "orac_hip_fft_pme_alchemy", "orac_hip_ferrf_alchemy", "orac_hip_cself_alchemy", "orac_hip_int_corr_erf_spline",
+    "orac_hip_erf_corr_cutoff",
     "orac_hip_decide_force", "orac_hip_ferrf", "orac_hip_cself", "orac_hip_fnbgrp", "orac_hip_system_create",
     "orac_hip_system_destroy", "orac_hip_system_set_coords", "orac_hip_system_device_ptrs", "orac_hip_system_eval",
     "orac_hip_system_results", "orac_hip_system_profile", "orac_hip_system_stage_ms", "orac_hip_system_bytes",
@@ -280,17 +281,36 @@ class Context:
 
     # -- ferrf.f:1
     def ferrf(self, ss_index, alphal, charge, fudge, x0, y0, z0, iz, list, nlist, fpx, fpy, fpz, erf_corr=False,
-              nbonds_added=0, mb=None):
-        """Returns (fscnstr_slt, fscnstr_slv); forces accumulated in place when iz = 1."""
+              nbonds_added=0, mb=None, erf_arr_corr=None, delew=0.0, rlew=0.0):
+        """Returns (fscnstr_slt, fscnstr_slv); forces accumulated in place when iz = 1.
+        erf_corr uses the ERF_CORR table erf_arr_corr (4, n) with spacing delew from rlew."""
         a, b = C.c_double(), C.c_double()
         fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
         mb = len(fpx) if mb is None else mb
-        keep = [_i(ss_index), _f(charge), _f(x0), _f(y0), _f(z0), _i(list)]
+        keep = [_i(ss_index), _f(charge), _f(x0), _f(y0), _f(z0), _i(list),
+                _f(erf_arr_corr) if erf_arr_corr is not None else np.zeros(4)]
         _check(lib().orac_hip_ferrf(self._h, _p(keep[0]), C.c_double(alphal), _p(keep[1]), C.c_double(fudge),
                                     _p(keep[2]), _p(keep[3]), _p(keep[4]), C.c_int32(iz), _p(keep[5]),
                                     C.c_int32(nlist), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz),
-                                    C.c_int32(int(bool(erf_corr))), C.c_int32(nbonds_added), C.c_int32(mb)))
+                                    C.c_int32(int(bool(erf_corr))), _p(keep[6]), C.c_double(delew),
+                                    C.c_double(rlew), C.c_int32(nbonds_added), C.c_int32(mb)))
         return a.value, b.value
+
+    @staticmethod
+    def int_corr_erf_spline(rmin, rmax, nbin, alpha, kcut):
+        """The ERF_CORR spline table: returns (erf_arr_corr (4, nbin + 1) Fortran order, tau)."""
+        tab = np.zeros((4, nbin + 1), order="F")
+        tau = np.zeros(nbin + 1)
+        _check(lib().orac_hip_int_corr_erf_spline(C.c_double(rmin), C.c_double(rmax), C.c_int32(nbin),
+                                                  C.c_double(alpha), C.c_double(kcut), _p(tab), _p(tau)))
+        return tab, tau
+
+    @staticmethod
+    def erf_corr_cutoff(oc, nfft1, nfft2, nfft3):
+        rk, dl = C.c_double(), C.c_double()
+        _check(lib().orac_hip_erf_corr_cutoff(_p(_f(oc)), C.byref(dl), C.byref(rk), C.c_int32(nfft1),
+                                              C.c_int32(nfft2), C.c_int32(nfft3)))
+        return rk.value
 
     # -- ferrf_alchemy.f:1
     def ferrf_alchemy(self, ss_index, alphal, charge, charge0, fudge, x0, y0, z0, iz, list, nlist, fpx, fpy, fpz,

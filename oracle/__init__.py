@@ -228,15 +228,18 @@ def decide_force(rinn, alpha):
     return bool(lib().orc_decide_force(C.c_double(rinn), C.c_double(alpha)))
 
 
-def ferrf(s, pairs, iz, fudge=1.0, fp=None):
+def ferrf(s, pairs, iz, fudge=1.0, fp=None, erf=None):
+    """ferrf.f; erf = (erf_arr_corr (4, n), delew, rlew) selects the ERF_CORR branch."""
     x, y, z = s.cartesian()
     n = s.natoms
     fpx, fpy, fpz = [np.zeros(n) for _ in range(3)] if fp is None else [_f(a).copy() for a in fp]
     a, b = C.c_double(), C.c_double()
     lst = _i(pairs)
+    tab = _f(erf[0]) if erf is not None else np.zeros(4)
     rc = lib().orc_ferrf(_p(_i(s.ss_index)), C.c_double(s.alpha), _p(_f(s.charge)), C.c_double(fudge), _p(_f(x)),
                          _p(_f(y)), _p(_f(z)), C.c_int(iz), _p(lst), C.c_int(lst.size // 2), C.byref(a), C.byref(b),
-                         _p(fpx), _p(fpy), _p(fpz), C.c_int32(0), C.c_int(0))
+                         _p(fpx), _p(fpy), _p(fpz), C.c_int32(0 if erf is None else 1), C.c_int(0), _p(tab),
+                         C.c_double(erf[1] if erf is not None else 0.0), C.c_double(erf[2] if erf is not None else 0.0))
     assert rc == 0
     return dict(slt=a.value, slv=b.value, f=np.stack([fpx, fpy, fpz]))
 
@@ -390,18 +393,19 @@ def ref_force(s, nnlpp, *, rinn, rout, rtolinn, rtolout, rneigh, iz=1, lskip_ewa
     return out
 
 
-def ref_ferrf(s, pairs, iz, fudge=1.0):
+def ref_ferrf(s, pairs, iz, fudge=1.0, erf=None):
     R = ref()
     x, y, z = s.cartesian()
     n = s.natoms
     fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
     a, b = C.c_double(), C.c_double()
     lst = _i(pairs)
-    dummy = np.zeros(8)
+    tab = _f(erf[0]) if erf is not None else np.zeros(8)
     keep = [_i(s.ss_index), _f(s.charge), _f(x), _f(y), _f(z)]
     R.ferrf_(_p(keep[0]), _rd(s.alpha), _p(keep[1]), _rd(fudge), _p(keep[2]), _p(keep[3]), _p(keep[4]), _ri(iz),
-             _p(lst), _ri(lst.size // 2), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _ri(0), _p(dummy),
-             _rd(0.0), _rd(0.0), _ri(0), _ri(n))
+             _p(lst), _ri(lst.size // 2), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz),
+             _ri(0 if erf is None else 1), _p(tab), _rd(erf[1] if erf is not None else 0.0),
+             _rd(erf[2] if erf is not None else 0.0), _ri(0), _ri(n))
     return dict(slt=a.value, slv=b.value, f=np.stack([fpx, fpy, fpz]))
 
 
@@ -656,3 +660,18 @@ def ref_fptors(s, ltor, pota, potb, coords=None):
     R.fptors_(_p(keep[0]), _p(keep[1]), _ri(keep[1].size // 4), _p(keep[2]), _p(keep[3]), _p(keep[4]), _p(keep[5]),
               _p(keep[6]), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _ri(n), _ri(1))
     return dict(utors_slt=a.value, utors_slv=b.value, f=np.stack([fpx, fpy, fpz]))
+
+
+
+def ref_int_corr_erf_spline(rmin, rmax, nbin, alpha, kcut):
+    """The reference's ERF_CORR table builder: (erf_arr_corr (4, nbin + 1), tau)."""
+    tab = np.zeros((4, nbin + 1), order="F")
+    tau = np.zeros(nbin + 1)
+    ref().int_corr_erf_spline_(_rd(rmin), _rd(rmax), _ri(nbin), _rd(alpha), _rd(kcut), _p(tab), _p(tau))
+    return tab, tau
+
+
+def ref_erf_corr_cutoff(oc, nfft):
+    rk, dl = C.c_double(), C.c_double()
+    ref().erf_corr_cutoff_(_p(_f(oc)), C.byref(dl), C.byref(rk), _ri(nfft[0]), _ri(nfft[1]), _ri(nfft[2]))
+    return rk.value

@@ -1151,13 +1151,52 @@ int32_t orc_decide_force(double rinn, double alphal) {
   return erfcst < 10.0e-4;
 }
 
-/* ferrf.f:1-244 (the ERF_CORR spline branches are not restated) */
+/* ferrf.f:1-244; with erf_corr the ERF_CORR branches (ferrf.f:68-105, 170-193)
+ * read the spline table erf_arr_corr(4, *) at r = rlew + (ic - 1) delew */
 int orc_ferrf(const int32_t *ss_index, double alphal, const double *charge, double fudge, const double *x0,
               const double *y0, const double *z0, int iz, const int32_t *list, int nlist, double *fs_slt,
-              double *fs_slv, double *fpx, double *fpy, double *fpz, int32_t erf_corr, int nbonds_added) {
-  if (erf_corr) return 1;
+              double *fs_slv, double *fpx, double *fpy, double *fpz, int32_t erf_corr, int nbonds_added,
+              const double *erf_arr_corr, double delew, double rlew) {
   double twrtpi = 2.0 / sqrt(4.0 * atan(1.0));
   double fs[2] = {0.0, 0.0};
+  if (erf_corr) {
+    for (int i = 0; i < nlist - nbonds_added; ++i) {
+      const int ia = list[2 * i], ib = list[2 * i + 1];
+      const double qq = charge[ia - 1] * charge[ib - 1];
+      const double xab = x0[ia - 1] - x0[ib - 1], yab = y0[ia - 1] - y0[ib - 1], zab = z0[ia - 1] - z0[ib - 1];
+      const double rsq = xab * xab + yab * yab + zab * zab;
+      const double rsp = sqrt(rsq);
+      const int ic = (int)(1 + (rsp - rlew) / delew);
+      const double h = rsp - (rlew + (ic - 1) * delew);
+      const double *c = erf_arr_corr + 4 * (ic - 1);
+      const double alphar = alphal * rsp;
+      const double qt = 1.0 / (1.0 + QP * alphar);
+      const double expcst = exp(-alphar * alphar);
+      const double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+      const double erfst = 1.0 - erfcst;
+      const int type = ss_index[ia - 1];
+      if (iz == 1) {
+        const double furpar = fudge * qq;
+        const double corr = qq * (c[0] + h * (c[1] + h * (c[2] + h * c[3] / 3) / 2));
+        const double dcorr = qq * (c[1] + h * (c[2] + h * c[3] / 2)) / rsp;
+        const double qforce = -dcorr - furpar * (erfst - twrtpi * alphar * expcst) / (rsp * rsq);
+        fs[type - 1] = fs[type - 1] - (furpar * erfst / rsp - corr);
+        fpx[ia - 1] = fpx[ia - 1] + qforce * xab;
+        fpy[ia - 1] = fpy[ia - 1] + qforce * yab;
+        fpz[ia - 1] = fpz[ia - 1] + qforce * zab;
+        fpx[ib - 1] = fpx[ib - 1] - qforce * xab;
+        fpy[ib - 1] = fpy[ib - 1] - qforce * yab;
+        fpz[ib - 1] = fpz[ib - 1] - qforce * zab;
+      } else {
+        const double furpar = qq;
+        const double corr = c[0] + h * (c[1] + h * (c[2] + h * c[3] / 3) / 2);
+        fs[type - 1] = fs[type - 1] - furpar * (erfst / rsp - corr);
+      }
+    }
+    *fs_slt = fs[0];
+    *fs_slv = fs[1];
+    return 0;
+  }
   for (int i = 0; i < nlist - nbonds_added; ++i) {
     int ia = list[2 * i], ib = list[2 * i + 1];
     int type = ss_index[ia - 1];

@@ -96,7 +96,8 @@ int32_t orc_decide_force(double rinn, double alphal);
 /* ferrf.f:1-244 without the ERF_CORR spline branch. Returns 0, 1 if erf_corr set. */
 int orc_ferrf(const int32_t *ss_index, double alphal, const double *charge, double fudge, const double *x0,
               const double *y0, const double *z0, int iz, const int32_t *list, int nlist, double *fs_slt,
-              double *fs_slv, double *fpx, double *fpy, double *fpz, int32_t erf_corr, int nbonds_added);
+              double *fs_slv, double *fpx, double *fpy, double *fpz, int32_t erf_corr, int nbonds_added,
+              const double *erf_arr_corr, double delew, double rlew);
 /* cself.f:1-64 */
 void orc_cself(const int32_t *ss_index, int nato, double alphal, double kcut, const double *charge,
                double *self_slt, double *self_slv);

@@ -11,7 +11,8 @@ cutoff 10 A, list skin 1.8 A, 18^3 linked cells) and writes:
   the linked-cell list, force() on the outer shell, fft_pme (fftpack branch),
   ferrf on the solute's bonds and 1-3 pairs, fnbgrp (with and without
   Ewald) on the intra-group pairs and on the bond + 1-3 lists, fnb14 on the
-  topology's 1-4 pairs (both branches) and fptors on its proper torsions.  Exact arrays are stored as
+  topology's 1-4 pairs (both branches), fptors on its proper torsions, and
+  the ERF_CORR table (int_corr_erf_spline, erf_corr_cutoff) with ferrf on it.  Exact arrays are stored as
   SHA-256 digests of their bytes; energies, stress and a sample of the PME
   forces as values.
 
@@ -40,6 +41,7 @@ E6 = ("uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"
 
 FNB_E = ("uconf_slt", "uconf_slv", "ucoul_slt", "ucoul_slv")
 FUDGE14 = (0.83333, 0.5)
+ERFC = (300, 0.5, 3.5)  # ERF_CORR nbinew rlew ruew
 
 
 def intra_group_pairs(s):
@@ -114,6 +116,17 @@ def main():
     r = O.ref_fptors(s, top["ltor"], top["potto"][:, 1], top["potto"][:, 0])
     out["ref_fptors_e"] = np.array([r["utors_slt"], r["utors_slv"]])
     out["ref_fptors_f_sha"] = digest(r["f"])
+    # ERF_CORR: the reference's reciprocal-cutoff correction table
+    # (ERF_CORR 300 0.5 3.5; rkcut from erf_corr_cutoff) and ferrf with it
+    rk = O.ref_erf_corr_cutoff(s.oc, s.nfft)
+    tab, tau = O.ref_int_corr_erf_spline(ERFC[1], ERFC[2], ERFC[0], s.alpha, rk)
+    out["erf_params"] = np.array([ERFC[0], ERFC[1], ERFC[2], rk])
+    out["ref_erf_table"], out["ref_erf_tau"] = tab, tau
+    delew = (ERFC[2] - ERFC[1]) / ERFC[0]
+    for iz in (1, 0):
+        c = O.ref_ferrf(s, pairs, iz, 0.8, erf=(tab, delew, ERFC[1]))
+        out[f"ref_ferrf_erf{iz}_e"] = np.array([c["slt"], c["slv"]])
+        out[f"ref_ferrf_erf{iz}_f_sha"] = digest(c["f"])
     path = os.path.join(HERE, "fkbp12.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")

@@ -192,3 +192,21 @@ def test_fptors_every_multiplicity(gpu_ctx):
     assert o["status"] == 0
     assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
     assert rel(a + b, o["utors_slt"] + o["utors_slv"]) < 1e-12
+
+
+@pytest.mark.parametrize("iz", [1, 0])
+def test_ferrf_erf_corr_matches_oracle(gpu_ctx, fk, iz):
+    """ferrf with ERF_CORR on the reference's correction table."""
+    s, _, g = fk
+    nbin, rlew, ruew, _ = g["erf_params"]
+    tab, delew = g["ref_erf_table"], (ruew - rlew) / int(nbin)
+    pairs = np.asfortranarray(np.concatenate([s.bonds, s.bends], axis=1))
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+    a, b = gpu_ctx.ferrf(s.ss_index, s.alpha, s.charge, 0.8, x, y, z, iz, pairs, pairs.shape[1], fpx, fpy, fpz,
+                         erf_corr=True, erf_arr_corr=tab, delew=delew, rlew=rlew)
+    o = O.ferrf(s, pairs, iz, 0.8, erf=(tab, delew, rlew))
+    assert rel(a, o["slt"]) < 1e-12 and b == o["slv"]
+    if iz == 1:
+        assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12

@@ -8,6 +8,7 @@ force() and ferrf results must agree bit for bit (compared as SHA-256 digests
 of the arrays' bytes); fft_pme's fftpack branch agrees to rounding.
 """
 import hashlib
+import os
 
 import numpy as np
 import pytest
@@ -118,3 +119,55 @@ def test_fptors_matches_reference(fk):
     assert r["status"] == 0
     assert np.array_equal(np.array([r["utors_slt"], r["utors_slv"]]), g["ref_fptors_e"])
     assert np.array_equal(digest(r["f"]), g["ref_fptors_f_sha"])
+
+
+def _erf(g):
+    nbin, rlew, ruew, _ = g["erf_params"]
+    return g["ref_erf_table"], (ruew - rlew) / int(nbin), rlew
+
+
+@pytest.mark.parametrize("iz", [1, 0])
+def test_ferrf_erf_corr_matches_reference(fk, iz):
+    """ferrf's ERF_CORR branches on the reference's own correction table."""
+    s, _, g = fk
+    pairs = np.concatenate([s.bonds, s.bends], axis=1)
+    c = O.ferrf(s, pairs, iz, 0.8, erf=_erf(g))
+    assert np.array_equal(np.array([c["slt"], c["slv"]]), g[f"ref_ferrf_erf{iz}_e"])
+    assert np.array_equal(digest(c["f"]), g[f"ref_ferrf_erf{iz}_f_sha"])
+
+
+def test_erf_corr_table_builders_match_reference(fk):
+    """The library's host restatement of int_corr_erf_spline (with de Boor's
+    CUBSPL) and erf_corr_cutoff reproduces the reference's table and cutoff."""
+    from orac_hip import api
+    if not os.path.exists(api.library_path()):
+        pytest.skip("liborac_hip.so not built")
+    s, _, g = fk
+    nbin, rlew, ruew, rk = g["erf_params"]
+    assert api.Context.erf_corr_cutoff(s.oc, *s.nfft) == rk
+    tab, tau = api.Context.int_corr_erf_spline(rlew, ruew, int(nbin), s.alpha, rk)
+    assert np.array_equal(tab, g["ref_erf_table"]) and np.array_equal(tau, g["ref_erf_tau"])
+
+
+def test_erf_corr_table_is_the_cutoff_integral(fk):
+    """Node values are (2/pi) int_kcut^{20 alpha} exp(-k^2/4a^2) sin(kr)/(kr) dk
+    (to the builder's Simpson tolerance, 1e-6) and the spline interpolates them
+    with continuous first and second derivatives at the interior nodes."""
+    from scipy.integrate import quad
+    s, _, g = fk
+    nbin, rlew, ruew, rk = g["erf_params"]
+    tab, tau = g["ref_erf_table"], g["ref_erf_tau"]
+    a = s.alpha
+    for i in (0, 37, 150, 300):
+        r = tau[i]
+        v = quad(lambda k: np.exp(-k * k / (4 * a * a)) * np.sin(k * r) / (k * r), rk, 20 * a, epsabs=1e-14,
+                 limit=200)[0] * 2 / np.pi
+        assert abs(tab[0, i] - v) <= 2e-6 * abs(v), i
+    h = np.diff(tau)[:-1]
+    c = tab[:, :-2]
+    val = c[0] + h * (c[1] + h * (c[2] / 2 + h * c[3] / 6))
+    der = c[1] + h * (c[2] + h * c[3] / 2)
+    sec = c[2] + h * c[3]
+    assert np.abs(val - tab[0, 1:-1]).max() <= 1e-12 * np.abs(tab[0]).max()
+    assert np.abs(der - tab[1, 1:-1]).max() <= 1e-9 * np.abs(tab[1]).max()
+    assert np.abs(sec - tab[2, 1:-1]).max() <= 1e-7 * np.abs(tab[2]).max()
