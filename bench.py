@@ -382,7 +382,8 @@ def main():
     GRAPH = not args.no_graph
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (one system or one batch per device); --gpus {args.gpus} is not "
-              "supported: sharding across GPUs is not part of this build.", file=sys.stderr)
+              "supported: the library runs one system or one batch per device and does not shard across GPUs.",
+              file=sys.stderr)
         sys.exit(2)
 
     from orac_hip import synth
