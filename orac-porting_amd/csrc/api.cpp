@@ -31,6 +31,7 @@ extern "C" int orac_hip_create(orac_hip_ctx **ctx, int32_t device) {
     c->device = device;
     ORAC_HIP(hipSetDevice(device));
     ORAC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    ORAC_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     *ctx = c.release();
   });
 }
@@ -43,6 +44,8 @@ extern "C" int orac_hip_destroy(orac_hip_ctx *ctx) {
     engine_destroy(ctx->engine);
     ctx->engine = nullptr;
     ctx->B.bufs.clear();
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });

@@ -78,6 +78,7 @@ struct Batch {
   ListDesc *dl = nullptr;
   CorrDesc *dc = nullptr;
   bool any_corr = false;
+  hipEvent_t fork = nullptr, join = nullptr;  // reciprocal sum on the side stream
   hipGraph_t graph[2] = {nullptr, nullptr};  // [rebuild]
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   bool profile = false;
@@ -107,6 +108,8 @@ struct Batch {
     }
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
   }
 };
 
@@ -393,10 +396,14 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   m.q = q;
   m.fixscale = fix_scale_for(d.charge, n);
   m.fixinv = 1.0 / m.fixscale;
-  m.fx = S.fx;
-  m.fy = S.fy;
-  m.fz = S.fz;
-  m.accumulate = 1;
+  // own force arrays: the reciprocal sum runs beside the direct sum and is added after both
+  m.fx = B.get<double>("fxp", n);
+  m.fy = B.get<double>("fyp", n);
+  m.fz = B.get<double>("fzp", n);
+  m.accumulate = 0;
+  m.fx_dst = S.fx;
+  m.fy_dst = S.fy;
+  m.fz_dst = S.fz;
   m.atomp = d_ap;
   m.xpcm = xpcm;
   m.ypcm = ypcm;
@@ -531,6 +538,8 @@ static int batch_create(orac_hip_ctx *ctx, const int32_t *ids, int n) {
     b->sys.push_back(&S);
   }
   for (auto &e : b->ev) ORAC_HIP(hipEventCreate(&e));
+  ORAC_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  ORAC_HIP(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
   batch_upload(ctx, *b);
   const int id = E.next_batch++;
   E.batches[id] = std::move(b);
@@ -541,12 +550,30 @@ static int batch_create(orac_hip_ctx *ctx, const int32_t *ids, int n) {
 static void batch_enqueue(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool events, bool md_lists = false) {
   hipStream_t st = ctx->stream;
   const int n = (int)b.sys.size();
+  const int npme = (int)b.hp.size();
   auto mark = [&](int k) {
     if (events) ORAC_HIP(hipEventRecord(b.ev[k], st));
+  };
+  auto pme = [&](hipStream_t s) {
+    if (b.precision == 0)
+      launch_pme<double>(b.dp, b.hp.data(), npme, s, b.tu);
+    else
+      launch_pme<float>(b.dp, b.hp.data(), npme, s, b.tu);
   };
   mark(ST_BEGIN);
   launch_centres(b.dq, b.hq.data(), n, st);
   mark(ST_CENTRES);
+  // The reciprocal sum needs the coordinates and the molecule centres only:
+  // unless the stages are being timed it runs on the side stream beside the
+  // list build and the pair kernel (both latency bound), and joins before
+  // its forces are added.
+  const bool side = !events && ctx->side;
+  if (side) {
+    ORAC_HIP(hipEventRecord(b.fork, st));
+    ORAC_HIP(hipStreamWaitEvent(ctx->side, b.fork, 0));
+    pme(ctx->side);
+    ORAC_HIP(hipEventRecord(b.join, ctx->side));
+  }
   if (md_lists)  // every system on its own update schedule (device step counters)
     launch_list_build(b.dl_md, b.hl_md.data(), n, LIST_COUNT | LIST_WRITE, st);
   else if (rebuild)
@@ -558,11 +585,11 @@ static void batch_enqueue(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool events
   else
     launch_pairs<float>(b.dq, b.hq.data(), n, st, b.tu, kb, ke);
   mark(ST_PAIRS);
-  const int npme = (int)b.hp.size();
-  if (b.precision == 0)
-    launch_pme<double>(b.dp, b.hp.data(), npme, st, b.tu);
+  if (side)
+    ORAC_HIP(hipStreamWaitEvent(st, b.join, 0));
   else
-    launch_pme<float>(b.dp, b.hp.data(), npme, st, b.tu);
+    pme(st);
+  launch_pme_combine(b.dp, b.hp.data(), npme, st);
   mark(ST_PME);
   if (b.any_corr) launch_corr(b.dc, b.hc.data(), n, st);
   mark(ST_END);
@@ -1130,10 +1157,11 @@ extern "C" int orac_hip_system_set_alchemy(orac_hip_ctx *ctx, int32_t sys, const
     e.q = S.q0;
     e.fixscale = S.pme.fixscale;
     e.fixinv = S.pme.fixinv;
-    e.fx = S.fx;
-    e.fy = S.fy;
-    e.fz = S.fz;
-    e.accumulate = 1;
+    e.fx = S.pme.fx;  // never written (energy only)
+    e.fy = S.pme.fy;
+    e.fz = S.pme.fz;
+    e.accumulate = 0;
+    e.fx_dst = e.fy_dst = e.fz_dst = nullptr;
     e.atomp = S.pme.atomp;
     e.xpcm = S.pme.xpcm;
     e.ypcm = S.pme.ypcm;

@@ -113,6 +113,7 @@ void engine_destroy(Engine *e);
 struct orac_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // reciprocal sum beside the direct sum (same, default priority)
   orac::BufSet B;
   orac::PmePlan plan;
   bool plan_ok = false;

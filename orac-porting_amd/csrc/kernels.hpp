@@ -27,6 +27,8 @@ struct Tuning {
 void pme_set_tiles(PmeDesc &d, int lds_elems);
 template <typename T>
 void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st, const Tuning &tu);
+// fx_dst += fx (and y, z) for every descriptor that has a destination
+void launch_pme_combine(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st);
 // tile edge of the LDS-tiled spread along an axis of K points (0: use the atomic spread)
 int spread_tile_edge(int K, int order, bool smallest);
 // Waves per owner group of the pair kernel for a system of ngrp groups (1

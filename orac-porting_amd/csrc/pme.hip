@@ -793,6 +793,25 @@ void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t s
   hipLaunchKernelGGL(k_pme_finalize, dim3(16, nsys), dim3(256), 0, st, d_desc);
 }
 
+// The reciprocal forces of a system evaluated beside its direct sum are
+// written to their own arrays; this adds them to the direct forces once both
+// are done, so the sum is the same as accumulating in sequence.
+__global__ void __launch_bounds__(256) k_pme_combine(const PmeDesc *__restrict__ D) {
+  const PmeDesc &d = D[blockIdx.y];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!d.fx_dst || a >= d.natoms) return;
+  d.fx_dst[a] += d.fx[a];
+  d.fy_dst[a] += d.fy[a];
+  d.fz_dst[a] += d.fz[a];
+}
+
+void launch_pme_combine(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st) {
+  int max_atoms = 1;
+  for (int s = 0; s < nsys; ++s)
+    if (h[s].fx_dst) max_atoms = std::max(max_atoms, h[s].natoms);
+  hipLaunchKernelGGL(k_pme_combine, dim3((max_atoms + 255) / 256, nsys), dim3(256), 0, st, d_desc);
+}
+
 template void launch_pme<double>(const PmeDesc *, const PmeDesc *, int, hipStream_t, const Tuning &);
 template void launch_pme<float>(const PmeDesc *, const PmeDesc *, int, hipStream_t, const Tuning &);
 
