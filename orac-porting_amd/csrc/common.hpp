@@ -86,6 +86,7 @@ struct PmeDesc {
   double *out;                  // [20]: eer, vir (3x3 col-major), stc1..stc9
   int energy_only;              // 1: spread, forward FFT and energy only (a lambda0 charge grid)
   double *fx_dst, *fy_dst, *fz_dst;  // when set, launch_pme_combine adds fx/fy/fz into these
+  const double *cfx, *cfy, *cfz;     // and then these (the system's correction forces), when set
 };
 
 // ---------------------------------------------------------------- helpers

@@ -453,9 +453,16 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   h.fudge = 1.0;
   h.iz = 1;
   h.kind = 0;
-  h.fx = S.fx;
-  h.fy = S.fy;
-  h.fz = S.fz;
+  // own force arrays, added with the reciprocal ones after the side stream joins
+  h.fx = B.get<double>("fxc", n);
+  h.fy = B.get<double>("fyc", n);
+  h.fz = B.get<double>("fzc", n);
+  h.write = 1;
+  if (h.npairs > 0) {
+    m.cfx = h.fx;
+    m.cfy = h.fy;
+    m.cfz = h.fz;
+  }
   h.out = S.out + 36;
 
   // self energy (cself.f, kcut = 1e9): constant for fixed charges
@@ -572,6 +579,7 @@ static void batch_enqueue(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool events
     ORAC_HIP(hipEventRecord(b.fork, st));
     ORAC_HIP(hipStreamWaitEvent(ctx->side, b.fork, 0));
     pme(ctx->side);
+    if (b.any_corr) launch_corr(b.dc, b.hc.data(), n, ctx->side);
     ORAC_HIP(hipEventRecord(b.join, ctx->side));
   }
   if (md_lists)  // every system on its own update schedule (device step counters)
@@ -585,13 +593,15 @@ static void batch_enqueue(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool events
   else
     launch_pairs<float>(b.dq, b.hq.data(), n, st, b.tu, kb, ke);
   mark(ST_PAIRS);
-  if (side)
+  if (side) {
     ORAC_HIP(hipStreamWaitEvent(st, b.join, 0));
-  else
+    mark(ST_PME);
+  } else {
     pme(st);
+    mark(ST_PME);
+    if (b.any_corr) launch_corr(b.dc, b.hc.data(), n, st);
+  }
   launch_pme_combine(b.dp, b.hp.data(), npme, st);
-  mark(ST_PME);
-  if (b.any_corr) launch_corr(b.dc, b.hc.data(), n, st);
   mark(ST_END);
 }
 

@@ -185,7 +185,8 @@ struct CorrDesc {
   double *pe;                         // per pair energies (4 per pair: coul slt, coul slv, lj slt, lj slv)
   const int *g_off, *g_pair;          // per atom gather CSR: signed pair index (+/-(p+1))
   int natoms;
-  double *fx, *fy, *fz;               // accumulated (+=)
+  double *fx, *fy, *fz;               // accumulated (+=), or written (=) when write is set
+  int write;
   double *out;                        // [4]: coul slt, coul slv, lj slt, lj slv
   const double *q0;                   // ferrf_alchemy: lambda0 charges; en0 goes to the lj slt column
   const double *erf_tab;              // ferrf ERF_CORR: spline table (4, erf_n) over r = rlew + (i - 1) delew

@@ -743,9 +743,15 @@ __global__ void __launch_bounds__(256) k_corr_gather(const CorrDesc *__restrict_
     sy += s * d.pf[3 * (size_t)p + 1];
     sz += s * d.pf[3 * (size_t)p + 2];
   }
-  d.fx[a] += sx;
-  d.fy[a] += sy;
-  d.fz[a] += sz;
+  if (d.write) {
+    d.fx[a] = sx;
+    d.fy[a] = sy;
+    d.fz[a] = sz;
+  } else {
+    d.fx[a] += sx;
+    d.fy[a] += sy;
+    d.fz[a] += sz;
+  }
 }
 
 // d.out[4]: coul(slt), coul(slv), lj(slt), lj(slv); block k reduces column k in fixed order

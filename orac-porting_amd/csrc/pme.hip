@@ -800,9 +800,15 @@ __global__ void __launch_bounds__(256) k_pme_combine(const PmeDesc *__restrict__
   const PmeDesc &d = D[blockIdx.y];
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (!d.fx_dst || a >= d.natoms) return;
-  d.fx_dst[a] += d.fx[a];
-  d.fy_dst[a] += d.fy[a];
-  d.fz_dst[a] += d.fz[a];
+  double x = d.fx_dst[a] + d.fx[a], y = d.fy_dst[a] + d.fy[a], z = d.fz_dst[a] + d.fz[a];
+  if (d.cfx) {  // the corrections, in the order of a sequential evaluation
+    x += d.cfx[a];
+    y += d.cfy[a];
+    z += d.cfz[a];
+  }
+  d.fx_dst[a] = x;
+  d.fy_dst[a] = y;
+  d.fz_dst[a] = z;
 }
 
 void launch_pme_combine(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st) {
