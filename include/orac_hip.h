@@ -206,6 +206,27 @@ int orac_hip_fptors(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *l
                     const double *yp0, const double *zp0, const double *pota, const double *potb, double *utors_slt,
                     double *utors_slv, double *fpx, double *fpy, double *fpz, int32_t mb);
 
+/* Replaces fpbond (fpbond.f:1-110) as mts_intra_n0.f:73-74 calls it:
+ * harmonic bonds lbnd(2, lbond) on the coordinates as given, pota the rest
+ * length (potbo(1,2)) and potb the constant (potbo(1,1)); mb atoms.  Forces
+ * accumulate, energies (by the ss_index of each bond's first atom) are
+ * overwritten.  The reference's icnstr and itask select OpenMP work and have
+ * no counterpart. */
+int orac_hip_fpbond(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *lbnd, int32_t lbond, const double *xp0,
+                    const double *yp0, const double *zp0, const double *pota, const double *potb, double *ubond_slt,
+                    double *ubond_slv, double *fpx, double *fpy, double *fpz, int32_t mb);
+
+/* Replaces fpbend (fpbend.f:1-228) as mts_intra_n0.f:82-85 calls it: bends
+ * lbndg(3, lbend) (vertex second), pota the rest angle in radians, potb its
+ * constant, potc/potd the Urey-Bradley rest distance and constant, and
+ * pote/potf/potg the stretch-stretch coupling, with the reference's rules for
+ * which of these terms apply; a rest angle within 0.01 rad of pi uses the
+ * 2 potb (cos + 1) form.  Forces accumulate, energies are overwritten. */
+int orac_hip_fpbend(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *lbndg, int32_t lbend, const double *xp0,
+                    const double *yp0, const double *zp0, const double *pota, const double *potb, const double *potc,
+                    const double *potd, const double *pote, const double *potf, const double *potg,
+                    double *ubend_slt, double *ubend_slv, double *fpx, double *fpy, double *fpz, int32_t mb);
+
 /* Replaces fnb14 (fnb14.f:1-219): the 1-4 pairs int14(2, int14p) on the
  * coordinates as given (no PBC), LJ with the packed index type14(i) scaled
  * by lj_fudge, Coulomb scaled by fudge, erfc (A&S 7.1.26) when ewald and

@@ -842,6 +842,90 @@ extern "C" int orac_hip_cself_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index
   });
 }
 
+// Bonded terms of `arity` atoms each (lst(arity, n), 1-based) on the
+// coordinates as given: per-term force components on the device, gathered per
+// atom in term order and added to fp; e = energies by the class of each term's
+// first atom.  pots: pota, potb (and potc..potg for bends).  Returns the
+// kernel's status word (torsions: 1 singular, 2 unsupported multiplicity).
+static int bonded_eval(orac_hip_ctx *ctx, const char *name, int arity, const int32_t *ss_index, const int32_t *lst,
+                       int32_t n, const double *xp0, const double *yp0, const double *zp0,
+                       const double *const *pots, int npots, double *fpx, double *fpy, double *fpz, int32_t mb,
+                       double e[2]) {
+  ORAC_REQUIRE(ss_index && lst && xp0 && yp0 && zp0 && fpx && fpy && fpz && mb > 0 && n > 0, E_ARG, "null argument");
+  for (int k = 0; k < npots; ++k) ORAC_REQUIRE(pots[k], E_ARG, "null argument");
+  hipStream_t st = ctx->stream;
+  BufSet &B = ctx->B;
+  std::vector<int4> at(n);
+  std::vector<std::vector<int>> per(mb);
+  for (int i = 0; i < n; ++i) {
+    int l[4] = {0, 0, 0, 0};
+    for (int k = 0; k < arity; ++k) {
+      l[k] = lst[arity * i + k] - 1;
+      ORAC_REQUIRE(l[k] >= 0 && l[k] < mb, E_ARG, std::string(name) + ": atom index out of range");
+      per[l[k]].push_back(4 * i + k);
+    }
+    at[i] = make_int4(l[0], l[1], l[2], l[3]);
+  }
+  for (int a = 0; a < mb; ++a) ORAC_REQUIRE(ss_index[a] == 1 || ss_index[a] == 2, E_ARG, "ss_index must be 1 or 2");
+  std::vector<int> goff(mb + 1, 0), gs;
+  for (int a = 0; a < mb; ++a) {
+    for (int v : per[a]) gs.push_back(v);
+    goff[a + 1] = (int)gs.size();
+  }
+  static const char *pn[7] = {"t_pa", "t_pb", "t_pc", "t_pd", "t_pe", "t_pf", "t_pg"};
+  const double *dp[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (int k = 0; k < npots; ++k) dp[k] = B.put(pn[k], pots[k], n, st, false);
+  TorsDesc h;
+  std::memset(&h, 0, sizeof h);
+  h.ntors = n;
+  h.natoms = mb;
+  h.arity = arity;
+  h.atoms = B.put_vec("t_at", at, st, false);
+  h.pota = dp[0];
+  h.potb = dp[1];
+  h.potc = dp[2];
+  h.potd = dp[3];
+  h.pote = dp[4];
+  h.potf = dp[5];
+  h.potg = dp[6];
+  h.x = B.put("t_x", xp0, mb, st, false);
+  h.y = B.put("t_y", yp0, mb, st, false);
+  h.z = B.put("t_z", zp0, mb, st, false);
+  h.ss = B.put("t_ss", ss_index, mb, st, true);
+  h.tf = B.get<double>("t_tf", 12 * (size_t)n);
+  h.te = B.get<double>("t_te", 2 * (size_t)n);
+  h.status = B.get<int>("t_status", 2);
+  launch_zero_u64(reinterpret_cast<unsigned long long *>(h.status), 1, st);
+  h.g_off = B.put_vec("t_goff", goff, st, false);
+  h.g_slot = B.put_vec("t_gs", gs, st, false);
+  h.fx = B.get<double>("t_fx", mb);
+  h.fy = B.get<double>("t_fy", mb);
+  h.fz = B.get<double>("t_fz", mb);
+  launch_zero_f64(h.fx, mb, st);
+  launch_zero_f64(h.fy, mb, st);
+  launch_zero_f64(h.fz, mb, st);
+  double *dout = B.get<double>("t_out", 2);
+  h.out = dout;
+  TorsDesc *dd = B.put("t_desc", &h, 1, st, false);
+  launch_tors(dd, h, st);
+  ORAC_HIP(hipGetLastError());
+  std::vector<double> fx(mb), fy(mb), fz(mb);
+  int status = 0;
+  ORAC_HIP(hipMemcpyAsync(fx.data(), h.fx, mb * sizeof(double), hipMemcpyDeviceToHost, st));
+  ORAC_HIP(hipMemcpyAsync(fy.data(), h.fy, mb * sizeof(double), hipMemcpyDeviceToHost, st));
+  ORAC_HIP(hipMemcpyAsync(fz.data(), h.fz, mb * sizeof(double), hipMemcpyDeviceToHost, st));
+  ORAC_HIP(hipMemcpyAsync(e, dout, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  ORAC_HIP(hipMemcpyAsync(&status, h.status, sizeof(int), hipMemcpyDeviceToHost, st));
+  ORAC_HIP(hipStreamSynchronize(st));
+  if (status == 0)
+    for (int a = 0; a < mb; ++a) {
+      fpx[a] = fpx[a] + fx[a];
+      fpy[a] = fpy[a] + fy[a];
+      fpz[a] = fpz[a] + fz[a];
+    }
+  return status;
+}
+
 extern "C" int orac_hip_fptors(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *ltor, int32_t ltors,
                                const double *xp0, const double *yp0, const double *zp0, const double *pota,
                                const double *potb, double *utors_slt, double *utors_slv, double *fpx, double *fpy,
@@ -851,74 +935,53 @@ extern "C" int orac_hip_fptors(orac_hip_ctx *ctx, const int32_t *ss_index, const
     ORAC_REQUIRE(utors_slt && utors_slv, E_ARG, "null output");
     *utors_slt = *utors_slv = 0.0;
     if (ltors == 0) return;
-    ORAC_REQUIRE(ss_index && ltor && xp0 && yp0 && zp0 && pota && potb && fpx && fpy && fpz && mb > 0 && ltors > 0,
-                 E_ARG, "null argument");
-    hipStream_t st = ctx->stream;
-    BufSet &B = ctx->B;
-    std::vector<int4> at(ltors);
-    std::vector<std::vector<int>> per(mb);
-    for (int i = 0; i < ltors; ++i) {
-      int l[4];
-      for (int k = 0; k < 4; ++k) {
-        l[k] = ltor[4 * i + k] - 1;
-        ORAC_REQUIRE(l[k] >= 0 && l[k] < mb, E_ARG, "fptors: torsion atom out of range");
-        per[l[k]].push_back(4 * i + k);
-      }
-      at[i] = make_int4(l[0], l[1], l[2], l[3]);
-    }
-    for (int a = 0; a < mb; ++a) ORAC_REQUIRE(ss_index[a] == 1 || ss_index[a] == 2, E_ARG, "ss_index must be 1 or 2");
-    std::vector<int> goff(mb + 1, 0), gs;
-    for (int a = 0; a < mb; ++a) {
-      for (int v : per[a]) gs.push_back(v);
-      goff[a + 1] = (int)gs.size();
-    }
-    TorsDesc h;
-    std::memset(&h, 0, sizeof h);
-    h.ntors = ltors;
-    h.natoms = mb;
-    h.atoms = B.put_vec("t_at", at, st, false);
-    h.pota = B.put("t_pa", pota, ltors, st, false);
-    h.potb = B.put("t_pb", potb, ltors, st, false);
-    h.x = B.put("t_x", xp0, mb, st, false);
-    h.y = B.put("t_y", yp0, mb, st, false);
-    h.z = B.put("t_z", zp0, mb, st, false);
-    h.ss = B.put("t_ss", ss_index, mb, st, true);
-    h.tf = B.get<double>("t_tf", 12 * (size_t)ltors);
-    h.te = B.get<double>("t_te", 2 * (size_t)ltors);
-    h.status = B.get<int>("t_status", 2);
-    launch_zero_u64(reinterpret_cast<unsigned long long *>(h.status), 1, st);
-    h.g_off = B.put_vec("t_goff", goff, st, false);
-    h.g_slot = B.put_vec("t_gs", gs, st, false);
-    h.fx = B.get<double>("t_fx", mb);
-    h.fy = B.get<double>("t_fy", mb);
-    h.fz = B.get<double>("t_fz", mb);
-    launch_zero_f64(h.fx, mb, st);
-    launch_zero_f64(h.fy, mb, st);
-    launch_zero_f64(h.fz, mb, st);
-    double *dout = B.get<double>("t_out", 2);
-    h.out = dout;
-    TorsDesc *dd = B.put("t_desc", &h, 1, st, false);
-    launch_tors(dd, h, st);
-    ORAC_HIP(hipGetLastError());
-    std::vector<double> fx(mb), fy(mb), fz(mb);
+    ORAC_REQUIRE(ltors > 0, E_ARG, "negative count");
+    const double *pots[2] = {pota, potb};
     double e[2];
-    int status = 0;
-    ORAC_HIP(hipMemcpyAsync(fx.data(), h.fx, mb * sizeof(double), hipMemcpyDeviceToHost, st));
-    ORAC_HIP(hipMemcpyAsync(fy.data(), h.fy, mb * sizeof(double), hipMemcpyDeviceToHost, st));
-    ORAC_HIP(hipMemcpyAsync(fz.data(), h.fz, mb * sizeof(double), hipMemcpyDeviceToHost, st));
-    ORAC_HIP(hipMemcpyAsync(e, dout, sizeof e, hipMemcpyDeviceToHost, st));
-    ORAC_HIP(hipMemcpyAsync(&status, h.status, sizeof(int), hipMemcpyDeviceToHost, st));
-    ORAC_HIP(hipStreamSynchronize(st));
+    const int status = bonded_eval(ctx, "fptors", 4, ss_index, ltor, ltors, xp0, yp0, zp0, pots, 2, fpx, fpy, fpz,
+                                   mb, e);
     ORAC_REQUIRE(status != 2, E_UNSUPPORTED, "fptors: torsion multiplicity outside 1-8, 12, 14");
     ORAC_REQUIRE(status != 1, E_STATE,
                  "fptors: a torsion with collinear atoms (the reference perturbs it at random; not reproduced)");
-    for (int a = 0; a < mb; ++a) {
-      fpx[a] = fpx[a] + fx[a];
-      fpy[a] = fpy[a] + fy[a];
-      fpz[a] = fpz[a] + fz[a];
-    }
     *utors_slt = e[0];
     *utors_slv = e[1];
+  });
+}
+
+extern "C" int orac_hip_fpbond(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *lbnd, int32_t lbond,
+                               const double *xp0, const double *yp0, const double *zp0, const double *pota,
+                               const double *potb, double *ubond_slt, double *ubond_slv, double *fpx, double *fpy,
+                               double *fpz, int32_t mb) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(ubond_slt && ubond_slv, E_ARG, "null output");
+    *ubond_slt = *ubond_slv = 0.0;
+    if (lbond == 0) return;
+    ORAC_REQUIRE(lbond > 0, E_ARG, "negative count");
+    const double *pots[2] = {pota, potb};
+    double e[2];
+    bonded_eval(ctx, "fpbond", 2, ss_index, lbnd, lbond, xp0, yp0, zp0, pots, 2, fpx, fpy, fpz, mb, e);
+    *ubond_slt = e[0];
+    *ubond_slv = e[1];
+  });
+}
+
+extern "C" int orac_hip_fpbend(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *lbndg, int32_t lbend,
+                               const double *xp0, const double *yp0, const double *zp0, const double *pota,
+                               const double *potb, const double *potc, const double *potd, const double *pote,
+                               const double *potf, const double *potg, double *ubend_slt, double *ubend_slv,
+                               double *fpx, double *fpy, double *fpz, int32_t mb) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(ubend_slt && ubend_slv, E_ARG, "null output");
+    *ubend_slt = *ubend_slv = 0.0;
+    if (lbend == 0) return;
+    ORAC_REQUIRE(lbend > 0, E_ARG, "negative count");
+    const double *pots[7] = {pota, potb, potc, potd, pote, potf, potg};
+    double e[2];
+    bonded_eval(ctx, "fpbend", 3, ss_index, lbndg, lbend, xp0, yp0, zp0, pots, 7, fpx, fpy, fpz, mb, e);
+    *ubend_slt = e[0];
+    *ubend_slv = e[1];
   });
 }
 

@@ -1,6 +1,9 @@
-// bonded.hip -- proper torsions (fptors.f:1-345) on gfx950.
+// bonded.hip -- proper torsions (fptors.f:1-345), harmonic bonds
+// (fpbond.f:1-110) and bends with their Urey-Bradley and stretch-stretch
+// coupling terms (fpbend.f:1-228) on gfx950.
 //
-// One thread per torsion computes the dihedral cosine from the three bond
+// One thread per term (a torsion is described here; bonds and bends follow
+// the same scheme with two and three atoms) computes the dihedral cosine from the three bond
 // angles, the energy and the scalar force of its multiplicity, and the
 // derivative vectors for its four atoms, and stores the twelve force
 // components; a per-atom gather then adds the components of every torsion
@@ -143,6 +146,113 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
   }
 }
 
+// Bonds (arity 2) and bends (arity 3): the force components of each term in
+// the torsion layout (x of l1..l4, then y, z; the unused slots stay zero).
+__global__ void __launch_bounds__(256) k_bond_bend_terms(const TorsDesc *__restrict__ D) {
+#pragma clang fp contract(off)
+  const TorsDesc &d = *D;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.ntors) return;
+  double *tf = d.tf + 12 * (size_t)i;
+  double *te = d.te + 2 * (size_t)i;
+  for (int k = 0; k < 12; ++k) tf[k] = 0.0;
+  te[0] = te[1] = 0.0;
+  const int4 l = d.atoms[i];
+  const int type = d.ss[l.x] - 1;
+  if (d.arity == 2) {  // fpbond.f:60-80
+    const double x21 = d.x[l.y] - d.x[l.x], y21 = d.y[l.y] - d.y[l.x], z21 = d.z[l.y] - d.z[l.x];
+    const double rs21 = sqrt(x21 * x21 + y21 * y21 + z21 * z21);
+    const double pa = d.pota[i], pb = d.potb[i];
+    const double qforce = -2.0 * pb * (pa - rs21) / rs21;
+    const double u[3] = {qforce * x21, qforce * y21, qforce * z21};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      tf[4 * c + 0] = u[c];
+      tf[4 * c + 1] = -u[c];
+    }
+    const double dr = rs21 - pa;
+    te[type] = pb * (dr * dr);
+    return;
+  }
+  // fpbend.f:88-215
+  const double pb = d.potb[i];
+  if (fabs(pb) < 1.0e-12) return;
+  const double pa = d.pota[i], pc = d.potc[i], pd = d.potd[i], pe = d.pote[i], pf = d.potf[i], pg = d.potg[i];
+  const double r12[3] = {d.x[l.x] - d.x[l.y], d.y[l.x] - d.y[l.y], d.z[l.x] - d.z[l.y]};
+  const double r32[3] = {d.x[l.z] - d.x[l.y], d.y[l.z] - d.y[l.y], d.z[l.z] - d.z[l.y]};
+  const double rs12 = r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2];
+  const double rs32 = r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2];
+  const double dcc2 = sqrt(rs12 * rs32);
+  double cb = (r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2]) / dcc2;
+  if (fabs(cb) > 1.0) cb = cb >= 0.0 ? 1.0 : -1.0;
+  const double sb = sqrt(1.0 - cb * cb);
+  const double bb = acos(cb);
+  const double pi = 3.141592653589793;
+  const bool linear = fabs(pa - pi) < (double)0.01f;  // default-real literal
+  const double qforce = (linear || sb < 1.0e-6) ? -2.0 * pb : 2.0 * pb * (bb - pa) / sb;
+  double f1[3], f2[3], f3[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double u1 = r32[c] / dcc2 - cb * r12[c] / rs12;
+    const double u3 = r12[c] / dcc2 - cb * r32[c] / rs32;
+    f1[c] = qforce * u1;
+    f3[c] = qforce * u3;
+    f2[c] = qforce * (-u1 - u3);
+  }
+  double e;
+  if (linear) {
+    e = 2.0 * pb * (cb + 1.0);
+  } else {
+    const double da = bb - pa;
+    e = pb * (da * da);
+  }
+  if (fabs(pd) > 1.0e-5 && (fabs(pe) < 1.0e-5 || fabs(pf) > 1.0e-5)) {  // Urey-Bradley 1-3 term
+    const double r31[3] = {r32[0] - r12[0], r32[1] - r12[1], r32[2] - r12[2]};
+    const double rsp31 = sqrt(r31[0] * r31[0] + r31[1] * r31[1] + r31[2] * r31[2]);
+    const double qf = -2.0 * pd * (pc - rsp31);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      f1[c] += qf * (r31[c] / rsp31);
+      f3[c] -= qf * (r31[c] / rsp31);
+    }
+    const double dr = rsp31 - pc;
+    e += pd * (dr * dr);
+  }
+  bool coupling = false;
+  double k12 = 0.0, q1 = 0.0, q2 = 0.0;
+  if (fabs(pd) > 1.0e-5 && fabs(pe) > 1.0e-5 && fabs(pf) < 1.0e-5) {
+    coupling = true;
+    k12 = pd;
+    q1 = pc;
+    q2 = pe;
+  }
+  if (fabs(pe) > 1.0e-5 && fabs(pf) > 1.0e-5 && fabs(pg) > 1.0e-5) {
+    coupling = true;
+    k12 = pe;
+    q1 = pf;
+    q2 = pg;
+  }
+  if (coupling) {  // stretch-stretch k12 (r12 - r1)(r32 - r2)
+    const double rsp12 = sqrt(rs12), rsp32 = sqrt(rs32);
+    const double qf = k12 * (rsp12 - q1), pfc = k12 * (rsp32 - q2);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double a1 = r12[c] / rsp12, a2 = r32[c] / rsp32;
+      f1[c] -= pfc * a1;
+      f2[c] += pfc * a1 + qf * a2;
+      f3[c] -= qf * a2;
+    }
+    e += k12 * (rsp12 - q1) * (rsp32 - q2);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    tf[4 * c + 0] = f1[c];
+    tf[4 * c + 1] = f2[c];
+    tf[4 * c + 2] = f3[c];
+  }
+  te[type] = e;
+}
+
 __global__ void __launch_bounds__(256) k_tors_gather(const TorsDesc *__restrict__ D) {
   const TorsDesc &d = *D;
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -173,7 +283,12 @@ __global__ void __launch_bounds__(256) k_tors_energy(const TorsDesc *__restrict_
 }
 
 void launch_tors(const TorsDesc *d_desc, const TorsDesc &h, hipStream_t st) {
-  if (h.ntors > 0) hipLaunchKernelGGL(k_tors_terms, dim3((h.ntors + 255) / 256), dim3(256), 0, st, d_desc);
+  if (h.ntors > 0) {
+    if (h.arity == 4)
+      hipLaunchKernelGGL(k_tors_terms, dim3((h.ntors + 255) / 256), dim3(256), 0, st, d_desc);
+    else
+      hipLaunchKernelGGL(k_bond_bend_terms, dim3((h.ntors + 255) / 256), dim3(256), 0, st, d_desc);
+  }
   if (h.ntors > 0 && h.natoms > 0)
     hipLaunchKernelGGL(k_tors_gather, dim3((h.natoms + 255) / 256), dim3(256), 0, st, d_desc);
   hipLaunchKernelGGL(k_tors_energy, dim3(2), dim3(256), 0, st, d_desc);

@@ -201,11 +201,13 @@ void launch_corr(const CorrDesc *d_desc, const CorrDesc *h, int nsys, hipStream_
 // forces and its energy; a per-atom gather sums them in a fixed order.
 struct TorsDesc {
   int ntors, natoms;
-  const int4 *atoms;                  // 0-based l1..l4
-  const double *pota, *potb;          // signed multiplicity, barrier
+  const int4 *atoms;                  // 0-based l1..l4 (the first arity used)
+  const double *pota, *potb;          // torsions: signed multiplicity, barrier; bonds, bends: rest value, constant
+  const double *potc, *potd, *pote, *potf, *potg;  // bends: Urey-Bradley and coupling parameters
+  int arity;                          // atoms per term: 2 bonds, 3 bends, 4 torsions
   const double *x, *y, *z;            // coordinates as given
   const int *ss;                      // ss_index of each atom
-  double *tf;                         // [ntors][12] force components: x of l1..l4, y, z
+  double *tf;                         // [ntors][12] force components: x of l1..l4, y, z (unused slots zero)
   double *te;                         // [ntors][2] energy by the class of l1
   int *status;                        // [1]: 1 singular torsion, 2 unsupported multiplicity (sticky)
   const int *g_off, *g_slot;          // per atom gather CSR: 4 * torsion + position

@@ -142,6 +142,23 @@ module orac_hip_iface
       real(c_double), value :: alphal, kcut
       real(c_double) :: charge(*), charge0(*), self_slt, dwrk
     end function
+    integer(c_int) function c_fpbond(ctx, ss_index, lbnd, lbond, xp0, yp0, zp0, pota, potb, ub_slt, ub_slv, fpx, &
+         fpy, fpz, mb) bind(C, name="orac_hip_fpbond")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), lbnd(*)
+      integer(c_int32_t), value :: lbond, mb
+      real(c_double) :: xp0(*), yp0(*), zp0(*), pota(*), potb(*), ub_slt, ub_slv, fpx(*), fpy(*), fpz(*)
+    end function
+    integer(c_int) function c_fpbend(ctx, ss_index, lbndg, lbend, xp0, yp0, zp0, pota, potb, potc, potd, pote, &
+         potf, potg, ub_slt, ub_slv, fpx, fpy, fpz, mb) bind(C, name="orac_hip_fpbend")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), lbndg(*)
+      integer(c_int32_t), value :: lbend, mb
+      real(c_double) :: xp0(*), yp0(*), zp0(*), pota(*), potb(*), potc(*), potd(*), pote(*), potf(*), potg(*)
+      real(c_double) :: ub_slt, ub_slv, fpx(*), fpy(*), fpz(*)
+    end function
     integer(c_int) function c_fptors(ctx, ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, ut_slt, ut_slv, fpx, &
          fpy, fpz, mb) bind(C, name="orac_hip_fptors")
       import :: c_ptr, c_int, c_int32_t, c_double
@@ -338,6 +355,29 @@ subroutine fnbgrp(ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alp
   logical :: ewald
   call check(c_fnbgrp(ctx(), ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, lg(ewald), alphal, ingrp, ingrpp, &
        fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, nato))
+end subroutine
+
+! fpbond.f:1-2 (mts_intra_n0.f:73-74); icnstr and itask select OpenMP work in the reference
+subroutine fpbond(ss_index, lbnd, lbond, xp0, yp0, zp0, pota, potb, ubond_slt, ubond_slv, fpx, fpy, fpz, icnstr, &
+     mb, itask)
+  use orac_hip_iface
+  implicit none
+  integer :: lbond, icnstr, mb, itask, lbnd(2,*), ss_index(*)
+  real(8) :: ubond_slt, ubond_slv, pota(*), potb(*), xp0(*), yp0(*), zp0(*), fpx(mb), fpy(mb), fpz(mb)
+  call check(c_fpbond(ctx(), ss_index, lbnd, lbond, xp0, yp0, zp0, pota, potb, ubond_slt, ubond_slv, fpx, fpy, fpz, &
+       mb))
+end subroutine
+
+! fpbend.f:1-3 (mts_intra_n0.f:82-85)
+subroutine fpbend(ss_index, lbndg, lbend, xp0, yp0, zp0, pota, potb, potc, potd, pote, potf, potg, ubend_slt, &
+     ubend_slv, fpx, fpy, fpz, mb, itask)
+  use orac_hip_iface
+  implicit none
+  integer :: lbend, mb, itask, lbndg(3,*), ss_index(*)
+  real(8) :: ubend_slt, ubend_slv, xp0(*), yp0(*), zp0(*), fpx(mb), fpy(mb), fpz(mb)
+  real(8) :: pota(*), potb(*), potc(*), potd(*), pote(*), potf(*), potg(*)
+  call check(c_fpbend(ctx(), ss_index, lbndg, lbend, xp0, yp0, zp0, pota, potb, potc, potd, pote, potf, potg, &
+       ubend_slt, ubend_slv, fpx, fpy, fpz, mb))
 end subroutine
 
 ! fptors.f:1-2 (mts_intra_n1.f:118-119)

@@ -76,7 +76,9 @@ def read_tpgprm(path, amphi=False):
     """Topology-parameter dump, readtp.f:45-150 (the non-amphi potential record
     unless amphi).  Arrays come back 1-based as stored, Fortran layouts as
     (rows, n): grppt (2, ngrp), lbnd (2, lbond), lbndg (3, lbend), int14 (2, int14p),
-    int13 (2, int13p), ltor (4, ltors); potto is (ltors, 2): barrier, then the signed
+    int13 (2, int13p), ltor (4, ltors); potbo (lbond, 2) and potbe (lbend, 7) as
+    mts_intra_n0.f:73-85 passes them (fpbond: rest length potbo(:,2), constant
+    potbo(:,1); fpbend: potbe columns 2, 1, 4, 3, 5, 6, 7); potto is (ltors, 2): barrier, then the signed
     multiplicity (fptors receives potto(1,2) as pota and potto(1,1) as potb,
     mts_intra_n1.f:118-119).  Charges are in electrons, masses in amu."""
     r = _ListDirected(path)
@@ -111,6 +113,9 @@ def read_tpgprm(path, amphi=False):
     npot = 2 * n["lbond"] + 7 * n["lbend"] + (4 if amphi else 2) * n["ltors"] + 3 * n["litor"] + \
         (6 if amphi else 4) * L + 2 * n["lphyd"]
     p = r.reals(npot)
+    t["potbo"] = p[:2 * n["lbond"]].reshape(2, n["lbond"]).T      # potbo(lbond, 2): constant, rest length
+    o = 2 * n["lbond"]                                             # potbe(lbend, 7) (readtp.f:199-200)
+    t["potbe"] = p[o:o + 7 * n["lbend"]].reshape(7, n["lbend"]).T
     if not amphi:                                                  # potto(ltors, 2): barrier, multiplicity
         o = 2 * n["lbond"] + 7 * n["lbend"]
         t["potto"] = p[o:o + 2 * n["ltors"]].reshape(2, n["ltors"]).T

@@ -663,6 +663,59 @@ def ref_fptors(s, ltor, pota, potb, coords=None):
 
 
 
+def fpbond(s, lbnd, pota, potb, coords=None):
+    """fpbond.f on the Cartesian coordinates of System s (or coords): bonds
+    lbnd (2, n) 1-based, pota = rest length, potb = force constant."""
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b = C.c_double(), C.c_double()
+    lb = _i(lbnd)
+    lib().orc_fpbond(_p(_i(s.ss_index)), _p(lb), C.c_int(lb.size // 2), _p(_f(x)), _p(_f(y)), _p(_f(z)),
+                     _p(_f(pota)), _p(_f(potb)), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz))
+    return dict(ubond_slt=a.value, ubond_slv=b.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def ref_fpbond(s, lbnd, pota, potb, coords=None):
+    R = ref()
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b = C.c_double(), C.c_double()
+    keep = [_i(s.ss_index), _i(lbnd), _f(x), _f(y), _f(z), _f(pota), _f(potb)]
+    R.fpbond_(_p(keep[0]), _p(keep[1]), _ri(keep[1].size // 2), _p(keep[2]), _p(keep[3]), _p(keep[4]), _p(keep[5]),
+              _p(keep[6]), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _ri(0), _ri(n), _ri(1))
+    return dict(ubond_slt=a.value, ubond_slv=b.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def fpbend(s, lbndg, pots, coords=None):
+    """fpbend.f on the Cartesian coordinates of System s (or coords): bends
+    lbndg (3, n) 1-based, pots = (pota, potb, potc, potd, pote, potf, potg) as
+    fpbend receives them (rest angle, constant, Urey-Bradley rest distance and
+    constant, coupling parameters)."""
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b = C.c_double(), C.c_double()
+    lb = _i(lbndg)
+    pk = [_f(v) for v in pots]
+    lib().orc_fpbend(_p(_i(s.ss_index)), _p(lb), C.c_int(lb.size // 3), _p(_f(x)), _p(_f(y)), _p(_f(z)),
+                     *[_p(v) for v in pk], C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz))
+    return dict(ubend_slt=a.value, ubend_slv=b.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def ref_fpbend(s, lbndg, pots, coords=None):
+    R = ref()
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b = C.c_double(), C.c_double()
+    keep = [_i(s.ss_index), _i(lbndg), _f(x), _f(y), _f(z)] + [_f(v) for v in pots]
+    R.fpbend_(_p(keep[0]), _p(keep[1]), _ri(keep[1].size // 3), _p(keep[2]), _p(keep[3]), _p(keep[4]),
+              *[_p(v) for v in keep[5:]], C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _ri(n), _ri(1))
+    return dict(ubend_slt=a.value, ubend_slv=b.value, f=np.stack([fpx, fpy, fpz]))
+
+
 def ref_int_corr_erf_spline(rmin, rmax, nbin, alpha, kcut):
     """The reference's ERF_CORR table builder: (erf_arr_corr (4, nbin + 1), tau)."""
     tab = np.zeros((4, nbin + 1), order="F")

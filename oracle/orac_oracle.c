@@ -1904,3 +1904,125 @@ int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const do
   *utors_slv = utors[1];
   return status;
 }
+
+/* fpbond.f:1-110 (serial): harmonic bonds lbnd(2, lbond), pota the rest
+ * length and potb the force constant; forces accumulate, energies by the
+ * ss_index of each bond's first atom. */
+void orc_fpbond(const int32_t *ss_index, const int32_t *lbnd, int lbond, const double *xp0, const double *yp0,
+                const double *zp0, const double *pota, const double *potb, double *ubond_slt, double *ubond_slv,
+                double *fpx, double *fpy, double *fpz) {
+  double ub[2] = {0.0, 0.0};
+  for (int i = 0; i < lbond; ++i) {
+    const int la = lbnd[2 * i] - 1, lb = lbnd[2 * i + 1] - 1;
+    const int type = ss_index[la];
+    const double x21 = xp0[lb] - xp0[la], y21 = yp0[lb] - yp0[la], z21 = zp0[lb] - zp0[la];
+    const double rs21 = sqrt(x21 * x21 + y21 * y21 + z21 * z21);
+    const double qforce = -2.0 * potb[i] * (pota[i] - rs21) / rs21;
+    const double ux = qforce * x21, uy = qforce * y21, uz = qforce * z21;
+    fpx[la] = fpx[la] + ux;
+    fpy[la] = fpy[la] + uy;
+    fpz[la] = fpz[la] + uz;
+    fpx[lb] = fpx[lb] - ux;
+    fpy[lb] = fpy[lb] - uy;
+    fpz[lb] = fpz[lb] - uz;
+    const double d = rs21 - pota[i];
+    ub[type - 1] = ub[type - 1] + potb[i] * (d * d);
+  }
+  *ubond_slt = ub[0];
+  *ubond_slv = ub[1];
+}
+
+/* fpbend.f:1-228 (serial): bends lbndg(3, lbend) with the vertex in the
+ * middle; pota the rest angle (rad), potb its constant; the Urey-Bradley 1-3
+ * term (potc rest distance, potd constant) when potd is set and pote is not,
+ * or potf is; the stretch-stretch coupling (k12, r1, r2) = (potd, potc, pote)
+ * or (pote, potf, potg) when those are set.  A rest angle within 0.01 of pi
+ * uses 2 potb (cos + 1). */
+void orc_fpbend(const int32_t *ss_index, const int32_t *lbndg, int lbend, const double *xp0, const double *yp0,
+                const double *zp0, const double *pota, const double *potb, const double *potc, const double *potd,
+                const double *pote, const double *potf, const double *potg, double *ubend_slt, double *ubend_slv,
+                double *fpx, double *fpy, double *fpz) {
+  double ub[2] = {0.0, 0.0};
+  const double pi = 4.0 * atan(1.0);
+  const double near_pi = (double)0.01f; /* default-real literal (fpbend.f) */
+  for (int i = 0; i < lbend; ++i) {
+    if (fabs(potb[i]) < 1.0e-12) continue;
+    const int la = lbndg[3 * i] - 1, lb = lbndg[3 * i + 1] - 1, lc = lbndg[3 * i + 2] - 1;
+    const int type = ss_index[la] - 1;
+    const double x12 = xp0[la] - xp0[lb], y12 = yp0[la] - yp0[lb], z12 = zp0[la] - zp0[lb];
+    const double x32 = xp0[lc] - xp0[lb], y32 = yp0[lc] - yp0[lb], z32 = zp0[lc] - zp0[lb];
+    const double rs12 = x12 * x12 + y12 * y12 + z12 * z12;
+    const double rs32 = x32 * x32 + y32 * y32 + z32 * z32;
+    const double dcc2 = sqrt(rs12 * rs32);
+    double cb = (x12 * x32 + y12 * y32 + z12 * z32) / dcc2;
+    if (fabs(cb) > 1.0) cb = cb >= 0.0 ? 1.0 : -1.0;
+    const double sb = sqrt(1.0 - cb * cb);
+    const double bb = acos(cb);
+    const int linear = fabs(pota[i] - pi) < near_pi;
+    const double qforce = (linear || sb < 1.0e-6) ? -2.0 * potb[i] : 2.0 * potb[i] * (bb - pota[i]) / sb;
+    const double ux1 = x32 / dcc2 - cb * x12 / rs12, ux3 = x12 / dcc2 - cb * x32 / rs32, ux2 = -ux1 - ux3;
+    const double uy1 = y32 / dcc2 - cb * y12 / rs12, uy3 = y12 / dcc2 - cb * y32 / rs32, uy2 = -uy1 - uy3;
+    const double uz1 = z32 / dcc2 - cb * z12 / rs12, uz3 = z12 / dcc2 - cb * z32 / rs32, uz2 = -uz1 - uz3;
+    fpx[la] = fpx[la] + qforce * ux1;
+    fpx[lc] = fpx[lc] + qforce * ux3;
+    fpx[lb] = fpx[lb] + qforce * ux2;
+    fpy[la] = fpy[la] + qforce * uy1;
+    fpy[lc] = fpy[lc] + qforce * uy3;
+    fpy[lb] = fpy[lb] + qforce * uy2;
+    fpz[la] = fpz[la] + qforce * uz1;
+    fpz[lc] = fpz[lc] + qforce * uz3;
+    fpz[lb] = fpz[lb] + qforce * uz2;
+    if (linear) {
+      ub[type] = ub[type] + 2.0 * potb[i] * (cb + 1.0);
+    } else {
+      const double d = bb - pota[i];
+      ub[type] = ub[type] + potb[i] * (d * d);
+    }
+    if (fabs(potd[i]) > 1.0e-5 && (fabs(pote[i]) < 1.0e-5 || fabs(potf[i]) > 1.0e-5)) {
+      const double x31 = xp0[lc] - xp0[la], y31 = yp0[lc] - yp0[la], z31 = zp0[lc] - zp0[la];
+      const double rsp31 = sqrt(x31 * x31 + y31 * y31 + z31 * z31);
+      const double qf = -2.0 * potd[i] * (potc[i] - rsp31);
+      const double vx = x31 / rsp31, vy = y31 / rsp31, vz = z31 / rsp31;
+      fpx[la] = fpx[la] + qf * vx;
+      fpy[la] = fpy[la] + qf * vy;
+      fpz[la] = fpz[la] + qf * vz;
+      fpx[lc] = fpx[lc] + qf * -vx;
+      fpy[lc] = fpy[lc] + qf * -vy;
+      fpz[lc] = fpz[lc] + qf * -vz;
+      const double d = rsp31 - potc[i];
+      ub[type] = ub[type] + potd[i] * (d * d);
+    }
+    int coupling = 0;
+    double k12 = 0.0, r1 = 0.0, r2 = 0.0;
+    if (fabs(potd[i]) > 1.0e-5 && fabs(pote[i]) > 1.0e-5 && fabs(potf[i]) < 1.0e-5) {
+      coupling = 1;
+      k12 = potd[i];
+      r1 = potc[i];
+      r2 = pote[i];
+    }
+    if (fabs(pote[i]) > 1.0e-5 && fabs(potf[i]) > 1.0e-5 && fabs(potg[i]) > 1.0e-5) {
+      coupling = 1;
+      k12 = pote[i];
+      r1 = potf[i];
+      r2 = potg[i];
+    }
+    if (coupling) {
+      const double rsp12 = sqrt(rs12), rsp32 = sqrt(rs32);
+      const double ax = x12 / rsp12, ay = y12 / rsp12, az = z12 / rsp12;
+      const double cx = x32 / rsp32, cy = y32 / rsp32, cz = z32 / rsp32;
+      const double qf = k12 * (rsp12 - r1), pf = k12 * (rsp32 - r2);
+      fpx[la] = fpx[la] - pf * ax;
+      fpy[la] = fpy[la] - pf * ay;
+      fpz[la] = fpz[la] - pf * az;
+      fpx[lb] = fpx[lb] + pf * ax + qf * cx;
+      fpy[lb] = fpy[lb] + pf * ay + qf * cy;
+      fpz[lb] = fpz[lb] + pf * az + qf * cz;
+      fpx[lc] = fpx[lc] - qf * cx;
+      fpy[lc] = fpy[lc] - qf * cy;
+      fpz[lc] = fpz[lc] - qf * cz;
+      ub[type] = ub[type] + k12 * (rsp12 - r1) * (rsp32 - r2);
+    }
+  }
+  *ubend_slt = ub[0];
+  *ubend_slv = ub[1];
+}

@@ -138,3 +138,10 @@ void orc_fnb14(const int32_t *ss_index, const double *xp0, const double *yp0, co
 int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const double *xp0, const double *yp0,
                const double *zp0, const double *pota, const double *potb, double *utors_slt, double *utors_slv,
                double *fpx, double *fpy, double *fpz);
+void orc_fpbond(const int32_t *ss_index, const int32_t *lbnd, int lbond, const double *xp0, const double *yp0,
+                const double *zp0, const double *pota, const double *potb, double *ubond_slt, double *ubond_slv,
+                double *fpx, double *fpy, double *fpz);
+void orc_fpbend(const int32_t *ss_index, const int32_t *lbndg, int lbend, const double *xp0, const double *yp0,
+                const double *zp0, const double *pota, const double *potb, const double *potc, const double *potd,
+                const double *pote, const double *potf, const double *potg, double *ubend_slt, double *ubend_slv,
+                double *fpx, double *fpy, double *fpz);

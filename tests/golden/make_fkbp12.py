@@ -11,7 +11,9 @@ cutoff 10 A, list skin 1.8 A, 18^3 linked cells) and writes:
   the linked-cell list, force() on the outer shell, fft_pme (fftpack branch),
   ferrf on the solute's bonds and 1-3 pairs, fnbgrp (with and without
   Ewald) on the intra-group pairs and on the bond + 1-3 lists, fnb14 on the
-  topology's 1-4 pairs (both branches), fptors on its proper torsions, and
+  topology's 1-4 pairs (both branches), fptors on its proper torsions,
+  fpbond and fpbend on its bonds and bends (and on bends with every fpbend
+  branch set), and
   the ERF_CORR table (int_corr_erf_spline, erf_corr_cutoff) with ferrf on it.  Exact arrays are stored as
   SHA-256 digests of their bytes; energies, stress and a sample of the PME
   forces as values.
@@ -65,6 +67,27 @@ def digest(a):
     return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(), dtype=np.uint8)
 
 
+def bend_branches(bend):
+    """Bend parameters (fpbend order: pota..potg) with each fifth bend in one
+    of fpbend's branches: plain, Urey-Bradley, coupling (potd, potc, pote),
+    coupling (pote, potf, potg) with Urey-Bradley, rest angle pi."""
+    b = np.array(bend, dtype=float)
+    rng = np.random.default_rng(7)
+    n = b.shape[1]
+    k = np.arange(n) % 5
+    b[2:] = 0.0
+    ub = (k == 1) | (k == 3)
+    b[2, ub] = rng.uniform(2.2, 2.6, ub.sum())        # Urey-Bradley rest distance
+    b[3, ub] = rng.uniform(10.0, 40.0, ub.sum())      # and constant
+    c1 = k == 2
+    b[3, c1], b[2, c1], b[4, c1] = rng.uniform(5.0, 20.0, c1.sum()), rng.uniform(1.0, 1.5, c1.sum()), \
+        rng.uniform(1.0, 1.5, c1.sum())
+    b[4, k == 3], b[5, k == 3], b[6, k == 3] = rng.uniform(5.0, 20.0, (k == 3).sum()), \
+        rng.uniform(1.0, 1.5, (k == 3).sum()), rng.uniform(1.0, 1.5, (k == 3).sum())
+    b[0, k == 4] = np.pi
+    return b
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libref.so is missing: build it with `make -C oracle ref` where the reference exists")
@@ -116,6 +139,23 @@ def main():
     r = O.ref_fptors(s, top["ltor"], top["potto"][:, 1], top["potto"][:, 0])
     out["ref_fptors_e"] = np.array([r["utors_slt"], r["utors_slv"]])
     out["ref_fptors_f_sha"] = digest(r["f"])
+    # bonds and bends of the topology, as mts_intra_n0.f:73-85 passes them
+    # (rest angles in radians as covbd.f:54 converts them; constants as dumped),
+    # and a set of bends with every branch of fpbend set on purpose
+    # (Urey-Bradley, both coupling forms, the near-linear rest angle)
+    pb = top["potbo"]
+    out["lbnd"], out["bond_pots"] = top["lbnd"].astype(np.int16), np.stack([pb[:, 1], pb[:, 0]])
+    r = O.ref_fpbond(s, top["lbnd"], pb[:, 1], pb[:, 0])
+    out["ref_fpbond_e"], out["ref_fpbond_f_sha"] = np.array([r["ubond_slt"], r["ubond_slv"]]), digest(r["f"])
+    pe = top["potbe"]
+    bend = np.stack([pe[:, 1] * (np.pi / 180.0), pe[:, 0], pe[:, 3], pe[:, 2], pe[:, 4], pe[:, 5], pe[:, 6]])
+    out["lbndg"], out["bend_pots"] = top["lbndg"].astype(np.int16), bend
+    r = O.ref_fpbend(s, top["lbndg"], list(bend))
+    out["ref_fpbend_e"], out["ref_fpbend_f_sha"] = np.array([r["ubend_slt"], r["ubend_slv"]]), digest(r["f"])
+    out["bend_branch_pots"] = bend_branches(bend[:, :600])
+    r = O.ref_fpbend(s, top["lbndg"][:, :600], list(out["bend_branch_pots"]))
+    out["ref_fpbend_branch_e"] = np.array([r["ubend_slt"], r["ubend_slv"]])
+    out["ref_fpbend_branch_f_sha"] = digest(r["f"])
     # ERF_CORR: the reference's reciprocal-cutoff correction table
     # (ERF_CORR 300 0.5 3.5; rkcut from erf_corr_cutoff) and ferrf with it
     rk = O.ref_erf_corr_cutoff(s.oc, s.nfft)

@@ -210,3 +210,34 @@ def test_ferrf_erf_corr_matches_oracle(gpu_ctx, fk, iz):
     assert rel(a, o["slt"]) < 1e-12 and b == o["slv"]
     if iz == 1:
         assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
+
+
+def test_fpbond_matches_oracle(gpu_ctx, fk):
+    s, _, g = fk
+    x, y, z = s.cartesian()
+    n = s.natoms
+    lb = np.asfortranarray(g["lbnd"].astype(np.int32))
+    bp = g["bond_pots"]
+    fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+    a, b = gpu_ctx.fpbond(s.ss_index, lb, lb.shape[1], x, y, z, bp[0], bp[1], fpx, fpy, fpz, n)
+    o = O.fpbond(s, lb, bp[0], bp[1])
+    assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
+    assert rel(a, o["ubond_slt"]) < 1e-12 and rel(b, o["ubond_slv"]) < 1e-9
+
+
+@pytest.mark.parametrize("which", ["topology", "branch"])
+def test_fpbend_matches_oracle(gpu_ctx, fk, which):
+    """fpbend on the topology's bends and on bends with every branch set."""
+    s, _, g = fk
+    x, y, z = s.cartesian()
+    n = s.natoms
+    lb = g["lbndg"].astype(np.int32)
+    pots = g["bend_pots"]
+    if which == "branch":
+        lb, pots = lb[:, :600], g["bend_branch_pots"]
+    lb = np.asfortranarray(lb)
+    fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+    a, b = gpu_ctx.fpbend(s.ss_index, lb, lb.shape[1], x, y, z, *pots, fpx, fpy, fpz, n)
+    o = O.fpbend(s, lb, list(pots))
+    assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
+    assert rel(a, o["ubend_slt"]) < 1e-12 and b == o["ubend_slv"] == 0.0

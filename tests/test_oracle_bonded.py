@@ -1,4 +1,4 @@
-"""The oracle's bonded terms (fptors, fnb14) against what the mathematics
+"""The oracle's bonded terms (fptors, fnb14, fpbond, fpbend) against what the mathematics
 fixes, on small synthetic geometries: forces are minus the energy gradient
 (central differences) for every multiplicity fptors knows, the energy is
 invariant under rigid rotation and the forces of each term sum to zero; the
@@ -85,3 +85,72 @@ def test_one_four_pair_closed_form():
         assert abs(r["uconf_slt"] - e_lj) <= 1e-12 * abs(e_lj)
         assert abs(r["ucoul_slt"] - e_q) <= 1e-12 * abs(e_q)
         assert np.abs(r["f"].sum(axis=1)).max() <= 1e-12 * np.abs(r["f"]).max()
+
+
+def _bb_system(rng, n=3):
+    s, pos = _system(n, rng, scale=1.0)
+    return s, pos
+
+
+def _coords(s, pos):
+    n = s.natoms
+    x = np.zeros(n); y = np.zeros(n); z = np.zeros(n)
+    x[:len(pos)], y[:len(pos)], z[:len(pos)] = pos[:, 0], pos[:, 1], pos[:, 2]
+    return x, y, z
+
+
+def _grad_check(efun, pos, f, h=1e-6, tol=1e-6):
+    num = np.zeros_like(pos)
+    for a in range(pos.shape[0]):
+        for c in range(3):
+            p, m = pos.copy(), pos.copy()
+            p[a, c] += h
+            m[a, c] -= h
+            num[a, c] = -(efun(p) - efun(m)) / (2 * h)
+    assert np.abs(f.T - num).max() <= tol * max(1.0, np.abs(num).max())
+    assert np.abs(f.sum(axis=1)).max() <= 1e-9 * max(1.0, np.abs(f).max())
+
+
+def test_bond_force_is_minus_gradient_and_closed_form():
+    rng = np.random.default_rng(21)
+    s, pos = _bb_system(rng, 2)
+    lb = np.array([[1], [2]], np.int32)
+    r0, k = 1.3, 75.0
+
+    def run(p):
+        return O.fpbond(s, lb, [r0], [k], coords=_coords(s, p))
+
+    r = run(pos)
+    d = np.linalg.norm(pos[1] - pos[0])
+    assert math.isclose(r["ubond_slt"], k * (d - r0) ** 2, rel_tol=1e-14)
+    _grad_check(lambda p: run(p)["ubond_slt"], pos, r["f"][:, :2])
+
+
+@pytest.mark.parametrize("branch", ["plain", "urey_bradley", "coupling_d", "coupling_efg", "linear"])
+def test_bend_force_is_minus_gradient_and_closed_form(branch):
+    rng = np.random.default_rng({"plain": 1, "urey_bradley": 2, "coupling_d": 3, "coupling_efg": 4,
+                                 "linear": 5}[branch])
+    s, pos = _bb_system(rng, 3)
+    lb = np.array([[1], [2], [3]], np.int32)
+    th0, k = 1.9, 60.0
+    pots = {"plain": [th0, k, 0, 0, 0, 0, 0], "urey_bradley": [th0, k, 2.4, 30.0, 0, 0, 0],
+            "coupling_d": [th0, k, 1.2, 15.0, 1.4, 0, 0], "coupling_efg": [th0, k, 0, 0, 12.0, 1.1, 1.3],
+            "linear": [math.pi, k, 0, 0, 0, 0, 0]}[branch]
+
+    def run(p):
+        return O.fpbend(s, lb, [[v] for v in pots], coords=_coords(s, p))
+
+    r = run(pos)
+    a, b, c = pos
+    r12, r32 = np.linalg.norm(a - b), np.linalg.norm(c - b)
+    cos = np.dot(a - b, c - b) / (r12 * r32)
+    th = math.acos(cos)
+    e = 2 * k * (cos + 1) if branch == "linear" else k * (th - th0) ** 2
+    if branch == "urey_bradley":
+        e += 30.0 * (np.linalg.norm(c - a) - 2.4) ** 2
+    if branch == "coupling_d":
+        e += 15.0 * (r12 - 1.2) * (r32 - 1.4)
+    if branch == "coupling_efg":
+        e += 12.0 * (r12 - 1.1) * (r32 - 1.3)
+    assert math.isclose(r["ubend_slt"], e, rel_tol=1e-12)
+    _grad_check(lambda p: run(p)["ubend_slt"], pos, r["f"][:, :3])

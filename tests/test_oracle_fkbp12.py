@@ -171,3 +171,26 @@ def test_erf_corr_table_is_the_cutoff_integral(fk):
     assert np.abs(val - tab[0, 1:-1]).max() <= 1e-12 * np.abs(tab[0]).max()
     assert np.abs(der - tab[1, 1:-1]).max() <= 1e-9 * np.abs(tab[1]).max()
     assert np.abs(sec - tab[2, 1:-1]).max() <= 1e-7 * np.abs(tab[2]).max()
+
+
+def test_fpbond_matches_reference(fk):
+    """fpbond on the topology's 14,919 bonds (solute bonds and the waters' rigid ones)."""
+    s, _, g = fk
+    bp = g["bond_pots"]
+    r = O.fpbond(s, g["lbnd"].astype(np.int32), bp[0], bp[1])
+    assert np.array_equal(np.array([r["ubond_slt"], r["ubond_slv"]]), g["ref_fpbond_e"])
+    assert np.array_equal(digest(r["f"]), g["ref_fpbond_f_sha"])
+
+
+@pytest.mark.parametrize("which", ["topology", "branch"])
+def test_fpbend_matches_reference(fk, which):
+    """fpbend on the topology's 3,141 bends, and on 600 of them with every
+    branch set (Urey-Bradley, both coupling forms, a rest angle of pi)."""
+    s, _, g = fk
+    if which == "topology":
+        lb, pots, key = g["lbndg"].astype(np.int32), g["bend_pots"], "ref_fpbend"
+    else:
+        lb, pots, key = g["lbndg"].astype(np.int32)[:, :600], g["bend_branch_pots"], "ref_fpbend_branch"
+    r = O.fpbend(s, lb, list(pots))
+    assert np.array_equal(np.array([r["ubend_slt"], r["ubend_slv"]]), g[key + "_e"])
+    assert np.array_equal(digest(r["f"]), g[key + "_f_sha"])
