@@ -206,6 +206,16 @@ int orac_hip_fptors(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *l
                     const double *yp0, const double *zp0, const double *pota, const double *potb, double *utors_slt,
                     double *utors_slv, double *fpx, double *fpy, double *fpz, int32_t mb);
 
+/* Replaces fitors (fitors.f:1-286) as mts_intra_n0.f:97-99 calls it:
+ * improper torsions litr(4, litor) on the coordinates as given; potc > 0
+ * selects the harmonic form potb (phi - pota)^2 on the signed dihedral,
+ * otherwise pota is the signed multiplicity (1, 2, 3, 4 or 6) and potb the
+ * barrier.  Forces accumulate, energies are overwritten.  Another
+ * multiplicity returns 4 (the reference stops). */
+int orac_hip_fitors(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *litr, int32_t litor, const double *xp0,
+                    const double *yp0, const double *zp0, const double *pota, const double *potb, const double *potc,
+                    double *utors_slt, double *utors_slv, double *fpx, double *fpy, double *fpz, int32_t mb);
+
 /* Replaces fpbond (fpbond.f:1-110) as mts_intra_n0.f:73-74 calls it:
  * harmonic bonds lbnd(2, lbond) on the coordinates as given, pota the rest
  * length (potbo(1,2)) and potb the constant (potbo(1,1)); mb atoms.  Forces

@@ -850,7 +850,7 @@ extern "C" int orac_hip_cself_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index
 static int bonded_eval(orac_hip_ctx *ctx, const char *name, int arity, const int32_t *ss_index, const int32_t *lst,
                        int32_t n, const double *xp0, const double *yp0, const double *zp0,
                        const double *const *pots, int npots, double *fpx, double *fpy, double *fpz, int32_t mb,
-                       double e[2]) {
+                       double e[2], bool improper = false) {
   ORAC_REQUIRE(ss_index && lst && xp0 && yp0 && zp0 && fpx && fpy && fpz && mb > 0 && n > 0, E_ARG, "null argument");
   for (int k = 0; k < npots; ++k) ORAC_REQUIRE(pots[k], E_ARG, "null argument");
   hipStream_t st = ctx->stream;
@@ -880,6 +880,7 @@ static int bonded_eval(orac_hip_ctx *ctx, const char *name, int arity, const int
   h.ntors = n;
   h.natoms = mb;
   h.arity = arity;
+  h.improper = improper ? 1 : 0;
   h.atoms = B.put_vec("t_at", at, st, false);
   h.pota = dp[0];
   h.potb = dp[1];
@@ -943,6 +944,26 @@ extern "C" int orac_hip_fptors(orac_hip_ctx *ctx, const int32_t *ss_index, const
     ORAC_REQUIRE(status != 2, E_UNSUPPORTED, "fptors: torsion multiplicity outside 1-8, 12, 14");
     ORAC_REQUIRE(status != 1, E_STATE,
                  "fptors: a torsion with collinear atoms (the reference perturbs it at random; not reproduced)");
+    *utors_slt = e[0];
+    *utors_slv = e[1];
+  });
+}
+
+extern "C" int orac_hip_fitors(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *litr, int32_t litor,
+                               const double *xp0, const double *yp0, const double *zp0, const double *pota,
+                               const double *potb, const double *potc, double *utors_slt, double *utors_slv,
+                               double *fpx, double *fpy, double *fpz, int32_t mb) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(utors_slt && utors_slv, E_ARG, "null output");
+    *utors_slt = *utors_slv = 0.0;
+    if (litor == 0) return;
+    ORAC_REQUIRE(litor > 0, E_ARG, "negative count");
+    const double *pots[3] = {pota, potb, potc};
+    double e[2];
+    const int status = bonded_eval(ctx, "fitors", 4, ss_index, litr, litor, xp0, yp0, zp0, pots, 3, fpx, fpy, fpz,
+                                   mb, e, true);
+    ORAC_REQUIRE(status != 2, E_UNSUPPORTED, "fitors: improper torsion multiplicity other than 1, 2, 3, 4, 6");
     *utors_slt = e[0];
     *utors_slv = e[1];
   });

@@ -1,4 +1,5 @@
-// bonded.hip -- proper torsions (fptors.f:1-345), harmonic bonds
+// bonded.hip -- proper and improper torsions (fptors.f:1-345,
+// fitors.f:1-286), harmonic bonds
 // (fpbond.f:1-110) and bends with their Urey-Bradley and stretch-stretch
 // coupling terms (fpbend.f:1-228) on gfx950.
 //
@@ -47,9 +48,12 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
   const double cb1 = (x21 * x32 + y21 * y32 + z21 * z32) / (rsp21 * rsp32);
   const double cb2 = (x43 * x32 + y43 * y32 + z43 * z32) / (rsp43 * rsp32);
   const double cb3 = (x21 * x43 + y21 * y43 + z21 * z43) / (rsp21 * rsp43);
-  const double sb1 = sqrt(fabs(1.0 - cb1 * cb1)), sb2 = sqrt(fabs(1.0 - cb2 * cb2)), sb3 = sqrt(fabs(1.0 - cb3 * cb3));
+  // fitors.f takes the square roots as they come (no absolute value) and has no singular case
+  const double s1 = 1.0 - cb1 * cb1, s2 = 1.0 - cb2 * cb2, s3 = 1.0 - cb3 * cb3;
+  const double sb1 = sqrt(d.improper ? s1 : fabs(s1)), sb2 = sqrt(d.improper ? s2 : fabs(s2)),
+               sb3 = sqrt(d.improper ? s3 : fabs(s3));
   const double aux = sb1 * sb2;
-  if (fabs(aux * sb3) < 1.0e-25) {  // singular angle: the reference perturbs at random (not restated)
+  if (!d.improper && fabs(aux * sb3) < 1.0e-25) {  // singular angle: the reference perturbs at random (not restated)
     d.status[0] = 1;
     return;
   }
@@ -57,11 +61,12 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
   const double pa = d.pota[i];
   const int tmp = (int)(pa >= 0.0 ? floor(pa + 0.5) : -floor(-pa + 0.5));  // IDNINT
   const int ntph = abs(tmp);
-  if (ntph == 0) {
+  const bool harmonic = d.improper && d.potc[i] > 0.0;
+  if (ntph == 0 && !harmonic) {
     d.status[0] = 2;
     return;
   }
-  const double sig = (double)(tmp / ntph);
+  const double sig = ntph ? (double)(tmp / ntph) : 0.0;
   const double lim = (double)1.000001f;  // default-real literal (fptors.f:139-140)
   if (coa > 1.0 && coa < lim) coa = 1.0;
   if (coa < -1.0 && coa > -lim) coa = -1.0;
@@ -74,7 +79,34 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
   const double aux3 = rsp21 * rsp43 * sb3;
   double enrg, qforce;
   const double c2 = ipow_ltr(coa, 2), c3 = ipow_ltr(coa, 3), c4 = ipow_ltr(coa, 4);
-  switch (ntph) {
+  if (harmonic) {  // fitors.f harmonic form on the signed dihedral
+    double bb = acos(coa);
+    const double xn = y21 * z32 - y32 * z21, yn = z21 * x32 - z32 * x21, zn = x21 * y32 - x32 * y21;
+    const double aux4 = fsign(bb, xn * x43 + yn * y43 + zn * z43);
+    const double q4 = (double)1.57079f;  // default-real literal
+    if (aux4 < 0.0 && fabs(aux4) >= q4) {
+      bb = 2.0 * 3.141592653589793 + aux4;
+      if (fabs(pa - bb) > 3.14159) bb = -2 * 3.14159 + bb;
+    } else if (aux4 < 0.0 && fabs(aux4) < q4) {
+      bb = aux4;
+      if (fabs(pa - bb) > 3.14159) bb = 2 * 3.14159 + bb;
+    } else {
+      bb = aux4;
+      if (fabs(pa - bb) > 3.14159) bb = 2 * 3.14159 + bb;
+      if (fabs(pa - bb) > 3.14159) bb = -4 * 3.14159 + bb;
+    }
+    if (fabs(pa) < 1.0e-3)
+      qforce = 2.0 * b;
+    else if (fabs(pa - 3.14159) < 1.0e-3)
+      qforce = -2.0 * b;
+    else
+      qforce = (2.0 * b * (bb - pa)) / sin(bb);
+    const double da = bb - pa;
+    enrg = b * (da * da);
+  } else if (d.improper && (ntph == 5 || ntph > 6)) {  // fitors.f stops
+    d.status[0] = 2;
+    return;
+  } else switch (ntph) {
     case 1: enrg = sig * (fabs(b) - coa * b); qforce = sig * b; break;
     case 2: enrg = sig * (fabs(b) - (2.0 * c2 - 1.0) * b); qforce = sig * (4.0 * b * coa); break;
     case 3: enrg = sig * (fabs(b) - (4.0 * c3 - 3.0 * coa) * b); qforce = sig * ((12.0 * c2 - 3.0) * b); break;

@@ -205,6 +205,7 @@ struct TorsDesc {
   const double *pota, *potb;          // torsions: signed multiplicity, barrier; bonds, bends: rest value, constant
   const double *potc, *potd, *pote, *potf, *potg;  // bends: Urey-Bradley and coupling parameters
   int arity;                          // atoms per term: 2 bonds, 3 bends, 4 torsions
+  int improper;                       // torsions: fitors rules (potc > 0 harmonic; cosine n = 1-4, 6)
   const double *x, *y, *z;            // coordinates as given
   const int *ss;                      // ss_index of each atom
   double *tf;                         // [ntors][12] force components: x of l1..l4, y, z (unused slots zero)

@@ -142,6 +142,14 @@ module orac_hip_iface
       real(c_double), value :: alphal, kcut
       real(c_double) :: charge(*), charge0(*), self_slt, dwrk
     end function
+    integer(c_int) function c_fitors(ctx, ss_index, litr, litor, xp0, yp0, zp0, pota, potb, potc, ut_slt, ut_slv, &
+         fpx, fpy, fpz, mb) bind(C, name="orac_hip_fitors")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), litr(*)
+      integer(c_int32_t), value :: litor, mb
+      real(c_double) :: xp0(*), yp0(*), zp0(*), pota(*), potb(*), potc(*), ut_slt, ut_slv, fpx(*), fpy(*), fpz(*)
+    end function
     integer(c_int) function c_fpbond(ctx, ss_index, lbnd, lbond, xp0, yp0, zp0, pota, potb, ub_slt, ub_slv, fpx, &
          fpy, fpz, mb) bind(C, name="orac_hip_fpbond")
       import :: c_ptr, c_int, c_int32_t, c_double
@@ -355,6 +363,17 @@ subroutine fnbgrp(ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alp
   logical :: ewald
   call check(c_fnbgrp(ctx(), ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, lg(ewald), alphal, ingrp, ingrpp, &
        fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, nato))
+end subroutine
+
+! fitors.f:1-2 (mts_intra_n0.f:97-99)
+subroutine fitors(ss_index, litr, litor, xp0, yp0, zp0, pota, potb, potc, utors_slt, utors_slv, fpx, fpy, fpz, mb, &
+     itask)
+  use orac_hip_iface
+  implicit none
+  integer :: litor, mb, itask, litr(4,*), ss_index(*)
+  real(8) :: utors_slt, utors_slv, pota(*), potb(*), potc(*), xp0(*), yp0(*), zp0(*), fpx(mb), fpy(mb), fpz(mb)
+  call check(c_fitors(ctx(), ss_index, litr, litor, xp0, yp0, zp0, pota, potb, potc, utors_slt, utors_slv, fpx, fpy, &
+       fpz, mb))
 end subroutine
 
 ! fpbond.f:1-2 (mts_intra_n0.f:73-74); icnstr and itask select OpenMP work in the reference

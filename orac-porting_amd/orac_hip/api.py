@@ -53,7 +53,7 @@ EXPORTS = [
     "orac_hip_create", "orac_hip_destroy", "orac_hip_last_error", "orac_hip_invalidate", "orac_hip_device_name",
     "orac_hip_probe_copy",
     "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
-    "orac_hip_force_alchemy", "orac_hip_force_alchemy_shift", "orac_hip_fptors", "orac_hip_fpbond", "orac_hip_fpbend", "orac_hip_fnb14",
+    "orac_hip_force_alchemy", "orac_hip_force_alchemy_shift", "orac_hip_fptors", "orac_hip_fpbond", "orac_hip_fitors", "orac_hip_fpbend", "orac_hip_fnb14",
     "orac_hip_fft_pme_alchemy", "orac_hip_ferrf_alchemy", "orac_hip_cself_alchemy", "orac_hip_int_corr_erf_spline",
     "orac_hip_erf_corr_cutoff",
     "orac_hip_decide_force", "orac_hip_ferrf", "orac_hip_cself", "orac_hip_fnbgrp", "orac_hip_system_create",
@@ -256,6 +256,16 @@ class Context:
         _check(lib().orac_hip_fptors(self._h, _p(keep[0]), _p(keep[1]), C.c_int32(ltors), _p(keep[2]), _p(keep[3]),
                                      _p(keep[4]), _p(keep[5]), _p(keep[6]), C.byref(a), C.byref(b), _p(fpx), _p(fpy),
                                      _p(fpz), C.c_int32(mb)))
+        return a.value, b.value
+
+    # -- fitors.f
+    def fitors(self, ss_index, litr, litor, xp0, yp0, zp0, pota, potb, potc, fpx, fpy, fpz, mb):
+        """Returns (utors_slt, utors_slv); fpx/fpy/fpz accumulated in place."""
+        a, b = C.c_double(), C.c_double()
+        fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
+        keep = [_i(ss_index), _i(litr), _f(xp0), _f(yp0), _f(zp0), _f(pota), _f(potb), _f(potc)]
+        _check(lib().orac_hip_fitors(self._h, _p(keep[0]), _p(keep[1]), C.c_int32(litor), *[_p(v) for v in keep[2:]],
+                                     C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), C.c_int32(mb)))
         return a.value, b.value
 
     # -- fpbond.f

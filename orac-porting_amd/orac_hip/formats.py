@@ -78,7 +78,8 @@ def read_tpgprm(path, amphi=False):
     (rows, n): grppt (2, ngrp), lbnd (2, lbond), lbndg (3, lbend), int14 (2, int14p),
     int13 (2, int13p), ltor (4, ltors); potbo (lbond, 2) and potbe (lbend, 7) as
     mts_intra_n0.f:73-85 passes them (fpbond: rest length potbo(:,2), constant
-    potbo(:,1); fpbend: potbe columns 2, 1, 4, 3, 5, 6, 7); potto is (ltors, 2): barrier, then the signed
+    potbo(:,1); fpbend: potbe columns 2, 1, 4, 3, 5, 6, 7); litr (4, litor) and
+    potit (litor, 3) (fitors: columns 2, 1, 3, mts_intra_n0.f:97-98); potto is (ltors, 2): barrier, then the signed
     multiplicity (fptors receives potto(1,2) as pota and potto(1,1) as potb,
     mts_intra_n1.f:118-119).  Charges are in electrons, masses in amu."""
     r = _ListDirected(path)
@@ -98,7 +99,7 @@ def read_tpgprm(path, amphi=False):
     r.counted_lists(n["ntap"])                                     # concta
     t["lbndg"] = r.ints(3 * n["lbend"]).reshape(3, n["lbend"])
     t["ltor"] = r.ints(4 * n["ltors"]).reshape(4, n["ltors"])      # ((ltor(j,i),i=1,ltors),j=1,4)
-    r.ints(4 * n["litor"])
+    t["litr"] = r.ints(4 * n["litor"]).reshape(4, n["litor"])   # improper torsions, as ltor
     v = r.ints(3 * n["int14p"] + 2 * n["int13p"] + n["lpnbd"] ** 2)
     t["int14"] = v[:2 * n["int14p"]].reshape(n["int14p"], 2).T
     t["type14"] = v[2 * n["int14p"]:3 * n["int14p"]]
@@ -119,6 +120,8 @@ def read_tpgprm(path, amphi=False):
     if not amphi:                                                  # potto(ltors, 2): barrier, multiplicity
         o = 2 * n["lbond"] + 7 * n["lbend"]
         t["potto"] = p[o:o + 2 * n["ltors"]].reshape(2, n["ltors"]).T
+    o = 2 * n["lbond"] + 7 * n["lbend"] + (4 if amphi else 2) * n["ltors"]
+    t["potit"] = p[o:o + 3 * n["litor"]].reshape(3, n["litor"]).T  # potit(litor, 3)
     o = 2 * n["lbond"] + 7 * n["lbend"] + (4 if amphi else 2) * n["ltors"] + 3 * n["litor"]
     t["pnbd"] = p[o:o + 4 * L].reshape(4, L)                       # pnbd1..4 (readtp.f:140-143)
     if iz == 1:

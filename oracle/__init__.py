@@ -663,6 +663,33 @@ def ref_fptors(s, ltor, pota, potb, coords=None):
 
 
 
+def fitors(s, litr, pota, potb, potc, coords=None):
+    """fitors.f on the Cartesian coordinates of System s (or coords): improper
+    torsions litr (4, n) 1-based; potc > 0 selects the harmonic form (pota the
+    rest dihedral, potb the constant), else pota is the signed multiplicity.
+    Returns utors_slt, utors_slv, f and status (2: unsupported multiplicity)."""
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b = C.c_double(), C.c_double()
+    lt = _i(litr)
+    rc = lib().orc_fitors(_p(_i(s.ss_index)), _p(lt), C.c_int(lt.size // 4), _p(_f(x)), _p(_f(y)), _p(_f(z)),
+                          _p(_f(pota)), _p(_f(potb)), _p(_f(potc)), C.byref(a), C.byref(b), _p(fpx), _p(fpy),
+                          _p(fpz))
+    return dict(utors_slt=a.value, utors_slv=b.value, f=np.stack([fpx, fpy, fpz]), status=rc)
+
+
+def ref_fitors(s, litr, pota, potb, potc, coords=None):
+    R = ref()
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b = C.c_double(), C.c_double()
+    keep = [_i(s.ss_index), _i(litr), _f(x), _f(y), _f(z), _f(pota), _f(potb), _f(potc)]
+    R.fitors_(_p(keep[0]), _p(keep[1]), _ri(keep[1].size // 4), _p(keep[2]), _p(keep[3]), _p(keep[4]), _p(keep[5]),
+              _p(keep[6]), _p(keep[7]), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _ri(n), _ri(1))
+    return dict(utors_slt=a.value, utors_slv=b.value, f=np.stack([fpx, fpy, fpz]))
+
 def fpbond(s, lbnd, pota, potb, coords=None):
     """fpbond.f on the Cartesian coordinates of System s (or coords): bonds
     lbnd (2, n) 1-based, pota = rest length, potb = force constant."""

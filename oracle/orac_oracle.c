@@ -2026,3 +2026,121 @@ void orc_fpbend(const int32_t *ss_index, const int32_t *lbndg, int lbend, const 
   *ubend_slt = ub[0];
   *ubend_slv = ub[1];
 }
+
+/* fitors.f:1-286 (serial): improper torsions litr(4, litor).  potc > 0: the
+ * harmonic form potb (phi - pota)^2 on the signed dihedral phi brought within
+ * pi of pota; otherwise the cosine series of multiplicity |pota| (1, 2, 3, 4
+ * or 6; another returns 2, where the reference stops).  The cosine-derivative
+ * vectors are fptors'. */
+int orc_fitors(const int32_t *ss_index, const int32_t *litr, int litor, const double *xp0, const double *yp0,
+               const double *zp0, const double *pota, const double *potb, const double *potc, double *utors_slt,
+               double *utors_slv, double *fpx, double *fpy, double *fpz) {
+  const double lim = (double)1.000001f, half_pi4 = (double)1.57079f; /* default-real literals */
+  double utors[2] = {0.0, 0.0};
+  int status = 0;
+  for (int i = 0; i < litor; ++i) {
+    if (fabs(potb[i]) < 1.0e-12) continue;
+    const int l[4] = {litr[4 * i] - 1, litr[4 * i + 1] - 1, litr[4 * i + 2] - 1, litr[4 * i + 3] - 1};
+    const int type = ss_index[l[0]];
+    const double x21 = xp0[l[1]] - xp0[l[0]], y21 = yp0[l[1]] - yp0[l[0]], z21 = zp0[l[1]] - zp0[l[0]];
+    const double x32 = xp0[l[2]] - xp0[l[1]], y32 = yp0[l[2]] - yp0[l[1]], z32 = zp0[l[2]] - zp0[l[1]];
+    const double x43 = xp0[l[3]] - xp0[l[2]], y43 = yp0[l[3]] - yp0[l[2]], z43 = zp0[l[3]] - zp0[l[2]];
+    const double rsq21 = x21 * x21 + y21 * y21 + z21 * z21;
+    const double rsq32 = x32 * x32 + y32 * y32 + z32 * z32;
+    const double rsq43 = x43 * x43 + y43 * y43 + z43 * z43;
+    const double rsp21 = sqrt(rsq21), rsp32 = sqrt(rsq32), rsp43 = sqrt(rsq43);
+    const double cb1 = (x21 * x32 + y21 * y32 + z21 * z32) / (rsp21 * rsp32);
+    const double cb2 = (x43 * x32 + y43 * y32 + z43 * z32) / (rsp43 * rsp32);
+    const double cb3 = (x21 * x43 + y21 * y43 + z21 * z43) / (rsp21 * rsp43);
+    const double sb1 = sqrt(1.0 - cb1 * cb1), sb2 = sqrt(1.0 - cb2 * cb2), sb3 = sqrt(1.0 - cb3 * cb3);
+    const double aux = sb1 * sb2;
+    double coa = (cb1 * cb2 - cb3) / aux;
+    if (coa > 1.0 && coa < lim) coa = 1.0;
+    if (coa < -1.0 && coa > -lim) coa = -1.0;
+    double aux1 = cb1 * sb2, aux2 = sb1 * cb2;
+    const double auxa = -(aux1 * coa + aux2) / aux;
+    const double auxb = -(aux2 * coa + aux1) / aux;
+    const double auxc = sb3 / aux;
+    aux1 = rsp21 * rsp32 * sb1;
+    aux2 = rsp32 * rsp43 * sb2;
+    const double aux3 = rsp21 * rsp43 * sb3;
+    double d1[3][4], d2[3][4], d3[3][4];
+    const double r21[3] = {x21, y21, z21}, r32[3] = {x32, y32, z32}, r43[3] = {x43, y43, z43};
+    for (int c = 0; c < 3; ++c) {
+      d1[c][0] = r32[c] / aux1 - cb1 * r21[c] / (rsq21 * sb1);
+      d1[c][2] = -r21[c] / aux1 + cb1 * r32[c] / (rsq32 * sb1);
+      d1[c][1] = -d1[c][2] - d1[c][0];
+      d2[c][1] = r43[c] / aux2 - cb2 * r32[c] / (rsq32 * sb2);
+      d2[c][3] = -r32[c] / aux2 + cb2 * r43[c] / (rsq43 * sb2);
+      d2[c][2] = -d2[c][1] - d2[c][3];
+      d3[c][0] = r43[c] / aux3 - cb3 * r21[c] / (rsq21 * sb3);
+      d3[c][2] = r21[c] / aux3 - cb3 * r43[c] / (rsq43 * sb3);
+      d3[c][1] = -d3[c][0];
+      d3[c][3] = -d3[c][2];
+    }
+    const double a = pota[i], b = potb[i];
+    double qforce;
+    if (potc[i] > 0.0) {
+      double bb = acos(coa);
+      const double xn = y21 * z32 - y32 * z21, yn = z21 * x32 - z32 * x21, zn = x21 * y32 - x32 * y21;
+      const double aux4 = (xn * x43 + yn * y43 + zn * z43) >= 0.0 ? fabs(bb) : -fabs(bb);
+      if (aux4 < 0.0 && fabs(aux4) >= half_pi4) {
+        bb = 2.0 * acos(-1.0) + aux4;
+        if (fabs(a - bb) > 3.14159) bb = -2 * 3.14159 + bb;
+      } else if (aux4 < 0.0 && fabs(aux4) < half_pi4) {
+        bb = aux4;
+        if (fabs(a - bb) > 3.14159) bb = 2 * 3.14159 + bb;
+      } else {
+        bb = aux4;
+        if (fabs(a - bb) > 3.14159) bb = 2 * 3.14159 + bb;
+        if (fabs(a - bb) > 3.14159) bb = -4 * 3.14159 + bb;
+      }
+      if (fabs(a) < 1.0e-3)
+        qforce = 2.0 * b;
+      else if (fabs(a - 3.14159) < 1.0e-3)
+        qforce = -2.0 * b;
+      else
+        qforce = (2.0 * b * (bb - a)) / sin(bb);
+      const double da = bb - a;
+      utors[type - 1] = utors[type - 1] + b * (da * da);
+    } else {
+      const int tmp = (int)(a >= 0.0 ? floor(a + 0.5) : -floor(-a + 0.5)); /* IDNINT */
+      const int ntph = abs(tmp);
+      if (ntph == 0) { status = 2; continue; }
+      const double sig = (double)(tmp / ntph);
+      double enrg;
+      switch (ntph) {
+        case 1: enrg = sig * (fabs(b) - coa * b); qforce = sig * b; break;
+        case 2: enrg = sig * (fabs(b) - (2.0 * powl_ltr(coa, 2) - 1.0) * b); qforce = sig * (4.0 * b * coa); break;
+        case 3:
+          enrg = sig * (fabs(b) - (4.0 * powl_ltr(coa, 3) - 3.0 * coa) * b);
+          qforce = sig * ((12.0 * powl_ltr(coa, 2) - 3.0) * b);
+          break;
+        case 4:
+          enrg = sig * (fabs(b) - (8.0 * powl_ltr(coa, 4) - 8.0 * powl_ltr(coa, 2) + 1.0) * b);
+          qforce = sig * ((32.0 * powl_ltr(coa, 3) - 16.0 * coa) * b);
+          break;
+        case 6:
+          enrg = sig * (fabs(b) - b * (32.0 * powl_ltr(coa, 6) - 48.0 * powl_ltr(coa, 4) + 18.0 * powl_ltr(coa, 2) - 1.0));
+          qforce = sig * (b * (192.0 * powl_ltr(coa, 5) - 192.0 * powl_ltr(coa, 3) + 36.0 * coa));
+          break;
+        default: status = 2; continue;
+      }
+      utors[type - 1] = utors[type - 1] + enrg;
+    }
+    for (int c = 0; c < 3; ++c) {
+      double *f = c == 0 ? fpx : (c == 1 ? fpy : fpz);
+      const double u1 = auxa * d1[c][0] + auxc * d3[c][0];
+      const double u2 = auxa * d1[c][1] + auxb * d2[c][1] + auxc * d3[c][1];
+      const double u3 = auxa * d1[c][2] + auxb * d2[c][2] + auxc * d3[c][2];
+      const double u4 = auxb * d2[c][3] + auxc * d3[c][3];
+      f[l[0]] = f[l[0]] + qforce * u1;
+      f[l[1]] = f[l[1]] + qforce * u2;
+      f[l[2]] = f[l[2]] + qforce * u3;
+      f[l[3]] = f[l[3]] + qforce * u4;
+    }
+  }
+  *utors_slt = utors[0];
+  *utors_slv = utors[1];
+  return status;
+}

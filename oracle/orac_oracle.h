@@ -138,6 +138,9 @@ void orc_fnb14(const int32_t *ss_index, const double *xp0, const double *yp0, co
 int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const double *xp0, const double *yp0,
                const double *zp0, const double *pota, const double *potb, double *utors_slt, double *utors_slv,
                double *fpx, double *fpy, double *fpz);
+int orc_fitors(const int32_t *ss_index, const int32_t *litr, int litor, const double *xp0, const double *yp0,
+               const double *zp0, const double *pota, const double *potb, const double *potc, double *utors_slt,
+               double *utors_slv, double *fpx, double *fpy, double *fpz);
 void orc_fpbond(const int32_t *ss_index, const int32_t *lbnd, int lbond, const double *xp0, const double *yp0,
                 const double *zp0, const double *pota, const double *potb, double *ubond_slt, double *ubond_slv,
                 double *fpx, double *fpy, double *fpz);

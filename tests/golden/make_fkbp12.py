@@ -13,7 +13,8 @@ cutoff 10 A, list skin 1.8 A, 18^3 linked cells) and writes:
   Ewald) on the intra-group pairs and on the bond + 1-3 lists, fnb14 on the
   topology's 1-4 pairs (both branches), fptors on its proper torsions,
   fpbond and fpbend on its bonds and bends (and on bends with every fpbend
-  branch set), and
+  branch set), fitors on its improper torsions (and on impropers of every
+  form), and
   the ERF_CORR table (int_corr_erf_spline, erf_corr_cutoff) with ferrf on it.  Exact arrays are stored as
   SHA-256 digests of their bytes; energies, stress and a sample of the PME
   forces as values.
@@ -88,6 +89,22 @@ def bend_branches(bend):
     return b
 
 
+def itor_branches(n):
+    """Improper parameters (fitors order: pota, potb, potc) over n quadruples:
+    half harmonic (rest dihedrals across (-pi, pi), some exactly 0 and
+    3.14159), half cosine series of multiplicity 1, 2, 3, 4, 6 of both signs."""
+    rng = np.random.default_rng(9)
+    pota = np.empty(n)
+    potb = rng.uniform(0.5, 5.0, n)
+    potc = np.where(np.arange(n) < n // 2, 1.0, -1.0)
+    h = n // 2
+    pota[:h] = rng.uniform(-np.pi, np.pi, h)
+    pota[:h:10], pota[5:h:10] = 0.0, 3.14159
+    m = np.array([1, 2, 3, 4, 6])
+    pota[h:] = m[np.arange(n - h) % 5] * np.where(np.arange(n - h) % 2 == 0, 1.0, -1.0)
+    return np.stack([pota, potb, potc])
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libref.so is missing: build it with `make -C oracle ref` where the reference exists")
@@ -156,6 +173,15 @@ def main():
     r = O.ref_fpbend(s, top["lbndg"][:, :600], list(out["bend_branch_pots"]))
     out["ref_fpbend_branch_e"] = np.array([r["ubend_slt"], r["ubend_slv"]])
     out["ref_fpbend_branch_f_sha"] = digest(r["f"])
+    # improper torsions of the topology (all cosine, multiplicity 2), and
+    # quadruples of its proper torsions as impropers of every other form
+    out["litr"], out["itor_pots"] = top["litr"].astype(np.int16), top["potit"][:, [1, 0, 2]].T.copy()
+    r = O.ref_fitors(s, top["litr"], *out["itor_pots"])
+    out["ref_fitors_e"], out["ref_fitors_f_sha"] = np.array([r["utors_slt"], r["utors_slv"]]), digest(r["f"])
+    out["itor_branch_pots"] = itor_branches(800)
+    r = O.ref_fitors(s, top["ltor"][:, :800], *out["itor_branch_pots"])
+    out["ref_fitors_branch_e"] = np.array([r["utors_slt"], r["utors_slv"]])
+    out["ref_fitors_branch_f_sha"] = digest(r["f"])
     # ERF_CORR: the reference's reciprocal-cutoff correction table
     # (ERF_CORR 300 0.5 3.5; rkcut from erf_corr_cutoff) and ferrf with it
     rk = O.ref_erf_corr_cutoff(s.oc, s.nfft)

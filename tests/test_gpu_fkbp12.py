@@ -241,3 +241,21 @@ def test_fpbend_matches_oracle(gpu_ctx, fk, which):
     o = O.fpbend(s, lb, list(pots))
     assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
     assert rel(a, o["ubend_slt"]) < 1e-12 and b == o["ubend_slv"] == 0.0
+
+
+@pytest.mark.parametrize("which", ["topology", "branch"])
+def test_fitors_matches_oracle(gpu_ctx, fk, which):
+    """fitors on the topology's impropers and on impropers of every form."""
+    s, _, g = fk
+    x, y, z = s.cartesian()
+    n = s.natoms
+    if which == "topology":
+        lt, pots = g["litr"].astype(np.int32), g["itor_pots"]
+    else:
+        lt, pots = g["ltor"].astype(np.int32)[:, :800], g["itor_branch_pots"]
+    lt = np.asfortranarray(lt)
+    fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+    a, b = gpu_ctx.fitors(s.ss_index, lt, lt.shape[1], x, y, z, *pots, fpx, fpy, fpz, n)
+    o = O.fitors(s, lt, *pots)
+    assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
+    assert rel(a, o["utors_slt"]) < 1e-12 and b == o["utors_slv"] == 0.0

@@ -1,4 +1,4 @@
-"""The oracle's bonded terms (fptors, fnb14, fpbond, fpbend) against what the mathematics
+"""The oracle's bonded terms (fptors, fitors, fnb14, fpbond, fpbend) against what the mathematics
 fixes, on small synthetic geometries: forces are minus the energy gradient
 (central differences) for every multiplicity fptors knows, the energy is
 invariant under rigid rotation and the forces of each term sum to zero; the
@@ -154,3 +154,37 @@ def test_bend_force_is_minus_gradient_and_closed_form(branch):
         e += 12.0 * (r12 - 1.1) * (r32 - 1.3)
     assert math.isclose(r["ubend_slt"], e, rel_tol=1e-12)
     _grad_check(lambda p: run(p)["ubend_slt"], pos, r["f"][:, :3])
+
+
+@pytest.mark.parametrize("form", ["harmonic", 1, 2, 3, 4, 6])
+def test_improper_force_is_minus_gradient(form):
+    """fitors: the harmonic form is potb (phi - pota)^2 on the signed dihedral
+    (checked against an independent atan2 dihedral), the cosine forms agree
+    with fptors; forces are minus the gradient."""
+    rng = np.random.default_rng(31 if form == "harmonic" else 40 + form)
+    s, pos = _bb_system(rng, 4)
+    lt = np.array([[1], [2], [3], [4]], np.int32)
+    if form == "harmonic":
+        pa, pb, pc = 0.4, 3.0, 1.0
+    else:
+        pa, pb, pc = float(form), 1.7, -1.0
+
+    def run(p):
+        return O.fitors(s, lt, [pa], [pb], [pc], coords=_coords(s, p))
+
+    r = run(pos)
+    assert r["status"] == 0
+    if form == "harmonic":
+        b1, b2, b3 = pos[1] - pos[0], pos[2] - pos[1], pos[3] - pos[2]
+        n1, n2 = np.cross(b1, b2), np.cross(b2, b3)
+        phi = math.atan2(np.dot(np.cross(n1, n2), b2 / np.linalg.norm(b2)), np.dot(n1, n2))
+        d = (phi - pa + math.pi) % (2 * math.pi) - math.pi
+        # the reference wraps the dihedral with 3.14159, not pi: a wrapped
+        # dihedral is off by up to 4 (pi - 3.14159) ~ 1e-5 rad
+        assert abs(math.sqrt(r["utors_slt"] / pb) - abs(d)) < 2.2e-5
+    else:
+        t = O.fptors(s, lt, [pa], [pb], coords=_coords(s, pos))
+        assert math.isclose(r["utors_slt"], t["utors_slt"], rel_tol=1e-14)
+        assert np.allclose(r["f"], t["f"], rtol=0, atol=1e-12 * np.abs(t["f"]).max())
+    # the harmonic force divides by sin of the (3.14159-wrapped) dihedral: ~1e-5 relative
+    _grad_check(lambda p: run(p)["utors_slt"], pos, r["f"][:, :4], tol=2e-5 if form == "harmonic" else 1e-6)

@@ -194,3 +194,19 @@ def test_fpbend_matches_reference(fk, which):
     r = O.fpbend(s, lb, list(pots))
     assert np.array_equal(np.array([r["ubend_slt"], r["ubend_slv"]]), g[key + "_e"])
     assert np.array_equal(digest(r["f"]), g[key + "_f_sha"])
+
+
+@pytest.mark.parametrize("which", ["topology", "branch"])
+def test_fitors_matches_reference(fk, which):
+    """fitors on the topology's 339 improper torsions (cosine, multiplicity 2)
+    and on 800 quadruples with the harmonic form (rest dihedrals across
+    (-pi, pi), 0 and 3.14159) and every cosine multiplicity, both signs."""
+    s, _, g = fk
+    if which == "topology":
+        lt, pots, key = g["litr"].astype(np.int32), g["itor_pots"], "ref_fitors"
+    else:
+        lt, pots, key = g["ltor"].astype(np.int32)[:, :800], g["itor_branch_pots"], "ref_fitors_branch"
+    r = O.fitors(s, lt, *pots)
+    assert r["status"] == 0
+    assert np.array_equal(np.array([r["utors_slt"], r["utors_slv"]]), g[key + "_e"])
+    assert np.array_equal(digest(r["f"]), g[key + "_f_sha"])
