@@ -256,6 +256,49 @@ int orac_hip_fnbgrp(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp
                     double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt,
                     double *ucoul_slv, int32_t nato);
 
+/* Replaces ferrf_alchemy_add (ferrf_alchemy_add.f:1-204) as mtsmd.f:3034-3040
+ * calls it on every M step when Ewald and alchemy are both on: the
+ * intra-alchemical pairs beyond 1-4, list(3, nlist) = (ia, ib, facQ) as
+ * initialize_lambda.f builds it, on cartesian x0/y0/z0 and the created
+ * charges.  Each pair's erf(alpha r) q_a q_b / r term is scaled by
+ * factl = (1 - |lambdaq_a|)(1 - |lambdaq_b|) - facQ (and by the lambdaq0
+ * factor for en0); with intramol the facQ = 1 pairs are skipped; with
+ * erf_corr the spline correction of erf_arr_corr is included as in ferrf.
+ * Forces accumulate; fscnstr_slt/slv (by the ss_index of each pair's first
+ * atom) and dwrk = slt + slv - en0 are overwritten; nato atoms. */
+int orac_hip_ferrf_alchemy_add(orac_hip_ctx *ctx, const int32_t *ss_index, double alphal, const double *charge,
+                               const double *lambdaq, const double *lambdaq0, const double *x0, const double *y0,
+                               const double *z0, const int32_t *list, int32_t nlist, double *fscnstr_slt,
+                               double *fscnstr_slv, double *fpx, double *fpy, double *fpz, int32_t erf_corr,
+                               const double *erf_arr_corr, double delew, double rlew, double *dwrk, int32_t nato,
+                               int32_t intramol);
+
+/* Replaces fnb14_alchemy (fnb14_alchemy.f:1-361) as mts_intra_n1.f:79-83
+ * calls it: fnb14's 1-4 pairs with the alchemical factors of each pair's
+ * per-atom lambda, lambdaq (and lambda0, lambdaq0 for dwrk), keeping the
+ * reference's own arithmetic (without Ewald the forces are unscaled; with
+ * Ewald only the repulsive LJ force term is scaled, and pairs of two
+ * alchemical atoms add an erf term).  Forces accumulate; energies and
+ * dwrk = U(lambda) - U(lambda0) are overwritten (the itask argument selects
+ * OpenMP work in the reference and has no counterpart). */
+int orac_hip_fnb14_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                           const double *zp0, const double *charge, int32_t nato, const double *ecc12,
+                           const double *ecc6, int32_t cut, int32_t ewald, double alphal, const int32_t *int14,
+                           int32_t int14p, const int32_t *type14, double fudge, double lj_fudge, double *fpx,
+                           double *fpy, double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt,
+                           double *ucoul_slv, const double *lambda, const double *lambdaq, const double *lambda0,
+                           const double *lambdaq0, double *dwrk);
+
+/* Replaces fnbgrp_alchemy (fnbgrp_alchemy.f:1-220) as mts_intra_n1.f:97-100
+ * calls it: the arithmetic is fnbgrp's (the reference derives alphal0 from
+ * lambdaq and never reads it), so lambdaq is accepted and not read; nato is
+ * the number of atoms the arrays hold. */
+int orac_hip_fnbgrp_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                            const double *zp0, const double *charge, const int32_t *type, const double *ecc12,
+                            const double *ecc6, int32_t ewald, double alphal, const int32_t *ingrp, int32_t ingrpp,
+                            double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv,
+                            double *ucoul_slt, double *ucoul_slv, const double *lambdaq, int32_t nato);
+
 /* ---- device-resident systems (no reference counterpart) -------------------- */
 /* A system whose topology, coordinates, lists and grids stay in HBM.  The
  * caller describes it once with the same arrays the plain-mode routines
@@ -324,7 +367,9 @@ int orac_hip_stream(orac_hip_ctx *ctx, void **stream);
  * on a second, energy-only grid (do_pmesh_alchemy.f90), in the corrections
  * and in the self term, for dwrk = U(lambda) - U(lambda0).  ss_alchemy,
  * rmin, epsm, qsoft, ljsoft, intramol, ljintra and opls as in
- * orac_hip_force_alchemy.  Once set the system stays alchemical; its
+ * orac_hip_force_alchemy.  listqq adds ferrf_alchemy_add's intra-alchemical
+ * correction to every evaluation (mtsmd.f:3034-3040; its energy joins the
+ * correction energies and its dwrk the corrections' dwrk).  Once set the system stays alchemical; its
  * energies record reports the self energy of the solute only
  * (cself_alchemy.f), slv 0. */
 typedef struct orac_hip_alchemy_desc {
@@ -334,6 +379,8 @@ typedef struct orac_hip_alchemy_desc {
   int32_t intramol, ljintra, opls;
   int32_t shifted;                    /* 1: shifted potential (force_alchemy_shifted.f) with qshift, ljshift */
   double qshift, ljshift;
+  int32_t nqq;                        /* intra-alchemical pairs of ferrf_alchemy_add, 0 for none */
+  const int32_t *listqq;              /* (3, nqq): ia, ib (1-based), facQ (initialize_lambda.f:66-140) */
 } orac_hip_alchemy_desc;
 int orac_hip_system_set_alchemy(orac_hip_ctx *ctx, int32_t sys, const orac_hip_alchemy_desc *desc);
 /* lam = ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0 (initially 1, 0, 1, 0, 1, 0, 1, 0). */

@@ -108,8 +108,15 @@ extern "C" int orac_hip_stream(orac_hip_ctx *ctx, void **stream) {
 extern "C" int orac_hip_decide_force(double rinn, double alphal, int32_t *lskip_ewald) {
   return guarded([&] {
     ORAC_REQUIRE(lskip_ewald, E_ARG, "null output");
-    // mts_forces.f:232-239
-    *lskip_ewald = erfc_as(alphal * rinn) < 10.0e-4 ? 1 : 0;
+    // mts_forces.f:232-239, in the reference's association: (qp alphal) rinn in
+    // the rational term and ((alphal rinn) alphal) rinn in the exponent, so the
+    // decision agrees with the reference to the last bit at the 1e-3 threshold
+    const double a1 = 0.2548296, a2 = -0.28449674, a3 = 1.4214137, a4 = -1.453152, a5 = 1.0614054;
+    const double qp = 0.3275911;
+    const double qt = 1.0 / (1.0 + qp * alphal * rinn);
+    const double expcst = std::exp(-alphal * rinn * alphal * rinn);
+    const double erfcst = ((((a5 * qt + a4) * qt + a3) * qt + a2) * qt + a1) * qt * expcst;
+    *lskip_ewald = erfcst < 10.0e-4 ? 1 : 0;
   });
 }
 
@@ -629,53 +636,90 @@ extern "C" int orac_hip_force_alchemy_shift(
 
 // ---------------------------------------------------------------- corrections
 
+// Per-atom gather index of a pair (or term) list: for atom a, the signed slots
+// +-(p + 1) of the terms that touch it, in term order (a counting sort).  It is
+// rebuilt and uploaded only when the list bytes or the atom count change, so a
+// caller that passes the same topology every step pays one memcmp per call.
+struct GatherIndex {
+  const int *ia = nullptr, *ib = nullptr, *g_off = nullptr, *g_slot = nullptr;
+};
+static GatherIndex gather_index(orac_hip_ctx *ctx, const std::string &pfx, const int32_t *list, int stride,
+                                int arity, int n, int natoms) {
+  BufSet &B = ctx->B;
+  hipStream_t st = ctx->stream;
+  auto &key = B.cache[pfx + "#list"];
+  const size_t nb = (size_t)n * stride * sizeof(int32_t);
+  GatherIndex g;
+  const bool same = key.size() == nb + sizeof(int) && (nb == 0 || std::memcmp(key.data(), list, nb) == 0) &&
+                    std::memcmp(key.data() + nb, &natoms, sizeof(int)) == 0 && B.bufs.count(pfx + "goff");
+  if (!same) {
+    std::vector<int> first(std::max(1, n)), second(std::max(1, n)), cnt(natoms + 1, 0);
+    for (int p = 0; p < n; ++p)
+      for (int k = 0; k < arity; ++k) {
+        const int a = list[(size_t)stride * p + k] - 1;
+        ORAC_REQUIRE(a >= 0 && a < natoms, E_ARG, "pair or term list out of range");
+        ++cnt[a + 1];
+      }
+    for (int a = 0; a < natoms; ++a) cnt[a + 1] += cnt[a];
+    std::vector<int> slot(std::max<size_t>(1, (size_t)cnt[natoms])), cur(cnt.begin(), cnt.end() - 1);
+    for (int p = 0; p < n; ++p) {
+      for (int k = 0; k < arity; ++k) {
+        const int a = list[(size_t)stride * p + k] - 1;
+        // pairs: +(p+1) for the first atom, -(p+1) for the second; terms: arity*p + position
+        slot[cur[a]++] = arity == 2 ? (k == 0 ? p + 1 : -(p + 1)) : arity * p + k;
+      }
+      first[p] = list[(size_t)stride * p] - 1;
+      second[p] = arity >= 2 ? list[(size_t)stride * p + 1] - 1 : 0;
+    }
+    B.put_vec(pfx + "ia", first, st, false);
+    B.put_vec(pfx + "ib", second, st, false);
+    B.put_vec(pfx + "goff", cnt, st, false);
+    B.put_vec(pfx + "gp", slot, st, false);
+    key.assign(reinterpret_cast<const char *>(list), reinterpret_cast<const char *>(list) + nb);
+    key.insert(key.end(), reinterpret_cast<const char *>(&natoms), reinterpret_cast<const char *>(&natoms) + sizeof(int));
+  }
+  g.ia = B.get<int>(pfx + "ia", std::max(1, n));
+  g.ib = B.get<int>(pfx + "ib", std::max(1, n));
+  g.g_off = B.get<int>(pfx + "goff", natoms + 1);
+  g.g_slot = B.get<int>(pfx + "gp", 1);
+  return g;
+}
+
+// One correction launch on pairs list(stride, nlist) (1-based atoms in the
+// first two rows): forces added to fp when h.iz == 1, e_out[h.ncol] energies.
 static void run_corr_pairs(orac_hip_ctx *ctx, CorrDesc &h, const int32_t *list, int nlist, int natoms,
-                           const std::string &pfx, double e_out[4], double *fpx, double *fpy, double *fpz) {
+                           const std::string &pfx, double *e_out, double *fpx, double *fpy, double *fpz,
+                           int stride = 2) {
   hipStream_t st = ctx->stream;
   BufSet &B = ctx->B;
-  std::vector<int> ia(std::max(1, nlist)), ib(std::max(1, nlist));
-  std::vector<std::vector<int>> per(natoms);
-  for (int p = 0; p < nlist; ++p) {
-    ia[p] = list[2 * p] - 1;
-    ib[p] = list[2 * p + 1] - 1;
-    ORAC_REQUIRE(ia[p] >= 0 && ia[p] < natoms && ib[p] >= 0 && ib[p] < natoms, E_ARG, "pair list out of range");
-    per[ia[p]].push_back(p + 1);
-    per[ib[p]].push_back(-(p + 1));
-  }
-  std::vector<int> goff(natoms + 1, 0), gp;
-  for (int a = 0; a < natoms; ++a) {
-    for (int v : per[a]) gp.push_back(v);
-    goff[a + 1] = (int)gp.size();
-  }
-  if (gp.empty()) gp.push_back(0);
+  const int ncol = h.ncol ? h.ncol : 4;
+  GatherIndex gi = gather_index(ctx, pfx, list, stride, 2, nlist, natoms);
   h.npairs = nlist;
   h.natoms = natoms;
-  h.ia = B.put_vec(pfx + "ia", ia, st, false);
-  h.ib = B.put_vec(pfx + "ib", ib, st, false);
-  h.g_off = B.put_vec(pfx + "goff", goff, st, false);
-  h.g_pair = B.put_vec(pfx + "gp", gp, st, false);
+  h.ia = gi.ia;
+  h.ib = gi.ib;
+  h.g_off = gi.g_off;
+  h.g_pair = gi.g_slot;
   h.pf = B.get<double>(pfx + "pf", 3 * (size_t)std::max(1, nlist));
-  h.pe = B.get<double>(pfx + "pe", 4 * (size_t)std::max(1, nlist));
+  h.pe = B.get<double>(pfx + "pe", ncol * (size_t)std::max(1, nlist));
   h.fx = B.get<double>(pfx + "fx", natoms);
   h.fy = B.get<double>(pfx + "fy", natoms);
   h.fz = B.get<double>(pfx + "fz", natoms);
-  launch_zero_f64(h.fx, natoms, st);
-  launch_zero_f64(h.fy, natoms, st);
-  launch_zero_f64(h.fz, natoms, st);
-  double *dout = B.get<double>(pfx + "out", 4);
+  h.write = 1;  // the gather writes every atom's sum (zero where no pair touches it)
+  double *dout = B.get<double>(pfx + "out", ncol);
   h.out = dout;
   CorrDesc *dd = B.put(pfx + "desc", &h, 1, st, false);
   launch_corr(dd, &h, 1, st);
   ORAC_HIP(hipGetLastError());
   std::vector<double> fx(natoms), fy(natoms), fz(natoms);
-  if (h.iz == 1 && natoms) {
+  if (h.iz == 1 && natoms && nlist > 0) {
     ORAC_HIP(hipMemcpyAsync(fx.data(), h.fx, natoms * sizeof(double), hipMemcpyDeviceToHost, st));
     ORAC_HIP(hipMemcpyAsync(fy.data(), h.fy, natoms * sizeof(double), hipMemcpyDeviceToHost, st));
     ORAC_HIP(hipMemcpyAsync(fz.data(), h.fz, natoms * sizeof(double), hipMemcpyDeviceToHost, st));
   }
-  ORAC_HIP(hipMemcpyAsync(e_out, dout, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+  ORAC_HIP(hipMemcpyAsync(e_out, dout, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipStreamSynchronize(st));
-  if (h.iz == 1)
+  if (h.iz == 1 && nlist > 0)
     for (int a = 0; a < natoms; ++a) {
       fpx[a] = fpx[a] + fx[a];
       fpy[a] = fpy[a] + fy[a];
@@ -687,11 +731,11 @@ static void run_corr_pairs(orac_hip_ctx *ctx, CorrDesc &h, const int32_t *list, 
 // uploaded for the correction kernel; a pair below the table's first node is an error
 static void erf_table_for(orac_hip_ctx *ctx, CorrDesc &h, const std::string &pfx, const double *tab, double delew,
                           double rlew, const int32_t *list, int n, int mb, const double *x0, const double *y0,
-                          const double *z0) {
+                          const double *z0, int stride = 2) {
   ORAC_REQUIRE(tab && delew > 0.0, E_ARG, "ERF_CORR needs its table and a positive spacing");
   int icmax = 1;
   for (int p = 0; p < n; ++p) {
-    const int a = list[2 * p] - 1, b = list[2 * p + 1] - 1;
+    const int a = list[(size_t)stride * p] - 1, b = list[(size_t)stride * p + 1] - 1;
     ORAC_REQUIRE(a >= 0 && a < mb && b >= 0 && b < mb, E_ARG, "pair list out of range");
     const double xab = x0[a] - x0[b], yab = y0[a] - y0[b], zab = z0[a] - z0[b];
     const double rsp = std::sqrt(xab * xab + yab * yab + zab * zab);
@@ -815,6 +859,54 @@ extern "C" int orac_hip_ferrf_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index
   });
 }
 
+extern "C" int orac_hip_ferrf_alchemy_add(orac_hip_ctx *ctx, const int32_t *ss_index, double alphal,
+                                          const double *charge, const double *lambdaq, const double *lambdaq0,
+                                          const double *x0, const double *y0, const double *z0, const int32_t *list,
+                                          int32_t nlist, double *fscnstr_slt, double *fscnstr_slv, double *fpx,
+                                          double *fpy, double *fpz, int32_t erf_corr, const double *erf_arr_corr,
+                                          double delew, double rlew, double *dwrk, int32_t nato, int32_t intramol) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(ss_index && charge && lambdaq && lambdaq0 && x0 && y0 && z0 && fscnstr_slt && fscnstr_slv && dwrk &&
+                     fpx && fpy && fpz,
+                 E_ARG, "null argument");
+    ORAC_REQUIRE(nato >= 0 && nlist >= 0, E_ARG, "bad sizes");
+    ORAC_REQUIRE(nlist == 0 || list, E_ARG, "null pair list");
+    for (int p = 0; p < nlist; ++p)
+      ORAC_REQUIRE(list[3 * p + 2] == 0 || list[3 * p + 2] == 1, E_ARG, "ferrf_alchemy_add: facQ must be 0 or 1");
+    for (int a = 0; a < nato; ++a)
+      ORAC_REQUIRE(ss_index[a] == 1 || ss_index[a] == 2, E_ARG, "ss_index must be 1 or 2");
+    hipStream_t st = ctx->stream;
+    BufSet &B = ctx->B;
+    CorrDesc h;
+    std::memset(&h, 0, sizeof h);
+    if (erf_corr && nlist > 0)
+      erf_table_for(ctx, h, "cq_", erf_arr_corr, delew, rlew, list, nlist, nato, x0, y0, z0, 3);
+    std::vector<int> facq(std::max(1, nlist));
+    for (int p = 0; p < nlist; ++p) facq[p] = list[3 * p + 2];
+    h.x = B.put("cq_x", x0, nato, st, false);
+    h.y = B.put("cq_y", y0, nato, st, false);
+    h.z = B.put("cq_z", z0, nato, st, false);
+    h.qa = B.put("cq_q", charge, nato, st, true);
+    h.q = h.qa;
+    h.lq = B.put("cq_lq", lambdaq, nato, st, false);
+    h.lq0 = B.put("cq_lq0", lambdaq0, nato, st, false);
+    h.ss = B.put("cq_ss", ss_index, nato, st, true);
+    h.facq = B.put_vec("cq_facq", facq, st, true);
+    h.alpha = alphal;
+    h.twrtpi = 2.0 / std::sqrt(4.0 * std::atan(1.0));
+    h.iz = 1;
+    h.kind = 3;
+    h.ncol = 8;
+    h.intramol = intramol ? 1 : 0;
+    double e[8];
+    run_corr_pairs(ctx, h, list, nlist, nato, "cq_", e, fpx, fpy, fpz, 3);
+    *fscnstr_slt = e[4];
+    *fscnstr_slv = e[5];
+    *dwrk = e[4] + e[5] - e[6];  // ferrf_alchemy_add.f:201-203: fsrtal(1) + fsrtal(2) - en0
+  });
+}
+
 extern "C" int orac_hip_cself_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, int32_t nato, double alphal,
                                       double kcut, const double *charge, const double *charge0, double *self_slt,
                                       double *dwrk) {
@@ -842,11 +934,23 @@ extern "C" int orac_hip_cself_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index
   });
 }
 
+// True when `nb` bytes at `host` differ from the copy cached under `name`
+// (and then the cache holds the new bytes).
+static bool bytes_changed(BufSet &B, const std::string &name, const void *host, size_t nb) {
+  auto &c = B.cache[name];
+  if (c.size() == nb && (nb == 0 || std::memcmp(c.data(), host, nb) == 0)) return false;
+  c.assign(reinterpret_cast<const char *>(host), reinterpret_cast<const char *>(host) + nb);
+  return true;
+}
+
 // Bonded terms of `arity` atoms each (lst(arity, n), 1-based) on the
 // coordinates as given: per-term force components on the device, gathered per
 // atom in term order and added to fp; e = energies by the class of each term's
-// first atom.  pots: pota, potb (and potc..potg for bends).  Returns the
-// kernel's status word (torsions: 1 singular, 2 unsupported multiplicity).
+// first atom.  pots: pota, potb (and potc..potg for bends).  The topology, the
+// parameters and the per-atom gather index are rebuilt and uploaded only when
+// their bytes change (these routines run at the innermost r-RESPA level with
+// the same topology every call); the coordinates go up on every call.
+// Returns 0, 1 (a singular torsion) or 2 (an unsupported multiplicity).
 static int bonded_eval(orac_hip_ctx *ctx, const char *name, int arity, const int32_t *ss_index, const int32_t *lst,
                        int32_t n, const double *xp0, const double *yp0, const double *zp0,
                        const double *const *pots, int npots, double *fpx, double *fpy, double *fpz, int32_t mb,
@@ -855,33 +959,51 @@ static int bonded_eval(orac_hip_ctx *ctx, const char *name, int arity, const int
   for (int k = 0; k < npots; ++k) ORAC_REQUIRE(pots[k], E_ARG, "null argument");
   hipStream_t st = ctx->stream;
   BufSet &B = ctx->B;
-  std::vector<int4> at(n);
-  std::vector<std::vector<int>> per(mb);
-  for (int i = 0; i < n; ++i) {
-    int l[4] = {0, 0, 0, 0};
-    for (int k = 0; k < arity; ++k) {
-      l[k] = lst[arity * i + k] - 1;
-      ORAC_REQUIRE(l[k] >= 0 && l[k] < mb, E_ARG, std::string(name) + ": atom index out of range");
-      per[l[k]].push_back(4 * i + k);
+  const std::string pfx = std::string("t_") + name + "_";
+  std::vector<char> key((size_t)n * arity * sizeof(int32_t) + sizeof(int32_t));
+  std::memcpy(key.data(), lst, (size_t)n * arity * sizeof(int32_t));
+  std::memcpy(key.data() + key.size() - sizeof(int32_t), &mb, sizeof(int32_t));
+  if (bytes_changed(B, pfx + "#topo", key.data(), key.size()) || !B.bufs.count(pfx + "gs")) {
+    std::vector<int4> at(n);
+    std::vector<int> cnt(mb + 1, 0);
+    for (int i = 0; i < n; ++i) {
+      int l[4] = {0, 0, 0, 0};
+      for (int k = 0; k < arity; ++k) {
+        l[k] = lst[(size_t)arity * i + k] - 1;
+        if (l[k] < 0 || l[k] >= mb) {
+          B.cache.erase(pfx + "#topo");
+          ORAC_REQUIRE(false, E_ARG, std::string(name) + ": atom index out of range");
+        }
+        ++cnt[l[k] + 1];
+      }
+      at[i] = make_int4(l[0], l[1], l[2], l[3]);
     }
-    at[i] = make_int4(l[0], l[1], l[2], l[3]);
+    for (int a = 0; a < mb; ++a) cnt[a + 1] += cnt[a];
+    std::vector<int> gs(std::max(1, cnt[mb])), cur(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < arity; ++k) gs[cur[lst[(size_t)arity * i + k] - 1]++] = 4 * i + k;
+    B.put_vec(pfx + "at", at, st, false);
+    B.put_vec(pfx + "goff", cnt, st, false);
+    B.put_vec(pfx + "gs", gs, st, false);
   }
-  for (int a = 0; a < mb; ++a) ORAC_REQUIRE(ss_index[a] == 1 || ss_index[a] == 2, E_ARG, "ss_index must be 1 or 2");
-  std::vector<int> goff(mb + 1, 0), gs;
-  for (int a = 0; a < mb; ++a) {
-    for (int v : per[a]) gs.push_back(v);
-    goff[a + 1] = (int)gs.size();
+  if (bytes_changed(B, pfx + "#ss", ss_index, (size_t)mb * sizeof(int32_t))) {
+    for (int a = 0; a < mb; ++a)
+      if (ss_index[a] != 1 && ss_index[a] != 2) {
+        B.cache.erase(pfx + "#ss");
+        ORAC_REQUIRE(false, E_ARG, "ss_index must be 1 or 2");
+      }
+    B.put(pfx + "ss", ss_index, mb, st, false);
   }
-  static const char *pn[7] = {"t_pa", "t_pb", "t_pc", "t_pd", "t_pe", "t_pf", "t_pg"};
+  static const char *pn[7] = {"pa", "pb", "pc", "pd", "pe", "pf", "pg"};
   const double *dp[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  for (int k = 0; k < npots; ++k) dp[k] = B.put(pn[k], pots[k], n, st, false);
+  for (int k = 0; k < npots; ++k) dp[k] = B.put(pfx + pn[k], pots[k], n, st, true);
   TorsDesc h;
   std::memset(&h, 0, sizeof h);
   h.ntors = n;
   h.natoms = mb;
   h.arity = arity;
   h.improper = improper ? 1 : 0;
-  h.atoms = B.put_vec("t_at", at, st, false);
+  h.atoms = B.get<int4>(pfx + "at", n);
   h.pota = dp[0];
   h.potb = dp[1];
   h.potc = dp[2];
@@ -892,39 +1014,38 @@ static int bonded_eval(orac_hip_ctx *ctx, const char *name, int arity, const int
   h.x = B.put("t_x", xp0, mb, st, false);
   h.y = B.put("t_y", yp0, mb, st, false);
   h.z = B.put("t_z", zp0, mb, st, false);
-  h.ss = B.put("t_ss", ss_index, mb, st, true);
+  h.ss = B.get<int>(pfx + "ss", mb);
   h.tf = B.get<double>("t_tf", 12 * (size_t)n);
   h.te = B.get<double>("t_te", 2 * (size_t)n);
   h.status = B.get<int>("t_status", 2);
-  launch_zero_u64(reinterpret_cast<unsigned long long *>(h.status), 1, st);
-  h.g_off = B.put_vec("t_goff", goff, st, false);
-  h.g_slot = B.put_vec("t_gs", gs, st, false);
+  launch_zero_u64(reinterpret_cast<unsigned long long *>(h.status), 1, st);  // both status words
+  h.g_off = B.get<int>(pfx + "goff", mb + 1);
+  h.g_slot = B.get<int>(pfx + "gs", 1);
   h.fx = B.get<double>("t_fx", mb);
   h.fy = B.get<double>("t_fy", mb);
   h.fz = B.get<double>("t_fz", mb);
-  launch_zero_f64(h.fx, mb, st);
-  launch_zero_f64(h.fy, mb, st);
-  launch_zero_f64(h.fz, mb, st);
   double *dout = B.get<double>("t_out", 2);
   h.out = dout;
   TorsDesc *dd = B.put("t_desc", &h, 1, st, false);
   launch_tors(dd, h, st);
   ORAC_HIP(hipGetLastError());
   std::vector<double> fx(mb), fy(mb), fz(mb);
-  int status = 0;
+  int status[2] = {0, 0};
   ORAC_HIP(hipMemcpyAsync(fx.data(), h.fx, mb * sizeof(double), hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipMemcpyAsync(fy.data(), h.fy, mb * sizeof(double), hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipMemcpyAsync(fz.data(), h.fz, mb * sizeof(double), hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipMemcpyAsync(e, dout, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-  ORAC_HIP(hipMemcpyAsync(&status, h.status, sizeof(int), hipMemcpyDeviceToHost, st));
+  ORAC_HIP(hipMemcpyAsync(status, h.status, sizeof status, hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipStreamSynchronize(st));
-  if (status == 0)
+  // an unsupported multiplicity (the reference stops) outranks a singular torsion
+  const int code = status[1] ? 2 : (status[0] ? 1 : 0);
+  if (code == 0)
     for (int a = 0; a < mb; ++a) {
       fpx[a] = fpx[a] + fx[a];
       fpy[a] = fpy[a] + fy[a];
       fpz[a] = fpz[a] + fz[a];
     }
-  return status;
+  return code;
 }
 
 extern "C" int orac_hip_fptors(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *ltor, int32_t ltors,
@@ -1055,6 +1176,66 @@ extern "C" int orac_hip_fnb14(orac_hip_ctx *ctx, const int32_t *ss_index, const 
   });
 }
 
+extern "C" int orac_hip_fnb14_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0,
+                                      const double *yp0, const double *zp0, const double *charge, int32_t nato,
+                                      const double *ecc12, const double *ecc6, int32_t cut, int32_t ewald,
+                                      double alphal, const int32_t *int14, int32_t int14p, const int32_t *type14,
+                                      double fudge, double lj_fudge, double *fpx, double *fpy, double *fpz,
+                                      double *uconf_slt, double *uconf_slv, double *ucoul_slt, double *ucoul_slv,
+                                      const double *lambda, const double *lambdaq, const double *lambda0,
+                                      const double *lambdaq0, double *dwrk) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(uconf_slt && uconf_slv && ucoul_slt && ucoul_slv && dwrk, E_ARG, "null output");
+    *uconf_slt = *uconf_slv = *ucoul_slt = *ucoul_slv = 0.0;
+    *dwrk = 0.0;  // fnb14_alchemy.f:117 (serial build; mts_intra_n1.f:61 zeroes it in the OpenMP one)
+    if (int14p == 0) return;
+    ORAC_REQUIRE(ss_index && xp0 && yp0 && zp0 && charge && ecc12 && ecc6 && int14 && type14 && lambda && lambdaq &&
+                     lambda0 && lambdaq0 && fpx && fpy && fpz && nato > 0 && int14p > 0,
+                 E_ARG, "null argument");
+    hipStream_t st = ctx->stream;
+    BufSet &B = ctx->B;
+    int lmax = 1;
+    for (int k = 0; k < int14p; ++k) {
+      ORAC_REQUIRE(type14[k] >= 1, E_ARG, "fnb14_alchemy: type14 entries are 1-based packed indices");
+      lmax = std::max(lmax, type14[k]);
+    }
+    for (int a = 0; a < nato; ++a)
+      ORAC_REQUIRE(ss_index[a] == 1 || ss_index[a] == 2, E_ARG, "ss_index must be 1 or 2");
+    CorrDesc h;
+    std::memset(&h, 0, sizeof h);
+    h.xp = B.put("na_x", xp0, nato, st, false);
+    h.yp = B.put("na_y", yp0, nato, st, false);
+    h.zp = B.put("na_z", zp0, nato, st, false);
+    h.q = B.put("na_q", charge, nato, st, true);
+    h.ss = B.put("na_ss", ss_index, nato, st, true);
+    h.pair_type = B.put("na_t14", type14, int14p, st, true);
+    h.ecc12 = B.put("na_e12", ecc12, lmax, st, true);
+    h.ecc6 = B.put("na_e6", ecc6, lmax, st, true);
+    h.lam = B.put("na_l", lambda, nato, st, false);
+    h.lq = B.put("na_lq", lambdaq, nato, st, false);
+    h.lam0 = B.put("na_l0", lambda0, nato, st, false);
+    h.lq0 = B.put("na_lq0", lambdaq0, nato, st, false);
+    h.alpha = alphal;
+    h.twrtpi = 2.0 / std::sqrt(4.0 * std::atan(1.0));
+    h.fudge = fudge;
+    h.lj_fudge = lj_fudge;
+    h.iz = 1;
+    h.kind = 4;
+    h.ncol = 8;
+    h.ewald = (ewald && cut) ? 1 : 0;  // fnb14_alchemy.f:119: the erfc branch needs ewald and cut
+    double e[8];
+    run_corr_pairs(ctx, h, int14, int14p, nato, "na_", e, fpx, fpy, fpz);
+    *ucoul_slt = e[0];
+    *ucoul_slv = e[1];
+    *uconf_slt = e[2];
+    *uconf_slv = e[3];
+    double dw = 0.0;  // fnb14_alchemy.f:349-351, class by class
+    for (int k = 0; k < 2; ++k) dw = dw + e[2 + k] + e[k] - (e[6 + k] + e[4 + k]);
+    *dwrk = dw;
+  });
+}
+
 extern "C" int orac_hip_fnbgrp(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
                                const double *zp0, const double *charge, const int32_t *type, const double *ecc12,
                                const double *ecc6, int32_t ewald, double alphal, const int32_t *ingrp,
@@ -1100,3 +1281,16 @@ extern "C" int orac_hip_fnbgrp(orac_hip_ctx *ctx, const int32_t *ss_index, const
   });
 }
 
+
+extern "C" int orac_hip_fnbgrp_alchemy(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0,
+                                       const double *yp0, const double *zp0, const double *charge, const int32_t *type,
+                                       const double *ecc12, const double *ecc6, int32_t ewald, double alphal,
+                                       const int32_t *ingrp, int32_t ingrpp, double *fpx, double *fpy, double *fpz,
+                                       double *uconf_slt, double *uconf_slv, double *ucoul_slt, double *ucoul_slv,
+                                       const double *lambdaq, int32_t nato) {
+  // fnbgrp_alchemy.f:1-220 computes alphal0 from lambdaq (:152-157) and never
+  // reads it: its arithmetic is fnbgrp.f's, term for term
+  (void)lambdaq;
+  return orac_hip_fnbgrp(ctx, ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alphal, ingrp, ingrpp, fpx,
+                         fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, nato);
+}

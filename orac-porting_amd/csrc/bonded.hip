@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
                sb3 = sqrt(d.improper ? s3 : fabs(s3));
   const double aux = sb1 * sb2;
   if (!d.improper && fabs(aux * sb3) < 1.0e-25) {  // singular angle: the reference perturbs at random (not restated)
-    d.status[0] = 1;
+    d.status[0] = 1;  // status[0]: singular torsion; status[1]: unsupported multiplicity
     return;
   }
   double coa = (cb1 * cb2 - cb3) / aux;
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
   const int ntph = abs(tmp);
   const bool harmonic = d.improper && d.potc[i] > 0.0;
   if (ntph == 0 && !harmonic) {
-    d.status[0] = 2;
+    d.status[1] = 2;
     return;
   }
   const double sig = ntph ? (double)(tmp / ntph) : 0.0;
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
     const double da = bb - pa;
     enrg = b * (da * da);
   } else if (d.improper && (ntph == 5 || ntph > 6)) {  // fitors.f stops
-    d.status[0] = 2;
+    d.status[1] = 2;
     return;
   } else switch (ntph) {
     case 1: enrg = sig * (fabs(b) - coa * b); qforce = sig * b; break;
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
       qforce = sig * (-(qforce + dauxn));
       break;
     }
-    default: d.status[0] = 2; return;  // the reference stops (fptors.f:291-297)
+    default: d.status[1] = 2; return;  // the reference stops (fptors.f:291-297)
   }
   te[type - 1] = enrg;
   // fptors.f:152-183 (angle-cosine derivatives) and 299-322 (force)
@@ -239,7 +239,8 @@ __global__ void __launch_bounds__(256) k_bond_bend_terms(const TorsDesc *__restr
     e = pb * (da * da);
   }
   if (fabs(pd) > 1.0e-5 && (fabs(pe) < 1.0e-5 || fabs(pf) > 1.0e-5)) {  // Urey-Bradley 1-3 term
-    const double r31[3] = {r32[0] - r12[0], r32[1] - r12[1], r32[2] - r12[2]};
+    // fpbend.f:154-156: the 1-3 vector from the coordinates themselves
+    const double r31[3] = {d.x[l.z] - d.x[l.x], d.y[l.z] - d.y[l.x], d.z[l.z] - d.z[l.x]};
     const double rsp31 = sqrt(r31[0] * r31[0] + r31[1] * r31[1] + r31[2] * r31[2]);
     const double qf = -2.0 * pd * (pc - rsp31);
 #pragma unroll
@@ -296,9 +297,9 @@ __global__ void __launch_bounds__(256) k_tors_gather(const TorsDesc *__restrict_
     sy += d.tf[12 * (size_t)i + 4 + k];
     sz += d.tf[12 * (size_t)i + 8 + k];
   }
-  d.fx[a] += sx;
-  d.fy[a] += sy;
-  d.fz[a] += sz;
+  d.fx[a] = sx;  // every atom's sum is written (zero where no term touches it)
+  d.fy[a] = sy;
+  d.fz[a] = sz;
 }
 
 // d.out[k] = sum over torsions of te[k], one block per column, fixed order

@@ -50,6 +50,12 @@ struct System {
   double lam[8] = {1, 0, 1, 0, 1, 0, 1, 0};
   double self_dw = 0.0;
   double *qc = nullptr, *q0 = nullptr, *afac = nullptr;
+  // per-step reciprocal corrections as created (ferrf pairs, 0-based) and, for
+  // an alchemical system, the intra-alchemical pairs of ferrf_alchemy_add
+  std::vector<int> corr_ia, corr_ib;
+  std::vector<double> corr_fud;
+  int nqq = 0;
+  double *lq = nullptr, *lq0 = nullptr;  // per-atom lambdaq, lambdaq0 (device)
   PmeDesc pme0;
   bool have_list = false;
   int list_fmt = -1;      // cell_order tuning the current list was built with
@@ -143,6 +149,56 @@ static Batch &get_batch(orac_hip_ctx *ctx, int32_t id) {
 }
 
 // ---------------------------------------------------------------- system build
+
+// The system's correction list: its ferrf pairs (kind 0) and, when given, the
+// intra-alchemical pairs qq(3, nqq) of ferrf_alchemy_add (kind 3, 1-based with
+// facQ), in that order, with the per-atom gather index in pair order.
+static void corr_upload(System &S, const int32_t *qq, int nqq, hipStream_t st) {
+  BufSet &B = S.B;
+  CorrDesc &h = S.corr;
+  const int n = S.natoms, nc = (int)S.corr_ia.size(), np = nc + nqq;
+  std::vector<int> ia(std::max(1, np)), ib(std::max(1, np)), kind(std::max(1, np), 0), facq(std::max(1, np), 0);
+  std::vector<double> fud(std::max(1, np), 1.0);
+  for (int k = 0; k < nc; ++k) {
+    ia[k] = S.corr_ia[k];
+    ib[k] = S.corr_ib[k];
+    fud[k] = S.corr_fud[k];
+  }
+  for (int k = 0; k < nqq; ++k) {
+    ia[nc + k] = qq[3 * k] - 1;
+    ib[nc + k] = qq[3 * k + 1] - 1;
+    facq[nc + k] = qq[3 * k + 2];
+    kind[nc + k] = 3;
+    ORAC_REQUIRE(ia[nc + k] >= 0 && ia[nc + k] < n && ib[nc + k] >= 0 && ib[nc + k] < n, E_ARG,
+                 "intra-alchemical pair out of range");
+    ORAC_REQUIRE(facq[nc + k] == 0 || facq[nc + k] == 1, E_ARG, "facQ must be 0 or 1");
+  }
+  std::vector<int> cnt(n + 1, 0);
+  for (int k = 0; k < np; ++k) {
+    ++cnt[ia[k] + 1];
+    ++cnt[ib[k] + 1];
+  }
+  for (int a = 0; a < n; ++a) cnt[a + 1] += cnt[a];
+  std::vector<int> gp(std::max(1, cnt[n])), cur(cnt.begin(), cnt.end() - 1);
+  for (int k = 0; k < np; ++k) {
+    gp[cur[ia[k]]++] = k + 1;
+    gp[cur[ib[k]]++] = -(k + 1);
+  }
+  const int ncol = nqq > 0 ? 8 : 4;
+  h.npairs = np;
+  h.natoms = n;
+  h.ncol = ncol;
+  h.ia = B.put_vec("cia", ia, st, false);
+  h.ib = B.put_vec("cib", ib, st, false);
+  h.fudge_p = B.put_vec("cfud", fud, st, false);
+  h.pkind = nqq > 0 ? B.put_vec("ckind", kind, st, false) : nullptr;
+  h.facq = nqq > 0 ? B.put_vec("cfacq", facq, st, false) : nullptr;
+  h.g_off = B.put_vec("cgoff", cnt, st, false);
+  h.g_pair = B.put_vec("cgp", gp, st, false);
+  h.pf = B.get<double>("cpf", 3 * (size_t)std::max(1, np));
+  h.pe = B.get<double>("cpe", ncol * (size_t)std::max(1, np));
+  S.nqq = nqq;
+}
 
 static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_desc &d) {
   hipStream_t st = ctx->stream;
@@ -415,32 +471,17 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   std::memset(&h, 0, sizeof h);
   {
     const int nc = d.ncorr;
-    std::vector<int> ia(std::max(1, nc)), ib(std::max(1, nc));
-    std::vector<double> fud(std::max(1, nc), 1.0);
-    std::vector<std::vector<int>> per(n);
+    S.corr_ia.resize(nc);
+    S.corr_ib.resize(nc);
+    S.corr_fud.assign(nc, 1.0);
     for (int k = 0; k < nc; ++k) {
-      ia[k] = d.corr_pairs[2 * k] - 1;
-      ib[k] = d.corr_pairs[2 * k + 1] - 1;
-      ORAC_REQUIRE(ia[k] >= 0 && ia[k] < n && ib[k] >= 0 && ib[k] < n, E_ARG, "correction pair out of range");
-      if (d.corr_fudge) fud[k] = d.corr_fudge[k];
-      per[ia[k]].push_back(k + 1);
-      per[ib[k]].push_back(-(k + 1));
+      S.corr_ia[k] = d.corr_pairs[2 * k] - 1;
+      S.corr_ib[k] = d.corr_pairs[2 * k + 1] - 1;
+      ORAC_REQUIRE(S.corr_ia[k] >= 0 && S.corr_ia[k] < n && S.corr_ib[k] >= 0 && S.corr_ib[k] < n, E_ARG,
+                   "correction pair out of range");
+      if (d.corr_fudge) S.corr_fud[k] = d.corr_fudge[k];
     }
-    std::vector<int> goff(n + 1, 0), gp;
-    for (int a = 0; a < n; ++a) {
-      for (int v : per[a]) gp.push_back(v);
-      goff[a + 1] = (int)gp.size();
-    }
-    if (gp.empty()) gp.push_back(0);
-    h.npairs = nc;
-    h.natoms = n;
-    h.ia = B.put_vec("cia", ia, st, false);
-    h.ib = B.put_vec("cib", ib, st, false);
-    h.fudge_p = B.put_vec("cfud", fud, st, false);
-    h.g_off = B.put_vec("cgoff", goff, st, false);
-    h.g_pair = B.put_vec("cgp", gp, st, false);
-    h.pf = B.get<double>("cpf", 3 * (size_t)std::max(1, nc));
-    h.pe = B.get<double>("cpe", 4 * (size_t)std::max(1, nc));
+    corr_upload(S, nullptr, 0, st);
   }
   h.xp = S.xp;
   h.yp = S.yp;
@@ -1082,8 +1123,9 @@ extern "C" int orac_hip_system_results(orac_hip_ctx *ctx, int32_t sys, double *f
       energies[4] = out[20 + 3];
       energies[5] = out[20 + 5];
       energies[6] = out[20 + 4];
-      energies[7] = out[36 + 0];
-      energies[8] = out[36 + 1];
+      // corrections; the intra-alchemical term joins them (mtsmd.f:3041: coul_bnd_slt += falch_slt)
+      energies[7] = S.nqq > 0 ? out[36 + 0] + out[36 + 4] : out[36 + 0];
+      energies[8] = S.nqq > 0 ? out[36 + 1] + out[36 + 5] : out[36 + 1];
       // stress: direct st.co^T + (-vir + molecular correction)
       const double *s = out + 26;
       for (int r = 0; r < 3; ++r)
@@ -1114,6 +1156,18 @@ static void alch_apply_lambda(orac_hip_ctx *ctx, System &S) {
   ORAC_HIP(hipMemcpyAsync(S.afac, fac, sizeof fac, hipMemcpyHostToDevice, st));
   ORAC_HIP(hipMemcpyAsync(S.qc, qc.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
   ORAC_HIP(hipMemcpyAsync(S.q0, q0.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+  if (S.nqq > 0) {
+    // per-atom lambdaq: +laddq added, -lrmq removed, 0 otherwise (mtsmd.f:746-757)
+    std::vector<double> lq(n), lq0(n);
+    for (int a = 0; a < n; ++a) {
+      const int k = S.ss_alch[a];
+      lq[a] = k == 2 ? S.lam[4] : (k == -2 ? -S.lam[5] : 0.0);
+      lq0[a] = k == 2 ? S.lam[6] : (k == -2 ? -S.lam[7] : 0.0);
+    }
+    ORAC_HIP(hipMemcpyAsync(S.lq, lq.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    ORAC_HIP(hipMemcpyAsync(S.lq0, lq0.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    ORAC_HIP(hipStreamSynchronize(st));
+  }
   // cself_alchemy.f: solute atoms only, kcut = 1e9 (erfc term 0)
   double fs = 0.0, en0 = 0.0;
   for (int a = 0; a < n; ++a)
@@ -1158,6 +1212,18 @@ extern "C" int orac_hip_system_set_alchemy(orac_hip_ctx *ctx, int32_t sys, const
     S.pme.q = S.qc;
     S.corr.q = S.qc;
     S.corr.q0 = S.q0;
+    if (a->nqq > 0) {
+      // ferrf_alchemy_add on every evaluation (mtsmd.f:3034-3040), in the
+      // system's correction launch after its ferrf pairs
+      ORAC_REQUIRE(a->listqq, E_ARG, "null intra-alchemical pair list");
+      corr_upload(S, a->listqq, a->nqq, st);
+      S.lq = B.get<double>("lq", n);
+      S.lq0 = B.get<double>("lq0", n);
+      S.corr.qa = B.put("qa", S.q_host.data(), n, st, false);
+      S.corr.lq = S.lq;
+      S.corr.lq0 = S.lq0;
+      S.corr.intramol = S.intramol ? 1 : 0;
+    }
     PmeDesc &e = S.pme0;
     e = pme_desc_base(B, "a0_", S.plan, n, S.oc, S.volume, S.alpha, 1.0e9, S.precision == 0, ctx->tuning, st);
     e.xp = S.pme.xp;
@@ -1207,7 +1273,9 @@ extern "C" int orac_hip_system_alchemy_results(orac_hip_ctx *ctx, int32_t sys, d
     double o[72];
     ORAC_HIP(hipMemcpyAsync(o, S.out, sizeof o, hipMemcpyDeviceToHost, ctx->stream));
     ORAC_HIP(hipStreamSynchronize(ctx->stream));
-    const double direct = o[35], recip = o[0] - o[48], corr = o[36] + o[37] - o[38];
+    double corr = o[36] + o[37] - o[38];
+    if (S.nqq > 0) corr += o[40] + o[41] - o[42];  // ferrf_alchemy_add.f:203: dwrk_al
+    const double direct = o[35], recip = o[0] - o[48];
     out[1] = direct;
     out[2] = recip;
     out[3] = corr;

@@ -194,6 +194,16 @@ struct CorrDesc {
   double delew, rlew;
   const int *pair_type;               // fnb14: packed LJ index of each pair (type14, 1-based)
   double lj_fudge;                    // fnb14: LJ-FUDGE (fudge is QQ-FUDGE)
+  // kinds 3 (ferrf_alchemy_add) and 4 (fnb14_alchemy) write 8 energy columns per pair:
+  //   kind 3: [4 + class] the term on lambdaq, [6] en0 on lambdaq0
+  //   kind 4: coul, lj by class at [0..3] on lambda, at [4..7] on lambda0
+  int ncol;                           // energy columns per pair and in out (4 when 0)
+  const int *pkind;                   // per pair kind (overrides kind when set)
+  const int *facq;                    // kind 3: facQ of each pair (1: same alchemical kind, 0: added-removed)
+  const double *qa;                   // kind 3: the created charges (q holds the lambda charges of kind 0)
+  const double *lq, *lq0;             // per atom lambdaq, lambdaq0
+  const double *lam, *lam0;           // per atom lambda, lambda0 (kind 4)
+  int intramol;                       // kind 3: INTRAMOL (skip the facQ = 1 pairs)
 };
 void launch_corr(const CorrDesc *d_desc, const CorrDesc *h, int nsys, hipStream_t st);
 
@@ -210,7 +220,7 @@ struct TorsDesc {
   const int *ss;                      // ss_index of each atom
   double *tf;                         // [ntors][12] force components: x of l1..l4, y, z (unused slots zero)
   double *te;                         // [ntors][2] energy by the class of l1
-  int *status;                        // [1]: 1 singular torsion, 2 unsupported multiplicity (sticky)
+  int *status;                        // [0]: 1 singular torsion; [1]: 2 unsupported multiplicity (sticky words)
   const int *g_off, *g_slot;          // per atom gather CSR: 4 * torsion + position
   double *fx, *fy, *fz;               // accumulated (+=)
   double *out;                        // [2]: utors slt, slv

@@ -602,50 +602,90 @@ void launch_inner_list(int ngrp, int iz, const int *h_off, const int *h_idx, con
 
 // ---------------------------------------------------------------- ferrf / fnbgrp
 
+// The scaling factors of one alchemical 1-4 pair (fnb14_alchemy.f:131-155,
+// 161-185, 214-255): normal pairs and pairs of equal nonzero lambda keep the full
+// potential, a pair with one alchemical atom is scaled by 1 - |lambda| of that
+// atom, a pair of an added and a removed atom is switched off.
+__device__ __forceinline__ void alch14_fact(double li, double lj, double li0, double lj0, double &f, double &f0) {
+  if (li == 0.0 && lj != 0.0) {
+    f = 1.0 - fabs(lj);
+    f0 = 1.0 - fabs(lj0);
+  } else if (li != 0.0 && lj == 0.0) {
+    f = 1.0 - fabs(li);
+    f0 = 1.0 - fabs(li0);
+  } else if (li == 0.0 && lj == 0.0) {
+    f = 1.0;
+    f0 = 1.0;
+  } else if (li != lj) {
+    f = 0.0;
+    f0 = 0.0;
+  } else {
+    f = 1.0;
+    f0 = 1.0;
+  }
+}
+
+__device__ __forceinline__ double erfc_as_dev(double alphar, double &expcst) {
+  const double qt = 1.0 / (1.0 + AS_QP * alphar);
+  expcst = exp(-(alphar * alphar));
+  return ((((AS_A5 * qt + AS_A4) * qt + AS_A3) * qt + AS_A2) * qt + AS_A1) * qt * expcst;
+}
+
 __global__ void __launch_bounds__(256) k_corr_pairs(const CorrDesc *__restrict__ D) {
   const CorrDesc &d = D[blockIdx.y];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= d.npairs) return;
+  const int kind = d.pkind ? d.pkind[p] : d.kind;
   int ia = d.ia[p], ib = d.ib[p];
-  double xa, ya, za, xb, yb, zb;
-  if (d.x) {
-    xa = d.x[ia]; ya = d.y[ia]; za = d.z[ia];
-    xb = d.x[ib]; yb = d.y[ib]; zb = d.z[ib];
+  double xab, yab, zab;
+  if (kind == 1 || kind == 2 || kind == 4) {
+    // fnbgrp.f, fnb14.f, fnb14_alchemy.f: the coordinates as given, no PBC
+    xab = d.xp[ia] - d.xp[ib];
+    yab = d.yp[ia] - d.yp[ib];
+    zab = d.zp[ia] - d.zp[ib];
+  } else if (d.x) {
+    xab = d.x[ia] - d.x[ib];
+    yab = d.y[ia] - d.y[ib];
+    zab = d.z[ia] - d.z[ib];
   } else {
     const double *co = d.co;
-    xa = co[0] * d.xp[ia] + co[3] * d.yp[ia] + co[6] * d.zp[ia];
-    ya = co[1] * d.xp[ia] + co[4] * d.yp[ia] + co[7] * d.zp[ia];
-    za = co[2] * d.xp[ia] + co[5] * d.yp[ia] + co[8] * d.zp[ia];
-    xb = co[0] * d.xp[ib] + co[3] * d.yp[ib] + co[6] * d.zp[ib];
-    yb = co[1] * d.xp[ib] + co[4] * d.yp[ib] + co[7] * d.zp[ib];
-    zb = co[2] * d.xp[ib] + co[5] * d.yp[ib] + co[8] * d.zp[ib];
+    const double xa = co[0] * d.xp[ia] + co[3] * d.yp[ia] + co[6] * d.zp[ia];
+    const double ya = co[1] * d.xp[ia] + co[4] * d.yp[ia] + co[7] * d.zp[ia];
+    const double za = co[2] * d.xp[ia] + co[5] * d.yp[ia] + co[8] * d.zp[ia];
+    const double xb = co[0] * d.xp[ib] + co[3] * d.yp[ib] + co[6] * d.zp[ib];
+    const double yb = co[1] * d.xp[ib] + co[4] * d.yp[ib] + co[7] * d.zp[ib];
+    const double zb = co[2] * d.xp[ib] + co[5] * d.yp[ib] + co[8] * d.zp[ib];
+    xab = xa - xb;
+    yab = ya - yb;
+    zab = za - zb;
   }
-  double xab = xa - xb, yab = ya - yb, zab = za - zb;
-  double rsq = xab * xab + yab * yab + zab * zab;
-  double rsp = sqrt(rsq);
-  double ecoul = 0.0, elj = 0.0, qforce = 0.0;
-  if (d.kind == 0) {
+  const double rsq = xab * xab + yab * yab + zab * zab;
+  const double rsp = sqrt(rsq);
+  const int c = d.ss[ia] - 1;  // energies by the class of the first atom (ferrf.f:169, fnbgrp.f:130)
+  double ev[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double qforce = 0.0;
+  if (kind == 0) {
     // ferrf.f:162-188 (iz = 1) and 212-226 (iz = 0)
     double fud = d.fudge_p ? d.fudge_p[p] : d.fudge;
     double furpar = d.iz == 1 ? fud * d.q[ia] * d.q[ib] : d.q[ia] * d.q[ib];
     double alphar = d.alpha * rsp;
-    double qt = 1.0 / (1.0 + AS_QP * alphar);
-    double expcst = exp(-(alphar * alphar));
-    double erfcst = ((((AS_A5 * qt + AS_A4) * qt + AS_A3) * qt + AS_A2) * qt + AS_A1) * qt * expcst;
+    double expcst;
+    const double erfcst = erfc_as_dev(alphar, expcst);
     double erfst = 1.0 - erfcst;
+    double ecoul;
     if (d.erf_tab) {
       // ERF_CORR (ferrf.f:68-105 and 170-193): the reciprocal-cutoff correction
       // from its cubic spline table, interval ic with offset h
       int ic = (int)(1.0 + (rsp - d.rlew) / d.delew);
       ic = min(max(ic, 1), d.erf_n);
       const double h = rsp - (d.rlew + (ic - 1) * d.delew);
-      const double *c = d.erf_tab + 4 * (size_t)(ic - 1);
-      const double poly = c[0] + h * (c[1] + h * (c[2] + h * c[3] / 3) / 2);
+      const double *cs = d.erf_tab + 4 * (size_t)(ic - 1);
+      const double poly = cs[0] + h * (cs[1] + h * (cs[2] + h * cs[3] / 3) / 2);
       if (d.iz == 1) {
         const double qq = d.q[ia] * d.q[ib];
         furpar = fud * qq;
         const double corr = qq * poly;
-        const double dcorr = qq * (c[1] + h * (c[2] + h * c[3] / 2)) / rsp;
+        const double dcorr = qq * (cs[1] + h * (cs[2] + h * cs[3] / 2)) / rsp;
         qforce = -dcorr - furpar * (erfst - d.twrtpi * alphar * expcst) / (rsp * rsq);
         ecoul = -(furpar * erfst / rsp - corr);
       } else {
@@ -655,43 +695,115 @@ __global__ void __launch_bounds__(256) k_corr_pairs(const CorrDesc *__restrict__
       ecoul = -(furpar * erfst / rsp);
       if (d.iz == 1) qforce = -(furpar * (erfst - d.twrtpi * alphar * expcst) / (rsp * rsq));
     }
+    ev[c] = ecoul;
     if (d.q0) {
       // ferrf_alchemy.f:144-145,190-191: the same term on the lambda0 charges (en0)
       double furpar0 = d.iz == 1 ? fud * d.q0[ia] * d.q0[ib] : d.q0[ia] * d.q0[ib];
-      elj = -(furpar0 * erfst / rsp);
+      ev[2] = -(furpar0 * erfst / rsp);
     }
-  } else if (d.kind == 2) {
-    // fnb14.f:69-158: the coordinates as given, no PBC; LJ by the pair's packed
-    // type scaled by lj_fudge; charges by fudge; erfc only with ewald and cut
-    xab = d.xp[ia] - d.xp[ib];
-    yab = d.yp[ia] - d.yp[ib];
-    zab = d.zp[ia] - d.zp[ib];
-    rsq = xab * xab + yab * yab + zab * zab;
-    rsp = sqrt(rsq);
+  } else if (kind == 3) {
+    // ferrf_alchemy_add.f:94-141 (ERF_CORR) and 152-199: the intra-alchemical
+    // pairs beyond 1-4, with factl = (1 - |lambdaq_a|)(1 - |lambdaq_b|) - facQ on
+    // the created charges; with INTRAMOL only the added-removed pairs (:105)
+    const int fq = d.facq[p];
+    if (!(d.intramol && fq != 0)) {
+      const double facq = (double)fq;
+      const double factl = (1.0 - fabs(d.lq[ia])) * (1.0 - fabs(d.lq[ib])) - facq;
+      const double factl0 = (1.0 - fabs(d.lq0[ia])) * (1.0 - fabs(d.lq0[ib])) - facq;
+      const double qq = factl * d.qa[ia] * d.qa[ib];
+      const double qq0 = factl0 * d.qa[ia] * d.qa[ib];
+      const double alphar = d.alpha * rsp;
+      double expcst;
+      const double erfst = 1.0 - erfc_as_dev(alphar, expcst);
+      if (d.erf_tab) {
+        int ic = (int)(1.0 + (rsp - d.rlew) / d.delew);
+        ic = min(max(ic, 1), d.erf_n);
+        const double h = rsp - (d.rlew + (ic - 1) * d.delew);
+        const double *cs = d.erf_tab + 4 * (size_t)(ic - 1);
+        const double corr = cs[0] + h * (cs[1] + h * (cs[2] + h * cs[3] / 3) / 2);
+        const double dcorr = (cs[1] + h * (cs[2] + h * cs[3] / 2)) / rsp;
+        qforce = -qq * (dcorr + (erfst - d.twrtpi * alphar * expcst) / (rsp * rsq));
+        ev[4 + c] = -(qq * (erfst / rsp - corr));
+        ev[6] = -(qq0 * (erfst / rsp - corr));
+      } else {
+        qforce = -qq * (erfst - d.twrtpi * alphar * expcst) / (rsp * rsq);
+        ev[4 + c] = -(qq * erfst / rsp);
+        ev[6] = -(qq0 * erfst / rsp);
+      }
+    }
+  } else if (kind == 2) {
+    // fnb14.f:69-158: LJ by the pair's packed type scaled by lj_fudge; charges
+    // by fudge; erfc only with ewald and cut
     const int lij = d.pair_type[p] - 1;
     const double rsqi = 1.0 / rsq;
     const double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
     const double ssvir = 12.0 * d.ecc12[lij] * r12 - 6.0 * d.ecc6[lij] * r6;
-    elj = d.lj_fudge * (d.ecc12[lij] * r12 - d.ecc6[lij] * r6);
+    ev[2 + c] = d.lj_fudge * (d.ecc12[lij] * r12 - d.ecc6[lij] * r6);
     if (d.ewald) {
-      double alphar = d.alpha * rsp;
-      double qt = 1.0 / (1.0 + AS_QP * alphar);
-      double expcst = exp(-(alphar * alphar));
-      double erfcst = ((((AS_A5 * qt + AS_A4) * qt + AS_A3) * qt + AS_A2) * qt + AS_A1) * qt * expcst;
+      const double alphar = d.alpha * rsp;
+      double expcst;
+      const double erfcst = erfc_as_dev(alphar, expcst);
       const double furpar = d.fudge * d.q[ia] * d.q[ib];
-      ecoul = furpar * erfcst / rsp;
+      ev[c] = furpar * erfcst / rsp;
       qforce = d.lj_fudge * ssvir * rsqi + furpar * (erfcst + d.twrtpi * alphar * expcst) * (rsqi / rsp);
     } else {
-      ecoul = d.fudge * d.q[ia] * d.q[ib] / rsp;
+      const double ecoul = d.fudge * d.q[ia] * d.q[ib] / rsp;
+      ev[c] = ecoul;
       qforce = d.lj_fudge * ssvir * rsqi + ecoul * rsqi;
     }
+  } else if (kind == 4) {
+    // fnb14_alchemy.f:120-356, with the reference's own arithmetic: without
+    // Ewald the forces are unscaled and the LJ lambda0 energy uses alch_fact
+    // (:203-210); with Ewald only the repulsive LJ force term carries
+    // alch_fact (:286), and pairs of two alchemical atoms add the erf term
+    double af, af0, aq, aq0;
+    alch14_fact(d.lam[ia], d.lam[ib], d.lam0[ia], d.lam0[ib], af, af0);
+    alch14_fact(d.lq[ia], d.lq[ib], d.lq0[ia], d.lq0[ib], aq, aq0);
+    const int lij = d.pair_type[p] - 1;
+    const double rsqi = 1.0 / rsq;
+    const double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+    const double qi = d.q[ia], qj = d.q[ib];
+    if (!d.ewald) {
+      const double ssvir = 12.0 * d.ecc12[lij] * r12 - 6.0 * d.ecc6[lij] * r6;
+      qforce = d.lj_fudge * ssvir * rsqi;
+      const double elj = d.lj_fudge * (d.ecc12[lij] * r12 - d.ecc6[lij] * r6);
+      ev[2 + c] = af * elj;
+      ev[6 + c] = af * elj;
+      const double sv = d.fudge * qi * qj / rsp;
+      ev[c] = aq * sv;
+      ev[4 + c] = aq0 * sv;
+      qforce = qforce + sv * rsqi;
+    } else {
+      double aq2 = 0.0, aq20 = 0.0;
+      const double lqa = d.lq[ia], lqb = d.lq[ib];
+      if (lqa != 0.0 && lqb != 0.0) {
+        const double pr = (1.0 - fabs(lqa)) * (1.0 - fabs(lqb));
+        const double pr0 = (1.0 - fabs(d.lq0[ia])) * (1.0 - fabs(d.lq0[ib]));
+        aq2 = lqa != lqb ? -pr : 1.0 - pr;
+        aq20 = lqa != lqb ? -pr0 : 1.0 - pr0;
+      }
+      const double rspqi = rsqi / rsp;
+      const double ssvir = af * 12.0 * d.ecc12[lij] * r12 - 6.0 * d.ecc6[lij] * r6;
+      qforce = d.lj_fudge * ssvir * rsqi;
+      const double aux = d.lj_fudge * (d.ecc12[lij] * r12 - d.ecc6[lij] * r6);
+      ev[2 + c] = af * aux;
+      ev[6 + c] = af0 * aux;
+      const double alphar = d.alpha * rsp;
+      double expcst;
+      const double erfcst = erfc_as_dev(alphar, expcst);
+      const double furpar = aq * d.fudge * qi * qj;
+      const double furpar0 = aq0 * d.fudge * qi * qj;
+      const double erff = 1.0 - erfcst;
+      const double furpar2 = aq2 * d.fudge * qi * qj;
+      const double furpar20 = aq20 * d.fudge * qi * qj;
+      ev[c] = furpar * erfcst / rsp + furpar2 * erff / rsp;
+      ev[4 + c] = furpar0 * erfcst / rsp + furpar20 * erff / rsp;
+      const double a1 = (erfcst + d.twrtpi * alphar * expcst) * rspqi;
+      const double a2 = (erff - d.twrtpi * alphar * expcst) * rspqi;
+      qforce = qforce + furpar * a1 + furpar2 * a2;
+    }
   } else {
-    // fnbgrp.f:120-202 (reduced coordinates, no PBC)
-    xab = d.xp[ia] - d.xp[ib];
-    yab = d.yp[ia] - d.yp[ib];
-    zab = d.zp[ia] - d.zp[ib];
-    rsq = xab * xab + yab * yab + zab * zab;
-    rsp = sqrt(rsq);
+    // fnbgrp.f:120-202 (no PBC)
     int ti = d.type[ia], tj = d.type[ib];
     int lj = ti > tj ? ti : tj, li = ti < tj ? ti : tj;
     int lij = lj * (lj - 1) / 2 + li - 1;
@@ -699,35 +811,26 @@ __global__ void __launch_bounds__(256) k_corr_pairs(const CorrDesc *__restrict__
     double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
     double ssvir = 12.0 * d.ecc12[lij] * r12 - 6.0 * d.ecc6[lij] * r6;
     qforce = ssvir * rsqi;
-    elj = d.ecc12[lij] * r12 - d.ecc6[lij] * r6;
+    ev[2 + c] = d.ecc12[lij] * r12 - d.ecc6[lij] * r6;
     double furpar = d.q[ia] * d.q[ib];
     if (d.ewald) {
       double alphar = d.alpha * rsp;
-      double qt = 1.0 / (1.0 + AS_QP * alphar);
-      double expcst = exp(-(alphar * alphar));
-      double erfcst = ((((AS_A5 * qt + AS_A4) * qt + AS_A3) * qt + AS_A2) * qt + AS_A1) * qt * expcst;
-      ecoul = furpar * erfcst / rsp;
+      double expcst;
+      const double erfcst = erfc_as_dev(alphar, expcst);
+      ev[c] = furpar * erfcst / rsp;
       qforce += furpar * (erfcst + d.twrtpi * alphar * expcst) * (rsqi / rsp);
     } else {
-      ecoul = furpar / rsp;
+      const double ecoul = furpar / rsp;
+      ev[c] = ecoul;
       qforce += ecoul * rsqi;
     }
   }
   d.pf[3 * (size_t)p] = qforce * xab;
   d.pf[3 * (size_t)p + 1] = qforce * yab;
   d.pf[3 * (size_t)p + 2] = qforce * zab;
-  // energies by the class of the first atom (ferrf.f:169, fnbgrp.f:130): coul slt/slv, lj slt/slv
-  const int c = d.ss[ia] - 1;
-  double *pe = d.pe + 4 * (size_t)p;
-  pe[0] = c == 0 ? ecoul : 0.0;
-  pe[1] = c == 1 ? ecoul : 0.0;
-  if (d.kind == 0 && d.q0) {  // en0 of every pair in one column
-    pe[2] = elj;
-    pe[3] = 0.0;
-  } else {
-    pe[2] = c == 0 ? elj : 0.0;
-    pe[3] = c == 1 ? elj : 0.0;
-  }
+  const int ncol = d.ncol ? d.ncol : 4;
+  double *pe = d.pe + ncol * (size_t)p;
+  for (int k = 0; k < ncol; ++k) pe[k] = ev[k];
 }
 
 __global__ void __launch_bounds__(256) k_corr_gather(const CorrDesc *__restrict__ D) {
@@ -754,13 +857,16 @@ __global__ void __launch_bounds__(256) k_corr_gather(const CorrDesc *__restrict_
   }
 }
 
-// d.out[4]: coul(slt), coul(slv), lj(slt), lj(slv); block k reduces column k in fixed order
+// d.out[ncol]: coul(slt), coul(slv), lj(slt), lj(slv) (and the alchemical
+// columns of kinds 3 and 4); block k reduces column k in fixed order
 __global__ void __launch_bounds__(256) k_corr_energy(const CorrDesc *__restrict__ D) {
   __shared__ double red[4];
   const CorrDesc &d = D[blockIdx.y];
   const int k = blockIdx.x;
+  const int ncol = d.ncol ? d.ncol : 4;
+  if (k >= ncol) return;
   double s = 0.0;
-  for (int p = threadIdx.x; p < d.npairs; p += blockDim.x) s += d.pe[4 * (size_t)p + k];
+  for (int p = threadIdx.x; p < d.npairs; p += blockDim.x) s += d.pe[ncol * (size_t)p + k];
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -772,16 +878,17 @@ __global__ void __launch_bounds__(256) k_corr_energy(const CorrDesc *__restrict_
 }
 
 void launch_corr(const CorrDesc *d_desc, const CorrDesc *h, int nsys, hipStream_t st) {
-  int max_pairs = 0, max_atoms = 0;
+  int max_pairs = 0, max_atoms = 0, max_col = 4;
   for (int s = 0; s < nsys; ++s) {
     max_pairs = std::max(max_pairs, h[s].npairs);
     if (h[s].iz == 1 && h[s].npairs > 0) max_atoms = std::max(max_atoms, h[s].natoms);
+    max_col = std::max(max_col, h[s].ncol);
   }
   if (max_pairs > 0)
     hipLaunchKernelGGL(k_corr_pairs, dim3((max_pairs + 255) / 256, nsys), dim3(256), 0, st, d_desc);
   if (max_atoms > 0)
     hipLaunchKernelGGL(k_corr_gather, dim3((max_atoms + 255) / 256, nsys), dim3(256), 0, st, d_desc);
-  hipLaunchKernelGGL(k_corr_energy, dim3(4, nsys), dim3(256), 0, st, d_desc);
+  hipLaunchKernelGGL(k_corr_energy, dim3(max_col, nsys), dim3(256), 0, st, d_desc);
 }
 
 // ---------------------------------------------------------------- cself

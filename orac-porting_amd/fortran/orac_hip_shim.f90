@@ -8,9 +8,17 @@
 ! status calls the caller's xerror(msg, 80, 1, 2), ORAC's fatal path.
 !
 ! Layouts are the reference's serial (non-OpenMP) build: nnlpp, nnlpp1 and
-! mapnl are single lists.  List capacity defaults to mpp = _NAT_PP_*_TGROUP_
-! of the shipped config.H (524*50000), the bound the reference checks;
-! orac_hip_set_list_capacity changes it.
+! mapnl are single lists.  Compiled with ORAC's own cpp flags, an -D_OMP_
+! build whose thread count nthr exceeds 1 (thread-split nnlpp(mpp8,nthr) and
+! mapnl, omp_integr.f90:236-281) is refused through xerror by force and
+! lc_list.  The routines ORAC calls inside its own OpenMP regions with a
+! thread index itask (the 1-4 pairs, intra-group pairs and bonded terms,
+! mts_intra_n0.f:65-106, mts_intra_n1.f:58-130) do every term on itask = 1
+! and return zero energies on the other threads, whose contributions the
+! caller's REDUCTION clauses add; in the serial build itask is always 1.
+! List capacity defaults to mpp = _NAT_PP_*_TGROUP_ of the shipped config.H
+! (524*50000), the bound the reference checks; orac_hip_set_list_capacity
+! changes it.
 module orac_hip_iface
   use, intrinsic :: iso_c_binding
   implicit none
@@ -226,9 +234,59 @@ module orac_hip_iface
       integer(c_int32_t), value :: ewald, ingrpp, nato
       real(c_double), value :: alphal
     end function
+    integer(c_int) function c_ferrf_alchemy_add(ctx, ss_index, alphal, charge, lambdaq, lambdaq0, x0, y0, z0, list, &
+         nlist, fs_slt, fs_slv, fpx, fpy, fpz, erf_corr, erf_arr_corr, delew, rlew, dwrk, nato, intramol) &
+         bind(C, name="orac_hip_ferrf_alchemy_add")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), list(*)
+      real(c_double), value :: alphal, delew, rlew
+      real(c_double) :: charge(*), lambdaq(*), lambdaq0(*), x0(*), y0(*), z0(*), fs_slt, fs_slv
+      real(c_double) :: fpx(*), fpy(*), fpz(*), erf_arr_corr(*), dwrk
+      integer(c_int32_t), value :: nlist, erf_corr, nato, intramol
+    end function
+    integer(c_int) function c_fnb14_alchemy(ctx, ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, cut, ewald, &
+         alphal, int14, int14p, type14, fudge, lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, &
+         lambda, lambdaq, lambda0, lambdaq0, dwrk) bind(C, name="orac_hip_fnb14_alchemy")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), int14(*), type14(*)
+      integer(c_int32_t), value :: nato, cut, ewald, int14p
+      real(c_double) :: xp0(*), yp0(*), zp0(*), charge(*), ecc12(*), ecc6(*), fpx(*), fpy(*), fpz(*)
+      real(c_double) :: lambda(*), lambdaq(*), lambda0(*), lambdaq0(*)
+      real(c_double), value :: alphal, fudge, lj_fudge
+      real(c_double) :: uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, dwrk
+    end function
+    integer(c_int) function c_fnbgrp_alchemy(ctx, ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alphal, &
+         ingrp, ingrpp, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, lambdaq, nato) &
+         bind(C, name="orac_hip_fnbgrp_alchemy")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), type(*), ingrp(*)
+      real(c_double) :: xp0(*), yp0(*), zp0(*), charge(*), ecc12(*), ecc6(*), fpx(*), fpy(*), fpz(*), lambdaq(*)
+      real(c_double) :: uconf_slt, uconf_slv, ucoul_slt, ucoul_slv
+      integer(c_int32_t), value :: ewald, ingrpp, nato
+      real(c_double), value :: alphal
+    end function
   end interface
 
 contains
+
+  ! force and lc_list take ORAC's serial list layouts: an OpenMP build running
+  ! more than one thread hands them thread-split lists, which is fatal here
+  subroutine serial_layouts(name)
+#ifdef _OMP_
+    use omp_integr, only: nthr
+#endif
+    character(len=*), intent(in) :: name
+#ifdef _OMP_
+    character(len=80) :: msg
+    if (nthr > 1) then
+      msg = name // ': thread-split lists (OpenMP, nthr > 1) are not supported'
+      call xerror(msg, 80, 1, 2)
+    end if
+#endif
+  end subroutine
 
   subroutine orac_hip_set_list_capacity(n)
     integer, intent(in) :: n
@@ -304,6 +362,7 @@ subroutine lc_list(ncx, ncy, ncz, nind, indxi, indxj, indxk, rcut, co, xp, yp, z
   integer :: ncx, ncy, ncz, nind(2), indxi(2,*), indxj(2,*), indxk(2,*), natp, nnlpp(*), kprint
   real(8) :: rcut, co(3,3), xp(*), yp(*), zp(*)
   integer(c_int64_t) :: nent
+  call serial_layouts('lc_list')
   call check(c_lc_list(ctx(), ncx, ncy, ncz, nind, indxi, indxj, indxk, rcut, co, xp, yp, zp, natp, nnlpp, &
        orac_list_capacity, nent))
   write(kprint, '(/5x,''Neighbor Lists Dimensions *neighbor('',i8,'')* '')') nent
@@ -324,6 +383,7 @@ subroutine force(ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, m
   real(8) :: alphal, uconf_slt, uconf_slv, uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss, rneigh, rinn, rout
   real(8) :: rtolinn, rtolout, fpx(*), fpy(*), fpz(*)
   logical :: ewald, lskip_ewald
+  call serial_layouts('force')
   call check(c_force(ctx(), ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, &
        ypcm, zpcm, groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, uconf_ss, &
        ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, orac_list_capacity, rneigh, rinn, rout, &
@@ -352,7 +412,7 @@ subroutine cself(ss_index, nato, alphal, kcut, charge, self_slt, self_slv)
   call check(c_cself(ctx(), ss_index, nato, alphal, kcut, charge, self_slt, self_slv))
 end subroutine
 
-! fnbgrp.f:1-3 (serial build: itask is unused)
+! fnbgrp.f:1-3
 subroutine fnbgrp(ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alphal, ingrp, ingrpp, fpx, fpy, fpz, &
      uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, nato, itask)
   use orac_hip_iface
@@ -361,6 +421,10 @@ subroutine fnbgrp(ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alp
   real(8) :: xp0(*), yp0(*), zp0(*), fpx(nato), fpy(nato), fpz(nato), ecc6(*), ecc12(*), alphal, charge(*)
   real(8) :: uconf_slt, ucoul_slt, uconf_slv, ucoul_slv
   logical :: ewald
+  if (itask /= 1) then  ! another OpenMP thread: thread 1 does every term
+    uconf_slt = 0.0d0; uconf_slv = 0.0d0; ucoul_slt = 0.0d0; ucoul_slv = 0.0d0
+    return
+  end if
   call check(c_fnbgrp(ctx(), ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, lg(ewald), alphal, ingrp, ingrpp, &
        fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, nato))
 end subroutine
@@ -372,6 +436,10 @@ subroutine fitors(ss_index, litr, litor, xp0, yp0, zp0, pota, potb, potc, utors_
   implicit none
   integer :: litor, mb, itask, litr(4,*), ss_index(*)
   real(8) :: utors_slt, utors_slv, pota(*), potb(*), potc(*), xp0(*), yp0(*), zp0(*), fpx(mb), fpy(mb), fpz(mb)
+  if (itask /= 1) then  ! another OpenMP thread: thread 1 does every term
+    utors_slt = 0.0d0; utors_slv = 0.0d0
+    return
+  end if
   call check(c_fitors(ctx(), ss_index, litr, litor, xp0, yp0, zp0, pota, potb, potc, utors_slt, utors_slv, fpx, fpy, &
        fpz, mb))
 end subroutine
@@ -383,6 +451,10 @@ subroutine fpbond(ss_index, lbnd, lbond, xp0, yp0, zp0, pota, potb, ubond_slt, u
   implicit none
   integer :: lbond, icnstr, mb, itask, lbnd(2,*), ss_index(*)
   real(8) :: ubond_slt, ubond_slv, pota(*), potb(*), xp0(*), yp0(*), zp0(*), fpx(mb), fpy(mb), fpz(mb)
+  if (itask /= 1) then  ! another OpenMP thread: thread 1 does every term
+    ubond_slt = 0.0d0; ubond_slv = 0.0d0
+    return
+  end if
   call check(c_fpbond(ctx(), ss_index, lbnd, lbond, xp0, yp0, zp0, pota, potb, ubond_slt, ubond_slv, fpx, fpy, fpz, &
        mb))
 end subroutine
@@ -395,6 +467,10 @@ subroutine fpbend(ss_index, lbndg, lbend, xp0, yp0, zp0, pota, potb, potc, potd,
   integer :: lbend, mb, itask, lbndg(3,*), ss_index(*)
   real(8) :: ubend_slt, ubend_slv, xp0(*), yp0(*), zp0(*), fpx(mb), fpy(mb), fpz(mb)
   real(8) :: pota(*), potb(*), potc(*), potd(*), pote(*), potf(*), potg(*)
+  if (itask /= 1) then  ! another OpenMP thread: thread 1 does every term
+    ubend_slt = 0.0d0; ubend_slv = 0.0d0
+    return
+  end if
   call check(c_fpbend(ctx(), ss_index, lbndg, lbend, xp0, yp0, zp0, pota, potb, potc, potd, pote, potf, potg, &
        ubend_slt, ubend_slv, fpx, fpy, fpz, mb))
 end subroutine
@@ -405,6 +481,10 @@ subroutine fptors(ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, utors_slt, u
   implicit none
   integer :: ltors, mb, itask, ltor(4,*), ss_index(*)
   real(8) :: utors_slt, utors_slv, pota(*), potb(*), xp0(*), yp0(*), zp0(*), fpx(mb), fpy(mb), fpz(mb)
+  if (itask /= 1) then  ! another OpenMP thread: thread 1 does every term
+    utors_slt = 0.0d0; utors_slv = 0.0d0
+    return
+  end if
   call check(c_fptors(ctx(), ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, utors_slt, utors_slv, fpx, fpy, fpz, &
        mb))
 end subroutine
@@ -418,6 +498,10 @@ subroutine fnb14(ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, cut, ewald,
   real(8) :: xp0(nato), yp0(nato), zp0(nato), fpx(nato), fpy(nato), fpz(nato), ecc6(*), ecc12(*), alphal, &
        charge(nato), fudge, lj_fudge, ucoul_slt, ucoul_slv, uconf_slt, uconf_slv
   logical :: ewald, cut
+  if (itask /= 1) then  ! another OpenMP thread: thread 1 does every term
+    uconf_slt = 0.0d0; uconf_slv = 0.0d0; ucoul_slt = 0.0d0; ucoul_slv = 0.0d0
+    return
+  end if
   call check(c_fnb14(ctx(), ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, lg(cut), lg(ewald), alphal, int14, &
        int14p, type14, fudge, lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv))
 end subroutine
@@ -473,4 +557,55 @@ subroutine erf_corr_cutoff(oc, delew, rkcut, nfft1, nfft2, nfft3)
   integer :: nfft1, nfft2, nfft3
   real(8) :: oc(3,3), delew, rkcut
   call check(c_erf_corr_cutoff(oc, delew, rkcut, nfft1, nfft2, nfft3))
+end subroutine
+
+! ferrf_alchemy_add.f:1-3 (mtsmd.f:3036-3040)
+subroutine ferrf_alchemy_add(ss_index, alphal, charge, lambdaq, lambdaq0, x0, y0, z0, list, nlist, fscnstr_slt, &
+     fscnstr_slv, fpx, fpy, fpz, erf_corr, erf_arr_corr, delew, rlew, dwrk, nato, intramol)
+  use orac_hip_iface
+  implicit none
+  integer :: nlist, list(3,*), ss_index(*), nato
+  real(8) :: charge(*), lambdaq(*), lambdaq0(*), x0(*), y0(*), z0(*), fpx(nato), fpy(nato), fpz(nato)
+  real(8) :: fscnstr_slt, fscnstr_slv, erf_arr_corr(4,*), delew, rlew, dwrk, alphal
+  logical :: erf_corr, intramol
+  call check(c_ferrf_alchemy_add(ctx(), ss_index, alphal, charge, lambdaq, lambdaq0, x0, y0, z0, list, nlist, &
+       fscnstr_slt, fscnstr_slv, fpx, fpy, fpz, lg(erf_corr), erf_arr_corr, delew, rlew, dwrk, nato, lg(intramol)))
+end subroutine
+
+! fnb14_alchemy.f:1-4 (mts_intra_n1.f:79-83)
+subroutine fnb14_alchemy(ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, cut, ewald, alphal, int14, int14p, &
+     type14, fudge, lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, lambda, lambdaq, lambda0, &
+     lambdaq0, dwrk, itask)
+  use orac_hip_iface
+  implicit none
+  integer :: nato, int14p, int14(2,*), type14(*), ss_index(*), itask
+  real(8) :: xp0(nato), yp0(nato), zp0(nato), fpx(nato), fpy(nato), fpz(nato), ecc6(*), ecc12(*), alphal, &
+       charge(nato), fudge, lj_fudge, ucoul_slt, ucoul_slv, uconf_slt, uconf_slv, lambda(*), lambdaq(*), &
+       lambda0(*), lambdaq0(*), dwrk
+  logical :: ewald, cut
+  if (itask /= 1) then  ! another OpenMP thread: thread 1 does every term (dwrk is the caller's reduction)
+    uconf_slt = 0.0d0; uconf_slv = 0.0d0; ucoul_slt = 0.0d0; ucoul_slv = 0.0d0
+    return
+  end if
+  call check(c_fnb14_alchemy(ctx(), ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, lg(cut), lg(ewald), alphal, &
+       int14, int14p, type14, fudge, lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, lambda, &
+       lambdaq, lambda0, lambdaq0, dwrk))
+end subroutine
+
+! fnbgrp_alchemy.f:1-3 (mts_intra_n1.f:97-100); the arrays hold the atoms ingrp names
+subroutine fnbgrp_alchemy(ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alphal, ingrp, ingrpp, fpx, &
+     fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, lambdaq, itask)
+  use orac_hip_iface
+  implicit none
+  integer :: ingrpp, type(*), ingrp(2,*), ss_index(*), itask, nato
+  real(8) :: xp0(*), yp0(*), zp0(*), fpx(*), fpy(*), fpz(*), ecc6(*), ecc12(*), lambdaq(*), alphal, charge(*)
+  real(8) :: uconf_slt, ucoul_slt, uconf_slv, ucoul_slv
+  logical :: ewald
+  if (itask /= 1 .or. ingrpp == 0) then
+    uconf_slt = 0.0d0; uconf_slv = 0.0d0; ucoul_slt = 0.0d0; ucoul_slv = 0.0d0
+    return
+  end if
+  nato = maxval(ingrp(1:2, 1:ingrpp))
+  call check(c_fnbgrp_alchemy(ctx(), ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, lg(ewald), alphal, ingrp, &
+       ingrpp, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, lambdaq, nato))
 end subroutine

@@ -7,7 +7,8 @@
 ! twins, intramol, ljintra) and unit (ffield_type).  It needs those modules,
 ! so it is compiled inside ORAC's build (src/Makefile), next to
 ! orac_hip_shim.f90, whose module orac_hip_iface declares the C entry point.
-! The per-atom lambda arrays are not read (neither does force_alchemy).
+! The per-atom lambda arrays are not read (neither does force_alchemy).  As
+! force does, both refuse an OpenMP build running more than one thread.
 subroutine force_alchemy(ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, &
      ypcm, zpcm, groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, uconf_ss, &
      ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, rneigh, rinn, rout, rtolinn, rtolout, &
@@ -29,6 +30,7 @@ subroutine force_alchemy(ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype,
   real(8) :: lam(8)
   integer(c_int32_t) :: opls
   lam = [ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0]
+  call serial_layouts('force_alchemy')
   opls = 0
   if (ffield_type(1:4) == 'opls') opls = 1
   call check(c_force_alchemy(ctx(), ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, &
@@ -60,6 +62,7 @@ subroutine force_alchemy_shift(ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, n
   real(8) :: lam(8)
   integer(c_int32_t) :: opls
   lam = [ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0]
+  call serial_layouts('force_alchemy')
   opls = 0
   if (ffield_type(1:4) == 'opls') opls = 1
   call check(c_force_alchemy_shift(ctx(), ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, &

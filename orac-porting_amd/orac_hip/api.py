@@ -64,7 +64,8 @@ EXPORTS = [
     "orac_hip_batch_autotune", "orac_hip_md_setup", "orac_hip_md_set_state", "orac_hip_md_get_state",
     "orac_hip_batch_md_run", "orac_hip_batch_md_relax", "orac_hip_batch_md_scale_velocities",
     "orac_hip_system_md_energies", "orac_hip_system_set_alchemy", "orac_hip_system_set_lambda",
-    "orac_hip_system_alchemy_results",
+    "orac_hip_system_alchemy_results", "orac_hip_ferrf_alchemy_add", "orac_hip_fnb14_alchemy",
+    "orac_hip_fnbgrp_alchemy",
 ]
 
 # stage_ms order of the resident evaluation
@@ -389,6 +390,53 @@ class Context:
                                      *[C.byref(v) for v in e], C.c_int32(nato)))
         return tuple(v.value for v in e)
 
+    # -- ferrf_alchemy_add.f:1
+    def ferrf_alchemy_add(self, ss_index, alphal, charge, lambdaq, lambdaq0, x0, y0, z0, list, nlist, fpx, fpy, fpz,
+                          erf_corr=False, erf_arr_corr=None, delew=0.0, rlew=0.0, nato=None, intramol=False):
+        """Returns (fscnstr_slt, fscnstr_slv, dwrk); forces accumulated in place.
+        list is (3, nlist): ia, ib (1-based) and facQ."""
+        a, b, dw = C.c_double(), C.c_double(), C.c_double()
+        fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
+        nato = len(fpx) if nato is None else nato
+        keep = [_i(ss_index), _f(charge), _f(lambdaq), _f(lambdaq0), _f(x0), _f(y0), _f(z0), _i(list),
+                _f(erf_arr_corr) if erf_arr_corr is not None else np.zeros(4)]
+        _check(lib().orac_hip_ferrf_alchemy_add(self._h, _p(keep[0]), C.c_double(alphal), *[_p(k) for k in keep[1:8]],
+                                                C.c_int32(nlist), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz),
+                                                C.c_int32(int(bool(erf_corr))), _p(keep[8]), C.c_double(delew),
+                                                C.c_double(rlew), C.byref(dw), C.c_int32(nato),
+                                                C.c_int32(int(bool(intramol)))))
+        return a.value, b.value, dw.value
+
+    # -- fnb14_alchemy.f:1
+    def fnb14_alchemy(self, ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, cut, ewald, alphal, int14, int14p,
+                      type14, fudge, lj_fudge, fpx, fpy, fpz, lambda_, lambdaq, lambda0, lambdaq0):
+        """Returns (uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, dwrk); forces accumulated in place."""
+        e = [C.c_double() for _ in range(5)]
+        fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
+        keep = [_i(ss_index), _f(xp0), _f(yp0), _f(zp0), _f(charge), _f(ecc12), _f(ecc6), _i(int14), _i(type14),
+                _f(lambda_), _f(lambdaq), _f(lambda0), _f(lambdaq0)]
+        _check(lib().orac_hip_fnb14_alchemy(self._h, _p(keep[0]), _p(keep[1]), _p(keep[2]), _p(keep[3]), _p(keep[4]),
+                                            C.c_int32(nato), _p(keep[5]), _p(keep[6]), C.c_int32(int(bool(cut))),
+                                            C.c_int32(int(bool(ewald))), C.c_double(alphal), _p(keep[7]),
+                                            C.c_int32(int14p), _p(keep[8]), C.c_double(fudge), C.c_double(lj_fudge),
+                                            _p(fpx), _p(fpy), _p(fpz), *[C.byref(v) for v in e[:4]],
+                                            *[_p(k) for k in keep[9:]], C.byref(e[4])))
+        return tuple(v.value for v in e)
+
+    # -- fnbgrp_alchemy.f:1
+    def fnbgrp_alchemy(self, ss_index, xp0, yp0, zp0, charge, type, ecc12, ecc6, ewald, alphal, ingrp, ingrpp, fpx,
+                       fpy, fpz, lambdaq, nato=None):
+        """Returns (uconf_slt, uconf_slv, ucoul_slt, ucoul_slv); forces accumulated in place."""
+        e = [C.c_double() for _ in range(4)]
+        fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
+        nato = len(fpx) if nato is None else nato
+        keep = [_i(ss_index), _f(xp0), _f(yp0), _f(zp0), _f(charge), _i(type), _f(ecc12), _f(ecc6), _i(ingrp),
+                _f(lambdaq)]
+        _check(lib().orac_hip_fnbgrp_alchemy(self._h, *[_p(k) for k in keep[:8]], C.c_int32(int(bool(ewald))),
+                                             C.c_double(alphal), _p(keep[8]), C.c_int32(ingrpp), _p(fpx), _p(fpy),
+                                             _p(fpz), *[C.byref(v) for v in e], _p(keep[9]), C.c_int32(nato)))
+        return tuple(v.value for v in e)
+
     def sync(self):
         _check(lib().orac_hip_sync(self._h))
 
@@ -413,7 +461,8 @@ class _AlchDesc(C.Structure):
     """orac_hip_alchemy_desc (include/orac_hip.h)."""
     _fields_ = [("ss_alchemy", C.c_void_p), ("rmin", C.c_void_p), ("epsm", C.c_void_p), ("qsoft", C.c_double),
                 ("ljsoft", C.c_double), ("intramol", C.c_int32), ("ljintra", C.c_int32), ("opls", C.c_int32),
-                ("shifted", C.c_int32), ("qshift", C.c_double), ("ljshift", C.c_double)]
+                ("shifted", C.c_int32), ("qshift", C.c_double), ("ljshift", C.c_double), ("nqq", C.c_int32),
+                ("listqq", C.c_void_p)]
 
 
 class ResidentSystem:
@@ -465,13 +514,16 @@ class ResidentSystem:
         return out
 
     def set_alchemy(self, al, lam=None):
-        """Make the system alchemical with orac_hip.synth.Alchemy al, then set
-        the eight lambdas lam (synth.lambda_state order) if given."""
-        self._alch = dict(ss=_i(al.ss_alchemy), rmin=_f(al.rmin), epsm=_f(al.epsm))
+        """Make the system alchemical with orac_hip.synth.Alchemy al (its
+        listqq, if set, adds ferrf_alchemy_add's intra-alchemical term), then
+        set the eight lambdas lam (synth.lambda_state order) if given."""
+        qq = np.zeros((3, 0), np.int32) if al.listqq is None else np.asarray(al.listqq, np.int32)
+        self._alch = dict(ss=_i(al.ss_alchemy), rmin=_f(al.rmin), epsm=_f(al.epsm), qq=_i(qq))
         d = _AlchDesc(ss_alchemy=self._alch["ss"].ctypes.data, rmin=self._alch["rmin"].ctypes.data,
                       epsm=self._alch["epsm"].ctypes.data, qsoft=al.qsoft, ljsoft=al.ljsoft,
                       intramol=int(al.intramol), ljintra=int(al.ljintra), opls=int(al.opls),
-                      shifted=int(al.shifted), qshift=al.qshift, ljshift=al.ljshift)
+                      shifted=int(al.shifted), qshift=al.qshift, ljshift=al.ljshift, nqq=qq.shape[1],
+                      listqq=self._alch["qq"].ctypes.data)
         _check(lib().orac_hip_system_set_alchemy(self.ctx._h, self.id, C.byref(d)))
         if lam is not None:
             self.set_lambda(lam)

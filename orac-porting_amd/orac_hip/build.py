@@ -60,16 +60,17 @@ def build(verbose=False):
     if os.path.exists(shim) and os.path.exists(flang):
         mod = os.path.join(LIBDIR, "fmod")
         os.makedirs(mod, exist_ok=True)
-        _run([flang, "-O2", "-fPIC", "-shared", "-module-dir", mod, shim, "-o", FLIB + ".tmp", f"-L{LIBDIR}",
+        _run([flang, "-O2", "-cpp", "-fPIC", "-shared", "-module-dir", mod, shim, "-o", FLIB + ".tmp", f"-L{LIBDIR}",
               "-lorac_hip", "-Wl,-rpath,$ORIGIN"], verbose)
         os.replace(FLIB + ".tmp", FLIB)
-        # the Fortran caller used by the integration test (tests/fortran)
-        drv = os.path.join(os.path.dirname(ROOT), "tests", "fortran", "drive_shim.f90")
-        if os.path.exists(drv):
-            exe = drv[:-4]
-            _run([flang, "-O2", drv, "-o", exe + ".tmp", f"-L{LIBDIR}", "-lorac_hip_fortran", "-lorac_hip",
-                  "-Wl,-rpath,$ORIGIN/../../orac-porting_amd/lib"], verbose)
-            os.replace(exe + ".tmp", exe)
+        # the Fortran callers used by the integration tests (tests/fortran)
+        for name in ("drive_shim.f90", "drive_intra.f90"):
+            drv = os.path.join(os.path.dirname(ROOT), "tests", "fortran", name)
+            if os.path.exists(drv):
+                exe = drv[:-4]
+                _run([flang, "-O2", drv, "-o", exe + ".tmp", f"-L{LIBDIR}", "-lorac_hip_fortran", "-lorac_hip",
+                      "-Wl,-rpath,$ORIGIN/../../orac-porting_amd/lib"], verbose)
+                os.replace(exe + ".tmp", exe)
     return LIB
 
 

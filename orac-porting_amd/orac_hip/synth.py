@@ -427,7 +427,8 @@ class Alchemy:
     of alch_set (defaults false, blkdta.f90:473-474).  shifted selects the
     shifted potential (force_alchemy_shifted.f, SHIFT-POT) with qshift, ljshift
     (defaults 4.0, 0.35, blkdta.f90:479-480) in place of the soft core
-    (mts_forces.f:130).
+    (mts_forces.f:130).  listqq (3, n), when set, is the intra-alchemical
+    pair list of ferrf_alchemy_add (alchemy_qq_list).
     """
     ss_alchemy: np.ndarray
     rmin: np.ndarray
@@ -440,6 +441,7 @@ class Alchemy:
     shifted: bool = False
     qshift: float = 4.0
     ljshift: float = 0.35
+    listqq: np.ndarray = None
 
     def soft_params(self):
         """(q, lj) parameters of the selected form."""
@@ -470,6 +472,44 @@ def alchemy_charges(charge, ss_alchemy, laddq, lrmq):
     lq = np.where(ss_alchemy == 2, laddq, np.where(ss_alchemy == -2, -lrmq, 0.0)).astype(np.float64)
     a = np.abs(lq)
     return np.where(a == 1.0, 0.0, np.where(a == 0.0, charge, (1.0 - a) * charge))
+
+
+def alchemy_lambdas(ss_alchemy, lam):
+    """Per-atom (lambda, lambdaq, lambda0, lambdaq0) of the eight-lambda state
+    lam (ladd, lrm, ladd0, lrm0, laddq, lrmq, laddq0, lrmq0): added atoms carry
+    +ladd, removed atoms -lrm, the others 0 (mtsmd.f:746-757, 2976-2991)."""
+    ss = np.asarray(ss_alchemy)
+    lam = np.asarray(lam, dtype=np.float64)
+
+    def per(a, r):
+        return np.where(ss == 2, a, np.where(ss == -2, -r, 0.0)).astype(np.float64)
+    return per(lam[0], lam[1]), per(lam[4], lam[5]), per(lam[2], lam[3]), per(lam[6], lam[7])
+
+
+def alchemy_qq_list(added, removed, excluded_pairs=()):
+    """The intra-alchemical pair list of initialize_lambda.f:66-140: every pair
+    of the added atoms followed by the removed atoms (1-based), in that order,
+    minus the pairs listed as bonds, constraints, 1-3 or 1-4 (excluded_pairs:
+    arrays (2, n), 1-based); facQ is 1 when both atoms are added or both
+    removed, else 0.  initialize_lambda.f:135 looks the kinds up by atom
+    number in an array filled by position; this builder takes each atom's
+    kind by position, which agrees with the reference whenever the
+    alchemical atoms are atoms 1..n of the topology (the synthetic ligands)."""
+    atoms = [int(a) for a in added] + [int(a) for a in removed]
+    kind = [1] * len(added) + [0] * len(removed)
+    ex = set()
+    for arr in excluded_pairs:
+        arr = np.asarray(arr)
+        for a, b in zip(arr[0], arr[1]):
+            ex.add((int(a), int(b)))
+            ex.add((int(b), int(a)))
+    out = []
+    for i in range(len(atoms) - 1):
+        for j in range(i + 1, len(atoms)):
+            if (atoms[i], atoms[j]) in ex:
+                continue
+            out.append((atoms[i], atoms[j], 1 if kind[i] == kind[j] else 0))
+    return np.array(out, dtype=np.int32).T.reshape(3, -1)
 
 
 def alchemy_for(s, alchemical_atoms, removed=True, **kw):
@@ -524,6 +564,7 @@ def alchemy_windows(n_windows=16, seed=3001, L=40.0, n_ligand=48, radius=5.5, n_
     s = _assemble(L, pos, q, sig, eps, sites, alpha=alpha, nfft=nfft, order=order, rcut=rcut, solute_bonds=bonds,
                   solute_bends=bends, name=f"ligand{n_ligand}+water{n_water}")
     al = alchemy_for(s, np.arange(n_ligand))
+    al.listqq = alchemy_qq_list([], np.arange(1, n_ligand + 1), (s.bonds, s.bends))
     lam = np.linspace(0.0, 1.0, n_windows) if n_windows > 1 else np.zeros(1)
     states = []
     for w in range(n_windows):

@@ -755,3 +755,105 @@ def ref_erf_corr_cutoff(oc, nfft):
     rk, dl = C.c_double(), C.c_double()
     ref().erf_corr_cutoff_(_p(_f(oc)), C.byref(dl), C.byref(rk), _ri(nfft[0]), _ri(nfft[1]), _ri(nfft[2]))
     return rk.value
+
+
+# ---------------------------------------------------------------- alchemical intramolecular terms
+
+def ref_decide_force(rinn, alpha):
+    """The reference's decide_force (mts_forces.f:212-241)."""
+    out = C.c_int32(0)
+    ref().decide_force_(_rd(rinn), _rd(alpha), C.byref(out))
+    return bool(out.value)
+
+
+def ferrf_alchemy_add(s, lambdaq, lambdaq0, listqq, intramol=False, erf=None, charge=None):
+    """ferrf_alchemy_add.f on the Cartesian coordinates of System s: listqq
+    (3, n) = (ia, ib, facQ), 1-based; per-atom lambdaq, lambdaq0; the created
+    charges (s.charge unless given); erf = (table, delew, rlew) selects the
+    ERF_CORR branch."""
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b, dw = C.c_double(), C.c_double(), C.c_double()
+    lst = _i(listqq)
+    tab = _f(erf[0]) if erf is not None else np.zeros(4)
+    q = s.charge if charge is None else charge
+    lib().orc_ferrf_alchemy_add(_p(_i(s.ss_index)), C.c_double(s.alpha), _p(_f(q)), _p(_f(lambdaq)),
+                                _p(_f(lambdaq0)), _p(_f(x)), _p(_f(y)), _p(_f(z)), _p(lst), C.c_int(lst.size // 3),
+                                C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), C.c_int32(0 if erf is None else 1),
+                                _p(tab), C.c_double(erf[1] if erf is not None else 0.0),
+                                C.c_double(erf[2] if erf is not None else 0.0), C.byref(dw), C.c_int(int(intramol)))
+    return dict(slt=a.value, slv=b.value, dwrk=dw.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def ref_ferrf_alchemy_add(s, lambdaq, lambdaq0, listqq, intramol=False, erf=None, charge=None):
+    R = ref()
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b, dw = C.c_double(), C.c_double(), C.c_double()
+    lst = _i(listqq)
+    tab = _f(erf[0]) if erf is not None else np.zeros(8)
+    q = s.charge if charge is None else charge
+    keep = [_i(s.ss_index), _f(q), _f(lambdaq), _f(lambdaq0), _f(x), _f(y), _f(z)]
+    R.ferrf_alchemy_add_(_p(keep[0]), _rd(s.alpha), _p(keep[1]), _p(keep[2]), _p(keep[3]), _p(keep[4]), _p(keep[5]),
+                         _p(keep[6]), _p(lst), _ri(lst.size // 3), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz),
+                         _ri(0 if erf is None else 1), _p(tab), _rd(erf[1] if erf is not None else 0.0),
+                         _rd(erf[2] if erf is not None else 0.0), C.byref(dw), _ri(n), _ri(int(intramol)))
+    return dict(slt=a.value, slv=b.value, dwrk=dw.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def fnb14_alchemy(s, int14, type14, fudge, lj_fudge, lams, ewald=True, cut=True):
+    """fnb14_alchemy.f on the Cartesian coordinates of System s; lams =
+    (lambda, lambdaq, lambda0, lambdaq0) per atom."""
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    e = [C.c_double() for _ in range(4)]
+    dw = C.c_double()
+    lst, t14 = _i(int14), _i(type14)
+    lf = [_f(v) for v in lams]
+    lib().orc_fnb14_alchemy(_p(_i(s.ss_index)), _p(_f(x)), _p(_f(y)), _p(_f(z)), _p(_f(s.charge)), C.c_int(n),
+                            _p(_f(s.ecc12)), _p(_f(s.ecc6)), C.c_int(int(cut)), C.c_int(int(ewald)),
+                            C.c_double(s.alpha), _p(lst), C.c_int(lst.size // 2), _p(t14), C.c_double(fudge),
+                            C.c_double(lj_fudge), _p(fpx), _p(fpy), _p(fpz), *[C.byref(v) for v in e],
+                            *[_p(v) for v in lf], C.byref(dw))
+    return dict(zip(FNB_E, [v.value for v in e]), dwrk=dw.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def ref_fnb14_alchemy(s, int14, type14, fudge, lj_fudge, lams, ewald=True, cut=True):
+    R = ref()
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    e = [C.c_double() for _ in range(4)]
+    dw = C.c_double(0.0)
+    keep = [_i(s.ss_index), _f(x), _f(y), _f(z), _f(s.charge), _f(s.ecc12), _f(s.ecc6), _i(int14), _i(type14)]
+    lf = [_f(v) for v in lams]
+    R.fnb14_alchemy_(_p(keep[0]), _p(keep[1]), _p(keep[2]), _p(keep[3]), _p(keep[4]), _ri(n), _p(keep[5]),
+                     _p(keep[6]), _ri(int(cut)), _ri(int(ewald)), _rd(s.alpha), _p(keep[7]), _ri(keep[7].size // 2),
+                     _p(keep[8]), _rd(fudge), _rd(lj_fudge), _p(fpx), _p(fpy), _p(fpz), *[C.byref(v) for v in e],
+                     *[_p(v) for v in lf], C.byref(dw), _ri(1))
+    return dict(zip(FNB_E, [v.value for v in e]), dwrk=dw.value, f=np.stack([fpx, fpy, fpz]))
+
+
+def fnbgrp_alchemy(s, ingrp, typ, ewald, lambdaq):
+    """fnbgrp_alchemy.f: the arithmetic of fnbgrp.f (the alphal0 it derives
+    from lambdaq at :152-157 is never read), so the oracle is fnbgrp()."""
+    del lambdaq
+    return fnbgrp(s, ingrp, typ, ewald)
+
+
+def ref_fnbgrp_alchemy(s, ingrp, typ, ewald, lambdaq):
+    R = ref()
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    e = [C.c_double() for _ in range(4)]
+    keep = [_i(s.ss_index), _f(x), _f(y), _f(z), _f(s.charge), _i(typ), _f(s.ecc12), _f(s.ecc6), _i(ingrp),
+            _f(lambdaq)]
+    R.fnbgrp_alchemy_(*[_p(k) for k in keep[:8]], _ri(int(ewald)), _rd(s.alpha), _p(keep[8]),
+                      _ri(keep[8].size // 2), _p(fpx), _p(fpy), _p(fpz), *[C.byref(v) for v in e], _p(keep[9]),
+                      _ri(1))
+    return dict(uconf_slt=e[0].value, uconf_slv=e[1].value, ucoul_slt=e[2].value, ucoul_slv=e[3].value,
+                f=np.stack([fpx, fpy, fpz]))

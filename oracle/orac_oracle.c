@@ -2144,3 +2144,189 @@ int orc_fitors(const int32_t *ss_index, const int32_t *litr, int litor, const do
   *utors_slv = utors[1];
   return status;
 }
+
+/* ------------------------------------------------------------------ */
+/* Intra-alchemical reciprocal correction and alchemical 1-4 pairs      */
+/* ------------------------------------------------------------------ */
+
+/* ferrf_alchemy_add.f:1-204, serial build.  list(3, nlist): atoms ia, ib
+ * (1-based) and facQ (1: both added or both removed, 0: one of each).
+ * factl = (1 - |lambdaq_a|)(1 - |lambdaq_b|) - facQ scales the created
+ * charges; with intramol only the facQ = 0 pairs are kept (:105, :175).
+ * erf_corr selects the spline branch (:94-141).  dwrk = slt + slv - en0. */
+int orc_ferrf_alchemy_add(const int32_t *ss_index, double alphal, const double *charge, const double *lambdaq,
+                          const double *lambdaq0, const double *x0, const double *y0, const double *z0,
+                          const int32_t *list, int nlist, double *fs_slt, double *fs_slv, double *fpx, double *fpy,
+                          double *fpz, int32_t erf_corr, const double *erf_arr_corr, double delew, double rlew,
+                          double *dwrk, int intramol) {
+  const double twrtpi = 2.0 / sqrt(4.0 * atan(1.0));
+  double fs[2] = {0.0, 0.0}, en0 = 0.0;
+  for (int i = 0; i < nlist; ++i) {
+    const int ia = list[3 * i], ib = list[3 * i + 1];
+    const int type = ss_index[ia - 1];
+    const double facq = (double)list[3 * i + 2];
+    if (intramol && list[3 * i + 2] != 0) continue;
+    const double factl = (1.0 - fabs(lambdaq[ia - 1])) * (1.0 - fabs(lambdaq[ib - 1])) - facq;
+    const double factl0 = (1.0 - fabs(lambdaq0[ia - 1])) * (1.0 - fabs(lambdaq0[ib - 1])) - facq;
+    const double qq = factl * charge[ia - 1] * charge[ib - 1];
+    const double qq0 = factl0 * charge[ia - 1] * charge[ib - 1];
+    const double xab = x0[ia - 1] - x0[ib - 1], yab = y0[ia - 1] - y0[ib - 1], zab = z0[ia - 1] - z0[ib - 1];
+    const double rsq = xab * xab + yab * yab + zab * zab;
+    const double rsp = sqrt(rsq);
+    double qforce;
+    if (erf_corr) {
+      const int ic = (int)(1 + (rsp - rlew) / delew);
+      const double h = rsp - (rlew + (ic - 1) * delew);
+      const double *c = erf_arr_corr + 4 * (ic - 1);
+      const double corr = c[0] + h * (c[1] + h * (c[2] + h * c[3] / 3) / 2);
+      const double dcorr = (c[1] + h * (c[2] + h * c[3] / 2)) / rsp;
+      const double alphar = alphal * rsp;
+      const double qt = 1.0 / (1.0 + QP * alphar);
+      const double expcst = exp(-alphar * alphar);
+      const double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+      const double erfst = 1.0 - erfcst;
+      qforce = -qq * (dcorr + (erfst - twrtpi * alphar * expcst) / (rsp * rsq));
+      fs[type - 1] = fs[type - 1] - qq * (erfst / rsp - corr);
+      en0 = en0 - qq0 * (erfst / rsp - corr);
+    } else {
+      const double alphar = alphal * rsp;
+      const double qt = 1.0 / (1.0 + QP * alphar);
+      const double expcst = exp(-alphar * alphar);
+      const double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+      const double erfst = 1.0 - erfcst;
+      qforce = -qq * (erfst - twrtpi * alphar * expcst) / (rsp * rsq);
+      fs[type - 1] = fs[type - 1] - qq * erfst / rsp;
+      en0 = en0 - qq0 * erfst / rsp;
+    }
+    fpx[ia - 1] = fpx[ia - 1] + qforce * xab;
+    fpy[ia - 1] = fpy[ia - 1] + qforce * yab;
+    fpz[ia - 1] = fpz[ia - 1] + qforce * zab;
+    fpx[ib - 1] = fpx[ib - 1] - qforce * xab;
+    fpy[ib - 1] = fpy[ib - 1] - qforce * yab;
+    fpz[ib - 1] = fpz[ib - 1] - qforce * zab;
+  }
+  *fs_slt = fs[0];
+  *fs_slv = fs[1];
+  *dwrk = fs[0] + fs[1] - en0;
+  return 0;
+}
+
+/* The scaling factors of one alchemical 1-4 pair (fnb14_alchemy.f:131-155 for
+ * the LJ lambdas, :161-185 and :214-255 for the charge lambdas): normal pairs
+ * and pairs of equal nonzero lambda keep the full potential, a pair with one
+ * alchemical atom is scaled by 1 - |lambda| of that atom, and a pair of an
+ * added and a removed atom is switched off. */
+static void fnb14_alch_fact(double li, double lj, double li0, double lj0, double *f, double *f0) {
+  if (li == 0.0 && lj != 0.0) {
+    *f = 1.0 - fabs(lj);
+    *f0 = 1.0 - fabs(lj0);
+  } else if (li != 0.0 && lj == 0.0) {
+    *f = 1.0 - fabs(li);
+    *f0 = 1.0 - fabs(li0);
+  } else if (li == 0.0 && lj == 0.0) {
+    *f = 1.0;
+    *f0 = 1.0;
+  } else if (li != lj) {
+    *f = 0.0;
+    *f0 = 0.0;
+  } else {
+    *f = 1.0;
+    *f0 = 1.0;
+  }
+}
+
+/* fnb14_alchemy.f:1-361, serial build (dwrk zeroed on entry, :117).  The
+ * reference's own arithmetic is kept where it departs from the plain
+ * fnb14: without Ewald the LJ and Coulomb forces are unscaled and the LJ
+ * lambda0 energy uses alch_fact (:203-210); with Ewald only the repulsive
+ * LJ force term is scaled by alch_fact (:286), and the pairs of two
+ * alchemical atoms add the erf term with alch_factq2 (:239-255, :305-311). */
+void orc_fnb14_alchemy(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                       const double *charge, int nato, const double *ecc12, const double *ecc6, int cut, int ewald,
+                       double alphal, const int32_t *int14, int int14p, const int32_t *type14, double fudge,
+                       double lj_fudge, double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv,
+                       double *ucoul_slt, double *ucoul_slv, const double *lambda, const double *lambdaq,
+                       const double *lambda0, const double *lambdaq0, double *dwrk) {
+  (void)nato;
+  double uconf[2] = {0.0, 0.0}, ucoul[2] = {0.0, 0.0}, uconf0[2] = {0.0, 0.0}, ucoul0[2] = {0.0, 0.0};
+  *dwrk = 0.0;
+  *uconf_slt = *uconf_slv = *ucoul_slt = *ucoul_slv = 0.0;
+  if (int14p == 0) return;
+  const double twrtpi = 2.0 / sqrt(4.0 * atan(1.0));
+  for (int i = 0; i < int14p; ++i) {
+    const int i1 = int14[2 * i], i2 = int14[2 * i + 1];
+    const int a = i1 - 1, b = i2 - 1;
+    double af, af0, aq, aq0;
+    fnb14_alch_fact(lambda[a], lambda[b], lambda0[a], lambda0[b], &af, &af0);
+    fnb14_alch_fact(lambdaq[a], lambdaq[b], lambdaq0[a], lambdaq0[b], &aq, &aq0);
+    const int type = ss_index[a];
+    const int lij = type14[i];
+    const double chrgei = charge[a], chrgej = charge[b];
+    const double xa = xp0[a] - xp0[b], ya = yp0[a] - yp0[b], za = zp0[a] - zp0[b];
+    const double rsq = xa * xa + ya * ya + za * za;
+    const double rsp = sqrt(rsq);
+    const double rsqi = 1.0 / rsq;
+    const double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
+    double qforce;
+    if (!ewald || !cut) {
+      const double ssvir = 12.0 * ecc12[lij - 1] * r12 - 6.0 * ecc6[lij - 1] * r6;
+      qforce = lj_fudge * ssvir * rsqi;
+      const double elj = lj_fudge * (ecc12[lij - 1] * r12 - ecc6[lij - 1] * r6);
+      uconf[type - 1] = uconf[type - 1] + af * elj;
+      uconf0[type - 1] = uconf0[type - 1] + af * elj;
+      const double sv = fudge * chrgei * chrgej / rsp;
+      ucoul[type - 1] = ucoul[type - 1] + aq * sv;
+      ucoul0[type - 1] = ucoul0[type - 1] + aq0 * sv;
+      qforce = qforce + sv * rsqi;
+    } else {
+      double aq2 = 0.0, aq20 = 0.0;
+      const double lqa = lambdaq[a], lqb = lambdaq[b];
+      if (lqa != 0.0 && lqb != 0.0) {
+        const double p = (1.0 - fabs(lqa)) * (1.0 - fabs(lqb));
+        const double p0 = (1.0 - fabs(lambdaq0[a])) * (1.0 - fabs(lambdaq0[b]));
+        if (lqa != lqb) {
+          aq2 = -p;
+          aq20 = -p0;
+        } else {
+          aq2 = 1.0 - p;
+          aq20 = 1.0 - p0;
+        }
+      }
+      const double rspqi = rsqi / rsp;
+      const double ssvir = af * 12.0 * ecc12[lij - 1] * r12 - 6.0 * ecc6[lij - 1] * r6;
+      qforce = lj_fudge * ssvir * rsqi;
+      double aux = lj_fudge * (ecc12[lij - 1] * r12 - ecc6[lij - 1] * r6);
+      uconf[type - 1] = uconf[type - 1] + af * aux;
+      uconf0[type - 1] = uconf0[type - 1] + af0 * aux;
+      const double alphar = alphal * rsp;
+      const double qt = 1.0 / (1.0 + QP * alphar);
+      const double expcst = exp(-alphar * alphar);
+      const double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
+      const double furpar = aq * fudge * chrgei * chrgej;
+      const double furpar0 = aq0 * fudge * chrgei * chrgej;
+      ucoul[type - 1] = ucoul[type - 1] + furpar * erfcst / rsp;
+      ucoul0[type - 1] = ucoul0[type - 1] + furpar0 * erfcst / rsp;
+      const double erff = 1.0 - erfcst;
+      const double furpar2 = aq2 * fudge * chrgei * chrgej;
+      ucoul[type - 1] = ucoul[type - 1] + furpar2 * erff / rsp;
+      const double furpar20 = aq20 * fudge * chrgei * chrgej;
+      ucoul0[type - 1] = ucoul0[type - 1] + furpar20 * erff / rsp;
+      aux = (erfcst + twrtpi * alphar * expcst) * rspqi;
+      const double aux2 = (erff - twrtpi * alphar * expcst) * rspqi;
+      qforce = qforce + furpar * aux + furpar2 * aux2;
+    }
+    fpx[a] = fpx[a] + qforce * xa;
+    fpy[a] = fpy[a] + qforce * ya;
+    fpz[a] = fpz[a] + qforce * za;
+    fpx[b] = fpx[b] - qforce * xa;
+    fpy[b] = fpy[b] - qforce * ya;
+    fpz[b] = fpz[b] - qforce * za;
+  }
+  *uconf_slt = uconf[0];
+  *uconf_slv = uconf[1];
+  *ucoul_slt = ucoul[0];
+  *ucoul_slv = ucoul[1];
+  double dw = 0.0;
+  for (int k = 0; k < 2; ++k) dw = dw + uconf[k] + ucoul[k] - (uconf0[k] + ucoul0[k]);
+  *dwrk = dw;
+}

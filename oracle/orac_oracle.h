@@ -148,3 +148,16 @@ void orc_fpbend(const int32_t *ss_index, const int32_t *lbndg, int lbend, const 
                 const double *zp0, const double *pota, const double *potb, const double *potc, const double *potd,
                 const double *pote, const double *potf, const double *potg, double *ubend_slt, double *ubend_slv,
                 double *fpx, double *fpy, double *fpz);
+/* ferrf_alchemy_add.f:1-204 (list(3, nlist): ia, ib, facQ) */
+int orc_ferrf_alchemy_add(const int32_t *ss_index, double alphal, const double *charge, const double *lambdaq,
+                          const double *lambdaq0, const double *x0, const double *y0, const double *z0,
+                          const int32_t *list, int nlist, double *fs_slt, double *fs_slv, double *fpx, double *fpy,
+                          double *fpz, int32_t erf_corr, const double *erf_arr_corr, double delew, double rlew,
+                          double *dwrk, int intramol);
+/* fnb14_alchemy.f:1-361 (serial) */
+void orc_fnb14_alchemy(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                       const double *charge, int nato, const double *ecc12, const double *ecc6, int cut, int ewald,
+                       double alphal, const int32_t *int14, int int14p, const int32_t *type14, double fudge,
+                       double lj_fudge, double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv,
+                       double *ucoul_slt, double *ucoul_slv, const double *lambda, const double *lambdaq,
+                       const double *lambda0, const double *lambdaq0, double *dwrk);

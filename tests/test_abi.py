@@ -44,6 +44,10 @@ def test_host_only_entry_points_without_gpu():
     assert all(np.array_equal(a, b) for a, b in zip(g, o))
     for r in (0.0, 5.1, 7.8, 10.0):
         assert Context.decide_force(r, 0.37) == O.decide_force(r, 0.37)
+    # at the 1e-3 threshold, within a few ulps, against the reference's own results
+    g = np.load(os.path.join(ROOT, "tests", "golden", "alchemy_intra.npz"))
+    got = [int(Context.decide_force(float(r), float(a))) for r, a in g["decide_pts"]]
+    assert got == g["decide_ref"].tolist()
 
 
 def test_error_path_is_a_status_not_an_exception():
@@ -73,3 +77,48 @@ def test_alchemy_shim_compiles_against_the_reference_modules(tmp_path):
     syms = subprocess.run(["nm", str(tmp_path / "alch.o")], check=True, capture_output=True, text=True).stdout
     assert re.search(r"\bT force_alchemy_\b", syms) and "orac_hip_force_alchemy" in syms
     assert re.search(r"\bT force_alchemy_shift_\b", syms) and "orac_hip_force_alchemy_shift" in syms
+
+
+def test_shim_refuses_thread_split_lists(tmp_path):
+    """Compiled with ORAC's -D_OMP_, the shim reads omp_integr's nthr: with
+    nthr > 1 (thread-split nnlpp and mapnl) force stops through ORAC's own
+    xerror before touching the device; with nthr = 1 the guard passes."""
+    import shutil
+    import subprocess
+    flang = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    mods = os.path.join(ROOT, "oracle", "_ref", "build")
+    lib = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    hip = os.path.join(ROOT, "orac-porting_amd", "lib")
+    if not (os.path.exists(flang) and os.path.exists(os.path.join(mods, "omp_integr.mod")) and os.path.exists(lib)
+            and os.path.exists(os.path.join(hip, "liborac_hip.so"))):
+        pytest.skip("flang, the reference build (make -C oracle ref) or liborac_hip.so not available")
+    src = os.path.join(ROOT, "orac-porting_amd", "fortran", "orac_hip_shim.f90")
+    drv = tmp_path / "drv.f90"
+    drv.write_text("""program drv
+  use omp_integr, only: nthr
+  use unit, only: kprint
+  implicit none
+  integer :: iv(8), n
+  real(8) :: rv(8)
+  character(len=8) :: arg
+  call get_command_argument(1, arg)
+  read(arg, *) n
+  nthr = n
+  kprint = 6
+  iv = 0
+  rv = 0.0d0
+  call force(iv, rv, rv, rv, rv, rv, rv, rv, iv, iv, 1, 0, iv, rv, rv, rv, iv, iv, rv, rv, rv, 0.3d0, iv, 0, iv, &
+       rv(1), rv(1), rv(1), rv(1), rv(1), rv(1), rv, rv, rv, rv, iv, iv, 0.0d0, 0.0d0, 8.0d0, rv(1), 0.5d0, rv, 1, &
+       iv(1), rv, 1.0d20, .false., .true.)
+  print *, 'returned'
+end program
+""")
+    exe = tmp_path / "drv"
+    subprocess.run([flang, "-cpp", "-D_OMP_", "-c", "-fPIC", "-I", mods, "-module-dir", str(tmp_path), src, "-o",
+                    str(tmp_path / "shim.o")], check=True)
+    subprocess.run([flang, "-I", mods, "-I", str(tmp_path), str(drv), str(tmp_path / "shim.o"), "-o", str(exe),
+                    lib, f"-L{hip}", "-lorac_hip", f"-Wl,-rpath,{os.path.dirname(lib)}:{hip}"], check=True)
+    syms = subprocess.run(["nm", str(tmp_path / "shim.o")], check=True, capture_output=True, text=True).stdout
+    assert "omp_integr" in syms
+    r = subprocess.run([str(exe), "2"], capture_output=True, text=True, timeout=60)
+    assert "thread-split lists" in r.stdout + r.stderr and "returned" not in r.stdout

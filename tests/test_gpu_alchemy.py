@@ -85,18 +85,29 @@ def oracle_alchemy_total(s, al, lam, cp, cf, synth):
         c = O.ferrf_alchemy(s, q, q0, cp[:, sel], 1, float(fud))
         f = f + c["f"]
         cdw += c["dwrk"]
+    if al.listqq is not None:
+        # ferrf_alchemy_add.f on the intra-alchemical pairs (mtsmd.f:3034-3044)
+        lams = synth.alchemy_lambdas(al.ss_alchemy, lam)
+        c = O.ferrf_alchemy_add(s, lams[1], lams[3], al.listqq, intramol=al.intramol)
+        f = f + c["f"]
+        cdw += c["dwrk"]
     self_slt, sdw = O.cself_alchemy(s, q, q0)
     return dict(f=f, d=d, p=p, corr_dw=cdw, self_slt=self_slt, self_dw=sdw,
                 dwrk=d["dwrk"] + p["dwrk"] + cdw + sdw)
 
 
-@pytest.mark.parametrize("precision,shifted", [(0, False), (1, False), (0, True), (1, True)])
-def test_resident_alchemy_matches_oracle(gpu_ctx, alch_sys, precision, shifted):
+@pytest.mark.parametrize("precision,shifted,intra", [(0, False, False), (1, False, False), (0, True, False),
+                                                     (1, True, False), (0, False, True)])
+def test_resident_alchemy_matches_oracle(gpu_ctx, alch_sys, precision, shifted, intra):
+    """The resident alchemical step, with ferrf_alchemy_add on the 12 removed
+    atoms' intra-alchemical pairs (bonds and 1-3 pairs excluded)."""
     from orac_hip.api import ResidentSystem
     s, atoms, _, _, synth = alch_sys
     cp = np.concatenate([s.bonds, s.bends], axis=1)
     cf = np.ones(cp.shape[1])
-    al = synth.alchemy_for(s, atoms, shifted=shifted)
+    al = synth.alchemy_for(s, atoms, shifted=shifted, intramol=intra)
+    al.listqq = synth.alchemy_qq_list([], atoms + 1, (s.bonds, s.bends))
+    assert al.listqq.shape[1] > 30
     rs = ResidentSystem(gpu_ctx, s, precision=precision, ncell=2, corr_pairs=cp, corr_fudge=cf)
     rs.set_alchemy(al)
     for lrm, lrm0, lrmq, lrmq0 in [(0.35, 0.4, 0.6, 0.55), (0.0, 0.05, 0.0, 0.05), (1.0, 0.9, 1.0, 0.9)]:
@@ -187,3 +198,54 @@ def test_plain_reciprocal_and_corrections(gpu_ctx, alch_sys):
     a, d = gpu_ctx.cself_alchemy(s.ss_index, n, s.alpha, 1e9, q, q0)
     ra, rd = O.cself_alchemy(s, q, q0)
     assert rel(a, ra) < 1e-13 and abs(d - rd) <= 1e-13 * abs(ra)
+
+
+def test_plain_intra_alchemical_terms(gpu_ctx):
+    """orac_hip_ferrf_alchemy_add (plain and ERF_CORR, intramol on and off),
+    orac_hip_fnb14_alchemy (Ewald and plain) and orac_hip_fnbgrp_alchemy
+    against the oracle on the added/removed solute of
+    tests/golden/make_alchemy_intra.py, for every lambda state there
+    (forces relative RMS <= 1e-12, energies and dwrk <= 1e-12 of their scale)."""
+    import make_alchemy_intra as GI
+    s, ss = GI.system()
+    n = s.natoms
+    lq = GI.qq_list(s)
+    p14, t14 = GI.pairs14(s)
+    x, y, z = s.cartesian()
+    nb = 400
+    tab, _ = gpu_ctx.int_corr_erf_spline(0.5, 12.0, nb, s.alpha, 2.0)   # the ERF_CORR table (host builder)
+    delew, rlew = (12.0 - 0.5) / nb, 0.5
+    for st in GI.STATES:
+        lam, lamq, lam0, lamq0 = GI.lams(ss, st)
+        for intra in (False, True):
+            for ec in (False, True):
+                fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+                a, b, dw = gpu_ctx.ferrf_alchemy_add(s.ss_index, s.alpha, s.charge, lamq, lamq0, x, y, z, lq,
+                                                     lq.shape[1], fpx, fpy, fpz, erf_corr=ec,
+                                                     erf_arr_corr=tab if ec else None, delew=delew, rlew=rlew,
+                                                     intramol=intra)
+                o = O.ferrf_alchemy_add(s, lamq, lamq0, lq, intramol=intra, erf=(tab, delew, rlew) if ec else None)
+                scale = abs(o["slt"]) + abs(o["slv"]) + 1e-30
+                print(st[0], intra, ec, a, o["slt"], dw, o["dwrk"])
+                assert abs(a - o["slt"]) <= 1e-12 * scale and abs(b - o["slv"]) <= 1e-12 * scale
+                assert abs(dw - o["dwrk"]) <= 1e-12 * scale
+                if np.abs(o["f"]).max() > 0:
+                    assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
+        for ew in (True, False):
+            fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+            e = gpu_ctx.fnb14_alchemy(s.ss_index, x, y, z, s.charge, n, s.ecc12, s.ecc6, True, ew, s.alpha, p14,
+                                      p14.shape[1], t14, *GI.FNB_FUDGE, fpx, fpy, fpz, lam, lamq, lam0, lamq0)
+            o = O.fnb14_alchemy(s, p14, t14, *GI.FNB_FUDGE, (lam, lamq, lam0, lamq0), ewald=ew)
+            scale = sum(abs(o[k]) for k in O.FNB_E)
+            for v, k in zip(e[:4], O.FNB_E):
+                assert abs(v - o[k]) <= 1e-12 * scale, k
+            assert abs(e[4] - o["dwrk"]) <= 1e-12 * scale
+            assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12
+            fpx, fpy, fpz = np.zeros(n), np.zeros(n), np.zeros(n)
+            e = gpu_ctx.fnbgrp_alchemy(s.ss_index, x, y, z, s.charge, s.nbtype, s.ecc12, s.ecc6, ew, s.alpha, p14,
+                                       p14.shape[1], fpx, fpy, fpz, lamq)
+            o = O.fnbgrp_alchemy(s, p14, s.nbtype, ew, lamq)
+            scale = sum(abs(o[k]) for k in O.FNB_E)
+            for v, k in zip(e, O.FNB_E):
+                assert abs(v - o[k]) <= 1e-12 * scale, k
+            assert relrms(np.stack([fpx, fpy, fpz]), o["f"]) < 1e-12

@@ -97,3 +97,83 @@ def test_fortran_caller_error_reaches_xerror(tmp_path, water216):
     r = subprocess.run([EXE, inp, out], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "xerror" in r.stderr and "lc_index" in r.stderr
+
+
+INTRA = os.path.join(ROOT, "tests", "fortran", "drive_intra")
+
+
+@pytest.mark.skipif(not os.path.exists(INTRA), reason="Fortran driver not built (flang absent at build time)")
+def test_fortran_intra_and_alchemy_through_shim(tmp_path):
+    """A Fortran caller (tests/fortran/drive_intra.f90) calls fpbond, fpbend,
+    fptors, fitors, fnb14 (itask 1 and 2), fnbgrp, ferrf_alchemy_add,
+    fnb14_alchemy, fnbgrp_alchemy, ferrf_alchemy and cself_alchemy through
+    the shim on the FKBP12 topology (atoms 1-12 removed and 13-24 added
+    alchemically), against the oracle (forces relative RMS <= 1e-12,
+    energies <= 1e-12 of their scale)."""
+    from orac_hip import formats, synth
+    s, _ = formats.fkbp12()
+    g = np.load(formats.FKBP12_PATH)
+    n = s.natoms
+    x, y, z = s.cartesian()
+    i32 = lambda a: np.asarray(a, np.int32)  # noqa: E731
+    lb, bp = i32(g["lbnd"]), g["bond_pots"]
+    lbg, bep = i32(g["lbndg"]), g["bend_pots"]
+    lt, pt = i32(g["ltor"]), g["potto"]
+    li, ip = i32(g["litr"]), g["itor_pots"]
+    i14, t14 = i32(g["int14"]), i32(g["type14"])
+    ingrp = i32(g["fnb_intra"])
+    ss_al = np.ones(n, np.int32)
+    ss_al[:12], ss_al[12:24] = -2, 2
+    lamv = np.array([0.3, 0.6, 0.35, 0.55, 0.4, 0.7, 0.45, 0.65])
+    lams = synth.alchemy_lambdas(ss_al, lamv)
+    qq = synth.alchemy_qq_list(np.arange(13, 25), np.arange(1, 13), (s.bonds, s.bends, i14))
+    qa = synth.alchemy_charges(s.charge, ss_al, lamv[4], lamv[5])
+    qa0 = synth.alchemy_charges(s.charge, ss_al, lamv[6], lamv[7])
+    lbc = i32(np.concatenate([s.bonds, s.bends], axis=1))
+    inp, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    with open(inp, "wb") as f:
+        np.array([n, lb.shape[1], lbg.shape[1], lt.shape[1], li.shape[1], i14.shape[1], qq.shape[1], ingrp.shape[1],
+                  len(s.ecc12), lbc.shape[1]], np.int32).tofile(f)
+        np.array([s.alpha, 0.83333, 0.5], np.float64).tofile(f)
+        i32(s.ss_index).tofile(f)
+        for a in (x, y, z, s.charge):
+            np.asarray(a, np.float64).tofile(f)
+        i32(s.nbtype).tofile(f)
+        for a in (s.ecc12, s.ecc6):
+            np.asarray(a, np.float64).tofile(f)
+        for a, t in ((lb, np.int32), (bp, np.float64), (lbg, np.int32), (bep, np.float64), (lt, np.int32),
+                     (pt, np.float64), (li, np.int32), (ip, np.float64), (i14, np.int32), (t14, np.int32),
+                     (ingrp, np.int32), (qq, np.int32), (lbc, np.int32)):
+            np.asarray(a, t).ravel(order="F").tofile(f)
+        for a in (*lams, qa, qa0):
+            np.asarray(a, np.float64).tofile(f)
+    r = subprocess.run([INTRA, inp, out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    rec = np.fromfile(out, np.float64).reshape(12, 5 + 3 * n)
+    E = [rr[:5] for rr in rec]
+    F = [rr[5:].reshape(3, n) for rr in rec]
+
+    def check(k, o, names, extra=()):
+        vals = [o[nm] for nm in names] + list(extra)
+        scale = sum(abs(v) for v in vals) + 1e-300
+        for got, want in zip(E[k], vals):
+            assert abs(got - want) <= 1e-12 * scale, (k, got, want)
+        if "f" in o and np.abs(o["f"]).max() > 0:
+            assert relrms(F[k], o["f"]) < 1e-12, k
+
+    check(0, O.fpbond(s, lb, bp[0], bp[1]), ["ubond_slt", "ubond_slv"])
+    check(1, O.fpbend(s, lbg, list(bep)), ["ubend_slt", "ubend_slv"])
+    check(2, O.fptors(s, lt, pt[:, 1], pt[:, 0]), ["utors_slt", "utors_slv"])
+    check(3, O.fitors(s, li, *ip), ["utors_slt", "utors_slv"])
+    fnb = ["uconf_slt", "uconf_slv", "ucoul_slt", "ucoul_slv"]
+    check(4, O.fnb14(s, i14, t14, 0.83333, 0.5), fnb)
+    check(5, O.fnbgrp(s, ingrp, s.nbtype, True), fnb)
+    assert np.all(E[6] == 0.0) and np.all(F[6] == 0.0)          # itask 2: another thread's share is empty
+    o = O.ferrf_alchemy_add(s, lams[1], lams[3], qq)
+    check(7, o, ["slt", "slv", "dwrk"])
+    o = O.fnb14_alchemy(s, i14, t14, 0.83333, 0.5, lams)
+    check(8, o, fnb + ["dwrk"])
+    check(9, O.fnbgrp_alchemy(s, ingrp, s.nbtype, True, lams[1]), fnb)
+    check(10, O.ferrf_alchemy(s, qa, qa0, lbc, 1, 0.8), ["slt", "slv", "dwrk"])
+    a, d = O.cself_alchemy(s, qa, qa0)
+    assert abs(E[11][0] - a) <= 1e-13 * abs(a) and abs(E[11][1] - d) <= 1e-13 * abs(a)
