@@ -225,7 +225,8 @@ def test_plain_intra_alchemical_terms(gpu_ctx):
                                                      erf_arr_corr=tab if ec else None, delew=delew, rlew=rlew,
                                                      intramol=intra)
                 o = O.ferrf_alchemy_add(s, lamq, lamq0, lq, intramol=intra, erf=(tab, delew, rlew) if ec else None)
-                scale = abs(o["slt"]) + abs(o["slv"]) + 1e-30
+                # dwrk is its own scale where intramol leaves both energies exactly zero
+                scale = abs(o["slt"]) + abs(o["slv"]) + abs(o["dwrk"]) + 1e-30
                 print(st[0], intra, ec, a, o["slt"], dw, o["dwrk"])
                 assert abs(a - o["slt"]) <= 1e-12 * scale and abs(b - o["slv"]) <= 1e-12 * scale
                 assert abs(dw - o["dwrk"]) <= 1e-12 * scale
