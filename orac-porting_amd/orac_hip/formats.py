@@ -1,7 +1,8 @@
 """Readers of ORAC's on-disk formats: the ASCII topology-parameter dump
 (``tpgprm``, written and read record by record as ``readtp.f:45-228`` lists
-them) and the ASCII restart file (``dumprs.f:118-126`` / ``readrs.f:216-233``),
-and the assembly of a ``synth.System`` from the two.
+them) and the ASCII restart file (``dumprs.f:118-147`` / ``readrs.f:216-233``),
+the writer of that restart file, and the assembly of a ``synth.System`` from
+the two.
 
 Both files are Fortran list-directed text: every READ starts on a new line,
 takes as many values as it needs from that line and the following ones, and
@@ -135,6 +136,28 @@ def read_restart(path):
     step and temperature, atom positions (Cartesian, A, as the integrator
     holds them when it dumps) and velocities; the group and molecule centres
     that follow are recomputed by the path and not returned."""
+    r = read_restart_full(path, positions_only=True)
+    return {k: r[k] for k in ("co", "oc", "nstep", "temp", "x", "v", "ngrp", "nprot")}
+
+
+def _logical(tok):
+    t = tok.strip().upper().lstrip(".")
+    if t.startswith("T"):
+        return True
+    if t.startswith("F"):
+        return False
+    raise ValueError(f"not a Fortran logical: {tok!r}")
+
+
+def read_restart_full(path, positions_only=False):
+    """Every record of an ASCII restart in the order dumprs.f:118-147 writes
+    them (no replica-exchange or tempering header): co and oc row by row;
+    nstep, temp; ntap, ngrp, nprot; positions, velocities, group centres,
+    molecule centres and their velocities; the previous velocities vpx1..;
+    na1, na2 and the running averages; the hoover and cpress flags, with the
+    thermostat (nh; gh, vh, vh1) and barostat (zz, zz1 row by row, lzz, lzz1)
+    records they switch on; igr, with igrn and krdf.  positions_only stops
+    after the positions, velocities and centres."""
     r = _ListDirected(path)
     m = r.reals(18)
     co = m[:9].reshape(3, 3)            # written as ((co(i,j), j=1,3), i=1,3): row by row
@@ -142,8 +165,113 @@ def read_restart(path):
     nstep, temp = r.values(2)
     ntap, ngrp, nprot = (int(x) for x in r.values(3))
     v = r.reals(6 * ntap + 3 * ngrp + 6 * nprot)
-    return dict(co=np.asfortranarray(co), oc=np.asfortranarray(oc), nstep=int(float(nstep)), temp=float(temp),
-                x=v[:3 * ntap].reshape(3, ntap), v=v[3 * ntap:6 * ntap].reshape(3, ntap), ngrp=ngrp, nprot=nprot)
+    o = 6 * ntap
+    out = dict(co=np.asfortranarray(co), oc=np.asfortranarray(oc), nstep=int(float(nstep)), temp=float(temp),
+               x=v[:3 * ntap].reshape(3, ntap), v=v[3 * ntap:6 * ntap].reshape(3, ntap), ngrp=ngrp, nprot=nprot,
+               xpg=v[o:o + 3 * ngrp].reshape(3, ngrp), xcm=v[o + 3 * ngrp:o + 3 * ngrp + 3 * nprot].reshape(3, nprot),
+               vcm=v[o + 3 * ngrp + 3 * nprot:].reshape(3, nprot))
+    if positions_only:
+        return out
+    out["v1"] = r.reals(3 * ntap).reshape(3, ntap)
+    na1, na2 = (int(x) for x in r.values(2))
+    avg = r.reals(na1 + na2)
+    out.update(avg1=avg[:na1], avg2=avg[na1:])
+    hoover, cpress = (_logical(t) for t in r.values(2))
+    out.update(hoover=hoover, cpress=cpress)
+    if hoover:
+        nh = int(r.values(1)[0])
+        h = r.reals(3 * nh)
+        out.update(nh=nh, gh=h[:nh], vh=h[nh:2 * nh], vh1=h[2 * nh:])
+    if cpress:
+        z = r.reals(20)
+        out.update(zz=z[:9].reshape(3, 3), zz1=z[9:18].reshape(3, 3), lzz=float(z[18]), lzz1=float(z[19]))
+    igr = _logical(r.values(1)[0])
+    out["igr"] = igr
+    if igr:
+        igrn = int(r.values(1)[0])
+        out.update(igrn=igrn, krdf=r.ints(igrn))
+    out["complete"] = all(not ln.strip() for ln in r.lines[r.i:])  # nothing left after the last record
+    return out
+
+
+def _record(vals, per_line=4):
+    """One list-directed WRITE: values on lines of per_line (a READ takes as
+    many lines as it needs, and the next record starts on a new line)."""
+    toks = []
+    for v in vals:
+        if isinstance(v, (bool, np.bool_)):
+            toks.append("T" if v else "F")
+        elif isinstance(v, (int, np.integer)):
+            toks.append(str(int(v)))
+        else:
+            toks.append(repr(float(v)))  # shortest string that reads back to the same double
+    if not toks:
+        return "\n"
+    return "\n".join(" " + " ".join(toks[i:i + per_line]) for i in range(0, len(toks), per_line)) + "\n"
+
+
+def write_restart(path, rst):
+    """Write an ASCII restart with the records of dumprs.f:118-147 (serial,
+    no tempering header) from a dict in read_restart_full's layout: co, oc,
+    nstep, temp, x and v (3, ntap), xpg (3, ngrp), xcm and vcm (3, nprot);
+    optional v1 (default v), avg1, avg2, hoover (nh, gh, vh, vh1), cpress (zz,
+    zz1, lzz, lzz1), igr (krdf).  Reals are written as the shortest decimal
+    that reads back to the same double, so a write and a read are exact."""
+    x, v = np.asarray(rst["x"], float), np.asarray(rst["v"], float)
+    xpg, xcm, vcm = (np.asarray(rst[k], float).reshape(3, -1) for k in ("xpg", "xcm", "vcm"))
+    ntap, ngrp, nprot = x.shape[1], xpg.shape[1], xcm.shape[1]
+    co, oc = np.asarray(rst["co"], float), np.asarray(rst["oc"], float)
+    v1 = np.asarray(rst.get("v1", v), float)
+    avg1, avg2 = np.asarray(rst.get("avg1", ()), float), np.asarray(rst.get("avg2", ()), float)
+    hoover, cpress, igr = bool(rst.get("hoover", False)), bool(rst.get("cpress", False)), bool(rst.get("igr", False))
+    parts = [
+        _record(list(co.ravel(order="C")) + list(oc.ravel(order="C"))),
+        _record([int(rst["nstep"]), float(rst["temp"])]),
+        _record([ntap, ngrp, nprot]),
+        _record(list(x[0]) + list(x[1]) + list(x[2]) + list(v[0]) + list(v[1]) + list(v[2]) + list(xpg[0])
+                + list(xpg[1]) + list(xpg[2]) + list(xcm[0]) + list(xcm[1]) + list(xcm[2]) + list(vcm[0])
+                + list(vcm[1]) + list(vcm[2])),
+        _record(list(v1[0]) + list(v1[1]) + list(v1[2])),
+        _record([len(avg1), len(avg2)]),
+        _record(list(avg1) + list(avg2)),
+        _record([hoover, cpress]),
+    ]
+    if hoover:
+        gh, vh, vh1 = (np.asarray(rst[k], float) for k in ("gh", "vh", "vh1"))
+        parts += [_record([len(gh)]), _record(list(gh) + list(vh) + list(vh1))]
+    if cpress:
+        zz, zz1 = np.asarray(rst["zz"], float), np.asarray(rst["zz1"], float)
+        parts.append(_record(list(zz.ravel(order="C")) + list(zz1.ravel(order="C")) + [float(rst["lzz"]),
+                                                                                       float(rst["lzz1"])]))
+    parts.append(_record([igr]))
+    if igr:
+        krdf = np.asarray(rst["krdf"], int)
+        parts += [_record([len(krdf)]), _record(list(krdf))]
+    with open(path, "w") as f:
+        f.write("".join(parts))
+
+
+def restart_state(s, x, v, nstep, temp):
+    """The restart records of an NVE state of system s: Cartesian positions x
+    and velocities v (3, N) in the reference's units, the cell of s, the
+    mass-weighted group centres (appbou.f) and molecule centres of mass with
+    their velocities; the previous velocities are the current ones."""
+    x = np.asarray(x, float)
+    v = np.asarray(v, float)
+    first = s.grppt[0] - 1
+    last = s.grppt[1] - 1
+    xpg = np.zeros((3, s.ngrp))
+    for g in range(s.ngrp):
+        sl = slice(first[g], last[g] + 1)
+        xpg[:, g] = (x[:, sl] * s.pmass[sl]).sum(axis=1)
+    nprot = int(s.atomp.max())
+    m = s.mass
+    mol = s.atomp - 1
+    mt = np.bincount(mol, weights=m, minlength=nprot)
+    xcm = np.stack([np.bincount(mol, weights=m * x[k], minlength=nprot) / mt for k in range(3)])
+    vcm = np.stack([np.bincount(mol, weights=m * v[k], minlength=nprot) / mt for k in range(3)])
+    return dict(co=s.co, oc=s.oc, nstep=int(nstep), temp=float(temp), x=x, v=v, xpg=xpg, xcm=xcm, vcm=vcm, v1=v,
+                avg1=np.zeros(0), avg2=np.zeros(0), hoover=False, cpress=False, igr=False)
 
 
 def lj_tables_amber(rmin_half, eps):
