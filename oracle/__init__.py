@@ -224,6 +224,39 @@ def force(s, nnlpp, *, rinn, rout, rtolinn, rtolout, rneigh, iz=1, lskip_ewald=F
     return out
 
 
+def force_rem(s, nnlpp, rematom, *, rem_kind, rem_f3, rinn, rout, rtolinn, rtolout, rneigh, iz=1,
+              lskip_ewald=False, ewald=True, lskip_force=False, ljintra=False, fa=None, massinfty=1.0e20):
+    """force_REM.f on System s: force() plus the segment scaling of the pairs
+    seg_select(rem_kind, rematom) picks; returns force()'s keys and seg_ene."""
+    fa = fa or ForceArgs(s)
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    st = np.zeros(9)
+    e = [C.c_double() for _ in range(6)]
+    cap = len(nnlpp) + s.ngrp + 16 if iz == 1 else s.ngrp * (s.ngrp // 2 + 2)
+    nn1 = np.zeros(cap, dtype=np.int32)
+    nc1 = C.c_int32()
+    rti = C.c_double(rtolinn)
+    seg = C.c_double()
+    rc = lib().orc_force_rem(
+        _p(_i(s.ss_index)), _p(_f(s.xp0)), _p(_f(s.yp0)), _p(_f(s.zp0)), _p(fa.xpg), _p(fa.ypg), _p(fa.zpg),
+        _p(_f(s.charge)), _p(_i(s.nbtype)), _p(fa.tt), C.c_int(s.ntypes), C.c_int(n), _p(_i(s.atomg)),
+        _p(fa.xpcm), _p(fa.ypcm), _p(fa.zpcm), _p(_i(s.atomp)), _p(_f(s.co)), _p(_f(s.ecc12)), _p(_f(s.ecc6)),
+        C.c_double(s.alpha), _p(fa.mapnl), C.c_int(s.ngrp), _p(fa.grppt), *[C.byref(x) for x in e],
+        _p(fpx), _p(fpy), _p(fpz), _p(st), _p(_i(nnlpp)), _p(nn1), C.c_long(cap), C.c_double(rneigh),
+        C.c_double(rinn), C.c_double(rout), C.byref(rti), C.c_double(rtolout), _p(_f(s.pmass)), C.c_int(iz),
+        C.byref(nc1), _p(_f(s.mass)), C.c_double(massinfty), C.byref(seg), _p(_i(rematom)), C.c_int(rem_kind),
+        C.c_double(rem_f3), C.c_int32(int(lskip_force)), C.c_int32(int(ljintra)), C.c_int32(int(lskip_ewald)),
+        C.c_int32(int(ewald)))
+    if rc != 0:
+        raise RuntimeError("orc_force_rem: nnlpp1 capacity exceeded")
+    names = ["uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"]
+    out = {k: v.value for k, v in zip(names, e)}
+    out.update(f=np.stack([fpx, fpy, fpz]), stress=st.reshape(3, 3, order="F"), nnlpp1=nn1[:nc1.value].copy(),
+               ncount1=nc1.value, rtolinn=rti.value, seg_ene=seg.value)
+    return out
+
+
 def decide_force(rinn, alpha):
     return bool(lib().orc_decide_force(C.c_double(rinn), C.c_double(alpha)))
 
@@ -390,6 +423,44 @@ def ref_force(s, nnlpp, *, rinn, rout, rtolinn, rtolout, rneigh, iz=1, lskip_ewa
     out = {k: v.value for k, v in zip(names, e)}
     out.update(f=np.stack([fpx, fpy, fpz]), stress=st.reshape(3, 3, order="F"), nnlpp1=nn1[:nc1.value].copy(),
                ncount1=nc1.value, rtolinn=rti.value)
+    return out
+
+
+def ref_force_rem(s, nnlpp, rematom, *, rem_kind, rem_f3, rinn, rout, rtolinn, rtolout, rneigh, iz=1,
+                  lskip_ewald=False, ewald=True, lskip_force=False, ljintra=False, massinfty=1.0e20, fa=None):
+    """The reference's force_rem (force_REM.f, serial) with module rem's
+    rem_segkind, rem_factor(3) and lskip_force, and cpropar's ljintra set."""
+    R = ref()
+    R.refglue_set_rem(C.c_int(int(rem_kind)), C.c_double(float(rem_f3)), C.c_int(int(lskip_force)),
+                      C.c_int(int(ljintra)))
+    fa = fa or ForceArgs(s)
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    st = np.zeros(9)
+    e = [C.c_double() for _ in range(6)]
+    cap = len(nnlpp) + s.ngrp + 16 if iz == 1 else s.ngrp * (s.ngrp // 2 + 2)
+    nn1 = np.zeros(cap, dtype=np.int32)
+    nc1 = C.c_int32()
+    rti = C.c_double(rtolinn)
+    seg = C.c_double()
+    keep = [_i(s.ss_index), _f(s.xp0), _f(s.yp0), _f(s.zp0), _f(s.charge).copy(), _i(s.nbtype), _i(s.atomg),
+            _i(s.atomp), _f(s.co), _f(s.ecc12), _f(s.ecc6), _i(nnlpp), _f(s.pmass), _f(s.mass),
+            np.zeros(s.ngrp, dtype=np.int32), _i(rematom)]
+    (ss, x, y, z, q, nbt, atg, atp, co, e12, e6, nl, pm, ms, groupp, rem) = keep
+
+    def call():
+        R.force_rem_(_p(ss), _p(x), _p(y), _p(z), _p(fa.xpg), _p(fa.ypg), _p(fa.zpg), _p(q), _p(nbt), _p(fa.tt),
+                     _ri(s.ntypes), _ri(n), _p(atg), _p(fa.xpcm), _p(fa.ypcm), _p(fa.zpcm), _p(groupp), _p(atp),
+                     _p(co), _p(e12), _p(e6), _rd(s.alpha), _p(fa.mapnl), _ri(s.ngrp), _p(fa.grppt),
+                     *[C.byref(v) for v in e], _p(fpx), _p(fpy), _p(fpz), _p(st), _p(nl), _p(nn1), _rd(rneigh),
+                     _rd(rinn), _rd(rout), C.byref(rti), _rd(rtolout), _p(pm), _ri(iz), C.byref(nc1), _p(ms),
+                     _rd(massinfty), C.byref(seg), _p(rem), _ri(int(lskip_ewald)), _ri(int(ewald)))
+
+    _big_stack(call)
+    names = ["uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"]
+    out = {k: v.value for k, v in zip(names, e)}
+    out.update(f=np.stack([fpx, fpy, fpz]), stress=st.reshape(3, 3, order="F"), nnlpp1=nn1[:nc1.value].copy(),
+               ncount1=nc1.value, rtolinn=rti.value, seg_ene=seg.value)
     return out
 
 
@@ -689,6 +760,105 @@ def ref_fitors(s, litr, pota, potb, potc, coords=None):
     R.fitors_(_p(keep[0]), _p(keep[1]), _ri(keep[1].size // 4), _p(keep[2]), _p(keep[3]), _p(keep[4]), _p(keep[5]),
               _p(keep[6]), _p(keep[7]), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _ri(n), _ri(1))
     return dict(utors_slt=a.value, utors_slv=b.value, f=np.stack([fpx, fpy, fpz]))
+
+# ---------------------------------------------------------------- REM twins of the bonded terms
+
+def _flags(v, n):
+    f = np.zeros(n, np.int32)
+    f[: len(v)] = np.asarray(v, dtype=bool)
+    return f
+
+
+def fnb14_rem(s, int14, type14, fudge, lj_fudge, remint14, f2, ewald=True, cut=True):
+    """fnb14_rem.f: fnb14 with per-pair flags remint14 (energy to seg_ene, force x f2)."""
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    e = [C.c_double() for _ in range(4)]
+    seg = C.c_double()
+    lst, t14 = _i(int14), _i(type14)
+    fl = _flags(remint14, lst.size // 2)
+    lib().orc_fnb14_rem(_p(_i(s.ss_index)), _p(_f(x)), _p(_f(y)), _p(_f(z)), _p(_f(s.charge)), C.c_int(n),
+                        _p(_f(s.ecc12)), _p(_f(s.ecc6)), C.c_int(int(cut)), C.c_int(int(ewald)), C.c_double(s.alpha),
+                        _p(lst), C.c_int(lst.size // 2), _p(t14), C.c_double(fudge), C.c_double(lj_fudge), _p(fpx),
+                        _p(fpy), _p(fpz), *[C.byref(v) for v in e], _p(fl), C.c_double(f2), C.byref(seg))
+    return dict(zip(FNB_E, [v.value for v in e]), f=np.stack([fpx, fpy, fpz]), seg_ene=seg.value)
+
+
+def ref_fnb14_rem(s, int14, type14, fudge, lj_fudge, remint14, f2, ewald=True, cut=True):
+    R = ref()
+    R.refglue_set_rem_factor(C.c_int(2), C.c_double(float(f2)))
+    x, y, z = s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    e = [C.c_double() for _ in range(4)]
+    seg = C.c_double()
+    keep = [_i(s.ss_index), _f(x), _f(y), _f(z), _f(s.charge), _f(s.ecc12), _f(s.ecc6), _i(int14), _i(type14)]
+    fl = _flags(remint14, keep[7].size // 2)
+    R.fnb14_rem_(_p(keep[0]), _p(keep[1]), _p(keep[2]), _p(keep[3]), _p(keep[4]), _ri(n), _p(keep[5]), _p(keep[6]),
+                 _ri(int(cut)), _ri(int(ewald)), _rd(s.alpha), _p(keep[7]), _ri(keep[7].size // 2), _p(keep[8]),
+                 _rd(fudge), _rd(lj_fudge), _p(fpx), _p(fpy), _p(fpz), *[C.byref(v) for v in e], _p(fl), _ri(1),
+                 C.byref(seg))
+    return dict(zip(FNB_E, [v.value for v in e]), f=np.stack([fpx, fpy, fpz]), seg_ene=seg.value)
+
+
+def fptors_rem(s, ltor, pota, potb, remltor, f2, coords=None):
+    """fptors_rem.f: fptors with per-torsion flags remltor."""
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b, seg = C.c_double(), C.c_double(), C.c_double()
+    lt = _i(ltor)
+    fl = _flags(remltor, lt.size // 4)
+    rc = lib().orc_fptors_rem(_p(_i(s.ss_index)), _p(lt), C.c_int(lt.size // 4), _p(_f(x)), _p(_f(y)), _p(_f(z)),
+                              _p(_f(pota)), _p(_f(potb)), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz),
+                              _p(fl), C.c_double(f2), C.byref(seg))
+    return dict(utors_slt=a.value, utors_slv=b.value, f=np.stack([fpx, fpy, fpz]), status=rc, seg_ene=seg.value)
+
+
+def ref_fptors_rem(s, ltor, pota, potb, remltor, f2, coords=None):
+    R = ref()
+    R.refglue_set_rem_factor(C.c_int(2), C.c_double(float(f2)))
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b, seg = C.c_double(), C.c_double(), C.c_double()
+    keep = [_i(s.ss_index), _i(ltor), _f(x), _f(y), _f(z), _f(pota), _f(potb)]
+    fl = _flags(remltor, keep[1].size // 4)
+    R.fptors_rem_(_p(keep[0]), _p(keep[1]), _ri(keep[1].size // 4), _p(keep[2]), _p(keep[3]), _p(keep[4]),
+                  _p(keep[5]), _p(keep[6]), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _p(fl), _ri(1),
+                  C.byref(seg))
+    return dict(utors_slt=a.value, utors_slv=b.value, f=np.stack([fpx, fpy, fpz]), seg_ene=seg.value)
+
+
+def fitors_rem(s, litr, pota, potb, potc, remitor, f2, coords=None):
+    """fitors_rem.f: fitors with per-term flags remitor."""
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b, seg = C.c_double(), C.c_double(), C.c_double()
+    lt = _i(litr)
+    fl = _flags(remitor, lt.size // 4)
+    rc = lib().orc_fitors_rem(_p(_i(s.ss_index)), _p(lt), C.c_int(lt.size // 4), _p(_f(x)), _p(_f(y)), _p(_f(z)),
+                              _p(_f(pota)), _p(_f(potb)), _p(_f(potc)), C.byref(a), C.byref(b), _p(fpx), _p(fpy),
+                              _p(fpz), _p(fl), C.c_double(f2), C.byref(seg))
+    return dict(utors_slt=a.value, utors_slv=b.value, f=np.stack([fpx, fpy, fpz]), status=rc, seg_ene=seg.value)
+
+
+def ref_fitors_rem(s, litr, pota, potb, potc, remitor, f2, coords=None):
+    R = ref()
+    R.refglue_set_rem_factor(C.c_int(2), C.c_double(float(f2)))
+    x, y, z = coords if coords is not None else s.cartesian()
+    n = s.natoms
+    fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+    a, b, seg = C.c_double(), C.c_double(), C.c_double()
+    keep = [_i(s.ss_index), _i(litr), _f(x), _f(y), _f(z), _f(pota), _f(potb), _f(potc)]
+    fl = _flags(remitor, keep[1].size // 4)
+    R.fitors_rem_(_p(keep[0]), _p(keep[1]), _ri(keep[1].size // 4), _p(keep[2]), _p(keep[3]), _p(keep[4]),
+                  _p(keep[5]), _p(keep[6]), _p(keep[7]), C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _p(fl),
+                  _ri(n), _ri(1), C.byref(seg))
+    return dict(utors_slt=a.value, utors_slv=b.value, f=np.stack([fpx, fpy, fpz]), seg_ene=seg.value)
+
 
 def fpbond(s, lbnd, pota, potb, coords=None):
     """fpbond.f on the Cartesian coordinates of System s (or coords): bonds

@@ -695,6 +695,9 @@ long orc_lc_list(int ncx, int ncy, int ncz, const int32_t *nind, const int32_t *
 typedef struct {
   double st[9];
   double uconf[3], ucoul[3];
+  /* force_REM.f only: the segment energy, and the last values its locals
+   * coul and rsp hold (both carry over from one pair to the next there) */
+  double seg, coul_last, rsp_last;
 } force_acc;
 
 /* One owner group i (1-based), restating force.f:235-835 for that group.
@@ -736,7 +739,18 @@ typedef struct {
   double massinfty; int lskip_ewald, ewald, iz;
   double r2inn, r2out, rinn0, r2inn0, rout0, r2out0, r2neigh, twrtpi, arsout1, arsout2, arsinn1, arsinn2;
   double rinn, rout;
+  /* force_REM.f: rem = 1 selects the segment pairs (seg_select, rem.f90:631-645) */
+  int rem, rem_kind; const int32_t *rematom; double rem_f3;
 } force_par;
+
+/* rem.f90:631-645: kind 0 either atom in the segment, 1 both, 2 exactly one */
+static int seg_select(int kind, const int32_t *rematom, int a, int b) {
+  const int ra = rematom[a - 1] != 0, rb = rematom[b - 1] != 0;
+  if (kind == 0) return ra || rb;
+  if (kind == 1) return ra && rb;
+  if (kind == 2) return ra != rb;
+  return 0;
+}
 
 /* pair body shared by the four pair loops; returns qforce, updates energies */
 #define PAIR_GEOM_AND_LJ_EWALD(SWITCHED)                                                     \
@@ -776,6 +790,7 @@ static void force_group(const force_par *P, int i, const int32_t *nbr, int m, in
         S->xmap2[mp1 - 1] = xc; S->ymap2[mp1 - 1] = yc; S->zmap2[mp1 - 1] = zc;
         S->index1[mp1 - 1] = j;
         double rsp = sqrt(drj);
+        A->rsp_last = rsp;
         double auxa = (P->arsout1 + 2.0 * rsp) / P->arsout2;
         double auxb = P->rout0 - rsp;
         S->swrs[mp1 - 1] = auxa * (auxb * auxb);
@@ -787,6 +802,7 @@ static void force_group(const force_par *P, int i, const int32_t *nbr, int m, in
         S->xmap2[mp1 - 1] = xc; S->ymap2[mp1 - 1] = yc; S->zmap2[mp1 - 1] = zc;
         S->index1[mp1 - 1] = j;
         double rsp = sqrt(drj);
+        A->rsp_last = rsp;
         double auxa = (P->arsinn1 + 2.0 * rsp) / P->arsinn2;
         double auxb = P->rinn0 - rsp;
         S->swrs[mp1 - 1] = 1.0 - auxa * (auxb * auxb);
@@ -835,12 +851,12 @@ static void force_group(const force_par *P, int i, const int32_t *nbr, int m, in
         double zc = co[8] * zg;
         double rsq = xc * xc + yc * yc + zc * zc;
         double rsqi = 1.0 / rsq;
-        double qforce = 0.0;
+        double qforce = 0.0, conf = 0.0;
         if (P->ecc6[lij - 1] > 1e-12) {
           double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
           double ssvir = 12.0 * P->ecc12[lij - 1] * r12 - 6.0 * P->ecc6[lij - 1] * r6;
           qforce = ssvir * rsqi;
-          double conf = P->ecc12[lij - 1] * r12 - P->ecc6[lij - 1] * r6;
+          conf = P->ecc12[lij - 1] * r12 - P->ecc6[lij - 1] * r6;
           uconfa = uconfa + conf;
         }
         if (!P->lskip_ewald) {
@@ -849,14 +865,24 @@ static void force_group(const force_par *P, int i, const int32_t *nbr, int m, in
           double expcst = exp(-(alphar * alphar));
           double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
           double furpar = chrgei * P->charge[j - 1];
-          ucoula = ucoula + furpar * erfcst / rsp;
+          double coul = furpar * erfcst / rsp;
+          ucoula = ucoula + coul;
           double aux1 = furpar * (erfcst + P->twrtpi * alphar * expcst) * rspqi;
           qforce = qforce + aux1;
+          A->rsp_last = rsp;
+          A->coul_last = coul;
         } else if (!P->ewald) {
           double rsp = sqrt(rsq);
           double ssvir = chrgei * P->charge[j - 1] / rsp;
           A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + ssvir;
           qforce = qforce + ssvir * rsqi;
+          A->rsp_last = rsp;
+          A->coul_last = ssvir;
+        }
+        if (P->rem && seg_select(P->rem_kind, P->rematom, i1, j)) {
+          /* force_REM.f:491-494: coul as left by this pair or an earlier one */
+          A->seg = A->seg + conf + A->coul_last;
+          qforce = qforce * P->rem_f3;
         }
         double emvir = qforce;
         fpx[i1 - 1] = fpx[i1 - 1] + qforce * xc;
@@ -897,13 +923,14 @@ static void force_group(const force_par *P, int i, const int32_t *nbr, int m, in
         double zc = co[8] * zg;
         double rsq = xc * xc + yc * yc + zc * zc;
         double rsqi = 1.0 / rsq;
-        double qforce = 0.0;
+        double qforce = 0.0, conf = 0.0;
+        const int sel = P->rem && seg_select(P->rem_kind, P->rematom, i1, j);
         if (P->ecc6[lij - 1] > 1e-12) {
           double r6 = rsqi * rsqi * rsqi, r12 = r6 * r6;
           double ssvir = 12.0 * P->ecc12[lij - 1] * r12 - 6.0 * P->ecc6[lij - 1] * r6;
           qforce = ssvir * rsqi * sw;
-          double conf = P->ecc12[lij - 1] * r12 - P->ecc6[lij - 1] * r6;
-          S->cmap2[jj - 1] = S->cmap2[jj - 1] + conf;
+          conf = P->ecc12[lij - 1] * r12 - P->ecc6[lij - 1] * r6;
+          S->cmap2[jj - 1] = S->cmap2[jj - 1] + (sel ? conf * P->rem_f3 : conf);
           A->uconf[typeij - 1] = A->uconf[typeij - 1] + sw * conf;
         }
         if (!P->lskip_ewald) {
@@ -912,16 +939,41 @@ static void force_group(const force_par *P, int i, const int32_t *nbr, int m, in
           double expcst = exp(-(alphar * alphar));
           double erfcst = ((((A5 * qt + A4) * qt + A3) * qt + A2) * qt + A1) * qt * expcst;
           double furpar = chrgei * P->charge[j - 1];
-          A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + sw * furpar * erfcst / rsp;
-          S->cmap2[jj - 1] = S->cmap2[jj - 1] + furpar * erfcst / rsp;
+          if (P->rem) {
+            /* force_REM.f:649-664 */
+            double coul = furpar * erfcst / rsp;
+            A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + coul * sw;
+            S->cmap2[jj - 1] = S->cmap2[jj - 1] + (sel ? coul * P->rem_f3 : coul);
+            A->coul_last = coul;
+          } else {
+            A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + sw * furpar * erfcst / rsp;
+            S->cmap2[jj - 1] = S->cmap2[jj - 1] + furpar * erfcst / rsp;
+          }
           double aux1 = furpar * (erfcst + P->twrtpi * alphar * expcst) * rspqi * sw;
           qforce = qforce + aux1;
+          A->rsp_last = rsp;
         } else if (!P->ewald) {
-          double rsp = sqrt(rsq);
-          double ssvir = chrgei * P->charge[j - 1] / rsp;
-          S->cmap2[jj - 1] = S->cmap2[jj - 1] + ssvir;
-          A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + ssvir * sw;
-          qforce = qforce + ssvir * rsqi * sw;
+          if (P->rem) {
+            /* force_REM.f:666-678: the distance is the routine's rsp as an
+             * earlier pair or group left it (not recomputed for this pair) */
+            double coul = chrgei * P->charge[j - 1] / A->rsp_last;
+            A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + coul * sw;
+            qforce = qforce + coul * rsqi * sw;
+            S->cmap2[jj - 1] = S->cmap2[jj - 1] + (sel ? coul * P->rem_f3 : coul);
+            A->coul_last = coul;
+          } else {
+            double rsp = sqrt(rsq);
+            double ssvir = chrgei * P->charge[j - 1] / rsp;
+            S->cmap2[jj - 1] = S->cmap2[jj - 1] + ssvir;
+            A->ucoul[typeij - 1] = A->ucoul[typeij - 1] + ssvir * sw;
+            qforce = qforce + ssvir * rsqi * sw;
+            A->rsp_last = rsp;
+          }
+        }
+        if (sel) {
+          /* force_REM.f:679-682 */
+          A->seg = A->seg + sw * conf + sw * A->coul_last;
+          qforce = qforce * P->rem_f3;
         }
         double emvir = qforce;
         fpx[i1 - 1] = fpx[i1 - 1] + qforce * xc;
@@ -985,6 +1037,19 @@ static void force_group(const force_par *P, int i, const int32_t *nbr, int m, in
   }
 }
 
+static int force_impl(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                      const double *xpg, const double *ypg, const double *zpg, const double *charge,
+                      const int32_t *nbtype, const int32_t *type, int ma, int nato, const int32_t *atomg,
+                      const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *atomp,
+                      const double *co, const double *ecc12, const double *ecc6, double alphal,
+                      const int32_t *mapnl, int ngrp, const int32_t *grppt, double *uconf_slt, double *uconf_slv,
+                      double *uconf_ss, double *ucoul_slt, double *ucoul_slv, double *ucoul_ss, double *fpx,
+                      double *fpy, double *fpz, double *stress, const int32_t *nnlpp, int32_t *nnlpp1,
+                      long nnlpp1_cap, double rneigh, double rinn, double rout, double *rtolinn, double rtolout,
+                      const double *gmass, int iz, int32_t *ncount1, const double *mass, double massinfty,
+                      int32_t lskip_ewald, int32_t ewald, int nthreads, int rem, int rem_kind,
+                      const int32_t *rematom, double rem_f3, double *seg_ene);
+
 int orc_force(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
               const double *xpg, const double *ypg, const double *zpg, const double *charge,
               const int32_t *nbtype, const int32_t *type, int ma, int nato, const int32_t *atomg,
@@ -996,7 +1061,66 @@ int orc_force(const int32_t *ss_index, const double *xp0, const double *yp0, con
               double rinn, double rout, double *rtolinn, double rtolout, const double *gmass, int iz,
               int32_t *ncount1, const double *mass, double massinfty, int32_t lskip_ewald, int32_t ewald,
               int nthreads) {
+  return force_impl(ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, ypcm,
+                    zpcm, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, uconf_ss,
+                    ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, nnlpp1_cap, rneigh, rinn,
+                    rout, rtolinn, rtolout, gmass, iz, ncount1, mass, massinfty, lskip_ewald, ewald, nthreads, 0, 0,
+                    NULL, 1.0, NULL);
+}
+
+/* force_REM.f:1-995 (serial): force.f with the pairs seg_select picks
+ * (rem.f90:631-645) scaled by rem_f3 = rem_factor(3) in force, and their
+ * energy (unscaled, switched) summed into seg_ene; lskip_force without
+ * ljintra returns at once, with ljintra the charges count as zero
+ * (force_REM.f:148-158).  The locals coul and rsp carry over from pair to
+ * pair as in the reference (the segment energy of an lskip_ewald call and
+ * the switched plain-Coulomb branch, force_REM.f:667, read them); they start
+ * at zero here, where the reference's start values are undefined. */
+int orc_force_rem(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                  const double *xpg, const double *ypg, const double *zpg, const double *charge,
+                  const int32_t *nbtype, const int32_t *type, int ma, int nato, const int32_t *atomg,
+                  const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *atomp,
+                  const double *co, const double *ecc12, const double *ecc6, double alphal, const int32_t *mapnl,
+                  int ngrp, const int32_t *grppt, double *uconf_slt, double *uconf_slv, double *uconf_ss,
+                  double *ucoul_slt, double *ucoul_slv, double *ucoul_ss, double *fpx, double *fpy, double *fpz,
+                  double *stress, const int32_t *nnlpp, int32_t *nnlpp1, long nnlpp1_cap, double rneigh,
+                  double rinn, double rout, double *rtolinn, double rtolout, const double *gmass, int iz,
+                  int32_t *ncount1, const double *mass, double massinfty, double *seg_ene,
+                  const int32_t *rematom, int rem_kind, double rem_f3, int32_t lskip_force, int32_t ljintra,
+                  int32_t lskip_ewald, int32_t ewald) {
+  if (lskip_force && !ljintra) return 0;
+  double *q0 = NULL;
+  if (lskip_force) {
+    q0 = calloc((size_t)nato, sizeof(double));
+    charge = q0;
+  }
+  *seg_ene = 0.0;
+  int st = force_impl(ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, ypcm,
+                      zpcm, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, uconf_ss,
+                      ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, nnlpp1_cap, rneigh, rinn,
+                      rout, rtolinn, rtolout, gmass, iz, ncount1, mass, massinfty, lskip_ewald, ewald, 1, 1,
+                      rem_kind, rematom, rem_f3, seg_ene);
+  free(q0);
+  return st;
+}
+
+static int force_impl(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                      const double *xpg, const double *ypg, const double *zpg, const double *charge,
+                      const int32_t *nbtype, const int32_t *type, int ma, int nato, const int32_t *atomg,
+                      const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *atomp,
+                      const double *co, const double *ecc12, const double *ecc6, double alphal,
+                      const int32_t *mapnl, int ngrp, const int32_t *grppt, double *uconf_slt, double *uconf_slv,
+                      double *uconf_ss, double *ucoul_slt, double *ucoul_slv, double *ucoul_ss, double *fpx,
+                      double *fpy, double *fpz, double *stress, const int32_t *nnlpp, int32_t *nnlpp1,
+                      long nnlpp1_cap, double rneigh, double rinn, double rout, double *rtolinn, double rtolout,
+                      const double *gmass, int iz, int32_t *ncount1, const double *mass, double massinfty,
+                      int32_t lskip_ewald, int32_t ewald, int nthreads, int rem, int rem_kind,
+                      const int32_t *rematom, double rem_f3, double *seg_ene) {
   force_par P;
+  P.rem = rem;
+  P.rem_kind = rem_kind;
+  P.rematom = rematom;
+  P.rem_f3 = rem_f3;
   P.ss_index = ss_index; P.xp0 = xp0; P.yp0 = yp0; P.zp0 = zp0; P.xpg = xpg; P.ypg = ypg; P.zpg = zpg;
   P.charge = charge; P.nbtype = nbtype; P.type = type; P.ma = ma; P.atomg = atomg; P.xpcm = xpcm;
   P.ypcm = ypcm; P.zpcm = zpcm; P.atomp = atomp; P.co = co; P.ecc12 = ecc12; P.ecc6 = ecc6;
@@ -1032,7 +1156,7 @@ int orc_force(const int32_t *ss_index, const double *xp0, const double *yp0, con
   }
   if (iz == 0) maxm = nnlpp[0];
   int cap = maxm + 1;
-  int nt = nthreads > 1 ? nthreads : 1;
+  int nt = (nthreads > 1 && !rem) ? nthreads : 1;  /* the segment pass follows the serial order */
   int status = 0;
   if (nt == 1) {
     force_scratch *S = scratch_new(nato, ngrp, cap);
@@ -1058,6 +1182,7 @@ int orc_force(const int32_t *ss_index, const double *xp0, const double *yp0, con
     }
     *ncount1 = (int32_t)c1;
     scratch_free(S);
+    if (seg_ene) *seg_ene = A.seg;
     if (!(iz == 0 || P.rinn0 >= rout)) {
       *ucoul_slt = A.ucoul[0]; *ucoul_ss = A.ucoul[1]; *ucoul_slv = A.ucoul[2];
       *uconf_slt = A.uconf[0]; *uconf_ss = A.uconf[1]; *uconf_slv = A.uconf[2];
@@ -1707,15 +1832,18 @@ void orc_cself_alchemy(const int32_t *ss_index, int nato, double alphal, double 
 /* fnb14.f:1-219, serial build: the 1-4 pairs with LJ-FUDGE and QQ-FUDGE;
  * erfc when ewald and cut, plain Coulomb otherwise.  Energies by the
  * ss_index of the first atom. */
-void orc_fnb14(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
-               const double *charge, int nato, const double *ecc12, const double *ecc6, int cut, int ewald,
-               double alphal, const int32_t *int14, int int14p, const int32_t *type14, double fudge, double lj_fudge,
-               double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt,
-               double *ucoul_slv) {
-  (void)nato;
+/* remint (fnb14_rem.f, null for fnb14.f): the flagged pairs' energy goes to
+ * *seg and their force is scaled by f2 = rem_factor(2) (fnb14_rem.f:146-149,
+ * :209-212) */
+static void fnb14_impl(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                       const double *charge, const double *ecc12, const double *ecc6, int cut, int ewald,
+                       double alphal, const int32_t *int14, int int14p, const int32_t *type14, double fudge,
+                       double lj_fudge, double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv,
+                       double *ucoul_slt, double *ucoul_slv, const int32_t *remint, double f2, double *seg) {
   double uconf[2] = {0.0, 0.0}, ucoul[2] = {0.0, 0.0};
   *uconf_slt = *uconf_slv = *ucoul_slt = *ucoul_slv = 0.0;
-  if (int14p == 0) return;
+  if (int14p == 0) return; /* (fnb14_rem.f:100: before seg_ene is zeroed, which is then left as it was) */
+  if (seg) *seg = 0.0;
   const double twrtpi = 2.0 / sqrt(4.0 * atan(1.0));
   const int erfc_branch = ewald && cut;
   for (int i = 0; i < int14p; ++i) {
@@ -1738,6 +1866,10 @@ void orc_fnb14(const int32_t *ss_index, const double *xp0, const double *yp0, co
       const double sv = fudge * chrgei * chrgej / rsp;
       ucoul[type - 1] = ucoul[type - 1] + sv;
       qforce = qforce + sv * rsqi;
+      if (remint && remint[i]) {
+        *seg = *seg + elj + sv;
+        qforce = qforce * f2;
+      }
     } else {
       /* fnb14.f:107-152 */
       const double rspqi = rsqi / rsp;
@@ -1751,6 +1883,10 @@ void orc_fnb14(const int32_t *ss_index, const double *xp0, const double *yp0, co
       const double ucou = furpar * erfcst / rsp;
       ucoul[type - 1] = ucoul[type - 1] + ucou;
       qforce = lj_fudge * ssvir * rsqi + furpar * (erfcst + twrtpi * alphar * expcst) * rspqi;
+      if (remint && remint[i]) {
+        *seg = *seg + ucon + ucou;
+        qforce = qforce * f2;
+      }
     }
     fpx[i1 - 1] = fpx[i1 - 1] + qforce * xa;
     fpy[i1 - 1] = fpy[i1 - 1] + qforce * ya;
@@ -1763,6 +1899,27 @@ void orc_fnb14(const int32_t *ss_index, const double *xp0, const double *yp0, co
   *uconf_slv = uconf[1];
   *ucoul_slt = ucoul[0];
   *ucoul_slv = ucoul[1];
+}
+
+void orc_fnb14(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+               const double *charge, int nato, const double *ecc12, const double *ecc6, int cut, int ewald,
+               double alphal, const int32_t *int14, int int14p, const int32_t *type14, double fudge, double lj_fudge,
+               double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt,
+               double *ucoul_slv) {
+  (void)nato;
+  fnb14_impl(ss_index, xp0, yp0, zp0, charge, ecc12, ecc6, cut, ewald, alphal, int14, int14p, type14, fudge,
+             lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, NULL, 1.0, NULL);
+}
+
+/* fnb14_rem.f:1-234 (serial): fnb14 with remint14 */
+void orc_fnb14_rem(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                   const double *charge, int nato, const double *ecc12, const double *ecc6, int cut, int ewald,
+                   double alphal, const int32_t *int14, int int14p, const int32_t *type14, double fudge,
+                   double lj_fudge, double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv,
+                   double *ucoul_slt, double *ucoul_slv, const int32_t *remint14, double f2, double *seg_ene) {
+  (void)nato;
+  fnb14_impl(ss_index, xp0, yp0, zp0, charge, ecc12, ecc6, cut, ewald, alphal, int14, int14p, type14, fudge,
+             lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, remint14, f2, seg_ene);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1780,12 +1937,18 @@ static double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); 
 /* Returns 0, or 1 at a torsion whose bond angles are singular (the reference
  * then perturbs the four atoms at random, fptors.f:107-130, which is not
  * restated), or 2 at an unsupported multiplicity (the reference stops). */
-int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const double *xp0, const double *yp0,
-               const double *zp0, const double *pota, const double *potb, double *utors_slt, double *utors_slv,
-               double *fpx, double *fpy, double *fpz) {
+/* rem (fptors_rem.f, 0 for fptors.f): the 60-degree-phase forms (12, 14)
+ * without the sign factor and with dsoa = 0 where |sin| < 1e-8
+ * (fptors_rem.f:262-301); the flagged torsions' energy goes to *seg and their
+ * force is scaled by f2 (:310-313) */
+static int fptors_impl(const int32_t *ss_index, const int32_t *ltor, int ltors, const double *xp0,
+                       const double *yp0, const double *zp0, const double *pota, const double *potb,
+                       double *utors_slt, double *utors_slv, double *fpx, double *fpy, double *fpz, int rem,
+                       const int32_t *remltor, double f2, double *seg) {
   const double sin60 = (double)0.866025f, lim = (double)1.000001f;
   double utors[2] = {0.0, 0.0};
   int status = 0;
+  if (seg) *seg = 0.0;
   for (int i = 0; i < ltors; ++i) {
     if (fabs(potb[i]) < 1.0e-12) continue;
     const int l1 = ltor[4 * i], l2 = ltor[4 * i + 1], l3 = ltor[4 * i + 2], l4 = ltor[4 * i + 3];
@@ -1867,7 +2030,7 @@ int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const do
         const double xn = y21 * z32 - y32 * z21, yn = z21 * x32 - z32 * x21, zn = x21 * y32 - x32 * y21;
         const double aux4 = fsign(bb, xn * x43 + yn * y43 + zn * z43);
         const double soa = fsign(sqrt(1.0 - pow(coa, 2.0)), aux4);
-        const double dsoa = -coa / soa;
+        const double dsoa = (rem && fabs(soa) < 1.0e-8) ? 0.0 : -coa / soa;
         double auxn, dauxn;
         if (ntph == 14) {
           enrg = b * (1.0 + 0.5 * (8.0 * P(4) - 8.0 * P(2) + 1.0));
@@ -1880,14 +2043,23 @@ int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const do
           dauxn = b * sin60 * 2.0 * (soa + dsoa * coa);
           qforce = 2.0 * coa * b;
         }
-        enrg = sig * (enrg + auxn);
-        qforce = sig * (-(qforce + dauxn));
+        if (rem) {
+          enrg = enrg + auxn;
+          qforce = -(qforce + dauxn);
+        } else {
+          enrg = sig * (enrg + auxn);
+          qforce = sig * (-(qforce + dauxn));
+        }
         break;
       }
       default: return 2;
     }
 #undef P
     utors[type - 1] = utors[type - 1] + enrg;
+    if (remltor && remltor[i]) {
+      *seg = *seg + enrg;
+      qforce = qforce * f2;
+    }
     double u[3][4];
     for (int c = 0; c < 3; ++c) {
       u[c][0] = auxa * d1[c][0] + auxc * d3[c][0];
@@ -1903,6 +2075,21 @@ int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const do
   *utors_slt = utors[0];
   *utors_slv = utors[1];
   return status;
+}
+
+int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const double *xp0, const double *yp0,
+               const double *zp0, const double *pota, const double *potb, double *utors_slt, double *utors_slv,
+               double *fpx, double *fpy, double *fpz) {
+  return fptors_impl(ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, utors_slt, utors_slv, fpx, fpy, fpz, 0, NULL,
+                     1.0, NULL);
+}
+
+/* fptors_rem.f:1-352 (serial) */
+int orc_fptors_rem(const int32_t *ss_index, const int32_t *ltor, int ltors, const double *xp0, const double *yp0,
+                   const double *zp0, const double *pota, const double *potb, double *utors_slt, double *utors_slv,
+                   double *fpx, double *fpy, double *fpz, const int32_t *remltor, double f2, double *seg_ene) {
+  return fptors_impl(ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, utors_slt, utors_slv, fpx, fpy, fpz, 1,
+                     remltor, f2, seg_ene);
 }
 
 /* fpbond.f:1-110 (serial): harmonic bonds lbnd(2, lbond), pota the rest
@@ -2032,12 +2219,16 @@ void orc_fpbend(const int32_t *ss_index, const int32_t *lbndg, int lbend, const 
  * pi of pota; otherwise the cosine series of multiplicity |pota| (1, 2, 3, 4
  * or 6; another returns 2, where the reference stops).  The cosine-derivative
  * vectors are fptors'. */
-int orc_fitors(const int32_t *ss_index, const int32_t *litr, int litor, const double *xp0, const double *yp0,
-               const double *zp0, const double *pota, const double *potb, const double *potc, double *utors_slt,
-               double *utors_slv, double *fpx, double *fpy, double *fpz) {
+/* remitor (fitors_rem.f, null for fitors.f): the flagged terms' energy goes
+ * to *seg and their force is scaled by f2 (fitors_rem.f:206-209, :255-261) */
+static int fitors_impl(const int32_t *ss_index, const int32_t *litr, int litor, const double *xp0,
+                       const double *yp0, const double *zp0, const double *pota, const double *potb,
+                       const double *potc, double *utors_slt, double *utors_slv, double *fpx, double *fpy,
+                       double *fpz, const int32_t *remitor, double f2, double *seg) {
   const double lim = (double)1.000001f, half_pi4 = (double)1.57079f; /* default-real literals */
   double utors[2] = {0.0, 0.0};
   int status = 0;
+  if (seg) *seg = 0.0;
   for (int i = 0; i < litor; ++i) {
     if (fabs(potb[i]) < 1.0e-12) continue;
     const int l[4] = {litr[4 * i] - 1, litr[4 * i + 1] - 1, litr[4 * i + 2] - 1, litr[4 * i + 3] - 1};
@@ -2103,6 +2294,10 @@ int orc_fitors(const int32_t *ss_index, const int32_t *litr, int litor, const do
         qforce = (2.0 * b * (bb - a)) / sin(bb);
       const double da = bb - a;
       utors[type - 1] = utors[type - 1] + b * (da * da);
+      if (remitor && remitor[i]) {
+        *seg = *seg + b * (da * da);
+        qforce = qforce * f2;
+      }
     } else {
       const int tmp = (int)(a >= 0.0 ? floor(a + 0.5) : -floor(-a + 0.5)); /* IDNINT */
       const int ntph = abs(tmp);
@@ -2127,6 +2322,10 @@ int orc_fitors(const int32_t *ss_index, const int32_t *litr, int litor, const do
         default: status = 2; continue;
       }
       utors[type - 1] = utors[type - 1] + enrg;
+      if (remitor && remitor[i]) {
+        *seg = *seg + enrg;
+        qforce = qforce * f2;
+      }
     }
     for (int c = 0; c < 3; ++c) {
       double *f = c == 0 ? fpx : (c == 1 ? fpy : fpz);
@@ -2143,6 +2342,22 @@ int orc_fitors(const int32_t *ss_index, const int32_t *litr, int litor, const do
   *utors_slt = utors[0];
   *utors_slv = utors[1];
   return status;
+}
+
+int orc_fitors(const int32_t *ss_index, const int32_t *litr, int litor, const double *xp0, const double *yp0,
+               const double *zp0, const double *pota, const double *potb, const double *potc, double *utors_slt,
+               double *utors_slv, double *fpx, double *fpy, double *fpz) {
+  return fitors_impl(ss_index, litr, litor, xp0, yp0, zp0, pota, potb, potc, utors_slt, utors_slv, fpx, fpy, fpz,
+                     NULL, 1.0, NULL);
+}
+
+/* fitors_rem.f:1-301 (serial) */
+int orc_fitors_rem(const int32_t *ss_index, const int32_t *litr, int litor, const double *xp0, const double *yp0,
+                   const double *zp0, const double *pota, const double *potb, const double *potc, double *utors_slt,
+                   double *utors_slv, double *fpx, double *fpy, double *fpz, const int32_t *remitor, double f2,
+                   double *seg_ene) {
+  return fitors_impl(ss_index, litr, litor, xp0, yp0, zp0, pota, potb, potc, utors_slt, utors_slv, fpx, fpy, fpz,
+                     remitor, f2, seg_ene);
 }
 
 /* ------------------------------------------------------------------ */

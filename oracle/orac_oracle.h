@@ -91,6 +91,23 @@ int orc_force(const int32_t *ss_index, const double *xp0, const double *yp0, con
               double rinn, double rout, double *rtolinn, double rtolout, const double *gmass, int iz,
               int32_t *ncount1, const double *mass, double massinfty, int32_t lskip_ewald, int32_t ewald,
               int nthreads);
+/* force_REM.f:1-995 (serial): orc_force with the segment pairs of
+ * seg_select(rem_kind, rematom) (rem.f90:631-645) scaled by rem_f3 in force
+ * and their energy summed into seg_ene; lskip_force / ljintra as
+ * force_REM.f:148-158.  The routine's locals coul and rsp carry over from
+ * pair to pair as in the reference (start values zero). */
+int orc_force_rem(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                  const double *xpg, const double *ypg, const double *zpg, const double *charge,
+                  const int32_t *nbtype, const int32_t *type, int ma, int nato, const int32_t *atomg,
+                  const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *atomp,
+                  const double *co, const double *ecc12, const double *ecc6, double alphal, const int32_t *mapnl,
+                  int ngrp, const int32_t *grppt, double *uconf_slt, double *uconf_slv, double *uconf_ss,
+                  double *ucoul_slt, double *ucoul_slv, double *ucoul_ss, double *fpx, double *fpy, double *fpz,
+                  double *stress, const int32_t *nnlpp, int32_t *nnlpp1, long nnlpp1_cap, double rneigh,
+                  double rinn, double rout, double *rtolinn, double rtolout, const double *gmass, int iz,
+                  int32_t *ncount1, const double *mass, double massinfty, double *seg_ene,
+                  const int32_t *rematom, int rem_kind, double rem_f3, int32_t lskip_force, int32_t ljintra,
+                  int32_t lskip_ewald, int32_t ewald);
 /* mts_forces.f:212-241 decide_force */
 int32_t orc_decide_force(double rinn, double alphal);
 /* ferrf.f:1-244 without the ERF_CORR spline branch. Returns 0, 1 if erf_corr set. */
@@ -141,6 +158,21 @@ int orc_fptors(const int32_t *ss_index, const int32_t *ltor, int ltors, const do
 int orc_fitors(const int32_t *ss_index, const int32_t *litr, int litor, const double *xp0, const double *yp0,
                const double *zp0, const double *pota, const double *potb, const double *potc, double *utors_slt,
                double *utors_slv, double *fpx, double *fpy, double *fpz);
+/* REM twins (fnb14_rem.f, fptors_rem.f, fitors_rem.f, serial): per-term flags
+ * (1 = in the segment) whose energy is returned in *seg_ene and whose force
+ * is scaled by f2 = rem_factor(2) */
+void orc_fnb14_rem(const int32_t *ss_index, const double *xp0, const double *yp0, const double *zp0,
+                   const double *charge, int nato, const double *ecc12, const double *ecc6, int cut, int ewald,
+                   double alphal, const int32_t *int14, int int14p, const int32_t *type14, double fudge,
+                   double lj_fudge, double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv,
+                   double *ucoul_slt, double *ucoul_slv, const int32_t *remint14, double f2, double *seg_ene);
+int orc_fptors_rem(const int32_t *ss_index, const int32_t *ltor, int ltors, const double *xp0, const double *yp0,
+                   const double *zp0, const double *pota, const double *potb, double *utors_slt, double *utors_slv,
+                   double *fpx, double *fpy, double *fpz, const int32_t *remltor, double f2, double *seg_ene);
+int orc_fitors_rem(const int32_t *ss_index, const int32_t *litr, int litor, const double *xp0, const double *yp0,
+                   const double *zp0, const double *pota, const double *potb, const double *potc, double *utors_slt,
+                   double *utors_slv, double *fpx, double *fpy, double *fpz, const int32_t *remitor, double f2,
+                   double *seg_ene);
 void orc_fpbond(const int32_t *ss_index, const int32_t *lbnd, int lbond, const double *xp0, const double *yp0,
                 const double *zp0, const double *pota, const double *potb, double *ubond_slt, double *ubond_slv,
                 double *fpx, double *fpy, double *fpz);

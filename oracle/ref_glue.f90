@@ -59,3 +59,26 @@ subroutine refglue_set_alchemy(n, ss, lam, intra, ljin, opls) bind(C, name="refg
     ffield_type = 'amber'
   end if
 end subroutine refglue_set_alchemy
+
+! refglue_set_rem: the module state force_REM.f reads (rem: rem_segkind,
+! rem_factor, lskip_force; cpropar: ljintra).  f3 goes to rem_factor(3).
+subroutine refglue_set_rem(kind, f3, lskip, ljin) bind(C, name="refglue_set_rem")
+  use rem, only: rem_segkind, rem_factor, lskip_force
+  use cpropar, only: ljintra
+  implicit none
+  integer, value :: kind, lskip, ljin
+  real(8), value :: f3
+  rem_segkind = kind
+  rem_factor(3) = f3
+  lskip_force = lskip /= 0
+  ljintra = ljin /= 0
+end subroutine refglue_set_rem
+
+! refglue_set_rem_factor: rem_factor(k) (the bonded _rem routines read k = 2)
+subroutine refglue_set_rem_factor(k, f) bind(C, name="refglue_set_rem_factor")
+  use rem, only: rem_factor
+  implicit none
+  integer, value :: k
+  real(8), value :: f
+  rem_factor(k) = f
+end subroutine refglue_set_rem_factor
