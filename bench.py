@@ -43,6 +43,28 @@ FLOP_PER_PAIR = 89
 # HBM bytes per launch of k_pairs<double,3,false> on the 24,460-atom system, from the
 # committed FETCH_SIZE / WRITE_SIZE profile (rocprof_summary traffic)
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bytes_k_pairs_fp64_solvated24k.txt")
+PAIR_KERNEL = "k_pairs_own<double,3>"
+
+
+def pme_stage_roofline(s, stages, fp64):
+    """HBM fraction of the reciprocal-sum stages of one evaluation: algorithmic
+    bytes (spread 32N + 8G in FP64, 16N + 8G mixed; FFT stage 2(8G + 16Gc) +
+    32Gc; gather 8G + 56N in FP64, 8G + 40N mixed) over the stage's event time."""
+    n = s.natoms
+    k1, k2, k3 = s.nfft
+    g = k1 * k2 * k3
+    gc = (k1 // 2 + 1) * k2 * k3
+    b = {"spread": (32 if fp64 else 16) * n + 8 * g, "fft": 2 * (8 * g + 16 * gc) + 32 * gc,
+         "gather": 8 * g + (56 if fp64 else 40) * n}
+    out = {}
+    for k, key in (("spread", "pme_spread"), ("fft", "pme_fft"), ("gather", "pme_gather")):
+        ms = float(stages.get(key, 0.0))
+        if ms <= 0:
+            continue
+        gbs = b[k] / (ms * 1e-3) / 1e9
+        out[k] = {"bound": "hbm", "bytes": b[k], "ms": ms, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": gbs / HBM_PEAK_GBS}
+    return out
 
 
 def log(msg):
@@ -184,20 +206,53 @@ def ns_per_day(steps, wall, dt=2.0):
     return steps * dt * 1e-6 / wall * 86400.0
 
 
-def serving_leg(ctx, nreq=16, rate=200.0, nsteps=500, budget_gib=64, seed=4242):
+def _served_physics(reqs):
+    """Per-request final temperature and NVE drift (kJ/mol/ns per atom) of a served run."""
+    temps = np.array([r.energies["temperature"] for r in reqs])
+    drift = np.array([r.energies.get("drift_kj_mol_per_ns_per_atom", np.nan) for r in reqs])
+    return dict(final_temperature_K={"min": float(temps.min()), "median": float(np.median(temps)),
+                                     "max": float(temps.max())},
+                nve_drift_kj_mol_per_ns_per_atom={"median_abs": float(np.nanmedian(np.abs(drift))),
+                                                  "max_abs": float(np.nanmax(np.abs(drift)))})
+
+
+def serving_leg(ctx, nreq=64, rate=200.0, nsteps=500, budget_gib=16, seed=4242, equil=100):
+    """Poisson stream of MD segment requests (solutes log-uniform in 2k-150k
+    atoms), each relaxed and equilibrated at 300 K before it is submitted;
+    the HBM budget is small enough that requests queue for memory."""
     from orac_hip import serve
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(2000), np.log(150000), nreq)).astype(int)
     arrivals = serve.poisson_arrivals(rng, nreq, rate)
     reqs = [serve.Request(rid=k, system=serve.serving_request(seed + k, int(n), rcut=float(rng.uniform(9.0, 10.0))),
                           nsteps=nsteps, arrival=float(arrivals[k]), seed=seed + k) for k, n in enumerate(sizes)]
-    log(f"serving: {nreq} requests, {sum(r.system.natoms for r in reqs)} atoms in all; relaxing start positions")
-    serve.relax_requests(ctx, reqs, niter=200)
+    log(f"serving: {nreq} requests, {sum(r.system.natoms for r in reqs)} atoms in all; relaxing and equilibrating")
+    serve.relax_requests(ctx, reqs, niter=200, equilibrate=equil)
     srv = serve.Server(ctx, hbm_budget=budget_gib << 30, chunk=25)
     srv.progress = log
     st = srv.run(reqs)
+    waits = np.array([r.admitted - r.arrival for r in reqs])
     st.update(sizes_atoms=[r.system.natoms for r in reqs], rate_per_s=rate, steps_per_request=nsteps,
-              budget_gib=budget_gib, fail=sum(r.energies["fail"] for r in reqs))
+              budget_gib=budget_gib, fail=sum(r.energies["fail"] for r in reqs),
+              queued_requests=int((waits > 0.05).sum()), max_admission_wait_ms=float(waits.max() * 1e3),
+              **_served_physics(reqs))
+    return st
+
+
+def served_hrem_leg(ctx, reps, nsteps=500, rate=200.0, seed=77, equil=100):
+    """The replicas of the batched leg as served requests: Poisson arrivals,
+    continuous batching in mixed precision, each request equilibrated at 300 K
+    beforehand; aggregate ns/day and p50/p99 of the segment latency."""
+    from orac_hip import serve
+    rng = np.random.default_rng(seed)
+    arrivals = serve.poisson_arrivals(rng, len(reps), rate)
+    reqs = [serve.Request(rid=k, system=s, nsteps=nsteps, arrival=float(arrivals[k]), seed=seed + k)
+            for k, s in enumerate(reps)]
+    serve.relax_requests(ctx, reqs, niter=300, equilibrate=equil)
+    srv = serve.Server(ctx, hbm_budget=64 << 30, chunk=50)
+    st = srv.run(reqs)
+    st.update(replicas=len(reps), steps_per_request=nsteps, rate_per_s=rate,
+              fail=sum(r.energies["fail"] for r in reqs), **_served_physics(reqs))
     return st
 
 
@@ -232,13 +287,45 @@ def c0_leg(ctx, steps, md=True):
     return out
 
 
+DUDL_H = 0.005
+
+
+def dudl_central(b, rss, lams):
+    """dU/dlambda of each window (internal units) from its dwrk = U(lambda) -
+    U(lambda0) at two lambda0 around lambda: (dwrk(l - h) - dwrk(l + h)) / 2h,
+    and at the ends of [0, 1] the one-sided three-point forms
+    (dwrk(2h) - 4 dwrk(h)) / 2h and (4 dwrk(1 - h) - dwrk(1 - 2h)) / 2h.
+    The windows' own lambda0 are restored afterwards."""
+    from orac_hip import synth
+    h = DUDL_H
+    plans = []
+    for lam in lams:
+        l = float(lam[1])
+        if l - h < 0.0:
+            plans.append(((l + 2 * h, 1.0), (l + h, -4.0)))
+        elif l + h > 1.0:
+            plans.append(((l - h, 4.0), (l - 2 * h, -1.0)))
+        else:
+            plans.append(((l - h, 1.0), (l + h, -1.0)))
+    du = np.zeros(len(lams))
+    for k in range(2):
+        for rs, lam, pl in zip(rss, lams, plans):
+            rs.set_lambda(synth.lambda_state(float(lam[1]), lrm0=pl[k][0]))
+        b.eval(rebuild_list=True)
+        for w, (rs, pl) in enumerate(zip(rss, plans)):
+            du[w] += pl[k][1] * rs.alchemy_results()["dwrk"] / (2 * h)
+    for rs, lam in zip(rss, lams):
+        rs.set_lambda(lam)
+    return du.tolist()
+
+
 def alchemy_leg(ctx, steps, md_steps=0):
     """16 lambda-windows of a 48-atom ligand decoupled in a 40 A water box
     (6,399 atoms, PME 48^3 order 5, rc 9 A), every window resident with its
     own charge grids, one batch: soft-core direct pairs, reciprocal sum on the
     window's charges and the energy-only grid of its neighbour window's
     charges (dwrk = U(lambda_w) - U(lambda_w')).  Mixed against FP64: forces
-    and per-window dU/dlambda ~ dwrk / (lambda_w - lambda_w')."""
+    and per-window dU/dlambda (dudl_central)."""
     from orac_hip import md, synth
     from orac_hip.api import Batch, ResidentSystem
     s, al, lams = synth.alchemy_windows()
@@ -262,7 +349,7 @@ def alchemy_leg(ctx, steps, md_steps=0):
         dt = time.perf_counter() - t0
         win = [(rs.results()["f"], rs.alchemy_results()) for rs in rss]
         stages = profile_stages(b, 3)
-        r = dict(dt=dt, win=win, stages=stages, tuning=tuning)
+        r = dict(dt=dt, win=win, stages=stages, tuning=tuning, dudl=dudl_central(b, rss, lams))
         if md_steps and prec == 1:
             dyns = [md.Dynamics(rs, s) for rs in rss]
             md.relax_batch(b, 200, dmax=0.03)
@@ -275,10 +362,9 @@ def alchemy_leg(ctx, steps, md_steps=0):
             rs.close()
         res[prec] = r
     n = len(lams)
-    dl = np.array([lam[1] - lam[3] for lam in lams])
     kj = synth.EFACT / 1000.0  # kJ/mol per internal energy unit
-    du64 = np.array([w[1]["dwrk"] for w in res[0]["win"]]) / dl * kj
-    du32 = np.array([w[1]["dwrk"] for w in res[1]["win"]]) / dl * kj
+    du64 = np.array(res[0]["dudl"]) * kj
+    du32 = np.array(res[1]["dudl"]) * kj
     ferr = max(relrms(a[0], b[0]) for a, b in zip(res[1]["win"], res[0]["win"]))
     out = {"workload": "16 lambda-windows (charge and soft-core LJ lambda 0->1) of a 48-atom ligand in a 40 A water box "
                        "(6,399 atoms), PME 48^3 order 5, alpha 0.35, rc 9 A; per window its own charge grid and the "
@@ -289,6 +375,8 @@ def alchemy_leg(ctx, steps, md_steps=0):
            "lambda": [float(lam[1]) for lam in lams], "lambda0": [float(lam[3]) for lam in lams],
            "dU_dlambda_fp64": du64.tolist(), "dU_dlambda_mixed": du32.tolist(),
            "dU_dlambda_unit": "kJ/mol per unit lambda",
+           "dU_dlambda_method": f"second-order finite difference of dwrk at lambda0 = lambda +- {DUDL_H} "
+                                "(one-sided three-point at lambda 0 and 1), both precisions",
            "dU_dlambda_max_abs_err_mixed": float(np.abs(du32 - du64).max()),
            "dU_dlambda_max_rel_err_mixed": float(np.abs(du32 - du64).max() / np.abs(du64).max()),
            "force_relrms_mixed_vs_f64_max": ferr, "tuning": res[1]["tuning"],
@@ -359,8 +447,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--replicas", type=int, default=32, help="HREM replicas of the batched leg (0 skips it)")
     ap.add_argument("--md-steps", type=int, default=200, help="outer steps of each dynamics leg (0 skips them)")
-    ap.add_argument("--serve-requests", type=int, default=8, help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--serve-requests", type=int, default=64, help="requests of the serving leg (0 skips it)")
     ap.add_argument("--serve-steps", type=int, default=500, help="outer steps per served request")
+    ap.add_argument("--serve-hrem", type=int, default=1,
+                    help="serve the HREM replicas of the batched leg as requests (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true", help="launch batched and dynamics steps eagerly")
     ap.add_argument("--no-water", action="store_true", help="skip the 648-atom water box leg")
@@ -462,6 +552,14 @@ def main():
                 for x in rss:
                     x.close()
         served = None
+        served_hrem = None
+        if args.serve_hrem and args.replicas > 0 and args.md_steps > 0:
+            served_hrem = served_hrem_leg(ctx, reps, nsteps=args.serve_steps)
+            log(f"served HREM x{args.replicas}: p50 {served_hrem['p50_ms']:.1f} ms p99 {served_hrem['p99_ms']:.1f} ms, "
+                f"{served_hrem['aggregate_ns_per_day']:.1f} ns/day")
+            if dyn:
+                served_hrem["speedup_vs_plain_fp64_ns_per_day"] = (served_hrem["aggregate_ns_per_day"]
+                                                                   / dyn["fp64"]["ns_per_day"])
         if args.serve_requests > 0:
             served = serving_leg(ctx, nreq=args.serve_requests, nsteps=args.serve_steps)
             log(f"serving: p50 {served['p50_ms']:.1f} ms p99 {served['p99_ms']:.1f} ms, "
@@ -473,13 +571,18 @@ def main():
     err32 = relrms(r32["f"], f64)
     kms = st64["pair_kernel"]
     achieved = FLOP_PER_PAIR * npairs / (kms * 1e-3) / 1e12
-    from orac_hip.rocprof_summary import read_traffic
+    from orac_hip.rocprof_summary import read_traffic, read_stamp
     traffic = read_traffic(TRAFFIC_PROFILE)
-    roof = {"bound": "mfma", "kernel": "k_pairs<double,3,false>", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+    # compute-bound on the FP64 rate: the pair arithmetic is FP64 VALU (no
+    # matrix contraction); MI355X's FP64 vector and matrix peaks are the same
+    # 78.6 TFLOP/s, the contract's "mfma" compute roof
+    roof = {"bound": "mfma", "pipe": "FP64 VALU", "kernel": PAIR_KERNEL, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
             "per_launch": {"pairs": npairs, "flop_per_pair": FLOP_PER_PAIR, "ms": kms,
                            "traffic_unit": "bytes (FETCH_SIZE + WRITE_SIZE, probe-scaled)",
-                           "note": "FP64 VALU (no dense contraction): peak is the FP64 vector rate"}}
+                           "traffic_source": read_stamp(TRAFFIC_PROFILE),
+                           "note": "FP64 VALU (no dense contraction): peak is the FP64 vector rate"},
+            "stages": pme_stage_roofline(s, st64, fp64=True)}
 
     def rnd(d):
         return {k: round(float(v), 4) for k, v in d.items()}
@@ -526,6 +629,11 @@ def main():
                            "(SHAKE/RATTLE), list every 10 steps; after 200 steepest-descent steps and "
                            "Maxwell-Boltzmann velocities at 300 K")
         out["dynamics"] = dyn
+    if served_hrem:
+        served_hrem["workload"] = (f"the {args.replicas} HREM replicas of the batched leg as served MD segment "
+                                   "requests (Poisson arrivals), each relaxed and equilibrated at 300 K, mixed "
+                                   "precision, continuous batching")
+        out["served_hrem"] = served_hrem
     if served:
         served["workload"] = ("Poisson stream of MD segment requests (lattice solutes of 2k-150k atoms, log-uniform, "
                               "in water boxes; grid spacing <= 1 A; order 5; cutoff 9-10 A), mixed precision, "

@@ -101,6 +101,29 @@ int orac_hip_force(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0
                    double rtolout, const double *gmass, int32_t iz, int32_t *ncount1, const double *mass,
                    double massinfty, int32_t lskip_ewald, int32_t ewald);
 
+/* Replaces force_rem (force_REM.f:1-995) as mts_forces.f:120-124 calls it
+ * for a REM segment: arguments as orac_hip_force, plus seg_ene (out: the
+ * energy of the selected pairs), rematom (per atom, nonzero in the segment)
+ * and what force_REM reads from modules rem and cpropar: rem_segkind (0
+ * either atom in the segment, 1 both, 2 exactly one; rem.f90:631-645),
+ * rem_factor(3) (the selected pairs' force factor), lskip_force and ljintra
+ * (force_REM.f:143-157: without ljintra nothing is computed, with it the
+ * charges count as zero).  Only the Ewald branch (lskip_ewald 0, ewald 1)
+ * is taken: the others read a distance or Coulomb energy that an earlier
+ * pair left in the routine's locals (force_REM.f:491, :667), status 4. */
+int orac_hip_force_rem(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                       const double *zp0, const double *xpg, const double *ypg, const double *zpg, const double *charge,
+                       const int32_t *nbtype, const int32_t *type, int32_t ma, int32_t nato, const int32_t *atomg,
+                       const double *xpcm, const double *ypcm, const double *zpcm, const int32_t *groupp,
+                       const int32_t *atomp, const double *co, const double *ecc12, const double *ecc6, double alphal,
+                       const int32_t *mapnl, int32_t ngrp, const int32_t *grppt, double *uconf_slt,
+                       double *uconf_slv, double *uconf_ss, double *ucoul_slt, double *ucoul_slv, double *ucoul_ss,
+                       double *fpx, double *fpy, double *fpz, double *stress, const int32_t *nnlpp, int32_t *nnlpp1,
+                       int64_t nnlpp1_capacity, double rneigh, double rinn, double rout, double *rtolinn,
+                       double rtolout, const double *gmass, int32_t iz, int32_t *ncount1, const double *mass,
+                       double massinfty, double *seg_ene, const int32_t *rematom, int32_t lskip_ewald, int32_t ewald,
+                       int32_t rem_segkind, double rem_factor3, int32_t lskip_force, int32_t ljintra);
+
 /* Replaces force_alchemy (force_alchemy.f:1-759, the soft-core GROMOS form)
  * as mts_forces.f calls it for an alchemical run.  Arguments as
  * orac_hip_force, plus what force_alchemy reads from module cpropar and its
@@ -215,6 +238,32 @@ int orac_hip_fptors(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *l
 int orac_hip_fitors(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *litr, int32_t litor, const double *xp0,
                     const double *yp0, const double *zp0, const double *pota, const double *potb, const double *potc,
                     double *utors_slt, double *utors_slv, double *fpx, double *fpy, double *fpz, int32_t mb);
+
+/* Replace the REM twins as mts_intra_n1.f:74-78, :112-116 and
+ * mts_intra_n0.f:91-95 call them for a REM segment: fptors_rem
+ * (fptors_rem.f:1-352), fitors_rem (fitors_rem.f:1-301) and fnb14_rem
+ * (fnb14_rem.f:1-234).  Arguments as orac_hip_fptors, orac_hip_fitors and
+ * orac_hip_fnb14, plus the per-term flags (remltor, remitor, remint14:
+ * nonzero in the segment), seg_ene (out: the flagged terms' energy) and
+ * rem_factor(2) (module rem), which scales the flagged terms' forces.
+ * fptors_rem evaluates the 60-degree-phase multiplicities 12 and 14 as the
+ * REM routine does (no sign factor; no sine-derivative term where the sine
+ * is below 1e-8).  With no 1-4 pair, fnb14_rem leaves seg_ene as it was
+ * (fnb14_rem.f:100). */
+int orac_hip_fptors_rem(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *ltor, int32_t ltors,
+                        const double *xp0, const double *yp0, const double *zp0, const double *pota, const double *potb,
+                        double *utors_slt, double *utors_slv, double *fpx, double *fpy, double *fpz,
+                        const int32_t *remltor, double *seg_ene, double rem_factor2, int32_t mb);
+int orac_hip_fitors_rem(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *litr, int32_t litor,
+                        const double *xp0, const double *yp0, const double *zp0, const double *pota, const double *potb,
+                        const double *potc, double *utors_slt, double *utors_slv, double *fpx, double *fpy,
+                        double *fpz, const int32_t *remitor, double *seg_ene, double rem_factor2, int32_t mb);
+int orac_hip_fnb14_rem(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                       const double *zp0, const double *charge, int32_t nato, const double *ecc12, const double *ecc6,
+                       int32_t cut, int32_t ewald, double alphal, const int32_t *int14, int32_t int14p,
+                       const int32_t *type14, double fudge, double lj_fudge, double *fpx, double *fpy, double *fpz,
+                       double *uconf_slt, double *uconf_slv, double *ucoul_slt, double *ucoul_slv,
+                       const int32_t *remint14, double *seg_ene, double rem_factor2);
 
 /* Replaces fpbond (fpbond.f:1-110) as mts_intra_n0.f:73-74 calls it:
  * harmonic bonds lbnd(2, lbond) on the coordinates as given, pota the rest
@@ -348,8 +397,10 @@ int orac_hip_system_results(orac_hip_ctx *ctx, int32_t sys, double *fx, double *
 /* Stage timing: when enabled, eval records HIP events between its stages;
  * stage_ms returns, for the last eval, the milliseconds of
  * [0] group and molecule centres, [1] list build, [2] direct-space pairs,
- * [3] PME (spread, FFT, convolution, gather), [4] corrections, and
- * [5] the pair kernel alone (within [2]). */
+ * [3] PME (spread, FFT, convolution, gather), [4] corrections,
+ * [5] the pair kernel alone (within [2]), and within [3] [6] the spread
+ * chain (binning, B-splines, tiles), [7] the FFT stage (both transforms and
+ * the convolution) and [8] the gather. */
 int orac_hip_system_profile(orac_hip_ctx *ctx, int32_t sys, int32_t enable);
 int orac_hip_system_stage_ms(orac_hip_ctx *ctx, int32_t sys, float *ms, int32_t n);
 /* Device bytes held by a system (for admission against an HBM budget). */

@@ -291,7 +291,7 @@ extern "C" int orac_hip_lc_list(orac_hip_ctx *ctx, int32_t ncx, int32_t ncy, int
     L.cell_fill = B.get<int>("l_fill", natp);
     L.count = B.get<int>("l_count", natp + 1);
     L.off = B.get<int>("l_off", natp + 1);
-    L.total = B.get<int>("l_total", 2);
+    L.total = B.get<int>("l_total", 4);  // entries, overflow, largest count, sticky overflow
     L.cap = nnlpp_capacity;
     L.mode = 1;
     // pass 1: counts and the capacity check against the caller's array
@@ -328,6 +328,17 @@ struct AlchIn {
   int shifted;
 };
 
+// force_REM's extra inputs: rematom (per atom, nonzero in the segment),
+// rem_segkind, rem_factor(3), lskip_force and ljintra (module state in the
+// reference, arguments here), and the segment energy it returns
+struct RemIn {
+  const int32_t *rematom;
+  int kind;
+  double f3;
+  int lskip_force, ljintra;
+  double *seg_ene;
+};
+
 static int force_impl(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
                       const double *zp0, const double *xpg, const double *ypg, const double *zpg, const double *charge,
                       const int32_t *nbtype, const int32_t *type, int32_t ma, int32_t nato, const int32_t *atomg,
@@ -338,11 +349,31 @@ static int force_impl(orac_hip_ctx *ctx, const int32_t *ss_index, const double *
                       double *fpy, double *fpz, double *stress, const int32_t *nnlpp, int32_t *nnlpp1,
                       int64_t nnlpp1_capacity, double rneigh, double rinn, double rout, double *rtolinn,
                       double rtolout, const double *gmass, int32_t iz, int32_t *ncount1, const double *mass,
-                      double massinfty, int32_t lskip_ewald, int32_t ewald, const AlchIn *al) {
+                      double massinfty, int32_t lskip_ewald, int32_t ewald, const AlchIn *al,
+                      const RemIn *rem = nullptr) {
   return guarded([&] {
     set_device(ctx);
     (void)groupp;
     (void)atomg;
+    std::vector<double> q_off;
+    if (rem) {
+      ORAC_REQUIRE(rem->rematom && rem->seg_ene, E_ARG, "null rematom or seg_ene");
+      ORAC_REQUIRE(rem->kind >= 0 && rem->kind <= 2, E_ARG, "rem_segkind must be 0, 1 or 2");
+      // force_REM.f:143-157: lskip_force without ljintra computes nothing
+      // (the outputs are left as they are); with it the charges are zero
+      if (rem->lskip_force && !rem->ljintra) return;
+      // force_REM.f:491 and :667 read the Coulomb energy or the distance a
+      // previous pair or group left in the routine's locals when lskip_ewald
+      // is set: an order-dependent result with no parallel form
+      ORAC_REQUIRE(!lskip_ewald && ewald, E_UNSUPPORTED,
+                   "force_rem: only the Ewald branch (lskip_ewald = .false., ewald = .true.) is supported; the "
+                   "others carry over locals from earlier pairs (force_REM.f:491, :667)");
+      *rem->seg_ene = 0.0;
+      if (rem->lskip_force) {
+        q_off.assign((size_t)std::max(nato, 0), 0.0);
+        charge = q_off.data();
+      }
+    }
     ORAC_REQUIRE(nato >= 0 && ngrp >= 0 && ma >= 1, E_ARG, "bad sizes");
     ORAC_REQUIRE(nato < (1 << 26), E_UNSUPPORTED, "more than 2^26 atoms");
     ORAC_REQUIRE(ss_index && xp0 && yp0 && zp0 && xpg && ypg && zpg && charge && nbtype && type && co && ecc12 &&
@@ -537,8 +568,16 @@ static int force_impl(orac_hip_ctx *ctx, const int32_t *ss_index, const double *
     p.sgp = B.get<double4>("f_sgp", std::max(1, ngrp));
     p.sgi = B.get<int4>("f_sgi", std::max(1, ngrp));
     p.sap = B.get<double4>("f_sap", std::max(1, nato));
+    p.sapc = B.get<double4>("f_sapc", std::max(1, nato));
     p.sai = B.get<int2>("f_sai", std::max(1, nato));
     p.max_group = max_group;
+    if (rem) {
+      std::vector<int> rf(nato);
+      for (int a = 0; a < nato; ++a) rf[a] = rem->rematom[a] != 0;
+      p.rematom = B.put_vec("f_rematom", rf, st, false);
+      p.rem_kind = rem->kind;
+      p.rem_f3 = rem->f3;
+    }
     double *dout = B.get<double>("f_out", 16);
     p.out = dout;
     PairDesc *dd = B.put("f_desc", &p, 1, st, false);
@@ -575,7 +614,30 @@ static int force_impl(orac_hip_ctx *ctx, const int32_t *ss_index, const double *
         for (int k = 0; k < 3; ++k) stress[r + 3 * c] = stress[r + 3 * c] + s[3 * r + k] * co[c + 3 * k];
     // force_alchemy.f:684-688: dwrk += sum of U(lambda) - U(lambda0)
     if (al) *al->dwrk = *al->dwrk + out[15];
+    if (rem) *rem->seg_ene = out[15];
   });
+}
+
+extern "C" int orac_hip_force_rem(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                                  const double *zp0, const double *xpg, const double *ypg, const double *zpg,
+                                  const double *charge, const int32_t *nbtype, const int32_t *type, int32_t ma,
+                                  int32_t nato, const int32_t *atomg, const double *xpcm, const double *ypcm,
+                                  const double *zpcm, const int32_t *groupp, const int32_t *atomp, const double *co,
+                                  const double *ecc12, const double *ecc6, double alphal, const int32_t *mapnl,
+                                  int32_t ngrp, const int32_t *grppt, double *uconf_slt, double *uconf_slv,
+                                  double *uconf_ss, double *ucoul_slt, double *ucoul_slv, double *ucoul_ss,
+                                  double *fpx, double *fpy, double *fpz, double *stress, const int32_t *nnlpp,
+                                  int32_t *nnlpp1, int64_t nnlpp1_capacity, double rneigh, double rinn, double rout,
+                                  double *rtolinn, double rtolout, const double *gmass, int32_t iz, int32_t *ncount1,
+                                  const double *mass, double massinfty, double *seg_ene, const int32_t *rematom,
+                                  int32_t lskip_ewald, int32_t ewald, int32_t rem_segkind, double rem_factor3,
+                                  int32_t lskip_force, int32_t ljintra) {
+  const RemIn r{rematom, rem_segkind, rem_factor3, lskip_force, ljintra, seg_ene};
+  return force_impl(ctx, ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, ypcm,
+                    zpcm, groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, uconf_ss,
+                    ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, nnlpp1_capacity, rneigh,
+                    rinn, rout, rtolinn, rtolout, gmass, iz, ncount1, mass, massinfty, lskip_ewald, ewald, nullptr,
+                    &r);
 }
 
 extern "C" int orac_hip_force(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
@@ -954,7 +1016,8 @@ static bool bytes_changed(BufSet &B, const std::string &name, const void *host, 
 static int bonded_eval(orac_hip_ctx *ctx, const char *name, int arity, const int32_t *ss_index, const int32_t *lst,
                        int32_t n, const double *xp0, const double *yp0, const double *zp0,
                        const double *const *pots, int npots, double *fpx, double *fpy, double *fpz, int32_t mb,
-                       double e[2], bool improper = false) {
+                       double *e, bool improper = false, const int32_t *remf = nullptr, double remf2 = 1.0,
+                       bool rem_variant = false) {
   ORAC_REQUIRE(ss_index && lst && xp0 && yp0 && zp0 && fpx && fpy && fpz && mb > 0 && n > 0, E_ARG, "null argument");
   for (int k = 0; k < npots; ++k) ORAC_REQUIRE(pots[k], E_ARG, "null argument");
   hipStream_t st = ctx->stream;
@@ -1024,7 +1087,17 @@ static int bonded_eval(orac_hip_ctx *ctx, const char *name, int arity, const int
   h.fx = B.get<double>("t_fx", mb);
   h.fy = B.get<double>("t_fy", mb);
   h.fz = B.get<double>("t_fz", mb);
-  double *dout = B.get<double>("t_out", 2);
+  // REM twins: e[2] returns the flagged terms' energy
+  const int ne = remf ? 3 : 2;
+  if (remf) {
+    std::vector<int> fl(n);
+    for (int i = 0; i < n; ++i) fl[i] = remf[i] != 0;
+    h.remf = B.put_vec("t_remf", fl, st, false);
+    h.remf2 = remf2;
+    h.rem_variant = rem_variant ? 1 : 0;
+    h.tseg = B.get<double>("t_tseg", n);
+  }
+  double *dout = B.get<double>("t_out", 3);
   h.out = dout;
   TorsDesc *dd = B.put("t_desc", &h, 1, st, false);
   launch_tors(dd, h, st);
@@ -1034,7 +1107,7 @@ static int bonded_eval(orac_hip_ctx *ctx, const char *name, int arity, const int
   ORAC_HIP(hipMemcpyAsync(fx.data(), h.fx, mb * sizeof(double), hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipMemcpyAsync(fy.data(), h.fy, mb * sizeof(double), hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipMemcpyAsync(fz.data(), h.fz, mb * sizeof(double), hipMemcpyDeviceToHost, st));
-  ORAC_HIP(hipMemcpyAsync(e, dout, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  ORAC_HIP(hipMemcpyAsync(e, dout, ne * sizeof(double), hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipMemcpyAsync(status, h.status, sizeof status, hipMemcpyDeviceToHost, st));
   ORAC_HIP(hipStreamSynchronize(st));
   // an unsupported multiplicity (the reference stops) outranks a singular torsion
@@ -1087,6 +1160,52 @@ extern "C" int orac_hip_fitors(orac_hip_ctx *ctx, const int32_t *ss_index, const
     ORAC_REQUIRE(status != 2, E_UNSUPPORTED, "fitors: improper torsion multiplicity other than 1, 2, 3, 4, 6");
     *utors_slt = e[0];
     *utors_slv = e[1];
+  });
+}
+
+extern "C" int orac_hip_fptors_rem(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *ltor, int32_t ltors,
+                                   const double *xp0, const double *yp0, const double *zp0, const double *pota,
+                                   const double *potb, double *utors_slt, double *utors_slv, double *fpx, double *fpy,
+                                   double *fpz, const int32_t *remltor, double *seg_ene, double rem_factor2,
+                                   int32_t mb) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(utors_slt && utors_slv && seg_ene, E_ARG, "null output");
+    *utors_slt = *utors_slv = *seg_ene = 0.0;
+    if (ltors == 0) return;
+    ORAC_REQUIRE(ltors > 0 && remltor, E_ARG, "negative count or null remltor");
+    const double *pots[2] = {pota, potb};
+    double e[3];
+    const int status = bonded_eval(ctx, "fptors", 4, ss_index, ltor, ltors, xp0, yp0, zp0, pots, 2, fpx, fpy, fpz,
+                                   mb, e, false, remltor, rem_factor2, true);
+    ORAC_REQUIRE(status != 2, E_UNSUPPORTED, "fptors_rem: torsion multiplicity outside 1-8, 12, 14");
+    ORAC_REQUIRE(status != 1, E_STATE,
+                 "fptors_rem: a torsion with collinear atoms (the reference perturbs it at random; not reproduced)");
+    *utors_slt = e[0];
+    *utors_slv = e[1];
+    *seg_ene = e[2];
+  });
+}
+
+extern "C" int orac_hip_fitors_rem(orac_hip_ctx *ctx, const int32_t *ss_index, const int32_t *litr, int32_t litor,
+                                   const double *xp0, const double *yp0, const double *zp0, const double *pota,
+                                   const double *potb, const double *potc, double *utors_slt, double *utors_slv,
+                                   double *fpx, double *fpy, double *fpz, const int32_t *remitor, double *seg_ene,
+                                   double rem_factor2, int32_t mb) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(utors_slt && utors_slv && seg_ene, E_ARG, "null output");
+    *utors_slt = *utors_slv = *seg_ene = 0.0;
+    if (litor == 0) return;
+    ORAC_REQUIRE(litor > 0 && remitor, E_ARG, "negative count or null remitor");
+    const double *pots[3] = {pota, potb, potc};
+    double e[3];
+    const int status = bonded_eval(ctx, "fitors", 4, ss_index, litr, litor, xp0, yp0, zp0, pots, 3, fpx, fpy, fpz,
+                                   mb, e, true, remitor, rem_factor2);
+    ORAC_REQUIRE(status != 2, E_UNSUPPORTED, "fitors_rem: improper torsion multiplicity other than 1, 2, 3, 4, 6");
+    *utors_slt = e[0];
+    *utors_slv = e[1];
+    *seg_ene = e[2];
   });
 }
 
@@ -1173,6 +1292,62 @@ extern "C" int orac_hip_fnb14(orac_hip_ctx *ctx, const int32_t *ss_index, const 
     *ucoul_slv = e[1];
     *uconf_slt = e[2];
     *uconf_slv = e[3];
+  });
+}
+
+extern "C" int orac_hip_fnb14_rem(orac_hip_ctx *ctx, const int32_t *ss_index, const double *xp0, const double *yp0,
+                                  const double *zp0, const double *charge, int32_t nato, const double *ecc12,
+                                  const double *ecc6, int32_t cut, int32_t ewald, double alphal, const int32_t *int14,
+                                  int32_t int14p, const int32_t *type14, double fudge, double lj_fudge, double *fpx,
+                                  double *fpy, double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt,
+                                  double *ucoul_slv, const int32_t *remint14, double *seg_ene, double rem_factor2) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(uconf_slt && uconf_slv && ucoul_slt && ucoul_slv && seg_ene, E_ARG, "null output");
+    *uconf_slt = *uconf_slv = *ucoul_slt = *ucoul_slv = 0.0;
+    if (int14p == 0) return;  // fnb14_rem.f:100, before seg_ene is zeroed (it is left as it was)
+    *seg_ene = 0.0;
+    ORAC_REQUIRE(ss_index && xp0 && yp0 && zp0 && charge && ecc12 && ecc6 && int14 && type14 && remint14 &&
+                     nato > 0 && int14p > 0,
+                 E_ARG, "null argument");
+    hipStream_t st = ctx->stream;
+    BufSet &B = ctx->B;
+    int lmax = 1;
+    for (int k = 0; k < int14p; ++k) {
+      ORAC_REQUIRE(type14[k] >= 1, E_ARG, "fnb14_rem: type14 entries are 1-based packed indices");
+      lmax = std::max(lmax, type14[k]);
+    }
+    for (int a = 0; a < nato; ++a)
+      ORAC_REQUIRE(ss_index[a] == 1 || ss_index[a] == 2, E_ARG, "ss_index must be 1 or 2");
+    std::vector<int> fl(int14p);
+    for (int k = 0; k < int14p; ++k) fl[k] = remint14[k] != 0;
+    CorrDesc h;
+    std::memset(&h, 0, sizeof h);
+    h.xp = B.put("n_x", xp0, nato, st, false);
+    h.yp = B.put("n_y", yp0, nato, st, false);
+    h.zp = B.put("n_z", zp0, nato, st, false);
+    h.q = B.put("n_q", charge, nato, st, true);
+    h.ss = B.put("n_ss", ss_index, nato, st, true);
+    h.pair_type = B.put("n_t14", type14, int14p, st, true);
+    h.ecc12 = B.put("n_e12", ecc12, lmax, st, true);
+    h.ecc6 = B.put("n_e6", ecc6, lmax, st, true);
+    h.remf = B.put_vec("n_remf", fl, st, false);
+    h.remf2 = rem_factor2;
+    h.ncol = 5;
+    h.alpha = alphal;
+    h.twrtpi = 2.0 / std::sqrt(4.0 * std::atan(1.0));
+    h.fudge = fudge;
+    h.lj_fudge = lj_fudge;
+    h.iz = 1;
+    h.kind = 2;
+    h.ewald = (ewald && cut) ? 1 : 0;
+    double e[5];
+    run_corr_pairs(ctx, h, int14, int14p, nato, "nr_", e, fpx, fpy, fpz);
+    *ucoul_slt = e[0];
+    *ucoul_slv = e[1];
+    *uconf_slt = e[2];
+    *uconf_slv = e[3];
+    *seg_ene = e[4];
   });
 }
 

@@ -34,6 +34,7 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
   double *te = d.te + 2 * (size_t)i;
   for (int k = 0; k < 12; ++k) tf[k] = 0.0;
   te[0] = te[1] = 0.0;
+  if (d.tseg) d.tseg[i] = 0.0;
   const double b = d.potb[i];
   if (fabs(b) < 1.0e-12) return;  // fptors.f:92
   const int4 l = d.atoms[i];
@@ -138,7 +139,8 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
       const double xn = y21 * z32 - y32 * z21, yn = z21 * x32 - z32 * x21, zn = x21 * y32 - x32 * y21;
       const double aux4 = fsign(bb, xn * x43 + yn * y43 + zn * z43);
       const double soa = fsign(sqrt(1.0 - coa * coa), aux4);
-      const double dsoa = -coa / soa;
+      // fptors_rem.f:269-273: no derivative term where the sine vanishes
+      const double dsoa = (d.rem_variant && fabs(soa) < 1.0e-8) ? 0.0 : -coa / soa;
       double auxn, dauxn;
       if (ntph == 14) {
         enrg = b * (1.0 + 0.5 * (8.0 * c4 - 8.0 * c2 + 1.0));
@@ -151,13 +153,22 @@ __global__ void __launch_bounds__(256) k_tors_terms(const TorsDesc *__restrict__
         dauxn = b * sin60 * 2.0 * (soa + dsoa * coa);
         qforce = 2.0 * coa * b;
       }
-      enrg = sig * (enrg + auxn);
-      qforce = sig * (-(qforce + dauxn));
+      if (d.rem_variant) {  // fptors_rem.f:278-281, 297-300: no sign factor
+        enrg = enrg + auxn;
+        qforce = -(qforce + dauxn);
+      } else {
+        enrg = sig * (enrg + auxn);
+        qforce = sig * (-(qforce + dauxn));
+      }
       break;
     }
     default: d.status[1] = 2; return;  // the reference stops (fptors.f:291-297)
   }
   te[type - 1] = enrg;
+  if (d.remf && d.remf[i]) {  // fptors_rem.f:310-313, fitors_rem.f:206-209, 255-261
+    d.tseg[i] = enrg;
+    qforce = qforce * d.remf2;
+  }
   // fptors.f:152-183 (angle-cosine derivatives) and 299-322 (force)
   const double r21[3] = {x21, y21, z21}, r32[3] = {x32, y32, z32}, r43[3] = {x43, y43, z43};
 #pragma unroll
@@ -302,13 +313,16 @@ __global__ void __launch_bounds__(256) k_tors_gather(const TorsDesc *__restrict_
   d.fz[a] = sz;
 }
 
-// d.out[k] = sum over torsions of te[k], one block per column, fixed order
+// d.out[k] = sum over torsions of te[k] (k = 2: of tseg), one block per column, fixed order
 __global__ void __launch_bounds__(256) k_tors_energy(const TorsDesc *__restrict__ D) {
   __shared__ double red[4];
   const TorsDesc &d = *D;
   const int k = blockIdx.x;
   double s = 0.0;
-  for (int i = threadIdx.x; i < d.ntors; i += blockDim.x) s += d.te[2 * (size_t)i + k];
+  if (k < 2)
+    for (int i = threadIdx.x; i < d.ntors; i += blockDim.x) s += d.te[2 * (size_t)i + k];
+  else
+    for (int i = threadIdx.x; i < d.ntors; i += blockDim.x) s += d.tseg[i];
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -324,7 +338,7 @@ void launch_tors(const TorsDesc *d_desc, const TorsDesc &h, hipStream_t st) {
   }
   if (h.ntors > 0 && h.natoms > 0)
     hipLaunchKernelGGL(k_tors_gather, dim3((h.natoms + 255) / 256), dim3(256), 0, st, d_desc);
-  hipLaunchKernelGGL(k_tors_energy, dim3(2), dim3(256), 0, st, d_desc);
+  hipLaunchKernelGGL(k_tors_energy, dim3(h.tseg ? 3 : 2), dim3(256), 0, st, d_desc);
 }
 
 }  // namespace orac

@@ -18,7 +18,8 @@
 
 namespace orac {
 
-enum Stage { ST_BEGIN, ST_CENTRES, ST_LIST, ST_PAIRS, ST_PME, ST_END, ST_KBEG, ST_KEND, NSTAGE };
+enum Stage { ST_BEGIN, ST_CENTRES, ST_LIST, ST_PAIRS, ST_PME, ST_END, ST_KBEG, ST_KEND, ST_P0, ST_P1, ST_P2, ST_P3,
+             NSTAGE };
 constexpr int NSTAGE_MS = 5;  // centres, list, pairs, pme, corrections; then the pair kernel alone
 
 struct System {
@@ -67,6 +68,19 @@ struct System {
   MdDesc md;
   MdDesc *md_d = nullptr;  // device copy for per-system kernels
 };
+
+// Neighbour-side force slots of the half-list form: at least `need`, in three
+// FP64 arrays (grown, never shrunk; the descriptors follow through gen).
+static void size_half_slots(System &S, long long need) {
+  if (need <= S.list.pcap) return;
+  const size_t cap = (size_t)need;
+  S.pair.jpx = S.B.get<double>("jpx", cap);
+  S.pair.jpy = S.B.get<double>("jpy", cap);
+  S.pair.jpz = S.B.get<double>("jpz", cap);
+  S.list.pcap = need;
+  S.pair.pcap = need;
+  S.gen++;
+}
 
 struct Batch {
   std::vector<int> ids;
@@ -383,6 +397,7 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   p.sgi = B.get<int4>("sgi", ng);
   p.sap = B.get<double4>("sap", n);
   if (d.precision == 1) p.sapf = B.get<float4>("sapf", n);
+  else p.sapc = B.get<double4>("sapc", n);
   p.sai = B.get<int2>("sai", n);
   p.out = S.out + 20;
   p.max_group = S.max_group;
@@ -440,6 +455,26 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   p.nb_slab = L.slab;
   p.perm = L.cell_items;  // cell order of the last list build
   p.soff = L.soff;
+  // half-list form: groups of at most OWN_REG atoms, one wave per owner, non-alchemical
+  // (set_alchemy switches back)
+  if (use_owner_kernel() && S.max_group <= OWN_REG && p.split == 1) {
+    L.split = 1;
+    L.rout = B.get<int>("nbrout", S.list_cap);
+    L.mirror = B.get<int>("nbmirror", S.list_cap);
+    L.rcount = B.get<int>("nbrcount", ng);
+    L.rbase = B.get<int>("nbrbase", ng + 1);
+    ORAC_HIP(hipMemsetAsync(L.rcount, 0, sizeof(int) * ng, st));
+    ORAC_HIP(hipMemsetAsync(L.rbase, 0, sizeof(int) * (ng + 1), st));
+    // neighbour-side force slots: (list entries / 2) x atoms per group, with room to grow (size_list)
+    const double dens = ng / S.volume, rl = d.rcut + d.rtol + d.rnei;
+    const double pairs = 0.5 * ng * (4.0 / 3.0 * M_PI * rl * rl * rl * dens);
+    size_half_slots(S, (long long)(1.5 * pairs * S.max_group) + 4096);
+    p.half = 1;
+    p.nb_mirror = L.mirror;
+    p.nb_rcount = L.rcount;
+    p.nb_rbase = L.rbase;
+    p.part2 = B.get<double>("ppart2", (size_t)p.nblk * 9);
+  }
 
   // ---- reciprocal space
   build_pme_plan(B, "", S.plan, d.nfft1, d.nfft2, d.nfft3, d.order, st);
@@ -603,10 +638,11 @@ static void batch_enqueue(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool events
     if (events) ORAC_HIP(hipEventRecord(b.ev[k], st));
   };
   auto pme = [&](hipStream_t s) {
+    const hipEvent_t *pev = events ? &b.ev[ST_P0] : nullptr;
     if (b.precision == 0)
-      launch_pme<double>(b.dp, b.hp.data(), npme, s, b.tu);
+      launch_pme<double>(b.dp, b.hp.data(), npme, s, b.tu, pev);
     else
-      launch_pme<float>(b.dp, b.hp.data(), npme, s, b.tu);
+      launch_pme<float>(b.dp, b.hp.data(), npme, s, b.tu, pev);
   };
   mark(ST_BEGIN);
   launch_centres(b.dq, b.hq.data(), n, st);
@@ -713,8 +749,15 @@ static void size_list(orac_hip_ctx *ctx, int32_t id) {
     S.list.slab = need;
     S.pair.nb_idx = S.list.out;
     S.pair.nb_slab = need;
+    if (S.list.split) {
+      S.list.rout = S.B.get<int>("nbrout", S.list_cap);
+      S.list.mirror = S.B.get<int>("nbmirror", S.list_cap);
+      S.pair.nb_mirror = S.list.mirror;
+    }
     S.gen++;
   }
+  // the count pass counted both directions: slots for half of them, each group pair's upper-group atoms
+  if (S.list.split) size_half_slots(S, (long long)(0.75 * tot[0] * S.max_group) + 4096);
 }
 
 // Systems of b whose neighbour list overflowed its slab at some build since
@@ -1088,6 +1131,9 @@ static void stage_ms(Batch &b, float *ms, int32_t n) {
   ORAC_HIP(hipEventSynchronize(b.ev[ST_END]));
   for (int k = 0; k < n && k < NSTAGE_MS; ++k) ORAC_HIP(hipEventElapsedTime(&ms[k], b.ev[k], b.ev[k + 1]));
   if (n > NSTAGE_MS) ORAC_HIP(hipEventElapsedTime(&ms[NSTAGE_MS], b.ev[ST_KBEG], b.ev[ST_KEND]));
+  // the reciprocal sum's spread chain, FFT stage (both transforms and the convolution) and gather
+  for (int k = 0; k < 3 && NSTAGE_MS + 1 + k < n; ++k)
+    ORAC_HIP(hipEventElapsedTime(&ms[NSTAGE_MS + 1 + k], b.ev[ST_P0 + k], b.ev[ST_P1 + k]));
 }
 
 extern "C" int orac_hip_system_stage_ms(orac_hip_ctx *ctx, int32_t sys, float *ms, int32_t n) {
@@ -1202,6 +1248,10 @@ extern "C" int orac_hip_system_set_alchemy(orac_hip_ctx *ctx, int32_t sys, const
     S.qsoft = S.shifted ? a->qshift : a->qsoft;
     S.ljsoft = S.shifted ? a->ljshift : a->ljsoft;
     S.pair.alch_shifted = S.shifted ? 1 : 0;
+    // the soft-core pairs walk the full list (k_pairs): leave the split layout
+    S.list.split = 0;
+    S.pair.half = 0;
+    S.have_list = false;
     S.pair.type = B.put_vec("atype", A.cls, st, false);
     S.pair.ntypes = A.ncls;
     S.pair.alch_pair = B.put_vec("apair", A.pair, st, false);

@@ -26,7 +26,10 @@ struct Tuning {
 
 void pme_set_tiles(PmeDesc &d, int lds_elems);
 template <typename T>
-void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st, const Tuning &tu);
+// ev (optional, 4 events): recorded before the spread, before the forward FFT,
+// before the gather and after it (the stage timings of a profiled evaluation)
+void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st, const Tuning &tu,
+                const hipEvent_t *ev = nullptr);
 // fx_dst += fx (and y, z) for every descriptor that has a destination
 void launch_pme_combine(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st);
 // tile edge of the LDS-tiled spread along an axis of K points (0: use the atomic spread)
@@ -83,6 +86,25 @@ struct PairDesc {
   int4 *sgi;                        // [ngrp] {sorted first atom, first atom, natoms | gss << 8, group}
   double4 *sap;                     // [natoms] {xp, yp, zp, q}
   float4 *sapf;                     // [natoms] {Cartesian offset from the group centre, q}: mixed-precision pairs
+  double4 *sapc;                    // [natoms] the same in FP64 (FP64 pairs of the owner-atom loop form)
+  // Half-list form (half = 1, split list layout): the list holds each group
+  // pair once (forward entries, upper group l > owner); the forces on the
+  // neighbour's atoms go to per-pair slots, mirror[t * nb_slab + e] the first
+  // slot of forward entry e, and each group sums its reverse entries' slots
+  // (nb_rcount, nb_rbase) in the order of its sorted reverse list.
+  int half;
+  const int *nb_mirror;
+  const int *nb_rcount, *nb_rbase;
+  double *jpx, *jpy, *jpz;          // neighbour-side force slots
+  long long pcap;                   //   (pcap of them)
+  double *part2;                    // [nblk][9] molecular virial of the total forces (half form)
+  // REM segment scaling (force_REM.f): rematom[a] != 0 marks the segment's
+  // atoms (original numbering); the pairs rem_kind picks (0 either atom, 1
+  // both, 2 exactly one, rem.f90:631-645) have their force scaled by rem_f3
+  // and their energy summed in out[15]; null: no REM
+  const int *rematom;
+  int rem_kind;
+  double rem_f3;
   const float2 *lj_dense32;         // the dense LJ table in FP32 (mixed-precision pairs), or null
   int2 *sai;                        // [natoms] {type, flags}; flags bit 0 = ex_inter
   double *out;                      // [16]: uconf[3], ucoul[3] by typeij, st[9] row-major, dwrk
@@ -98,6 +120,9 @@ struct PairDesc {
 };
 
 constexpr int PAIR_WAVES = 4;       // owner groups per 256-thread block
+constexpr int OWN_REG = 3;          // largest owner group of the owner-atom loop form (k_pairs_own)
+// whether the owner-atom loop form (and with it the half-list layout) is in use
+bool use_owner_kernel();
 constexpr int MOL_SMALL = 16;       // molecules up to this size get one thread in k_mol_com
 // LJ parameters as a dense ntypes x ntypes table of {c12, c6} (one gather per
 // pair) up to this many types (64 MB); beyond it the packed table and
@@ -147,6 +172,14 @@ struct ListDesc {
   // mode 0 only.  slab > 0: one pass writes owner o's entries at out[o * slab]
   // and its count at count[o]; a count above slab sets the overflow flag.
   int slab;
+  // split slab layout (split = 1): out / count hold the forward entries (group
+  // l > owner) in walk order; rout[o * slab] the reverse entries (l < owner)
+  // as ascending group numbers, rcount[o] of them; rbase[o] the first per-pair slot of
+  // o's reverse entries (owner-major, atom-minor); mirror[o * slab + e] the
+  // slot of forward entry e; pcap the slots available
+  int split;
+  int *rout, *rcount, *rbase, *mirror;
+  long long pcap;
 };
 enum ListPhase { LIST_COUNT = 1, LIST_WRITE = 2 };
 // LIST_COUNT: cells, per-group counts, offsets and the capacity check;
@@ -204,6 +237,10 @@ struct CorrDesc {
   const double *lq, *lq0;             // per atom lambdaq, lambdaq0
   const double *lam, *lam0;           // per atom lambda, lambda0 (kind 4)
   int intramol;                       // kind 3: INTRAMOL (skip the facQ = 1 pairs)
+  // fnb14_rem.f (kind 2): remf[p] != 0 puts the pair's energy in column 4
+  // (ncol = 5) and scales its force by remf2
+  const int *remf;
+  double remf2;
 };
 void launch_corr(const CorrDesc *d_desc, const CorrDesc *h, int nsys, hipStream_t st);
 
@@ -223,7 +260,14 @@ struct TorsDesc {
   int *status;                        // [0]: 1 singular torsion; [1]: 2 unsupported multiplicity (sticky words)
   const int *g_off, *g_slot;          // per atom gather CSR: 4 * torsion + position
   double *fx, *fy, *fz;               // accumulated (+=)
-  double *out;                        // [2]: utors slt, slv
+  double *out;                        // [2]: utors slt, slv ([3] with the segment energy under REM)
+  // REM twins (fptors_rem.f, fitors_rem.f): remf[i] != 0 puts term i's energy
+  // in tseg[i] (summed to out[2]) and scales its force by remf2; rem_variant
+  // selects fptors_rem's 60-degree-phase forms (no sign factor, dsoa guard)
+  const int *remf;
+  double remf2;
+  int rem_variant;
+  double *tseg;
 };
 void launch_tors(const TorsDesc *d_desc, const TorsDesc &h, hipStream_t st);
 

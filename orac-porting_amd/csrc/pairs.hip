@@ -395,6 +395,7 @@ __global__ void __launch_bounds__(256) k_list_write(const ListDesc *__restrict__
     const int nv = list_walk_wave<true>(o, d, d.out + w + 1, 0);
     if ((threadIdx.x & 63) == 0) d.out[w] = nv;
   } else if (LAYOUT == LW_SLAB) {
+    if (d.split) return;  // k_list_write_split
     const int n = d.cellorder ? d.cell_items[o] : o;
     const int c = list_walk_wave<true>(n, d, d.out + (size_t)o * d.slab, d.cellorder ? 1 : 2, d.slab);
     if ((threadIdx.x & 63) == 0) d.count[o] = c;
@@ -404,9 +405,203 @@ __global__ void __launch_bounds__(256) k_list_write(const ListDesc *__restrict__
   }
 }
 
+// ---------------------------------------------------------------- split slab layout
+//
+// The stencil walk of a full list, written as two lists per owner n: the
+// forward entries (group l > n) to out[o * slab] in walk order, and the
+// reverse entries (l < n) to rout[o * slab] as group numbers, ascending.  The pair
+// kernel's half-list form evaluates each group pair once, from the forward
+// list of its lower group; the forces on the upper group's atoms go to
+// per-pair slots that the upper group sums in the order of its sorted
+// reverse list (mirror[] gives each forward entry its slot).
+__device__ inline void split_put(int *outf, int *lrev, int cap, bool fwd, int posf, int posr, int v) {
+  if (fwd) {
+    if (posf < cap) outf[posf] = v;
+  } else {
+    if (posr < cap) lrev[posr] = v;
+  }
+}
+
+// returns {forward, reverse} counts; lrev (LDS) receives the reverse entries in walk order
+__device__ inline int2 list_walk_split(int n, const ListDesc &d, int *outf, int *lrev, int cap, int entry) {
+  const CellGrid &cg = d.cg;
+  const int lane = threadIdx.x & 63;
+  const double x1 = gmem(d.xpg)[n], y1 = gmem(d.ypg)[n], z1 = gmem(d.zpg)[n];
+  const int c = gmem(d.cell_of)[n];
+  if (c >= cg.ncx * cg.ncy * cg.ncz) return make_int2(0, 0);  // parked (non-finite centre)
+  const int i = c / (cg.ncy * cg.ncz), j = (c / cg.ncz) % cg.ncy, k = c % cg.ncz;
+  int basef = 0, baser = 0;
+  if (cg.nsten < 64) {
+    for (int m = 0; m < cg.nsten; ++m) {
+      const int nx = fmodp(i + gmem(cg.sten)[3 * m], cg.ncx);
+      const int ny = fmodp(j + gmem(cg.sten)[3 * m + 1], cg.ncy);
+      const int nz = fmodp(k + gmem(cg.sten)[3 * m + 2], cg.ncz);
+      const int cc = nz + cg.ncz * (ny + cg.ncy * nx);
+      const int s = gmem(d.cell_start)[cc], e = s + gmem(d.cell_cnt)[cc];
+      for (int t0 = s; t0 < e; t0 += 64) {
+        const int t = t0 + lane;
+        bool hit = false;
+        int l = 0;
+        if (t < e) {
+          const double4 p = gload(cg.cpos + t);
+          l = (int)p.w;
+          hit = l != n && list_hit(x1, y1, z1, p, cg.co, cg.sqcut);
+        }
+        const bool f = hit && l > n, r = hit && l < n;
+        const unsigned long long mf = __ballot(f), mr = __ballot(r);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        // only hits are written (a lane past the cell, t >= e, holds no entry)
+        if (f || r)
+          split_put(outf, lrev, cap, f, basef + __popcll(mf & below), baser + __popcll(mr & below),
+                          entry == 1 ? t : l);
+        basef += __popcll(mf);
+        baser += __popcll(mr);
+      }
+    }
+    return make_int2(basef, baser);
+  }
+  for (int m0 = 0; m0 < cg.nsten; m0 += 64) {
+    const int m = m0 + lane;
+    int s = 0, e = 0;
+    if (m < cg.nsten) {
+      int nx = fmodp(i + gmem(cg.sten)[3 * m], cg.ncx);
+      int ny = fmodp(j + gmem(cg.sten)[3 * m + 1], cg.ncy);
+      int nz = fmodp(k + gmem(cg.sten)[3 * m + 2], cg.ncz);
+      int cc = nz + cg.ncz * (ny + cg.ncy * nx);
+      s = gmem(d.cell_start)[cc];
+      e = s + gmem(d.cell_cnt)[cc];
+    }
+    // this lane's cell: forward and reverse hits as bit masks over its (at most 64) first members
+    int cf = 0, cr = 0;
+    unsigned long long hf = 0ull, hr = 0ull;
+    for (int t = s; t < e; ++t) {
+      const double4 p = gload(cg.cpos + t);
+      const int l = (int)p.w;
+      if (l == n || !list_hit(x1, y1, z1, p, cg.co, cg.sqcut)) continue;
+      if (l > n) {
+        ++cf;
+        if (t - s < 64) hf |= 1ull << (t - s);
+      } else {
+        ++cr;
+        if (t - s < 64) hr |= 1ull << (t - s);
+      }
+    }
+    int pf = basef + wave_excl_scan_int(cf, lane), pr = baser + wave_excl_scan_int(cr, lane);
+    for (int t = s; t < e; ++t) {
+      const int r = t - s;
+      bool f, b;
+      int l;
+      if (r < 64) {
+        f = (hf >> r) & 1ull;
+        b = (hr >> r) & 1ull;
+        l = ((f || b) && entry != 1) ? (int)gload(cg.cpos + t).w : 0;
+      } else {
+        const double4 p = gload(cg.cpos + t);
+        l = (int)p.w;
+        const bool hit = l != n && list_hit(x1, y1, z1, p, cg.co, cg.sqcut);
+        f = hit && l > n;
+        b = hit && l < n;
+      }
+      if (f || b) {
+        split_put(outf, lrev, cap, f, pf, pr, entry == 1 ? t : l);
+        if (f) ++pf;
+        else ++pr;
+      }
+    }
+    basef += wave_sum_int(cf);
+    baser += wave_sum_int(cr);
+  }
+  return make_int2(basef, baser);
+}
+
+// Split slab layout: the walk, then the reverse entries ranked in LDS by group
+// number (distinct) and stored in ascending order, as group numbers.
+// Dynamic LDS: one slab of ints per wavefront.
+__global__ void __launch_bounds__(256) k_list_write_split(const ListDesc *__restrict__ D) {
+  extern __shared__ int lrev_all[];
+  const ListDesc &d = D[blockIdx.y];
+  const int o = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (!d.split || o >= d.ngrp || d.total[1] || !list_due(d)) return;
+  int *lrev = lrev_all + (threadIdx.x >> 6) * d.slab;
+  const int n = d.cellorder ? d.cell_items[o] : o;
+  const int2 c = list_walk_split(n, d, d.out + (size_t)o * d.slab, lrev, d.slab, d.cellorder ? 1 : 2);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // the reverse entries as group numbers, ranked: the order (and with it the
+  // summation order of the neighbour-side forces) is the same in cell order and in group order
+  const int nr = min(c.y, d.slab);
+  if (d.cellorder)
+    for (int i = lane; i < nr; i += 64) {
+      const int v = lrev[i];  // a cell-order position, below ngrp
+      lrev[i] = (v >= 0 && v < d.ngrp) ? gmem(d.cell_items)[v] : -1;
+    }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int *dst = d.rout + (size_t)o * d.slab;
+  for (int i = lane; i < nr; i += 64) {
+    const int v = lrev[i];
+    int rank = 0;
+    for (int q = 0; q < nr; ++q) rank += lrev[q] < v ? 1 : 0;
+    dst[rank] = v;
+  }
+  if (lane == 0) {
+    d.count[o] = c.x;
+    d.rcount[o] = c.y;
+  }
+}
+
+// entry value -> group size (entries are cell-order positions or group numbers)
+__device__ inline int entry_size(const ListDesc &d, int v) {
+  return gmem(d.gsize)[d.cellorder ? gmem(d.cell_items)[v] : v];
+}
+
+// one block per system: first partial slot of each owner's reverse entries
+// (owner-major, atom-minor, in reverse-list order); capacity check
+__global__ void __launch_bounds__(1024) k_list_rbase(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.x];
+  if (!d.split || !list_due(d) || d.total[1]) return;
+  const int n = d.ngrp;
+  // rbase[o] = rcount[o] * size(o) first, scanned in place
+  for (int o = threadIdx.x; o < n; o += blockDim.x) d.rbase[o] = min(d.rcount[o], d.slab) * entry_size(d, o);
+  __syncthreads();
+  const int tot = block_exclusive_scan(d.rbase, d.rbase, n);
+  if (threadIdx.x == 0 && (long long)tot > d.pcap) {
+    d.total[1] = 1;
+    d.total[3] = 1;  // sticky: cleared only by the host
+  }
+}
+
+// one wavefront per owner: each forward entry's partial slot, by a binary
+// search for the owner in the neighbour's sorted reverse list
+__global__ void __launch_bounds__(256) k_list_mirror(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.y];
+  const int o = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (!d.split || o >= d.ngrp || !list_due(d)) return;
+  const int nf = min(gmem(d.count)[o], d.slab);
+  const bool over = d.total[1] != 0;
+  for (int e = lane; e < nf; e += 64) {
+    const int v = gmem(d.out)[(size_t)o * d.slab + e];
+    int slot = -1;
+    if (!over) {
+      const int key = d.cellorder ? gmem(d.cell_items)[o] : o;  // the owner's group number
+      const int *rl = d.rout + (size_t)v * d.slab;
+      int lo = 0, hi = min(gmem(d.rcount)[v], d.slab);
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (gmem(rl)[mid] < key) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < min(gmem(d.rcount)[v], d.slab) && gmem(rl)[lo] == key) slot = gmem(d.rbase)[v] + lo * entry_size(d, v);
+    }
+    gmem(d.mirror)[(size_t)o * d.slab + e] = slot;
+  }
+}
+
 // slab layout, one block per system: total and largest count (fixed order),
 // overflow when a count exceeds the slab
-__global__ void __launch_bounds__(1024) k_list_total(const ListDesc *__restrict__ D) {
+__global__ void __launch_bounds__(1024) k_list_total(const ListDesc *__restrict__ D, int split_written) {
   __shared__ int wmax[16];
   const ListDesc &d = D[blockIdx.x];
   if (!list_due(d)) return;
@@ -416,9 +611,10 @@ __global__ void __launch_bounds__(1024) k_list_total(const ListDesc *__restrict_
   int tot = 0, mx = 0, s = 0;
   if (d.slab > 0) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const int c = d.count[i];
-      s += c;
-      mx = max(mx, c);
+      // after a split write: forward and reverse entries (both directions), the larger part against the slab
+      const int c = d.count[i], r = (split_written && d.split) ? d.rcount[i] : 0;
+      s += c + r;
+      mx = max(mx, max(c, r));
     }
   } else {
     tot = block_exclusive_scan(d.count, d.off, n);
@@ -455,10 +651,20 @@ void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int 
     max_grp = std::max(max_grp, h[s].ngrp);
   }
   const dim3 b(256), gg((max_grp + 255) / 256, nsys), gw((max_grp + 3) / 4, nsys);
-  // a batch uses one layout: all slab (resident systems) or none (plain lc_list)
+  // a batch uses one layout: all slab (resident systems) or none (plain lc_list);
+  // slab systems may mix the split and the plain slab layouts
   const bool slab = h[0].slab > 0;
-  for (int s = 1; s < nsys; ++s)
+  bool any_split = false, any_plain = false;
+  int max_slab = 0;
+  for (int s = 0; s < nsys; ++s) {
     if ((h[s].slab > 0) != slab || h[s].mode != h[0].mode) throw Error(E_STATE, "list layouts differ within a batch");
+    if (h[s].split) {
+      any_split = true;
+      max_slab = std::max(max_slab, h[s].slab);
+    } else {
+      any_plain = true;
+    }
+  }
   const int layout = h[0].mode == 1 ? LW_REF : (slab ? LW_SLAB : LW_CSR);
   if (phases & LIST_COUNT) {
     hipLaunchKernelGGL(k_list_zero, dim3(std::min(64, (max_cell + 255) / 256), nsys), b, 0, st, d_desc);
@@ -472,7 +678,7 @@ void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int 
       // counts only (or the compact layout's first pass)
       hipLaunchKernelGGL(k_list_count, gw, b, 0, st, d_desc);
       if (slab)
-        hipLaunchKernelGGL(k_list_total, dim3(nsys), dim3(1024), 0, st, d_desc);
+        hipLaunchKernelGGL(k_list_total, dim3(nsys), dim3(1024), 0, st, d_desc, 0);
       else
         hipLaunchKernelGGL(k_list_offsets, dim3(nsys), dim3(1024), 0, st, d_desc);
     }
@@ -480,11 +686,21 @@ void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int 
   if (phases & LIST_WRITE) {
     if (layout == LW_REF)
       hipLaunchKernelGGL(k_list_write<LW_REF>, gw, b, 0, st, d_desc);
-    else if (layout == LW_SLAB)
-      hipLaunchKernelGGL(k_list_write<LW_SLAB>, gw, b, 0, st, d_desc);
-    else
+    else if (layout == LW_SLAB) {
+      if (any_plain) hipLaunchKernelGGL(k_list_write<LW_SLAB>, gw, b, 0, st, d_desc);
+      if (any_split) {
+        const size_t shm = (size_t)4 * max_slab * sizeof(int);
+        if (shm > 65536) throw Error(E_CAPACITY, "neighbour-list slab too large for the split layout's LDS sort");
+        hipLaunchKernelGGL(k_list_write_split, gw, b, shm, st, d_desc);
+      }
+    } else {
       hipLaunchKernelGGL(k_list_write<LW_CSR>, gw, b, 0, st, d_desc);
-    if (slab) hipLaunchKernelGGL(k_list_total, dim3(nsys), dim3(1024), 0, st, d_desc);
+    }
+    if (slab) hipLaunchKernelGGL(k_list_total, dim3(nsys), dim3(1024), 0, st, d_desc, any_split ? 1 : 0);
+    if (any_split) {
+      hipLaunchKernelGGL(k_list_rbase, dim3(nsys), dim3(1024), 0, st, d_desc);
+      hipLaunchKernelGGL(k_list_mirror, gw, b, 0, st, d_desc);
+    }
   }
 }
 
@@ -750,6 +966,10 @@ __global__ void __launch_bounds__(256) k_corr_pairs(const CorrDesc *__restrict__
       const double ecoul = d.fudge * d.q[ia] * d.q[ib] / rsp;
       ev[c] = ecoul;
       qforce = d.lj_fudge * ssvir * rsqi + ecoul * rsqi;
+    }
+    if (d.remf && d.remf[p]) {  // fnb14_rem.f:146-149, 209-212
+      ev[4] = ev[2 + c] + ev[c];
+      qforce = qforce * d.remf2;
     }
   } else if (kind == 4) {
     // fnb14_alchemy.f:120-356, with the reference's own arithmetic: without

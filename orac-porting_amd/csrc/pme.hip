@@ -286,27 +286,38 @@ __global__ void __launch_bounds__(256) k_bin_fill(const PmeDesc *__restrict__ D)
 
 constexpr int SPREAD_TILE_MAX = 8;
 
-// Stencil points i = 0..order-1 at tile-local offset (off + i) mod K: returns
-// (off mod K) and the range [ia, ib) of i whose point lies in [0, st).
-__device__ inline int tile_range(int off, int K, int st, int order, int &ia, int &ib) {
-  const int l0 = ((off % K) + K) % K;
-  if (st >= K) {  // one tile along the axis: every point, wrapped or not
-    ia = 0;
-    ib = order;
-  } else if (l0 < st) {  // (with two or more tiles, st >= order - 1 keeps wrapped points out)
-    ia = 0;
-    ib = min(order, st - l0);
+// The tile-local range of an atom's stencil along one axis.  The stencil
+// points are base + i, i = 0..order-1, base = b - order + 1 (+ K when the atom
+// comes from the wrapped neighbour bin) minus the tile origin.  With two or
+// more tiles along the axis (st >= order - 1) the points inside the tile are
+// one range [ia, ib) and none of them wraps; with one tile (st = K) every
+// point is inside after a wrap of the negative ones.
+struct AxisRange {
+  int l0, ia, ib, wrap;
+};
+__device__ inline AxisRange axis_range(int b, int order, int t0, int st, int K, int wrapped) {
+  AxisRange r;
+  r.l0 = b - order + 1 + (wrapped ? K : 0) - t0;
+  if (st >= K) {
+    r.ia = 0;
+    r.ib = order;
+    r.wrap = K;
   } else {
-    ia = min(order, K - l0);  // the first point that wraps to 0
-    ib = min(order, K - l0 + st);
+    r.ia = max(0, -r.l0);
+    r.ib = min(order, st - r.l0);
+    r.wrap = 0;
   }
-  return l0;
+  return r;
 }
 
 // The tile is held in LDS with padded strides (st1 + 1, st2 + 1): z-planes and
 // y-rows of one column then fall on different banks, which the int64 atomics
-// of neighbouring stencil points would otherwise serialise on.
-constexpr int SPREAD_THREADS = 512;  // per tile: ~2 work items per thread at 8^3 tiles, order 5
+// of neighbouring stencil points would otherwise serialise on.  Threads take
+// (atom, z-plane) pairs, 8 planes per atom (order <= 8), 64 atoms per sweep;
+// a pair whose plane lies outside the tile stops at once, the others add
+// their in-tile rows.  Local indices need no modulo: the bin offset tells
+// which atoms come across the periodic boundary.
+constexpr int SPREAD_THREADS = 512;  // 64 atoms x 8 z-planes per sweep
 
 template <typename T>
 __global__ void __launch_bounds__(SPREAD_THREADS) k_spread_tile(const PmeDesc *__restrict__ D) {
@@ -321,36 +332,35 @@ __global__ void __launch_bounds__(SPREAD_THREADS) k_spread_tile(const PmeDesc *_
   for (int i = threadIdx.x; i < P1 * P2 * st3; i += blockDim.x) tile[i] = 0ull;
   __syncthreads();
   const double scale = d.fixscale;
+  const int slot = threadIdx.x >> 3, i3 = threadIdx.x & 7;
   for (int sb = 0; sb < 8; ++sb) {
-    const int B1 = (T1 + (sb & 1)) % d.nb1, B2 = (T2 + ((sb >> 1) & 1)) % d.nb2, B3 = (T3 + (sb >> 2)) % d.nb3;
+    const int o1 = sb & 1, o2 = (sb >> 1) & 1, o3 = sb >> 2;
     // with a single bin along an axis, the +1 neighbour is the same bin: visit it once
-    if (((sb & 1) && d.nb1 == 1) || (((sb >> 1) & 1) && d.nb2 == 1) || ((sb >> 2) && d.nb3 == 1)) continue;
+    if ((o1 && d.nb1 == 1) || (o2 && d.nb2 == 1) || (o3 && d.nb3 == 1)) continue;
+    const int w1 = T1 + o1 >= d.nb1, w2 = T2 + o2 >= d.nb2, w3 = T3 + o3 >= d.nb3;
+    const int B1 = T1 + o1 - (w1 ? d.nb1 : 0), B2 = T2 + o2 - (w2 ? d.nb2 : 0), B3 = T3 + o3 - (w3 ? d.nb3 : 0);
     const int bin = B1 + d.nb1 * (B2 + d.nb2 * B3);
     const int s = d.bin_start[bin], n = d.bin_cnt[bin];
-    for (int w = threadIdx.x; w < n * order; w += blockDim.x) {
-      const int at = w / order, i3 = w - at * order;
+    for (int at = slot; at < n; at += SPREAD_THREADS / 8) {
+      if (i3 >= order) continue;
       const double4 f = d.sfr[s + at];
-      const int b1 = (int)f.x, b2 = (int)f.y, b3 = (int)f.z;
-      // tile-local z of this stencil plane
-      const int kk = b3 - order + i3 + 1;
-      const int lz = ((kk - t03) % d.K3 + d.K3) % d.K3;
-      if (lz >= st3) continue;
+      const AxisRange z = axis_range((int)f.z, order, t03, st3, d.K3, w3);
+      int lz = z.l0 + i3;
+      if (lz < 0) lz += z.wrap;
+      if (lz < 0 || lz >= st3) continue;
+      const AxisRange x = axis_range((int)f.x, order, t01, st1, d.K1, w1);
+      const AxisRange y = axis_range((int)f.y, order, t02, st2, d.K2, w2);
       const double *wt = d.swt + (size_t)(s + at) * 3 * order;
       const T q3 = T(wt[2 * order + i3]) * T(f.w);
-      // tile-local x and y of the first stencil point; the next ones step by
-      // one and wrap once (order <= K), so the points inside the tile are one
-      // contiguous range of i1 (and of i2)
-      int i1a, i1b, i2a, i2b;
-      const int lx0 = tile_range(b1 - order + 1 - t01, d.K1, st1, order, i1a, i1b);
-      const int ly0 = tile_range(b2 - order + 1 - t02, d.K2, st2, order, i2a, i2b);
-      for (int i2 = i2a; i2 < i2b; ++i2) {
-        int ly = ly0 + i2;
-        if (ly >= d.K2) ly -= d.K2;
+      for (int i2 = y.ia; i2 < y.ib; ++i2) {
+        int ly = y.l0 + i2;
+        if (ly < 0) ly += y.wrap;
         const T prod = T(wt[order + i2]) * q3;
-        for (int i1 = i1a; i1 < i1b; ++i1) {
-          int lx = lx0 + i1;
-          if (lx >= d.K1) lx -= d.K1;
-          atomicAdd(&tile[lx + P1 * (ly + P2 * lz)], to_fix((double)(T(wt[i1]) * prod), scale));
+        unsigned long long *row = tile + P1 * (ly + P2 * lz);
+        for (int i1 = x.ia; i1 < x.ib; ++i1) {
+          int lx = x.l0 + i1;
+          if (lx < 0) lx += x.wrap;
+          atomicAdd(&row[lx], to_fix((double)(T(wt[i1]) * prod), scale));
         }
       }
     }
@@ -746,7 +756,8 @@ void pme_set_tiles(PmeDesc &d, int lds_elems) {
 }
 
 template <typename T>
-void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st, const Tuning &tu) {
+void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t st, const Tuning &tu,
+                const hipEvent_t *ev) {
   (void)tu;
   if (nsys <= 0) return;
   int max_atoms = 1, max_bins = 1, any_tiled = 0, any_atomic = 0, max_gx = 1, max_gy = 1, max_gz = 1, max_gg = 1;
@@ -771,6 +782,7 @@ void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t s
     }
   }
   dim3 blk(256);
+  if (ev) ORAC_HIP(hipEventRecord(ev[0], st));
   if (any_tiled) {
     dim3 ga((max_atoms + 255) / 256, nsys);
     hipLaunchKernelGGL(k_bin_zero, dim3((max_bins + 255) / 256, nsys), blk, 0, st, d_desc);
@@ -784,12 +796,15 @@ void launch_pme(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipStream_t s
     hipLaunchKernelGGL(k_grid_zero, dim3(gzero, nsys), blk, 0, st, d_desc);
     hipLaunchKernelGGL(k_spread<T>, dim3((unsigned)((max_work + 255) / 256), nsys), blk, 0, st, d_desc);
   }
+  if (ev) ORAC_HIP(hipEventRecord(ev[1], st));
   hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(max_gx, nsys), blk, shx, st, d_desc);
   hipLaunchKernelGGL((k_fft_pencil<T, -1>), dim3(max_gy, nsys), blk, shy, st, d_desc, 2);
   hipLaunchKernelGGL(k_fft_z_conv<T>, dim3(max_gz, nsys), blk, shz, st, d_desc);
   hipLaunchKernelGGL((k_fft_pencil<T, +1>), dim3(max_gy, nsys), blk, shy, st, d_desc, 2);
   hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(max_gx, nsys), blk, shx, st, d_desc);
+  if (ev) ORAC_HIP(hipEventRecord(ev[2], st));
   hipLaunchKernelGGL(k_gather<T>, dim3(max_gg, nsys), blk, 0, st, d_desc);
+  if (ev) ORAC_HIP(hipEventRecord(ev[3], st));
   hipLaunchKernelGGL(k_pme_finalize, dim3(16, nsys), dim3(256), 0, st, d_desc);
 }
 
@@ -818,7 +833,9 @@ void launch_pme_combine(const PmeDesc *d_desc, const PmeDesc *h, int nsys, hipSt
   hipLaunchKernelGGL(k_pme_combine, dim3((max_atoms + 255) / 256, nsys), dim3(256), 0, st, d_desc);
 }
 
-template void launch_pme<double>(const PmeDesc *, const PmeDesc *, int, hipStream_t, const Tuning &);
-template void launch_pme<float>(const PmeDesc *, const PmeDesc *, int, hipStream_t, const Tuning &);
+template void launch_pme<double>(const PmeDesc *, const PmeDesc *, int, hipStream_t, const Tuning &,
+                                 const hipEvent_t *);
+template void launch_pme<float>(const PmeDesc *, const PmeDesc *, int, hipStream_t, const Tuning &,
+                                const hipEvent_t *);
 
 }  // namespace orac

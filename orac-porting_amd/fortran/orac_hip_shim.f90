@@ -87,6 +87,22 @@ module orac_hip_iface
       real(c_double), value :: alphal, rneigh, rinn, rout, rtolout, massinfty
       integer(c_int64_t), value :: cap
     end function
+    integer(c_int) function c_force_rem(ctx, ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, &
+         atomg, xpcm, ypcm, zpcm, groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, uconf_slv, &
+         uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, cap, rneigh, rinn, rout, &
+         rtolinn, rtolout, gmass, iz, ncount1, mass, massinfty, seg_ene, rematom, lskip_ewald, ewald, segkind, &
+         factor3, lskip_force, ljintra) bind(C, name="orac_hip_force_rem")
+      import :: c_ptr, c_int, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), nbtype(*), type(*), atomg(*), groupp(*), atomp(*), mapnl(*), grppt(*)
+      integer(c_int32_t) :: nnlpp(*), nnlpp1(*), ncount1, rematom(*)
+      integer(c_int32_t), value :: ma, nato, ngrp, iz, lskip_ewald, ewald, segkind, lskip_force, ljintra
+      real(c_double) :: xp0(*), yp0(*), zp0(*), xpg(*), ypg(*), zpg(*), charge(*), xpcm(*), ypcm(*), zpcm(*)
+      real(c_double) :: co(*), ecc12(*), ecc6(*), gmass(*), mass(*), fpx(*), fpy(*), fpz(*), stress(*)
+      real(c_double) :: uconf_slt, uconf_slv, uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss, rtolinn, seg_ene
+      real(c_double), value :: alphal, rneigh, rinn, rout, rtolout, massinfty, factor3
+      integer(c_int64_t), value :: cap
+    end function
     integer(c_int) function c_force_alchemy(ctx, ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, &
          nato, atomg, xpcm, ypcm, zpcm, groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, uconf_slt, &
          uconf_slv, uconf_ss, ucoul_slt, ucoul_slv, ucoul_ss, fpx, fpy, fpz, stress, nnlpp, nnlpp1, cap, rneigh, &
@@ -174,6 +190,36 @@ module orac_hip_iface
       integer(c_int32_t), value :: lbend, mb
       real(c_double) :: xp0(*), yp0(*), zp0(*), pota(*), potb(*), potc(*), potd(*), pote(*), potf(*), potg(*)
       real(c_double) :: ub_slt, ub_slv, fpx(*), fpy(*), fpz(*)
+    end function
+    integer(c_int) function c_fptors_rem(ctx, ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, ut_slt, ut_slv, &
+         fpx, fpy, fpz, remf, seg, f2, mb) bind(C, name="orac_hip_fptors_rem")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), ltor(*), remf(*)
+      integer(c_int32_t), value :: ltors, mb
+      real(c_double) :: xp0(*), yp0(*), zp0(*), pota(*), potb(*), ut_slt, ut_slv, fpx(*), fpy(*), fpz(*), seg
+      real(c_double), value :: f2
+    end function
+    integer(c_int) function c_fitors_rem(ctx, ss_index, litr, litor, xp0, yp0, zp0, pota, potb, potc, ut_slt, &
+         ut_slv, fpx, fpy, fpz, remf, seg, f2, mb) bind(C, name="orac_hip_fitors_rem")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), litr(*), remf(*)
+      integer(c_int32_t), value :: litor, mb
+      real(c_double) :: xp0(*), yp0(*), zp0(*), pota(*), potb(*), potc(*), ut_slt, ut_slv, fpx(*), fpy(*), fpz(*)
+      real(c_double) :: seg
+      real(c_double), value :: f2
+    end function
+    integer(c_int) function c_fnb14_rem(ctx, ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, cut, ewald, alphal, &
+         int14, int14p, type14, fudge, lj_fudge, fpx, fpy, fpz, uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, remf, &
+         seg, f2) bind(C, name="orac_hip_fnb14_rem")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: ss_index(*), int14(*), type14(*), remf(*)
+      integer(c_int32_t), value :: nato, cut, ewald, int14p
+      real(c_double), value :: alphal, fudge, lj_fudge, f2
+      real(c_double) :: xp0(*), yp0(*), zp0(*), charge(*), ecc12(*), ecc6(*), fpx(*), fpy(*), fpz(*)
+      real(c_double) :: uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, seg
     end function
     integer(c_int) function c_fptors(ctx, ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, ut_slt, ut_slv, fpx, &
          fpy, fpz, mb) bind(C, name="orac_hip_fptors")

@@ -53,6 +53,7 @@ EXPORTS = [
     "orac_hip_create", "orac_hip_destroy", "orac_hip_last_error", "orac_hip_invalidate", "orac_hip_device_name",
     "orac_hip_probe_copy",
     "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
+    "orac_hip_force_rem", "orac_hip_fptors_rem", "orac_hip_fitors_rem", "orac_hip_fnb14_rem",
     "orac_hip_force_alchemy", "orac_hip_force_alchemy_shift", "orac_hip_fptors", "orac_hip_fpbond", "orac_hip_fitors", "orac_hip_fpbend", "orac_hip_fnb14",
     "orac_hip_fft_pme_alchemy", "orac_hip_ferrf_alchemy", "orac_hip_cself_alchemy", "orac_hip_int_corr_erf_spline",
     "orac_hip_erf_corr_cutoff",
@@ -69,7 +70,7 @@ EXPORTS = [
 ]
 
 # stage_ms order of the resident evaluation
-STAGES = ["centres", "list", "pairs", "pme", "corrections", "pair_kernel"]
+STAGES = ["centres", "list", "pairs", "pme", "corrections", "pair_kernel", "pme_spread", "pme_fft", "pme_gather"]
 TUNING_FIELDS = ["lds_elems", "pair_waves", "spread_tiled", "xcd_remap", "cell_order", "persistent"]
 
 
@@ -198,10 +199,13 @@ class Context:
     # -- force.f:1
     def force(self, ss_index, xp0, yp0, zp0, xpg, ypg, zpg, charge, nbtype, type, ma, nato, atomg, xpcm, ypcm, zpcm,
               groupp, atomp, co, ecc12, ecc6, alphal, mapnl, ngrp, grppt, fpx, fpy, fpz, stress, nnlpp, nnlpp1,
-              rneigh, rinn, rout, rtolinn, rtolout, gmass, iz, mass, massinfty, lskip_ewald, ewald, alchemy=None):
+              rneigh, rinn, rout, rtolinn, rtolout, gmass, iz, mass, massinfty, lskip_ewald, ewald, alchemy=None,
+              rem=None):
         """Returns a dict with the six energies, ncount1 and the updated rtolinn.
 
         fpx/fpy/fpz and stress are accumulated in place, nnlpp1 filled.
+        rem = (rematom, rem_segkind, rem_factor3, lskip_force, ljintra) calls
+        force_rem instead (force_REM.f) and adds seg_ene to the result.
         """
         e = [C.c_double() for _ in range(6)]
         nc1 = C.c_int32()
@@ -222,7 +226,15 @@ class Context:
             C.c_double(massinfty)]
         flags = [C.c_int32(int(bool(lskip_ewald))), C.c_int32(int(bool(ewald)))]
         dw = None
-        if alchemy is None:
+        seg = None
+        if rem is not None:
+            rematom, kind, f3, lskip_force, ljintra = rem
+            ra = _i(np.asarray(rematom) != 0)
+            seg = C.c_double()
+            _check(lib().orac_hip_force_rem(*common, C.byref(seg), _p(ra), *flags, C.c_int32(int(kind)),
+                                            C.c_double(float(f3)), C.c_int32(int(bool(lskip_force))),
+                                            C.c_int32(int(bool(ljintra)))))
+        elif alchemy is None:
             _check(lib().orac_hip_force(*common, *flags))
         else:
             al, lam, dwrk = alchemy
@@ -237,6 +249,8 @@ class Context:
         names = ["uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"]
         out = {k: v.value for k, v in zip(names, e)}
         out.update(ncount1=nc1.value, rtolinn=rti.value)
+        if seg is not None:
+            out["seg_ene"] = seg.value
         if dw is not None:
             out["dwrk"] = dw.value
         return out
@@ -268,6 +282,29 @@ class Context:
         _check(lib().orac_hip_fitors(self._h, _p(keep[0]), _p(keep[1]), C.c_int32(litor), *[_p(v) for v in keep[2:]],
                                      C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), C.c_int32(mb)))
         return a.value, b.value
+
+    # -- fptors_rem.f, fitors_rem.f
+    def fptors_rem(self, ss_index, ltor, ltors, xp0, yp0, zp0, pota, potb, fpx, fpy, fpz, mb, remltor, rem_factor2):
+        """Returns (utors_slt, utors_slv, seg_ene); fpx/fpy/fpz accumulated in place."""
+        a, b, g = C.c_double(), C.c_double(), C.c_double()
+        fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
+        keep = [_i(ss_index), _i(ltor), _f(xp0), _f(yp0), _f(zp0), _f(pota), _f(potb), _i(np.asarray(remltor) != 0)]
+        _check(lib().orac_hip_fptors_rem(self._h, _p(keep[0]), _p(keep[1]), C.c_int32(ltors), *[_p(v) for v in keep[2:7]],
+                                         C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _p(keep[7]), C.byref(g),
+                                         C.c_double(rem_factor2), C.c_int32(mb)))
+        return a.value, b.value, g.value
+
+    def fitors_rem(self, ss_index, litr, litor, xp0, yp0, zp0, pota, potb, potc, fpx, fpy, fpz, mb, remitor,
+                   rem_factor2):
+        """Returns (utors_slt, utors_slv, seg_ene); fpx/fpy/fpz accumulated in place."""
+        a, b, g = C.c_double(), C.c_double(), C.c_double()
+        fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
+        keep = [_i(ss_index), _i(litr), _f(xp0), _f(yp0), _f(zp0), _f(pota), _f(potb), _f(potc),
+                _i(np.asarray(remitor) != 0)]
+        _check(lib().orac_hip_fitors_rem(self._h, _p(keep[0]), _p(keep[1]), C.c_int32(litor), *[_p(v) for v in keep[2:8]],
+                                         C.byref(a), C.byref(b), _p(fpx), _p(fpy), _p(fpz), _p(keep[8]), C.byref(g),
+                                         C.c_double(rem_factor2), C.c_int32(mb)))
+        return a.value, b.value, g.value
 
     # -- fpbond.f
     def fpbond(self, ss_index, lbnd, lbond, xp0, yp0, zp0, pota, potb, fpx, fpy, fpz, mb):
@@ -305,6 +342,24 @@ class Context:
                                     _p(keep[8]), C.c_double(fudge), C.c_double(lj_fudge), _p(fpx), _p(fpy), _p(fpz),
                                     *[C.byref(v) for v in e]))
         return tuple(v.value for v in e)
+
+    # -- fnb14_rem.f
+    def fnb14_rem(self, ss_index, xp0, yp0, zp0, charge, nato, ecc12, ecc6, cut, ewald, alphal, int14, int14p, type14,
+                  fudge, lj_fudge, fpx, fpy, fpz, remint14, rem_factor2, seg_ene=0.0):
+        """Returns (uconf_slt, uconf_slv, ucoul_slt, ucoul_slv, seg_ene); forces accumulated in place.
+        seg_ene is the incoming value, returned unchanged when there is no pair."""
+        e = [C.c_double() for _ in range(4)]
+        g = C.c_double(seg_ene)
+        fpx, fpy, fpz = (_out(v, np.float64) for v in (fpx, fpy, fpz))
+        keep = [_i(ss_index), _f(xp0), _f(yp0), _f(zp0), _f(charge), _f(ecc12), _f(ecc6), _i(int14), _i(type14),
+                _i(np.asarray(remint14) != 0)]
+        _check(lib().orac_hip_fnb14_rem(self._h, _p(keep[0]), _p(keep[1]), _p(keep[2]), _p(keep[3]), _p(keep[4]),
+                                        C.c_int32(nato), _p(keep[5]), _p(keep[6]), C.c_int32(int(bool(cut))),
+                                        C.c_int32(int(bool(ewald))), C.c_double(alphal), _p(keep[7]), C.c_int32(int14p),
+                                        _p(keep[8]), C.c_double(fudge), C.c_double(lj_fudge), _p(fpx), _p(fpy),
+                                        _p(fpz), *[C.byref(v) for v in e], _p(keep[9]), C.byref(g),
+                                        C.c_double(rem_factor2)))
+        return tuple(v.value for v in e) + (g.value,)
 
     # -- mts_forces.f:212
     @staticmethod

@@ -17,6 +17,8 @@ not matter.
 from __future__ import annotations
 
 import csv
+import hashlib
+import os
 import sys
 from collections import defaultdict
 
@@ -110,6 +112,7 @@ def traffic(fetch_csv, write_csv, kernel, probe_bytes, out, cmd_fetch, cmd_write
     ws = probe_bytes / (w_p * 1024.0)
     fetched, written = f_k * 1024.0 * fs, w_k * 1024.0 * ws
     with open(out, "w") as f:
+        f.write(f"kernel_source_sha16: {source_sha16()}\n")
         f.write(f"command (FETCH_SIZE): {cmd_fetch}\n")
         f.write(f"command (WRITE_SIZE): {cmd_write}\n")
         f.write("HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the\n")
@@ -126,6 +129,29 @@ def traffic(fetch_csv, write_csv, kernel, probe_bytes, out, cmd_fetch, cmd_write
         f.write(f"fetched_bytes_per_launch: {fetched:.0f}\n")
         f.write(f"written_bytes_per_launch: {written:.0f}\n")
         f.write(f"traffic_bytes_per_launch: {fetched + written:.0f}\n")
+
+
+PAIR_SOURCE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "csrc", "force_pairs.hip")
+
+
+def source_sha16(path=PAIR_SOURCE):
+    """First 16 hex digits of the SHA-256 of the pair kernel's source file."""
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
+def read_stamp(path):
+    """Which source a committed traffic summary was measured on, and whether
+    that is still the current pair kernel source."""
+    stamp = None
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("kernel_source_sha16:"):
+                    stamp = line.split(":", 1)[1].strip()
+    except OSError:
+        return {"profile": os.path.basename(path), "kernel_source_sha16": None, "current": False}
+    return {"profile": os.path.basename(path), "kernel_source_sha16": stamp, "current": stamp == source_sha16()}
 
 
 def read_traffic(path):

@@ -21,10 +21,12 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
 import time
 
 import numpy as np
 
+from . import formats
 from . import md as _md
 from . import synth
 from .api import Batch, ResidentSystem
@@ -148,6 +150,7 @@ class _Active:
         else:
             self.dyn.maxwell_boltzmann(req.temperature, req.seed)
         self.left = req.nsteps
+        self.e0, self.s0 = None, 0
         req.device_bytes = self.rs.device_bytes()
 
     def close(self):
@@ -168,10 +171,11 @@ class Server:
     """Continuous batching of MD segments on one device (see the module doc)."""
 
     def __init__(self, ctx, hbm_budget, max_active=64, chunk=10, dt=2.0, ninner=4, list_every=10, precision=1,
-                 autotune=False, clock=time.perf_counter):
+                 autotune=False, clock=time.perf_counter, restart_dir=None):
         self.ctx, self.budget, self.max_active, self.chunk = ctx, int(hbm_budget), max_active, chunk
         self.dt, self.ninner, self.list_every, self.precision = dt, ninner, list_every, precision
         self.autotune, self.clock = autotune, clock
+        self.restart_dir = restart_dir  # finished requests are checkpointed there in ORAC's restart format
         self.log = []
         self.progress = None  # optional callable(str): admission and completion events
 
@@ -220,12 +224,24 @@ class Server:
             done = [a for a in active if a.left == n]
             for a in active:
                 a.left -= n
+                if a.e0 is None and a.left > 0:
+                    # NVE reference point: the total energy after the request's first chunk
+                    a.e0, a.s0 = a.dyn.energies()["total"], a.req.nsteps - a.left
             if done:
                 for a in done:
                     r = a.req
                     r.final_positions, r.final_velocities = a.dyn.state()
                     r.energies = a.dyn.energies()
                     r.finished = self.clock() - t0
+                    if a.e0 is not None and r.nsteps > a.s0:
+                        ns = (r.nsteps - a.s0) * self.dt * 1e-6
+                        r.energies["drift_kj_mol_per_ns_per_atom"] = float(
+                            (r.energies["total"] - a.e0) * synth.EFACT / 1000.0 / ns / r.system.natoms)
+                    if self.restart_dir:
+                        path = os.path.join(self.restart_dir, f"request{r.rid:05d}.rst")
+                        formats.write_restart(path, formats.restart_state(r.system, r.final_positions,
+                                                                          r.final_velocities, r.nsteps,
+                                                                          r.energies["temperature"]))
                     if self.progress:
                         self.progress(f"t={r.finished:.3f}s done request {r.rid}: latency {r.latency * 1e3:.0f} ms, "
                                       f"T {r.energies['temperature']:.0f} K")
@@ -245,13 +261,22 @@ class Server:
                     peak_bytes=max((u for _, _, u in self.log), default=0))
 
 
-def relax_requests(ctx, requests, niter=300, dmax=0.03, precision=1):
-    """Steepest-descent start positions for each request (preparation, not
-    served time): the generators place atoms without regard to LJ overlap."""
+def relax_requests(ctx, requests, niter=300, dmax=0.03, precision=1, equilibrate=0):
+    """Start states for each request (preparation, not served time): the
+    generators place atoms without regard to LJ overlap, so niter
+    steepest-descent steps first; then, with equilibrate > 0, that many NVE
+    steps from Maxwell-Boltzmann velocities at the request's temperature with
+    the velocities rescaled to it every 10 steps, and the request starts from
+    the equilibrated positions and velocities."""
     for r in requests:
         a = _Active(ctx, dataclasses.replace(r, positions=None, velocities=None), precision)
         b = Batch(ctx, [a.rs])
         _md.relax_batch(b, niter, dmax=dmax)
-        r.positions, _ = a.dyn.state()
+        if equilibrate > 0:
+            a.dyn.maxwell_boltzmann(r.temperature, r.seed)
+            _md.equilibrate(b, [a.dyn], r.temperature, equilibrate)
+            r.positions, r.velocities = a.dyn.state()
+        else:
+            r.positions, _ = a.dyn.state()
         b.close()
         a.close()

@@ -15,6 +15,14 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP C ABI)")
 
 
+def pytest_runtest_logstart(nodeid, location):
+    """With ORAC_TEST_TRACE=1 each test's id goes to stderr (flushed) before it
+    runs, so a run that ends abruptly still names the test it was in."""
+    if os.environ.get("ORAC_TEST_TRACE") == "1":
+        sys.stderr.write(f"start {nodeid}\n")
+        sys.stderr.flush()
+
+
 def relrms(a, b):
     """Relative RMS error of a against reference b (the reference's check_force, pme_fftw3.f90:390-405)."""
     a, b = np.asarray(a), np.asarray(b)
