@@ -62,6 +62,20 @@ int orac_hip_fft_pme(orac_hip_ctx *ctx, int32_t ntap, const double *xp, const do
                      const double *co, const double *oc, double volume, double alphal, int32_t order,
                      int32_t nfft1, int32_t nfft2, int32_t nfft3, double *eer, double *fpx, double *fpy, double *fpz,
                      double *stressc, const int32_t *atomp, const int32_t *grppt, int32_t pressure, double rkcut);
+/* No counterpart in the reference: orac_hip_fft_pme with the same arguments
+ * and results, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every per-call device buffer lives in a static 32 MiB region per GPU, so
+ * the call allocates nothing; it needs fft_pme_init on the same context
+ * first (set-up allocates the plan) and takes at most 65,536 atoms and
+ * 64^3 grid points (status 2 beyond that).  Calls that overlap on one GPU
+ * share the region and race; the region keeps the last call's contents. */
+int orac_hip_fft_pme_noalloc(orac_hip_ctx *ctx, int32_t ntap, const double *xp, const double *yp, const double *zp,
+                             const double *xpcm, const double *ypcm, const double *zpcm, const double *pmechg,
+                             const double *co, const double *oc, double volume, double alphal, int32_t order,
+                             int32_t nfft1, int32_t nfft2, int32_t nfft3, double *eer, double *fpx, double *fpy,
+                             double *fpz, double *stressc, const int32_t *atomp, const int32_t *grppt,
+                             int32_t pressure, double rkcut);
 
 /* ---- linked cells --------------------------------------------------------- */
 /* Replaces lc_index (lc_index.f:3-125) with dist_ijk (dist_ijk.f): the

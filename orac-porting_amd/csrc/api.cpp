@@ -7,10 +7,47 @@
 // the copy last uploaded and sent again only when their bytes change.
 // Resident systems and batches are in engine.cpp.
 #include <cmath>
+#include <mutex>
 
 #include "host.hpp"
 
 using namespace orac;
+
+// ---------------------------------------------------------------- static scratch
+//
+// The device region behind orac_hip_fft_pme_noalloc.  Each GPU the library
+// runs on holds its own copy for the life of the process.  Host code never
+// uses the array's name as an address: the device address of each GPU's copy
+// is looked up once per device, at context creation with that device
+// current, and kept in g_scratch_ptr.
+namespace {
+constexpr size_t SCRATCH_BYTES = (size_t)32 << 20;  // 32 MiB
+constexpr int SCRATCH_MAX_ATOMS = 65536;
+constexpr int SCRATCH_MAX_POINTS = 64 * 64 * 64;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+__device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// device address of this device's scratch copy (the device must be current)
+void *scratch_for_device(int device) {
+  ORAC_REQUIRE(device >= 0 && device < SCRATCH_MAX_DEVICES, E_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    ORAC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ORAC_REQUIRE(p != nullptr, E_HIP, "hipGetSymbolAddress returned no address for the scratch region");
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
+// atomp (1-based, fft_pme.f) to the 0-based molecule index the gather reads
+__global__ void __launch_bounds__(256) k_atomp_to_zero_based(int *__restrict__ ap, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ap[i] -= 1;
+}
+}  // namespace
 
 // ================================================================ C ABI
 
@@ -32,6 +69,8 @@ extern "C" int orac_hip_create(orac_hip_ctx **ctx, int32_t device) {
     ORAC_HIP(hipSetDevice(device));
     ORAC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     ORAC_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    c->S.arena = static_cast<char *>(scratch_for_device(device));
+    c->S.arena_cap = SCRATCH_BYTES;
     *ctx = c.release();
   });
 }
@@ -247,6 +286,83 @@ extern "C" int orac_hip_fft_pme_alchemy(orac_hip_ctx *ctx, int32_t ntap, const d
   if (!charg0 || !dwrk) return guarded([] { ORAC_REQUIRE(false, E_ARG, "null charg0 or dwrk"); });
   return fft_pme_impl(ctx, ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, co, oc, volume, alphal, order, nfft1, nfft2,
                       nfft3, eer, fpx, fpy, fpz, stressc, atomp, grppt, pressure, rkcut, charg0, dwrk);
+}
+
+// fft_pme without allocation: every per-call device buffer is carved out of
+// this device's static scratch region (ctx->S), the forces are accumulated on
+// the device into the caller's values, and nothing is allocated on the host
+// or the device.  Calls that overlap on one GPU share the region and race.
+extern "C" int orac_hip_fft_pme_noalloc(orac_hip_ctx *ctx, int32_t ntap, const double *xp, const double *yp,
+                                        const double *zp, const double *xpcm, const double *ypcm, const double *zpcm,
+                                        const double *pmechg, const double *co, const double *oc, double volume,
+                                        double alphal, int32_t order, int32_t nfft1, int32_t nfft2, int32_t nfft3,
+                                        double *eer, double *fpx, double *fpy, double *fpz, double *stressc,
+                                        const int32_t *atomp, const int32_t *grppt, int32_t pressure, double rkcut) {
+  return guarded([&] {
+    set_device(ctx);
+    (void)grppt;
+    ORAC_REQUIRE(ntap >= 0, E_ARG, "negative atom count");
+    ORAC_REQUIRE(ctx->plan_ok && ctx->plan.K1 == nfft1 && ctx->plan.K2 == nfft2 && ctx->plan.K3 == nfft3 &&
+                     ctx->plan.order == order,
+                 E_STATE, "fft_pme called with a grid or order different from fft_pme_init");
+    ORAC_REQUIRE(eer && fpx && fpy && fpz && stressc && xp && yp && zp && pmechg && co && oc, E_ARG,
+                 "null array argument");
+    ORAC_REQUIRE(ntap <= SCRATCH_MAX_ATOMS && (int64_t)nfft1 * nfft2 * nfft3 <= SCRATCH_MAX_POINTS, E_CAPACITY,
+                 "fft_pme_noalloc: at most " + std::to_string(SCRATCH_MAX_ATOMS) + " atoms and " +
+                     std::to_string(SCRATCH_MAX_POINTS) + " grid points fit the static scratch region (got " +
+                     std::to_string(ntap) + " atoms, " + std::to_string((int64_t)nfft1 * nfft2 * nfft3) +
+                     " points)");
+    int nprot = 0;
+    if (pressure) {
+      ORAC_REQUIRE(atomp && xpcm && ypcm && zpcm, E_ARG, "pressure needs atomp and xpcm");
+      for (int i = 0; i < ntap; ++i) {
+        ORAC_REQUIRE(atomp[i] >= 1, E_ARG, "atomp holds an index below 1");
+        nprot = std::max(nprot, atomp[i]);
+      }
+      ORAC_REQUIRE(nprot <= ntap, E_ARG, "atomp holds an index above the atom count");
+    }
+    hipStream_t st = ctx->stream;
+    BufSet &B = ctx->S;
+    B.reset();
+    PmeDesc d = pme_desc_base(B, "p_", ctx->plan, ntap, oc, volume, alphal, rkcut, true, ctx->tuning, st);
+    d.xp = B.put("p_xp", xp, ntap, st, false);
+    d.yp = B.put("p_yp", yp, ntap, st, false);
+    d.zp = B.put("p_zp", zp, ntap, st, false);
+    d.co = B.put("p_co", co, 9, st, false);
+    d.q = B.put("p_q", pmechg, ntap, st, false);
+    d.fixscale = fix_scale_for(pmechg, ntap);
+    d.fixinv = 1.0 / d.fixscale;
+    // the gather adds the reciprocal forces to the caller's (fp += f, fft_pme.f:108-112)
+    d.fx = B.put("p_fx", fpx, ntap, st, false);
+    d.fy = B.put("p_fy", fpy, ntap, st, false);
+    d.fz = B.put("p_fz", fpz, ntap, st, false);
+    d.accumulate = 1;
+    if (pressure) {
+      int *ap = B.put("p_atomp", atomp, ntap, st, false);
+      if (ntap > 0) hipLaunchKernelGGL(k_atomp_to_zero_based, dim3((ntap + 255) / 256), dim3(256), 0, st, ap, ntap);
+      d.atomp = ap;
+      d.xpcm = B.put("p_xpcm", xpcm, nprot, st, false);
+      d.ypcm = B.put("p_ypcm", ypcm, nprot, st, false);
+      d.zpcm = B.put("p_zpcm", zpcm, nprot, st, false);
+    }
+    double *dout = B.get<double>("p_out", 40);
+    d.out = dout;
+    PmeDesc *dd = B.put("p_desc", &d, 1, st, false);
+    launch_pme<double>(dd, &d, 1, st, ctx->tuning);
+    ORAC_HIP(hipGetLastError());
+    double out[40];
+    ORAC_HIP(hipMemcpyAsync(fpx, d.fx, ntap * sizeof(double), hipMemcpyDeviceToHost, st));
+    ORAC_HIP(hipMemcpyAsync(fpy, d.fy, ntap * sizeof(double), hipMemcpyDeviceToHost, st));
+    ORAC_HIP(hipMemcpyAsync(fpz, d.fz, ntap * sizeof(double), hipMemcpyDeviceToHost, st));
+    ORAC_HIP(hipMemcpyAsync(out, dout, sizeof out, hipMemcpyDeviceToHost, st));
+    ORAC_HIP(hipStreamSynchronize(st));
+    *eer = out[0];
+    if (pressure) {
+      // stressc(r,c) = stressc(r,c) - vir(r,c) + stc(3r+c)   (fft_pme.f:97-105)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) stressc[r + 3 * c] = stressc[r + 3 * c] - out[1 + r + 3 * c] + out[10 + 3 * r + c];
+    }
+  });
 }
 
 // ---------------------------------------------------------------- lists

@@ -44,16 +44,32 @@ struct DevBuf {
 
 // Named grow-only device buffers.  A buffer keeps its address until a larger
 // size is asked for.
+//
+// Arena mode (arena != nullptr): get() carves consecutive 256-byte aligned
+// pieces out of a fixed device region the caller owns, never allocates, and
+// throws E_CAPACITY when the region is full; reset() starts over.  Uploads
+// are never skipped in arena mode, since other calls reuse the region.
 struct BufSet {
   std::map<std::string, std::unique_ptr<DevBuf>> bufs;
   std::map<std::string, std::vector<char>> cache;
+  char *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
   size_t bytes() const {
-    size_t s = 0;
+    size_t s = arena_used;
     for (auto &kv : bufs) s += kv.second->bytes;
     return s;
   }
+  void reset() { arena_used = 0; }
   template <class T>
   T *get(const std::string &name, size_t n) {
+    if (arena) {
+      const size_t nb = std::max<size_t>(n * sizeof(T), 8), off = (arena_used + 255) & ~(size_t)255;
+      ORAC_REQUIRE(off + nb <= arena_cap, E_CAPACITY,
+                   "scratch region full: " + name + " needs " + std::to_string(off + nb) + " bytes of " +
+                       std::to_string(arena_cap));
+      arena_used = off + nb;
+      return reinterpret_cast<T *>(arena + off);
+    }
     auto &b = bufs[name];
     if (!b) b.reset(new DevBuf());
     return static_cast<T *>(b->ensure(n * sizeof(T)));
@@ -63,7 +79,7 @@ struct BufSet {
   T *put(const std::string &name, const T *host, size_t n, hipStream_t st, bool cached) {
     T *d = get<T>(name, n);
     size_t nb = n * sizeof(T);
-    if (cached) {
+    if (cached && !arena) {
       auto &c = cache[name];
       if (c.size() == nb && nb && std::memcmp(c.data(), host, nb) == 0) return d;
       c.assign(reinterpret_cast<const char *>(host), reinterpret_cast<const char *>(host) + nb);
@@ -115,6 +131,7 @@ struct orac_hip_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // reciprocal sum beside the direct sum (same, default priority)
   orac::BufSet B;
+  orac::BufSet S;  // arena over this device's static scratch region (the no-allocation entry point)
   orac::PmePlan plan;
   bool plan_ok = false;
   orac::Tuning tuning;

@@ -52,7 +52,7 @@ def lib():
 EXPORTS = [
     "orac_hip_create", "orac_hip_destroy", "orac_hip_last_error", "orac_hip_invalidate", "orac_hip_device_name",
     "orac_hip_probe_copy",
-    "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
+    "orac_hip_fft_pme_init", "orac_hip_fft_pme", "orac_hip_fft_pme_noalloc", "orac_hip_lc_index", "orac_hip_lc_list", "orac_hip_force",
     "orac_hip_force_rem", "orac_hip_fptors_rem", "orac_hip_fitors_rem", "orac_hip_fnb14_rem",
     "orac_hip_force_alchemy", "orac_hip_force_alchemy_shift", "orac_hip_fptors", "orac_hip_fpbond", "orac_hip_fitors", "orac_hip_fpbend", "orac_hip_fnb14",
     "orac_hip_fft_pme_alchemy", "orac_hip_ferrf_alchemy", "orac_hip_cself_alchemy", "orac_hip_int_corr_erf_spline",
@@ -159,6 +159,19 @@ class Context:
                                       C.c_double(alphal), C.c_int32(order), C.c_int32(nfft1), C.c_int32(nfft2),
                                       C.c_int32(nfft3), C.byref(eer), _p(fpx), _p(fpy), _p(fpz), _p(stressc),
                                       _p(ap), _p(gp), C.c_int32(1 if pressure else 0), C.c_double(rkcut)))
+        return eer.value
+
+    def fft_pme_noalloc(self, ntap, xp, yp, zp, xpcm, ypcm, zpcm, pmechg, co, oc, volume, alphal, order, nfft1,
+                        nfft2, nfft3, fpx, fpy, fpz, stressc, atomp, grppt, pressure, rkcut):
+        """fft_pme through orac_hip_fft_pme_noalloc (the static scratch region, no allocation)."""
+        eer = C.c_double()
+        args = [_f(xp), _f(yp), _f(zp), _f(xpcm), _f(ypcm), _f(zpcm), _f(pmechg), _f(co), _f(oc)]
+        fpx, fpy, fpz, stressc = (_out(a, np.float64) for a in (fpx, fpy, fpz, stressc))
+        ap, gp = _i(atomp), _i(grppt)
+        _check(lib().orac_hip_fft_pme_noalloc(
+            self._h, C.c_int32(ntap), *[_p(a) for a in args], C.c_double(volume), C.c_double(alphal),
+            C.c_int32(order), C.c_int32(nfft1), C.c_int32(nfft2), C.c_int32(nfft3), C.byref(eer), _p(fpx), _p(fpy),
+            _p(fpz), _p(stressc), _p(ap), _p(gp), C.c_int32(1 if pressure else 0), C.c_double(rkcut)))
         return eer.value
 
     # -- fft_pme_alchemy.f:1
