@@ -43,7 +43,7 @@ FLOP_PER_PAIR = 89
 # HBM bytes per launch of k_pairs<double,3,false> on the 24,460-atom system, from the
 # committed FETCH_SIZE / WRITE_SIZE profile (rocprof_summary traffic)
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bytes_k_pairs_fp64_solvated24k.txt")
-PAIR_KERNEL = "k_pairs_own<double,3>"
+PAIR_KERNEL = "k_pairs<double,3,false>"
 
 
 def pme_stage_roofline(s, stages, fp64):

@@ -84,11 +84,12 @@ __device__ inline bool is_excluded(P ex_off, P ex_idx, int a, int b) {
 }
 
 // Pair arithmetic built on one reciprocal square root per pair.  FP64: the
-// hardware reciprocal square root and reciprocal (half-precision-class
-// seeds) refined by two Newton-Raphson steps each, which leaves a few ulp of
-// error (the reference's 1/sqrt and divides are correctly rounded; the
-// difference is ~1e-16 relative, far below the force tolerance) for a
-// fraction of the instructions of an IEEE divide and square root.  FP32
+// hardware reciprocal square root and reciprocal seeds refined by one
+// Newton-Raphson step each (~1e-14 relative; the reference's 1/sqrt and
+// divides are correctly rounded, and the difference stays far below the
+// force tolerance) for a fraction of the instructions of an IEEE divide and
+// square root, and exp by a two-constant ln 2 reduction and a degree-10
+// Taylor series (~3e-13 relative) in place of the libm routine.  FP32
 // (mixed precision): the hardware reciprocal, reciprocal square root and
 // exponential, 1-2 ulp each.  Pair distances are positive and finite, so no
 // special cases arise.
@@ -96,11 +97,8 @@ template <typename T>
 __device__ inline T pair_rsqrt(T x);
 template <>
 __device__ inline double pair_rsqrt<double>(double x) {
-  const double hx = 0.5 * x;
-  double r = __builtin_amdgcn_rsq(x);
-  r = fma(r, fma(-hx, r * r, 0.5), r);
-  r = fma(r, fma(-hx, r * r, 0.5), r);
-  return r;
+  const double r = __builtin_amdgcn_rsq(x);
+  return fma(r, fma(-0.5 * x, r * r, 0.5), r);
 }
 template <>
 __device__ inline float pair_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -108,17 +106,34 @@ template <typename T>
 __device__ inline T pair_rcp(T x);
 template <>
 __device__ inline double pair_rcp<double>(double x) {
-  double q = __builtin_amdgcn_rcp(x);
-  q = fma(q, fma(-x, q, 1.0), q);
-  q = fma(q, fma(-x, q, 1.0), q);
-  return q;
+  const double q = __builtin_amdgcn_rcp(x);
+  return fma(q, fma(-x, q, 1.0), q);
 }
 template <>
 __device__ inline float pair_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <typename T>
 __device__ inline T pair_exp(T x);
+// exp(x) for the Ewald factor exp(-(alpha r)^2) (x <= 0, modest): x = n ln2 +
+// r, |r| <= ln2 / 2 (ln 2 split in two constants, the reduction by FMA), e^r
+// by its Taylor series to degree 10 (truncation below 3e-13), scaled by 2^n
 template <>
-__device__ inline double pair_exp<double>(double x) { return exp(x); }
+__device__ inline double pair_exp<double>(double x) {
+  const double n = __builtin_rint(x * 1.4426950408889634);
+  double r = fma(-n, 0.6931471805599453, x);
+  r = fma(-n, 2.3190468138462996e-17, r);
+  double p = 2.755731922398589e-07;
+  p = fma(p, r, 2.7557319223985893e-06);
+  p = fma(p, r, 2.48015873015873e-05);
+  p = fma(p, r, 0.0001984126984126984);
+  p = fma(p, r, 0.001388888888888889);
+  p = fma(p, r, 0.008333333333333333);
+  p = fma(p, r, 0.041666666666666664);
+  p = fma(p, r, 0.16666666666666666);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return __builtin_amdgcn_ldexp(p, (int)n);
+}
 template <>
 __device__ inline float pair_exp<float>(float x) { return __expf(x); }
 
@@ -718,44 +733,19 @@ struct OwnShared {
 template <typename T>
 __device__ __forceinline__ T own_rsqrt(T x);
 template <>
-__device__ __forceinline__ double own_rsqrt<double>(double x) {
-  const double r = __builtin_amdgcn_rsq(x);
-  return fma(r, fma(-0.5 * x, r * r, 0.5), r);
-}
+__device__ __forceinline__ double own_rsqrt<double>(double x) { return pair_rsqrt<double>(x); }
 template <>
 __device__ __forceinline__ float own_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 template <typename T>
 __device__ __forceinline__ T own_rcp(T x);
 template <>
-__device__ __forceinline__ double own_rcp<double>(double x) {
-  const double q = __builtin_amdgcn_rcp(x);
-  return fma(q, fma(-x, q, 1.0), q);
-}
+__device__ __forceinline__ double own_rcp<double>(double x) { return pair_rcp<double>(x); }
 template <>
 __device__ __forceinline__ float own_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
-// exp(x) for the Ewald factor exp(-(alpha r)^2): x = n ln2 + r, |r| <= ln2 / 2
-// (ln 2 split in two constants, the reduction by FMA), e^r by its Taylor
-// series to degree 10 (truncation below 3e-13), scaled by 2^n
 template <typename T>
 __device__ __forceinline__ T own_exp(T x);
 template <>
-__device__ __forceinline__ double own_exp<double>(double x) {
-  const double n = __builtin_rint(x * 1.4426950408889634);
-  double r = fma(-n, 0.6931471805599453, x);
-  r = fma(-n, 2.3190468138462996e-17, r);
-  double p = 2.755731922398589e-07;
-  p = fma(p, r, 2.7557319223985893e-06);
-  p = fma(p, r, 2.48015873015873e-05);
-  p = fma(p, r, 0.0001984126984126984);
-  p = fma(p, r, 0.001388888888888889);
-  p = fma(p, r, 0.008333333333333333);
-  p = fma(p, r, 0.041666666666666664);
-  p = fma(p, r, 0.16666666666666666);
-  p = fma(p, r, 0.5);
-  p = fma(p, r, 1.0);
-  p = fma(p, r, 1.0);
-  return __builtin_amdgcn_ldexp(p, (int)n);
-}
+__device__ __forceinline__ double own_exp<double>(double x) { return pair_exp<double>(x); }
 template <>
 __device__ __forceinline__ float own_exp<float>(float x) { return __expf(x); }
 
@@ -1230,15 +1220,17 @@ static int pair_occupancy(int tuned) {
   return w;
 }
 
-// The owner-atom loop form serves every non-alchemical system with one wave
-// per owner; ORAC_HIP_PAIR_KERNEL=lanes selects the owner-lane form instead
-// (and no system is given the half-list layout).
+// The owner-atom loop form (and with it the half-list layout of resident
+// systems) is selected by ORAC_HIP_PAIR_KERNEL=own: read when a resident
+// system is created (its list layout) and at each launch of a system on the
+// full list; by default every system takes k_pairs on the full list.
+// Measured on MI355X on the 24,460-atom system (one evaluation with a list
+// rebuild): the half-list form's per-pair slots and its list build (split,
+// reverse ranking, mirror search) cost more than the halved arithmetic saves
+// (0.454 vs 0.346 ms in FP64).
 bool use_owner_kernel() {
-  static const bool v = [] {
-    const char *e = getenv("ORAC_HIP_PAIR_KERNEL");
-    return !(e && std::string(e) == "lanes");
-  }();
-  return v;
+  const char *e = getenv("ORAC_HIP_PAIR_KERNEL");
+  return e && std::string(e) == "own";
 }
 
 template <typename T>

@@ -296,3 +296,27 @@ def test_decide_force():
     for r in (0.0, 3.0, 5.1, 7.8, 10.0):
         for a in (0.3, 0.37, 0.4):
             assert Context.decide_force(r, a) == O.decide_force(r, a)
+
+
+@pytest.mark.parametrize("shell", [0, 1])
+def test_force_owner_form(gpu_ctx, small_solute, shell, monkeypatch):
+    """ORAC_HIP_PAIR_KERNEL=own: the force entry point takes the owner-atom
+    loop form (full list) for groups of at most three atoms; against the
+    oracle, including force.f's massinfty rule."""
+    import dataclasses
+    monkeypatch.setenv("ORAC_HIP_PAIR_KERNEL", "own")
+    s = small_solute
+    mass = s.mass.copy()
+    mass[:20] = 1e21
+    s2 = dataclasses.replace(s, mass=mass)
+    nnlpp, fa, _ = _oracle_list(s2)
+    sh = dict(SHELLS[shell])
+    if sh["rout"] is None:
+        sh["rout"] = s.rcut
+    g = _force_gpu(gpu_ctx, s2, nnlpp, fa, massinfty=1e20, **sh)
+    o = O.force(s2, nnlpp, fa=fa, massinfty=1e20, **sh)
+    assert relrms(g["f"], o["f"]) < F_TOL
+    for k in ("uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"):
+        if o[k] != 0.0:
+            assert rel(g[k], o[k]) < E_TOL, k
+    assert _stress_close(g["stress"], o["stress"])

@@ -167,3 +167,51 @@ def test_packed_lj_table_path(gpu_ctx, small_solute):
     rs.eval(rebuild_list=True)
     assert relrms(rs.results()["f"], g["f"]) < 1e-12
     rs.close()
+
+
+# ---------------------------------------------------------------- owner-atom loop form (opt-in)
+
+@pytest.mark.parametrize("sysname", ["water216", "solvated24k"])
+def test_owner_form_half_list_matches_oracle(gpu_ctx, sysname, request, monkeypatch):
+    """ORAC_HIP_PAIR_KERNEL=own: resident systems of groups of at most three
+    atoms take the half-list layout and the owner-atom loop kernel; FP64
+    forces, energies and virial against the oracle at the plain path's
+    tolerances, and bit-identical on a second evaluation."""
+    from orac_hip.api import ResidentSystem
+    monkeypatch.setenv("ORAC_HIP_PAIR_KERNEL", "own")
+    s = request.getfixturevalue(sysname)
+    cp, cf = _corr(s)
+    rs = ResidentSystem(gpu_ctx, s, precision=0, corr_pairs=cp, corr_fudge=cf)
+    rs.eval(rebuild_list=True)
+    g = rs.results()
+    rs.eval(rebuild_list=True)
+    assert np.array_equal(g["f"], rs.results()["f"])
+    rs.close()
+    o = oracle_total(s, cp, cf)
+    assert relrms(g["f"], o["f"]) < 1e-9
+    for k in ("uconf_slt", "uconf_slv", "uconf_ss", "ucoul_slt", "ucoul_slv", "ucoul_ss"):
+        if o["d"][k] != 0.0:
+            assert rel(g[k], o["d"][k]) < 1e-10, k
+    assert np.abs(g["stress"] - o["stress"]).max() <= 1e-10 * np.abs(o["stress"]).max()
+
+
+def test_owner_form_mixed_batch_equals_alone(gpu_ctx, monkeypatch):
+    """The half-list layout in mixed precision: a batch of three systems gives
+    each one its forces alone, bit for bit."""
+    from orac_hip import synth
+    from orac_hip.api import Batch, ResidentSystem
+    monkeypatch.setenv("ORAC_HIP_PAIR_KERNEL", "own")
+    systems = [ResidentSystem(gpu_ctx, s, precision=1) for s in
+               (synth.config("water216"), synth.small_solvated(rcut=7.0),
+                synth.water_box(seed=77, L=24.0, nside=8, nfft=(24, 30, 25), rcut=9.0))]
+    alone = []
+    for rs in systems:
+        rs.eval(rebuild_list=True)
+        alone.append(rs.results()["f"])
+    b = Batch(gpu_ctx, systems)
+    b.eval(rebuild_list=True)
+    for r, a in zip(b.results(), alone):
+        assert np.array_equal(r["f"], a)
+    b.close()
+    for x in systems:
+        x.close()
