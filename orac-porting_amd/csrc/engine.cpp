@@ -854,14 +854,10 @@ static void md_setup(orac_hip_ctx *ctx, System &S, int nrigid, const int32_t *ri
     goff[a + 1] = (int)gt.size();
   }
   if (gt.empty()) gt.push_back(0);
-  std::vector<double> invm(n);
   {
     std::vector<double> m(n);
     ORAC_HIP(hipMemcpy(m.data(), S.pair.mass, n * sizeof(double), hipMemcpyDeviceToHost));
-    for (int a = 0; a < n; ++a) {
-      ORAC_REQUIRE(m[a] > 0.0, E_ARG, "dynamics needs positive masses");
-      invm[a] = 1.0 / m[a];
-    }
+    for (int a = 0; a < n; ++a) ORAC_REQUIRE(m[a] > 0.0, E_ARG, "dynamics needs positive masses");
   }
   MdDesc &d = S.md;
   std::memset(&d, 0, sizeof d);
@@ -883,7 +879,7 @@ static void md_setup(orac_hip_ctx *ctx, System &S, int nrigid, const int32_t *ri
   d.bfx = B.get<double>("md_bfx", n);
   d.bfy = B.get<double>("md_bfy", n);
   d.bfz = B.get<double>("md_bfz", n);
-  d.invm = B.put_vec("md_invm", invm, st, false);
+  d.mass = S.pair.mass;
   d.ufirst = B.put_vec("md_ufirst", ufirst, st, false);
   d.urigid = B.put_vec("md_urigid", urigid, st, false);
   d.ud2 = B.put_vec("md_ud2", ud2, st, false);
@@ -980,14 +976,17 @@ static void md_enqueue_step(orac_hip_ctx *ctx, Batch &b, double dt, int ninner) 
   hipStream_t st = ctx->stream;
   const int n = (int)b.sys.size();
   const double h = dt / ninner;
-  launch_md_units(b.dm, b.hm.data(), n, 0.5 * dt, 0.5 * h, h, 1 | 2 | 4, st);
+  // slow half kick, then the first fast substep's drift
+  launch_md_units(b.dm, b.hm.data(), n, dt, h, MD_KICK_SLOW | MD_VERLET, st);
   launch_md_fast_forces(b.dm, b.hm.data(), n, st);
+  // end of one fast substep (half kick), start of the next (drift)
   for (int k = 1; k < ninner; ++k) {
-    launch_md_units(b.dm, b.hm.data(), n, 0.0, h, h, 2 | 4, st);
+    launch_md_units(b.dm, b.hm.data(), n, dt, h, MD_KICK_FAST | MD_VERLET, st);
     launch_md_fast_forces(b.dm, b.hm.data(), n, st);
   }
   batch_enqueue(ctx, b, true, false, true);
-  launch_md_units(b.dm, b.hm.data(), n, 0.5 * dt, 0.5 * h, 0.0, 1 | 2 | 8, st);
+  // end of the last fast substep, then the slow half kick
+  launch_md_units(b.dm, b.hm.data(), n, dt, h, MD_KICK_FAST | MD_KICK_SLOW, st);
   launch_md_tick(b.dm, n, st);
 }
 
@@ -1498,7 +1497,7 @@ extern "C" int orac_hip_md_set_state(orac_hip_ctx *ctx, int32_t sys, const doubl
       ORAC_HIP(hipMemcpyAsync(S.md.vy, vy, nb, hipMemcpyHostToDevice, st));
       ORAC_HIP(hipMemcpyAsync(S.md.vz, vz, nb, hipMemcpyHostToDevice, st));
       // velocities onto the constraint surface of the rigid units (RATTLE)
-      launch_md_units(S.md_d, &S.md, 1, 0.0, 0.0, 0.0, 8, st);
+      launch_md_units(S.md_d, &S.md, 1, 0.0, 0.0, MD_RATTLE, st);
     }
     ORAC_HIP(hipStreamSynchronize(st));
     if (moved) size_list(ctx, sys);

@@ -282,7 +282,7 @@ struct MdDesc {
   double *xp, *yp, *zp;              // reduced copy read by the force path
   const double *fx, *fy, *fz;        // slow forces (the nonbonded evaluation)
   double *bfx, *bfy, *bfz;           // fast forces (harmonic distance terms)
-  const double *invm;
+  const double *mass;                // atom masses
   const int *ufirst;                 // first atom of each unit
   const unsigned char *urigid;       // 1: rigid 3-atom unit
   const double *ud2;                 // [nunits][3] squared lengths (0,1), (0,2), (1,2)
@@ -295,10 +295,11 @@ struct MdDesc {
   int *fail;                         // constraint iterations that did not converge
   int *step;                         // outer steps taken (device counter, drives the list schedule)
 };
-// flags: 1 kick by the slow forces (cslow), 2 kick by the fast forces (cfast),
-// 4 drift by hdt with SHAKE, 8 RATTLE
-void launch_md_units(const MdDesc *d, const MdDesc *h, int nsys, double cslow, double cfast, double hdt, int flags,
-                     hipStream_t st);
+// stages of launch_md_units, applied in this order when combined: half kick
+// by the fast forces over h + RATTLE; half kick by the slow forces over dt +
+// RATTLE; drift over h (ORAC's verlet with the fast forces) + SHAKE; RATTLE
+constexpr int MD_KICK_FAST = 1, MD_KICK_SLOW = 2, MD_VERLET = 4, MD_RATTLE = 8;
+void launch_md_units(const MdDesc *d, const MdDesc *h, int nsys, double dt, double hin, int flags, hipStream_t st);
 void launch_md_fast_forces(const MdDesc *d, const MdDesc *h, int nsys, hipStream_t st);
 void launch_md_energy(const MdDesc *d, int nsys, hipStream_t st);
 void launch_md_tick(const MdDesc *d, int nsys, hipStream_t st);

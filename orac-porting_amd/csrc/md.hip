@@ -1,16 +1,19 @@
 // md.hip -- on-device r-RESPA integration of resident systems on gfx950.
 //
-// Two time levels, as ORAC's multiple-time-step integrator nests them
-// (mtsmd.f): the fast level holds the solute's intramolecular harmonic
-// distance terms and runs ninner substeps of dt/ninner; the slow level is the
-// full nonbonded force (direct space, PME, corrections) once per outer step
-// of dt.  Rigid units (3-site waters) keep their bond lengths by SHAKE after
-// every drift and RATTLE after every kick.  Velocity Verlet at both levels:
+// Two time levels of ORAC's multiple-time-step integrator (mtsmd.f), in its
+// order of operations: the fast level (ORAC's n0 level: the solute's
+// intramolecular harmonic distance terms) runs ninner substeps of
+// h = dt/ninner inside one step dt of the slow level (ORAC's m level: the full
+// nonbonded force -- direct space, PME, corrections -- with a single shell).
+// Rigid units (3-site waters) are constrained by ORAC's matrix-inversion
+// SHAKE after every drift and its RATTLE after every kick:
 //
-//   v += dt/2 F_slow/m
-//   ninner times:  v += h/2 F_fast/m;  x += h v (SHAKE);  F_fast(x);  v += h/2 F_fast/m (RATTLE)
-//   F_slow(x)      (the force evaluation of the engine)
-//   v += dt/2 F_slow/m (RATTLE)
+//   v += Fs*0.5*dt/m; RATTLE                                 (:2598-2604)
+//   ninner times:  x += h*v + Ff*0.5*h^2/m; v += Ff*0.5*h/m;
+//                  SHAKE against the positions before        (:2663-2683)
+//                  Ff(x); v += Ff*0.5*h/m; RATTLE            (:2843-2884)
+//   Fs(x)          (the force evaluation of the engine)
+//   v += Fs*0.5*dt/m; RATTLE                                 (:3121-3146)
 //
 // Positions are Cartesian FP64 with the reduced copy xp = oc x refreshed after
 // every drift; the force path reads the reduced copy.  A unit is one atom or
@@ -30,119 +33,280 @@ __device__ inline void to_reduced(const double *oc, double x, double y, double z
   zp = oc[2] * x + oc[5] * y + oc[8] * z;
 }
 
-// SHAKE on one rigid triangle: q[k] are the drifted positions, p[k] the
-// positions before the drift (which satisfy the constraints).  Returns the
-// positions moved onto the constraint surface; dv[k] += the velocity change
-// (displacement / h).  Constraints (0,1), (0,2), (1,2) with squared lengths d2.
-__device__ inline bool shake3(double3 *q, const double3 *p, const double *im, const double *d2, double h,
-                              double3 *dv, double tol, int maxit) {
-  const int ci[3] = {0, 0, 1}, cj[3] = {1, 2, 2};
-  for (int it = 0; it < maxit; ++it) {
-    bool done = true;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int i = ci[c], j = cj[c];
-      const double sx = q[i].x - q[j].x, sy = q[i].y - q[j].y, sz = q[i].z - q[j].z;
-      const double diff = d2[c] - (sx * sx + sy * sy + sz * sz);
-      if (fabs(diff) > tol * d2[c]) {
-        done = false;
-        const double rx = p[i].x - p[j].x, ry = p[i].y - p[j].y, rz = p[i].z - p[j].z;
-        const double g = diff / (2.0 * (sx * rx + sy * ry + sz * rz) * (im[i] + im[j]));
-        q[i].x += g * im[i] * rx; q[i].y += g * im[i] * ry; q[i].z += g * im[i] * rz;
-        q[j].x -= g * im[j] * rx; q[j].y -= g * im[j] * ry; q[j].z -= g * im[j] * rz;
-        dv[i].x += g * im[i] * rx / h; dv[i].y += g * im[i] * ry / h; dv[i].z += g * im[i] * rz / h;
-        dv[j].x -= g * im[j] * rx / h; dv[j].y -= g * im[j] * ry / h; dv[j].z -= g * im[j] * rz / h;
-      }
-    }
-    if (done) return true;
-  }
-  return false;
+// Constraints of one rigid triangle by ORAC's matrix-inversion method (MIM),
+// the branch ORAC takes for molecules of at most mim_lim constraints
+// (default 20, blkdta.f90:361): SHAKE after a drift as rattle_verlet.f:228-560
+// does it, RATTLE after a kick as rattle_correc.f:176-385.  Constraint order
+// (0,1), (0,2), (1,2); im = 1/m; dss the squared lengths.  mim_dir gives the
+// direction in which atom `at` moves for constraint (ca, cb) with bond vector
+// d (rattle_verlet.f:286-303): -d for its first atom, +d for its second.
+__device__ inline double3 mim_dir(int at, int ca, int cb, double3 d) {
+  if (at == ca) return make_double3(-d.x, -d.y, -d.z);
+  if (at == cb) return d;
+  return make_double3(0.0, 0.0, 0.0);
 }
 
-// RATTLE velocity stage on one rigid triangle: removes the velocity
-// components along the bonds, r_ij . v_ij = 0.
-__device__ inline bool rattle3(const double3 *r, double3 *v, const double *im, double tol, int maxit) {
-  const int ci[3] = {0, 0, 1}, cj[3] = {1, 2, 2};
-  for (int it = 0; it < maxit; ++it) {
-    bool done = true;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int i = ci[c], j = cj[c];
-      const double rx = r[i].x - r[j].x, ry = r[i].y - r[j].y, rz = r[i].z - r[j].z;
-      const double vx = v[i].x - v[j].x, vy = v[i].y - v[j].y, vz = v[i].z - v[j].z;
-      const double rr = rx * rx + ry * ry + rz * rz;
-      const double rv = rx * vx + ry * vy + rz * vz;
-      if (fabs(rv) > tol * rr) {
-        done = false;
-        const double k = rv / (rr * (im[i] + im[j]));
-        v[i].x -= k * im[i] * rx; v[i].y -= k * im[i] * ry; v[i].z -= k * im[i] * rz;
-        v[j].x += k * im[j] * rx; v[j].y += k * im[j] * ry; v[j].z += k * im[j] * rz;
-      }
-    }
-    if (done) return true;
-  }
-  return false;
+// 3x3 solve by cofactors (rattle_verlet.f:381-397); later solves reuse them
+struct MimInv {
+  double b11, b12, b13, b21, b22, b23, b31, b32, b33, det;
+};
+
+__device__ inline MimInv mim_inverse(const double (&m)[3][3]) {
+  MimInv r;
+  const double a11 = m[0][0], a12 = m[0][1], a13 = m[0][2], a21 = m[1][0], a22 = m[1][1], a23 = m[1][2],
+               a31 = m[2][0], a32 = m[2][1], a33 = m[2][2];
+  r.b11 = a22 * a33 - a32 * a23;
+  r.b12 = -a21 * a33 + a31 * a23;
+  r.b13 = a21 * a32 - a22 * a31;
+  r.b21 = -a12 * a33 + a32 * a13;
+  r.b22 = a11 * a33 - a31 * a13;
+  r.b23 = -a11 * a32 + a31 * a12;
+  r.b31 = a12 * a23 - a22 * a13;
+  r.b32 = -a11 * a23 + a21 * a13;
+  r.b33 = a11 * a22 - a21 * a12;
+  r.det = 1.0 / (a11 * r.b11 + a21 * r.b21 + a31 * r.b31);
+  return r;
 }
 
-constexpr double SHAKE_TOL = 1.0e-12;
-constexpr double RATTLE_TOL = 1.0e-12;
-constexpr int CONSTRAINT_MAXIT = 500;
+__device__ inline void mim_solve(const MimInv &r, double (&g)[3]) {
+  const double xd = r.b11 * g[0] + r.b21 * g[1] + r.b31 * g[2];
+  const double yd = r.b12 * g[0] + r.b22 * g[1] + r.b32 * g[2];
+  const double zd = r.b13 * g[0] + r.b23 * g[1] + r.b33 * g[2];
+  g[0] = xd * r.det;
+  g[1] = yd * r.det;
+  g[2] = zd * r.det;
+}
 
-// Kick (v += c F/m, F = ff (fast) or the slow forces), optionally a drift by h
-// with SHAKE, optionally RATTLE.  flags: 1 kick slow, 2 kick fast, 4 drift,
-// 8 rattle.  One thread per unit; NA atoms per unit as a template parameter
-// so the unit's arrays stay in registers.
+constexpr double MIM_TOL = 1.0e-7;  // relative change of the multipliers (rattle_verlet.f:77)
+constexpr int MIM_MAXIT = 120;      // rattle_verlet.f:461
+
+// SHAKE: q are the drifted positions, p the positions before the drift (whose
+// bond vectors give the directions of the corrections); v += correction * dti.
+// The multipliers solve dss = |q_ij + corrections|^2 by iterating on the
+// quadratic term; that term weights every constraint with the masses of the
+// last constraint's atoms, as rattle_verlet.f:406-407 does.  Returns false
+// when MIM_MAXIT iterations do not converge (the reference then starts the
+// molecule over; here the unit is counted as failed and keeps what it has).
+__device__ inline bool shake_mim3(double3 (&q)[3], const double3 (&p)[3], const double (&im)[3], const double *dss,
+                                  double dti, double3 (&v)[3]) {
+  constexpr int ca[3] = {0, 0, 1}, cb[3] = {1, 2, 2};
+  double3 x0[3], xc[3], da[3][3], db[3][3];
+  double dd[3], mat[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    x0[k] = make_double3(p[ca[k]].x - p[cb[k]].x, p[ca[k]].y - p[cb[k]].y, p[ca[k]].z - p[cb[k]].z);
+    xc[k] = make_double3(q[ca[k]].x - q[cb[k]].x, q[ca[k]].y - q[cb[k]].y, q[ca[k]].z - q[cb[k]].z);
+  }
+#pragma unroll
+  for (int ka = 0; ka < 3; ++ka) {
+    const int la = ca[ka], lb = cb[ka];
+    dd[ka] = dss[ka] - (xc[ka].x * xc[ka].x + xc[ka].y * xc[ka].y + xc[ka].z * xc[ka].z);
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb) {
+      const double3 a = mim_dir(la, ca[kb], cb[kb], x0[kb]);
+      const double3 b = mim_dir(lb, ca[kb], cb[kb], x0[kb]);
+      da[ka][kb] = a;
+      db[ka][kb] = b;
+      double aux3 = (-a.x * xc[ka].x + -a.y * xc[ka].y + -a.z * xc[ka].z) * im[la];
+      aux3 = aux3 + (b.x * xc[ka].x + b.y * xc[ka].y + b.z * xc[ka].z) * im[lb];
+      mat[ka][kb] = -(aux3 * 4.0);
+    }
+  }
+  const MimInv inv = mim_inverse(mat);
+  const double gg1 = im[ca[2]] * 2.0, gg2 = im[cb[2]] * 2.0;  // the last constraint's atoms
+  double gam[3] = {0.0, 0.0, 0.0};
+  bool ok = true;
+  for (int iter = 0;;) {
+    const double gamo[3] = {gam[0], gam[1], gam[2]};
+    double g[3];
+    if (iter == 0) {
+#pragma unroll
+      for (int ka = 0; ka < 3; ++ka) g[ka] = dd[ka];
+    } else {
+#pragma unroll
+      for (int ka = 0; ka < 3; ++ka) {
+        double xd = 0.0, yd = 0.0, zd = 0.0;
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb) {
+          double aux = gam[kb] * gg1;
+          xd = xd + da[ka][kb].x * aux;
+          yd = yd + da[ka][kb].y * aux;
+          zd = zd + da[ka][kb].z * aux;
+          aux = gam[kb] * gg2;
+          xd = xd + -db[ka][kb].x * aux;
+          yd = yd + -db[ka][kb].y * aux;
+          zd = zd + -db[ka][kb].z * aux;
+        }
+        g[ka] = dd[ka] - (xd * xd + yd * yd + zd * zd);
+      }
+    }
+    mim_solve(inv, g);
+#pragma unroll
+    for (int ka = 0; ka < 3; ++ka) gam[ka] = g[ka];
+    ++iter;
+    if (iter > MIM_MAXIT) {
+      ok = false;
+      break;
+    }
+    bool more = false;
+#pragma unroll
+    for (int ka = 0; ka < 3; ++ka)
+      if (gam[ka] != 0.0 && fabs((gam[ka] - gamo[ka]) / gam[ka]) > MIM_TOL) more = true;
+    if (!more) break;
+  }
+  // corrections, each atom once: by the first constraint that holds it
+  // (rattle_verlet.f:514-541)
+  bool mask[3] = {true, true, true};
+#pragma unroll
+  for (int ka = 0; ka < 3; ++ka) {
+    const int la = ca[ka], lb = cb[ka];
+    const double xd = im[la] * 2.0, yd = im[lb] * 2.0;
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb) {
+      if (mask[la]) {
+        double aux = gam[kb] * xd;
+        q[la].x = q[la].x + da[ka][kb].x * aux;
+        q[la].y = q[la].y + da[ka][kb].y * aux;
+        q[la].z = q[la].z + da[ka][kb].z * aux;
+        aux = aux * dti;
+        v[la].x = v[la].x + da[ka][kb].x * aux;
+        v[la].y = v[la].y + da[ka][kb].y * aux;
+        v[la].z = v[la].z + da[ka][kb].z * aux;
+      }
+      if (mask[lb]) {
+        double aux = gam[kb] * yd;
+        q[lb].x = q[lb].x + db[ka][kb].x * aux;
+        q[lb].y = q[lb].y + db[ka][kb].y * aux;
+        q[lb].z = q[lb].z + db[ka][kb].z * aux;
+        aux = aux * dti;
+        v[lb].x = v[lb].x + db[ka][kb].x * aux;
+        v[lb].y = v[lb].y + db[ka][kb].y * aux;
+        v[lb].z = v[lb].z + db[ka][kb].z * aux;
+      }
+    }
+    mask[la] = false;
+    mask[lb] = false;
+  }
+  return ok;
+}
+
+// RATTLE: velocities made orthogonal to the bonds at positions r by one
+// linear solve (rattle_correc.f:176-385).
+__device__ inline void rattle_mim3(const double3 (&r)[3], double3 (&v)[3], const double (&im)[3]) {
+  constexpr int ca[3] = {0, 0, 1}, cb[3] = {1, 2, 2};
+  double3 bv[3], da[3][3], db[3][3];
+  double mat[3][3], gam[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    bv[k] = make_double3(r[ca[k]].x - r[cb[k]].x, r[ca[k]].y - r[cb[k]].y, r[ca[k]].z - r[cb[k]].z);
+#pragma unroll
+  for (int ka = 0; ka < 3; ++ka) {
+    const int la = ca[ka], lb = cb[ka];
+    const double3 xc = make_double3(v[la].x - v[lb].x, v[la].y - v[lb].y, v[la].z - v[lb].z);
+    gam[ka] = -(xc.x * bv[ka].x + xc.y * bv[ka].y + xc.z * bv[ka].z);
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb) {
+      const double3 a = mim_dir(la, ca[kb], cb[kb], bv[kb]);
+      const double3 b = mim_dir(lb, ca[kb], cb[kb], bv[kb]);
+      da[ka][kb] = a;
+      db[ka][kb] = b;
+      double aux3 = (bv[ka].x * -a.x + bv[ka].y * -a.y + bv[ka].z * -a.z) * im[la];
+      aux3 = aux3 + (bv[ka].x * b.x + bv[ka].y * b.y + bv[ka].z * b.z) * im[lb];
+      mat[ka][kb] = -(aux3 * 2.0);
+    }
+  }
+  mim_solve(mim_inverse(mat), gam);
+  bool mask[3] = {true, true, true};
+#pragma unroll
+  for (int ka = 0; ka < 3; ++ka) {
+    const int la = ca[ka], lb = cb[ka];
+    const double xd = im[la] * 2.0, yd = im[lb] * 2.0;
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb) {
+      if (mask[la]) {
+        v[la].x = v[la].x + (da[ka][kb].x * gam[kb]) * xd;
+        v[la].y = v[la].y + (da[ka][kb].y * gam[kb]) * xd;
+        v[la].z = v[la].z + (da[ka][kb].z * gam[kb]) * xd;
+      }
+      if (mask[lb]) {
+        v[lb].x = v[lb].x + (db[ka][kb].x * gam[kb]) * yd;
+        v[lb].y = v[lb].y + (db[ka][kb].y * gam[kb]) * yd;
+        v[lb].z = v[lb].z + (db[ka][kb].z * gam[kb]) * yd;
+      }
+    }
+    mask[la] = false;
+    mask[lb] = false;
+  }
+}
+
+// One unit's share of an integration stage, in ORAC's order (mtsmd.f):
+//   MD_KICK_FAST  v += Ff*0.5*h/m, RATTLE        correc + rattle_correc (:2876-2884)
+//   MD_KICK_SLOW  v += Fs*0.5*dt/m, RATTLE       correc + rattle_correc (:2598-2604, :3121-3146)
+//   MD_VERLET     x += h*v + Ff*0.5*h^2/m, v += Ff*0.5*h/m, then SHAKE against the
+//                 positions before the drift      verlet + rattle_verlet (:2663-2683)
+//   MD_RATTLE     RATTLE alone
+// applied in that order when several are set (a stage that ends one level's
+// step and starts the next).  Ff are the fast forces, Fs the slow ones; the
+// expressions keep verlet.f's and correc.f's association.  One thread per
+// unit; NA atoms per unit as a template parameter so the unit stays in
+// registers.
 template <int NA>
-__device__ __forceinline__ void md_unit(const MdDesc &d, int u, int a0, double cslow, double cfast, double h,
-                                        int flags) {
-  double3 x[NA], v[NA], p[NA], dv[NA];
-  double im[NA];
+__device__ __forceinline__ void md_unit(const MdDesc &d, int u, int a0, double dt, double h, int flags) {
+  double3 x[NA], v[NA], ff[NA];
+  double m[NA], im[NA];
 #pragma unroll
   for (int k = 0; k < NA; ++k) {
     const int a = a0 + k;
-    im[k] = gmem(d.invm)[a];
+    m[k] = gmem(d.mass)[a];
+    im[k] = 1.0 / m[k];
     x[k] = make_double3(gmem(d.x)[a], gmem(d.y)[a], gmem(d.z)[a]);
     v[k] = make_double3(gmem(d.vx)[a], gmem(d.vy)[a], gmem(d.vz)[a]);
-    if (flags & 1) {
-      v[k].x += cslow * im[k] * gmem(d.fx)[a];
-      v[k].y += cslow * im[k] * gmem(d.fy)[a];
-      v[k].z += cslow * im[k] * gmem(d.fz)[a];
-    }
-    if (flags & 2) {
-      v[k].x += cfast * im[k] * gmem(d.bfx)[a];
-      v[k].y += cfast * im[k] * gmem(d.bfy)[a];
-      v[k].z += cfast * im[k] * gmem(d.bfz)[a];
-    }
+    ff[k] = make_double3(gmem(d.bfx)[a], gmem(d.bfy)[a], gmem(d.bfz)[a]);
   }
-  if (flags & 4) {
+  if (flags & MD_KICK_FAST) {
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+      v[k].x = v[k].x + ff[k].x * 0.5 * h / m[k];
+      v[k].y = v[k].y + ff[k].y * 0.5 * h / m[k];
+      v[k].z = v[k].z + ff[k].z * 0.5 * h / m[k];
+    }
+    if constexpr (NA == 3) rattle_mim3(x, v, im);
+  }
+  if (flags & MD_KICK_SLOW) {
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+      const int a = a0 + k;
+      v[k].x = v[k].x + gmem(d.fx)[a] * 0.5 * dt / m[k];
+      v[k].y = v[k].y + gmem(d.fy)[a] * 0.5 * dt / m[k];
+      v[k].z = v[k].z + gmem(d.fz)[a] * 0.5 * dt / m[k];
+    }
+    if constexpr (NA == 3) rattle_mim3(x, v, im);
+  }
+  if (flags & MD_VERLET) {
+    double3 p[NA];
+    const double ts2 = h * h;
 #pragma unroll
     for (int k = 0; k < NA; ++k) {
       p[k] = x[k];
-      x[k].x += h * v[k].x;
-      x[k].y += h * v[k].y;
-      x[k].z += h * v[k].z;
-      dv[k] = make_double3(0.0, 0.0, 0.0);
+      x[k].x = x[k].x + h * v[k].x + ff[k].x * 0.5 * ts2 / m[k];
+      x[k].y = x[k].y + h * v[k].y + ff[k].y * 0.5 * ts2 / m[k];
+      x[k].z = x[k].z + h * v[k].z + ff[k].z * 0.5 * ts2 / m[k];
+      v[k].x = v[k].x + ff[k].x * 0.5 * h / m[k];
+      v[k].y = v[k].y + ff[k].y * 0.5 * h / m[k];
+      v[k].z = v[k].z + ff[k].z * 0.5 * h / m[k];
     }
-    if constexpr (NA == 3) {
-      if (!shake3(x, p, im, d.ud2 + 3 * (size_t)u, h, dv, SHAKE_TOL, CONSTRAINT_MAXIT)) atomicAdd(d.fail, 1);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        v[k].x += dv[k].x;
-        v[k].y += dv[k].y;
-        v[k].z += dv[k].z;
-      }
-    }
+    if constexpr (NA == 3)
+      if (!shake_mim3(x, p, im, d.ud2 + 3 * (size_t)u, h != 0.0 ? 1.0 / h : 0.0, v)) atomicAdd(d.fail, 1);
   }
   if constexpr (NA == 3)
-    if (flags & 8)
-      if (!rattle3(x, v, im, RATTLE_TOL, CONSTRAINT_MAXIT)) atomicAdd(d.fail, 1);
+    if (flags & MD_RATTLE) rattle_mim3(x, v, im);
 #pragma unroll
   for (int k = 0; k < NA; ++k) {
     const int a = a0 + k;
     gmem(d.vx)[a] = v[k].x;
     gmem(d.vy)[a] = v[k].y;
     gmem(d.vz)[a] = v[k].z;
-    if (flags & 4) {
+    if (flags & MD_VERLET) {
       gmem(d.x)[a] = x[k].x;
       gmem(d.y)[a] = x[k].y;
       gmem(d.z)[a] = x[k].z;
@@ -155,16 +319,15 @@ __device__ __forceinline__ void md_unit(const MdDesc &d, int u, int a0, double c
   }
 }
 
-__global__ void __launch_bounds__(256) k_md_units(const MdDesc *__restrict__ D, double cslow, double cfast,
-                                                  double h, int flags) {
+__global__ void __launch_bounds__(256) k_md_units(const MdDesc *__restrict__ D, double dt, double h, int flags) {
   const MdDesc &d = D[blockIdx.y];
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= d.nunits) return;
   const int a0 = gmem(d.ufirst)[u];
   if (gmem(d.urigid)[u])
-    md_unit<3>(d, u, a0, cslow, cfast, h, flags);
+    md_unit<3>(d, u, a0, dt, h, flags);
   else
-    md_unit<1>(d, u, a0, cslow, cfast, h, flags);
+    md_unit<1>(d, u, a0, dt, h, flags);
 }
 
 __global__ void __launch_bounds__(256) k_md_bonds(const MdDesc *__restrict__ D) {
@@ -212,7 +375,7 @@ __global__ void __launch_bounds__(256) k_md_energy(const MdDesc *__restrict__ D)
     for (int t = threadIdx.x; t < d.nbond; t += blockDim.x) s += d.bpe[t];
   } else {
     for (int a = threadIdx.x; a < d.natoms; a += blockDim.x) {
-      const double m = 1.0 / d.invm[a];
+      const double m = d.mass[a];
       if (k == 0) s += 0.5 * m * (d.vx[a] * d.vx[a] + d.vy[a] * d.vy[a] + d.vz[a] * d.vz[a]);
       else if (k == 2) s += m * d.vx[a];
       else if (k == 3) s += m * d.vy[a];
@@ -238,7 +401,7 @@ __device__ __forceinline__ void md_relax_unit(const MdDesc &d, int u, int a0, do
 #pragma unroll
   for (int k = 0; k < NA; ++k) {
     const int a = a0 + k;
-    im[k] = gmem(d.invm)[a];
+    im[k] = 1.0 / gmem(d.mass)[a];
     p[k] = make_double3(gmem(d.x)[a], gmem(d.y)[a], gmem(d.z)[a]);
     double fx = gmem(d.fx)[a] + gmem(d.bfx)[a], fy = gmem(d.fy)[a] + gmem(d.bfy)[a],
            fz = gmem(d.fz)[a] + gmem(d.bfz)[a];
@@ -254,7 +417,7 @@ __device__ __forceinline__ void md_relax_unit(const MdDesc &d, int u, int a0, do
     dv[k] = make_double3(0.0, 0.0, 0.0);
   }
   if constexpr (NA == 3)
-    if (!shake3(x, p, im, d.ud2 + 3 * (size_t)u, 1.0, dv, SHAKE_TOL, CONSTRAINT_MAXIT)) atomicAdd(d.fail, 1);
+    if (!shake_mim3(x, p, im, d.ud2 + 3 * (size_t)u, 0.0, dv)) atomicAdd(d.fail, 1);
 #pragma unroll
   for (int k = 0; k < NA; ++k) {
     const int a = a0 + k;
@@ -310,11 +473,10 @@ static void md_dims(const MdDesc *h, int nsys, int &max_units, int &max_atoms, i
   }
 }
 
-void launch_md_units(const MdDesc *d, const MdDesc *h, int nsys, double cslow, double cfast, double hdt, int flags,
-                     hipStream_t st) {
+void launch_md_units(const MdDesc *d, const MdDesc *h, int nsys, double dt, double hin, int flags, hipStream_t st) {
   int mu, ma, mb;
   md_dims(h, nsys, mu, ma, mb);
-  if (mu > 0) hipLaunchKernelGGL(k_md_units, dim3((mu + 255) / 256, nsys), dim3(256), 0, st, d, cslow, cfast, hdt, flags);
+  if (mu > 0) hipLaunchKernelGGL(k_md_units, dim3((mu + 255) / 256, nsys), dim3(256), 0, st, d, dt, hin, flags);
 }
 
 void launch_md_fast_forces(const MdDesc *d, const MdDesc *h, int nsys, hipStream_t st) {

@@ -1027,3 +1027,179 @@ def ref_fnbgrp_alchemy(s, ingrp, typ, ewald, lambdaq):
                       _ri(1))
     return dict(uconf_slt=e[0].value, uconf_slv=e[1].value, ucoul_slt=e[2].value, ucoul_slv=e[3].value,
                 f=np.stack([fpx, fpy, fpz]))
+
+
+# ---------------------------------------------------------------- constraints and the integrator step
+
+def constraint_tables(first, lens, mass):
+    """ORAC's constraint arrays for rigid 3-site units: first (1-based first
+    atoms), lens (n, 3) lengths of (0,1), (0,2), (1,2).  Returns cnst (2, 3n)
+    1-based, dss (squared lengths), coeff (bcnstp.f:79-80) and cnst_protl
+    (per molecule [3, k, k+1, k+2])."""
+    first = np.asarray(first, np.int64)
+    n = len(first)
+    pairs = ((0, 1), (0, 2), (1, 2))
+    cnst = np.zeros((2, 3 * n), np.int32)
+    for c, (i, j) in enumerate(pairs):
+        cnst[0, c::3] = first + i
+        cnst[1, c::3] = first + j
+    dss = (np.asarray(lens, np.float64).reshape(n, 3) ** 2).ravel()
+    m = np.asarray(mass, np.float64)
+    ma, mb = m[cnst[0] - 1], m[cnst[1] - 1]
+    coeff = 2.0 * (ma + mb) / (ma * mb)
+    prot = np.zeros((n, 4), np.int32)
+    prot[:, 0] = 3
+    prot[:, 1:] = 1 + np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    return cnst, dss, coeff, prot.ravel()
+
+
+def _cons_args(x, v, cnst, dss, coeff, mass, prot):
+    keep = [np.array(a, dtype=np.float64, order="C") for a in (x[0], x[1], x[2], v[0], v[1], v[2])]
+    tabs = [_i(cnst), _f(dss), _f(coeff), _f(mass), _i(prot)]
+    return keep, tabs
+
+
+def rattle_verlet(dt, x1, x0, v, cnst, dss, coeff, mass, prot, mim_lim=20):
+    """rattle_verlet.f: positions x1 (3, n) moved onto the constraints along
+    the bonds of x0, velocities v corrected by the displacement / dt.
+    Returns (x1, v, dnit increment, status)."""
+    keep, (c, d, cf, m, p) = _cons_args(x1, v, cnst, dss, coeff, mass, prot)
+    x0 = [_f(a) for a in x0]
+    dnit = C.c_double(0.0)
+    n = len(keep[0])
+    rc = lib().orc_rattle_verlet(C.c_double(dt), *[_p(a) for a in keep[:3]], *[_p(a) for a in x0],
+                                 *[_p(a) for a in keep[3:]], C.c_int(n), _p(c), _p(d), _p(cf), C.c_int(len(d)),
+                                 _p(m), C.c_int(_nprot(p)),
+                                 _p(p), C.c_int(mim_lim), C.byref(dnit))
+    return np.stack(keep[:3]), np.stack(keep[3:]), dnit.value, rc
+
+
+def _nprot(prot):
+    """Molecules in a cnst_protl array."""
+    n, k = 0, 0
+    while k < len(prot):
+        k += int(prot[k]) + 1
+        n += 1
+    return n
+
+
+def rattle_correc(x, v, cnst, dss, coeff, mass, prot, mim_lim=20):
+    """rattle_correc.f: velocities v made orthogonal to the bonds at x.
+    Returns (v, dnit increment, status)."""
+    keep, (c, d, cf, m, p) = _cons_args(x, v, cnst, dss, coeff, mass, prot)
+    dnit = C.c_double(0.0)
+    n = len(keep[0])
+    rc = lib().orc_rattle_correc(C.c_double(0.0), *[_p(a) for a in keep[:3]], *[_p(a) for a in keep[3:]], C.c_int(n),
+                                 _p(c), _p(d), _p(cf), C.c_int(len(d)), _p(m), C.c_int(_nprot(p)), _p(p),
+                                 C.c_int(mim_lim), C.byref(dnit))
+    return np.stack(keep[3:]), dnit.value, rc
+
+
+def ref_rattle_verlet(dt, x1, x0, v, cnst, dss, coeff, mass, prot, mim_lim=20):
+    """The reference's rattle_verlet (compiled in place), same arguments."""
+    R = ref()
+    keep, (c, d, cf, m, p) = _cons_args(x1, v, cnst, dss, coeff, mass, prot)
+    x0 = [_f(a) for a in x0]
+    dnit, iret = C.c_double(0.0), C.c_int(0)
+    err = C.create_string_buffer(80)
+    n = len(keep[0])
+    R.rattle_verlet_(C.byref(C.c_double(dt)), *[_p(a) for a in keep[:3]], *[_p(a) for a in x0],
+                     *[_p(a) for a in keep[3:]], _ri(n), _p(c), _p(d), _p(cf), _ri(len(d)), _p(m), C.byref(dnit),
+                     _ri(_nprot(p)), _p(p), _ri(mim_lim), C.byref(iret), err, C.c_size_t(80))
+    return np.stack(keep[:3]), np.stack(keep[3:]), dnit.value, iret.value
+
+
+def ref_rattle_correc(x, v, cnst, dss, coeff, mass, prot, mim_lim=20):
+    R = ref()
+    keep, (c, d, cf, m, p) = _cons_args(x, v, cnst, dss, coeff, mass, prot)
+    dnit, iret = C.c_double(0.0), C.c_int(0)
+    err = C.create_string_buffer(80)
+    n = len(keep[0])
+    R.rattle_correc_(C.byref(C.c_double(0.0)), *[_p(a) for a in keep[:3]], *[_p(a) for a in keep[3:]], _ri(n),
+                     _p(c), _p(d), _p(cf), _ri(len(d)), _p(m), C.byref(dnit), _ri(_nprot(p)), _p(p), _ri(mim_lim),
+                     C.byref(iret), err, C.c_size_t(80))
+    return np.stack(keep[3:]), dnit.value, iret.value
+
+
+def correc(v, f, mass, dt):
+    """correc.f:1-46: v += f*0.5*dt/mass (a half kick), in the reference's
+    association order."""
+    return v + f * 0.5 * dt / mass
+
+
+def verlet(x, v, f, mass, dt):
+    """verlet.f:1-52: x += dt*v + f*0.5*dt^2/mass, v += f*0.5*dt/mass."""
+    ts2 = dt * dt
+    return x + dt * v + f * 0.5 * ts2 / mass, v + f * 0.5 * dt / mass
+
+
+def ref_correc(v, f, mass, dt):
+    R = ref()
+    n = v.shape[1]
+    vv = [np.array(a, dtype=np.float64) for a in v]
+    ff = [_f(a) for a in f]
+    R.correc_(*[_p(a) for a in vv], *[_p(a) for a in ff], _p(_f(mass)), _ri(n), C.byref(C.c_double(dt)))
+    return np.stack(vv)
+
+
+def ref_verlet(x, v, f, mass, dt):
+    R = ref()
+    n = x.shape[1]
+    xx = [np.array(a, dtype=np.float64) for a in x]
+    vv = [np.array(a, dtype=np.float64) for a in v]
+    ff = [_f(a) for a in f]
+    R.verlet_(_p(_f(mass)), _ri(n), *[_p(a) for a in xx], *[_p(a) for a in vv], *[_p(a) for a in ff],
+              C.byref(C.c_double(dt)))
+    return np.stack(xx), np.stack(vv)
+
+
+def md_steps(x, v, mass, nsteps, dt, ninner, slow, fast, cons):
+    """The device integrator's step, restated: two levels of ORAC's r-RESPA
+    (mtsmd.f), the slow m level (full nonbonded forces, once per dt) around
+    ninner fast n0 substeps of h = dt/ninner, in mtsmd.f's order:
+
+        correc(Fs, dt); rattle_correc                       (:2598-2604)
+        ninner times: verlet(Ff, h); rattle_verlet(h)       (:2663-2683)
+                      Ff = fast(x); correc(Ff, h); rattle_correc  (:2843-2884)
+        Fs = slow(x); correc(Fs, dt); rattle_correc         (:3030-3146)
+
+    x, v (3, n); slow(x) and fast(x) return (3, n) forces at Cartesian
+    positions x; cons = constraint_tables(...).  Returns (x, v, failures)."""
+    cnst, dss, coeff, prot = cons
+    h = dt / ninner
+    fs, ff = slow(x), fast(x)
+    fails = 0
+    for _ in range(nsteps):
+        v = correc(v, fs, mass, dt)
+        v, _, rc = rattle_correc(x, v, cnst, dss, coeff, mass, prot)
+        fails += rc != 0
+        for _ in range(ninner):
+            xo = x
+            x, v = verlet(x, v, ff, mass, h)
+            x, v, _, rc = rattle_verlet(h, x, xo, v, cnst, dss, coeff, mass, prot)
+            fails += rc != 0
+            ff = fast(x)
+            v = correc(v, ff, mass, h)
+            v, _, rc = rattle_correc(x, v, cnst, dss, coeff, mass, prot)
+            fails += rc != 0
+        fs = slow(x)
+        v = correc(v, fs, mass, dt)
+        v, _, rc = rattle_correc(x, v, cnst, dss, coeff, mass, prot)
+        fails += rc != 0
+    return x, v, fails
+
+
+def harmonic_forces(x, pairs, k, r0):
+    """Forces of E = k (r - r0)^2 over 1-based pairs (2, m) at Cartesian x
+    (3, n): the device integrator's fast terms."""
+    f = np.zeros_like(x)
+    if pairs.shape[1] == 0:
+        return f
+    i, j = pairs[0] - 1, pairs[1] - 1
+    d = x[:, i] - x[:, j]
+    r = np.sqrt((d * d).sum(axis=0))
+    c = -2.0 * k * (r - r0) / r
+    for a in range(3):
+        np.add.at(f[a], i, c * d[a])
+        np.add.at(f[a], j, -c * d[a])
+    return f

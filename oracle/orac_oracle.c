@@ -2545,3 +2545,472 @@ void orc_fnb14_alchemy(const int32_t *ss_index, const double *xp0, const double 
   for (int k = 0; k < 2; ++k) dw = dw + uconf[k] + ucoul[k] - (uconf0[k] + ucoul0[k]);
   *dwrk = dw;
 }
+
+/* ------------------------------------------------------------------ */
+/* Constraints: SHAKE / RATTLE as ORAC applies them in its integrator. */
+/* rattle_verlet.f (positions after a drift) and rattle_correc.f       */
+/* (velocities after a kick).  Molecules of more than mim_lim          */
+/* constraints iterate bond by bond; the others take the matrix-       */
+/* inversion method (MIM), closed-form for up to 3 constraints (the    */
+/* reference calls LAPACK dgesv beyond that; this restatement returns  */
+/* 2 there).  cnst(2, ncnstr) is 1-based; cnst_protl holds, molecule   */
+/* by molecule, the count of its constraints followed by their 1-based */
+/* indices.                                                            */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  double b11, b12, b13, b21, b22, b23, b31, b32, b33, det;
+} mim_inv;
+
+/* rattle_verlet.f:366-397 (cnstp = 1, 2, 3 forms of the first solve) */
+static void mim_first_solve(int cnstp, double mat[3][3], mim_inv *r, double *gam) {
+  if (cnstp == 1) {
+    gam[0] = gam[0] / mat[0][0];
+  } else if (cnstp == 2) {
+    r->b11 = mat[1][1];
+    r->b12 = -mat[1][0];
+    r->b21 = -mat[0][1];
+    r->b22 = mat[0][0];
+    r->det = r->b11 * r->b22 - r->b21 * r->b12;
+    r->det = 1.0 / r->det;
+    double xd = r->b11 * gam[0] + r->b21 * gam[1];
+    double yd = r->b12 * gam[0] + r->b22 * gam[1];
+    gam[0] = xd * r->det;
+    gam[1] = yd * r->det;
+  } else {
+    double a11 = mat[0][0], a12 = mat[0][1], a13 = mat[0][2], a21 = mat[1][0], a22 = mat[1][1], a23 = mat[1][2],
+           a31 = mat[2][0], a32 = mat[2][1], a33 = mat[2][2];
+    r->b11 = a22 * a33 - a32 * a23;
+    r->b12 = -a21 * a33 + a31 * a23;
+    r->b13 = a21 * a32 - a22 * a31;
+    r->b21 = -a12 * a33 + a32 * a13;
+    r->b22 = a11 * a33 - a31 * a13;
+    r->b23 = -a11 * a32 + a31 * a12;
+    r->b31 = a12 * a23 - a22 * a13;
+    r->b32 = -a11 * a23 + a21 * a13;
+    r->b33 = a11 * a22 - a21 * a12;
+    r->det = a11 * r->b11 + a21 * r->b21 + a31 * r->b31;
+    r->det = 1.0 / r->det;
+    double xd = r->b11 * gam[0] + r->b21 * gam[1] + r->b31 * gam[2];
+    double yd = r->b12 * gam[0] + r->b22 * gam[1] + r->b32 * gam[2];
+    double zd = r->b13 * gam[0] + r->b23 * gam[1] + r->b33 * gam[2];
+    gam[0] = xd * r->det;
+    gam[1] = yd * r->det;
+    gam[2] = zd * r->det;
+  }
+}
+
+/* rattle_verlet.f:430-444: later solves reuse the cofactors of the first */
+static void mim_resolve(int cnstp, double mat[3][3], const mim_inv *r, double *gam) {
+  if (cnstp == 1) {
+    gam[0] = gam[0] / mat[0][0];
+  } else if (cnstp == 2) {
+    double xd = r->b11 * gam[0] + r->b21 * gam[1];
+    double yd = r->b12 * gam[0] + r->b22 * gam[1];
+    gam[0] = xd * r->det;
+    gam[1] = yd * r->det;
+  } else {
+    double xd = r->b11 * gam[0] + r->b21 * gam[1] + r->b31 * gam[2];
+    double yd = r->b12 * gam[0] + r->b22 * gam[1] + r->b32 * gam[2];
+    double zd = r->b13 * gam[0] + r->b23 * gam[1] + r->b33 * gam[2];
+    gam[0] = xd * r->det;
+    gam[1] = yd * r->det;
+    gam[2] = zd * r->det;
+  }
+}
+
+/* direction of atom `at`'s move for constraint (lab, lbb) with bond vector d:
+   rattle_verlet.f:286-303 */
+static void mim_dir(int at, int lab, int lbb, const double *d, double *o) {
+  if (at == lab) {
+    o[0] = -d[0];
+    o[1] = -d[1];
+    o[2] = -d[2];
+  } else if (at == lbb) {
+    o[0] = d[0];
+    o[1] = d[1];
+    o[2] = d[2];
+  } else {
+    o[0] = o[1] = o[2] = 0.0;
+  }
+}
+
+/* The MIM forms store the atom directions (matx) only for constraint pairs
+   that share an atom (rattle_verlet.f:277-331) but read them back two per
+   pair for every pair (:408-424, :514-541); a molecule with two constraints
+   on disjoint atoms would read entries that were never written.  Such
+   molecules are refused (return 3); triangles (rigid 3-site waters), pairs of
+   bonds on a common atom and single constraints are what the path uses. */
+static int mim_pairs_share(int cnstp, const int32_t *kl, const int32_t *cnst) {
+  for (int ka = 0; ka < cnstp; ++ka)
+    for (int kb = 0; kb < cnstp; ++kb) {
+      int k1 = kl[ka] - 1, k2 = kl[kb] - 1;
+      int la = cnst[2 * k1], lb = cnst[2 * k1 + 1], lab = cnst[2 * k2], lbb = cnst[2 * k2 + 1];
+      if (la != lab && la != lbb && lb != lab && lb != lbb) return 0;
+    }
+  return 1;
+}
+
+int orc_rattle_verlet(double dt, double *xp1, double *yp1, double *zp1, const double *xp0, const double *yp0,
+                      const double *zp0, double *vpx, double *vpy, double *vpz, int nato, const int32_t *cnst,
+                      const double *dss, const double *coeff, int ncnstr, const double *mass, int nprot,
+                      const int32_t *cnst_protl, int mim_lim, double *dnit) {
+  const double tol = 1.0e-8, tol_mim = 1.0e-7; /* rattle_verlet.f:77 */
+  if (ncnstr == 0) return 0;
+  if (mim_lim > 20) return 1; /* :87-92 (nmat = 20) */
+  double dti = dt != 0.0 ? 1.0 / dt : 0.0; /* :111-115 */
+  double *mass0 = malloc(sizeof(double) * (size_t)nato);
+  char *mask = malloc((size_t)nato);
+  double *x0 = malloc(sizeof(double) * 3 * (size_t)ncnstr);
+  for (int i = 0; i < nato; ++i) { /* :117-120 */
+    mass0[i] = 1.0 / mass[i];
+    mask[i] = 1;
+  }
+  for (int k = 0; k < ncnstr; ++k) { /* :127-136 old bond vectors */
+    int la = cnst[2 * k] - 1, lb = cnst[2 * k + 1] - 1;
+    x0[3 * k] = xp0[la] - xp0[lb];
+    x0[3 * k + 1] = yp0[la] - yp0[lb];
+    x0[3 * k + 2] = zp0[la] - zp0[lb];
+  }
+  int rc = 0, iter1 = 0, count = 0;
+  for (int i = 0; i < nprot && rc == 0; ++i) {
+    const int cnstp = cnst_protl[count];
+    const int32_t *kl = cnst_protl + count + 1;
+    if (cnstp > mim_lim) {
+      /* bond-by-bond SHAKE, :157-220 */
+      int iter = 0;
+      for (;;) {
+        int iox = 0;
+        for (int ka = 0; ka < cnstp; ++ka) {
+          int k = kl[ka] - 1, la = cnst[2 * k] - 1, lb = cnst[2 * k + 1] - 1;
+          double dpx = xp1[la] - xp1[lb], dpy = yp1[la] - yp1[lb], dpz = zp1[la] - zp1[lb];
+          double dpp = dpx * dpx + dpy * dpy + dpz * dpz;
+          double dcnt = dpp - dss[k];
+          if (fabs(dcnt) >= tol) {
+            iox = 1;
+            double xk = x0[3 * k], yk = x0[3 * k + 1], zk = x0[3 * k + 2];
+            double dps = dpx * xk + dpy * yk + dpz * zk;
+            double gg = dcnt / (dps * coeff[k]);
+            double gg1 = -gg * mass0[la], gg2 = gg * mass0[lb];
+            xp1[la] = xp1[la] + xk * gg1;
+            xp1[lb] = xp1[lb] + xk * gg2;
+            yp1[la] = yp1[la] + yk * gg1;
+            yp1[lb] = yp1[lb] + yk * gg2;
+            zp1[la] = zp1[la] + zk * gg1;
+            zp1[lb] = zp1[lb] + zk * gg2;
+            gg1 = gg1 * dti;
+            gg2 = gg2 * dti;
+            vpx[la] = vpx[la] + xk * gg1;
+            vpx[lb] = vpx[lb] + xk * gg2;
+            vpy[la] = vpy[la] + yk * gg1;
+            vpy[lb] = vpy[lb] + yk * gg2;
+            vpz[la] = vpz[la] + zk * gg1;
+            vpz[lb] = vpz[lb] + zk * gg2;
+          }
+        }
+        if (!iox) break;
+        iter1 += 1;
+        iter += 1;
+        if (iter > 5000) { /* the reference stops the program here */
+          rc = 1;
+          break;
+        }
+      }
+      /* The reference advances its cursor into cnst_protl only in the MIM
+         branch (:562): after a molecule of more than mim_lim constraints the
+         next molecule reads this one's entry again.  Kept. */
+      continue;
+    } else if (cnstp > 3) {
+      rc = 2; /* dgesv / dgetrs in the reference (:398, :446) */
+    } else if (!mim_pairs_share(cnstp, kl, cnst)) {
+      rc = 3; /* see mim_pairs_share */
+    } else {
+      /* matrix-inversion method, :228-560 */
+      int redo, nredo = 0;
+      do {
+        redo = 0;
+        double gam[3] = {0, 0, 0}, gamo[3], dd[3], xc[3][3], mat[3][3], matx[18][3];
+        int la = 0, lb = 0;
+        for (int ka = 0; ka < cnstp; ++ka) { /* :238-251 */
+          int k1 = kl[ka] - 1, a = cnst[2 * k1] - 1, b = cnst[2 * k1 + 1] - 1;
+          xc[ka][0] = xp1[a] - xp1[b];
+          xc[ka][1] = yp1[a] - yp1[b];
+          xc[ka][2] = zp1[a] - zp1[b];
+        }
+        int count1 = 0;
+        for (int ka = 0; ka < cnstp; ++ka) { /* :257-336 */
+          int k1 = kl[ka] - 1;
+          la = cnst[2 * k1] - 1;
+          lb = cnst[2 * k1 + 1] - 1;
+          dd[ka] = dss[k1] - (xc[ka][0] * xc[ka][0] + xc[ka][1] * xc[ka][1] + xc[ka][2] * xc[ka][2]);
+          for (int kb = 0; kb < cnstp; ++kb) {
+            int k2 = kl[kb] - 1, lab = cnst[2 * k2] - 1, lbb = cnst[2 * k2 + 1] - 1;
+            if (la != lab && la != lbb && lb != lab && lb != lbb) {
+              mat[ka][kb] = 0.0;
+            } else {
+              double dpa[3], dpb[3];
+              mim_dir(la, lab, lbb, x0 + 3 * k2, dpa);
+              mim_dir(lb, lab, lbb, x0 + 3 * k2, dpb);
+              double dpx = -dpa[0], dpy = -dpa[1], dpz = -dpa[2];
+              matx[count1][0] = dpa[0];
+              matx[count1][1] = dpa[1];
+              matx[count1][2] = dpa[2];
+              count1++;
+              double aux3 = (dpx * xc[ka][0] + dpy * xc[ka][1] + dpz * xc[ka][2]) * mass0[la];
+              matx[count1][0] = dpb[0];
+              matx[count1][1] = dpb[1];
+              matx[count1][2] = dpb[2];
+              count1++;
+              aux3 = aux3 + (dpb[0] * xc[ka][0] + dpb[1] * xc[ka][1] + dpb[2] * xc[ka][2]) * mass0[lb];
+              aux3 = aux3 * 4.0;
+              mat[ka][kb] = -aux3;
+            }
+          }
+        }
+        /* Non-linear iteration, :342-508.  The quadratic term weights every
+           constraint's displacement with the masses of the LAST constraint's
+           atoms (la, lb as the loop above leaves them, :408-409); kept. */
+        mim_inv inv = {0};
+        int iox = 1, iter = 0;
+        while (iox == 1) {
+          for (int ka = 0; ka < cnstp; ++ka) gamo[ka] = gam[ka];
+          if (iter == 0) {
+            for (int ka = 0; ka < cnstp; ++ka) gam[ka] = dd[ka];
+            mim_first_solve(cnstp, mat, &inv, gam);
+          } else {
+            double rhs[3];
+            int c1 = 0;
+            for (int ka = 0; ka < cnstp; ++ka) {
+              double xd = 0.0, yd = 0.0, zd = 0.0;
+              double gg1 = mass0[la] * 2.0, gg2 = mass0[lb] * 2.0;
+              for (int kb = 0; kb < cnstp; ++kb) {
+                double aux = gam[kb] * gg1;
+                xd = xd + matx[c1][0] * aux;
+                yd = yd + matx[c1][1] * aux;
+                zd = zd + matx[c1][2] * aux;
+                c1++;
+                double aux1 = -matx[c1][0], aux2 = -matx[c1][1], aux3 = -matx[c1][2];
+                aux = gam[kb] * gg2;
+                xd = xd + aux1 * aux;
+                yd = yd + aux2 * aux;
+                zd = zd + aux3 * aux;
+                c1++;
+              }
+              rhs[ka] = dd[ka] - (xd * xd + yd * yd + zd * zd);
+            }
+            for (int ka = 0; ka < cnstp; ++ka) gam[ka] = rhs[ka];
+            mim_resolve(cnstp, mat, &inv, gam);
+          }
+          iter = iter + 1;
+          if (iter > 120) redo = 1; /* :461-496: correct with what there is, then start over */
+          iox = 0;
+          for (int ka = 0; ka < cnstp; ++ka)
+            if (gam[ka] != 0.0) {
+              double aux = fabs((gam[ka] - gamo[ka]) / gam[ka]);
+              if (aux > tol_mim) iox = 1;
+            }
+          if (redo) iox = 0;
+        }
+        iter1 += iter;
+        /* corrected positions and velocities, :514-558 */
+        count1 = 0;
+        for (int ka = 0; ka < cnstp; ++ka) {
+          int k1 = kl[ka] - 1, a = cnst[2 * k1] - 1, b = cnst[2 * k1 + 1] - 1;
+          double xd = mass0[a] * 2.0, yd = mass0[b] * 2.0;
+          for (int kb = 0; kb < cnstp; ++kb) {
+            if (mask[a]) {
+              double aux = gam[kb] * xd;
+              xp1[a] = xp1[a] + matx[count1][0] * aux;
+              yp1[a] = yp1[a] + matx[count1][1] * aux;
+              zp1[a] = zp1[a] + matx[count1][2] * aux;
+              aux = aux * dti;
+              vpx[a] = vpx[a] + matx[count1][0] * aux;
+              vpy[a] = vpy[a] + matx[count1][1] * aux;
+              vpz[a] = vpz[a] + matx[count1][2] * aux;
+            }
+            count1++;
+            if (mask[b]) {
+              double aux = gam[kb] * yd;
+              xp1[b] = xp1[b] + matx[count1][0] * aux;
+              yp1[b] = yp1[b] + matx[count1][1] * aux;
+              zp1[b] = zp1[b] + matx[count1][2] * aux;
+              aux = aux * dti;
+              vpx[b] = vpx[b] + matx[count1][0] * aux;
+              vpy[b] = vpy[b] + matx[count1][1] * aux;
+              vpz[b] = vpz[b] + matx[count1][2] * aux;
+            }
+            count1++;
+          }
+          mask[a] = 0;
+          mask[b] = 0;
+        }
+        if (redo && ++nredo > 1000) { /* the reference starts over without bound */
+          rc = 1;
+          break;
+        }
+        if (redo) /* :546-557 */
+          for (int ka = 0; ka < cnstp; ++ka) {
+            int k1 = kl[ka] - 1;
+            mask[cnst[2 * k1] - 1] = 1;
+            mask[cnst[2 * k1 + 1] - 1] = 1;
+          }
+      } while (redo);
+    }
+    count = count + cnstp + 1;
+  }
+  if (dnit) *dnit = *dnit + (double)iter1 / (double)nprot;
+  free(mass0);
+  free(mask);
+  free(x0);
+  return rc;
+}
+
+int orc_rattle_correc(double dt, const double *xp0, const double *yp0, const double *zp0, double *vpx, double *vpy,
+                      double *vpz, int nato, const int32_t *cnst, const double *dss, const double *coeff, int ncnstr,
+                      const double *mass, int nprot, const int32_t *cnst_protl, int mim_lim, double *dnit) {
+  (void)dt;
+  const double tolh = 1.0e-6; /* rattle_correc.f:75,91 */
+  double *mass0 = malloc(sizeof(double) * (size_t)(nato > 0 ? nato : 1));
+  char *mask = malloc((size_t)(nato > 0 ? nato : 1));
+  double *bv = malloc(sizeof(double) * 3 * (size_t)(ncnstr > 0 ? ncnstr : 1));
+  double *coeffi = malloc(sizeof(double) * (size_t)(ncnstr > 0 ? ncnstr : 1));
+  double *dssi = malloc(sizeof(double) * (size_t)(ncnstr > 0 ? ncnstr : 1));
+  for (int i = 0; i < nato; ++i) { /* :92-95 */
+    mass0[i] = 1.0 / mass[i];
+    mask[i] = 1;
+  }
+  for (int k = 0; k < ncnstr; ++k) { /* :96-106 bond vectors at the current positions */
+    int la = cnst[2 * k] - 1, lb = cnst[2 * k + 1] - 1;
+    bv[3 * k] = xp0[la] - xp0[lb];
+    bv[3 * k + 1] = yp0[la] - yp0[lb];
+    bv[3 * k + 2] = zp0[la] - zp0[lb];
+    coeffi[k] = 1.0 / coeff[k];
+    dssi[k] = 1.0 / dss[k];
+  }
+  int rc = 0, iter1 = 0, count = 0;
+  for (int i = 0; i < nprot && rc == 0; ++i) {
+    const int cnstp = cnst_protl[count];
+    const int32_t *kl = cnst_protl + count + 1;
+    if (cnstp > mim_lim) {
+      /* bond-by-bond RATTLE, :125-172 */
+      int iter = 0;
+      for (;;) {
+        int iox = 0;
+        for (int ka = 0; ka < cnstp; ++ka) {
+          int k = kl[ka] - 1, la = cnst[2 * k] - 1, lb = cnst[2 * k + 1] - 1;
+          double dpx = bv[3 * k], dpy = bv[3 * k + 1], dpz = bv[3 * k + 2];
+          double vdpx = vpx[la] - vpx[lb], vdpy = vpy[la] - vpy[lb], vdpz = vpz[la] - vpz[lb];
+          double dpp = dpx * vdpx + dpy * vdpy + dpz * vdpz;
+          if (fabs(dpp) > tolh) {
+            double gab = -dpp * dssi[k] * coeffi[k];
+            iox = 1;
+            double gg1 = mass0[la] * gab, gg2 = -mass0[lb] * gab;
+            vpx[la] = vpx[la] + gg1 * dpx;
+            vpx[lb] = vpx[lb] + gg2 * dpx;
+            vpy[la] = vpy[la] + gg1 * dpy;
+            vpy[lb] = vpy[lb] + gg2 * dpy;
+            vpz[la] = vpz[la] + gg1 * dpz;
+            vpz[lb] = vpz[lb] + gg2 * dpz;
+          }
+        }
+        if (!iox) break;
+        iter1 += 1;
+        iter += 1;
+        if (iter > 5000) {
+          rc = 1;
+          break;
+        }
+      }
+    } else if (cnstp > 3) {
+      rc = 2; /* dgesv in the reference (:339) */
+    } else if (!mim_pairs_share(cnstp, kl, cnst)) {
+      rc = 3;
+    } else {
+      /* matrix-inversion method, :176-385 (a linear system: no iteration) */
+      double xc[3][3], dd[3], mat[3][3], matx[18][3], gam[3];
+      for (int ka = 0; ka < cnstp; ++ka) { /* :179-190 relative velocities */
+        int k1 = kl[ka] - 1, a = cnst[2 * k1] - 1, b = cnst[2 * k1 + 1] - 1;
+        xc[ka][0] = vpx[a] - vpx[b];
+        xc[ka][1] = vpy[a] - vpy[b];
+        xc[ka][2] = vpz[a] - vpz[b];
+      }
+      int count1 = 0;
+      for (int ka = 0; ka < cnstp; ++ka) { /* :196-272 */
+        int k1 = kl[ka] - 1, la = cnst[2 * k1] - 1, lb = cnst[2 * k1 + 1] - 1;
+        const double *r1 = bv + 3 * k1;
+        dd[ka] = xc[ka][0] * r1[0] + xc[ka][1] * r1[1] + xc[ka][2] * r1[2];
+        for (int kb = 0; kb < cnstp; ++kb) {
+          int k2 = kl[kb] - 1, lab = cnst[2 * k2] - 1, lbb = cnst[2 * k2 + 1] - 1;
+          if (la != lab && la != lbb && lb != lab && lb != lbb) {
+            mat[ka][kb] = 0.0;
+          } else {
+            double dpa[3], dpb[3];
+            mim_dir(la, lab, lbb, bv + 3 * k2, dpa);
+            mim_dir(lb, lab, lbb, bv + 3 * k2, dpb);
+            double dpx = -dpa[0], dpy = -dpa[1], dpz = -dpa[2];
+            matx[count1][0] = dpa[0];
+            matx[count1][1] = dpa[1];
+            matx[count1][2] = dpa[2];
+            count1++;
+            double aux3 = (r1[0] * dpx + r1[1] * dpy + r1[2] * dpz) * mass0[la];
+            matx[count1][0] = dpb[0];
+            matx[count1][1] = dpb[1];
+            matx[count1][2] = dpb[2];
+            count1++;
+            aux3 = aux3 + (r1[0] * dpb[0] + r1[1] * dpb[1] + r1[2] * dpb[2]) * mass0[lb];
+            aux3 = aux3 * 2.0;
+            mat[ka][kb] = -aux3;
+          }
+        }
+      }
+      for (int ka = 0; ka < cnstp; ++ka) gam[ka] = -dd[ka]; /* :278-281 */
+      if (cnstp == 2) {
+        /* rattle_correc.f:294-309 forms its 2x2 cofactors from named entries */
+        double a11 = mat[0][0], a12 = mat[0][1], a21 = mat[1][0], a22 = mat[1][1];
+        double b11 = a22, b12 = -a21, b21 = -a12, b22 = a11;
+        double det = a11 * a22 - a21 * a12;
+        det = 1.0 / det;
+        double xd = b11 * gam[0] + b21 * gam[1];
+        double yd = b12 * gam[0] + b22 * gam[1];
+        gam[0] = xd * det;
+        gam[1] = yd * det;
+      } else {
+        mim_inv inv;
+        mim_first_solve(cnstp, mat, &inv, gam);
+      }
+      count1 = 0; /* :355-383 */
+      for (int ka = 0; ka < cnstp; ++ka) {
+        int k1 = kl[ka] - 1, a = cnst[2 * k1] - 1, b = cnst[2 * k1 + 1] - 1;
+        double xd = mass0[a] * 2.0, yd = mass0[b] * 2.0;
+        for (int kb = 0; kb < cnstp; ++kb) {
+          if (mask[a]) {
+            double aux = gam[kb];
+            double aux1 = matx[count1][0] * aux, aux2 = matx[count1][1] * aux, aux3 = matx[count1][2] * aux;
+            vpx[a] = vpx[a] + aux1 * xd;
+            vpy[a] = vpy[a] + aux2 * xd;
+            vpz[a] = vpz[a] + aux3 * xd;
+          }
+          count1++;
+          if (mask[b]) {
+            double aux = gam[kb];
+            double aux1 = matx[count1][0] * aux, aux2 = matx[count1][1] * aux, aux3 = matx[count1][2] * aux;
+            vpx[b] = vpx[b] + aux1 * yd;
+            vpy[b] = vpy[b] + aux2 * yd;
+            vpz[b] = vpz[b] + aux3 * yd;
+          }
+          count1++;
+        }
+        mask[a] = 0;
+        mask[b] = 0;
+      }
+    }
+    count = count + cnstp + 1;
+  }
+  if (dnit) *dnit = *dnit + (double)iter1 / (double)nprot;
+  free(mass0);
+  free(mask);
+  free(bv);
+  free(coeffi);
+  free(dssi);
+  return rc;
+}

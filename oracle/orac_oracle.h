@@ -124,10 +124,6 @@ void orc_fnbgrp(const int32_t *ss_index, const double *xp0, const double *yp0, c
                 int32_t ewald, double alphal, const int32_t *ingrp, int ingrpp, double *fpx, double *fpy,
                 double *fpz, double *uconf_slt, double *uconf_slv, double *ucoul_slt, double *ucoul_slv);
 
-#ifdef __cplusplus
-}
-#endif
-#endif
 
 /* alchemy (force_alchemy.f, force_alchemy_shifted.f:732, ferrf_alchemy.f, cself_alchemy.f) */
 void orc_alch_set(double *fact, double ladd, double lrm, int intramol, int ljintra, int qq);
@@ -193,3 +189,23 @@ void orc_fnb14_alchemy(const int32_t *ss_index, const double *xp0, const double 
                        double lj_fudge, double *fpx, double *fpy, double *fpz, double *uconf_slt, double *uconf_slv,
                        double *ucoul_slt, double *ucoul_slv, const double *lambda, const double *lambdaq,
                        const double *lambda0, const double *lambdaq0, double *dwrk);
+/* rattle_verlet.f:1-598 (SHAKE after a drift: xp1 corrected towards the
+   constraints with displacements along the bonds at xp0, velocities by the
+   displacement / dt) and rattle_correc.f:1-414 (RATTLE: velocities made
+   orthogonal to the bonds at xp0).  cnst(2, ncnstr) 1-based; cnst_protl =
+   per molecule [count, 1-based constraint indices...].  Return 0, or 1
+   (no convergence / mim_lim beyond 20), 2 (a MIM molecule of more than 3
+   constraints: LAPACK in the reference), 3 (a MIM molecule with two
+   constraints on disjoint atoms). */
+int orc_rattle_verlet(double dt, double *xp1, double *yp1, double *zp1, const double *xp0, const double *yp0,
+                      const double *zp0, double *vpx, double *vpy, double *vpz, int nato, const int32_t *cnst,
+                      const double *dss, const double *coeff, int ncnstr, const double *mass, int nprot,
+                      const int32_t *cnst_protl, int mim_lim, double *dnit);
+int orc_rattle_correc(double dt, const double *xp0, const double *yp0, const double *zp0, double *vpx, double *vpy,
+                      double *vpz, int nato, const int32_t *cnst, const double *dss, const double *coeff, int ncnstr,
+                      const double *mass, int nprot, const int32_t *cnst_protl, int mim_lim, double *dnit);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
