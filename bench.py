@@ -145,6 +145,7 @@ def time_batched(ctx, reps, precision, steps, warmup, tune=True):
     b = Batch(ctx, systems)
     # the autotuner times graph replays, so eager runs keep the default tuning
     tuning, tune_ms = (b.autotune(reps=3) if tune and GRAPH else (b.tuning(), None))
+    trace = tune_summary(b.tune_log()) if tune_ms is not None else None
     for _ in range(warmup):
         b.eval(rebuild_list=True, graph=GRAPH)
     ctx.sync()
@@ -159,7 +160,18 @@ def time_batched(ctx, reps, precision, steps, warmup, tune=True):
     b.close()
     for x in systems:
         x.close()
-    return dict(dt=dt, res0=res0, stages=stages, tuning=tuning, tune_ms=tune_ms, device_bytes=mem)
+    return dict(dt=dt, res0=res0, stages=stages, tuning=tuning, tune_ms=tune_ms, device_bytes=mem, trace=trace)
+
+
+def tune_summary(log):
+    """Per tuning knob and value, the fastest candidate's ms per step among
+    those the autotuner timed with that value (e.g. persistent 0 vs 1)."""
+    out = {}
+    for t, ms in log:
+        for k, v in t.items():
+            d = out.setdefault(k, {})
+            d[str(v)] = min(ms, d.get(str(v), float("inf")))
+    return {k: v for k, v in out.items() if len(v) > 1}
 
 
 def prepare_md(ctx, systems, precision, relax=300, equil=200, temperature=300.0, seed=2024):
@@ -177,7 +189,7 @@ def prepare_md(ctx, systems, precision, relax=300, equil=200, temperature=300.0,
     return rss, dyns, b
 
 
-def time_md(ctx, b, nsteps, warmup, dt=2.0, ninner=4, list_every=10):
+def time_md(ctx, b, nsteps, warmup, dt=2.0, ninner=4, list_every=5):
     from orac_hip import md
     md.run_batch(b, warmup, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH)
     ctx.sync()
@@ -187,18 +199,23 @@ def time_md(ctx, b, nsteps, warmup, dt=2.0, ninner=4, list_every=10):
     return time.perf_counter() - t0
 
 
-def nve_drift(ctx, b, dyn, nsteps, dt=2.0, ninner=4, list_every=10):
-    """Total-energy change over nsteps (per ns, kJ/mol) of the first system."""
+def nve_drift(ctx, b, dyn, nsteps, dt=2.0, ninner=4, list_every=5):
+    """Total-energy drift (kJ/mol per ns) of the first system over nsteps: the
+    slope of a linear fit to the total energy sampled every 10 steps."""
     from orac_hip import md, synth
     e0 = dyn.energies()
-    md.run_batch(b, nsteps, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH)
+    tot, t_ns = [e0["total"]], [0.0]
+    for k in range(max(1, nsteps // 10)):
+        md.run_batch(b, 10, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH)
+        tot.append(dyn.energies()["total"])
+        t_ns.append((k + 1) * 10 * dt * 1e-6)
     ctx.sync()
     e1 = dyn.energies()
     kj = synth.EFACT / 1000.0
-    ns = nsteps * dt * 1e-6
-    return dict(kj_mol_per_ns=(e1["total"] - e0["total"]) * kj / ns,
-                kj_mol_per_ns_per_atom=(e1["total"] - e0["total"]) * kj / ns / dyn.s.natoms,
-                rel_per_ns=(e1["total"] - e0["total"]) / abs(e0["total"]) / ns,
+    slope = float(np.polyfit(np.array(t_ns), np.array(tot) * kj, 1)[0])
+    return dict(kj_mol_per_ns=slope,
+                kj_mol_per_ns_per_atom=slope / dyn.s.natoms, energy_sd_kj_mol=float(np.std(np.array(tot) * kj)),
+                rel_per_ns=slope / (abs(e0["total"]) * kj),
                 temperature=(e0["temperature"], e1["temperature"]), steps=nsteps, fail=e1["fail"])
 
 
@@ -216,7 +233,7 @@ def _served_physics(reqs):
                                                   "max_abs": float(np.nanmax(np.abs(drift)))})
 
 
-def serving_leg(ctx, nreq=64, rate=200.0, nsteps=500, budget_gib=16, seed=4242, equil=100):
+def serving_leg(ctx, nreq=64, rate=200.0, nsteps=500, budget_gib=16, seed=4242, equil=400, tuning=None):
     """Poisson stream of MD segment requests (solutes log-uniform in 2k-150k
     atoms), each relaxed and equilibrated at 300 K before it is submitted;
     the HBM budget is small enough that requests queue for memory."""
@@ -228,7 +245,7 @@ def serving_leg(ctx, nreq=64, rate=200.0, nsteps=500, budget_gib=16, seed=4242, 
                           nsteps=nsteps, arrival=float(arrivals[k]), seed=seed + k) for k, n in enumerate(sizes)]
     log(f"serving: {nreq} requests, {sum(r.system.natoms for r in reqs)} atoms in all; relaxing and equilibrating")
     serve.relax_requests(ctx, reqs, niter=200, equilibrate=equil)
-    srv = serve.Server(ctx, hbm_budget=budget_gib << 30, chunk=25)
+    srv = serve.Server(ctx, hbm_budget=budget_gib << 30, chunk=25, tuning=tuning)
     srv.progress = log
     st = srv.run(reqs)
     waits = np.array([r.admitted - r.arrival for r in reqs])
@@ -239,7 +256,7 @@ def serving_leg(ctx, nreq=64, rate=200.0, nsteps=500, budget_gib=16, seed=4242, 
     return st
 
 
-def served_hrem_leg(ctx, reps, nsteps=500, rate=200.0, seed=77, equil=100):
+def served_hrem_leg(ctx, reps, nsteps=500, rate=200.0, seed=77, equil=400, tuning=None):
     """The replicas of the batched leg as served requests: Poisson arrivals,
     continuous batching in mixed precision, each request equilibrated at 300 K
     beforehand; aggregate ns/day and p50/p99 of the segment latency."""
@@ -249,7 +266,7 @@ def served_hrem_leg(ctx, reps, nsteps=500, rate=200.0, seed=77, equil=100):
     reqs = [serve.Request(rid=k, system=s, nsteps=nsteps, arrival=float(arrivals[k]), seed=seed + k)
             for k, s in enumerate(reps)]
     serve.relax_requests(ctx, reqs, niter=300, equilibrate=equil)
-    srv = serve.Server(ctx, hbm_budget=64 << 30, chunk=50)
+    srv = serve.Server(ctx, hbm_budget=64 << 30, chunk=50, tuning=tuning)
     st = srv.run(reqs)
     st.update(replicas=len(reps), steps_per_request=nsteps, rate_per_s=rate,
               fail=sum(r.energies["fail"] for r in reqs), **_served_physics(reqs))
@@ -436,6 +453,29 @@ def cpu_reference(s, corr, budget_s=20.0):
     return nsample / dt, nsample, f
 
 
+def cpu_reference_omp(s, corr, nthr, budget_s=8.0):
+    """The reference's own OpenMP build (oracle/_ref/libref_omp.so: -D_OMP_
+    -fopenmp, ORAC's per-thread lists and contact maps) on nthr threads, on
+    the same evaluation: list, force, fft_pme (fftpack), ferrf; as many
+    evaluations as fit in budget_s (at least one)."""
+    import oracle as O
+    if not O.ref_omp_available():
+        return None
+    f, done, secs = O.ref_omp_evaluation(s, corr[0], nthr, nsample=1000, budget_s=budget_s)
+    return done / secs, done, f
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def relrms(a, b):
     return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
 
@@ -554,14 +594,16 @@ def main():
         served = None
         served_hrem = None
         if args.serve_hrem and args.replicas > 0 and args.md_steps > 0:
-            served_hrem = served_hrem_leg(ctx, reps, nsteps=args.serve_steps)
+            tun = dyn["batched_mixed"]["tuning"] if dyn and "batched_mixed" in dyn else None
+            served_hrem = served_hrem_leg(ctx, reps, nsteps=args.serve_steps, tuning=tun)
             log(f"served HREM x{args.replicas}: p50 {served_hrem['p50_ms']:.1f} ms p99 {served_hrem['p99_ms']:.1f} ms, "
                 f"{served_hrem['aggregate_ns_per_day']:.1f} ns/day")
             if dyn:
                 served_hrem["speedup_vs_plain_fp64_ns_per_day"] = (served_hrem["aggregate_ns_per_day"]
                                                                    / dyn["fp64"]["ns_per_day"])
         if args.serve_requests > 0:
-            served = serving_leg(ctx, nreq=args.serve_requests, nsteps=args.serve_steps)
+            tun = dyn["batched_mixed"]["tuning"] if dyn and "batched_mixed" in dyn else None
+            served = serving_leg(ctx, nreq=args.serve_requests, nsteps=args.serve_steps, tuning=tun)
             log(f"serving: p50 {served['p50_ms']:.1f} ms p99 {served['p99_ms']:.1f} ms, "
                 f"{served['aggregate_ns_per_day']:.1f} ns/day")
 
@@ -619,15 +661,17 @@ def main():
             "fp64_evals_per_s": R * K / bd["dt"], "fp64_ms_per_step": 1e3 * bd["dt"] / K,
             "speedup_mixed_batched_vs_plain_fp64": agg / evals64,
             "force_relrms_mixed_vs_f64": batched["err"], "force_relrms_batched_f64_vs_alone": batched["err64"],
-            "tuning": bm["tuning"], "stage_ms_mixed": rnd(bm["stages"]), "device_bytes": bm["device_bytes"]}
+            "tuning": bm["tuning"], "tuning_candidates_best_ms": bm["trace"],
+            "stage_ms_mixed": rnd(bm["stages"]), "device_bytes": bm["device_bytes"]}
     if c0:
         out["water216"] = c0
     if alch:
         out["alchemy"] = alch
     if dyn:
-        dyn["workload"] = ("NVE r-RESPA: 2 fs outer step, 4 inner steps for the solute harmonic terms, rigid waters "
-                           "(SHAKE/RATTLE), list every 10 steps; after 200 steepest-descent steps and "
-                           "Maxwell-Boltzmann velocities at 300 K")
+        dyn["workload"] = ("NVE r-RESPA in ORAC's step order: 2 fs outer step, 4 inner steps for the solute harmonic "
+                           "terms, rigid waters (ORAC's matrix-inversion SHAKE/RATTLE), list every 5 steps; after 300 "
+                           "steepest-descent steps and 200 steps of velocity rescaling at 300 K from "
+                           "Maxwell-Boltzmann velocities")
         out["dynamics"] = dyn
     if served_hrem:
         served_hrem["workload"] = (f"the {args.replicas} HREM replicas of the batched leg as served MD segment "
@@ -640,12 +684,29 @@ def main():
                               "continuous batching with HBM-budget admission")
         out["serving"] = served
     if not args.no_cpu:
+        # the host's cores, at most 16 (the reference's OpenMP build allows 16 threads, omp_integr mcor3)
         nthr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        host = {"cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
+                "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
         v, nsample, cref = cpu_baseline(s, corr, nthr)
-        out["cpu_baseline"] = {"value": v, "unit": "force evals/s", "cores": nthr, "kind": "port",
-                               "sample": f"{nsample} full FP64 evaluations (list + force + fft_pme + ferrf) of the "
-                                         f"same 24,460-atom system, OpenMP C oracle",
-                               "gpu_vs_cpu_force_relrms": relrms(f64, cref["f"])}
+        port = {"value": v, "unit": "force evals/s", "cores": nthr, "kind": "port",
+                "sample": f"{nsample} full FP64 evaluations (list + force + fft_pme + ferrf) of the "
+                          f"same 24,460-atom system, OpenMP C oracle",
+                "gpu_vs_cpu_force_relrms": relrms(f64, cref["f"])}
+        omp = cpu_reference_omp(s, corr, nthr)
+        if omp:
+            out["cpu_baseline"] = {
+                "value": omp[0], "unit": "force evals/s", "cores": nthr, "kind": "reference",
+                "sample": f"{omp[1]} full FP64 evaluations (lc_index + lc_list + force + fft_pme + ferrf) of the same "
+                          f"24,460-atom system by the reference's own OpenMP build (-D_OMP_ -fopenmp, compiled in "
+                          f"place; fftpack PME: FFTW is not in the image) on {nthr} threads",
+                "gpu_vs_reference_force_relrms": relrms(f64, omp[2]), "host": host, "port": port}
+            if nthr > 8:
+                o8 = cpu_reference_omp(s, corr, 8)
+                out["cpu_baseline"]["reference_8_threads"] = {"value": o8[0], "unit": "force evals/s", "cores": 8,
+                                                              "kind": "reference", "sample": f"{o8[1]} evaluations"}
+        else:
+            out["cpu_baseline"] = dict(port, host=host)
         ref = cpu_reference(s, corr)
         if ref:
             out["cpu_baseline"]["reference_serial"] = {

@@ -484,6 +484,11 @@ int orac_hip_batch_get_tuning(orac_hip_ctx *ctx, int32_t batch, int32_t *tuning,
 /* Times the batch's step for the tuning candidates (reps evaluations each)
  * and keeps the fastest; best_ms (may be null) receives its time. */
 int orac_hip_batch_autotune(orac_hip_ctx *ctx, int32_t batch, int32_t reps, int32_t rebuild_list, double *best_ms);
+/* Every candidate the batch's last autotune timed, in order (the starting
+ * tuning first): tunings[k*6 .. k*6+5] in the order of
+ * orac_hip_batch_get_tuning and ms[k] its time per step, for k < min(cap, *n);
+ * *n receives the number of candidates. */
+int orac_hip_batch_tune_log(orac_hip_ctx *ctx, int32_t batch, int32_t *tunings, double *ms, int32_t cap, int32_t *n);
 
 /* ---- dynamics (no reference counterpart: the caller side of the path) ------ */
 /* On-device r-RESPA NVE integration in the scheme of ORAC's multiple-time-step

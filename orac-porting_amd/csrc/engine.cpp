@@ -88,6 +88,7 @@ struct Batch {
   std::vector<int> gens;  // system generations the descriptor copies were made from
   int precision = 0;
   Tuning tu;
+  std::vector<std::pair<Tuning, double>> tune_log;  // every candidate the last autotune timed, ms per step
   BufSet B;
   std::vector<PmeDesc> hp;
   std::vector<PairDesc> hq;
@@ -1426,6 +1427,7 @@ extern "C" int orac_hip_batch_autotune(orac_hip_ctx *ctx, int32_t batch, int32_t
     Tuning best = b.tu;
     batch_upload(ctx, b);
     double tbest = batch_time(ctx, b, r, rebuild);
+    b.tune_log.assign(1, {best, tbest});
     const std::vector<int> cands[6] = {{256, 512, 1024}, {2, 3, 4}, {1, 2, 0}, {1, 0}, {0, 1}, {0, 1}};
     for (int sweep = 0; sweep < 2; ++sweep) {
       for (int param = 0; param < 6; ++param) {
@@ -1440,6 +1442,7 @@ extern "C" int orac_hip_batch_autotune(orac_hip_ctx *ctx, int32_t batch, int32_t
           b.tu = t;
           batch_upload(ctx, b);
           const double tm = batch_time(ctx, b, r, rebuild);
+          b.tune_log.push_back({t, tm});
           if (tm < tbest) {
             tbest = tm;
             best = t;
@@ -1451,6 +1454,19 @@ extern "C" int orac_hip_batch_autotune(orac_hip_ctx *ctx, int32_t batch, int32_t
     batch_upload(ctx, b);
     b.profile = prof;
     if (best_ms) *best_ms = tbest;
+  });
+}
+
+extern "C" int orac_hip_batch_tune_log(orac_hip_ctx *ctx, int32_t batch, int32_t *tunings, double *ms, int32_t cap,
+                                       int32_t *n) {
+  return guarded([&] {
+    ORAC_REQUIRE(n && cap >= 0 && (cap == 0 || (tunings && ms)), E_ARG, "null argument");
+    const Batch &b = get_batch(ctx, batch);
+    *n = (int32_t)b.tune_log.size();
+    for (int k = 0; k < cap && k < *n; ++k) {
+      tuning_to(b.tune_log[k].first, tunings + 6 * (size_t)k, 6);
+      ms[k] = b.tune_log[k].second;
+    }
   });
 }
 

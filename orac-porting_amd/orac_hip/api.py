@@ -62,7 +62,7 @@ EXPORTS = [
     "orac_hip_system_results", "orac_hip_system_profile", "orac_hip_system_stage_ms", "orac_hip_system_bytes",
     "orac_hip_sync", "orac_hip_stream", "orac_hip_batch_create", "orac_hip_batch_destroy", "orac_hip_batch_eval",
     "orac_hip_batch_profile", "orac_hip_batch_stage_ms", "orac_hip_batch_set_tuning", "orac_hip_batch_get_tuning",
-    "orac_hip_batch_autotune", "orac_hip_md_setup", "orac_hip_md_set_state", "orac_hip_md_get_state",
+    "orac_hip_batch_autotune", "orac_hip_batch_tune_log", "orac_hip_md_setup", "orac_hip_md_set_state", "orac_hip_md_get_state",
     "orac_hip_batch_md_run", "orac_hip_batch_md_relax", "orac_hip_batch_md_scale_velocities",
     "orac_hip_system_md_energies", "orac_hip_system_set_alchemy", "orac_hip_system_set_lambda",
     "orac_hip_system_alchemy_results", "orac_hip_ferrf_alchemy_add", "orac_hip_fnb14_alchemy",
@@ -662,6 +662,15 @@ class Batch:
         cur.update(kw)
         t = np.array([cur[k] for k in TUNING_FIELDS], np.int32)
         _check(lib().orac_hip_batch_set_tuning(self.ctx._h, self.id, _p(t), C.c_int32(len(t))))
+
+    def tune_log(self):
+        """Every candidate the last autotune timed: a list of (tuning dict, ms per step)."""
+        n = C.c_int32()
+        _check(lib().orac_hip_batch_tune_log(self.ctx._h, self.id, None, None, C.c_int32(0), C.byref(n)))
+        t = np.zeros((max(1, n.value), len(TUNING_FIELDS)), np.int32)
+        ms = np.zeros(max(1, n.value))
+        _check(lib().orac_hip_batch_tune_log(self.ctx._h, self.id, _p(t), _p(ms), C.c_int32(n.value), C.byref(n)))
+        return [(dict(zip(TUNING_FIELDS, (int(v) for v in t[k]))), float(ms[k])) for k in range(n.value)]
 
     def autotune(self, reps=5, rebuild_list=True):
         """Pick the fastest tuning by timing the batch; returns (tuning, ms per eval)."""

@@ -104,7 +104,7 @@ class Dynamics:
                     temperature=2.0 * ke / (self.dof * KB_INTERNAL), momentum=out[2:5], fail=fail.value)
 
 
-def run_batch(batch, nsteps, dt=2.0, ninner=4, list_every=10, graph=True):
+def run_batch(batch, nsteps, dt=2.0, ninner=4, list_every=5, graph=True):
     _check(lib().orac_hip_batch_md_run(batch.ctx._h, batch.id, C.c_int32(nsteps), C.c_double(dt),
                                        C.c_int32(ninner), C.c_int32(list_every), C.c_int32(1 if graph else 0)))
 
@@ -119,7 +119,7 @@ def scale_velocities(batch, scale):
     _check(lib().orac_hip_batch_md_scale_velocities(batch.ctx._h, batch.id, _p(sc)))
 
 
-def equilibrate(batch, dyns, temperature, nsteps, every=10, dt=2.0, ninner=4, list_every=10, graph=True):
+def equilibrate(batch, dyns, temperature, nsteps, every=10, dt=2.0, ninner=4, list_every=5, graph=True):
     """Velocity rescaling to temperature every `every` steps (preparation only)."""
     for _ in range(max(1, nsteps // every)):
         run_batch(batch, every, dt=dt, ninner=ninner, list_every=list_every, graph=graph)

@@ -170,11 +170,12 @@ def estimate_bytes(s):
 class Server:
     """Continuous batching of MD segments on one device (see the module doc)."""
 
-    def __init__(self, ctx, hbm_budget, max_active=64, chunk=10, dt=2.0, ninner=4, list_every=10, precision=1,
-                 autotune=False, clock=time.perf_counter, restart_dir=None):
+    def __init__(self, ctx, hbm_budget, max_active=64, chunk=10, dt=2.0, ninner=4, list_every=5, precision=1,
+                 autotune=False, clock=time.perf_counter, restart_dir=None, tuning=None):
         self.ctx, self.budget, self.max_active, self.chunk = ctx, int(hbm_budget), max_active, chunk
         self.dt, self.ninner, self.list_every, self.precision = dt, ninner, list_every, precision
         self.autotune, self.clock = autotune, clock
+        self.tuning = dict(tuning) if tuning else None  # launch knobs every batch takes (Batch.set_tuning)
         self.restart_dir = restart_dir  # finished requests are checkpointed there in ORAC's restart format
         self.log = []
         self.progress = None  # optional callable(str): admission and completion events
@@ -185,6 +186,11 @@ class Server:
         queue, active, used = [], [], 0
         batch = None
         ticks = steps_done = 0
+        # seconds forming batches, integrating (the first tick of a new batch
+        # apart: it also primes the forces of new systems and captures the
+        # step graph), and the rest (admission, results)
+        t_batch = t_steps = t_first = 0.0
+        fresh = False
         t0 = self.clock()
         while pending or queue or active:
             now = self.clock() - t0
@@ -210,14 +216,25 @@ class Server:
                     time.sleep(max(0.0, min(pending[0].arrival - (self.clock() - t0), 0.05)))
                 continue
             if changed or batch is None:
+                tb = self.clock()
                 if batch is not None:
                     batch.close()
                 batch = Batch(self.ctx, [a.rs for a in active])
-                if self.autotune:
+                if self.tuning:
+                    batch.set_tuning(**self.tuning)
+                elif self.autotune:
                     batch.autotune(reps=2, rebuild_list=False)
+                t_batch += self.clock() - tb
+                fresh = True
             n = min(self.chunk, min(a.left for a in active))
+            ts = self.clock()
             _md.run_batch(batch, n, dt=self.dt, ninner=self.ninner, list_every=self.list_every, graph=True)
             self.ctx.sync()
+            if fresh:
+                t_first += self.clock() - ts
+                fresh = False
+            else:
+                t_steps += self.clock() - ts
             ticks += 1
             steps_done += n * len(active)
             self.log.append((self.clock() - t0, len(active), used))
@@ -258,25 +275,47 @@ class Server:
                     p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
                     mean_ms=float(lat.mean() * 1e3), aggregate_ns_per_day=sim_ns / wall * 86400.0,
                     max_active=max((k for _, k, _ in self.log), default=0),
-                    peak_bytes=max((u for _, _, u in self.log), default=0))
+                    peak_bytes=max((u for _, _, u in self.log), default=0),
+                    wall_split_s={"forming_batches": t_batch, "first_tick_of_a_batch": t_first,
+                                  "integrating": t_steps, "other": wall - t_batch - t_first - t_steps})
 
 
-def relax_requests(ctx, requests, niter=300, dmax=0.03, precision=1, equilibrate=0):
+def relax_requests(ctx, requests, niter=300, dmax=0.03, precision=1, equilibrate=0, group_bytes=48 << 30,
+                   list_every=5):
     """Start states for each request (preparation, not served time): the
     generators place atoms without regard to LJ overlap, so niter
     steepest-descent steps first; then, with equilibrate > 0, that many NVE
     steps from Maxwell-Boltzmann velocities at the request's temperature with
     the velocities rescaled to it every 10 steps, and the request starts from
-    the equilibrated positions and velocities."""
+    the equilibrated positions and velocities.  Requests are prepared in
+    batches of at most group_bytes estimated device bytes; a request's start
+    state does not depend on its batch-mates."""
+    groups, cur, used = [], [], 0
     for r in requests:
-        a = _Active(ctx, dataclasses.replace(r, positions=None, velocities=None), precision)
-        b = Batch(ctx, [a.rs])
+        need = estimate_bytes(r.system)
+        if cur and used + need > group_bytes:
+            groups.append(cur)
+            cur, used = [], 0
+        cur.append(r)
+        used += need
+    if cur:
+        groups.append(cur)
+    for g in groups:
+        act = [_Active(ctx, dataclasses.replace(r, positions=None, velocities=None), precision) for r in g]
+        b = Batch(ctx, [a.rs for a in act])
         _md.relax_batch(b, niter, dmax=dmax)
         if equilibrate > 0:
-            a.dyn.maxwell_boltzmann(r.temperature, r.seed)
-            _md.equilibrate(b, [a.dyn], r.temperature, equilibrate)
-            r.positions, r.velocities = a.dyn.state()
-        else:
-            r.positions, _ = a.dyn.state()
+            for a, r in zip(act, g):
+                a.dyn.maxwell_boltzmann(r.temperature, r.seed)
+            temps = {r.temperature for r in g}
+            if len(temps) != 1:
+                raise ValueError("requests prepared together share one temperature")
+            _md.equilibrate(b, [a.dyn for a in act], temps.pop(), equilibrate, list_every=list_every)
+        for a, r in zip(act, g):
+            if equilibrate > 0:
+                r.positions, r.velocities = a.dyn.state()
+            else:
+                r.positions, _ = a.dyn.state()
         b.close()
-        a.close()
+        for a in act:
+            a.close()

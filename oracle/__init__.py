@@ -1203,3 +1203,110 @@ def harmonic_forces(x, pairs, k, r0):
         np.add.at(f[a], i, c * d[a])
         np.add.at(f[a], j, -c * d[a])
     return f
+
+
+# ---------------------------------------------------------------- the reference's OpenMP build (CPU baseline)
+
+_ref_omp = None
+_omp_state = {}
+
+
+def ref_omp_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libref_omp.so"))
+
+
+def ref_omp(nthr):
+    """The reference's hot path as ORAC's OpenMP build compiles it
+    (_ref/libref_omp.so), set up for nthr threads (at most 16, the
+    reference's mcor3).  Returns (library, mpp8, m8t)."""
+    global _ref_omp
+    if not 1 <= nthr <= 16:
+        raise ValueError("the reference's OpenMP build runs 1 to 16 threads (omp_integr mcor3)")
+    if _ref_omp is None:
+        # the per-thread private arrays of force.f need more than the default
+        # worker stack; read when the OpenMP runtime starts
+        os.environ.setdefault("OMP_STACKSIZE", "512M")
+        _ref_omp = C.CDLL(os.path.join(HERE, "_ref", "libref_omp.so"))
+    if _omp_state.get("nthr") != nthr:
+        mpp8, m8t = C.c_int(), C.c_int()
+        _ref_omp.refglue_omp_setup(C.c_int(nthr), C.byref(mpp8), C.byref(m8t))
+        _omp_state.update(nthr=nthr, mpp8=mpp8.value, m8t=m8t.value)
+    return _ref_omp, _omp_state["mpp8"], _omp_state["m8t"]
+
+
+def ref_omp_evaluation(s, corr_pairs, nthr, fa=None, nsample=1, budget_s=None):
+    """One full evaluation (or nsample of them, stopping early after budget_s
+    seconds) through the reference's OpenMP build on nthr threads, in ORAC's
+    layouts: lc_index, lc_list into per-thread lists nnlpp(mpp8, nthr), force
+    with the per-thread contact maps (split_map), fft_pme (fftpack branch)
+    and ferrf over corr_pairs (fudge 1).  Returns (forces (3, N), evaluations
+    done, seconds)."""
+    import time as _t
+    R, mpp8, m8t = ref_omp(nthr)
+    fa = fa or ForceArgs(s)
+    n = s.natoms
+    rc = s.rcut + s.rtol + s.rnei
+    nc = int(s.L // 3.0)
+    # contact maps per thread, once (ORAC splits them at start-up)
+    mapnlt = np.zeros(m8t * nthr, dtype=np.int32)
+    R.refglue_omp_split_map(C.c_int(n), _p(fa.mapnl), _p(fa.grppt), C.c_int(s.ngrp), _p(mapnlt))
+    keep = [_i(s.ss_index), _f(s.xp0), _f(s.yp0), _f(s.zp0), _f(s.charge), _i(s.nbtype), _i(s.atomg),
+            _i(s.atomp), _f(s.co), _f(s.ecc12), _f(s.ecc6), _f(s.pmass), _f(s.mass),
+            np.zeros(s.ngrp, dtype=np.int32)]
+    (ss, x, y, z, q, nbt, atg, atp, co, e12, e6, pm, ms, groupp) = keep
+    nnlpp = np.zeros(mpp8 * nthr, dtype=np.int32)
+    nnlpp1 = np.zeros(mpp8 * nthr, dtype=np.int32)
+    ncount1 = np.zeros(nthr, dtype=np.int32)
+    nind = np.zeros(2, dtype=np.int32)
+    indmax = 2500
+    ii, jj, kk = [np.zeros(2 * indmax, dtype=np.int32) for _ in range(3)]
+    cm = s.molecule_coms()
+    parr = [_f(s.xp0), _f(s.yp0), _f(s.zp0), _f(cm[0]), _f(cm[1]), _f(cm[2]), _f(s.charge), _f(s.co), _f(s.oc)]
+    cx, cy, cz = [_f(a) for a in s.cartesian()]
+    lst = _i(corr_pairs)
+    tab = np.zeros(8)
+    f = {}
+
+    def one():
+        R.lc_index_(_ri(indmax), _ri(nc), _ri(nc), _ri(nc), _p(nind), _p(ii), _p(jj), _p(kk), _rd(rc), _p(co))
+        R.lc_list_(_ri(nc), _ri(nc), _ri(nc), _p(nind), _p(ii), _p(jj), _p(kk), _rd(rc), _p(co), _p(fa.xpg),
+                   _p(fa.ypg), _p(fa.zpg), _ri(s.ngrp), _p(nnlpp), _ri(6))
+        fpx, fpy, fpz = [np.zeros(n) for _ in range(3)]
+        st = np.zeros(9)
+        e = [C.c_double() for _ in range(6)]
+        rti = C.c_double(0.0)
+        R.force_(_p(ss), _p(x), _p(y), _p(z), _p(fa.xpg), _p(fa.ypg), _p(fa.zpg), _p(q), _p(nbt), _p(fa.tt),
+                 _ri(s.ntypes), _ri(n), _p(atg), _p(fa.xpcm), _p(fa.ypcm), _p(fa.zpcm), _p(groupp), _p(atp),
+                 _p(co), _p(e12), _p(e6), _rd(s.alpha), _p(mapnlt), _ri(s.ngrp), _p(fa.grppt),
+                 *[C.byref(v) for v in e], _p(fpx), _p(fpy), _p(fpz), _p(st), _p(nnlpp), _p(nnlpp1), _rd(0.0),
+                 _rd(0.0), _rd(s.rcut), C.byref(rti), _rd(s.rtol), _p(pm), _ri(1), _p(ncount1), _p(ms),
+                 _rd(1.0e20), _ri(0), _ri(1))
+        gx, gy, gz = [np.zeros(n) for _ in range(3)]
+        st2 = np.zeros(9)
+        eer = C.c_double()
+        R.refglue_omp_pme_init(C.c_int(n), C.c_int(s.nfft[0]), C.c_int(s.nfft[1]), C.c_int(s.nfft[2]),
+                               C.c_int(s.order))
+        R.fft_pme_(_ri(n), *[_p(a) for a in parr[:7]], _p(parr[7]), _p(parr[8]), _rd(s.volume), _rd(s.alpha),
+                   _ri(s.order), _ri(s.nfft[0]), _ri(s.nfft[1]), _ri(s.nfft[2]), C.byref(eer), _p(gx), _p(gy),
+                   _p(gz), _p(st2), _p(atp), _p(fa.grppt), _ri(1), _rd(1e9))
+        hx, hy, hz = [np.zeros(n) for _ in range(3)]
+        a, b = C.c_double(), C.c_double()
+        if lst.size:
+            R.ferrf_(_p(ss), _rd(s.alpha), _p(q), _rd(1.0), _p(cx), _p(cy), _p(cz), _ri(1), _p(lst),
+                     _ri(lst.size // 2), C.byref(a), C.byref(b), _p(hx), _p(hy), _p(hz), _ri(0), _p(tab), _rd(0.0),
+                     _rd(0.0), _ri(0), _ri(n))
+        f["f"] = np.stack([fpx + gx + hx, fpy + gy + hy, fpz + gz + hz])
+
+    def run():
+        R.refglue_omp_threads(C.c_int(nthr))  # the team size is a setting of the calling thread
+        t0 = _t.perf_counter()
+        done = 0
+        while done < nsample:
+            one()
+            done += 1
+            if budget_s is not None and _t.perf_counter() - t0 > budget_s:
+                break
+        return done, _t.perf_counter() - t0
+
+    done, secs = _big_stack(run)
+    return f["f"], done, secs

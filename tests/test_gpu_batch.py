@@ -112,6 +112,10 @@ def test_autotune_keeps_forces(gpu_ctx, mixed_sizes):
     ref = b.results()
     tun, ms = b.autotune(reps=2)
     assert ms > 0 and tun["lds_elems"] in (256, 512, 1024) and tun["pair_waves"] in (0, 2, 3, 4)
+    # the log holds every timed candidate, the kept one the fastest of them
+    log = b.tune_log()
+    assert len(log) > 6 and {t["persistent"] for t, _ in log} == {0, 1}
+    assert min(m for _, m in log) == ms and (tun, ms) in log
     for t in ({"lds_elems": 256, "pair_waves": 2, "spread_tiled": 0, "xcd_remap": 0, "cell_order": 1, "persistent": 1},
               {"lds_elems": 1024, "pair_waves": 4, "spread_tiled": 1, "xcd_remap": 1, "cell_order": 0, "persistent": 0},
               {"lds_elems": 512, "pair_waves": 3, "spread_tiled": 2, "xcd_remap": 1, "cell_order": 1, "persistent": 1},

@@ -152,3 +152,24 @@ def test_ferrf_exact(small_solute, which):
 def test_cself_exact(small_solute):
     for kcut in (1e9, 3.0):
         assert O.cself(small_solute, kcut) == O.ref_cself(small_solute, kcut)
+
+
+@pytest.mark.skipif(not O.ref_omp_available(), reason="oracle/_ref/libref_omp.so not built (no reference tree)")
+def test_reference_openmp_build_matches_serial():
+    """The reference's OpenMP build (ORAC's per-thread lists and contact
+    maps, timed as the CPU baseline) gives the serial build's forces on the
+    24,460-atom system, to the rounding of its thread reductions."""
+    from orac_hip import synth
+    s = synth.config("solvated24k")
+    corr = np.concatenate([s.bonds, s.bends], axis=1)
+    fa = O.ForceArgs(s)
+    rc = s.rcut + s.rtol + s.rnei
+    nc = int(s.L // 3.0)
+    st = O.ref_lc_index(nc, nc, nc, rc, s.co)
+    nn = O.ref_lc_list(nc, nc, nc, st, rc, s.co, fa.xpg, fa.ypg, fa.zpg)
+    d = O.ref_force(s, nn, rinn=0.0, rout=s.rcut, rtolinn=0.0, rtolout=s.rtol, rneigh=0.0, fa=fa)
+    fr = d["f"] + O.ref_fft_pme(s)["f"] + O.ref_ferrf(s, corr, 1, 1.0)["f"]
+    for nthr in (1, 3):
+        f, done, _ = O.ref_omp_evaluation(s, corr, nthr, fa=fa)
+        assert done == 1
+        assert np.sqrt(((f - fr) ** 2).sum() / (fr ** 2).sum()) < 1e-13
