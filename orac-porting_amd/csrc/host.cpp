@@ -1,5 +1,6 @@
 // host.cpp -- host-side tables and plans of the nonbonded path: FFT plans and
 // twiddles, B-spline moduli, linked-cell stencils, exclusion maps, PME descriptors.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 
@@ -105,17 +106,43 @@ bool lc_index(int indmax, int ncx, int ncy, int ncz, int32_t *nind, int32_t *ind
   double sqcut = ctoff * ctoff;
   double dx = 2.0 / ncx, dy = 2.0 / ncy, dz = 2.0 / ncz;
   int imax = 0, jmax = 0, kmax = 0;
+  // Offsets that cannot be in the stencil are not visited (the reference
+  // visits every offset; dist_ijk is ~1,600 distance evaluations, so large
+  // boxes spent seconds here).  Two points of cells 0 and (i, j, k) differ by
+  // at least (|i| - 1) dx in the first reduced coordinate, so their distance
+  // is at least (|i| - 1) dx / |row 1 of co^-1|; dist_ijk measures distances
+  // between points of the two cells, never less than that.  An offset is
+  // skipped only when even |i| - 2 cells of separation reach the cut-off, so
+  // the stencil and its order are the reference's.
+  int reach[3];
+  {
+    const double *c = co;  // column-major 3x3
+    const double a = c[0], b = c[3], cc = c[6], d = c[1], e = c[4], f = c[7], g = c[2], h = c[5], k = c[8];
+    const double det = a * (e * k - f * h) - b * (d * k - f * g) + cc * (d * h - e * g);
+    // rows of the inverse, up to the factor 1/det
+    const double r[3][3] = {{e * k - f * h, cc * h - b * k, b * f - cc * e},
+                            {f * g - d * k, a * k - cc * g, cc * d - a * f},
+                            {d * h - e * g, b * g - a * h, a * e - b * d}};
+    const double cell[3] = {dx, dy, dz};
+    const int nmax[3] = {ncx, ncy, ncz};
+    for (int ax = 0; ax < 3; ++ax) {
+      const double nrm = std::sqrt(r[ax][0] * r[ax][0] + r[ax][1] * r[ax][1] + r[ax][2] * r[ax][2]) / std::fabs(det);
+      const double w = cell[ax] / nrm;  // distance between the planes one cell apart
+      reach[ax] = nmax[ax];
+      if (std::isfinite(w) && w > 0.0) reach[ax] = std::min(nmax[ax], (int)std::ceil(ctoff / w) + 2);
+    }
+  }
   nind[0] = 1;
   indxi[0] = indxj[0] = indxk[0] = 0;
   nind[1] = 1;
   indxi[1] = indxj[1] = indxk[1] = 0;
   for (int n = 1; n <= 2; ++n) {
     int istart = n == 1 ? 0 : 1 - ncx;
-    for (int i = istart; i <= ncx - 1; ++i) {
+    for (int i = std::max(istart, -reach[0]); i <= std::min(ncx - 1, reach[0]); ++i) {
       int jstart = (n == 1 && i == 0) ? 0 : 1 - ncy;
-      for (int j = jstart; j <= ncy - 1; ++j) {
+      for (int j = std::max(jstart, -reach[1]); j <= std::min(ncy - 1, reach[1]); ++j) {
         int kstart = (n == 1 && i == 0 && j == 0) ? 0 : 1 - ncz;
-        for (int k = kstart; k <= ncz - 1; ++k) {
+        for (int k = std::max(kstart, -reach[2]); k <= std::min(ncz - 1, reach[2]); ++k) {
           if (dist_ijk(i, j, k, dx, dy, dz, co) < sqcut) {
             nind[n - 1] += 1;
             if (nind[n - 1] > indmax) return false;

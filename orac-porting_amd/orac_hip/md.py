@@ -24,11 +24,11 @@ KCAL_MOL = 4184.0 / synth.EFACT                    # 1 kcal/mol in internal ener
 def rigid_waters(s):
     """First atoms (1-based) and lengths (O-H1, O-H2, H1-H2) of the rigid waters."""
     ss = np.asarray(s.ss_index)
-    solv_groups = [g for g in range(s.ngrp) if ss[s.grppt[0, g] - 1] == 2]
-    first = np.array([s.grppt[0, g] for g in solv_groups], np.int32)
-    for g in solv_groups:
-        if s.grppt[1, g] - s.grppt[0, g] != 2:
-            raise ValueError("solvent groups must be 3-site waters")
+    gp = np.asarray(s.grppt)[:, : s.ngrp]
+    solv = ss[gp[0] - 1] == 2
+    first = gp[0, solv].astype(np.int32)
+    if np.any(gp[1, solv] - gp[0, solv] != 2):
+        raise ValueError("solvent groups must be 3-site waters")
     half = np.radians(synth.ANG_HOH) / 2.0
     lens = np.tile([synth.R_OH, synth.R_OH, 2.0 * synth.R_OH * np.sin(half)], (len(first), 1))
     return first, lens

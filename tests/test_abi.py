@@ -50,6 +50,35 @@ def test_host_only_entry_points_without_gpu():
     assert got == g["decide_ref"].tolist()
 
 
+def test_lc_index_pruned_search_is_the_reference_stencil():
+    """The library's lc_index visits only the cell offsets that can reach the
+    cut-off; its stencil (entries and order) equals the oracle's, which visits
+    every offset as lc_index.f does, on cubic, orthorhombic and triclinic
+    boxes with fine and coarse cells; and a 50-cell box takes milliseconds."""
+    import time
+
+    import numpy as np
+
+    import oracle as O
+    from orac_hip.api import Context
+    boxes = [np.diag([61.0] * 3), np.diag([40.0, 55.0, 70.0]),
+             np.array([[50.0, 8.0, -5.0], [0.0, 47.0, 6.0], [0.0, 0.0, 52.0]])]
+    for co in boxes:
+        co = np.asfortranarray(co)
+        for nc in ((12, 14, 16), (17, 17, 17), (9, 11, 13)):
+            for rc in (9.6, 10.6, 12.0):
+                try:
+                    o = O.lc_index(*nc, rc, co)
+                except Exception:  # the oracle's own STOP (a cell visited twice)
+                    continue
+                g = Context.lc_index(2500, *nc, rc, co)
+                assert all(np.array_equal(a, b) for a, b in zip(g, o)), (nc, rc)
+    co = np.asfortranarray(np.diag([180.0] * 3))
+    t = time.perf_counter()
+    Context.lc_index(4096, 51, 51, 51, 10.6, co)
+    assert time.perf_counter() - t < 0.5
+
+
 def test_error_path_is_a_status_not_an_exception():
     from orac_hip.api import Context, OracHipError
     import numpy as np
