@@ -233,7 +233,7 @@ def _served_physics(reqs):
                                                   "max_abs": float(np.nanmax(np.abs(drift)))})
 
 
-def serving_leg(ctx, nreq=64, rate=200.0, nsteps=500, budget_gib=16, seed=4242, equil=400, tuning=None):
+def serving_leg(ctx, nreq=64, rate=200.0, nsteps=500, budget_gib=16, seed=4242, equil=600, tuning=None):
     """Poisson stream of MD segment requests (solutes log-uniform in 2k-150k
     atoms), each relaxed and equilibrated at 300 K before it is submitted;
     the HBM budget is small enough that requests queue for memory."""
@@ -245,7 +245,7 @@ def serving_leg(ctx, nreq=64, rate=200.0, nsteps=500, budget_gib=16, seed=4242, 
                           nsteps=nsteps, arrival=float(arrivals[k]), seed=seed + k) for k, n in enumerate(sizes)]
     log(f"serving: {nreq} requests, {sum(r.system.natoms for r in reqs)} atoms in all; relaxing and equilibrating")
     serve.relax_requests(ctx, reqs, niter=200, equilibrate=equil)
-    srv = serve.Server(ctx, hbm_budget=budget_gib << 30, chunk=25, tuning=tuning)
+    srv = serve.Server(ctx, hbm_budget=budget_gib << 30, chunk=25, tuning=tuning, min_batch_s=0.1)
     srv.progress = log
     st = srv.run(reqs)
     waits = np.array([r.admitted - r.arrival for r in reqs])
@@ -266,7 +266,7 @@ def served_hrem_leg(ctx, reps, nsteps=500, rate=200.0, seed=77, equil=400, tunin
     reqs = [serve.Request(rid=k, system=s, nsteps=nsteps, arrival=float(arrivals[k]), seed=seed + k)
             for k, s in enumerate(reps)]
     serve.relax_requests(ctx, reqs, niter=300, equilibrate=equil)
-    srv = serve.Server(ctx, hbm_budget=64 << 30, chunk=50, tuning=tuning)
+    srv = serve.Server(ctx, hbm_budget=64 << 30, chunk=50, tuning=tuning, min_batch_s=0.1)
     st = srv.run(reqs)
     st.update(replicas=len(reps), steps_per_request=nsteps, rate_per_s=rate,
               fail=sum(r.energies["fail"] for r in reqs), **_served_physics(reqs))

@@ -171,11 +171,15 @@ class Server:
     """Continuous batching of MD segments on one device (see the module doc)."""
 
     def __init__(self, ctx, hbm_budget, max_active=64, chunk=10, dt=2.0, ninner=4, list_every=5, precision=1,
-                 autotune=False, clock=time.perf_counter, restart_dir=None, tuning=None):
+                 autotune=False, clock=time.perf_counter, restart_dir=None, tuning=None, min_batch_s=0.0):
         self.ctx, self.budget, self.max_active, self.chunk = ctx, int(hbm_budget), max_active, chunk
         self.dt, self.ninner, self.list_every, self.precision = dt, ninner, list_every, precision
         self.autotune, self.clock = autotune, clock
         self.tuning = dict(tuning) if tuning else None  # launch knobs every batch takes (Batch.set_tuning)
+        # a running batch takes newcomers only after it has run this long
+        # (each new batch primes its newcomers' forces and captures a new step
+        # graph; a burst of arrivals then joins in few batches, not one each)
+        self.min_batch_s = float(min_batch_s)
         self.restart_dir = restart_dir  # finished requests are checkpointed there in ORAC's restart format
         self.log = []
         self.progress = None  # optional callable(str): admission and completion events
@@ -191,13 +195,15 @@ class Server:
         # step graph), and the rest (admission, results)
         t_batch = t_steps = t_first = 0.0
         fresh = False
+        formed = 0.0
         t0 = self.clock()
         while pending or queue or active:
             now = self.clock() - t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.pop(0))
             changed = False
-            while queue and len(active) < self.max_active:
+            hold = batch is not None and self.clock() - t0 - formed < self.min_batch_s
+            while queue and len(active) < self.max_active and not hold:
                 need = estimate_bytes(queue[0].system)
                 if active and used + need > self.budget:
                     break  # the head waits for memory (arrival order is kept)
@@ -226,6 +232,7 @@ class Server:
                     batch.autotune(reps=2, rebuild_list=False)
                 t_batch += self.clock() - tb
                 fresh = True
+                formed = self.clock() - t0
             n = min(self.chunk, min(a.left for a in active))
             ts = self.clock()
             _md.run_batch(batch, n, dt=self.dt, ninner=self.ninner, list_every=self.list_every, graph=True)
