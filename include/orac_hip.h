@@ -517,6 +517,22 @@ int orac_hip_md_get_state(orac_hip_ctx *ctx, int32_t sys, double *x, double *y, 
  * invalid from there on; setting its positions resizes the list). */
 int orac_hip_batch_md_run(orac_hip_ctx *ctx, int32_t batch, int32_t nsteps, double dt, int32_t ninner,
                           int32_t list_every, int32_t use_graph);
+/* Two nonbonded shells as ORAC's r-RESPA splits them (mts_forces.f:57-93,
+ * the m shell and the outer one): the inner shell 0 .. rcutm switched out
+ * over rtolm, the outer shell rcutm .. rcut switched in over rtolm with its
+ * real-space Ewald term skipped where decide_force says so
+ * (mts_forces.f:110-115), PME and the corrections with the outer shell.
+ * rcutm <= 0 goes back to one shell.  Evaluations then give both shells'
+ * forces and energies together; not for alchemical, REM or half-list
+ * systems (status 4). */
+int orac_hip_system_set_shells(orac_hip_ctx *ctx, int32_t sys, double rcutm, double rtolm);
+/* nsteps outer steps of dt for a batch whose systems all have two shells:
+ * the outer shell once per dt, the inner shell nmid times per dt (hm =
+ * dt/nmid) and the fast terms ninner times per hm, in mtsmd.f's order of
+ * kicks, drifts and constraints (its m and n0 levels inside h).  Other
+ * arguments and results as orac_hip_batch_md_run. */
+int orac_hip_batch_md_run_shells(orac_hip_ctx *ctx, int32_t batch, int32_t nsteps, double dt, int32_t nmid,
+                                 int32_t ninner, int32_t list_every, int32_t use_graph);
 /* niter steepest-descent steps: every unit moves along its total force by
  * hstep F/m, at most dmax A per atom, rigid units re-constrained; velocities
  * are zeroed. */

@@ -67,6 +67,13 @@ struct System {
   bool md_primed = false;  // slow and fast forces at the current positions are in place
   MdDesc md;
   MdDesc *md_d = nullptr;  // device copy for per-system kernels
+  // two nonbonded shells (orac_hip_system_set_shells, ORAC's m and outer
+  // shells, mts_forces.f:66-93): `pair` is the outer shell (rcutm .. rcut,
+  // switched in over rtolm), pair_m the inner one (0 .. rcutm, switched out
+  // over rtolm) with forces fm* and energies out_m of its own
+  double rcutm = 0.0, rtolm = 0.0;
+  PairDesc pair_m;
+  double *fmx = nullptr, *fmy = nullptr, *fmz = nullptr, *out_m = nullptr;
 };
 
 // Neighbour-side force slots of the half-list form: at least `need`, in three
@@ -94,6 +101,9 @@ struct Batch {
   std::vector<PairDesc> hq;
   std::vector<ListDesc> hl;
   std::vector<CorrDesc> hc;
+  std::vector<PairDesc> hqm;  // inner-shell descriptors (every system has two shells)
+  PairDesc *dqm = nullptr;
+  bool shells = false;
   PmeDesc *dp = nullptr;
   PairDesc *dq = nullptr;
   ListDesc *dl = nullptr;
@@ -112,7 +122,7 @@ struct Batch {
   long md_gen = -1;             // engine md generation the descriptors were built from
   int md_every = 0;             // list update period the md descriptors carry
   double md_dt = 0.0;
-  int md_ninner = 0;
+  int md_ninner = 0, md_nmid = 0;
   hipGraph_t mgraph = nullptr;  // one outer step of every system
   hipGraphExec_t mexec = nullptr;
   void drop_md_graphs() {
@@ -584,6 +594,18 @@ static void batch_upload(orac_hip_ctx *ctx, Batch &b) {
       b.hp.push_back(b.sys[k]->pme0);
       pme_apply_tuning(b.hp.back(), b.tu);
     }
+  b.shells = true;
+  for (System *S : b.sys) b.shells = b.shells && S->rcutm > 0.0;
+  b.hqm.clear();
+  if (b.shells) {
+    b.hqm.resize(n);
+    for (size_t k = 0; k < n; ++k) {
+      b.hqm[k] = b.sys[k]->pair_m;
+      b.hqm[k].perm = b.hq[k].perm;
+      b.hqm[k].soff = b.hq[k].soff;
+    }
+    b.dqm = b.B.put_vec("dqm", b.hqm, st, false);
+  }
   b.dp = b.B.put_vec("dp", b.hp, st, false);
   b.dq = b.B.put_vec("dq", b.hq, st, false);
   b.dl = b.B.put_vec("dl", b.hl, st, false);
@@ -681,6 +703,18 @@ static void batch_enqueue(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool events
   }
   launch_pme_combine(b.dp, b.hp.data(), npme, st);
   mark(ST_END);
+}
+
+// The inner shell of every system (systems with two shells): centres, then the
+// pair kernel on the inner-shell descriptors, on the list of the last build.
+static void batch_enqueue_mid(orac_hip_ctx *ctx, Batch &b) {
+  hipStream_t st = ctx->stream;
+  const int n = (int)b.sys.size();
+  launch_centres(b.dq, b.hq.data(), n, st);
+  if (b.precision == 0)
+    launch_pairs<double>(b.dqm, b.hqm.data(), n, st, b.tu, nullptr, nullptr);
+  else
+    launch_pairs<float>(b.dqm, b.hqm.data(), n, st, b.tu, nullptr, nullptr);
 }
 
 static void batch_eval(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool use_graph) {
@@ -881,6 +915,9 @@ static void md_setup(orac_hip_ctx *ctx, System &S, int nrigid, const int32_t *ri
   d.bfy = B.get<double>("md_bfy", n);
   d.bfz = B.get<double>("md_bfz", n);
   d.mass = S.pair.mass;
+  d.fmx = S.rcutm > 0.0 ? S.fmx : nullptr;
+  d.fmy = S.rcutm > 0.0 ? S.fmy : nullptr;
+  d.fmz = S.rcutm > 0.0 ? S.fmz : nullptr;
   d.ufirst = B.put_vec("md_ufirst", ufirst, st, false);
   d.urigid = B.put_vec("md_urigid", urigid, st, false);
   d.ud2 = B.put_vec("md_ud2", ud2, st, false);
@@ -963,6 +1000,7 @@ static void md_prime(orac_hip_ctx *ctx, Batch &b) {
     md_upload(ctx, p, 1);
     launch_md_fast_forces(p.dm, p.hm.data(), (int)ids.size(), ctx->stream);
     batch_eval(ctx, p, true, false);
+    if (p.shells) batch_enqueue_mid(ctx, p);
     ORAC_HIP(hipStreamSynchronize(ctx->stream));
   } catch (...) {
     E.batches.erase(pid);
@@ -973,26 +1011,51 @@ static void md_prime(orac_hip_ctx *ctx, Batch &b) {
 }
 
 // one outer r-RESPA step of every system in the batch (see md.hip)
-static void md_enqueue_step(orac_hip_ctx *ctx, Batch &b, double dt, int ninner) {
+static void md_enqueue_step(orac_hip_ctx *ctx, Batch &b, double dt, int nmid, int ninner) {
   hipStream_t st = ctx->stream;
   const int n = (int)b.sys.size();
-  const double h = dt / ninner;
-  // slow half kick, then the first fast substep's drift
-  launch_md_units(b.dm, b.hm.data(), n, dt, h, MD_KICK_SLOW | MD_VERLET, st);
-  launch_md_fast_forces(b.dm, b.hm.data(), n, st);
-  // end of one fast substep (half kick), start of the next (drift)
-  for (int k = 1; k < ninner; ++k) {
-    launch_md_units(b.dm, b.hm.data(), n, dt, h, MD_KICK_FAST | MD_VERLET, st);
+  if (nmid <= 0) {
+    const double h = dt / ninner;
+    // slow half kick, then the first fast substep's drift
+    launch_md_units(b.dm, b.hm.data(), n, dt, 0.0, h, MD_KICK_SLOW | MD_VERLET, st);
     launch_md_fast_forces(b.dm, b.hm.data(), n, st);
+    // end of one fast substep (half kick), start of the next (drift)
+    for (int k = 1; k < ninner; ++k) {
+      launch_md_units(b.dm, b.hm.data(), n, dt, 0.0, h, MD_KICK_FAST | MD_VERLET, st);
+      launch_md_fast_forces(b.dm, b.hm.data(), n, st);
+    }
+    batch_enqueue(ctx, b, true, false, true);
+    // end of the last fast substep, then the slow half kick
+    launch_md_units(b.dm, b.hm.data(), n, dt, 0.0, h, MD_KICK_FAST | MD_KICK_SLOW, st);
+  } else {
+    // two nonbonded shells, three levels in mtsmd.f's order: the outer shell
+    // with PME and corrections over dt, nmid steps of the inner shell over
+    // hm = dt/nmid, each with ninner fast substeps over h = hm/ninner
+    const double hm = dt / nmid, h = hm / ninner;
+    launch_md_units(b.dm, b.hm.data(), n, dt, hm, h, MD_KICK_SLOW, st);
+    for (int im = 0; im < nmid; ++im) {
+      launch_md_units(b.dm, b.hm.data(), n, dt, hm, h, MD_KICK_MID | MD_VERLET, st);
+      launch_md_fast_forces(b.dm, b.hm.data(), n, st);
+      for (int k = 1; k < ninner; ++k) {
+        launch_md_units(b.dm, b.hm.data(), n, dt, hm, h, MD_KICK_FAST | MD_VERLET, st);
+        launch_md_fast_forces(b.dm, b.hm.data(), n, st);
+      }
+      launch_md_units(b.dm, b.hm.data(), n, dt, hm, h, MD_KICK_FAST, st);
+      if (im == nmid - 1) batch_enqueue(ctx, b, true, false, true);  // the outer shell at the step's end
+      batch_enqueue_mid(ctx, b);
+      launch_md_units(b.dm, b.hm.data(), n, dt, hm, h, MD_KICK_MID, st);
+    }
+    launch_md_units(b.dm, b.hm.data(), n, dt, hm, h, MD_KICK_SLOW, st);
   }
-  batch_enqueue(ctx, b, true, false, true);
-  // end of the last fast substep, then the slow half kick
-  launch_md_units(b.dm, b.hm.data(), n, dt, h, MD_KICK_FAST | MD_KICK_SLOW, st);
   launch_md_tick(b.dm, n, st);
 }
 
-static void md_run(orac_hip_ctx *ctx, Batch &b, int nsteps, double dt, int ninner, int list_every, bool use_graph) {
-  ORAC_REQUIRE(nsteps >= 0 && dt > 0.0 && ninner >= 1 && list_every >= 1, E_ARG, "bad integration parameters");
+static void md_run(orac_hip_ctx *ctx, Batch &b, int nsteps, double dt, int nmid, int ninner, int list_every,
+                   bool use_graph) {
+  ORAC_REQUIRE(nsteps >= 0 && dt > 0.0 && ninner >= 1 && list_every >= 1 && nmid >= 0, E_ARG,
+               "bad integration parameters");
+  ORAC_REQUIRE(nmid == 0 || b.shells, E_STATE, "an inner-shell level needs two shells on every system");
+  ORAC_REQUIRE(nmid > 0 || !b.shells, E_STATE, "systems with two shells integrate with an inner-shell level");
   md_upload(ctx, b, list_every);
   md_prime(ctx, b);
   // lists built in another layout are rebuilt (and the forces with them)
@@ -1001,11 +1064,13 @@ static void md_run(orac_hip_ctx *ctx, Batch &b, int nsteps, double dt, int ninne
   if (fmt) {
     launch_md_fast_forces(b.dm, b.hm.data(), (int)b.sys.size(), ctx->stream);
     batch_eval(ctx, b, true, false);
+    if (b.shells) batch_enqueue_mid(ctx, b);
   }
-  if (b.md_dt != dt || b.md_ninner != ninner) {
+  if (b.md_dt != dt || b.md_ninner != ninner || b.md_nmid != nmid) {
     b.drop_md_graphs();
     b.md_dt = dt;
     b.md_ninner = ninner;
+    b.md_nmid = nmid;
   }
   hipStream_t st = ctx->stream;
   for (int s = 0; s < nsteps; ++s) {
@@ -1013,7 +1078,7 @@ static void md_run(orac_hip_ctx *ctx, Batch &b, int nsteps, double dt, int ninne
       if (!b.mexec) {
         ORAC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
-          md_enqueue_step(ctx, b, dt, ninner);
+          md_enqueue_step(ctx, b, dt, nmid, ninner);
         } catch (...) {
           hipGraph_t g = nullptr;
           (void)hipStreamEndCapture(st, &g);
@@ -1025,7 +1090,7 @@ static void md_run(orac_hip_ctx *ctx, Batch &b, int nsteps, double dt, int ninne
       }
       ORAC_HIP(hipGraphLaunch(b.mexec, st));
     } else {
-      md_enqueue_step(ctx, b, dt, ninner);
+      md_enqueue_step(ctx, b, dt, nmid, ninner);
     }
   }
   for (System *S : b.sys) S->have_list = true;
@@ -1114,7 +1179,9 @@ extern "C" int orac_hip_system_eval(orac_hip_ctx *ctx, int32_t sys, int32_t rebu
   return guarded([&] {
     set_device(ctx);
     // replayed from the system's own HIP graph (eager while profiling)
-    batch_eval(ctx, get_batch(ctx, solo_batch(ctx, sys)), rebuild_list != 0, true);
+    Batch &b = get_batch(ctx, solo_batch(ctx, sys));
+    batch_eval(ctx, b, rebuild_list != 0, true);
+    if (b.shells) batch_enqueue_mid(ctx, b);
   });
 }
 
@@ -1149,7 +1216,7 @@ extern "C" int orac_hip_system_results(orac_hip_ctx *ctx, int32_t sys, double *f
     set_device(ctx);
     System &S = get_sys(ctx, sys);
     hipStream_t st = ctx->stream;
-    double out[48];
+    double out[48], outm[16];
     int tot[2];
     ORAC_HIP(hipMemcpyAsync(out, S.out, sizeof out, hipMemcpyDeviceToHost, st));
     ORAC_HIP(hipMemcpyAsync(tot, S.list.total, sizeof tot, hipMemcpyDeviceToHost, st));
@@ -1157,7 +1224,26 @@ extern "C" int orac_hip_system_results(orac_hip_ctx *ctx, int32_t sys, double *f
     if (fx) ORAC_HIP(hipMemcpyAsync(fx, S.fx, nb, hipMemcpyDeviceToHost, st));
     if (fy) ORAC_HIP(hipMemcpyAsync(fy, S.fy, nb, hipMemcpyDeviceToHost, st));
     if (fz) ORAC_HIP(hipMemcpyAsync(fz, S.fz, nb, hipMemcpyDeviceToHost, st));
+    std::vector<double> mx, my, mz;
+    const bool two = S.rcutm > 0.0;
+    if (two) {  // the inner shell's forces and energies join the outer ones
+      ORAC_HIP(hipMemcpyAsync(outm, S.out_m, sizeof outm, hipMemcpyDeviceToHost, st));
+      mx.resize(S.natoms);
+      my.resize(S.natoms);
+      mz.resize(S.natoms);
+      ORAC_HIP(hipMemcpyAsync(mx.data(), S.fmx, nb, hipMemcpyDeviceToHost, st));
+      ORAC_HIP(hipMemcpyAsync(my.data(), S.fmy, nb, hipMemcpyDeviceToHost, st));
+      ORAC_HIP(hipMemcpyAsync(mz.data(), S.fmz, nb, hipMemcpyDeviceToHost, st));
+    }
     ORAC_HIP(hipStreamSynchronize(st));
+    if (two) {
+      for (int a = 0; a < S.natoms; ++a) {
+        if (fx) fx[a] += mx[a];
+        if (fy) fy[a] += my[a];
+        if (fz) fz[a] += mz[a];
+      }
+      for (int k = 0; k < 15; ++k) out[20 + k] += outm[k];
+    }
     ORAC_REQUIRE(tot[1] == 0, E_CAPACITY,
                  "neighbour list capacity exceeded (" + std::to_string(tot[0]) + " entries); results are invalid");
     if (energies) {
@@ -1357,7 +1443,9 @@ extern "C" int orac_hip_batch_destroy(orac_hip_ctx *ctx, int32_t batch) {
 extern "C" int orac_hip_batch_eval(orac_hip_ctx *ctx, int32_t batch, int32_t rebuild_list, int32_t use_graph) {
   return guarded([&] {
     set_device(ctx);
-    batch_eval(ctx, get_batch(ctx, batch), rebuild_list != 0, use_graph != 0);
+    Batch &b = get_batch(ctx, batch);
+    batch_eval(ctx, b, rebuild_list != 0, use_graph != 0);
+    if (b.shells) batch_enqueue_mid(ctx, b);
   });
 }
 
@@ -1470,6 +1558,72 @@ extern "C" int orac_hip_batch_tune_log(orac_hip_ctx *ctx, int32_t batch, int32_t
   });
 }
 
+// ================================================================ C ABI: two nonbonded shells
+
+extern "C" int orac_hip_system_set_shells(orac_hip_ctx *ctx, int32_t sys, double rcutm, double rtolm) {
+  return guarded([&] {
+    set_device(ctx);
+    System &S = get_sys(ctx, sys);
+    hipStream_t st = ctx->stream;
+    PairDesc &p = S.pair;
+    if (rcutm <= 0.0) {  // back to one shell: rinn = 0 (mts_forces.f:88-93)
+      S.rcutm = S.rtolm = 0.0;
+      p.r2inn = p.rinn0 = p.r2inn0 = p.arsinn1 = p.arsinn2 = 0.0;
+      p.lskip_ewald = 0;
+    } else {
+      ORAC_REQUIRE(rtolm >= 0.0 && rcutm + rtolm < S.rcut, E_ARG, "the inner shell must end before the cut-off");
+      ORAC_REQUIRE(!S.alch && p.rematom == nullptr && !p.half, E_UNSUPPORTED,
+                   "two shells are built for plain (not alchemical, REM or half-list) systems");
+      S.rcutm = rcutm;
+      S.rtolm = rtolm;
+      // outer shell: rinn = rcutm switched in over rtolm (force.f:145-166); its
+      // real-space Ewald term is skipped where erfc(alpha rcutm) < 1e-3
+      // (decide_force, mts_forces.f:110-115)
+      p.r2inn = rcutm * rcutm;
+      p.rinn0 = rcutm + rtolm;
+      p.r2inn0 = p.rinn0 * p.rinn0;
+      p.arsinn1 = p.rinn0 - 3.0 * rcutm;
+      p.arsinn2 = rtolm * rtolm * rtolm;
+      int32_t skip = 0;
+      ORAC_REQUIRE(orac_hip_decide_force(rcutm, S.alpha, &skip) == 0, E_STATE, "decide_force failed");
+      p.lskip_ewald = skip;
+      // inner shell: rinn = 0, rout = rcutm switched out over rtolm
+      PairDesc &m = S.pair_m;
+      m = p;
+      m.r2inn = m.rinn0 = m.r2inn0 = m.arsinn1 = m.arsinn2 = 0.0;
+      m.r2out = rcutm * rcutm;
+      m.rout0 = rcutm + rtolm;
+      m.r2out0 = m.rout0 * m.rout0;
+      m.arsout1 = m.rout0 - 3.0 * rcutm;
+      m.arsout2 = rtolm * rtolm * rtolm;
+      m.lskip_ewald = 0;
+      if (!S.fmx) {
+        S.fmx = S.B.get<double>("fmx", S.natoms);
+        S.fmy = S.B.get<double>("fmy", S.natoms);
+        S.fmz = S.B.get<double>("fmz", S.natoms);
+        S.out_m = S.B.get<double>("out_m", 16);
+        for (double *q : {S.fmx, S.fmy, S.fmz}) launch_zero_f64(q, S.natoms, st);
+        launch_zero_f64(S.out_m, 16, st);
+      }
+      m.fx = S.fmx;
+      m.fy = S.fmy;
+      m.fz = S.fmz;
+      m.out = S.out_m;
+      m.part = S.B.get<double>("ppart_m", (size_t)m.nblk * 16);
+    }
+    if (S.md_on) {
+      S.md.fmx = S.rcutm > 0.0 ? S.fmx : nullptr;
+      S.md.fmy = S.rcutm > 0.0 ? S.fmy : nullptr;
+      S.md.fmz = S.rcutm > 0.0 ? S.fmz : nullptr;
+      S.md_d = S.B.put("md_desc", &S.md, 1, st, false);
+      S.md_primed = false;
+      engine(ctx).md_gen++;
+    }
+    ORAC_HIP(hipStreamSynchronize(st));
+    S.gen++;  // every batch of the system uploads its descriptors again
+  });
+}
+
 // ================================================================ C ABI: dynamics
 
 extern "C" int orac_hip_md_setup(orac_hip_ctx *ctx, int32_t sys, int32_t nrigid, const int32_t *rigid_first,
@@ -1513,7 +1667,7 @@ extern "C" int orac_hip_md_set_state(orac_hip_ctx *ctx, int32_t sys, const doubl
       ORAC_HIP(hipMemcpyAsync(S.md.vy, vy, nb, hipMemcpyHostToDevice, st));
       ORAC_HIP(hipMemcpyAsync(S.md.vz, vz, nb, hipMemcpyHostToDevice, st));
       // velocities onto the constraint surface of the rigid units (RATTLE)
-      launch_md_units(S.md_d, &S.md, 1, 0.0, 0.0, MD_RATTLE, st);
+      launch_md_units(S.md_d, &S.md, 1, 0.0, 0.0, 0.0, MD_RATTLE, st);
     }
     ORAC_HIP(hipStreamSynchronize(st));
     if (moved) size_list(ctx, sys);
@@ -1540,7 +1694,16 @@ extern "C" int orac_hip_batch_md_run(orac_hip_ctx *ctx, int32_t batch, int32_t n
                                      int32_t list_every, int32_t use_graph) {
   return guarded([&] {
     set_device(ctx);
-    md_run(ctx, get_batch(ctx, batch), nsteps, dt, ninner, list_every, use_graph != 0);
+    md_run(ctx, get_batch(ctx, batch), nsteps, dt, 0, ninner, list_every, use_graph != 0);
+  });
+}
+
+extern "C" int orac_hip_batch_md_run_shells(orac_hip_ctx *ctx, int32_t batch, int32_t nsteps, double dt, int32_t nmid,
+                                            int32_t ninner, int32_t list_every, int32_t use_graph) {
+  return guarded([&] {
+    set_device(ctx);
+    ORAC_REQUIRE(nmid >= 1, E_ARG, "nmid must be at least 1");
+    md_run(ctx, get_batch(ctx, batch), nsteps, dt, nmid, ninner, list_every, use_graph != 0);
   });
 }
 
@@ -1561,6 +1724,7 @@ extern "C" int orac_hip_batch_md_relax(orac_hip_ctx *ctx, int32_t batch, int32_t
       }
       launch_md_fast_forces(b.dm, b.hm.data(), n, ctx->stream);
       batch_eval(ctx, b, it % list_every == 0, false);
+      if (b.shells) batch_enqueue_mid(ctx, b);
       launch_md_relax(b.dm, b.hm.data(), n, hstep, dmax, ctx->stream);
     }
     ORAC_HIP(hipGetLastError());

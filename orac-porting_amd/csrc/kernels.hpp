@@ -282,6 +282,7 @@ struct MdDesc {
   double *xp, *yp, *zp;              // reduced copy read by the force path
   const double *fx, *fy, *fz;        // slow forces (the nonbonded evaluation)
   double *bfx, *bfy, *bfz;           // fast forces (harmonic distance terms)
+  const double *fmx, *fmy, *fmz;     // inner-shell nonbonded forces (two shells; else null)
   const double *mass;                // atom masses
   const int *ufirst;                 // first atom of each unit
   const unsigned char *urigid;       // 1: rigid 3-atom unit
@@ -296,10 +297,12 @@ struct MdDesc {
   int *step;                         // outer steps taken (device counter, drives the list schedule)
 };
 // stages of launch_md_units, applied in this order when combined: half kick
-// by the fast forces over h + RATTLE; half kick by the slow forces over dt +
-// RATTLE; drift over h (ORAC's verlet with the fast forces) + SHAKE; RATTLE
-constexpr int MD_KICK_FAST = 1, MD_KICK_SLOW = 2, MD_VERLET = 4, MD_RATTLE = 8;
-void launch_md_units(const MdDesc *d, const MdDesc *h, int nsys, double dt, double hin, int flags, hipStream_t st);
+// by the fast forces over h + RATTLE; half kick by the inner-shell forces
+// over hm + RATTLE; half kick by the slow forces over dt + RATTLE; drift over
+// h (ORAC's verlet with the fast forces) + SHAKE; RATTLE
+constexpr int MD_KICK_FAST = 1, MD_KICK_SLOW = 2, MD_VERLET = 4, MD_RATTLE = 8, MD_KICK_MID = 16;
+void launch_md_units(const MdDesc *d, const MdDesc *h, int nsys, double dt, double hm, double hin, int flags,
+                     hipStream_t st);
 void launch_md_fast_forces(const MdDesc *d, const MdDesc *h, int nsys, hipStream_t st);
 void launch_md_energy(const MdDesc *d, int nsys, hipStream_t st);
 void launch_md_tick(const MdDesc *d, int nsys, hipStream_t st);

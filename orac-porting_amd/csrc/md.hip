@@ -241,6 +241,7 @@ __device__ inline void rattle_mim3(const double3 (&r)[3], double3 (&v)[3], const
 
 // One unit's share of an integration stage, in ORAC's order (mtsmd.f):
 //   MD_KICK_FAST  v += Ff*0.5*h/m, RATTLE        correc + rattle_correc (:2876-2884)
+//   MD_KICK_MID   v += Fm*0.5*hm/m, RATTLE       the inner nonbonded shell (two shells)
 //   MD_KICK_SLOW  v += Fs*0.5*dt/m, RATTLE       correc + rattle_correc (:2598-2604, :3121-3146)
 //   MD_VERLET     x += h*v + Ff*0.5*h^2/m, v += Ff*0.5*h/m, then SHAKE against the
 //                 positions before the drift      verlet + rattle_verlet (:2663-2683)
@@ -251,7 +252,7 @@ __device__ inline void rattle_mim3(const double3 (&r)[3], double3 (&v)[3], const
 // unit; NA atoms per unit as a template parameter so the unit stays in
 // registers.
 template <int NA>
-__device__ __forceinline__ void md_unit(const MdDesc &d, int u, int a0, double dt, double h, int flags) {
+__device__ __forceinline__ void md_unit(const MdDesc &d, int u, int a0, double dt, double hm, double h, int flags) {
   double3 x[NA], v[NA], ff[NA];
   double m[NA], im[NA];
 #pragma unroll
@@ -269,6 +270,16 @@ __device__ __forceinline__ void md_unit(const MdDesc &d, int u, int a0, double d
       v[k].x = v[k].x + ff[k].x * 0.5 * h / m[k];
       v[k].y = v[k].y + ff[k].y * 0.5 * h / m[k];
       v[k].z = v[k].z + ff[k].z * 0.5 * h / m[k];
+    }
+    if constexpr (NA == 3) rattle_mim3(x, v, im);
+  }
+  if (flags & MD_KICK_MID) {
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+      const int a = a0 + k;
+      v[k].x = v[k].x + gmem(d.fmx)[a] * 0.5 * hm / m[k];
+      v[k].y = v[k].y + gmem(d.fmy)[a] * 0.5 * hm / m[k];
+      v[k].z = v[k].z + gmem(d.fmz)[a] * 0.5 * hm / m[k];
     }
     if constexpr (NA == 3) rattle_mim3(x, v, im);
   }
@@ -319,15 +330,16 @@ __device__ __forceinline__ void md_unit(const MdDesc &d, int u, int a0, double d
   }
 }
 
-__global__ void __launch_bounds__(256) k_md_units(const MdDesc *__restrict__ D, double dt, double h, int flags) {
+__global__ void __launch_bounds__(256) k_md_units(const MdDesc *__restrict__ D, double dt, double hm, double h,
+                                                  int flags) {
   const MdDesc &d = D[blockIdx.y];
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= d.nunits) return;
   const int a0 = gmem(d.ufirst)[u];
   if (gmem(d.urigid)[u])
-    md_unit<3>(d, u, a0, dt, h, flags);
+    md_unit<3>(d, u, a0, dt, hm, h, flags);
   else
-    md_unit<1>(d, u, a0, dt, h, flags);
+    md_unit<1>(d, u, a0, dt, hm, h, flags);
 }
 
 __global__ void __launch_bounds__(256) k_md_bonds(const MdDesc *__restrict__ D) {
@@ -405,6 +417,11 @@ __device__ __forceinline__ void md_relax_unit(const MdDesc &d, int u, int a0, do
     p[k] = make_double3(gmem(d.x)[a], gmem(d.y)[a], gmem(d.z)[a]);
     double fx = gmem(d.fx)[a] + gmem(d.bfx)[a], fy = gmem(d.fy)[a] + gmem(d.bfy)[a],
            fz = gmem(d.fz)[a] + gmem(d.bfz)[a];
+    if (d.fmx) {
+      fx += gmem(d.fmx)[a];
+      fy += gmem(d.fmy)[a];
+      fz += gmem(d.fmz)[a];
+    }
     double sx = hstep * im[k] * fx, sy = hstep * im[k] * fy, sz = hstep * im[k] * fz;
     const double s = sqrt(sx * sx + sy * sy + sz * sz);
     if (!(s <= dmax)) {  // also catches a non-finite step
@@ -473,10 +490,11 @@ static void md_dims(const MdDesc *h, int nsys, int &max_units, int &max_atoms, i
   }
 }
 
-void launch_md_units(const MdDesc *d, const MdDesc *h, int nsys, double dt, double hin, int flags, hipStream_t st) {
+void launch_md_units(const MdDesc *d, const MdDesc *h, int nsys, double dt, double hm, double hin, int flags,
+                     hipStream_t st) {
   int mu, ma, mb;
   md_dims(h, nsys, mu, ma, mb);
-  if (mu > 0) hipLaunchKernelGGL(k_md_units, dim3((mu + 255) / 256, nsys), dim3(256), 0, st, d, dt, hin, flags);
+  if (mu > 0) hipLaunchKernelGGL(k_md_units, dim3((mu + 255) / 256, nsys), dim3(256), 0, st, d, dt, hm, hin, flags);
 }
 
 void launch_md_fast_forces(const MdDesc *d, const MdDesc *h, int nsys, hipStream_t st) {

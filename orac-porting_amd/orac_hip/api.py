@@ -63,7 +63,8 @@ EXPORTS = [
     "orac_hip_sync", "orac_hip_stream", "orac_hip_batch_create", "orac_hip_batch_destroy", "orac_hip_batch_eval",
     "orac_hip_batch_profile", "orac_hip_batch_stage_ms", "orac_hip_batch_set_tuning", "orac_hip_batch_get_tuning",
     "orac_hip_batch_autotune", "orac_hip_batch_tune_log", "orac_hip_md_setup", "orac_hip_md_set_state", "orac_hip_md_get_state",
-    "orac_hip_batch_md_run", "orac_hip_batch_md_relax", "orac_hip_batch_md_scale_velocities",
+    "orac_hip_batch_md_run", "orac_hip_batch_md_run_shells", "orac_hip_system_set_shells",
+    "orac_hip_batch_md_relax", "orac_hip_batch_md_scale_velocities",
     "orac_hip_system_md_energies", "orac_hip_system_set_alchemy", "orac_hip_system_set_lambda",
     "orac_hip_system_alchemy_results", "orac_hip_ferrf_alchemy_add", "orac_hip_fnb14_alchemy",
     "orac_hip_fnbgrp_alchemy",
@@ -580,6 +581,11 @@ class ResidentSystem:
         out.update(f=np.stack([fx, fy, fz]), stress=e[9:18].reshape(3, 3, order="F"), list_entries=int(e[18]),
                    self_slt=e[19], self_slv=e[20])
         return out
+
+    def set_shells(self, rcutm, rtolm):
+        """Two nonbonded shells (inner 0 .. rcutm, outer rcutm .. rcut, switched
+        over rtolm); rcutm <= 0 goes back to one shell."""
+        _check(lib().orac_hip_system_set_shells(self.ctx._h, self.id, C.c_double(rcutm), C.c_double(rtolm)))
 
     def set_alchemy(self, al, lam=None):
         """Make the system alchemical with orac_hip.synth.Alchemy al (its

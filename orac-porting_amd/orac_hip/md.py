@@ -104,7 +104,14 @@ class Dynamics:
                     temperature=2.0 * ke / (self.dof * KB_INTERNAL), momentum=out[2:5], fail=fail.value)
 
 
-def run_batch(batch, nsteps, dt=2.0, ninner=4, list_every=5, graph=True):
+def run_batch(batch, nsteps, dt=2.0, ninner=4, list_every=5, graph=True, nmid=None):
+    """nsteps outer steps of dt; with nmid (systems with two shells) the inner
+    shell runs nmid times per dt and the fast terms ninner times per dt/nmid."""
+    if nmid:
+        _check(lib().orac_hip_batch_md_run_shells(batch.ctx._h, batch.id, C.c_int32(nsteps), C.c_double(dt),
+                                                  C.c_int32(nmid), C.c_int32(ninner), C.c_int32(list_every),
+                                                  C.c_int32(1 if graph else 0)))
+        return
     _check(lib().orac_hip_batch_md_run(batch.ctx._h, batch.id, C.c_int32(nsteps), C.c_double(dt),
                                        C.c_int32(ninner), C.c_int32(list_every), C.c_int32(1 if graph else 0)))
 

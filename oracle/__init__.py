@@ -1310,3 +1310,47 @@ def ref_omp_evaluation(s, corr_pairs, nthr, fa=None, nsample=1, budget_s=None):
 
     done, secs = _big_stack(run)
     return f["f"], done, secs
+
+
+def md_steps_shells(x, v, mass, nsteps, dt, nmid, ninner, slow, mid, fast, cons):
+    """md_steps with two nonbonded shells: three levels of mtsmd.f, the outer
+    shell (slow: outer-shell direct forces, PME, corrections) over dt, the
+    inner shell (mid) nmid times per dt, the fast terms ninner times per
+    dt/nmid, in mtsmd.f's order (its m level inside h, :2598-3146):
+
+        correc(Fs, dt); rattle_correc
+        nmid times: correc(Fm, hm); rattle_correc
+                    ninner times: verlet(Ff, h); rattle_verlet; Ff = fast(x); correc(Ff, h); rattle_correc
+                    Fm = mid(x) (and Fs = slow(x) after the last); correc(Fm, hm); rattle_correc
+        correc(Fs, dt); rattle_correc
+
+    Returns (x, v, failures)."""
+    cnst, dss, coeff, prot = cons
+    hm = dt / nmid
+    h = hm / ninner
+    fs, fm, ff = slow(x), mid(x), fast(x)
+    fails = 0
+
+    def rc_(xx, vv):
+        nonlocal fails
+        vv, _, rc = rattle_correc(xx, vv, cnst, dss, coeff, mass, prot)
+        fails += rc != 0
+        return vv
+
+    for _ in range(nsteps):
+        v = rc_(x, correc(v, fs, mass, dt))
+        for im in range(nmid):
+            v = rc_(x, correc(v, fm, mass, hm))
+            for _ in range(ninner):
+                xo = x
+                x, v = verlet(x, v, ff, mass, h)
+                x, v, _, rc = rattle_verlet(h, x, xo, v, cnst, dss, coeff, mass, prot)
+                fails += rc != 0
+                ff = fast(x)
+                v = rc_(x, correc(v, ff, mass, h))
+            fm = mid(x)
+            if im == nmid - 1:
+                fs = slow(x)
+            v = rc_(x, correc(v, fm, mass, hm))
+        v = rc_(x, correc(v, fs, mass, dt))
+    return x, v, fails

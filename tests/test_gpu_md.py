@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from conftest import relrms
 
 pytestmark = pytest.mark.gpu
 KJ_MOL = 4.184  # kJ/mol per kcal/mol
@@ -80,6 +81,109 @@ def test_md_matches_oracle(gpu_ctx, sysname, ninner, request):
     print(sysname, "moved", np.abs(xo - x0).max(), "max |dx|", dx, "max |dv|", dv, "of", np.abs(vo).max())
     assert fail == 0 and fails == 0
     assert np.abs(xo - x0).max() > 1e-3  # it moved
+    assert dx <= 1e-9
+    assert dv <= 1e-9 * np.abs(vo).max()
+
+
+RCUTM, RTOLM = 5.0, 0.5  # inner shell of the two-shell tests
+
+
+def _oracle_forces_shells(s):
+    """slow(x), mid(x), fast(x) of a system with two shells: the outer shell
+    (rinn = RCUTM switched in over RTOLM, decide_force on its real-space
+    Ewald term) + PME + ferrf; the inner shell (0 .. RCUTM switched out);
+    the harmonic solute terms."""
+    from orac_hip import md
+    cp = np.concatenate([s.bonds, s.bends], axis=1) if s.bonds is not None else np.zeros((2, 0), np.int32)
+    oc = np.asarray(s.oc)
+    pairs, k, r0 = md.solute_terms(s)
+    skip = O.decide_force(RCUTM, s.alpha)
+
+    def lists(x):
+        xp = oc @ x
+        s2 = s.with_coords(xp[0], xp[1], xp[2])
+        rc = s2.rcut + s2.rtol + s2.rnei
+        fa = O.ForceArgs(s2)
+        if s2.L > 3 * rc:
+            nc = int(s2.L // 3.0)
+            st = O.lc_index(nc, nc, nc, rc, s2.co)
+            nnlpp = O.lc_list(nc, nc, nc, st, rc, s2.co, fa.xpg, fa.ypg, fa.zpg)
+        else:
+            nnlpp = O.force(s2, O.all_groups_list(s2.ngrp), rinn=rc, rout=rc, rtolinn=0.0, rtolout=0.0, rneigh=0.0,
+                            iz=0, fa=fa)["nnlpp1"]
+        return s2, fa, nnlpp
+
+    def slow(x):
+        s2, fa, nnlpp = lists(x)
+        d = O.force(s2, nnlpp, fa=fa, rinn=RCUTM, rout=s2.rcut, rtolinn=RTOLM, rtolout=s2.rtol, rneigh=0.0,
+                    lskip_ewald=skip, nthreads=4)
+        f = d["f"] + O.fft_pme(s2, variant="fftw", nthreads=4)["f"]
+        if cp.shape[1]:
+            f = f + O.ferrf(s2, cp, 1, 1.0)["f"]
+        return f
+
+    def mid(x):
+        s2, fa, nnlpp = lists(x)
+        return O.force(s2, nnlpp, fa=fa, rinn=0.0, rout=RCUTM, rtolinn=0.0, rtolout=RTOLM, rneigh=0.0,
+                       nthreads=4)["f"]
+
+    def fast(x):
+        return O.harmonic_forces(x, pairs, k, r0)
+
+    return slow, mid, fast
+
+
+@pytest.mark.parametrize("sysname", ["water", "small_solute"])
+def test_two_shells_sum_to_one_shell(gpu_ctx, sysname, request):
+    """The inner and outer shells' switches are complementary over
+    [rcutm, rcutm + rtolm] (force.f's inner and outer switch forms with the
+    same radii), so with the outer shell's Ewald term kept the two shells
+    give the single shell's forces and energies to rounding."""
+    from orac_hip.api import ResidentSystem
+    s = request.getfixturevalue(sysname)
+    assert not O.decide_force(RCUTM, s.alpha)  # erfc(alpha rcutm) above 1e-3: the outer shell keeps Ewald
+    rs = ResidentSystem(gpu_ctx, s, precision=0)
+    rs.eval(rebuild_list=True)
+    one = rs.results()
+    rs.set_shells(RCUTM, RTOLM)
+    rs.eval(rebuild_list=True)
+    two = rs.results()
+    rs.set_shells(0.0, 0.0)
+    rs.eval(rebuild_list=True)
+    back = rs.results()
+    rs.close()
+    assert relrms(two["f"], one["f"]) < 1e-12
+    for k in ("uconf_slv", "ucoul_slv", "eer"):
+        assert abs(two[k] - one[k]) <= 1e-12 * abs(one[k]), k
+    assert np.array_equal(back["f"], one["f"])
+
+
+@pytest.mark.parametrize("sysname", ["water", "small_solute"])
+def test_md_shells_match_oracle(gpu_ctx, sysname, request):
+    """Five outer steps of 2 fs with two shells (inner shell twice per step,
+    the fast terms twice per inner step) from the same state: the device
+    against oracle.md_steps_shells, list rebuilt every step."""
+    from orac_hip import md
+    s = request.getfixturevalue(sysname)
+    rs, dyn, b = _prepare(gpu_ctx, s, relax=20)
+    rs.set_shells(RCUTM, RTOLM)
+    b.close()
+    from orac_hip.api import Batch
+    b = Batch(gpu_ctx, [rs])
+    x0, v0 = dyn.state()
+    md.run_batch(b, 5, dt=2.0, ninner=2, nmid=2, list_every=1, graph=False)
+    xg, vg = dyn.state()
+    fail = dyn.energies()["fail"]
+    b.close()
+    rs.close()
+    first, lens = md.rigid_waters(s)
+    cons = O.constraint_tables(first, lens, s.mass)
+    slow, mid, fast = _oracle_forces_shells(s)
+    xo, vo, fails = O.md_steps_shells(x0, v0, np.asarray(s.mass, np.float64), 5, 2.0, 2, 2, slow, mid, fast, cons)
+    dx, dv = np.abs(xg - xo).max(), np.abs(vg - vo).max()
+    print(sysname, "shells: moved", np.abs(xo - x0).max(), "max |dx|", dx, "max |dv|", dv)
+    assert fail == 0 and fails == 0
+    assert np.abs(xo - x0).max() > 1e-3
     assert dx <= 1e-9
     assert dv <= 1e-9 * np.abs(vo).max()
 
