@@ -189,24 +189,34 @@ def prepare_md(ctx, systems, precision, relax=300, equil=200, temperature=300.0,
     return rss, dyns, b
 
 
-def time_md(ctx, b, nsteps, warmup, dt=2.0, ninner=4, list_every=5):
+def time_md(ctx, b, nsteps, warmup, dt=2.0, ninner=4, list_every=5, nmid=None):
     from orac_hip import md
-    md.run_batch(b, warmup, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH)
+    md.run_batch(b, warmup, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH, nmid=nmid)
     ctx.sync()
     t0 = time.perf_counter()
-    md.run_batch(b, nsteps, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH)
+    md.run_batch(b, nsteps, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH, nmid=nmid)
     ctx.sync()
     return time.perf_counter() - t0
 
 
-def nve_drift(ctx, b, dyn, nsteps, dt=2.0, ninner=4, list_every=5):
+def time_md2(ctx, b, nsteps, warmup):
+    """time_md for systems with two shells: outer step 4 fs, inner shell twice
+    per step, fast terms four times per inner step, lists every 3 outer steps."""
+    return time_md(ctx, b, nsteps, warmup, dt=4.0, ninner=4, list_every=3, nmid=2)
+
+
+def nve_drift2(ctx, b, dyn, nsteps):
+    return nve_drift(ctx, b, dyn, nsteps, dt=4.0, ninner=4, list_every=3, nmid=2)
+
+
+def nve_drift(ctx, b, dyn, nsteps, dt=2.0, ninner=4, list_every=5, nmid=None):
     """Total-energy drift (kJ/mol per ns) of the first system over nsteps: the
     slope of a linear fit to the total energy sampled every 10 steps."""
     from orac_hip import md, synth
     e0 = dyn.energies()
     tot, t_ns = [e0["total"]], [0.0]
     for k in range(max(1, nsteps // 10)):
-        md.run_batch(b, 10, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH)
+        md.run_batch(b, 10, dt=dt, ninner=ninner, list_every=list_every, graph=GRAPH, nmid=nmid)
         tot.append(dyn.energies()["total"])
         t_ns.append((k + 1) * 10 * dt * 1e-6)
     ctx.sync()
@@ -519,7 +529,7 @@ def main():
         sys.exit(2)
 
     from orac_hip import synth
-    from orac_hip.api import Context
+    from orac_hip.api import Batch, Context
 
     s = synth.config("solvated24k")
     corr = corrections(s)
@@ -574,10 +584,23 @@ def main():
             rss, dyns, b = prepare_md(ctx, [s], 0)
             w64 = time_md(ctx, b, K, 5)
             drift64 = nve_drift(ctx, b, dyns[0], max(K, 1000))  # 2 ps: drift, not fluctuation
+            # the same system with ORAC's nonbonded split in two shells: the
+            # outer shell, PME and corrections every 4 fs, the inner shell
+            # (5.0 A, switched over 0.5 A) every 2 fs, the fast terms every 0.5 fs
+            rss[0].set_shells(5.0, 0.5)
+            b.close()
+            b = Batch(ctx, rss)
+            w2 = time_md2(ctx, b, K, 5)
+            drift2 = nve_drift2(ctx, b, dyns[0], max(K // 2, 250))
             b.close()
             rss[0].close()
-            log(f"md fp64 single: {1e3 * w64 / K:.3f} ms/step, NVE {drift64}")
-            dyn = dict(fp64=dict(ns_per_day=ns_per_day(K, w64), ms_per_step=1e3 * w64 / K, nve=drift64))
+            log(f"md fp64 single: {1e3 * w64 / K:.3f} ms/step, NVE {drift64}; two shells {1e3 * w2 / K:.3f} ms/4 fs")
+            dyn = dict(fp64=dict(ns_per_day=ns_per_day(K, w64), ms_per_step=1e3 * w64 / K, nve=drift64),
+                       fp64_two_shells=dict(ns_per_day=ns_per_day(K, w2, dt=4.0), ms_per_outer_step=1e3 * w2 / K,
+                                            nve=drift2,
+                                            workload="outer shell (5.0-10 A) + PME + corrections every 4 fs, inner "
+                                                     "shell (0-5.0 A, switched over 0.5 A) every 2 fs, fast terms "
+                                                     "every 0.5 fs, lists every 3 outer steps"))
             if args.replicas > 0:
                 rss, dyns, b = prepare_md(ctx, reps, 1)
                 if GRAPH:  # the autotuner times graph replays

@@ -682,10 +682,13 @@ static void batch_enqueue(orac_hip_ctx *ctx, Batch &b, bool rebuild, bool events
     if (b.any_corr) launch_corr(b.dc, b.hc.data(), n, ctx->side);
     ORAC_HIP(hipEventRecord(b.join, ctx->side));
   }
-  if (md_lists)  // every system on its own update schedule (device step counters)
+  if (md_lists) {  // every system on its own update schedule (device step counters)
     launch_list_build(b.dl_md, b.hl_md.data(), n, LIST_COUNT | LIST_WRITE, st);
-  else if (rebuild)
+    if (b.shells) launch_list_inner(b.dl_md, b.hl_md.data(), n, st);
+  } else if (rebuild) {
     launch_list_build(b.dl, b.hl.data(), n, LIST_COUNT | LIST_WRITE, st);
+    if (b.shells) launch_list_inner(b.dl, b.hl.data(), n, st);
+  }
   mark(ST_LIST);
   hipEvent_t kb = events ? b.ev[ST_KBEG] : nullptr, ke = events ? b.ev[ST_KEND] : nullptr;
   if (b.precision == 0)
@@ -784,6 +787,11 @@ static void size_list(orac_hip_ctx *ctx, int32_t id) {
     S.list.slab = need;
     S.pair.nb_idx = S.list.out;
     S.pair.nb_slab = need;
+    if (S.list.out_m) {  // the inner list follows the slab
+      S.list.out_m = S.B.get<int>("nbidx_m", S.list_cap);
+      S.pair_m.nb_idx = S.list.out_m;
+      S.pair_m.nb_slab = need;
+    }
     if (S.list.split) {
       S.list.rout = S.B.get<int>("nbrout", S.list_cap);
       S.list.mirror = S.B.get<int>("nbmirror", S.list_cap);
@@ -1570,6 +1578,7 @@ extern "C" int orac_hip_system_set_shells(orac_hip_ctx *ctx, int32_t sys, double
       S.rcutm = S.rtolm = 0.0;
       p.r2inn = p.rinn0 = p.r2inn0 = p.arsinn1 = p.arsinn2 = 0.0;
       p.lskip_ewald = 0;
+      S.list.r2m = 0.0;
     } else {
       ORAC_REQUIRE(rtolm >= 0.0 && rcutm + rtolm < S.rcut, E_ARG, "the inner shell must end before the cut-off");
       ORAC_REQUIRE(!S.alch && p.rematom == nullptr && !p.half, E_UNSUPPORTED,
@@ -1610,6 +1619,17 @@ extern "C" int orac_hip_system_set_shells(orac_hip_ctx *ctx, int32_t sys, double
       m.fz = S.fmz;
       m.out = S.out_m;
       m.part = S.B.get<double>("ppart_m", (size_t)m.nblk * 16);
+      // the inner shell walks an inner list: the entries of the full list within
+      // rcutm + rtolm + the skin, filtered whenever the full list is rebuilt
+      ORAC_REQUIRE(S.list.slab > 0 && !S.list.split, E_STATE, "two shells need the slab list layout");
+      const double rm = rcutm + rtolm + S.rnei;
+      S.list.r2m = rm * rm;
+      S.list.out_m = S.B.get<int>("nbidx_m", S.list_cap);
+      S.list.count_m = S.B.get<int>("lcount_m", S.ngrp);
+      m.nb_idx = S.list.out_m;
+      m.nb_cnt = S.list.count_m;
+      m.nb_slab = S.list.slab;
+      S.have_list = false;  // the next evaluation builds both lists
     }
     if (S.md_on) {
       S.md.fmx = S.rcutm > 0.0 ? S.fmx : nullptr;

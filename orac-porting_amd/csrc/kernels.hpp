@@ -179,12 +179,18 @@ struct ListDesc {
   // slot of forward entry e; pcap the slots available
   int split;
   int *rout, *rcount, *rbase, *mirror;
+  // inner list (two shells): the entries of owner o whose centre distance is
+  // below sqrt(r2m), in list order, at out_m[o * slab], count_m[o] of them
+  int *out_m, *count_m;
+  double r2m;
   long long pcap;
 };
 enum ListPhase { LIST_COUNT = 1, LIST_WRITE = 2 };
 // LIST_COUNT: cells, per-group counts, offsets and the capacity check;
 // LIST_WRITE: the entries.  On overflow total[1] = 1 and nothing is written.
 void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int phases, hipStream_t st);
+// the inner lists of a batch's systems after a slab-layout build (systems with r2m > 0)
+void launch_list_inner(const ListDesc *d_desc, const ListDesc *h, int nsys, hipStream_t st);
 void launch_exclusive_scan(const int *in, int *out, int n, int *total, hipStream_t st);
 
 // half list (CSR) -> full list (CSR) with deterministic order

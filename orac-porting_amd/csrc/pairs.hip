@@ -704,6 +704,46 @@ void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int 
   }
 }
 
+// Inner list of owner o (one wavefront): the entries of its slab whose centre
+// distance is below sqrt(r2m), in list order (wave ballot compaction), so the
+// inner shell's pair pass walks only them.  Same schedule as the build.
+__global__ void __launch_bounds__(256) k_list_inner(const ListDesc *__restrict__ D) {
+  const ListDesc &d = D[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (o >= d.ngrp || !(d.r2m > 0.0) || !list_due(d)) return;  // uniform over the wave
+  const int n = d.cellorder ? gmem(d.cell_items)[o] : o;
+  const double x1 = gmem(d.xpg)[n], y1 = gmem(d.ypg)[n], z1 = gmem(d.zpg)[n];
+  const int cnt = min(gmem(d.count)[o], d.slab);
+  const int *in = d.out + (size_t)o * d.slab;
+  int *out = d.out_m + (size_t)o * d.slab;
+  int w = 0;
+  for (int e0 = 0; e0 < cnt; e0 += 64) {
+    const int e = e0 + lane;
+    bool keep = false;
+    int l = 0;
+    if (e < cnt) {
+      l = in[e];
+      const double4 p = d.cellorder ? gload(d.cg.cpos + l)
+                                    : make_double4(gmem(d.xpg)[l], gmem(d.ypg)[l], gmem(d.zpg)[l], 0.0);
+      keep = list_hit(x1, y1, z1, p, d.cg.co, d.r2m);
+    }
+    const unsigned long long m = __ballot(keep);
+    if (keep) out[w + __popcll(m & ((1ull << lane) - 1ull))] = l;
+    w += __popcll(m);
+  }
+  if (lane == 0) d.count_m[o] = w;
+}
+
+void launch_list_inner(const ListDesc *d_desc, const ListDesc *h, int nsys, hipStream_t st) {
+  int max_grp = 1;
+  for (int s = 0; s < nsys; ++s) {
+    if (h[s].r2m > 0.0 && (h[s].slab <= 0 || h[s].split)) throw Error(E_STATE, "inner lists need the plain slab layout");
+    max_grp = std::max(max_grp, h[s].ngrp);
+  }
+  hipLaunchKernelGGL(k_list_inner, dim3((max_grp + 3) / 4, nsys), dim3(256), 0, st, d_desc);
+}
+
 // ---------------------------------------------------------------- half -> full
 
 __global__ void k_h2f_count(int ngrp, const int *h_off, const int *h_idx, int *incount) {

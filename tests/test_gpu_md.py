@@ -188,6 +188,30 @@ def test_md_shells_match_oracle(gpu_ctx, sysname, request):
     assert dv <= 1e-9 * np.abs(vo).max()
 
 
+def test_md_shells_inner_list_ages(gpu_ctx, small_solute):
+    """The inner shell walks an inner list filtered (with the skin) at each
+    full-list build: six 2 fs outer steps with the lists rebuilt every third
+    step give the trajectory of the lists rebuilt every step, to rounding
+    (the list order, hence the summation order, differs between builds)."""
+    from orac_hip import md
+    from orac_hip.api import Batch
+    out = []
+    for every in (1, 3):
+        rs, dyn, b = _prepare(gpu_ctx, small_solute, relax=20)
+        rs.set_shells(RCUTM, RTOLM)
+        b.close()
+        b = Batch(gpu_ctx, [rs])
+        md.run_batch(b, 6, dt=2.0, ninner=2, nmid=2, list_every=every, graph=True)
+        out.append(dyn.state())
+        assert dyn.energies()["fail"] == 0
+        b.close()
+        rs.close()
+    (x1, v1), (x3, v3) = out
+    print("inner list aged 3 steps: max |dx|", np.abs(x3 - x1).max())
+    assert np.abs(x3 - x1).max() < 1e-9
+    assert np.abs(v3 - v1).max() < 1e-9 * np.abs(v1).max()
+
+
 def _rigid_error(dyn):
     from orac_hip import md
     x, _ = dyn.state()
