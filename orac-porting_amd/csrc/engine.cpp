@@ -348,19 +348,24 @@ static void system_build(orac_hip_ctx *ctx, System &S, const orac_hip_system_des
   p.ntypes = ma;
   const size_t npk = (size_t)ma * (ma + 1) / 2;
   if (ma <= LJ_DENSE_MAX) {
-    std::vector<double2> dense((size_t)ma * ma);
-    for (int i = 0; i < ma; ++i)
-      for (int j = 0; j < ma; ++j) {
-        int lij = d.type_table[i + (size_t)ma * j] - 1;
-        ORAC_REQUIRE(lij >= 0 && (size_t)lij < npk, E_ARG, "type table index out of range");
-        dense[(size_t)i * ma + j] = make_double2(d.ecc12[lij], d.ecc6[lij]);
-      }
-    p.lj_dense = B.put_vec("ljd", dense, st, false);
-    if (d.precision == 1) {
-      std::vector<float2> d32(dense.size());
-      for (size_t k = 0; k < dense.size(); ++k) d32[k] = make_float2((float)dense[k].x, (float)dense[k].y);
-      p.lj_dense32 = B.put_vec("ljd32", d32, st, false);
+    // the indices checked here, the table gathered on the device from the
+    // packed arrays (staged in the context's buffers, reused by every system)
+    const size_t n2 = (size_t)ma * ma;
+    int lo = 1, hi = 1;
+    if (n2 > 0) lo = hi = d.type_table[0];
+    for (size_t k = 0; k < n2; ++k) {
+      lo = std::min(lo, (int)d.type_table[k]);
+      hi = std::max(hi, (int)d.type_table[k]);
     }
+    ORAC_REQUIRE(lo >= 1 && (size_t)hi <= npk, E_ARG, "type table index out of range");
+    int *dtt = ctx->B.put("ljs_tt", d.type_table, n2, st, false);
+    double *d12 = ctx->B.put("ljs_e12", d.ecc12, npk, st, false);
+    double *d6 = ctx->B.put("ljs_e6", d.ecc6, npk, st, false);
+    double2 *dense = B.get<double2>("ljd", n2);
+    float2 *dense32 = d.precision == 1 ? B.get<float2>("ljd32", n2) : nullptr;
+    launch_lj_dense(ma, dtt, d12, d6, dense, dense32, st);
+    p.lj_dense = dense;
+    if (dense32) p.lj_dense32 = dense32;
   } else {
     std::vector<int> tt((size_t)ma * ma);
     for (size_t k = 0; k < tt.size(); ++k) {

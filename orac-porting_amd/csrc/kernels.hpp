@@ -191,6 +191,10 @@ enum ListPhase { LIST_COUNT = 1, LIST_WRITE = 2 };
 void launch_list_build(const ListDesc *d_desc, const ListDesc *h, int nsys, int phases, hipStream_t st);
 // the inner lists of a batch's systems after a slab-layout build (systems with r2m > 0)
 void launch_list_inner(const ListDesc *d_desc, const ListDesc *h, int nsys, hipStream_t st);
+// dense {ecc12, ecc6} table (and its FP32 copy if dense32) from the packed
+// type_table (ma x ma, 1-based, column-major) and ecc12/ecc6
+void launch_lj_dense(int ma, const int *tt, const double *e12, const double *e6, double2 *dense, float2 *dense32,
+                     hipStream_t st);
 void launch_exclusive_scan(const int *in, int *out, int n, int *total, hipStream_t st);
 
 // half list (CSR) -> full list (CSR) with deterministic order

@@ -744,6 +744,28 @@ void launch_list_inner(const ListDesc *d_desc, const ListDesc *h, int nsys, hipS
   hipLaunchKernelGGL(k_list_inner, dim3((max_grp + 3) / 4, nsys), dim3(256), 0, st, d_desc);
 }
 
+// Dense LJ table of a system: entry (i, j) = {ecc12, ecc6} of the packed
+// pair index type_table(i, j) (1-based, column-major), and its FP32 copy
+// when asked (the mixed-precision pair kernel's).  The host has checked the
+// indices.
+__global__ void __launch_bounds__(256) k_lj_dense(int ma, const int *__restrict__ tt, const double *__restrict__ e12,
+                                                  const double *__restrict__ e6, double2 *dense, float2 *dense32) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)ma * ma) return;
+  const int i = (int)(k / ma), j = (int)(k - (size_t)i * ma);
+  const int lij = tt[i + (size_t)ma * j] - 1;
+  const double2 v = make_double2(e12[lij], e6[lij]);
+  dense[k] = v;
+  if (dense32) dense32[k] = make_float2((float)v.x, (float)v.y);
+}
+
+void launch_lj_dense(int ma, const int *tt, const double *e12, const double *e6, double2 *dense, float2 *dense32,
+                     hipStream_t st) {
+  const size_t n = (size_t)ma * ma;
+  if (n > 0) hipLaunchKernelGGL(k_lj_dense, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ma, tt, e12, e6,
+                                dense, dense32);
+}
+
 // ---------------------------------------------------------------- half -> full
 
 __global__ void k_h2f_count(int ngrp, const int *h_off, const int *h_idx, int *incount) {
