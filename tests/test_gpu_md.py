@@ -212,6 +212,31 @@ def test_md_shells_inner_list_ages(gpu_ctx, small_solute):
     assert np.abs(v3 - v1).max() < 1e-9 * np.abs(v1).max()
 
 
+def test_md_shells_batch_equals_alone(gpu_ctx, small_solute, water):
+    """Two-shell dynamics keep a system's trajectory independent of its
+    batch: alone and batched with a system of another size, graph replay,
+    bit for bit."""
+    from orac_hip import md
+    from orac_hip.api import Batch
+    runs = []
+    for with_other in (False, True):
+        a, da, ba = _prepare(gpu_ctx, small_solute, relax=20)
+        w, dw, bw = _prepare(gpu_ctx, water, relax=20)
+        for rs in (a, w):
+            rs.set_shells(RCUTM, RTOLM)
+        ba.close()
+        bw.close()
+        b = Batch(gpu_ctx, [a, w] if with_other else [a])
+        md.run_batch(b, 6, dt=2.0, ninner=2, nmid=2, list_every=3, graph=True)
+        runs.append(da.state())
+        assert da.energies()["fail"] == 0
+        b.close()
+        a.close()
+        w.close()
+    (x1, v1), (x2, v2) = runs
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2)
+
+
 def _rigid_error(dyn):
     from orac_hip import md
     x, _ = dyn.state()
