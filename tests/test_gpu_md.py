@@ -88,7 +88,7 @@ def test_md_matches_oracle(gpu_ctx, sysname, ninner, request):
 RCUTM, RTOLM = 5.0, 0.5  # inner shell of the two-shell tests
 
 
-def _oracle_forces_shells(s):
+def _oracle_forces_shells(s, rcutm=RCUTM, rtolm=RTOLM):
     """slow(x), mid(x), fast(x) of a system with two shells: the outer shell
     (rinn = RCUTM switched in over RTOLM, decide_force on its real-space
     Ewald term) + PME + ferrf; the inner shell (0 .. RCUTM switched out);
@@ -97,7 +97,7 @@ def _oracle_forces_shells(s):
     cp = np.concatenate([s.bonds, s.bends], axis=1) if s.bonds is not None else np.zeros((2, 0), np.int32)
     oc = np.asarray(s.oc)
     pairs, k, r0 = md.solute_terms(s)
-    skip = O.decide_force(RCUTM, s.alpha)
+    skip = O.decide_force(rcutm, s.alpha)
 
     def lists(x):
         xp = oc @ x
@@ -115,7 +115,7 @@ def _oracle_forces_shells(s):
 
     def slow(x):
         s2, fa, nnlpp = lists(x)
-        d = O.force(s2, nnlpp, fa=fa, rinn=RCUTM, rout=s2.rcut, rtolinn=RTOLM, rtolout=s2.rtol, rneigh=0.0,
+        d = O.force(s2, nnlpp, fa=fa, rinn=rcutm, rout=s2.rcut, rtolinn=rtolm, rtolout=s2.rtol, rneigh=0.0,
                     lskip_ewald=skip, nthreads=4)
         f = d["f"] + O.fft_pme(s2, variant="fftw", nthreads=4)["f"]
         if cp.shape[1]:
@@ -124,7 +124,7 @@ def _oracle_forces_shells(s):
 
     def mid(x):
         s2, fa, nnlpp = lists(x)
-        return O.force(s2, nnlpp, fa=fa, rinn=0.0, rout=RCUTM, rtolinn=0.0, rtolout=RTOLM, rneigh=0.0,
+        return O.force(s2, nnlpp, fa=fa, rinn=0.0, rout=rcutm, rtolinn=0.0, rtolout=rtolm, rneigh=0.0,
                        nthreads=4)["f"]
 
     def fast(x):
@@ -158,15 +158,19 @@ def test_two_shells_sum_to_one_shell(gpu_ctx, sysname, request):
     assert np.array_equal(back["f"], one["f"])
 
 
-@pytest.mark.parametrize("sysname", ["water", "small_solute"])
-def test_md_shells_match_oracle(gpu_ctx, sysname, request):
+@pytest.mark.parametrize("sysname,rcutm", [("water", RCUTM), ("small_solute", RCUTM), ("water", 6.5)])
+def test_md_shells_match_oracle(gpu_ctx, sysname, rcutm, request):
     """Five outer steps of 2 fs with two shells (inner shell twice per step,
     the fast terms twice per inner step) from the same state: the device
-    against oracle.md_steps_shells, list rebuilt every step."""
+    against oracle.md_steps_shells, list rebuilt every step.  At rcutm 6.5
+    on the water box erfc(alpha rcutm) is below 1e-3, so the outer shell
+    drops its real-space Ewald term (decide_force)."""
     from orac_hip import md
     s = request.getfixturevalue(sysname)
+    if rcutm != RCUTM:
+        assert O.decide_force(rcutm, s.alpha)
     rs, dyn, b = _prepare(gpu_ctx, s, relax=20)
-    rs.set_shells(RCUTM, RTOLM)
+    rs.set_shells(rcutm, RTOLM)
     b.close()
     from orac_hip.api import Batch
     b = Batch(gpu_ctx, [rs])
@@ -178,7 +182,7 @@ def test_md_shells_match_oracle(gpu_ctx, sysname, request):
     rs.close()
     first, lens = md.rigid_waters(s)
     cons = O.constraint_tables(first, lens, s.mass)
-    slow, mid, fast = _oracle_forces_shells(s)
+    slow, mid, fast = _oracle_forces_shells(s, rcutm)
     xo, vo, fails = O.md_steps_shells(x0, v0, np.asarray(s.mass, np.float64), 5, 2.0, 2, 2, slow, mid, fast, cons)
     dx, dv = np.abs(xg - xo).max(), np.abs(vg - vo).max()
     print(sysname, "shells: moved", np.abs(xo - x0).max(), "max |dx|", dx, "max |dv|", dv)
