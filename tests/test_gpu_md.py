@@ -158,6 +158,25 @@ def test_two_shells_sum_to_one_shell(gpu_ctx, sysname, request):
     assert np.array_equal(back["f"], one["f"])
 
 
+def test_two_shells_mixed_precision(gpu_ctx, small_solute):
+    """Mixed precision with two shells: the shells' sum against the mixed
+    single shell to the FP32 pair arithmetic's rounding, and against FP64
+    within the mixed-precision budget (5e-6 relative RMS)."""
+    from orac_hip.api import ResidentSystem
+    s = small_solute
+    out = {}
+    for prec in (0, 1):
+        rs = ResidentSystem(gpu_ctx, s, precision=prec)
+        rs.eval(rebuild_list=True)
+        out[(prec, 1)] = rs.results()["f"]
+        rs.set_shells(RCUTM, RTOLM)
+        rs.eval(rebuild_list=True)
+        out[(prec, 2)] = rs.results()["f"]
+        rs.close()
+    assert relrms(out[(1, 2)], out[(1, 1)]) < 1e-6
+    assert relrms(out[(1, 2)], out[(0, 2)]) < 5e-6
+
+
 @pytest.mark.parametrize("sysname,rcutm", [("water", RCUTM), ("small_solute", RCUTM), ("water", 6.5)])
 def test_md_shells_match_oracle(gpu_ctx, sysname, rcutm, request):
     """Five outer steps of 2 fs with two shells (inner shell twice per step,
@@ -216,16 +235,17 @@ def test_md_shells_inner_list_ages(gpu_ctx, small_solute):
     assert np.abs(v3 - v1).max() < 1e-9 * np.abs(v1).max()
 
 
-def test_md_shells_batch_equals_alone(gpu_ctx, small_solute, water):
+@pytest.mark.parametrize("precision", [0, 1])
+def test_md_shells_batch_equals_alone(gpu_ctx, small_solute, water, precision):
     """Two-shell dynamics keep a system's trajectory independent of its
     batch: alone and batched with a system of another size, graph replay,
-    bit for bit."""
+    bit for bit, in both precisions."""
     from orac_hip import md
     from orac_hip.api import Batch
     runs = []
     for with_other in (False, True):
-        a, da, ba = _prepare(gpu_ctx, small_solute, relax=20)
-        w, dw, bw = _prepare(gpu_ctx, water, relax=20)
+        a, da, ba = _prepare(gpu_ctx, small_solute, precision, relax=20)
+        w, dw, bw = _prepare(gpu_ctx, water, precision, relax=20)
         for rs in (a, w):
             rs.set_shells(RCUTM, RTOLM)
         ba.close()
