@@ -1164,7 +1164,12 @@ def md_steps(x, v, mass, nsteps, dt, ninner, slow, fast, cons):
         Fs = slow(x); correc(Fs, dt); rattle_correc         (:3030-3146)
 
     x, v (3, n); slow(x) and fast(x) return (3, n) forces at Cartesian
-    positions x; cons = constraint_tables(...).  Returns (x, v, failures)."""
+    positions x; cons = constraint_tables(...).  Returns (x, v, failures).
+
+    Its pieces (correc, verlet, rattle_verlet, rattle_correc) are pinned bit
+    for bit against the reference; the order of the calls is restated from
+    mtsmd.f, which is the caller and is not compiled here: parity unpinned
+    for that order (it is checked by time reversibility and NVE drift)."""
     cnst, dss, coeff, prot = cons
     h = dt / ninner
     fs, ff = slow(x), fast(x)
