@@ -198,3 +198,34 @@ def test_md_steps_time_reversible():
     # MIM's tolerance (1e-7 relative on the multipliers) bounds the return
     assert np.abs(xb - x).max() < 1e-6
     assert np.abs(-vb - v).max() < 1e-6
+
+
+def test_md_steps_end_state_invariants():
+    """What mtsmd.f's order fixes at the end of an outer step, whatever the
+    forces: the last call is a RATTLE, so the relative velocities of every
+    constrained pair are orthogonal to its bond (to rounding); the last
+    drift was SHAKEn, so the bond lengths hold to the MIM method's second
+    order.  A step that ended on a kick, or skipped a SHAKE, fails one."""
+    x, v, mass, cons, rng = _waters(10, seed=21, vscale=0.01)
+    cnst, dss = cons[0], cons[1]
+
+    def slow(y):
+        return 1e-3 * np.sin(y)  # any smooth force field, of the size of the pair forces above
+
+    def fast(y):
+        return 3e-4 * np.cos(2.0 * y)
+
+    xa, va, fails = O.md_steps(x, v, mass, 4, 2.0, 3, slow, fast, cons)
+    assert fails == 0
+    i, j = cnst[0] - 1, cnst[1] - 1
+    r = xa[:, i] - xa[:, j]
+    dv = va[:, i] - va[:, j]
+    assert np.abs((r * dv).sum(axis=0)).max() < 1e-14
+    lens = np.sqrt((r * r).sum(axis=0))
+    assert np.abs(lens - np.sqrt(dss)).max() < 1e-4
+    # and the same with two shells
+    xb, vb, fails = O.md_steps_shells(x, v, mass, 2, 4.0, 2, 3, slow, lambda y: 0.5 * slow(y), fast, cons)
+    assert fails == 0
+    r = xb[:, i] - xb[:, j]
+    dv = vb[:, i] - vb[:, j]
+    assert np.abs((r * dv).sum(axis=0)).max() < 1e-14
