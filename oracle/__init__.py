@@ -1169,7 +1169,8 @@ def md_steps(x, v, mass, nsteps, dt, ninner, slow, fast, cons):
     Its pieces (correc, verlet, rattle_verlet, rattle_correc) are pinned bit
     for bit against the reference; the order of the calls is restated from
     mtsmd.f, which is the caller and is not compiled here: parity unpinned
-    for that order (it is checked by time reversibility and NVE drift)."""
+    for that order (it is checked by time reversibility, NVE drift and the
+    end-of-step invariants: RATTLE last, the last drift SHAKEn)."""
     cnst, dss, coeff, prot = cons
     h = dt / ninner
     fs, ff = slow(x), fast(x)
