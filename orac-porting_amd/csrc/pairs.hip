@@ -261,10 +261,10 @@ __device__ inline int wave_excl_scan_int(int v, int lane) {
   return x - v;
 }
 
-// Stencil walk for group n (lc_list.f:133-201) by one wavefront: lane l takes
-// stencil cells l, l+64, ...; write positions come from a wave prefix sum, so
-// the output order is the reference's: stencil order, then descending group
-// index inside a cell.  half: only l > n (the reference list); else every l != n.
+// Stencil walk for group n (lc_list.f:133-201) by one wavefront: the lanes
+// test 64 candidate groups at a time, in stencil order and then member order;
+// write positions come from a wave prefix sum of the hits, so the output order
+// is the reference's: stencil order, then descending group index inside a cell.  half: only l > n (the reference list); else every l != n.
 // Entries: group number + 1 (entry 0), cell-order position (1), group number (2).
 template <bool WRITE>
 __device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, int entry, int cap = 0x7fffffff) {
@@ -303,50 +303,48 @@ __device__ inline int list_walk_wave(int n, const ListDesc &d, int *out, int ent
     }
     return base;
   }
+  // Many stencil cells: lane m reads the bounds of cell m of each chunk of
+  // 64, then the chunk's members are tested 64 at a time as one flat sequence
+  // (stencil order, then member order: the order above), so each round issues
+  // one position load per lane in parallel rather than each lane walking its
+  // own cell's members one dependent load after another.  Candidate q finds
+  // its cell by binary search over the chunk's member offsets (an exclusive
+  // prefix sum, read across lanes): the last cell whose offset is <= q, which
+  // holds q since q < the chunk's total.
   for (int m0 = 0; m0 < cg.nsten; m0 += 64) {
     const int m = m0 + lane;
-    int s = 0, e = 0;
+    int s = 0, cnt = 0;
     if (m < cg.nsten) {
       int nx = fmodp(i + gmem(cg.sten)[3 * m], cg.ncx);
       int ny = fmodp(j + gmem(cg.sten)[3 * m + 1], cg.ncy);
       int nz = fmodp(k + gmem(cg.sten)[3 * m + 2], cg.ncz);
       int cc = nz + cg.ncz * (ny + cg.ncy * nx);
       s = gmem(d.cell_start)[cc];
-      e = s + gmem(d.cell_cnt)[cc];
+      cnt = gmem(d.cell_cnt)[cc];
     }
-    // hits of this lane's cell as a bit mask over its (at most 64) first members
-    int cnt = 0;
-    unsigned long long hits = 0ull;
-    for (int t = s; t < e; ++t) {
-      const double4 p = gload(cg.cpos + t);
-      const int l = (int)p.w;
-      if (cg.half ? l <= n : l == n) continue;
-      if (list_hit(x1, y1, z1, p, cg.co, cg.sqcut)) {
-        ++cnt;
-        if (t - s < 64) hits |= 1ull << (t - s);
+    const int ofs = wave_excl_scan_int(cnt, lane);
+    const int total = __shfl(ofs + cnt, 63, 64);
+    for (int q0 = 0; q0 < total; q0 += 64) {
+      const int q = q0 + lane;
+      int c = 0;  // c + step <= 63 at every step
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(ofs, c + step, 64) <= q) c += step;
+      const int t = __shfl(s, c, 64) + (q - __shfl(ofs, c, 64));
+      bool hit = false;
+      int l = 0;
+      if (q < total) {
+        const double4 p = gload(cg.cpos + t);
+        l = (int)p.w;
+        hit = !(cg.half ? l <= n : l == n) && list_hit(x1, y1, z1, p, cg.co, cg.sqcut);
       }
-    }
-    if (WRITE) {
-      int pos = base + wave_excl_scan_int(cnt, lane);
-      for (int t = s; t < e; ++t) {
-        const int r = t - s;
-        bool hit;
-        int l;
-        if (r < 64) {
-          hit = (hits >> r) & 1ull;
-          l = (hit && entry != 1) ? (int)gload(cg.cpos + t).w : 0;
-        } else {
-          const double4 p = gload(cg.cpos + t);
-          l = (int)p.w;
-          hit = !(cg.half ? l <= n : l == n) && list_hit(x1, y1, z1, p, cg.co, cg.sqcut);
-        }
-        if (hit) {
-          if (pos < cap) out[pos] = entry == 1 ? t : (entry == 0 ? l + 1 : l);
-          ++pos;
-        }
+      const unsigned long long mask = __ballot(hit);
+      if (WRITE && hit) {
+        const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+        if (pos < cap) out[pos] = entry == 1 ? t : (entry == 0 ? l + 1 : l);
       }
+      base += __popcll(mask);
     }
-    base += wave_sum_int(cnt);
   }
   return base;
 }
